@@ -1,0 +1,130 @@
+/*
+ * pysift_amd C-ABI (libpysift_amd.so): MI355X-native SIFT detect -> describe
+ * -> match -> RANSAC -> warp/feather.
+ *
+ * Conventions
+ *   - Every entry point returns int status: 0 on success, <0 on error
+ *     (MSIFT_E*); msift_last_error() returns the thread's last message.
+ *     Nothing throws across the ABI.
+ *   - Device buffers are caller-owned (torch tensors in the Python facade),
+ *     passed as raw pointers plus sizes.  `stream` is a hipStream_t; all work
+ *     is enqueued asynchronously on it.
+ *   - Counts are written to device memory, so a caller may capture the calls
+ *     in a HIP graph.  Workspaces are sized by the *_workspace_bytes queries.
+ *   - Not thread-safe per handle: use one handle per thread or stream.
+ *
+ * Each declaration names the reference interface it replaces
+ * (SivaIITM/PySIFT pysift/core.py:<line>).
+ */
+#ifndef PYSIFT_AMD_H
+#define PYSIFT_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSIFT_ABI_VERSION 1
+
+#define MSIFT_OK          0
+#define MSIFT_EINVAL     -1   /* bad argument (Python: ValueError) */
+#define MSIFT_EHIP       -2   /* HIP runtime failure */
+#define MSIFT_EWORKSPACE -3   /* workspace too small */
+#define MSIFT_ESTATE     -4   /* operation not valid in this state (RuntimeError) */
+
+/* Parameters of PySIFT.__init__ (core.py:1184-1232) plus the caps that the
+ * reference hard-codes (core.py:1395, 2078-2080, 1270). */
+typedef struct msift_params {
+    int    n_octaves;            /* core.py:1191 (5 on a >=12 GB device) */
+    int    n_scales;             /* S */
+    double sigma0;
+    double contrast_thresh;
+    double edge_thresh;
+    int    double_image;
+    int    rootsift;
+    int    double_px_cap;        /* core.py:1270: 16 000 000 on a >=12 GB device */
+    int    max_cands_per_octave; /* core.py:1395: 25 000 */
+    int    max_keypoints;        /* core.py:2078: 8000 */
+    int    border;               /* core.py:1381: 5 */
+    int    retry_below;          /* core.py:2064: 100 */
+    int    adaptive_thresh;      /* core.py:2045: 1 */
+} msift_params;
+
+typedef struct msift_handle msift_handle;
+
+int msift_abi_version(void);
+int msift_last_error(char* buf, size_t n);
+
+/* PySIFT.__init__ (core.py:1184): validates parameters, precomputes taps. */
+int msift_create(const msift_params* p, msift_handle** h);
+int msift_destroy(msift_handle* h);
+
+/* Workspace for detect_describe on an H x W image (pyramid, candidates,
+ * keypoint lists).  Replaces the CuPy memory-pool allocations of
+ * core.py:1234-1460. */
+int msift_workspace_bytes(msift_handle* h, int H, int W, size_t* bytes);
+
+/* PySIFT.detectAndCompute (core.py:2021-2172) on a device-resident uint8
+ * image.  Writes up to `cap` keypoints sorted by (x, y):
+ *   d_kp[cap*6]      = x, y, size, angle(deg), response, octave
+ *   d_desc_f32       = [cap,128] RootSIFT float descriptors (or NULL)
+ *   d_desc_u8        = [cap,128] uint8 = min(255, trunc(512 v + 0.5)) (or NULL)
+ *   d_count          = number of keypoints (before clipping to cap)
+ * Set phase_ms (8 floats) non-NULL to time the stages with HIP events
+ * (synchronises the stream; keys of core.py:2152-2161). */
+int msift_detect_describe(msift_handle* h, const uint8_t* d_img, int H, int W, int pitch,
+                          void* d_ws, size_t ws_bytes, int cap,
+                          float* d_kp, float* d_desc_f32, uint8_t* d_desc_u8, int* d_count,
+                          float* phase_ms, void* stream);
+
+/* Octave-level access for tests: Gaussian (S+3) and DoG (S+2) stacks of
+ * octave `o` inside a workspace filled by msift_detect_describe. */
+int msift_pyramid_view(msift_handle* h, int H, int W, void* d_ws, int o,
+                       int* Ho, int* Wo, float** d_gauss, float** d_dog);
+
+/* Brute-force ratio-test matcher, exact on uint8 descriptors: replaces
+ * _match_features / _match_symmetric (core.py:2840-2940, 2652-2701).
+ * d2 = sum (a-b)^2 is computed on INT8 MFMA with INT32 accumulation.
+ *   method 0: high/low byte split (a = 128 h + l)  1: signed shift (a-128)
+ * Outputs: d_fwd[N*3] = best j, d1, d2 over b;  d_bwd[M*3] = best i, d1, d2
+ * over a (ties keep the smallest index);  d_pairs[2*min(N,M)] mutual pairs
+ * (i, j) passing d1 < ratio^2 d2 in both directions, in increasing i;
+ * d_npairs = count. */
+int msift_match_workspace_bytes(int N, int M, size_t* bytes);
+int msift_match_u8(const uint8_t* d_a, int N, const uint8_t* d_b, int M, double ratio, int method,
+                   void* d_ws, size_t ws_bytes, int32_t* d_fwd, int32_t* d_bwd,
+                   int32_t* d_pairs, int* d_npairs, void* stream);
+
+/* Float descriptor path: d2 = sum (a-b)^2 via FP16 MFMA with FP32
+ * accumulation of a.b on fp16-rounded inputs; same outputs (distances as
+ * float bits in the int32 slots). */
+int msift_match_f16(const float* d_a, int N, const float* d_b, int M, double ratio,
+                    void* d_ws, size_t ws_bytes, int32_t* d_fwd, int32_t* d_bwd,
+                    int32_t* d_pairs, int* d_npairs, void* stream);
+
+/* GPU-batched RANSAC + LO refit: replaces _ransac / _ransac_batch_eval /
+ * _compute_homography (core.py:3011-3097, 2272-2313, 2959-2981).
+ * Hypotheses: Philox4x32-10, counter (b,0,0,0), key = seed; FP64 DLT.
+ * method 0 = MAGSAC soft score, 1 = inlier count.  d_H[9] row-major,
+ * d_mask[N] inliers, d_ninl = inlier count, d_best = winning hypothesis. */
+int msift_ransac_workspace_bytes(int N, int n_hyp, size_t* bytes);
+int msift_ransac_homography(const double* d_p1, const double* d_p2, int N, uint64_t seed, int n_hyp,
+                            double thr_px, double sigma_max, int method, int lo, int min_inliers,
+                            void* d_ws, size_t ws_bytes, double* d_H, uint8_t* d_mask,
+                            int* d_ninl, int* d_best, void* stream);
+
+/* _warp_image + compositing (core.py:3210-3239, 3794-3802) with a feather
+ * blend: img1 (BGR u8, h1 x w1) warped by H into a cw x ch canvas,
+ * img2 pasted at (round(ox), round(oy)); out pano [ch*cw*3], valid [ch*cw]. */
+int msift_warp_feather(const uint8_t* d_img1, int h1, int w1, const uint8_t* d_img2, int h2, int w2,
+                       const double* H, int cw, int ch, double ox, double oy,
+                       uint8_t* d_pano, uint8_t* d_valid, void* stream);
+
+/* cv2.cvtColor(BGR2GRAY) fixed-point BT.601 (core.py:2548). */
+int msift_bgr_to_gray(const uint8_t* d_bgr, int H, int W, uint8_t* d_gray, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

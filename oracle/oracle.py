@@ -1,0 +1,254 @@
+"""ctypes wrapper of the CPU oracle (oracle/libsift_oracle.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker; the pysift_amd package never
+imports it.  Each wrapper names the reference function (pysift/core.py:<line>)
+that the C routine restates.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libsift_oracle.so")
+_lib = None
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("n_octaves", ctypes.c_int), ("n_scales", ctypes.c_int), ("sigma0", ctypes.c_double),
+        ("contrast_thresh", ctypes.c_double), ("edge_thresh", ctypes.c_double),
+        ("double_image", ctypes.c_int), ("rootsift", ctypes.c_int), ("double_px_cap", ctypes.c_int),
+        ("max_cands_per_octave", ctypes.c_int), ("max_keypoints", ctypes.c_int), ("border", ctypes.c_int),
+        ("retry_below", ctypes.c_int), ("adaptive_thresh", ctypes.c_int),
+    ]
+
+
+def params(n_octaves=5, n_scales=3, sigma0=1.6, contrast_thresh=0.04, edge_thresh=10.0,
+           double_image=True, rootsift=True, double_px_cap=16_000_000, max_cands_per_octave=25000,
+           max_keypoints=8000, border=5, retry_below=100, adaptive_thresh=True):
+    """Defaults of PySIFT.__init__ on a >=12 GB device (core.py:1184-1270)."""
+    return Params(n_octaves, n_scales, sigma0, contrast_thresh, edge_thresh, int(double_image),
+                  int(rootsift), double_px_cap, max_cands_per_octave, max_keypoints, border,
+                  retry_below, int(adaptive_thresh))
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+
+def _declare(L):
+    L.or_exp.restype = _D; L.or_exp.argtypes = [_D]
+    L.or_log2.restype = _D; L.or_log2.argtypes = [_D]
+    L.or_atan2.restype = _D; L.or_atan2.argtypes = [_D, _D]
+    L.or_sincos.restype = None; L.or_sincos.argtypes = [_D, _P, _P]
+    L.or_radius.restype = _I; L.or_radius.argtypes = [_D]
+    L.or_taps.restype = None; L.or_taps.argtypes = [_D, _I, _P]
+    L.or_blur.restype = None; L.or_blur.argtypes = [_P, _P, _I, _I, _D]
+    L.or_upsample2x.restype = None; L.or_upsample2x.argtypes = [_P, _I, _I, _P]
+    L.or_adaptive_thresh.restype = _D; L.or_adaptive_thresh.argtypes = [_P, _I, _I, _I, _D]
+    L.or_detect_describe.restype = _I
+    L.or_detect_describe.argtypes = [_P, _I, _I, _I, ctypes.POINTER(Params), _I, _P, _P, _P]
+    L.or_pyramid_octave.restype = _I
+    L.or_pyramid_octave.argtypes = [_P, _I, _I, _I, ctypes.POINTER(Params), _I, _P, _P, _P, _P]
+    L.or_match_u8.restype = None
+    L.or_match_u8.argtypes = [_P, _I, _P, _I, _D, _P, _P, _P, _P]
+    L.or_philox4x32_10.restype = None; L.or_philox4x32_10.argtypes = [_P, _P, _P]
+    L.or_ransac_sample.restype = None; L.or_ransac_sample.argtypes = [_I, _I, ctypes.c_uint64, _P]
+    L.or_dlt_ls.restype = _I; L.or_dlt_ls.argtypes = [_P, _P, _I, _P]
+    L.or_ransac.restype = _I
+    L.or_ransac.argtypes = [_P, _P, _I, ctypes.c_uint64, _I, _D, _D, _I, _I, _I, _P, _P, _P, _P]
+    L.or_canvas.restype = None; L.or_canvas.argtypes = [_I, _I, _I, _I, _P, _P, _P, _P, _P]
+    L.or_warp_feather.restype = None
+    L.or_warp_feather.argtypes = [_P, _I, _I, _P, _I, _I, _P, _I, _I, _D, _D, _P, _P]
+    L.or_bgr_to_gray.restype = None; L.or_bgr_to_gray.argtypes = [_P, _I, _I, _P]
+    L.or_set_threads.restype = _I; L.or_set_threads.argtypes = [_I]
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def set_threads(n):
+    return lib().or_set_threads(int(n))
+
+
+# ---- elementary functions -------------------------------------------------
+def exp(x):
+    return lib().or_exp(float(x))
+
+
+def log2(x):
+    return lib().or_log2(float(x))
+
+
+def atan2(y, x):
+    return lib().or_atan2(float(y), float(x))
+
+
+def sincos(x):
+    s = ctypes.c_double(); c = ctypes.c_double()
+    lib().or_sincos(float(x), ctypes.byref(s), ctypes.byref(c))
+    return s.value, c.value
+
+
+# ---- pyramid ----------------------------------------------------------------
+def radius(sigma):
+    """core.py:561: int(truncate*sigma + 0.5), truncate=4 (core.py:1288)."""
+    return lib().or_radius(float(sigma))
+
+
+def taps(sigma):
+    r = radius(sigma)
+    t = np.empty(2 * r + 1, np.float32)
+    lib().or_taps(float(sigma), r, _ptr(t))
+    return t
+
+
+def blur(x, sigma):
+    """_fast_gaussian (core.py:549-608)."""
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(x)
+    lib().or_blur(_ptr(x), _ptr(out), x.shape[0], x.shape[1], float(sigma))
+    return out
+
+
+def upsample2x(x):
+    """cpnd.zoom(x, 2.0, order=3) (core.py:1274)."""
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty((2 * x.shape[0], 2 * x.shape[1]), np.float32)
+    lib().or_upsample2x(_ptr(x), x.shape[0], x.shape[1], _ptr(out))
+    return out
+
+
+def adaptive_thresh(img, base=0.04):
+    """_adaptive_contrast_thresh (core.py:1129-1148)."""
+    img = np.ascontiguousarray(img, np.uint8)
+    return lib().or_adaptive_thresh(_ptr(img), img.shape[0], img.shape[1], img.shape[1], float(base))
+
+
+def pyramid_octave(img, p, which):
+    img = np.ascontiguousarray(img, np.uint8)
+    H, W = img.shape
+    Ho = ctypes.c_int(); Wo = ctypes.c_int()
+    if lib().or_pyramid_octave(_ptr(img), H, W, W, ctypes.byref(p), which, ctypes.byref(Ho),
+                               ctypes.byref(Wo), None, None) != 0:
+        raise ValueError("octave out of range")
+    S = p.n_scales
+    g = np.empty((S + 3, Ho.value, Wo.value), np.float32)
+    d = np.empty((S + 2, Ho.value, Wo.value), np.float32)
+    lib().or_pyramid_octave(_ptr(img), H, W, W, ctypes.byref(p), which, ctypes.byref(Ho),
+                            ctypes.byref(Wo), _ptr(g), _ptr(d))
+    return g, d
+
+
+def detect_describe(img, p=None, cap=200000):
+    """PySIFT.detectAndCompute (core.py:2021-2172).  Returns
+    kp (N,6) [x,y,size,angle,response,octave], desc f32 (N,128), desc u8 (N,128)."""
+    p = p or params()
+    img = np.ascontiguousarray(img, np.uint8)
+    H, W = img.shape
+    kp = np.empty((cap, 6), np.float32)
+    d = np.empty((cap, 128), np.float32)
+    du = np.empty((cap, 128), np.uint8)
+    n = lib().or_detect_describe(_ptr(img), H, W, W, ctypes.byref(p), cap, _ptr(kp), _ptr(d), _ptr(du))
+    if n < 0:
+        raise ValueError("oracle detect failed")
+    return kp[:n].copy(), d[:n].copy(), du[:n].copy()
+
+
+# ---- matcher ----------------------------------------------------------------
+def match_u8(a, b, ratio=0.75):
+    """Exact integer restatement of _match_features (core.py:2840-2940)."""
+    a = np.ascontiguousarray(a, np.uint8); b = np.ascontiguousarray(b, np.uint8)
+    N, M = a.shape[0], b.shape[0]
+    fwd = np.empty((max(N, 1), 3), np.int32); bwd = np.empty((max(M, 1), 3), np.int32)
+    pairs = np.empty((max(min(N, M), 1), 2), np.int32)
+    npairs = ctypes.c_int()
+    lib().or_match_u8(_ptr(a), N, _ptr(b), M, float(ratio), _ptr(fwd), _ptr(bwd), _ptr(pairs),
+                      ctypes.byref(npairs))
+    return fwd[:N], bwd[:M], pairs[:npairs.value].copy()
+
+
+# ---- RANSAC -----------------------------------------------------------------
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, np.uint32); k = np.ascontiguousarray(key, np.uint32)
+    o = np.empty(4, np.uint32)
+    lib().or_philox4x32_10(_ptr(c), _ptr(k), _ptr(o))
+    return o
+
+
+def ransac_sample(B, N, seed):
+    idx = np.empty((B, 4), np.int32)
+    lib().or_ransac_sample(B, N, seed, _ptr(idx))
+    return idx
+
+
+def dlt_ls(p1, p2):
+    """_compute_homography (core.py:2959-2981)."""
+    p1 = np.ascontiguousarray(p1, np.float64); p2 = np.ascontiguousarray(p2, np.float64)
+    H = np.empty(9, np.float64)
+    if lib().or_dlt_ls(_ptr(p1), _ptr(p2), p1.shape[0], _ptr(H)) != 0:
+        raise ValueError("need >= 4 points")
+    return H.reshape(3, 3)
+
+
+def ransac(p1, p2, seed=42, n_hyp=2000, thr=3.0, sigma_max=None, method="magsac", lo=True,
+           min_inliers=10, return_all=False):
+    """_ransac (core.py:3011-3097) with Philox hypotheses."""
+    p1 = np.ascontiguousarray(p1, np.float64); p2 = np.ascontiguousarray(p2, np.float64)
+    N = p1.shape[0]
+    if N < 4:
+        raise ValueError(f"Need >= 4 correspondences, got {N}")
+    sm = thr if sigma_max is None else sigma_max
+    H = np.empty(9, np.float64); mask = np.empty(N, np.uint8)
+    sc = np.empty(n_hyp, np.float64); Hs = np.empty((n_hyp, 9), np.float64)
+    best = lib().or_ransac(_ptr(p1), _ptr(p2), N, seed, n_hyp, float(thr), float(sm),
+                           0 if method == "magsac" else 1, int(lo), int(min_inliers),
+                           _ptr(H), _ptr(mask), _ptr(sc), _ptr(Hs))
+    if return_all:
+        return H.reshape(3, 3), mask.astype(bool), best, sc, Hs.reshape(-1, 3, 3)
+    return H.reshape(3, 3), mask.astype(bool)
+
+
+# ---- warp / feather ---------------------------------------------------------
+def canvas(h1, w1, h2, w2, H):
+    """_compute_canvas_size (core.py:3171-3187)."""
+    H = np.ascontiguousarray(H, np.float64)
+    cw = ctypes.c_int(); ch = ctypes.c_int(); ox = ctypes.c_double(); oy = ctypes.c_double()
+    lib().or_canvas(h1, w1, h2, w2, _ptr(H), ctypes.byref(cw), ctypes.byref(ch), ctypes.byref(ox),
+                    ctypes.byref(oy))
+    return cw.value, ch.value, ox.value, oy.value
+
+
+def warp_feather(img1, img2, H, cw, ch, ox, oy):
+    img1 = np.ascontiguousarray(img1, np.uint8); img2 = np.ascontiguousarray(img2, np.uint8)
+    H = np.ascontiguousarray(H, np.float64)
+    pano = np.empty((ch, cw, 3), np.uint8); valid = np.empty((ch, cw), np.uint8)
+    lib().or_warp_feather(_ptr(img1), img1.shape[0], img1.shape[1], _ptr(img2), img2.shape[0],
+                          img2.shape[1], _ptr(H), cw, ch, float(ox), float(oy), _ptr(pano), _ptr(valid))
+    return pano, valid.astype(bool)
+
+
+def bgr_to_gray(bgr):
+    bgr = np.ascontiguousarray(bgr, np.uint8)
+    g = np.empty(bgr.shape[:2], np.uint8)
+    lib().or_bgr_to_gray(_ptr(bgr), bgr.shape[0], bgr.shape[1], _ptr(g))
+    return g

@@ -1,0 +1,1332 @@
+/*
+ * CPU oracle (test infrastructure, never shipped as product code) for the
+ * plain detect -> describe -> match -> RANSAC -> warp/feather path of
+ * PySIFT v0.1.4.  Reference citations are pysift/core.py:<line>.
+ *
+ * Numerics this build defines where the reference leaves them open; the GPU
+ * library must reproduce every one of them bit for bit:
+ *   - all float arithmetic is IEEE single, no FMA contraction (-ffp-contract=off);
+ *     explicit fmaf()/fma() where an accumulation is specified as fused;
+ *   - exp/atan2/sincos/log2 are this file's own double-precision routines
+ *     (Cody-Waite reduction + Taylor/atanh series), rounded to float at the use;
+ *   - per-keypoint histograms accumulate in 64 lane-private partials (pixel p
+ *     goes to partial p % 64, in increasing p), then each bin combines its 64
+ *     partials with the 64-lane xor tree of wave_tree_f();
+ *   - 128-bin and per-point sums use the 64-lane xor tree of wave_tree_*();
+ *   - keypoint refinement and every homography solve run in FP64.
+ */
+#include "sift_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define LANES 64
+
+/* ========================================================================
+ * Elementary functions.  Only +,-,*,/, floor and exact bit manipulation.
+ * ======================================================================== */
+static double ldexp_exact(double x, int k)
+{
+    /* x * 2^k for k in [-1022, 1023]: multiply by an exactly built power of two */
+    union { double d; uint64_t u; } v;
+    v.u = (uint64_t)(k + 1023) << 52;
+    return x * v.d;
+}
+
+double or_exp(double x)
+{
+    if (x != x) return x;
+    if (x > 709.0) return HUGE_VAL;
+    if (x < -708.0) return 0.0;                       /* flush: defined, not denormal */
+    const double INV_LN2 = 0x1.71547652b82fep+0;
+    const double LN2_HI = 0x1.62e42fee00000p-1;       /* 32 trailing zero bits */
+    const double LN2_LO = 0x1.a39ef35793c76p-33;
+    double kd = floor(x * INV_LN2 + 0.5);
+    int k = (int)kd;
+    double r = (x - kd * LN2_HI) - kd * LN2_LO;       /* |r| <= 0.3466 */
+    double p = 1.0 / 6227020800.0;                    /* 1/13! */
+    p = p * r + 1.0 / 479001600.0;
+    p = p * r + 1.0 / 39916800.0;
+    p = p * r + 1.0 / 3628800.0;
+    p = p * r + 1.0 / 362880.0;
+    p = p * r + 1.0 / 40320.0;
+    p = p * r + 1.0 / 5040.0;
+    p = p * r + 1.0 / 720.0;
+    p = p * r + 1.0 / 120.0;
+    p = p * r + 1.0 / 24.0;
+    p = p * r + 1.0 / 6.0;
+    p = p * r + 0.5;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    return ldexp_exact(p, k);
+}
+
+double or_log2(double x)
+{
+    /* x > 0, finite, normal */
+    union { double d; uint64_t u; } v;
+    v.d = x;
+    int e = (int)((v.u >> 52) & 0x7ff) - 1023;
+    v.u = (v.u & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL;   /* m in [1,2) */
+    double m = v.d;
+    if (m > 0x1.6a09e667f3bcdp+0) { m = m * 0.5; e += 1; }          /* m in [0.707,1.414] */
+    double s = (m - 1.0) / (m + 1.0);
+    double s2 = s * s;
+    double t = 1.0 / 23.0;
+    t = t * s2 + 1.0 / 21.0;
+    t = t * s2 + 1.0 / 19.0;
+    t = t * s2 + 1.0 / 17.0;
+    t = t * s2 + 1.0 / 15.0;
+    t = t * s2 + 1.0 / 13.0;
+    t = t * s2 + 1.0 / 11.0;
+    t = t * s2 + 1.0 / 9.0;
+    t = t * s2 + 1.0 / 7.0;
+    t = t * s2 + 1.0 / 5.0;
+    t = t * s2 + 1.0 / 3.0;
+    t = t * s2 + 1.0;
+    double ln_m = 2.0 * (s * t);
+    return (double)e + ln_m * 0x1.71547652b82fep+0;
+}
+
+static const double ATAN_K8[9] = {
+    0.0, 0x1.fd5ba9aac2f6ep-4, 0x1.f5b75f92c80ddp-3, 0x1.6f61941e4def1p-2,
+    0x1.dac670561bb4fp-2, 0x1.1e00babdefeb4p-1, 0x1.4978fa3269ee1p-1,
+    0x1.700a7c5784634p-1, 0x1.921fb54442d18p-1 };
+
+static double atan01(double t)
+{
+    /* t in [0,1]: atan(t) = atan(k/8) + atan((t - k/8) / (1 + t k/8)) */
+    int k = (int)floor(t * 8.0 + 0.5);
+    double c = (double)k * 0.125;
+    double u = (t - c) / (1.0 + t * c);               /* |u| <= 0.0625 */
+    double u2 = u * u;
+    double q = -1.0 / 15.0;
+    q = q * u2 + 1.0 / 13.0;
+    q = q * u2 - 1.0 / 11.0;
+    q = q * u2 + 1.0 / 9.0;
+    q = q * u2 - 1.0 / 7.0;
+    q = q * u2 + 1.0 / 5.0;
+    q = q * u2 - 1.0 / 3.0;
+    double at = u + u * (u2 * q);
+    return ATAN_K8[k] + at;
+}
+
+double or_atan2(double y, double x)
+{
+    const double PI = 0x1.921fb54442d18p+1;
+    const double PIO2 = 0x1.921fb54442d18p+0;
+    if (x != x || y != y) return x + y;
+    int ys = signbit(y) != 0, xs = signbit(x) != 0;
+    double ax = fabs(x), ay = fabs(y);
+    double a;
+    if (ay == 0.0) {
+        a = xs ? PI : 0.0;
+    } else if (ax == 0.0) {
+        a = PIO2;
+    } else if (isinf(ax) || isinf(ay)) {
+        if (isinf(ax) && isinf(ay)) a = xs ? 3.0 * 0x1.921fb54442d18p-1 : 0x1.921fb54442d18p-1;
+        else if (isinf(ax)) a = xs ? PI : 0.0;
+        else a = PIO2;
+    } else {
+        if (ay > ax) a = PIO2 - atan01(ax / ay);
+        else a = atan01(ay / ax);
+        if (xs) a = PI - a;
+    }
+    return ys ? -a : a;
+}
+
+void or_sincos(double x, double* s, double* c)
+{
+    /* |x| < 1e5; Cody-Waite pi/2 split with 33-bit high part */
+    const double TWO_OVER_PI = 0x1.45f306dc9c883p-1;
+    const double PIO2_1 = 0x1.921fb54400000p+0;
+    const double PIO2_1T = 0x1.0b4611a626331p-34;
+    double kd = floor(x * TWO_OVER_PI + 0.5);
+    int q = ((int)kd) & 3;
+    double r = (x - kd * PIO2_1) - kd * PIO2_1T;     /* |r| <= pi/4 + eps */
+    double r2 = r * r;
+    double sp = -1.0 / 1307674368000.0;               /* -1/15! */
+    sp = sp * r2 + 1.0 / 6227020800.0;
+    sp = sp * r2 - 1.0 / 39916800.0;
+    sp = sp * r2 + 1.0 / 362880.0;
+    sp = sp * r2 - 1.0 / 5040.0;
+    sp = sp * r2 + 1.0 / 120.0;
+    sp = sp * r2 - 1.0 / 6.0;
+    double sn = r + r * (r2 * sp);
+    double cp = 1.0 / 20922789888000.0;               /* 1/16! */
+    cp = cp * r2 - 1.0 / 87178291200.0;
+    cp = cp * r2 + 1.0 / 479001600.0;
+    cp = cp * r2 - 1.0 / 3628800.0;
+    cp = cp * r2 + 1.0 / 40320.0;
+    cp = cp * r2 - 1.0 / 720.0;
+    cp = cp * r2 + 1.0 / 24.0;
+    cp = cp * r2 - 0.5;
+    double cs = 1.0 + r2 * cp;
+    switch (q) {
+    case 0: *s = sn;  *c = cs;  break;
+    case 1: *s = cs;  *c = -sn; break;
+    case 2: *s = -sn; *c = -cs; break;
+    default: *s = -cs; *c = sn; break;
+    }
+}
+
+static float f_exp(float x) { return (float)or_exp((double)x); }
+static float f_atan2(float y, float x) { return (float)or_atan2((double)y, (double)x); }
+
+/* fmodf(x, 360) for |x| < 1440, exact (each subtraction is exact) */
+static float fmod360(float x)
+{
+    float a = fabsf(x);
+    while (a >= 360.0f) a = a - 360.0f;
+    return copysignf(a, x);
+}
+
+/* 64-lane xor tree: partial[l] for l<64 -> total, combining l and l+off for
+ * off = 32,16,8,4,2,1 (the lane-0 value of a __shfl_xor butterfly). */
+static float wave_tree_f(float* p)
+{
+    for (int off = 32; off >= 1; off >>= 1)
+        for (int l = 0; l < off; ++l) p[l] = p[l] + p[l + off];
+    return p[0];
+}
+static double wave_tree_d(double* p)
+{
+    for (int off = 32; off >= 1; off >>= 1)
+        for (int l = 0; l < off; ++l) p[l] = p[l] + p[l + off];
+    return p[0];
+}
+
+/* ========================================================================
+ * A2: separable Gaussian blur (core.py:475-608)
+ * ======================================================================== */
+int or_radius(double sigma) { return (int)(4.0 * sigma + 0.5); }
+
+void or_taps(double sigma, int r, float* taps)
+{
+    /* core.py:564-567 (fp32 exp of -0.5(x/sigma)^2, normalised); computed in
+     * double here and rounded once. */
+    double t[129], sum = 0.0;
+    for (int i = -r; i <= r; ++i) {
+        double q = (double)i / sigma;
+        t[i + r] = exp(-0.5 * (q * q));
+    }
+    for (int i = 0; i <= 2 * r; ++i) sum += t[i];
+    for (int i = 0; i <= 2 * r; ++i) taps[i] = (float)(t[i] / sum);
+}
+
+static int reflect_idx(int i, int N)
+{
+    /* core.py:477-483: half-sample symmetric, periodic in 2N */
+    int p = 2 * N;
+    i = i % p;
+    if (i < 0) i += p;
+    if (i >= N) i = 2 * N - 1 - i;
+    return i;
+}
+
+void or_blur(const float* in, float* out, int H, int W, double sigma)
+{
+    if (sigma < 0.01) { memcpy(out, in, sizeof(float) * (size_t)H * W); return; }
+    int r = or_radius(sigma);
+    float taps[129];
+    or_taps(sigma, r, taps);
+    float* tmp = (float*)malloc(sizeof(float) * (size_t)H * W);
+    /* horizontal pass (core.py:486-509): sum = fma(tile[c-r+k], kern[k], sum), k ascending */
+    #pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y) {
+        const float* row = in + (size_t)y * W;
+        for (int x = 0; x < W; ++x) {
+            float s = 0.0f;
+            for (int k = 0; k <= 2 * r; ++k) s = fmaf(row[reflect_idx(x - r + k, W)], taps[k], s);
+            tmp[(size_t)y * W + x] = s;
+        }
+    }
+    /* vertical pass (core.py:519-547) */
+    #pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y) {
+        for (int x = 0; x < W; ++x) {
+            float s = 0.0f;
+            for (int k = 0; k <= 2 * r; ++k) s = fmaf(tmp[(size_t)reflect_idx(y - r + k, H) * W + x], taps[k], s);
+            out[(size_t)y * W + x] = s;
+        }
+    }
+    free(tmp);
+}
+
+/* ========================================================================
+ * A3: cubic B-spline x2 zoom, scipy/cupyx zoom(order=3, mode='constant',
+ * grid_mode=False) semantics (core.py:1274).  The prefilter is the mirror-
+ * boundary inverse B-spline filter written as its symmetric impulse response
+ * sqrt(3) z^|k| (z = sqrt(3)-2) truncated at |k| <= 30 (|z|^31 < 2e-18).
+ * ======================================================================== */
+#define UPS_K 30
+static int mirror_idx(int i, int N)
+{
+    if (N == 1) return 0;
+    int p = 2 * N - 2;
+    if (i < 0) i = -i;
+    i = i % p;
+    return i >= N ? p - i : i;
+}
+
+static void ups_taps(double* h)
+{
+    h[0] = 0x1.bb67ae8584caap+0;                    /* sqrt(3) */
+    for (int k = 1; k <= UPS_K; ++k) h[k] = h[k - 1] * -0x1.126145e9ecd58p-2;
+}
+
+static void bspline_w(double t, double* w)
+{
+    double s = 1.0 - t;
+    double t2 = t * t, t3 = t2 * t, s2 = s * s, s3 = s2 * s;
+    w[0] = s3 * (1.0 / 6.0);
+    w[1] = (2.0 / 3.0 - t2) + 0.5 * t3;
+    w[2] = (2.0 / 3.0 - s2) + 0.5 * s3;
+    w[3] = t3 * (1.0 / 6.0);
+}
+
+void or_upsample2x(const float* in, int H, int W, float* out)
+{
+    double h[UPS_K + 1];
+    ups_taps(h);
+    double* t0 = (double*)malloc(sizeof(double) * (size_t)H * W);
+    double* cf = (double*)malloc(sizeof(double) * (size_t)H * W);
+    /* axis 0 (down columns) */
+    #pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            double acc = 0.0;
+            for (int k = -UPS_K; k <= UPS_K; ++k)
+                acc = fma(h[k < 0 ? -k : k], (double)in[(size_t)mirror_idx(y + k, H) * W + x], acc);
+            t0[(size_t)y * W + x] = acc;
+        }
+    /* axis 1 (along rows) */
+    #pragma omp parallel for schedule(static)
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            double acc = 0.0;
+            for (int k = -UPS_K; k <= UPS_K; ++k)
+                acc = fma(h[k < 0 ? -k : k], t0[(size_t)y * W + mirror_idx(x + k, W)], acc);
+            cf[(size_t)y * W + x] = acc;
+        }
+    int Ho = 2 * H, Wo = 2 * W;
+    double zy = (Ho > 1) ? (double)(H - 1) / (double)(Ho - 1) : 0.0;
+    double zx = (Wo > 1) ? (double)(W - 1) / (double)(Wo - 1) : 0.0;
+    #pragma omp parallel for schedule(static)
+    for (int oy = 0; oy < Ho; ++oy) {
+        double yy = (double)oy * zy;
+        double fy = floor(yy);
+        int iy = (int)fy;
+        double wy[4];
+        bspline_w(yy - fy, wy);
+        for (int ox = 0; ox < Wo; ++ox) {
+            double xx = (double)ox * zx;
+            double fx = floor(xx);
+            int ix = (int)fx;
+            double wx[4];
+            bspline_w(xx - fx, wx);
+            double v = 0.0;
+            for (int a = 0; a < 4; ++a) {
+                const double* row = cf + (size_t)mirror_idx(iy - 1 + a, H) * W;
+                double rv = 0.0;
+                for (int b = 0; b < 4; ++b) rv = fma(wx[b], row[mirror_idx(ix - 1 + b, W)], rv);
+                v = fma(wy[a], rv, v);
+            }
+            out[(size_t)oy * Wo + ox] = (float)v;
+        }
+    }
+    free(t0);
+    free(cf);
+}
+
+void or_u8_to_unit(const uint8_t* in, int H, int W, int pitch, float* out)
+{
+    /* core.py:1258: asarray(u8, float32) / float32(255) */
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) out[(size_t)y * W + x] = (float)in[(size_t)y * pitch + x] / 255.0f;
+}
+
+/* ========================================================================
+ * A1: adaptive contrast threshold (core.py:1129-1148), entropy in FP64 with
+ * bins summed in index order.
+ * ======================================================================== */
+double or_adaptive_thresh(const uint8_t* img, int H, int W, int pitch, double base)
+{
+    long long hist[256];
+    memset(hist, 0, sizeof(hist));
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) hist[img[(size_t)y * pitch + x]]++;
+    double total = (double)H * (double)W;
+    double ent = 0.0;
+    for (int b = 0; b < 256; ++b) {
+        if (hist[b] == 0) continue;
+        double p = (double)hist[b] / total;
+        ent = ent - p * or_log2(p);
+    }
+    double scale = 0.8 + ent / 40.0;
+    if (scale > 1.0) scale = 1.0;
+    return base * scale;
+}
+
+/* ========================================================================
+ * Pyramid (A3/A4, core.py:1234-1365)
+ * ======================================================================== */
+typedef struct {
+    int n_oct, S, doubled;
+    int Ho[16], Wo[16];
+    float* G[16];   /* (S+3) * Ho * Wo */
+    float* D[16];   /* (S+2) * Ho * Wo */
+} pyramid_t;
+
+static void pyr_free(pyramid_t* P)
+{
+    for (int o = 0; o < P->n_oct; ++o) { free(P->G[o]); free(P->D[o]); }
+}
+
+static void sigma_schedule(const or_params* p, int doubled, double* sd0, double* inc)
+{
+    int S = p->n_scales;
+    double k = pow(2.0, 1.0 / (double)S);
+    double sig[64];
+    for (int s = 0; s < S + 3; ++s) sig[s] = p->sigma0 * pow(k, (double)s);
+    double b = doubled ? 1.0 : 0.5;
+    double v = p->sigma0 * p->sigma0 - b * b;
+    *sd0 = sqrt(v > 1e-6 ? v : 1e-6);
+    for (int s = 1; s < S + 3; ++s) {
+        double q = sig[s] * sig[s] - sig[s - 1] * sig[s - 1];
+        inc[s - 1] = sqrt(q > 1e-6 ? q : 1e-6);
+    }
+}
+
+static int build_pyramid(const uint8_t* img, int H, int W, int pitch, const or_params* p, pyramid_t* P)
+{
+    int S = p->n_scales;
+    P->S = S;
+    P->n_oct = p->n_octaves;
+    P->doubled = p->double_image && ((long long)H * W < (long long)p->double_px_cap);
+    float* unit = (float*)malloc(sizeof(float) * (size_t)H * W);
+    or_u8_to_unit(img, H, W, pitch, unit);
+    int bh = H, bw = W;
+    float* base = unit;
+    if (P->doubled) {
+        base = (float*)malloc(sizeof(float) * (size_t)4 * H * W);
+        or_upsample2x(unit, H, W, base);
+        free(unit);
+        bh = 2 * H; bw = 2 * W;
+    }
+    double sd0, inc[64];
+    sigma_schedule(p, P->doubled, &sd0, inc);
+    float* b0 = (float*)malloc(sizeof(float) * (size_t)bh * bw);
+    or_blur(base, b0, bh, bw, sd0);
+    free(base);
+    for (int o = 0; o < P->n_oct; ++o) {
+        size_t n = (size_t)bh * bw;
+        P->Ho[o] = bh; P->Wo[o] = bw;
+        P->G[o] = (float*)malloc(sizeof(float) * n * (S + 3));
+        P->D[o] = (float*)malloc(sizeof(float) * n * (S + 2));
+        memcpy(P->G[o], b0, sizeof(float) * n);
+        for (int s = 0; s < S + 2; ++s) {
+            float* src = P->G[o] + n * s;
+            float* dst = P->G[o] + n * (s + 1);
+            or_blur(src, dst, bh, bw, inc[s]);
+            float* d = P->D[o] + n * s;
+            for (size_t i = 0; i < n; ++i) d[i] = dst[i] - src[i];
+        }
+        /* next base = level S decimated [::2, ::2] (core.py:1350) */
+        int nh = (bh + 1) / 2, nw = (bw + 1) / 2;
+        float* nb = (float*)malloc(sizeof(float) * (size_t)nh * nw);
+        const float* gs = P->G[o] + n * S;
+        for (int y = 0; y < nh; ++y)
+            for (int x = 0; x < nw; ++x) nb[(size_t)y * nw + x] = gs[(size_t)(2 * y) * bw + 2 * x];
+        free(b0);
+        b0 = nb; bh = nh; bw = nw;
+    }
+    free(b0);
+    return 0;
+}
+
+int or_pyramid_octave(const uint8_t* img, int H, int W, int pitch, const or_params* p,
+                      int which, int* Ho, int* Wo, float* gauss, float* dog)
+{
+    pyramid_t P;
+    memset(&P, 0, sizeof(P));
+    build_pyramid(img, H, W, pitch, p, &P);
+    if (which < 0 || which >= P.n_oct) { pyr_free(&P); return -1; }
+    size_t n = (size_t)P.Ho[which] * P.Wo[which];
+    *Ho = P.Ho[which]; *Wo = P.Wo[which];
+    if (gauss) memcpy(gauss, P.G[which], sizeof(float) * n * (P.S + 3));
+    if (dog) memcpy(dog, P.D[which], sizeof(float) * n * (P.S + 2));
+    pyr_free(&P);
+    return 0;
+}
+
+/* ========================================================================
+ * A5: 3x3x3 extrema (core.py:176-217, 1367-1460), raster order (s,r,c) per
+ * octave, first `cap` kept per octave.
+ * ======================================================================== */
+typedef struct { int o, s, r, c; } cand_t;
+typedef struct { int o, si; float ri, ci, resp; } kpt_t;
+
+static int find_extrema(const pyramid_t* P, float thr, int border, int cap, cand_t** out)
+{
+    int total = 0, capacity = 1024;
+    cand_t* c = (cand_t*)malloc(sizeof(cand_t) * capacity);
+    int S2 = P->S + 2;
+    for (int o = 0; o < P->n_oct; ++o) {
+        int H = P->Ho[o], W = P->Wo[o];
+        if (H < 3 || W < 3) continue;
+        size_t HW = (size_t)H * W;
+        const float* D = P->D[o];
+        int cnt = 0;
+        for (int s = 1; s < S2 - 1; ++s)
+            for (int r = border; r < H - border; ++r)
+                for (int col = border; col < W - border; ++col) {
+                    float v = D[s * HW + (size_t)r * W + col];
+                    if (fabsf(v) <= thr) continue;
+                    int is_max = 1, is_min = 1;
+                    for (int ds = -1; ds <= 1 && (is_max || is_min); ++ds)
+                        for (int dr = -1; dr <= 1 && (is_max || is_min); ++dr)
+                            for (int dc = -1; dc <= 1; ++dc) {
+                                if (!ds && !dr && !dc) continue;
+                                float nb = D[(s + ds) * HW + (size_t)(r + dr) * W + col + dc];
+                                is_max = is_max && (v >= nb);
+                                is_min = is_min && (v <= nb);
+                                if (!is_max && !is_min) break;
+                            }
+                    if (is_max || is_min) {
+                        if (cnt < cap) {
+                            if (total == capacity) { capacity *= 2; c = (cand_t*)realloc(c, sizeof(cand_t) * capacity); }
+                            c[total].o = o; c[total].s = s; c[total].r = r; c[total].c = col;
+                            total++;
+                        }
+                        cnt++;
+                    }
+                }
+    }
+    *out = c;
+    return total;
+}
+
+/* ========================================================================
+ * A6: sub-pixel refinement (core.py:343-463) in FP64.
+ * ======================================================================== */
+static int refine_one(const pyramid_t* P, cand_t cd, int border, double thr, double edge, kpt_t* kp)
+{
+    int o = cd.o, si = cd.s, row = cd.r, col = cd.c;
+    int H = P->Ho[o], W = P->Wo[o], S = P->S;
+    size_t HW = (size_t)H * W;
+    const float* D = P->D[o];
+#define DV(s_, r_, c_) ((double)D[(size_t)(s_) * HW + (size_t)(r_) * W + (c_)])
+    double ox = 0.0, oy = 0.0, os = 0.0;
+    int converged = 0;
+    for (int it = 0; it < 5; ++it) {
+        if (si < 1 || si > S || row < border || row >= H - border || col < border || col >= W - border) break;
+        double cv = DV(si, row, col);
+        double dx = (DV(si, row, col + 1) - DV(si, row, col - 1)) * 0.5;
+        double dy = (DV(si, row + 1, col) - DV(si, row - 1, col)) * 0.5;
+        double ds = (DV(si + 1, row, col) - DV(si - 1, row, col)) * 0.5;
+        double dxx = (DV(si, row, col + 1) - 2.0 * cv) + DV(si, row, col - 1);
+        double dyy = (DV(si, row + 1, col) - 2.0 * cv) + DV(si, row - 1, col);
+        double dss = (DV(si + 1, row, col) - 2.0 * cv) + DV(si - 1, row, col);
+        double dxy = (((DV(si, row + 1, col + 1) - DV(si, row + 1, col - 1)) - DV(si, row - 1, col + 1)) + DV(si, row - 1, col - 1)) * 0.25;
+        double dxs = (((DV(si + 1, row, col + 1) - DV(si + 1, row, col - 1)) - DV(si - 1, row, col + 1)) + DV(si - 1, row, col - 1)) * 0.25;
+        double dys = (((DV(si + 1, row + 1, col) - DV(si + 1, row - 1, col)) - DV(si - 1, row + 1, col)) + DV(si - 1, row - 1, col)) * 0.25;
+        double det = (dxx * (dyy * dss - dys * dys) - dxy * (dxy * dss - dys * dxs)) + dxs * (dxy * dys - dyy * dxs);
+        if (fabs(det) < 1e-12) break;
+        double p = -dx, q = -dy, rv = -ds;
+        ox = ((p * (dyy * dss - dys * dys) - dxy * (q * dss - dys * rv)) + dxs * (q * dys - dyy * rv)) / det;
+        oy = ((dxx * (q * dss - dys * rv) - p * (dxy * dss - dys * dxs)) + dxs * (dxy * rv - q * dxs)) / det;
+        os = ((dxx * (dyy * rv - q * dys) - dxy * (dxy * rv - q * dxs)) + p * (dxy * dys - dyy * dxs)) / det;
+        if (fabs(ox) < 0.5 && fabs(oy) < 0.5 && fabs(os) < 0.5) { converged = 1; break; }
+        col += (ox >= 0.0) ? (int)(ox + 0.5) : (int)(ox - 0.5);
+        row += (oy >= 0.0) ? (int)(oy + 0.5) : (int)(oy - 0.5);
+        si  += (os >= 0.0) ? (int)(os + 0.5) : (int)(os - 0.5);
+    }
+    if (!converged) return 0;
+    if (si < 1 || si > S || row < border || row >= H - border || col < border || col >= W - border) return 0;
+    double cv = DV(si, row, col);
+    double dx = (DV(si, row, col + 1) - DV(si, row, col - 1)) * 0.5;
+    double dy = (DV(si, row + 1, col) - DV(si, row - 1, col)) * 0.5;
+    double ds = (DV(si + 1, row, col) - DV(si - 1, row, col)) * 0.5;
+    double resp = fabs(cv + 0.5 * ((dx * ox + dy * oy) + ds * os));
+    if (resp < thr) return 0;
+    double dxx = (DV(si, row, col + 1) - 2.0 * cv) + DV(si, row, col - 1);
+    double dyy = (DV(si, row + 1, col) - 2.0 * cv) + DV(si, row - 1, col);
+    double dxy = (((DV(si, row + 1, col + 1) - DV(si, row + 1, col - 1)) - DV(si, row - 1, col + 1)) + DV(si, row - 1, col - 1)) * 0.25;
+    double tr = dxx + dyy;
+    double det2 = dxx * dyy - dxy * dxy;
+    if (det2 <= 0.0 || (tr * tr) / det2 >= edge) return 0;
+#undef DV
+    kp->o = o; kp->si = si;
+    kp->ri = (float)((double)row + oy);
+    kp->ci = (float)((double)col + ox);
+    kp->resp = (float)resp;
+    return 1;
+}
+
+static int detect_keypoints(const pyramid_t* P, const or_params* p, double ct, kpt_t** out)
+{
+    float thr32 = (float)(ct / (double)p->n_scales);
+    float edge32 = (float)((p->edge_thresh + 1.0) * (p->edge_thresh + 1.0) / p->edge_thresh);
+    cand_t* c = NULL;
+    int nc = find_extrema(P, thr32, p->border, p->max_cands_per_octave, &c);
+    kpt_t* k = (kpt_t*)malloc(sizeof(kpt_t) * (nc > 0 ? nc : 1));
+    int n = 0;
+    for (int i = 0; i < nc; ++i)
+        if (refine_one(P, c[i], p->border, (double)thr32, (double)edge32, &k[n])) n++;
+    free(c);
+    *out = k;
+    return n;
+}
+
+/* ========================================================================
+ * A8: orientation histogram (core.py:228-333)
+ * ======================================================================== */
+static int orientation_one(const float* G, int H, int W, float ri, float ci, float sigma, float* angles)
+{
+    int radius = (int)(4.5f * sigma + 0.5f);
+    if (radius < 1) radius = 1;
+    float sigma_w = 1.5f * sigma;
+    float inv_2sw2 = 1.0f / (2.0f * sigma_w * sigma_w + 1e-12f);
+    int r_min = (int)ri - radius; if (r_min < 1) r_min = 1;
+    int r_max = (int)ri + radius; if (r_max > H - 2) r_max = H - 2;
+    int c_min = (int)ci - radius; if (c_min < 1) c_min = 1;
+    int c_max = (int)ci + radius; if (c_max > W - 2) c_max = W - 2;
+    int n_cols = c_max - c_min + 1; if (n_cols < 1) n_cols = 1;
+    int n_rows = r_max - r_min + 1; if (n_rows < 0) n_rows = 0;
+    int total = n_rows * n_cols;
+    float part[LANES][36];
+    memset(part, 0, sizeof(part));
+    for (int pidx = 0; pidx < total; ++pidx) {
+        int lane = pidx % LANES;
+        int r = r_min + pidx / n_cols;
+        int c = c_min + pidx % n_cols;
+        float dx = G[(size_t)r * W + c + 1] - G[(size_t)r * W + c - 1];
+        float dy = G[(size_t)(r + 1) * W + c] - G[(size_t)(r - 1) * W + c];
+        float mag = sqrtf(dx * dx + dy * dy);
+        float ori = fmod360(f_atan2(dy, dx) * 57.29577951f + 360.0f);
+        float dr = (float)r - ri, dc = (float)c - ci;
+        float wt = f_exp(-(dr * dr + dc * dc) * inv_2sw2);
+        float bin_f = ori * 0.1f;
+        int bin_i = ((int)bin_f) % 36;
+        int bin_r = (bin_i + 1) % 36;
+        float frac = bin_f - floorf(bin_f);
+        part[lane][bin_i] += mag * wt * (1.0f - frac);
+        part[lane][bin_r] += mag * wt * frac;
+    }
+    float hist[36], smooth[36];
+    for (int b = 0; b < 36; ++b) {
+        float t[LANES];
+        for (int l = 0; l < LANES; ++l) t[l] = part[l][b];
+        hist[b] = wave_tree_f(t);
+    }
+    for (int it = 0; it < 2; ++it) {
+        for (int b = 0; b < 36; ++b)
+            smooth[b] = (0.25f * hist[(b + 35) % 36] + 0.5f * hist[b]) + 0.25f * hist[(b + 1) % 36];
+        for (int b = 0; b < 36; ++b) hist[b] = smooth[b];
+    }
+    float peak = 0.0f;
+    for (int b = 0; b < 36; ++b) if (hist[b] > peak) peak = hist[b];
+    float thresh = 0.8f * peak;
+    int n = 0;
+    for (int b = 0; b < 36; ++b) {
+        if (hist[b] >= thresh) {
+            float l = hist[(b + 35) % 36], rv = hist[(b + 1) % 36];
+            if (hist[b] >= l && hist[b] >= rv) {
+                float denom = (l - 2.0f * hist[b]) + rv;
+                float fine = (float)b + (fabsf(denom) > 1e-12f ? (0.5f * (l - rv)) / denom : 0.0f);
+                float angle = fmod360((fine / 36.0f) * 360.0f + 360.0f);
+                if (n < 4) angles[n++] = angle;
+            }
+        }
+    }
+    for (int k = n; k < 4; ++k) angles[k] = -1.0f;
+    return n;
+}
+
+/* ========================================================================
+ * A9: 4x4x8 descriptor (core.py:693-835) + RootSIFT (core.py:1870-1873)
+ * ======================================================================== */
+static void descriptor_one(const float* G, int H, int W, float ri, float ci, float ang_rad, int ph,
+                           int rootsift, float* out, uint8_t* out_u8)
+{
+    static __thread float part[LANES][128];
+    memset(part, 0, sizeof(part));
+    double sd, cd;
+    or_sincos((double)ang_rad, &sd, &cd);
+    float cos_a = (float)cd, sin_a = (float)sd;
+    float scale_sp = 4.0f / (2.0f * (float)ph);
+    float sigma_d = (float)ph * 0.5657f;
+    float inv_2s2 = 1.0f / (2.0f * sigma_d * sigma_d + 1e-12f);
+    int side = 2 * ph + 1;
+    int total_px = side * side;
+    int iri = (int)ri, ici = (int)ci;
+    for (int flat = 0; flat < total_px; ++flat) {
+        int lane = flat % LANES;
+        int r_off = flat / side - ph;
+        int c_off = flat % side - ph;
+        int rr = iri + r_off, cc = ici + c_off;
+        if (rr < 1 || rr >= H - 1 || cc < 1 || cc >= W - 1) continue;
+        float xr = cos_a * (float)c_off - sin_a * (float)r_off;
+        float yr = sin_a * (float)c_off + cos_a * (float)r_off;
+        float xb = xr * scale_sp + 1.5f;
+        float yb = yr * scale_sp + 1.5f;
+        if (xb < -0.5f || xb >= 3.5f || yb < -0.5f || yb >= 3.5f) continue;
+        float dx = G[(size_t)rr * W + cc + 1] - G[(size_t)rr * W + cc - 1];
+        float dy = G[(size_t)(rr + 1) * W + cc] - G[(size_t)(rr - 1) * W + cc];
+        float mag = sqrtf(dx * dx + dy * dy);
+        float ori_deg = fmod360((f_atan2(dy, dx) - ang_rad) * 57.29577951f + 360.0f);
+        float gw = f_exp(-(xr * xr + yr * yr) * inv_2s2);
+        float wmag = mag * gw;
+        float ob = ori_deg * 0.02222222f;
+        int oi0 = (int)floorf(ob);
+        float df = ob - (float)oi0;
+        int xi0 = (int)floorf(xb), yi0 = (int)floorf(yb);
+        float dxb = xb - (float)xi0, dyb = yb - (float)yi0;
+        int xi0c = xi0 < 0 ? 0 : (xi0 > 3 ? 3 : xi0);
+        int yi0c = yi0 < 0 ? 0 : (yi0 > 3 ? 3 : yi0);
+        int xi1c = xi0 + 1 < 0 ? 0 : (xi0 + 1 > 3 ? 3 : xi0 + 1);
+        int yi1c = yi0 + 1 < 0 ? 0 : (yi0 + 1 > 3 ? 3 : yi0 + 1);
+        int oi0w = ((oi0 % 8) + 8) % 8;
+        int oi1w = ((oi0 + 1) % 8 + 8) % 8;
+        float wy0 = 1.0f - dyb, wy1 = dyb, wx0 = 1.0f - dxb, wx1 = dxb, wo0 = 1.0f - df, wo1 = df;
+        float* h = part[lane];
+        h[yi0c * 32 + xi0c * 8 + oi0w] += wy0 * wx0 * wo0 * wmag;
+        h[yi0c * 32 + xi0c * 8 + oi1w] += wy0 * wx0 * wo1 * wmag;
+        h[yi0c * 32 + xi1c * 8 + oi0w] += wy0 * wx1 * wo0 * wmag;
+        h[yi0c * 32 + xi1c * 8 + oi1w] += wy0 * wx1 * wo1 * wmag;
+        h[yi1c * 32 + xi0c * 8 + oi0w] += wy1 * wx0 * wo0 * wmag;
+        h[yi1c * 32 + xi0c * 8 + oi1w] += wy1 * wx0 * wo1 * wmag;
+        h[yi1c * 32 + xi1c * 8 + oi0w] += wy1 * wx1 * wo0 * wmag;
+        h[yi1c * 32 + xi1c * 8 + oi1w] += wy1 * wx1 * wo1 * wmag;
+    }
+    float v[128];
+    for (int b = 0; b < 128; ++b) {
+        float t[LANES];
+        for (int l = 0; l < LANES; ++l) t[l] = part[l][b];
+        v[b] = wave_tree_f(t);
+    }
+    /* L2 -> clip 0.2 -> L2 (core.py:793-834); lane l holds bins 2l, 2l+1 */
+    float pp[LANES];
+    for (int l = 0; l < LANES; ++l) pp[l] = v[2 * l] * v[2 * l] + v[2 * l + 1] * v[2 * l + 1];
+    float inv = 1.0f / sqrtf(wave_tree_f(pp) + 1e-14f);
+    for (int b = 0; b < 128; ++b) { float t = v[b] * inv; v[b] = t > 0.2f ? 0.2f : t; }
+    for (int l = 0; l < LANES; ++l) pp[l] = v[2 * l] * v[2 * l] + v[2 * l + 1] * v[2 * l + 1];
+    inv = 1.0f / sqrtf(wave_tree_f(pp) + 1e-14f);
+    for (int b = 0; b < 128; ++b) v[b] = v[b] * inv;
+    if (rootsift) {
+        for (int l = 0; l < LANES; ++l) pp[l] = fabsf(v[2 * l]) + fabsf(v[2 * l + 1]);
+        float l1 = wave_tree_f(pp) + 1e-7f;
+        for (int b = 0; b < 128; ++b) { float t = v[b] / l1; v[b] = sqrtf(t > 0.0f ? t : 0.0f); }
+    }
+    for (int b = 0; b < 128; ++b) {
+        out[b] = v[b];
+        float q = v[b] * 512.0f + 0.5f;
+        int iq = (int)q;
+        out_u8[b] = (uint8_t)(iq > 255 ? 255 : (iq < 0 ? 0 : iq));
+    }
+}
+
+/* ========================================================================
+ * A7/A10: full detectAndCompute (core.py:2021-2172)
+ * ======================================================================== */
+typedef struct { int kp; float angle; float x, y; } okp_t;
+
+static int cmp_resp_desc(const void* a, const void* b)
+{
+    const kpt_t* ka = *(const kpt_t* const*)a;
+    const kpt_t* kb = *(const kpt_t* const*)b;
+    if (ka->resp > kb->resp) return -1;
+    if (ka->resp < kb->resp) return 1;
+    return (ka < kb) ? -1 : (ka > kb);   /* stable by original position */
+}
+
+static const okp_t* g_sort_base;
+static int cmp_xy(const void* a, const void* b)
+{
+    int ia = *(const int*)a, ib = *(const int*)b;
+    const okp_t* A = g_sort_base + ia;
+    const okp_t* B = g_sort_base + ib;
+    if (A->x < B->x) return -1;
+    if (A->x > B->x) return 1;
+    if (A->y < B->y) return -1;
+    if (A->y > B->y) return 1;
+    return (ia < ib) ? -1 : (ia > ib);
+}
+
+int or_detect_describe(const uint8_t* img, int H, int W, int pitch, const or_params* p,
+                       int cap, float* kp_out, float* desc_out, uint8_t* desc_u8)
+{
+    if (p->n_octaves < 1 || p->n_octaves > 16 || p->n_scales < 1 || p->n_scales > 32) return -1;
+    pyramid_t P;
+    memset(&P, 0, sizeof(P));
+    double ct = p->adaptive_thresh ? or_adaptive_thresh(img, H, W, pitch, p->contrast_thresh) : p->contrast_thresh;
+    build_pyramid(img, H, W, pitch, p, &P);
+    kpt_t* kps = NULL;
+    int n = detect_keypoints(&P, p, ct, &kps);
+    if (n < p->retry_below) {
+        free(kps);
+        n = detect_keypoints(&P, p, ct * 0.5, &kps);
+    }
+    /* top-max_keypoints by response, stable (core.py:2078-2080) */
+    kpt_t* sel = kps;
+    kpt_t* sorted = NULL;
+    if (n > p->max_keypoints) {
+        kpt_t** ptrs = (kpt_t**)malloc(sizeof(kpt_t*) * n);
+        for (int i = 0; i < n; ++i) ptrs[i] = &kps[i];
+        qsort(ptrs, n, sizeof(kpt_t*), cmp_resp_desc);
+        sorted = (kpt_t*)malloc(sizeof(kpt_t) * p->max_keypoints);
+        for (int i = 0; i < p->max_keypoints; ++i) sorted[i] = *ptrs[i];
+        free(ptrs);
+        sel = sorted;
+        n = p->max_keypoints;
+    }
+    int S = p->n_scales;
+    float kf = (float)pow(2.0, 1.0 / (double)S);
+    double kd = pow(2.0, 1.0 / (double)S);
+    double cs = P.doubled ? 0.5 : 1.0;
+    /* orientation (core.py:1525-1603): sigma per si in float32 */
+    float* ang = (float*)malloc(sizeof(float) * 4 * (n > 0 ? n : 1));
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < n; ++i) {
+        float pw = (float)pow((double)kf, (double)sel[i].si);
+        float sig = (float)((float)p->sigma0 * pw);
+        size_t HW = (size_t)P.Ho[sel[i].o] * P.Wo[sel[i].o];
+        orientation_one(P.G[sel[i].o] + HW * sel[i].si, P.Ho[sel[i].o], P.Wo[sel[i].o],
+                        sel[i].ri, sel[i].ci, sig, ang + 4 * i);
+    }
+    int m = 0;
+    for (int i = 0; i < 4 * n; ++i) if (ang[i] >= 0.0f) m++;
+    okp_t* ok = (okp_t*)malloc(sizeof(okp_t) * (m > 0 ? m : 1));
+    int j = 0;
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 4; ++a)
+            if (ang[4 * i + a] >= 0.0f) {
+                double sf = ldexp(1.0, sel[i].o);
+                ok[j].kp = i;
+                ok[j].angle = ang[4 * i + a];
+                ok[j].x = (float)(((double)sel[i].ci * sf) * cs);
+                ok[j].y = (float)(((double)sel[i].ri * sf) * cs);
+                j++;
+            }
+    int* order = (int*)malloc(sizeof(int) * (m > 0 ? m : 1));
+    for (int i = 0; i < m; ++i) order[i] = i;
+    g_sort_base = ok;
+    qsort(order, m, sizeof(int), cmp_xy);
+    int nout = m < cap ? m : cap;
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (int t = 0; t < nout; ++t) {
+        const okp_t* e = &ok[order[t]];
+        const kpt_t* k = &sel[e->kp];
+        double sf = ldexp(1.0, k->o);
+        float* ko = kp_out + 6 * (size_t)t;
+        ko[0] = e->x;
+        ko[1] = e->y;
+        ko[2] = (float)((((p->sigma0 * pow(kd, (double)k->si)) * sf) * cs) * 2.0);
+        ko[3] = e->angle;
+        ko[4] = k->resp;
+        ko[5] = (float)k->o;
+        /* descriptor geometry (core.py:1806-1810) in float32 */
+        float pw = (float)pow((double)kf, (double)k->si);
+        float t0 = (float)((float)(3.0 * p->sigma0) * pw);
+        t0 = t0 * 1.41421354f;
+        t0 = t0 * 2.5f;
+        float phf = rintf(t0);
+        int ph = phf < 4.0f ? 4 : (int)phf;
+        float rad = e->angle * 0.0174532924f;
+        size_t HW = (size_t)P.Ho[k->o] * P.Wo[k->o];
+        float dtmp[128];
+        uint8_t utmp[128];
+        descriptor_one(P.G[k->o] + HW * k->si, P.Ho[k->o], P.Wo[k->o], k->ri, k->ci, rad, ph,
+                       p->rootsift, dtmp, utmp);
+        if (desc_out) memcpy(desc_out + 128 * (size_t)t, dtmp, sizeof(dtmp));
+        if (desc_u8) memcpy(desc_u8 + 128 * (size_t)t, utmp, sizeof(utmp));
+    }
+    free(order); free(ok); free(ang); free(sorted); free(kps);
+    pyr_free(&P);
+    return nout;
+}
+
+/* ========================================================================
+ * Exact matcher (core.py:2840-2940 restated on uint8 descriptors):
+ * d = sum (a-b)^2 in integers; top-2 with the smallest index kept on ties;
+ * ratio test d1 < ratio^2 * d2 (double); mutual check.
+ * ======================================================================== */
+void or_match_u8(const uint8_t* a, int N, const uint8_t* b, int M, double ratio,
+                 int32_t* fwd, int32_t* bwd, int32_t* pairs, int* npairs)
+{
+    const int32_t BIG = 0x7fffffff;
+    for (int j = 0; j < M; ++j) { bwd[3 * j] = -1; bwd[3 * j + 1] = BIG; bwd[3 * j + 2] = BIG; }
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+        int32_t d1 = BIG, d2 = BIG, j1 = -1;
+        const uint8_t* ai = a + 128 * (size_t)i;
+        for (int j = 0; j < M; ++j) {
+            const uint8_t* bj = b + 128 * (size_t)j;
+            int32_t d = 0;
+            for (int k = 0; k < 128; ++k) { int t = (int)ai[k] - (int)bj[k]; d += t * t; }
+            if (d < d1) { d2 = d1; d1 = d; j1 = j; }
+            else if (d < d2) d2 = d;
+        }
+        fwd[3 * i] = j1; fwd[3 * i + 1] = d1; fwd[3 * i + 2] = d2;
+    }
+    #pragma omp parallel for schedule(static)
+    for (int j = 0; j < M; ++j) {
+        int32_t d1 = BIG, d2 = BIG, i1 = -1;
+        const uint8_t* bj = b + 128 * (size_t)j;
+        for (int i = 0; i < N; ++i) {
+            const uint8_t* ai = a + 128 * (size_t)i;
+            int32_t d = 0;
+            for (int k = 0; k < 128; ++k) { int t = (int)ai[k] - (int)bj[k]; d += t * t; }
+            if (d < d1) { d2 = d1; d1 = d; i1 = i; }
+            else if (d < d2) d2 = d;
+        }
+        bwd[3 * j] = i1; bwd[3 * j + 1] = d1; bwd[3 * j + 2] = d2;
+    }
+    double r2 = ratio * ratio;
+    int np = 0;
+    for (int i = 0; i < N; ++i) {
+        int j = fwd[3 * i];
+        if (j < 0 || M < 2) continue;
+        if (!((double)fwd[3 * i + 1] < r2 * (double)fwd[3 * i + 2])) continue;
+        if (bwd[3 * j] != i) continue;
+        if (N < 2 || !((double)bwd[3 * j + 1] < r2 * (double)bwd[3 * j + 2])) continue;
+        pairs[2 * np] = i; pairs[2 * np + 1] = j; np++;
+    }
+    *npairs = np;
+}
+
+/* ========================================================================
+ * Philox4x32-10 (Salmon et al., SC'11)
+ * ======================================================================== */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+void or_ransac_sample(int B, int N, uint64_t seed, int32_t* idx)
+{
+    /* hypothesis b draws counter (b,0,0,0) under key (seed lo, seed hi);
+     * index = (u * N) >> 32 (with replacement, like torch.randint) */
+    uint32_t key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) };
+    for (int b = 0; b < B; ++b) {
+        uint32_t ctr[4] = { (uint32_t)b, 0u, 0u, 0u }, o[4];
+        or_philox4x32_10(ctr, key, o);
+        for (int k = 0; k < 4; ++k) idx[4 * b + k] = (int32_t)(((uint64_t)o[k] * (uint64_t)N) >> 32);
+    }
+}
+
+/* ========================================================================
+ * Homography (A14-A16, core.py:2946-2981, 2272-2313, 3011-3097)
+ * ======================================================================== */
+static double wave_sum_d(const double* v, int n)
+{
+    double p[LANES];
+    for (int l = 0; l < LANES; ++l) p[l] = 0.0;
+    for (int i = 0; i < n; ++i) p[i % LANES] = p[i % LANES] + v[i];
+    return wave_tree_d(p);
+}
+
+/* Hartley normalisation (core.py:2946-2957): centroid, mean distance md,
+ * s = sqrt(2)/(md + 1e-10).  Sums use the 64-lane order. */
+static void hartley(const double* pts, int N, const uint8_t* sel, double* T)
+{
+    double* buf = (double*)calloc((size_t)(N > 0 ? N : 1), sizeof(double));
+    int m = 0;
+    for (int i = 0; i < N; ++i) if (!sel || sel[i]) buf[m++] = pts[2 * i];
+    double cx = wave_sum_d(buf, m) / (double)m;
+    m = 0;
+    for (int i = 0; i < N; ++i) if (!sel || sel[i]) buf[m++] = pts[2 * i + 1];
+    double cy = wave_sum_d(buf, m) / (double)m;
+    m = 0;
+    for (int i = 0; i < N; ++i) if (!sel || sel[i]) {
+        double dx = pts[2 * i] - cx, dy = pts[2 * i + 1] - cy;
+        buf[m++] = sqrt(dx * dx + dy * dy);
+    }
+    double md = wave_sum_d(buf, m) / (double)m;
+    double s = 0x1.6a09e667f3bcdp+0 / (md + 1e-10);
+    T[0] = s; T[1] = 0.0; T[2] = -s * cx;
+    T[3] = 0.0; T[4] = s; T[5] = -s * cy;
+    T[6] = 0.0; T[7] = 0.0; T[8] = 1.0;
+    free(buf);
+}
+
+/* H = inv(T2) * Hn * T1, then / H[2][2] when |H22| > 1e-10 */
+static void denorm(const double* Hn, const double* T1, const double* T2, double* H)
+{
+    double s2 = T2[0], inv_s2 = 1.0 / s2;
+    double cx2 = -T2[2] / s2, cy2 = -T2[5] / s2;
+    double Ti[9] = { inv_s2, 0.0, cx2, 0.0, inv_s2, cy2, 0.0, 0.0, 1.0 };
+    double A[9];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            A[3 * r + c] = (Ti[3 * r] * Hn[c] + Ti[3 * r + 1] * Hn[3 + c]) + Ti[3 * r + 2] * Hn[6 + c];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            H[3 * r + c] = (A[3 * r] * T1[c] + A[3 * r + 1] * T1[3 + c]) + A[3 * r + 2] * T1[6 + c];
+    double h22 = H[8];
+    if (fabs(h22) > 1e-10) for (int i = 0; i < 9; ++i) H[i] = H[i] / h22;
+}
+
+/* Minimal 4-point solve: null vector of the 8x9 DLT matrix by Gaussian
+ * elimination with partial pivoting (first max on ties), h8 = 1. */
+static int solve8(double A[8][9], double* h)
+{
+    double amax = 0.0;
+    for (int r = 0; r < 8; ++r) for (int c = 0; c < 9; ++c) if (fabs(A[r][c]) > amax) amax = fabs(A[r][c]);
+    double tiny = amax * 1e-12;
+    for (int c = 0; c < 8; ++c) {
+        int p = c;
+        double best = fabs(A[c][c]);
+        for (int r = c + 1; r < 8; ++r) if (fabs(A[r][c]) > best) { best = fabs(A[r][c]); p = r; }
+        if (!(best > tiny)) return 0;
+        if (p != c) for (int k = 0; k < 9; ++k) { double t = A[c][k]; A[c][k] = A[p][k]; A[p][k] = t; }
+        for (int r = c + 1; r < 8; ++r) {
+            double f = A[r][c] / A[c][c];
+            for (int k = c; k < 9; ++k) A[r][k] = A[r][k] - f * A[c][k];
+        }
+    }
+    h[8] = 1.0;
+    for (int c = 7; c >= 0; --c) {
+        double s = A[c][8];
+        for (int k = c + 1; k < 8; ++k) s = s + A[c][k] * h[k];
+        h[c] = -s / A[c][c];
+    }
+    return 1;
+}
+
+static void dlt_rows(double x1, double y1, double x2, double y2, double* re, double* ro)
+{
+    re[0] = -x1; re[1] = -y1; re[2] = -1.0; re[3] = 0.0; re[4] = 0.0; re[5] = 0.0;
+    re[6] = x2 * x1; re[7] = x2 * y1; re[8] = x2;
+    ro[0] = 0.0; ro[1] = 0.0; ro[2] = 0.0; ro[3] = -x1; ro[4] = -y1; ro[5] = -1.0;
+    ro[6] = y2 * x1; ro[7] = y2 * y1; ro[8] = y2;
+}
+
+/* smallest-eigenvalue eigenvector of a symmetric 9x9 by cyclic Jacobi */
+static void jacobi9(double* M, double* vec)
+{
+    double V[81];
+    for (int i = 0; i < 81; ++i) V[i] = (i % 10 == 0) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int i = 0; i < 9; ++i) for (int j = 0; j < 9; ++j) {
+            if (i != j) off = off + M[9 * i + j] * M[9 * i + j];
+            else diag = diag + M[9 * i + j] * M[9 * i + j];
+        }
+        if (off <= diag * 1e-30) break;
+        for (int p = 0; p < 8; ++p)
+            for (int q = p + 1; q < 9; ++q) {
+                double apq = M[9 * p + q];
+                if (apq == 0.0) continue;
+                double app = M[9 * p + p], aqq = M[9 * q + q];
+                double theta = (aqq - app) / (2.0 * apq);
+                double t = 1.0 / (fabs(theta) + sqrt(theta * theta + 1.0));
+                if (theta < 0.0) t = -t;
+                double c = 1.0 / sqrt(t * t + 1.0);
+                double s = t * c;
+                for (int k = 0; k < 9; ++k) {
+                    double mkp = M[9 * k + p], mkq = M[9 * k + q];
+                    M[9 * k + p] = c * mkp - s * mkq;
+                    M[9 * k + q] = s * mkp + c * mkq;
+                }
+                for (int k = 0; k < 9; ++k) {
+                    double mpk = M[9 * p + k], mqk = M[9 * q + k];
+                    M[9 * p + k] = c * mpk - s * mqk;
+                    M[9 * q + k] = s * mpk + c * mqk;
+                }
+                for (int k = 0; k < 9; ++k) {
+                    double vkp = V[9 * k + p], vkq = V[9 * k + q];
+                    V[9 * k + p] = c * vkp - s * vkq;
+                    V[9 * k + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    int best = 0;
+    for (int i = 1; i < 9; ++i) if (M[9 * i + i] < M[9 * best + best]) best = i;
+    for (int k = 0; k < 9; ++k) vec[k] = V[9 * k + best];
+}
+
+/* least-squares DLT on the selected points (core.py:2959-2981): null vector
+ * of A^T A (accumulated in point order, 64-lane partials per entry). */
+static int dlt_ls_sel(const double* p1, const double* p2, int N, const uint8_t* sel, double* H)
+{
+    int m = 0;
+    for (int i = 0; i < N; ++i) if (!sel || sel[i]) m++;
+    if (m < 4) return 0;
+    double T1[9], T2[9];
+    hartley(p1, N, sel, T1);
+    hartley(p2, N, sel, T2);
+    double part[LANES][45];
+    memset(part, 0, sizeof(part));
+    int k = 0;
+    for (int i = 0; i < N; ++i) {
+        if (sel && !sel[i]) continue;
+        double x1 = T1[0] * p1[2 * i] + T1[2], y1 = T1[4] * p1[2 * i + 1] + T1[5];
+        double x2 = T2[0] * p2[2 * i] + T2[2], y2 = T2[4] * p2[2 * i + 1] + T2[5];
+        double re[9], ro[9];
+        dlt_rows(x1, y1, x2, y2, re, ro);
+        int lane = k % LANES, e = 0;
+        for (int a = 0; a < 9; ++a)
+            for (int b = a; b < 9; ++b) {
+                part[lane][e] = part[lane][e] + (re[a] * re[b] + ro[a] * ro[b]);
+                e++;
+            }
+        k++;
+    }
+    double M[81];
+    int e = 0;
+    for (int a = 0; a < 9; ++a)
+        for (int b = a; b < 9; ++b) {
+            double t[LANES];
+            for (int l = 0; l < LANES; ++l) t[l] = part[l][e];
+            double v = wave_tree_d(t);
+            M[9 * a + b] = v; M[9 * b + a] = v;
+            e++;
+        }
+    double h[9];
+    jacobi9(M, h);
+    denorm(h, T1, T2, H);
+    return 1;
+}
+
+int or_dlt_ls(const double* p1, const double* p2, int N, double* H)
+{
+    return dlt_ls_sel(p1, p2, N, NULL, H) ? 0 : -1;
+}
+
+static double sym_err(const double* H, const double* Hi, double x1, double y1, double x2, double y2, int clampz)
+{
+    double fx = (H[0] * x1 + H[1] * y1) + H[2];
+    double fy = (H[3] * x1 + H[4] * y1) + H[5];
+    double fz = (H[6] * x1 + H[7] * y1) + H[8];
+    if (clampz && fz < 1e-10) fz = 1e-10;
+    double ex = fx / fz - x2, ey = fy / fz - y2;
+    double bx = (Hi[0] * x2 + Hi[1] * y2) + Hi[2];
+    double by = (Hi[3] * x2 + Hi[4] * y2) + Hi[5];
+    double bz = (Hi[6] * x2 + Hi[7] * y2) + Hi[8];
+    if (clampz && bz < 1e-10) bz = 1e-10;
+    double gx = bx / bz - x1, gy = by / bz - y1;
+    return (ex * ex + ey * ey) + (gx * gx + gy * gy);
+}
+
+/* adjugate inverse; det replaced by 1 when |det| <= 1e-12 (core.py:2298-2307) */
+static void inv3(const double* a, double* o)
+{
+    double det = (a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6])) + a[2] * (a[3] * a[7] - a[4] * a[6]);
+    if (!(fabs(det) > 1e-12)) det = 1.0;
+    o[0] = (a[4] * a[8] - a[5] * a[7]) / det; o[1] = (a[2] * a[7] - a[1] * a[8]) / det; o[2] = (a[1] * a[5] - a[2] * a[4]) / det;
+    o[3] = (a[5] * a[6] - a[3] * a[8]) / det; o[4] = (a[0] * a[8] - a[2] * a[6]) / det; o[5] = (a[2] * a[3] - a[0] * a[5]) / det;
+    o[6] = (a[3] * a[7] - a[4] * a[6]) / det; o[7] = (a[1] * a[6] - a[0] * a[7]) / det; o[8] = (a[0] * a[4] - a[1] * a[3]) / det;
+}
+
+int or_ransac(const double* p1, const double* p2, int N, uint64_t seed, int B,
+              double thr, double sigma_max, int method, int lo, int min_inliers,
+              double* Hout, uint8_t* mask, double* scores, double* Hs)
+{
+    if (N < 4) return -1;
+    double tq = thr * thr;
+    double T1[9], T2[9];
+    hartley(p1, N, NULL, T1);
+    hartley(p2, N, NULL, T2);
+    double* n1 = (double*)malloc(sizeof(double) * 2 * N);
+    double* n2 = (double*)malloc(sizeof(double) * 2 * N);
+    for (int i = 0; i < N; ++i) {
+        n1[2 * i] = T1[0] * p1[2 * i] + T1[2]; n1[2 * i + 1] = T1[4] * p1[2 * i + 1] + T1[5];
+        n2[2 * i] = T2[0] * p2[2 * i] + T2[2]; n2[2 * i + 1] = T2[4] * p2[2 * i + 1] + T2[5];
+    }
+    int32_t* idx = (int32_t*)malloc(sizeof(int32_t) * 4 * B);
+    or_ransac_sample(B, N, seed, idx);
+    double* sc = (double*)malloc(sizeof(double) * B);
+    double* HB = (double*)malloc(sizeof(double) * 9 * B);
+    double den = 2.0 * (sigma_max * sigma_max);
+    #pragma omp parallel for schedule(dynamic, 8)
+    for (int b = 0; b < B; ++b) {
+        double A[8][9], h[9], H[9], Hi[9];
+        for (int k = 0; k < 4; ++k) {
+            int i = idx[4 * b + k];
+            dlt_rows(n1[2 * i], n1[2 * i + 1], n2[2 * i], n2[2 * i + 1], A[2 * k], A[2 * k + 1]);
+        }
+        if (!solve8(A, h)) {
+            sc[b] = -1.0;
+            for (int k = 0; k < 9; ++k) HB[9 * b + k] = (k % 4 == 0) ? 1.0 : 0.0;
+            continue;
+        }
+        denorm(h, T1, T2, H);
+        inv3(H, Hi);
+        double part[LANES];
+        for (int l = 0; l < LANES; ++l) part[l] = 0.0;
+        for (int i = 0; i < N; ++i) {
+            double e = sym_err(H, Hi, p1[2 * i], p1[2 * i + 1], p2[2 * i], p2[2 * i + 1], 1);
+            double w = (method == 0) ? or_exp(-e / den) : (e < tq ? 1.0 : 0.0);
+            part[i % LANES] = part[i % LANES] + w;
+        }
+        sc[b] = wave_tree_d(part);
+        memcpy(HB + 9 * b, H, sizeof(H));
+    }
+    int best = 0;
+    for (int b = 1; b < B; ++b) if (sc[b] > sc[best]) best = b;
+    double Hb[9], Hbi[9];
+    memcpy(Hb, HB + 9 * best, sizeof(Hb));
+    inv3(Hb, Hbi);
+    int nb = 0;
+    for (int i = 0; i < N; ++i) {
+        double e = sym_err(Hb, Hbi, p1[2 * i], p1[2 * i + 1], p2[2 * i], p2[2 * i + 1], 1);
+        mask[i] = e < tq;
+        nb += mask[i];
+    }
+    memcpy(Hout, Hb, sizeof(Hb));
+    if (lo && nb >= min_inliers) {
+        double H1[9], H1i[9];
+        dlt_ls_sel(p1, p2, N, mask, H1);
+        inv3(H1, H1i);
+        uint8_t* m2 = (uint8_t*)malloc(N);
+        int n2c = 0;
+        for (int i = 0; i < N; ++i) {
+            double e = sym_err(H1, H1i, p1[2 * i], p1[2 * i + 1], p2[2 * i], p2[2 * i + 1], 0);
+            m2[i] = e < tq;
+            n2c += m2[i];
+        }
+        if (n2c >= min_inliers) {
+            dlt_ls_sel(p1, p2, N, m2, Hout);
+            memcpy(mask, m2, N);
+        } else {
+            memcpy(Hout, H1, sizeof(H1));
+        }
+        free(m2);
+    }
+    if (scores) memcpy(scores, sc, sizeof(double) * B);
+    if (Hs) memcpy(Hs, HB, sizeof(double) * 9 * B);
+    free(n1); free(n2); free(idx); free(sc); free(HB);
+    return best;
+}
+
+/* ========================================================================
+ * A17 + feather blend.  Canvas (core.py:3171-3187).  Warp: inverse map with
+ * H^-1 rounded to float32 (core.py:3225), bilinear with zero padding
+ * (core.py:3233), valid where the source point lies in [0,w-1]x[0,h-1].
+ * img2 is pasted at (round(ox), round(oy)) (core.py:3794-3800).  Feather
+ * weight of a valid pixel = min(sx+1, w-sx, sy+1, h-sy) in its own source
+ * coordinates; output = trunc(clip(sum w_i I_i / sum w_i)).
+ * ======================================================================== */
+void or_canvas(int h1, int w1, int h2, int w2, const double* H,
+               int* cw, int* ch, double* ox, double* oy)
+{
+    double cx[4] = { 0, (double)w1, (double)w1, 0 }, cy[4] = { 0, 0, (double)h1, (double)h1 };
+    double ax[8], ay[8];
+    for (int i = 0; i < 4; ++i) {
+        double X = (H[0] * cx[i] + H[1] * cy[i]) + H[2];
+        double Y = (H[3] * cx[i] + H[4] * cy[i]) + H[5];
+        double Z = (H[6] * cx[i] + H[7] * cy[i]) + H[8];
+        ax[i] = X / Z; ay[i] = Y / Z;
+    }
+    ax[4] = 0; ax[5] = w2; ax[6] = w2; ax[7] = 0;
+    ay[4] = 0; ay[5] = 0; ay[6] = h2; ay[7] = h2;
+    double mnx = ax[0], mxx = ax[0], mny = ay[0], mxy = ay[0];
+    for (int i = 1; i < 8; ++i) {
+        if (ax[i] < mnx) mnx = ax[i];
+        if (ax[i] > mxx) mxx = ax[i];
+        if (ay[i] < mny) mny = ay[i];
+        if (ay[i] > mxy) mxy = ay[i];
+    }
+    *ox = -mnx > 0.0 ? -mnx : 0.0;
+    *oy = -mny > 0.0 ? -mny : 0.0;
+    *cw = (int)ceil(mxx + *ox) + 1;
+    *ch = (int)ceil(mxy + *oy) + 1;
+}
+
+void or_warp_feather(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h2, int w2,
+                     const double* H, int cw, int ch, double ox, double oy,
+                     uint8_t* pano, uint8_t* valid)
+{
+    double Hi[9];
+    inv3(H, Hi);
+    float hf[9];
+    for (int i = 0; i < 9; ++i) hf[i] = (float)Hi[i];
+    float oxf = (float)ox, oyf = (float)oy;
+    int iox = (int)lround(ox), ioy = (int)lround(oy);
+    #pragma omp parallel for schedule(static)
+    for (int v = 0; v < ch; ++v)
+        for (int u = 0; u < cw; ++u) {
+            float X = (float)u - oxf, Y = (float)v - oyf;
+            float sxn = (hf[0] * X + hf[1] * Y) + hf[2];
+            float syn = (hf[3] * X + hf[4] * Y) + hf[5];
+            float szn = (hf[6] * X + hf[7] * Y) + hf[8];
+            if (szn < 1e-10f) szn = 1e-10f;
+            float sx = sxn / szn, sy = syn / szn;
+            float acc[3] = { 0, 0, 0 }, wsum = 0.0f;
+            if (sx >= 0.0f && sx <= (float)(w1 - 1) && sy >= 0.0f && sy <= (float)(h1 - 1)) {
+                float fx = floorf(sx), fy = floorf(sy);
+                int x0 = (int)fx, y0 = (int)fy;
+                float ax = sx - fx, ay = sy - fy;
+                float w00 = (1.0f - ax) * (1.0f - ay), w01 = ax * (1.0f - ay);
+                float w10 = (1.0f - ax) * ay, w11 = ax * ay;
+                float wf = sx + 1.0f;
+                if ((float)w1 - sx < wf) wf = (float)w1 - sx;
+                if (sy + 1.0f < wf) wf = sy + 1.0f;
+                if ((float)h1 - sy < wf) wf = (float)h1 - sy;
+                for (int c = 0; c < 3; ++c) {
+                    float s = 0.0f;
+                    float v00 = img1[((size_t)y0 * w1 + x0) * 3 + c];
+                    float v01 = (x0 + 1 < w1) ? img1[((size_t)y0 * w1 + x0 + 1) * 3 + c] : 0.0f;
+                    float v10 = (y0 + 1 < h1) ? img1[((size_t)(y0 + 1) * w1 + x0) * 3 + c] : 0.0f;
+                    float v11 = (x0 + 1 < w1 && y0 + 1 < h1) ? img1[((size_t)(y0 + 1) * w1 + x0 + 1) * 3 + c] : 0.0f;
+                    s = ((w00 * v00 + w01 * v01) + w10 * v10) + w11 * v11;
+                    acc[c] = wf * s;
+                }
+                wsum = wf;
+            }
+            int px = u - iox, py = v - ioy;
+            if (px >= 0 && px < w2 && py >= 0 && py < h2) {
+                float wf = (float)(px + 1);
+                if ((float)(w2 - px) < wf) wf = (float)(w2 - px);
+                if ((float)(py + 1) < wf) wf = (float)(py + 1);
+                if ((float)(h2 - py) < wf) wf = (float)(h2 - py);
+                for (int c = 0; c < 3; ++c) acc[c] = acc[c] + wf * (float)img2[((size_t)py * w2 + px) * 3 + c];
+                wsum = wsum + wf;
+            }
+            size_t o = (size_t)v * cw + u;
+            valid[o] = wsum > 0.0f;
+            for (int c = 0; c < 3; ++c) {
+                float val = wsum > 0.0f ? acc[c] / wsum : 0.0f;
+                if (val < 0.0f) val = 0.0f;
+                if (val > 255.0f) val = 255.0f;
+                pano[o * 3 + c] = (uint8_t)(int)val;
+            }
+        }
+}
+
+void or_bgr_to_gray(const uint8_t* bgr, int H, int W, uint8_t* gray)
+{
+    /* fixed-point BT.601 (B*1868 + G*9617 + R*4899 + 2^13) >> 14 */
+    for (size_t i = 0; i < (size_t)H * W; ++i) {
+        int b = bgr[3 * i], g = bgr[3 * i + 1], r = bgr[3 * i + 2];
+        gray[i] = (uint8_t)((b * 1868 + g * 9617 + r * 4899 + 8192) >> 14);
+    }
+}
+
+int or_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}

@@ -1,0 +1,1012 @@
+// Detect + describe on one image: Gaussian/DoG pyramid, 3x3x3 extrema with
+// deterministic raster-order compaction, FP64 sub-pixel refinement,
+// orientation and 4x4x8 descriptor histograms with wave64 lane-private
+// partials.  Restates PySIFT.detectAndCompute (pysift/core.py:2021-2172).
+#include "msift_internal.hpp"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#pragma clang fp contract(off)
+
+namespace msift {
+
+// ======================================================================
+// misc small kernels
+// ======================================================================
+__global__ void k_zero_i32(int* p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// 256-bin histogram of the u8 image (integer atomics: order-free, exact)
+__global__ __launch_bounds__(256) void k_hist256(const uint8_t* __restrict__ img, int H, int W,
+                                                  int pitch, int* __restrict__ hist) {
+    __shared__ int sh[256];
+    sh[threadIdx.x] = 0;
+    __syncthreads();
+    int y = blockIdx.y;
+    for (int x = blockIdx.x * 256 + threadIdx.x; x < W; x += gridDim.x * 256)
+        atomicAdd(&sh[img[(size_t)y * pitch + x]], 1);
+    __syncthreads();
+    if (sh[threadIdx.x]) atomicAdd(&hist[threadIdx.x], sh[threadIdx.x]);
+}
+
+// _adaptive_contrast_thresh (core.py:1129-1148): entropy in FP64, bins in
+// index order; writes thr[0] = f32(ct'/S), thr[1] = f32(ct'*0.5/S)
+__global__ void k_thresh(const int* __restrict__ hist, double total, double base, int S,
+                         int adaptive, float* __restrict__ thr) {
+    if (threadIdx.x != 0) return;
+    double ct = base;
+    if (adaptive) {
+        double ent = 0.0;
+        for (int b = 0; b < 256; ++b) {
+            if (hist[b] == 0) continue;
+            double p = (double)hist[b] / total;
+            ent = ent - p * log2_d(p);
+        }
+        double scale = 0.8 + ent / 40.0;
+        if (scale > 1.0) scale = 1.0;
+        ct = base * scale;
+    }
+    thr[0] = (float)(ct / (double)S);
+    thr[1] = (float)((ct * 0.5) / (double)S);
+}
+
+// ======================================================================
+// A3: cubic B-spline x2 zoom (core.py:1274), mirror-boundary prefilter as a
+// 61-tap symmetric FIR in FP64, then 4x4 tensor-product interpolation.
+// ======================================================================
+__global__ __launch_bounds__(256) void k_ups_pf_cols(const uint8_t* __restrict__ img, int pitch,
+                                                      int H, int W, UpsTaps h,
+                                                      double* __restrict__ out) {
+    int x = blockIdx.x * 256 + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= W) return;
+    double acc = 0.0;
+#pragma unroll 4
+    for (int k = -kUpsK; k <= kUpsK; ++k) {
+        float u = (float)img[(size_t)mirror_idx(y + k, H) * pitch + x] / 255.0f;
+        acc = fma(h.h[k < 0 ? -k : k], (double)u, acc);
+    }
+    out[(size_t)y * W + x] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_ups_pf_rows(const double* __restrict__ in, int H, int W,
+                                                      UpsTaps h, double* __restrict__ out) {
+    int x = blockIdx.x * 256 + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= W) return;
+    const double* row = in + (size_t)y * W;
+    double acc = 0.0;
+#pragma unroll 4
+    for (int k = -kUpsK; k <= kUpsK; ++k) acc = fma(h.h[k < 0 ? -k : k], row[mirror_idx(x + k, W)], acc);
+    out[(size_t)y * W + x] = acc;
+}
+
+MSIFT_HD void bspline_w(double t, double* w) {
+    double s = 1.0 - t;
+    double t2 = t * t, t3 = t2 * t, s2 = s * s, s3 = s2 * s;
+    w[0] = s3 * (1.0 / 6.0);
+    w[1] = (2.0 / 3.0 - t2) + 0.5 * t3;
+    w[2] = (2.0 / 3.0 - s2) + 0.5 * s3;
+    w[3] = t3 * (1.0 / 6.0);
+}
+
+__global__ __launch_bounds__(256) void k_ups_interp(const double* __restrict__ cf, int H, int W,
+                                                     double zy, double zx, float* __restrict__ out) {
+    int Wo = 2 * W;
+    int ox = blockIdx.x * 256 + threadIdx.x;
+    int oy = blockIdx.y;
+    if (ox >= Wo) return;
+    double yy = (double)oy * zy;
+    double fy = floor(yy);
+    int iy = (int)fy;
+    double wy[4];
+    bspline_w(yy - fy, wy);
+    double xx = (double)ox * zx;
+    double fx = floor(xx);
+    int ix = (int)fx;
+    double wx[4];
+    bspline_w(xx - fx, wx);
+    int xi[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) xi[b] = mirror_idx(ix - 1 + b, W);
+    double v = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const double* row = cf + (size_t)mirror_idx(iy - 1 + a, H) * W;
+        double rv = 0.0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) rv = fma(wx[b], row[xi[b]], rv);
+        v = fma(wy[a], rv, v);
+    }
+    out[(size_t)oy * Wo + ox] = (float)v;
+}
+
+__global__ __launch_bounds__(256) void k_u8_unit(const uint8_t* __restrict__ img, int pitch, int H,
+                                                  int W, float* __restrict__ out) {
+    int x = blockIdx.x * 256 + threadIdx.x;
+    int y = blockIdx.y;
+    if (x < W) out[(size_t)y * W + x] = (float)img[(size_t)y * pitch + x] / 255.0f;
+}
+
+__global__ __launch_bounds__(256) void k_copy_f32(const float* __restrict__ in, float* __restrict__ out,
+                                                   size_t n) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+
+// ======================================================================
+// A2/A4: one pyramid level.  G[s+1] = blur(G[s]) (H pass then V pass, each
+// output a fused-multiply-add chain over taps in ascending order, the
+// reference's gauss_h/gauss_v, core.py:475-547), D[s] = G[s+1] - G[s]
+// (core.py:146), and optionally the next octave's base G[S][::2, ::2].
+// A 64 x TH output tile is staged through LDS with its reflected halo.
+// ======================================================================
+template <int R, int TH>
+__global__ __launch_bounds__(256) void k_blur(const float* __restrict__ in, float* __restrict__ gout,
+                                               float* __restrict__ dout, float* __restrict__ dec,
+                                               int H, int W, int decW, Taps taps) {
+    constexpr int TW = 64;
+    constexpr int IW = TW + 2 * R;
+    constexpr int IWP = (IW + 4 + 3) & ~3;
+    constexpr int IH = TH + 2 * R;
+    constexpr int NV = 4 + 2 * R;
+    constexpr int NV4 = (NV + 3) / 4;
+    __shared__ __attribute__((aligned(16))) float sIn[IH * IWP];
+    __shared__ __attribute__((aligned(16))) float sH[IH * TW];
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    for (int i = tid; i < IH * IW; i += 256) {
+        int ly = i / IW, lx = i - ly * IW;
+        int gy = reflect_idx(y0 - R + ly, H), gx = reflect_idx(x0 - R + lx, W);
+        sIn[ly * IWP + lx] = in[(size_t)gy * W + gx];
+    }
+    __syncthreads();
+    // horizontal pass: 4 consecutive outputs per thread
+    for (int i = tid; i < IH * (TW / 4); i += 256) {
+        int ly = i / (TW / 4), q = i - ly * (TW / 4);
+        const float* row = sIn + ly * IWP + 4 * q;
+        float v[NV4 * 4];
+#pragma unroll
+        for (int k = 0; k < NV4; ++k) {
+            float4 t = *reinterpret_cast<const float4*>(row + 4 * k);
+            v[4 * k] = t.x; v[4 * k + 1] = t.y; v[4 * k + 2] = t.z; v[4 * k + 3] = t.w;
+        }
+        float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f, o3 = 0.0f;
+#pragma unroll
+        for (int k = 0; k <= 2 * R; ++k) {
+            float w = taps.w[k];
+            o0 = fmaf(v[k], w, o0);
+            o1 = fmaf(v[k + 1], w, o1);
+            o2 = fmaf(v[k + 2], w, o2);
+            o3 = fmaf(v[k + 3], w, o3);
+        }
+        *reinterpret_cast<float4*>(sH + ly * TW + 4 * q) = make_float4(o0, o1, o2, o3);
+    }
+    __syncthreads();
+    // vertical pass: 64 columns x 4 row groups of TH/4 rows
+    constexpr int RG = TH / 4;
+    const int lx = tid & 63, g = tid >> 6;
+    float v[RG + 2 * R];
+#pragma unroll
+    for (int k = 0; k < RG + 2 * R; ++k) v[k] = sH[(g * RG + k) * TW + lx];
+    const int gx = x0 + lx;
+#pragma unroll
+    for (int j = 0; j < RG; ++j) {
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 0; k <= 2 * R; ++k) s = fmaf(v[j + k], taps.w[k], s);
+        int gy = y0 + g * RG + j;
+        if (gy < H && gx < W) {
+            size_t o = (size_t)gy * W + gx;
+            gout[o] = s;
+            if (dout) dout[o] = s - sIn[(g * RG + j + R) * IWP + lx + R];
+            if (dec && !(gy & 1) && !(gx & 1)) dec[(size_t)(gy >> 1) * decW + (gx >> 1)] = s;
+        }
+    }
+}
+
+typedef void (*blur_fn)(const float*, float*, float*, float*, int, int, int, Taps);
+
+template <int R>
+static void blur_launch(const float* in, float* gout, float* dout, float* dec, int H, int W, int decW,
+                        const Taps& t, hipStream_t s) {
+    // tile height keeps LDS under 64 KB: 64 rows up to R=12, 32 up to R=24, else 16
+    if (R <= 12) {
+        dim3 grid((W + 63) / 64, (H + 63) / 64);
+        hipLaunchKernelGGL((k_blur<R, 64>), grid, dim3(256), 0, s, in, gout, dout, dec, H, W, decW, t);
+    } else if (R <= 24) {
+        dim3 grid((W + 63) / 64, (H + 31) / 32);
+        hipLaunchKernelGGL((k_blur<R, (R <= 24 ? 32 : 16)>), grid, dim3(256), 0, s, in, gout, dout, dec, H, W,
+                           decW, t);
+    } else {
+        dim3 grid((W + 63) / 64, (H + 15) / 16);
+        hipLaunchKernelGGL((k_blur<R, 16>), grid, dim3(256), 0, s, in, gout, dout, dec, H, W, decW, t);
+    }
+}
+
+static int blur(const float* in, float* gout, float* dout, float* dec, int H, int W, double sigma,
+                const Taps& t, hipStream_t s) {
+    int decW = (W + 1) / 2;
+    if (sigma < 0.01) {
+        // _fast_gaussian returns its input unchanged (core.py:559-560)
+        size_t n = (size_t)H * W;
+        hipLaunchKernelGGL(k_copy_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, gout, n);
+        return MSIFT_OK;
+    }
+    switch (t.r) {
+#define MSIFT_BLUR_CASE(RR) case RR: blur_launch<RR>(in, gout, dout, dec, H, W, decW, t, s); break;
+        MSIFT_BLUR_CASE(0) MSIFT_BLUR_CASE(1) MSIFT_BLUR_CASE(2) MSIFT_BLUR_CASE(3) MSIFT_BLUR_CASE(4)
+        MSIFT_BLUR_CASE(5) MSIFT_BLUR_CASE(6) MSIFT_BLUR_CASE(7) MSIFT_BLUR_CASE(8) MSIFT_BLUR_CASE(9)
+        MSIFT_BLUR_CASE(10) MSIFT_BLUR_CASE(11) MSIFT_BLUR_CASE(12) MSIFT_BLUR_CASE(13) MSIFT_BLUR_CASE(14)
+        MSIFT_BLUR_CASE(15) MSIFT_BLUR_CASE(16) MSIFT_BLUR_CASE(17) MSIFT_BLUR_CASE(18) MSIFT_BLUR_CASE(19)
+        MSIFT_BLUR_CASE(20) MSIFT_BLUR_CASE(21) MSIFT_BLUR_CASE(22) MSIFT_BLUR_CASE(23) MSIFT_BLUR_CASE(24)
+        MSIFT_BLUR_CASE(25) MSIFT_BLUR_CASE(26) MSIFT_BLUR_CASE(27) MSIFT_BLUR_CASE(28) MSIFT_BLUR_CASE(29)
+        MSIFT_BLUR_CASE(30) MSIFT_BLUR_CASE(31) MSIFT_BLUR_CASE(32)
+#undef MSIFT_BLUR_CASE
+    default:
+        set_error("blur radius above 32 (sigma too large)");
+        return MSIFT_EINVAL;
+    }
+    return MSIFT_OK;
+}
+
+// ======================================================================
+// A5: 3x3x3 extrema (core.py:176-217).  Blocks walk (octave, s, row,
+// 256-column segment) in raster order; pass 1 stores each wave's ballot and
+// the block count, a scan turns counts into offsets, pass 2 writes the
+// candidates at their raster-order position (first `capc` per octave kept).
+// ======================================================================
+struct ExtPos { int o, s, r, c; };
+
+__device__ __forceinline__ ExtPos ext_decode(const OctInfo& oi, int b) {
+    int o = 0;
+    while (o + 1 < oi.n_oct && b >= oi.blk0[o + 1]) ++o;
+    int lb = b - oi.blk0[o];
+    int bd = oi.border;
+    int nrows = oi.H[o] - 2 * bd, nseg = oi.nseg[o];
+    int per_s = nrows * nseg;
+    int s = 1 + lb / per_s;
+    int rem = lb - (s - 1) * per_s;
+    int r = bd + rem / nseg;
+    int seg = rem - (r - bd) * nseg;
+    ExtPos p;
+    p.o = o; p.s = s; p.r = r; p.c = bd + seg * 256 + (int)threadIdx.x;
+    return p;
+}
+
+__device__ __forceinline__ bool is_extremum(const float* __restrict__ D, int H, int W, int s, int r, int c,
+                                            float thr) {
+    size_t HW = (size_t)H * W;
+    float v = D[(size_t)s * HW + (size_t)r * W + c];
+    if (fabsf(v) <= thr) return false;
+    bool mx = true, mn = true;
+#pragma unroll
+    for (int ds = -1; ds <= 1; ++ds)
+#pragma unroll
+        for (int dr = -1; dr <= 1; ++dr)
+#pragma unroll
+            for (int dc = -1; dc <= 1; ++dc) {
+                if (ds == 0 && dr == 0 && dc == 0) continue;
+                float nb = D[(size_t)(s + ds) * HW + (size_t)(r + dr) * W + (c + dc)];
+                mx = mx && (v >= nb);
+                mn = mn && (v <= nb);
+            }
+    return mx || mn;
+}
+
+__global__ __launch_bounds__(256) void k_ext_count(OctInfo oi, const float* __restrict__ thr, int thr_idx,
+                                                    const int* __restrict__ gate, int gate_below,
+                                                    unsigned long long* __restrict__ masks,
+                                                    int* __restrict__ counts) {
+    if (gate && *gate >= gate_below) return;
+    __shared__ int wc[4];
+    ExtPos p = ext_decode(oi, blockIdx.x);
+    bool pred = false;
+    if (p.c < oi.W[p.o] - oi.border)
+        pred = is_extremum(oi.D[p.o], oi.H[p.o], oi.W[p.o], p.s, p.r, p.c, thr[thr_idx]);
+    unsigned long long m = __ballot(pred);
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        masks[(size_t)blockIdx.x * 4 + wave] = m;
+        wc[wave] = __popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+}
+
+__global__ __launch_bounds__(256) void k_ext_write(OctInfo oi, const unsigned long long* __restrict__ masks,
+                                                    const int* __restrict__ offs, int capc,
+                                                    int4* __restrict__ cands, const int* __restrict__ gate,
+                                                    int gate_below) {
+    if (gate && *gate >= gate_below) return;
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const unsigned long long* mb = masks + (size_t)blockIdx.x * 4;
+    unsigned long long m = mb[wave];
+    if (!((m >> lane) & 1ull)) return;
+    ExtPos p = ext_decode(oi, blockIdx.x);
+    int pre = 0;
+    for (int w = 0; w < wave; ++w) pre += __popcll(mb[w]);
+    int pos = offs[blockIdx.x] + pre + __popcll(m & lanemask_lt());
+    int q = pos - offs[oi.blk0[p.o]];
+    if (q < capc) cands[(size_t)p.o * capc + q] = make_int4(p.s, p.r, p.c, 0);
+}
+
+__global__ void k_oct_counts(OctInfo oi, const int* __restrict__ offs, int capc, int* __restrict__ ccount,
+                             const int* __restrict__ gate, int gate_below) {
+    if (gate && *gate >= gate_below) return;
+    int o = threadIdx.x;
+    if (o < oi.n_oct) {
+        int n = offs[oi.blk0[o + 1]] - offs[oi.blk0[o]];
+        ccount[o] = n < capc ? n : capc;
+    }
+}
+
+// ======================================================================
+// A6: sub-pixel refinement (core.py:343-463) in FP64
+// ======================================================================
+__global__ __launch_bounds__(256) void k_refine(OctInfo oi, const int4* __restrict__ cands,
+                                                 const int* __restrict__ ccount, int capc,
+                                                 const float* __restrict__ thr, int thr_idx, float edge32,
+                                                 uint8_t* __restrict__ valid, Kp* __restrict__ kslot,
+                                                 const int* __restrict__ gate, int gate_below) {
+    if (gate && *gate >= gate_below) return;
+    int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= oi.n_oct * capc) return;
+    int o = t / capc, i = t - o * capc;
+    uint8_t ok = 0;
+    if (i < ccount[o]) {
+        int4 cd = cands[t];
+        int si = cd.x, row = cd.y, col = cd.z;
+        const int H = oi.H[o], W = oi.W[o], S = oi.S, bd = oi.border;
+        const size_t HW = (size_t)H * W;
+        const float* D = oi.D[o];
+#define DV(s_, r_, c_) ((double)D[(size_t)(s_) * HW + (size_t)(r_) * W + (c_)])
+        double ox = 0.0, oy = 0.0, os = 0.0;
+        bool conv = false;
+        for (int it = 0; it < 5; ++it) {
+            if (si < 1 || si > S || row < bd || row >= H - bd || col < bd || col >= W - bd) break;
+            double cv = DV(si, row, col);
+            double dx = (DV(si, row, col + 1) - DV(si, row, col - 1)) * 0.5;
+            double dy = (DV(si, row + 1, col) - DV(si, row - 1, col)) * 0.5;
+            double ds = (DV(si + 1, row, col) - DV(si - 1, row, col)) * 0.5;
+            double dxx = (DV(si, row, col + 1) - 2.0 * cv) + DV(si, row, col - 1);
+            double dyy = (DV(si, row + 1, col) - 2.0 * cv) + DV(si, row - 1, col);
+            double dss = (DV(si + 1, row, col) - 2.0 * cv) + DV(si - 1, row, col);
+            double dxy = (((DV(si, row + 1, col + 1) - DV(si, row + 1, col - 1)) - DV(si, row - 1, col + 1)) +
+                          DV(si, row - 1, col - 1)) * 0.25;
+            double dxs = (((DV(si + 1, row, col + 1) - DV(si + 1, row, col - 1)) - DV(si - 1, row, col + 1)) +
+                          DV(si - 1, row, col - 1)) * 0.25;
+            double dys = (((DV(si + 1, row + 1, col) - DV(si + 1, row - 1, col)) - DV(si - 1, row + 1, col)) +
+                          DV(si - 1, row - 1, col)) * 0.25;
+            double det = (dxx * (dyy * dss - dys * dys) - dxy * (dxy * dss - dys * dxs)) + dxs * (dxy * dys - dyy * dxs);
+            if (fabs(det) < 1e-12) break;
+            double pp = -dx, q = -dy, rv = -ds;
+            ox = ((pp * (dyy * dss - dys * dys) - dxy * (q * dss - dys * rv)) + dxs * (q * dys - dyy * rv)) / det;
+            oy = ((dxx * (q * dss - dys * rv) - pp * (dxy * dss - dys * dxs)) + dxs * (dxy * rv - q * dxs)) / det;
+            os = ((dxx * (dyy * rv - q * dys) - dxy * (dxy * rv - q * dxs)) + pp * (dxy * dys - dyy * dxs)) / det;
+            if (fabs(ox) < 0.5 && fabs(oy) < 0.5 && fabs(os) < 0.5) { conv = true; break; }
+            col += (ox >= 0.0) ? (int)(ox + 0.5) : (int)(ox - 0.5);
+            row += (oy >= 0.0) ? (int)(oy + 0.5) : (int)(oy - 0.5);
+            si += (os >= 0.0) ? (int)(os + 0.5) : (int)(os - 0.5);
+        }
+        if (conv && !(si < 1 || si > S || row < bd || row >= H - bd || col < bd || col >= W - bd)) {
+            double cv = DV(si, row, col);
+            double dx = (DV(si, row, col + 1) - DV(si, row, col - 1)) * 0.5;
+            double dy = (DV(si, row + 1, col) - DV(si, row - 1, col)) * 0.5;
+            double ds = (DV(si + 1, row, col) - DV(si - 1, row, col)) * 0.5;
+            double resp = fabs(cv + 0.5 * ((dx * ox + dy * oy) + ds * os));
+            if (resp >= (double)thr[thr_idx]) {
+                double dxx = (DV(si, row, col + 1) - 2.0 * cv) + DV(si, row, col - 1);
+                double dyy = (DV(si, row + 1, col) - 2.0 * cv) + DV(si, row - 1, col);
+                double dxy = (((DV(si, row + 1, col + 1) - DV(si, row + 1, col - 1)) - DV(si, row - 1, col + 1)) +
+                              DV(si, row - 1, col - 1)) * 0.25;
+                double tr = dxx + dyy;
+                double det2 = dxx * dyy - dxy * dxy;
+                if (!(det2 <= 0.0 || (tr * tr) / det2 >= (double)edge32)) {
+                    Kp k;
+                    k.ri = (float)((double)row + oy);
+                    k.ci = (float)((double)col + ox);
+                    k.resp = (float)resp;
+                    k.o_si = (o << 8) | si;
+                    kslot[t] = k;
+                    ok = 1;
+                }
+            }
+        }
+#undef DV
+    }
+    valid[t] = ok;
+}
+
+// compaction of refine slots (slot order = octave, raster)
+__global__ __launch_bounds__(256) void k_flag_count(const uint8_t* __restrict__ flags, int n,
+                                                     int* __restrict__ counts, const int* __restrict__ gate,
+                                                     int gate_below) {
+    if (gate && *gate >= gate_below) return;
+    __shared__ int wc[4];
+    int t = blockIdx.x * 256 + threadIdx.x;
+    bool f = t < n && flags[t];
+    unsigned long long m = __ballot(f);
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) wc[wave] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+}
+
+__global__ __launch_bounds__(256) void k_flag_write(const uint8_t* __restrict__ flags, const Kp* __restrict__ src,
+                                                     const int* __restrict__ offs, int n, Kp* __restrict__ dst,
+                                                     const int* __restrict__ gate, int gate_below) {
+    if (gate && *gate >= gate_below) return;
+    __shared__ int wc[4];
+    int t = blockIdx.x * 256 + threadIdx.x;
+    bool f = t < n && flags[t];
+    unsigned long long m = __ballot(f);
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) wc[wave] = __popcll(m);
+    __syncthreads();
+    if (!f) return;
+    int pre = 0;
+    for (int w = 0; w < wave; ++w) pre += wc[w];
+    dst[offs[blockIdx.x] + pre + __popcll(m & lanemask_lt())] = src[t];
+}
+
+// ======================================================================
+// A7: keep the top max_keypoints by response (stable, descending;
+// core.py:2078-2080).  Rank by counting over LDS tiles.
+// ======================================================================
+__global__ __launch_bounds__(256) void k_select(const Kp* __restrict__ k1, const int* __restrict__ kcount,
+                                                 int maxkp, Kp* __restrict__ k2, int* __restrict__ n2) {
+    __shared__ float sr[256];
+    int n = *kcount;
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i == 0) *n2 = n < maxkp ? n : maxkp;
+    if (blockIdx.x * 256 >= n) return;
+    if (n <= maxkp) {
+        if (i < n) k2[i] = k1[i];
+        return;
+    }
+    float ri = i < n ? k1[i].resp : 0.0f;
+    int rank = 0;
+    for (int t0 = 0; t0 < n; t0 += 256) {
+        __syncthreads();
+        if (t0 + threadIdx.x < n) sr[threadIdx.x] = k1[t0 + threadIdx.x].resp;
+        __syncthreads();
+        int lim = min(256, n - t0);
+        for (int j = 0; j < lim; ++j) {
+            float rj = sr[j];
+            rank += (rj > ri) || (rj == ri && t0 + j < i);
+        }
+    }
+    if (i < n && rank < maxkp) k2[rank] = k1[i];
+}
+
+// ======================================================================
+// A8: orientation histogram (core.py:228-333).  One wave per keypoint;
+// pixel p of the window goes to lane p % 64's private 36-bin column in LDS.
+// ======================================================================
+__global__ __launch_bounds__(256) void k_orient(OctInfo oi, const Kp* __restrict__ k2,
+                                                 const int* __restrict__ n2p, KpTables tab,
+                                                 float4* __restrict__ ang, int* __restrict__ nang) {
+    __shared__ float part[4][36 * 64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + wave;
+    const int n2 = *n2p;
+    if (blockIdx.x * 4 >= n2) return;
+    float* P = part[wave];
+#pragma unroll
+    for (int b = 0; b < 36; ++b) P[b * 64 + lane] = 0.0f;
+    const bool active = i < n2;
+    Kp k;
+    if (active) {
+        k = k2[i];
+        int o = k.o_si >> 8, si = k.o_si & 255;
+        int H = oi.H[o], W = oi.W[o];
+        const float* G = oi.G[o] + (size_t)si * H * W;
+        float sigma = tab.sig_ori[si];
+        float ri = k.ri, ci = k.ci;
+        int radius = (int)(4.5f * sigma + 0.5f);
+        if (radius < 1) radius = 1;
+        float sigma_w = 1.5f * sigma;
+        float inv_2sw2 = 1.0f / (2.0f * sigma_w * sigma_w + 1e-12f);
+        int r_min = (int)ri - radius; if (r_min < 1) r_min = 1;
+        int r_max = (int)ri + radius; if (r_max > H - 2) r_max = H - 2;
+        int c_min = (int)ci - radius; if (c_min < 1) c_min = 1;
+        int c_max = (int)ci + radius; if (c_max > W - 2) c_max = W - 2;
+        int n_cols = c_max - c_min + 1; if (n_cols < 1) n_cols = 1;
+        int n_rows = r_max - r_min + 1; if (n_rows < 0) n_rows = 0;
+        int total = n_rows * n_cols;
+        for (int p = lane; p < total; p += 64) {
+            int rq = p / n_cols;
+            int r = r_min + rq;
+            int c = c_min + (p - rq * n_cols);
+            const float* g = G + (size_t)r * W + c;
+            float dx = g[1] - g[-1];
+            float dy = g[W] - g[-W];
+            float mag = sqrtf(dx * dx + dy * dy);
+            float ori = fmod360(atan2_f(dy, dx) * 57.29577951f + 360.0f);
+            float dr = (float)r - ri, dc = (float)c - ci;
+            float wt = exp_f(-(dr * dr + dc * dc) * inv_2sw2);
+            float bin_f = ori * 0.1f;
+            int bin_i = ((int)bin_f) % 36;
+            int bin_r = (bin_i + 1) % 36;
+            float frac = bin_f - floorf(bin_f);
+            P[bin_i * 64 + lane] += mag * wt * (1.0f - frac);
+            P[bin_r * 64 + lane] += mag * wt * frac;
+        }
+    }
+    float hist[36];
+#pragma unroll
+    for (int b = 0; b < 36; ++b) hist[b] = wave_tree_f(P[b * 64 + lane]);
+    if (!active || lane != 0) return;
+    float smooth[36];
+    for (int it = 0; it < 2; ++it) {
+#pragma unroll
+        for (int b = 0; b < 36; ++b)
+            smooth[b] = (0.25f * hist[(b + 35) % 36] + 0.5f * hist[b]) + 0.25f * hist[(b + 1) % 36];
+#pragma unroll
+        for (int b = 0; b < 36; ++b) hist[b] = smooth[b];
+    }
+    float peak = 0.0f;
+#pragma unroll
+    for (int b = 0; b < 36; ++b) if (hist[b] > peak) peak = hist[b];
+    float thresh = 0.8f * peak;
+    float out[4] = { -1.0f, -1.0f, -1.0f, -1.0f };
+    int n = 0;
+#pragma unroll
+    for (int b = 0; b < 36; ++b) {
+        if (hist[b] >= thresh) {
+            float l = hist[(b + 35) % 36], rv = hist[(b + 1) % 36];
+            if (hist[b] >= l && hist[b] >= rv) {
+                float denom = (l - 2.0f * hist[b]) + rv;
+                float fine = (float)b + (fabsf(denom) > 1e-12f ? (0.5f * (l - rv)) / denom : 0.0f);
+                float angle = fmod360((fine / 36.0f) * 360.0f + 360.0f);
+                if (n < 4) out[n++] = angle;
+            }
+        }
+    }
+    ang[i] = make_float4(out[0], out[1], out[2], out[3]);
+    nang[i] = n;
+}
+
+struct Ent { int kp; float angle, x, y; };
+
+// one entry per (keypoint, angle) in keypoint-major order (core.py:1594-1598)
+__global__ __launch_bounds__(256) void k_expand(const Kp* __restrict__ k2, const int* __restrict__ n2p,
+                                                 const float4* __restrict__ ang, const int* __restrict__ nang,
+                                                 const int* __restrict__ aoff, double cs, Ent* __restrict__ E) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= *n2p) return;
+    Kp k = k2[i];
+    int o = k.o_si >> 8;
+    double sf = (double)(1 << o);
+    float x = (float)(((double)k.ci * sf) * cs);
+    float y = (float)(((double)k.ri * sf) * cs);
+    float4 a = ang[i];
+    float av[4] = { a.x, a.y, a.z, a.w };
+    int base = aoff[i];
+    for (int j = 0; j < nang[i]; ++j) {
+        Ent e;
+        e.kp = i; e.angle = av[j]; e.x = x; e.y = y;
+        E[base + j] = e;
+    }
+}
+
+// canonical order: np.lexsort((ys, xs)) (core.py:2130), stable
+__global__ __launch_bounds__(256) void k_rank_xy(const Ent* __restrict__ E, const int* __restrict__ mp,
+                                                  int* __restrict__ rank) {
+    __shared__ float sx[256], sy[256];
+    int m = *mp;
+    if (blockIdx.x * 256 >= m) return;
+    int i = blockIdx.x * 256 + threadIdx.x;
+    float xi = 0.0f, yi = 0.0f;
+    if (i < m) { xi = E[i].x; yi = E[i].y; }
+    int r = 0;
+    for (int t0 = 0; t0 < m; t0 += 256) {
+        __syncthreads();
+        if (t0 + threadIdx.x < m) { sx[threadIdx.x] = E[t0 + threadIdx.x].x; sy[threadIdx.x] = E[t0 + threadIdx.x].y; }
+        __syncthreads();
+        int lim = min(256, m - t0);
+        for (int j = 0; j < lim; ++j) {
+            float xj = sx[j], yj = sy[j];
+            r += (xj < xi) || (xj == xi && (yj < yi || (yj == yi && t0 + j < i)));
+        }
+    }
+    if (i < m) rank[i] = r;
+}
+
+// ======================================================================
+// A9: 4x4x8 descriptor (core.py:693-835), L2 / clip 0.2 / L2, RootSIFT
+// (core.py:1870-1873), uint8 quantisation.  One wave per entry; pixel p of
+// the patch goes to lane p % 64's private 128-bin column in LDS.
+// ======================================================================
+__global__ __launch_bounds__(64) void k_descriptor(OctInfo oi, const Kp* __restrict__ k2,
+                                                    const Ent* __restrict__ E, const int* __restrict__ mp,
+                                                    const int* __restrict__ rank, int cap, KpTables tab,
+                                                    double cs, int rootsift, float* __restrict__ out_kp,
+                                                    float* __restrict__ out_f, uint8_t* __restrict__ out_u8) {
+    __shared__ float P[128 * 64];
+    const int e = blockIdx.x;
+    if (e >= *mp) return;
+    const int rk = rank[e];
+    if (rk >= cap) return;
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int b = 0; b < 128; ++b) P[b * 64 + lane] = 0.0f;
+    Ent en = E[e];
+    Kp k = k2[en.kp];
+    const int o = k.o_si >> 8, si = k.o_si & 255;
+    const int H = oi.H[o], W = oi.W[o];
+    const float* G = oi.G[o] + (size_t)si * H * W;
+    const float ang_rad = en.angle * 0.0174532924f;
+    const int ph = tab.ph[si];
+    double sd, cd;
+    sincos_d((double)ang_rad, &sd, &cd);
+    const float cos_a = (float)cd, sin_a = (float)sd;
+    const float scale_sp = 4.0f / (2.0f * (float)ph);
+    const float sigma_d = (float)ph * 0.5657f;
+    const float inv_2s2 = 1.0f / (2.0f * sigma_d * sigma_d + 1e-12f);
+    const int side = 2 * ph + 1;
+    const int total_px = side * side;
+    const int iri = (int)k.ri, ici = (int)k.ci;
+    for (int flat = lane; flat < total_px; flat += 64) {
+        int rq = flat / side;
+        int r_off = rq - ph;
+        int c_off = (flat - rq * side) - ph;
+        int rr = iri + r_off, cc = ici + c_off;
+        if (rr < 1 || rr >= H - 1 || cc < 1 || cc >= W - 1) continue;
+        float xr = cos_a * (float)c_off - sin_a * (float)r_off;
+        float yr = sin_a * (float)c_off + cos_a * (float)r_off;
+        float xb = xr * scale_sp + 1.5f;
+        float yb = yr * scale_sp + 1.5f;
+        if (xb < -0.5f || xb >= 3.5f || yb < -0.5f || yb >= 3.5f) continue;
+        const float* g = G + (size_t)rr * W + cc;
+        float dx = g[1] - g[-1];
+        float dy = g[W] - g[-W];
+        float mag = sqrtf(dx * dx + dy * dy);
+        float ori_deg = fmod360((atan2_f(dy, dx) - ang_rad) * 57.29577951f + 360.0f);
+        float gw = exp_f(-(xr * xr + yr * yr) * inv_2s2);
+        float wmag = mag * gw;
+        float ob = ori_deg * 0.02222222f;
+        int oi0 = (int)floorf(ob);
+        float df = ob - (float)oi0;
+        int xi0 = (int)floorf(xb), yi0 = (int)floorf(yb);
+        float dxb = xb - (float)xi0, dyb = yb - (float)yi0;
+        int xi0c = xi0 < 0 ? 0 : (xi0 > 3 ? 3 : xi0);
+        int yi0c = yi0 < 0 ? 0 : (yi0 > 3 ? 3 : yi0);
+        int xi1c = xi0 + 1 < 0 ? 0 : (xi0 + 1 > 3 ? 3 : xi0 + 1);
+        int yi1c = yi0 + 1 < 0 ? 0 : (yi0 + 1 > 3 ? 3 : yi0 + 1);
+        int oi0w = ((oi0 % 8) + 8) % 8;
+        int oi1w = ((oi0 + 1) % 8 + 8) % 8;
+        float wy0 = 1.0f - dyb, wy1 = dyb, wx0 = 1.0f - dxb, wx1 = dxb, wo0 = 1.0f - df, wo1 = df;
+        float* h = P + lane;
+        h[(yi0c * 32 + xi0c * 8 + oi0w) * 64] += wy0 * wx0 * wo0 * wmag;
+        h[(yi0c * 32 + xi0c * 8 + oi1w) * 64] += wy0 * wx0 * wo1 * wmag;
+        h[(yi0c * 32 + xi1c * 8 + oi0w) * 64] += wy0 * wx1 * wo0 * wmag;
+        h[(yi0c * 32 + xi1c * 8 + oi1w) * 64] += wy0 * wx1 * wo1 * wmag;
+        h[(yi1c * 32 + xi0c * 8 + oi0w) * 64] += wy1 * wx0 * wo0 * wmag;
+        h[(yi1c * 32 + xi0c * 8 + oi1w) * 64] += wy1 * wx0 * wo1 * wmag;
+        h[(yi1c * 32 + xi1c * 8 + oi0w) * 64] += wy1 * wx1 * wo0 * wmag;
+        h[(yi1c * 32 + xi1c * 8 + oi1w) * 64] += wy1 * wx1 * wo1 * wmag;
+    }
+    float v0 = 0.0f, v1 = 0.0f;
+#pragma unroll 8
+    for (int b = 0; b < 128; ++b) {
+        float t = wave_tree_f(P[b * 64 + lane]);
+        if ((b >> 1) == lane) { if (b & 1) v1 = t; else v0 = t; }
+    }
+    float inv = 1.0f / sqrtf(wave_tree_f(v0 * v0 + v1 * v1) + 1e-14f);
+    float t0 = v0 * inv, t1 = v1 * inv;
+    v0 = t0 > 0.2f ? 0.2f : t0;
+    v1 = t1 > 0.2f ? 0.2f : t1;
+    inv = 1.0f / sqrtf(wave_tree_f(v0 * v0 + v1 * v1) + 1e-14f);
+    v0 = v0 * inv;
+    v1 = v1 * inv;
+    if (rootsift) {
+        float l1 = wave_tree_f(fabsf(v0) + fabsf(v1)) + 1e-7f;
+        float q0 = v0 / l1, q1 = v1 / l1;
+        v0 = sqrtf(q0 > 0.0f ? q0 : 0.0f);
+        v1 = sqrtf(q1 > 0.0f ? q1 : 0.0f);
+    }
+    if (out_f) {
+        out_f[(size_t)rk * 128 + 2 * lane] = v0;
+        out_f[(size_t)rk * 128 + 2 * lane + 1] = v1;
+    }
+    if (out_u8) {
+        int a0 = (int)(v0 * 512.0f + 0.5f), a1 = (int)(v1 * 512.0f + 0.5f);
+        a0 = a0 > 255 ? 255 : (a0 < 0 ? 0 : a0);
+        a1 = a1 > 255 ? 255 : (a1 < 0 ? 0 : a1);
+        out_u8[(size_t)rk * 128 + 2 * lane] = (uint8_t)a0;
+        out_u8[(size_t)rk * 128 + 2 * lane + 1] = (uint8_t)a1;
+    }
+    if (lane == 0 && out_kp) {
+        double sf = (double)(1 << o);
+        float* ko = out_kp + (size_t)rk * 6;
+        ko[0] = en.x;
+        ko[1] = en.y;
+        ko[2] = (float)(((tab.base_size[si] * sf) * cs) * 2.0);
+        ko[3] = en.angle;
+        ko[4] = k.resp;
+        ko[5] = (float)o;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_zero_tail(int* __restrict__ p, const int* __restrict__ n2p, int n) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n && i >= *n2p) p[i] = 0;
+}
+
+__global__ void k_final_count(const int* __restrict__ mp, int* __restrict__ count) {
+    if (threadIdx.x == 0) *count = *mp;
+}
+
+// ======================================================================
+// host side: workspace plan and launch sequence
+// ======================================================================
+struct Carve {
+    size_t off = 0;
+    size_t take(size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    }
+};
+
+struct Plan {
+    int H, W, doubled, n_oct, S, capc, maxkp, cap_e, n_slots;
+    int Ho[kMaxOct], Wo[kMaxOct];
+    size_t oG[kMaxOct], oD[kMaxOct];
+    size_t o_unit, o_pf0, o_pf1, o_hist, o_thr, o_ints, o_masks, o_ecnt, o_eoff, o_cands, o_ccount,
+        o_valid, o_kslot, o_ccnt, o_coff, o_k1, o_k2, o_ang, o_nang, o_aoff, o_E, o_rank;
+    int n_ext_blk, n_cmp_blk;
+    int blk0[kMaxOct + 1], nseg[kMaxOct];
+    size_t total;
+};
+
+static void make_plan(const msift_handle* h, int H, int W, Plan* P) {
+    const msift_params& p = h->p;
+    memset(P, 0, sizeof(*P));
+    P->H = H; P->W = W; P->S = h->S;
+    P->doubled = p.double_image && ((long long)H * W < (long long)p.double_px_cap);
+    P->n_oct = p.n_octaves;
+    P->capc = p.max_cands_per_octave;
+    P->maxkp = p.max_keypoints;
+    P->cap_e = 4 * p.max_keypoints;
+    int bh = P->doubled ? 2 * H : H, bw = P->doubled ? 2 * W : W;
+    Carve c;
+    for (int o = 0; o < P->n_oct; ++o) {
+        P->Ho[o] = bh; P->Wo[o] = bw;
+        size_t n = (size_t)bh * bw;
+        P->oG[o] = c.take(n * sizeof(float) * (P->S + 3));
+        P->oD[o] = c.take(n * sizeof(float) * (P->S + 2));
+        bh = (bh + 1) / 2; bw = (bw + 1) / 2;
+    }
+    // scratch for the base image: aliases octave 0's DoG stack, which is
+    // written only after the base level exists
+    size_t n_in = (size_t)H * W;
+    size_t scratch = P->doubled ? n_in * (sizeof(double) * 2 + sizeof(float) * 4) : n_in * sizeof(float);
+    size_t d0 = (size_t)P->Ho[0] * P->Wo[0] * sizeof(float) * (P->S + 2);
+    size_t sbase;
+    if (scratch <= d0) sbase = P->oD[0];
+    else sbase = c.take(scratch);
+    if (P->doubled) {
+        P->o_pf0 = sbase;
+        P->o_pf1 = sbase + n_in * sizeof(double);
+        P->o_unit = sbase + 2 * n_in * sizeof(double);   // upsampled float image (4 n_in)
+    } else {
+        P->o_unit = sbase;
+        P->o_pf0 = P->o_pf1 = 0;
+    }
+    P->o_hist = c.take(256 * sizeof(int));
+    P->o_thr = c.take(4 * sizeof(float));
+    P->o_ints = c.take(16 * sizeof(int));
+    // extrema blocks
+    int nb = 0;
+    for (int o = 0; o < P->n_oct; ++o) {
+        P->blk0[o] = nb;
+        int nrows = P->Ho[o] - 2 * p.border, ncols = P->Wo[o] - 2 * p.border;
+        if (P->Ho[o] < 3 || P->Wo[o] < 3 || nrows <= 0 || ncols <= 0) { P->nseg[o] = 0; continue; }
+        P->nseg[o] = (ncols + 255) / 256;
+        nb += P->S * nrows * P->nseg[o];
+    }
+    P->blk0[P->n_oct] = nb;
+    P->n_ext_blk = nb;
+    P->o_masks = c.take((size_t)(nb > 0 ? nb : 1) * 4 * sizeof(unsigned long long));
+    P->o_ecnt = c.take((size_t)(nb + 1) * sizeof(int));
+    P->o_eoff = c.take((size_t)(nb + 1) * sizeof(int));
+    P->n_slots = P->n_oct * P->capc;
+    P->o_cands = c.take((size_t)P->n_slots * sizeof(int4));
+    P->o_ccount = c.take(kMaxOct * sizeof(int));
+    P->o_valid = c.take((size_t)P->n_slots);
+    P->o_kslot = c.take((size_t)P->n_slots * sizeof(Kp));
+    P->n_cmp_blk = (P->n_slots + 255) / 256;
+    P->o_ccnt = c.take((size_t)(P->n_cmp_blk + 1) * sizeof(int));
+    P->o_coff = c.take((size_t)(P->n_cmp_blk + 1) * sizeof(int));
+    P->o_k1 = c.take((size_t)P->n_slots * sizeof(Kp));
+    P->o_k2 = c.take((size_t)P->maxkp * sizeof(Kp));
+    P->o_ang = c.take((size_t)P->maxkp * sizeof(float4));
+    P->o_nang = c.take((size_t)(P->maxkp + 1) * sizeof(int));
+    P->o_aoff = c.take((size_t)(P->maxkp + 1) * sizeof(int));
+    P->o_E = c.take((size_t)P->cap_e * sizeof(Ent));
+    P->o_rank = c.take((size_t)P->cap_e * sizeof(int));
+    P->total = c.off;
+}
+
+static OctInfo oct_info(const Plan& P, char* ws, int border) {
+    OctInfo oi;
+    memset(&oi, 0, sizeof(oi));
+    oi.n_oct = P.n_oct; oi.S = P.S; oi.border = border;
+    for (int o = 0; o < P.n_oct; ++o) {
+        oi.H[o] = P.Ho[o]; oi.W[o] = P.Wo[o];
+        oi.G[o] = (float*)(ws + P.oG[o]);
+        oi.D[o] = (float*)(ws + P.oD[o]);
+        oi.blk0[o] = P.blk0[o];
+        oi.nseg[o] = P.nseg[o];
+    }
+    oi.blk0[P.n_oct] = P.blk0[P.n_oct];
+    return oi;
+}
+
+int detect_workspace(msift_handle* h, int H, int W, size_t* bytes) {
+    Plan P;
+    make_plan(h, H, W, &P);
+    *bytes = P.total;
+    return MSIFT_OK;
+}
+
+int pyramid_view(msift_handle* h, int H, int W, void* ws, int o, int* Ho, int* Wo, float** g, float** d) {
+    Plan P;
+    make_plan(h, H, W, &P);
+    if (o < 0 || o >= P.n_oct) { set_error("octave out of range"); return MSIFT_EINVAL; }
+    *Ho = P.Ho[o]; *Wo = P.Wo[o];
+    *g = (float*)((char*)ws + P.oG[o]);
+    *d = (float*)((char*)ws + P.oD[o]);
+    return MSIFT_OK;
+}
+
+// extrema -> refine -> compaction, once (gate NULL) or as the ct/2 retry
+static int keypoint_pass(const Plan& P, char* ws, const OctInfo& oi, msift_handle* h, int thr_idx,
+                         const int* gate, int* total_a, int* total_b, hipStream_t s) {
+    const int gb = h->p.retry_below;
+    float* thr = (float*)(ws + P.o_thr);
+    if (P.n_ext_blk > 0) {
+        hipLaunchKernelGGL(k_ext_count, dim3(P.n_ext_blk), dim3(256), 0, s, oi, thr, thr_idx, gate, gb,
+                           (unsigned long long*)(ws + P.o_masks), (int*)(ws + P.o_ecnt));
+    }
+    launch_scan((int*)(ws + P.o_ecnt), (int*)(ws + P.o_eoff), P.n_ext_blk, nullptr, nullptr, gate, gb, s);
+    if (P.n_ext_blk > 0) {
+        hipLaunchKernelGGL(k_ext_write, dim3(P.n_ext_blk), dim3(256), 0, s, oi,
+                           (const unsigned long long*)(ws + P.o_masks), (const int*)(ws + P.o_eoff), P.capc,
+                           (int4*)(ws + P.o_cands), gate, gb);
+    }
+    hipLaunchKernelGGL(k_oct_counts, dim3(1), dim3(64), 0, s, oi, (const int*)(ws + P.o_eoff), P.capc,
+                       (int*)(ws + P.o_ccount), gate, gb);
+    hipLaunchKernelGGL(k_refine, dim3((P.n_slots + 255) / 256), dim3(256), 0, s, oi,
+                       (const int4*)(ws + P.o_cands), (const int*)(ws + P.o_ccount), P.capc, thr, thr_idx,
+                       h->edge32, (uint8_t*)(ws + P.o_valid), (Kp*)(ws + P.o_kslot), gate, gb);
+    hipLaunchKernelGGL(k_flag_count, dim3(P.n_cmp_blk), dim3(256), 0, s, (const uint8_t*)(ws + P.o_valid),
+                       P.n_slots, (int*)(ws + P.o_ccnt), gate, gb);
+    launch_scan((int*)(ws + P.o_ccnt), (int*)(ws + P.o_coff), P.n_cmp_blk, total_a, total_b, gate, gb, s);
+    hipLaunchKernelGGL(k_flag_write, dim3(P.n_cmp_blk), dim3(256), 0, s, (const uint8_t*)(ws + P.o_valid),
+                       (const Kp*)(ws + P.o_kslot), (const int*)(ws + P.o_coff), P.n_slots, (Kp*)(ws + P.o_k1),
+                       gate, gb);
+    MSIFT_LAUNCH_CHECK();
+    return MSIFT_OK;
+}
+
+int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch, void* ws_, size_t ws_bytes,
+                    int cap, float* kp, float* df, uint8_t* du, int* count, float* phase_ms, hipStream_t s) {
+    if (H < 1 || W < 1 || pitch < W || !img || !ws_ || !count || cap < 0) {
+        set_error("detect_describe: bad arguments");
+        return MSIFT_EINVAL;
+    }
+    Plan P;
+    make_plan(h, H, W, &P);
+    if (ws_bytes < P.total) {
+        set_error("detect_describe: workspace too small");
+        return MSIFT_EWORKSPACE;
+    }
+    char* ws = (char*)ws_;
+    const msift_params& p = h->p;
+    const int S = P.S;
+    OctInfo oi = oct_info(P, ws, p.border);
+    int* ints = (int*)(ws + P.o_ints);
+    float* thr = (float*)(ws + P.o_thr);
+    bool prof = phase_ms != nullptr;
+    if (prof && !h->ev_init) {
+        for (int i = 0; i < 10; ++i) MSIFT_HIP(hipEventCreate(&h->ev[i]));
+        h->ev_init = true;
+    }
+    if (prof) MSIFT_HIP(hipEventRecord(h->ev[0], s));
+
+    // A1: adaptive threshold
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, s, (int*)(ws + P.o_hist), 256);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, ints, 16);
+    {
+        dim3 g((W + 255) / 256 < 4 ? (W + 255) / 256 : 4, H);
+        hipLaunchKernelGGL(k_hist256, g, dim3(256), 0, s, img, H, W, pitch, (int*)(ws + P.o_hist));
+        hipLaunchKernelGGL(k_thresh, dim3(1), dim3(64), 0, s, (const int*)(ws + P.o_hist), (double)H * (double)W,
+                           p.contrast_thresh, S, p.adaptive_thresh, thr);
+    }
+    // A3: base image
+    float* G0 = oi.G[0];
+    int rc;
+    if (P.doubled) {
+        double* pf0 = (double*)(ws + P.o_pf0);
+        double* pf1 = (double*)(ws + P.o_pf1);
+        float* up = (float*)(ws + P.o_unit);
+        hipLaunchKernelGGL(k_ups_pf_cols, dim3((W + 255) / 256, H), dim3(256), 0, s, img, pitch, H, W, h->ups, pf0);
+        hipLaunchKernelGGL(k_ups_pf_rows, dim3((W + 255) / 256, H), dim3(256), 0, s, pf0, H, W, h->ups, pf1);
+        double zy = (2 * H > 1) ? (double)(H - 1) / (double)(2 * H - 1) : 0.0;
+        double zx = (2 * W > 1) ? (double)(W - 1) / (double)(2 * W - 1) : 0.0;
+        hipLaunchKernelGGL(k_ups_interp, dim3((2 * W + 255) / 256, 2 * H), dim3(256), 0, s, pf1, H, W, zy, zx, up);
+        if ((rc = blur(up, G0, nullptr, nullptr, 2 * H, 2 * W, h->sd0[1], h->taps_base[1], s))) return rc;
+    } else {
+        float* unit = (float*)(ws + P.o_unit);
+        hipLaunchKernelGGL(k_u8_unit, dim3((W + 255) / 256, H), dim3(256), 0, s, img, pitch, H, W, unit);
+        if ((rc = blur(unit, G0, nullptr, nullptr, H, W, h->sd0[0], h->taps_base[0], s))) return rc;
+    }
+    MSIFT_LAUNCH_CHECK();
+    // A3/A4: octaves
+    for (int o = 0; o < P.n_oct; ++o) {
+        int Ho = P.Ho[o], Wo = P.Wo[o];
+        size_t n = (size_t)Ho * Wo;
+        for (int lv = 0; lv < S + 2; ++lv) {
+            float* dec = (lv + 1 == S && o + 1 < P.n_oct) ? oi.G[o + 1] : nullptr;
+            if ((rc = blur(oi.G[o] + n * lv, oi.G[o] + n * (lv + 1), oi.D[o] + n * lv, dec, Ho, Wo, h->inc[lv],
+                           h->taps_inc[lv], s)))
+                return rc;
+        }
+    }
+    MSIFT_LAUNCH_CHECK();
+    if (prof) MSIFT_HIP(hipEventRecord(h->ev[1], s));
+    if (prof) MSIFT_HIP(hipEventRecord(h->ev[2], s));
+    // A5/A6 (+ retry at ct/2 when fewer than retry_below keypoints, core.py:2064-2071)
+    if ((rc = keypoint_pass(P, ws, oi, h, 0, nullptr, &ints[0], &ints[1], s))) return rc;
+    if ((rc = keypoint_pass(P, ws, oi, h, 1, &ints[0], &ints[1], nullptr, s))) return rc;
+    if (prof) MSIFT_HIP(hipEventRecord(h->ev[3], s));
+    // A7: cap
+    hipLaunchKernelGGL(k_select, dim3((P.n_slots + 255) / 256), dim3(256), 0, s, (const Kp*)(ws + P.o_k1),
+                       (const int*)&ints[1], P.maxkp, (Kp*)(ws + P.o_k2), &ints[2]);
+    if (prof) MSIFT_HIP(hipEventRecord(h->ev[4], s));
+    // A8: orientation
+    hipLaunchKernelGGL(k_orient, dim3((P.maxkp + 3) / 4), dim3(256), 0, s, oi, (const Kp*)(ws + P.o_k2),
+                       (const int*)&ints[2], h->tab, (float4*)(ws + P.o_ang), (int*)(ws + P.o_nang));
+    MSIFT_LAUNCH_CHECK();
+    if (prof) MSIFT_HIP(hipEventRecord(h->ev[5], s));
+    // expansion + canonical order
+    {
+        // nang beyond n2 must read as zero for the scan
+        hipLaunchKernelGGL(k_zero_tail, dim3((P.maxkp + 255) / 256), dim3(256), 0, s, (int*)(ws + P.o_nang),
+                           (const int*)&ints[2], P.maxkp);
+        launch_scan((int*)(ws + P.o_nang), (int*)(ws + P.o_aoff), P.maxkp, &ints[3], nullptr, nullptr, 0, s);
+        double cs = P.doubled ? 0.5 : 1.0;
+        hipLaunchKernelGGL(k_expand, dim3((P.maxkp + 255) / 256), dim3(256), 0, s, (const Kp*)(ws + P.o_k2),
+                           (const int*)&ints[2], (const float4*)(ws + P.o_ang), (const int*)(ws + P.o_nang),
+                           (const int*)(ws + P.o_aoff), cs, (Ent*)(ws + P.o_E));
+        hipLaunchKernelGGL(k_rank_xy, dim3((P.cap_e + 255) / 256), dim3(256), 0, s, (const Ent*)(ws + P.o_E),
+                           (const int*)&ints[3], (int*)(ws + P.o_rank));
+        // A9: descriptors written at their sorted position
+        hipLaunchKernelGGL(k_descriptor, dim3(P.cap_e), dim3(64), 0, s, oi, (const Kp*)(ws + P.o_k2),
+                           (const Ent*)(ws + P.o_E), (const int*)&ints[3], (const int*)(ws + P.o_rank), cap, h->tab,
+                           cs, p.rootsift, kp, df, du);
+        hipLaunchKernelGGL(k_final_count, dim3(1), dim3(64), 0, s, (const int*)&ints[3], count);
+    }
+    MSIFT_LAUNCH_CHECK();
+    if (prof) {
+        MSIFT_HIP(hipEventRecord(h->ev[6], s));
+        MSIFT_HIP(hipEventSynchronize(h->ev[6]));
+        float t[7];
+        for (int i = 0; i < 6; ++i) MSIFT_HIP(hipEventElapsedTime(&t[i], h->ev[i], h->ev[i + 1]));
+        MSIFT_HIP(hipEventElapsedTime(&t[6], h->ev[0], h->ev[6]));
+        // gauss_pyr (incl. DoG), dog (fused: 0), extrema (+refine), stack/cap,
+        // orient, descriptor (+expand/sort), cv_kpt (0 on device), total
+        phase_ms[0] = t[0]; phase_ms[1] = t[1]; phase_ms[2] = t[2]; phase_ms[3] = t[3];
+        phase_ms[4] = t[4]; phase_ms[5] = t[5]; phase_ms[6] = 0.0f; phase_ms[7] = t[6];
+    }
+    return MSIFT_OK;
+}
+
+}  // namespace msift

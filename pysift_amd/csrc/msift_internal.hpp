@@ -1,0 +1,84 @@
+// Internal host-side definitions shared by the library's translation units.
+#pragma once
+#include "msift_common.hpp"
+#include "pysift_amd.h"
+#include <string>
+
+struct KpTables {
+    float sig_ori[32];     // orientation sigma per scale index (float32)
+    int ph[32];            // descriptor patch half-size per scale index
+    double base_size[32];  // sigma0 * k^si (double)
+};
+
+struct msift_handle {
+    msift_params p;
+    int S;
+    double sd0[2];                   // base blur: [0] single-size input, [1] doubled
+    double inc[64];                  // incremental sigmas, levels 1..S+2
+    msift::Taps taps_base[2];
+    msift::Taps taps_inc[64];
+    msift::UpsTaps ups;
+    KpTables tab;
+    float edge32;
+    hipEvent_t ev[10];
+    bool ev_init;
+};
+
+namespace msift {
+
+void set_error(const std::string& msg);
+
+#define MSIFT_HIP(call)                                                        \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) {                                                \
+            msift::set_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+            return MSIFT_EHIP;                                                 \
+        }                                                                      \
+    } while (0)
+
+#define MSIFT_LAUNCH_CHECK()                                                   \
+    do {                                                                       \
+        hipError_t e_ = hipGetLastError();                                     \
+        if (e_ != hipSuccess) {                                                \
+            msift::set_error(std::string("kernel launch: ") + hipGetErrorString(e_)); \
+            return MSIFT_EHIP;                                                 \
+        }                                                                      \
+    } while (0)
+
+void make_taps(double sigma, Taps* t);
+
+// detect.hip
+int detect_workspace(msift_handle* h, int H, int W, size_t* bytes);
+int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch, void* ws,
+                    size_t ws_bytes, int cap, float* kp, float* df, uint8_t* du, int* count,
+                    float* phase_ms, hipStream_t s);
+int pyramid_view(msift_handle* h, int H, int W, void* ws, int o, int* Ho, int* Wo, float** g,
+                 float** d);
+
+// match.hip
+size_t match_workspace(int N, int M);
+int match_u8(const uint8_t* a, int N, const uint8_t* b, int M, double ratio, int method, void* ws,
+             size_t ws_bytes, int32_t* fwd, int32_t* bwd, int32_t* pairs, int* npairs,
+             hipStream_t s);
+int match_f16(const float* a, int N, const float* b, int M, double ratio, void* ws,
+              size_t ws_bytes, int32_t* fwd, int32_t* bwd, int32_t* pairs, int* npairs,
+              hipStream_t s);
+
+// ransac.hip
+size_t ransac_workspace(int N, int B);
+int ransac(const double* p1, const double* p2, int N, uint64_t seed, int B, double thr,
+           double sigma_max, int method, int lo, int min_inl, void* ws, size_t ws_bytes,
+           double* H, uint8_t* mask, int* ninl, int* best, hipStream_t s);
+
+// warp.hip
+int warp_feather(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h2, int w2,
+                 const double* H, int cw, int ch, double ox, double oy, uint8_t* pano,
+                 uint8_t* valid, hipStream_t s);
+int bgr_to_gray(const uint8_t* bgr, int H, int W, uint8_t* gray, hipStream_t s);
+
+// scan.hip helpers (device-wide exclusive scan of int32, single workgroup)
+void launch_scan(const int* in, int* out, int n, int* total_a, int* total_b, const int* gate,
+                 int gate_below, hipStream_t s);
+
+}  // namespace msift

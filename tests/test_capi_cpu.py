@@ -1,0 +1,54 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every symbol the
+public header declares, and validates arguments without touching a GPU."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "pysift_amd.h")).read()
+    return sorted(set(re.findall(r"^int\s+(msift_\w+)\s*\(", src, re.M)))
+
+
+def test_header_declares_expected_entry_points():
+    names = _declared()
+    for n in ["msift_create", "msift_detect_describe", "msift_match_u8", "msift_ransac_homography",
+              "msift_warp_feather"]:
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    from pysift_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    for n in _declared():
+        assert hasattr(L, n), n
+    assert set(_declared()) == set(_lib.EXPORTS)
+
+
+def test_create_validates_parameters():
+    from pysift_amd import _lib
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    good = _lib.MsiftParams(5, 3, 1.6, 0.04, 10.0, 1, 1, 16_000_000, 25000, 8000, 5, 100, 1)
+    assert L.msift_create(ctypes.byref(good), ctypes.byref(h)) == 0
+    n = ctypes.c_size_t()
+    assert L.msift_workspace_bytes(h, 480, 640, ctypes.byref(n)) == 0
+    assert n.value > 480 * 640 * 4 * 44
+    assert L.msift_destroy(h) == 0
+    bad = _lib.MsiftParams(0, 3, 1.6, 0.04, 10.0, 1, 1, 16_000_000, 25000, 8000, 5, 100, 1)
+    assert L.msift_create(ctypes.byref(bad), ctypes.byref(h)) == _lib.MSIFT_EINVAL
+    assert "n_octaves" in _lib.last_error()
+    huge = _lib.MsiftParams(5, 3, 9.0, 0.04, 10.0, 1, 1, 16_000_000, 25000, 8000, 5, 100, 1)
+    assert L.msift_create(ctypes.byref(huge), ctypes.byref(h)) == _lib.MSIFT_EINVAL
+
+
+def test_pysift_constructor_errors():
+    from pysift_amd import PySIFT
+    with pytest.raises(ValueError):
+        PySIFT(dsp_n_scales=7)

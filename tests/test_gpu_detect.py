@@ -1,0 +1,109 @@
+"""GPU parity of detectAndCompute against the CPU oracle (bit-exact)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pysift_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+
+def _sift(**kw):
+    from pysift_amd import PySIFT
+    return PySIFT(**kw)
+
+
+def _oracle_params(s):
+    return O.params(n_octaves=s.n_octaves, n_scales=s.n_scales, sigma0=s.sigma0,
+                    contrast_thresh=s.contrast_thresh, edge_thresh=s.edge_thresh,
+                    double_image=s.double_image, rootsift=s.rootsift, double_px_cap=s.double_px_cap,
+                    max_cands_per_octave=s.max_cands_per_octave, max_keypoints=s.max_keypoints)
+
+
+def _pyramid_from_ws(s, H, W, o):
+    import torch
+    from pysift_amd import _lib
+    Ho = ctypes.c_int(); Wo = ctypes.c_int(); g = ctypes.c_void_p(); d = ctypes.c_void_p()
+    _lib.check(_lib.lib().msift_pyramid_view(s._h, H, W, _lib.ptr(s._ws), o, ctypes.byref(Ho), ctypes.byref(Wo),
+                                             ctypes.byref(g), ctypes.byref(d)))
+    base = s._ws.data_ptr()
+    n = Ho.value * Wo.value
+    S = s.n_scales
+    gs = s._ws[g.value - base: g.value - base + 4 * n * (S + 3)].view(torch.float32).reshape(S + 3, Ho.value, Wo.value)
+    ds = s._ws[d.value - base: d.value - base + 4 * n * (S + 2)].view(torch.float32).reshape(S + 2, Ho.value, Wo.value)
+    return gs.cpu().numpy(), ds.cpu().numpy()
+
+
+@pytest.mark.parametrize("H,W,dbl", [(97, 131, True), (240, 320, True), (61, 77, False)])
+def test_pyramid_bitwise(H, W, dbl):
+    print(f"[case] pyramid {H}x{W} doubled={dbl}", flush=True)
+    img = syn.synth_gray(H, W, 11)
+    s = _sift(double_image=dbl, n_octaves=4)
+    s.detect_raw(img)
+    p = _oracle_params(s)
+    for o in range(4):
+        g_ref, d_ref = O.pyramid_octave(img, p, o)
+        g, d = _pyramid_from_ws(s, H, W, o)
+        assert g.shape == g_ref.shape
+        np.testing.assert_array_equal(g, g_ref, err_msg=f"gauss octave {o}")
+        np.testing.assert_array_equal(d, d_ref, err_msg=f"dog octave {o}")
+
+
+def _compare(img, s, tag):
+    kp, df, du = s.detect_raw(img)
+    kp = kp.cpu().numpy(); df = df.cpu().numpy(); du = du.cpu().numpy()
+    rk, rd, ru = O.detect_describe(img, _oracle_params(s))
+    print(f"[case] {tag}: gpu {len(kp)} kp, oracle {len(rk)} kp", flush=True)
+    assert kp.shape == rk.shape, (kp.shape, rk.shape)
+    np.testing.assert_array_equal(kp, rk)
+    np.testing.assert_array_equal(df, rd)
+    np.testing.assert_array_equal(du, ru)
+    return len(kp)
+
+
+def test_detect_bitwise_640x480():
+    img = syn.synth_gray(480, 640, 0)
+    n = _compare(img, _sift(), "config0 640x480")
+    assert n > 500
+
+
+@pytest.mark.parametrize("H,W,kw", [
+    (257, 333, {}),
+    (300, 200, {"n_octaves": 4}),
+    (150, 151, {"double_image": False}),
+    (480, 640, {"max_keypoints": 300}),
+    (1100, 1300, {"double_px_cap": 1_000_000, "n_octaves": 4}),
+])
+def test_detect_bitwise_shapes(H, W, kw):
+    img = syn.synth_gray(H, W, H * 7 + W)
+    _compare(img, _sift(**kw), f"{H}x{W} {kw}")
+
+
+def test_detect_flat_image_is_empty():
+    img = np.full((200, 300), 77, np.uint8)
+    n = _compare(img, _sift(), "flat")
+    assert n == 0
+
+
+def test_detect_repeatable_bits():
+    img = syn.synth_gray(480, 640, 5)
+    s = _sift()
+    a = [t.cpu().numpy().copy() for t in s.detect_raw(img)]
+    b = [t.cpu().numpy().copy() for t in s.detect_raw(img)]
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_detect_and_compute_api():
+    img = syn.synth_gray(240, 320, 9)
+    s = _sift()
+    kps, desc = s.detectAndCompute(img)
+    assert desc.dtype == np.float32 and desc.shape == (len(kps), 128)
+    assert all(hasattr(k, "pt") and hasattr(k, "octave") for k in kps)
+    xs = np.array([k.pt[0] for k in kps])
+    assert np.all(np.diff(xs) >= 0)
+    k2, d2 = s.detectAndCompute(img, gpu_output=True, profile=True)
+    assert tuple(k2.shape) == (len(kps), 4) and k2.is_cuda
+    assert set(s._phase_timings) >= {"gauss_pyr_ms", "total_ms"}
