@@ -361,12 +361,19 @@ double or_adaptive_thresh(const uint8_t* img, int H, int W, int pitch, double ba
     for (int y = 0; y < H; ++y)
         for (int x = 0; x < W; ++x) hist[img[(size_t)y * pitch + x]]++;
     double total = (double)H * (double)W;
-    double ent = 0.0;
-    for (int b = 0; b < 256; ++b) {
-        if (hist[b] == 0) continue;
-        double p = (double)hist[b] / total;
-        ent = ent - p * or_log2(p);
+    /* lane l sums bins l, l+64, l+128, l+192; lanes combine by the xor tree */
+    double part[LANES];
+    for (int l = 0; l < LANES; ++l) {
+        double acc = 0.0;
+        for (int j = 0; j < 4; ++j) {
+            int b = l + 64 * j;
+            if (hist[b] == 0) continue;
+            double p = (double)hist[b] / total;
+            acc = acc - p * or_log2(p);
+        }
+        part[l] = acc;
     }
+    double ent = wave_tree_d(part);
     double scale = 0.8 + ent / 40.0;
     if (scale > 1.0) scale = 1.0;
     return base * scale;
@@ -1220,7 +1227,8 @@ int or_ransac(const double* p1, const double* p2, int N, uint64_t seed, int B,
  * A17 + feather blend.  Canvas (core.py:3171-3187).  Warp: inverse map with
  * H^-1 rounded to float32 (core.py:3225), bilinear with zero padding
  * (core.py:3233), valid where the source point lies in [0,w-1]x[0,h-1].
- * img2 is pasted at (round(ox), round(oy)) (core.py:3794-3800).  Feather
+ * img2 is pasted at (round(ox), round(oy)) (core.py:3794-3800, Python round =
+ * half to even).  Feather
  * weight of a valid pixel = min(sx+1, w-sx, sy+1, h-sy) in its own source
  * coordinates; output = trunc(clip(sum w_i I_i / sum w_i)).
  * ======================================================================== */
@@ -1259,7 +1267,7 @@ void or_warp_feather(const uint8_t* img1, int h1, int w1, const uint8_t* img2, i
     float hf[9];
     for (int i = 0; i < 9; ++i) hf[i] = (float)Hi[i];
     float oxf = (float)ox, oyf = (float)oy;
-    int iox = (int)lround(ox), ioy = (int)lround(oy);
+    int iox = (int)rint(ox), ioy = (int)rint(oy);   /* Python round(): half to even */
     #pragma omp parallel for schedule(static)
     for (int v = 0; v < ch; ++v)
         for (int u = 0; u < cw; ++u) {

@@ -19,38 +19,48 @@ __global__ void k_zero_i32(int* p, int n) {
     if (i < n) p[i] = 0;
 }
 
-// 256-bin histogram of the u8 image (integer atomics: order-free, exact)
+// 256-bin histogram of the u8 image (integer atomics: order-free, exact);
+// per-wave LDS sub-histograms, rows strided over the grid
 __global__ __launch_bounds__(256) void k_hist256(const uint8_t* __restrict__ img, int H, int W,
                                                   int pitch, int* __restrict__ hist) {
-    __shared__ int sh[256];
-    sh[threadIdx.x] = 0;
+    __shared__ int sh[4][256];
+    const int wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) sh[j][threadIdx.x] = 0;
     __syncthreads();
-    int y = blockIdx.y;
-    for (int x = blockIdx.x * 256 + threadIdx.x; x < W; x += gridDim.x * 256)
-        atomicAdd(&sh[img[(size_t)y * pitch + x]], 1);
+    for (int y = blockIdx.x; y < H; y += gridDim.x) {
+        const uint8_t* row = img + (size_t)y * pitch;
+        for (int x = threadIdx.x; x < W; x += 256) atomicAdd(&sh[wv][row[x]], 1);
+    }
     __syncthreads();
-    if (sh[threadIdx.x]) atomicAdd(&hist[threadIdx.x], sh[threadIdx.x]);
+    int t = (sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
+    if (t) atomicAdd(&hist[threadIdx.x], t);
 }
 
-// _adaptive_contrast_thresh (core.py:1129-1148): entropy in FP64, bins in
-// index order; writes thr[0] = f32(ct'/S), thr[1] = f32(ct'*0.5/S)
-__global__ void k_thresh(const int* __restrict__ hist, double total, double base, int S,
-                         int adaptive, float* __restrict__ thr) {
-    if (threadIdx.x != 0) return;
+// _adaptive_contrast_thresh (core.py:1129-1148): entropy in FP64; lane l
+// sums bins l, l+64, l+128, l+192, lanes combine by the xor tree; writes
+// thr[0] = f32(ct'/S), thr[1] = f32(ct'*0.5/S)
+__global__ __launch_bounds__(64) void k_thresh(const int* __restrict__ hist, double total, double base, int S,
+                                                int adaptive, float* __restrict__ thr) {
+    const int l = threadIdx.x;
     double ct = base;
     if (adaptive) {
-        double ent = 0.0;
-        for (int b = 0; b < 256; ++b) {
-            if (hist[b] == 0) continue;
-            double p = (double)hist[b] / total;
-            ent = ent - p * log2_d(p);
+        double acc = 0.0;
+        for (int j = 0; j < 4; ++j) {
+            int c = hist[l + 64 * j];
+            if (c == 0) continue;
+            double p = (double)c / total;
+            acc = acc - p * log2_d(p);
         }
+        double ent = wave_tree_d(acc);
         double scale = 0.8 + ent / 40.0;
         if (scale > 1.0) scale = 1.0;
         ct = base * scale;
     }
-    thr[0] = (float)(ct / (double)S);
-    thr[1] = (float)((ct * 0.5) / (double)S);
+    if (l == 0) {
+        thr[0] = (float)(ct / (double)S);
+        thr[1] = (float)((ct * 0.5) / (double)S);
+    }
 }
 
 // ======================================================================
@@ -64,10 +74,18 @@ __global__ __launch_bounds__(256) void k_ups_pf_cols(const uint8_t* __restrict__
     int y = blockIdx.y;
     if (x >= W) return;
     double acc = 0.0;
-#pragma unroll 4
-    for (int k = -kUpsK; k <= kUpsK; ++k) {
-        float u = (float)img[(size_t)mirror_idx(y + k, H) * pitch + x] / 255.0f;
-        acc = fma(h.h[k < 0 ? -k : k], (double)u, acc);
+    if (y >= kUpsK && y + kUpsK < H) {
+        const uint8_t* col = img + (size_t)(y - kUpsK) * pitch + x;
+#pragma unroll
+        for (int k = -kUpsK; k <= kUpsK; ++k) {
+            float u = (float)col[(size_t)(k + kUpsK) * pitch] / 255.0f;
+            acc = fma(h.h[k < 0 ? -k : k], (double)u, acc);
+        }
+    } else {
+        for (int k = -kUpsK; k <= kUpsK; ++k) {
+            float u = (float)img[(size_t)mirror_idx(y + k, H) * pitch + x] / 255.0f;
+            acc = fma(h.h[k < 0 ? -k : k], (double)u, acc);
+        }
     }
     out[(size_t)y * W + x] = acc;
 }
@@ -79,8 +97,12 @@ __global__ __launch_bounds__(256) void k_ups_pf_rows(const double* __restrict__ 
     if (x >= W) return;
     const double* row = in + (size_t)y * W;
     double acc = 0.0;
-#pragma unroll 4
-    for (int k = -kUpsK; k <= kUpsK; ++k) acc = fma(h.h[k < 0 ? -k : k], row[mirror_idx(x + k, W)], acc);
+    if (x >= kUpsK && x + kUpsK < W) {
+#pragma unroll
+        for (int k = -kUpsK; k <= kUpsK; ++k) acc = fma(h.h[k < 0 ? -k : k], row[x + k], acc);
+    } else {
+        for (int k = -kUpsK; k <= kUpsK; ++k) acc = fma(h.h[k < 0 ? -k : k], row[mirror_idx(x + k, W)], acc);
+    }
     out[(size_t)y * W + x] = acc;
 }
 
@@ -131,10 +153,18 @@ __global__ __launch_bounds__(256) void k_u8_unit(const uint8_t* __restrict__ img
     if (x < W) out[(size_t)y * W + x] = (float)img[(size_t)y * pitch + x] / 255.0f;
 }
 
-__global__ __launch_bounds__(256) void k_copy_f32(const float* __restrict__ in, float* __restrict__ out,
-                                                   size_t n) {
+// a level whose sigma is below 0.01 equals its input (core.py:559-560)
+__global__ __launch_bounds__(256) void k_copy_level(const float* __restrict__ in, float* __restrict__ gout,
+                                                     float* __restrict__ dout, float* __restrict__ dec, int H, int W,
+                                                     int decW) {
+    size_t n = (size_t)H * W;
     size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) out[i] = in[i];
+    if (i >= n) return;
+    float v = in[i];
+    if (gout) gout[i] = v;
+    if (dout) dout[i] = v - v;
+    int y = (int)(i / W), x = (int)(i - (size_t)y * W);
+    if (dec && !(y & 1) && !(x & 1)) dec[(size_t)(y >> 1) * decW + (x >> 1)] = v;
 }
 
 // ======================================================================
@@ -158,10 +188,36 @@ __global__ __launch_bounds__(256) void k_blur(const float* __restrict__ in, floa
     __shared__ __attribute__((aligned(16))) float sH[IH * TW];
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
-    for (int i = tid; i < IH * IW; i += 256) {
-        int ly = i / IW, lx = i - ly * IW;
-        int gy = reflect_idx(y0 - R + ly, H), gx = reflect_idx(x0 - R + lx, W);
-        sIn[ly * IWP + lx] = in[(size_t)gy * W + gx];
+    {
+        // all loads issued before the LDS stores; one reflection suffices
+        // unless the image is narrower than the halo
+        constexpr int NL = (IH * IW + 255) / 256;
+        const bool small = (H <= R) || (W <= R);
+        float tv[NL];
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+            int i = tid + 256 * j;
+            if (i < IH * IW) {
+                int ly = i / IW, lx = i - ly * IW;
+                int gy = y0 - R + ly, gx = x0 - R + lx;
+                if (small) {
+                    gy = reflect_idx(gy, H);
+                    gx = reflect_idx(gx, W);
+                } else {
+                    gy = gy < 0 ? -1 - gy : (gy >= H ? 2 * H - 1 - gy : gy);
+                    gx = gx < 0 ? -1 - gx : (gx >= W ? 2 * W - 1 - gx : gx);
+                }
+                tv[j] = in[(size_t)gy * W + gx];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+            int i = tid + 256 * j;
+            if (i < IH * IW) {
+                int ly = i / IW, lx = i - ly * IW;
+                sIn[ly * IWP + lx] = tv[j];
+            }
+        }
     }
     __syncthreads();
     // horizontal pass: 4 consecutive outputs per thread
@@ -201,7 +257,7 @@ __global__ __launch_bounds__(256) void k_blur(const float* __restrict__ in, floa
         int gy = y0 + g * RG + j;
         if (gy < H && gx < W) {
             size_t o = (size_t)gy * W + gx;
-            gout[o] = s;
+            if (gout) gout[o] = s;
             if (dout) dout[o] = s - sIn[(g * RG + j + R) * IWP + lx + R];
             if (dec && !(gy & 1) && !(gx & 1)) dec[(size_t)(gy >> 1) * decW + (gx >> 1)] = s;
         }
@@ -233,7 +289,8 @@ static int blur(const float* in, float* gout, float* dout, float* dec, int H, in
     if (sigma < 0.01) {
         // _fast_gaussian returns its input unchanged (core.py:559-560)
         size_t n = (size_t)H * W;
-        hipLaunchKernelGGL(k_copy_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, gout, n);
+        hipLaunchKernelGGL(k_copy_level, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, gout, dout, dec, H,
+                           W, decW);
         return MSIFT_OK;
     }
     switch (t.r) {
@@ -254,12 +311,16 @@ static int blur(const float* in, float* gout, float* dout, float* dec, int H, in
 }
 
 // ======================================================================
-// A5: 3x3x3 extrema (core.py:176-217).  Blocks walk (octave, s, row,
-// 256-column segment) in raster order; pass 1 stores each wave's ballot and
-// the block count, a scan turns counts into offsets, pass 2 writes the
-// candidates at their raster-order position (first `capc` per octave kept).
+// A5: 3x3x3 extrema (core.py:176-217).  Logical blocks walk (octave, s, row,
+// 1024-column segment) in raster order, 4 columns per thread.  Pass 1 stores
+// each thread's 4-bit predicate mask and the block count; a scan turns counts
+// into offsets; pass 2 writes candidates at their raster-order position
+// (first `capc` per octave kept).  Grid-stride loops keep a gated (skipped)
+// launch cheap.
 // ======================================================================
-struct ExtPos { int o, s, r, c; };
+constexpr int kSeg = 1024;
+
+struct ExtPos { int o, s, r, c0; };
 
 __device__ __forceinline__ ExtPos ext_decode(const OctInfo& oi, int b) {
     int o = 0;
@@ -273,7 +334,7 @@ __device__ __forceinline__ ExtPos ext_decode(const OctInfo& oi, int b) {
     int r = bd + rem / nseg;
     int seg = rem - (r - bd) * nseg;
     ExtPos p;
-    p.o = o; p.s = s; p.r = r; p.c = bd + seg * 256 + (int)threadIdx.x;
+    p.o = o; p.s = s; p.r = r; p.c0 = bd + seg * kSeg + 4 * (int)threadIdx.x;
     return p;
 }
 
@@ -297,50 +358,67 @@ __device__ __forceinline__ bool is_extremum(const float* __restrict__ D, int H, 
     return mx || mn;
 }
 
-__global__ __launch_bounds__(256) void k_ext_count(OctInfo oi, const float* __restrict__ thr, int thr_idx,
-                                                    const int* __restrict__ gate, int gate_below,
-                                                    unsigned long long* __restrict__ masks,
-                                                    int* __restrict__ counts) {
-    if (gate && *gate >= gate_below) return;
-    __shared__ int wc[4];
-    ExtPos p = ext_decode(oi, blockIdx.x);
-    bool pred = false;
-    if (p.c < oi.W[p.o] - oi.border)
-        pred = is_extremum(oi.D[p.o], oi.H[p.o], oi.W[p.o], p.s, p.r, p.c, thr[thr_idx]);
-    unsigned long long m = __ballot(pred);
-    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) {
-        masks[(size_t)blockIdx.x * 4 + wave] = m;
-        wc[wave] = __popcll(m);
+// block-wide exclusive scan of one int per thread (256 threads); returns the
+// exclusive prefix and stores the block total in *tot
+__device__ __forceinline__ int block_excl_scan256(int c, int* wsum, int* tot) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int x = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int y = __shfl_up(x, off);
+        if (lane >= off) x += y;
     }
+    if (lane == 63) wsum[wv] = x;
     __syncthreads();
-    if (threadIdx.x == 0) counts[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+    int pre = 0;
+    for (int w = 0; w < wv; ++w) pre += wsum[w];
+    *tot = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+    __syncthreads();
+    return pre + x - c;
 }
 
-__global__ __launch_bounds__(256) void k_ext_write(OctInfo oi, const unsigned long long* __restrict__ masks,
+__global__ __launch_bounds__(256) void k_ext_count(OctInfo oi, int nblk, const float* __restrict__ thr, int thr_idx,
+                                                    const int* __restrict__ gate, int gate_below,
+                                                    uint8_t* __restrict__ masks, int* __restrict__ counts) {
+    if (gate && *gate >= gate_below) return;
+    __shared__ int wsum[4];
+    const float t = thr[thr_idx];
+    for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
+        ExtPos p = ext_decode(oi, b);
+        const int H = oi.H[p.o], W = oi.W[p.o];
+        const float* D = oi.D[p.o];
+        int bits = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (p.c0 + j < W - oi.border && is_extremum(D, H, W, p.s, p.r, p.c0 + j, t)) bits |= 1 << j;
+        masks[(size_t)b * 256 + threadIdx.x] = (uint8_t)bits;
+        int tot;
+        block_excl_scan256(__popc(bits), wsum, &tot);
+        if (threadIdx.x == 0) counts[b] = tot;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ext_write(OctInfo oi, int nblk, const uint8_t* __restrict__ masks,
                                                     const int* __restrict__ offs, int capc,
                                                     int4* __restrict__ cands, const int* __restrict__ gate,
                                                     int gate_below) {
     if (gate && *gate >= gate_below) return;
-    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const unsigned long long* mb = masks + (size_t)blockIdx.x * 4;
-    unsigned long long m = mb[wave];
-    if (!((m >> lane) & 1ull)) return;
-    ExtPos p = ext_decode(oi, blockIdx.x);
-    int pre = 0;
-    for (int w = 0; w < wave; ++w) pre += __popcll(mb[w]);
-    int pos = offs[blockIdx.x] + pre + __popcll(m & lanemask_lt());
-    int q = pos - offs[oi.blk0[p.o]];
-    if (q < capc) cands[(size_t)p.o * capc + q] = make_int4(p.s, p.r, p.c, 0);
-}
-
-__global__ void k_oct_counts(OctInfo oi, const int* __restrict__ offs, int capc, int* __restrict__ ccount,
-                             const int* __restrict__ gate, int gate_below) {
-    if (gate && *gate >= gate_below) return;
-    int o = threadIdx.x;
-    if (o < oi.n_oct) {
-        int n = offs[oi.blk0[o + 1]] - offs[oi.blk0[o]];
-        ccount[o] = n < capc ? n : capc;
+    __shared__ int wsum[4];
+    for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
+        int bits = masks[(size_t)b * 256 + threadIdx.x];
+        int tot;
+        int pos = offs[b] + block_excl_scan256(__popc(bits), wsum, &tot);
+        if (!bits) continue;
+        ExtPos p = ext_decode(oi, b);
+        int base = offs[oi.blk0[p.o]];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if ((bits >> j) & 1) {
+                int q = pos - base;
+                if (q < capc) cands[(size_t)p.o * capc + q] = make_int4(p.s, p.r, p.c0 + j, 0);
+                pos++;
+            }
+        }
     }
 }
 
@@ -348,16 +426,17 @@ __global__ void k_oct_counts(OctInfo oi, const int* __restrict__ offs, int capc,
 // A6: sub-pixel refinement (core.py:343-463) in FP64
 // ======================================================================
 __global__ __launch_bounds__(256) void k_refine(OctInfo oi, const int4* __restrict__ cands,
-                                                 const int* __restrict__ ccount, int capc,
+                                                 const int* __restrict__ offs, int capc,
                                                  const float* __restrict__ thr, int thr_idx, float edge32,
                                                  uint8_t* __restrict__ valid, Kp* __restrict__ kslot,
                                                  const int* __restrict__ gate, int gate_below) {
     if (gate && *gate >= gate_below) return;
-    int t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= oi.n_oct * capc) return;
+    const int n_slots = oi.n_oct * capc;
+    for (int t = blockIdx.x * 256 + threadIdx.x; t < n_slots; t += gridDim.x * 256) {
     int o = t / capc, i = t - o * capc;
+    int cnt = offs[oi.blk0[o + 1]] - offs[oi.blk0[o]];
     uint8_t ok = 0;
-    if (i < ccount[o]) {
+    if (i < cnt && i < capc) {
         int4 cd = cands[t];
         int si = cd.x, row = cd.y, col = cd.z;
         const int H = oi.H[o], W = oi.W[o], S = oi.S, bd = oi.border;
@@ -419,38 +498,47 @@ __global__ __launch_bounds__(256) void k_refine(OctInfo oi, const int4* __restri
 #undef DV
     }
     valid[t] = ok;
+    }
 }
 
-// compaction of refine slots (slot order = octave, raster)
-__global__ __launch_bounds__(256) void k_flag_count(const uint8_t* __restrict__ flags, int n,
+// compaction of refine slots (slot order = octave, raster), logical blocks of 256
+__global__ __launch_bounds__(256) void k_flag_count(const uint8_t* __restrict__ flags, int n, int nblk,
                                                      int* __restrict__ counts, const int* __restrict__ gate,
                                                      int gate_below) {
     if (gate && *gate >= gate_below) return;
     __shared__ int wc[4];
-    int t = blockIdx.x * 256 + threadIdx.x;
-    bool f = t < n && flags[t];
-    unsigned long long m = __ballot(f);
-    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) wc[wave] = __popcll(m);
-    __syncthreads();
-    if (threadIdx.x == 0) counts[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
+        int t = b * 256 + threadIdx.x;
+        bool f = t < n && flags[t];
+        unsigned long long m = __ballot(f);
+        if (lane == 0) wc[wave] = __popcll(m);
+        __syncthreads();
+        if (threadIdx.x == 0) counts[b] = (wc[0] + wc[1]) + (wc[2] + wc[3]);
+        __syncthreads();
+    }
 }
 
 __global__ __launch_bounds__(256) void k_flag_write(const uint8_t* __restrict__ flags, const Kp* __restrict__ src,
-                                                     const int* __restrict__ offs, int n, Kp* __restrict__ dst,
-                                                     const int* __restrict__ gate, int gate_below) {
+                                                     const int* __restrict__ offs, int n, int nblk,
+                                                     Kp* __restrict__ dst, const int* __restrict__ gate,
+                                                     int gate_below) {
     if (gate && *gate >= gate_below) return;
     __shared__ int wc[4];
-    int t = blockIdx.x * 256 + threadIdx.x;
-    bool f = t < n && flags[t];
-    unsigned long long m = __ballot(f);
-    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) wc[wave] = __popcll(m);
-    __syncthreads();
-    if (!f) return;
-    int pre = 0;
-    for (int w = 0; w < wave; ++w) pre += wc[w];
-    dst[offs[blockIdx.x] + pre + __popcll(m & lanemask_lt())] = src[t];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
+        int t = b * 256 + threadIdx.x;
+        bool f = t < n && flags[t];
+        unsigned long long m = __ballot(f);
+        if (lane == 0) wc[wave] = __popcll(m);
+        __syncthreads();
+        if (f) {
+            int pre = 0;
+            for (int w = 0; w < wave; ++w) pre += wc[w];
+            dst[offs[b] + pre + __popcll(m & lanemask_lt())] = src[t];
+        }
+        __syncthreads();
+    }
 }
 
 // ======================================================================
@@ -460,27 +548,29 @@ __global__ __launch_bounds__(256) void k_flag_write(const uint8_t* __restrict__ 
 __global__ __launch_bounds__(256) void k_select(const Kp* __restrict__ k1, const int* __restrict__ kcount,
                                                  int maxkp, Kp* __restrict__ k2, int* __restrict__ n2) {
     __shared__ float sr[256];
-    int n = *kcount;
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i == 0) *n2 = n < maxkp ? n : maxkp;
-    if (blockIdx.x * 256 >= n) return;
-    if (n <= maxkp) {
-        if (i < n) k2[i] = k1[i];
-        return;
-    }
-    float ri = i < n ? k1[i].resp : 0.0f;
-    int rank = 0;
-    for (int t0 = 0; t0 < n; t0 += 256) {
-        __syncthreads();
-        if (t0 + threadIdx.x < n) sr[threadIdx.x] = k1[t0 + threadIdx.x].resp;
-        __syncthreads();
-        int lim = min(256, n - t0);
-        for (int j = 0; j < lim; ++j) {
-            float rj = sr[j];
-            rank += (rj > ri) || (rj == ri && t0 + j < i);
+    const int n = *kcount;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *n2 = n < maxkp ? n : maxkp;
+    const int nblk = (n + 255) / 256;
+    for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
+        const int i = b * 256 + threadIdx.x;
+        if (n <= maxkp) {
+            if (i < n) k2[i] = k1[i];
+            continue;
         }
+        float ri = i < n ? k1[i].resp : 0.0f;
+        int rank = 0;
+        for (int t0 = 0; t0 < n; t0 += 256) {
+            __syncthreads();
+            if (t0 + threadIdx.x < n) sr[threadIdx.x] = k1[t0 + threadIdx.x].resp;
+            __syncthreads();
+            int lim = min(256, n - t0);
+            for (int j = 0; j < lim; ++j) {
+                float rj = sr[j];
+                rank += (rj > ri) || (rj == ri && t0 + j < i);
+            }
+        }
+        if (i < n && rank < maxkp) k2[rank] = k1[i];
     }
-    if (i < n && rank < maxkp) k2[rank] = k1[i];
 }
 
 // ======================================================================
@@ -489,19 +579,18 @@ __global__ __launch_bounds__(256) void k_select(const Kp* __restrict__ k1, const
 // ======================================================================
 __global__ __launch_bounds__(256) void k_orient(OctInfo oi, const Kp* __restrict__ k2,
                                                  const int* __restrict__ n2p, KpTables tab,
-                                                 float4* __restrict__ ang, int* __restrict__ nang) {
+                                                 float4* __restrict__ ang, int* __restrict__ nang, int maxkp) {
     __shared__ float part[4][36 * 64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + wave;
     const int n2 = *n2p;
-    if (blockIdx.x * 4 >= n2) return;
+    // angle counts beyond n2 read as zero for the scan
+    for (int j = n2 + blockIdx.x * 256 + threadIdx.x; j < maxkp; j += gridDim.x * 256) nang[j] = 0;
     float* P = part[wave];
+    for (int i = blockIdx.x * 4 + wave; i < n2; i += gridDim.x * 4) {
 #pragma unroll
     for (int b = 0; b < 36; ++b) P[b * 64 + lane] = 0.0f;
-    const bool active = i < n2;
-    Kp k;
-    if (active) {
-        k = k2[i];
+    {
+        Kp k = k2[i];
         int o = k.o_si >> 8, si = k.o_si & 255;
         int H = oi.H[o], W = oi.W[o];
         const float* G = oi.G[o] + (size_t)si * H * W;
@@ -540,7 +629,7 @@ __global__ __launch_bounds__(256) void k_orient(OctInfo oi, const Kp* __restrict
     float hist[36];
 #pragma unroll
     for (int b = 0; b < 36; ++b) hist[b] = wave_tree_f(P[b * 64 + lane]);
-    if (!active || lane != 0) return;
+    if (lane != 0) continue;
     float smooth[36];
     for (int it = 0; it < 2; ++it) {
 #pragma unroll
@@ -569,6 +658,7 @@ __global__ __launch_bounds__(256) void k_orient(OctInfo oi, const Kp* __restrict
     }
     ang[i] = make_float4(out[0], out[1], out[2], out[3]);
     nang[i] = n;
+    }
 }
 
 struct Ent { int kp; float angle, x, y; };
@@ -598,23 +688,25 @@ __global__ __launch_bounds__(256) void k_expand(const Kp* __restrict__ k2, const
 __global__ __launch_bounds__(256) void k_rank_xy(const Ent* __restrict__ E, const int* __restrict__ mp,
                                                   int* __restrict__ rank) {
     __shared__ float sx[256], sy[256];
-    int m = *mp;
-    if (blockIdx.x * 256 >= m) return;
-    int i = blockIdx.x * 256 + threadIdx.x;
-    float xi = 0.0f, yi = 0.0f;
-    if (i < m) { xi = E[i].x; yi = E[i].y; }
-    int r = 0;
-    for (int t0 = 0; t0 < m; t0 += 256) {
-        __syncthreads();
-        if (t0 + threadIdx.x < m) { sx[threadIdx.x] = E[t0 + threadIdx.x].x; sy[threadIdx.x] = E[t0 + threadIdx.x].y; }
-        __syncthreads();
-        int lim = min(256, m - t0);
-        for (int j = 0; j < lim; ++j) {
-            float xj = sx[j], yj = sy[j];
-            r += (xj < xi) || (xj == xi && (yj < yi || (yj == yi && t0 + j < i)));
+    const int m = *mp;
+    const int nblk = (m + 255) / 256;
+    for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
+        int i = b * 256 + threadIdx.x;
+        float xi = 0.0f, yi = 0.0f;
+        if (i < m) { xi = E[i].x; yi = E[i].y; }
+        int r = 0;
+        for (int t0 = 0; t0 < m; t0 += 256) {
+            __syncthreads();
+            if (t0 + threadIdx.x < m) { sx[threadIdx.x] = E[t0 + threadIdx.x].x; sy[threadIdx.x] = E[t0 + threadIdx.x].y; }
+            __syncthreads();
+            int lim = min(256, m - t0);
+            for (int j = 0; j < lim; ++j) {
+                float xj = sx[j], yj = sy[j];
+                r += (xj < xi) || (xj == xi && (yj < yi || (yj == yi && t0 + j < i)));
+            }
         }
+        if (i < m) rank[i] = r;
     }
-    if (i < m) rank[i] = r;
 }
 
 // ======================================================================
@@ -626,13 +718,15 @@ __global__ __launch_bounds__(64) void k_descriptor(OctInfo oi, const Kp* __restr
                                                     const Ent* __restrict__ E, const int* __restrict__ mp,
                                                     const int* __restrict__ rank, int cap, KpTables tab,
                                                     double cs, int rootsift, float* __restrict__ out_kp,
-                                                    float* __restrict__ out_f, uint8_t* __restrict__ out_u8) {
+                                                    float* __restrict__ out_f, uint8_t* __restrict__ out_u8,
+                                                    int* __restrict__ count) {
     __shared__ float P[128 * 64];
-    const int e = blockIdx.x;
-    if (e >= *mp) return;
-    const int rk = rank[e];
-    if (rk >= cap) return;
+    const int m = *mp;
     const int lane = threadIdx.x;
+    if (blockIdx.x == 0 && lane == 0) *count = m;
+    for (int e = blockIdx.x; e < m; e += gridDim.x) {
+    const int rk = rank[e];
+    if (rk >= cap) continue;
 #pragma unroll
     for (int b = 0; b < 128; ++b) P[b * 64 + lane] = 0.0f;
     Ent en = E[e];
@@ -731,15 +825,7 @@ __global__ __launch_bounds__(64) void k_descriptor(OctInfo oi, const Kp* __restr
         ko[4] = k.resp;
         ko[5] = (float)o;
     }
-}
-
-__global__ __launch_bounds__(256) void k_zero_tail(int* __restrict__ p, const int* __restrict__ n2p, int n) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n && i >= *n2p) p[i] = 0;
-}
-
-__global__ void k_final_count(const int* __restrict__ mp, int* __restrict__ count) {
-    if (threadIdx.x == 0) *count = *mp;
+    }
 }
 
 // ======================================================================
@@ -799,21 +885,22 @@ static void make_plan(const msift_handle* h, int H, int W, Plan* P) {
         P->o_unit = sbase;
         P->o_pf0 = P->o_pf1 = 0;
     }
-    P->o_hist = c.take(256 * sizeof(int));
+    // histogram (256) + counters (16) zeroed together by one kernel
+    P->o_hist = c.take((256 + 16) * sizeof(int));
+    P->o_ints = P->o_hist + 256 * sizeof(int);
     P->o_thr = c.take(4 * sizeof(float));
-    P->o_ints = c.take(16 * sizeof(int));
     // extrema blocks
     int nb = 0;
     for (int o = 0; o < P->n_oct; ++o) {
         P->blk0[o] = nb;
         int nrows = P->Ho[o] - 2 * p.border, ncols = P->Wo[o] - 2 * p.border;
         if (P->Ho[o] < 3 || P->Wo[o] < 3 || nrows <= 0 || ncols <= 0) { P->nseg[o] = 0; continue; }
-        P->nseg[o] = (ncols + 255) / 256;
+        P->nseg[o] = (ncols + kSeg - 1) / kSeg;
         nb += P->S * nrows * P->nseg[o];
     }
     P->blk0[P->n_oct] = nb;
     P->n_ext_blk = nb;
-    P->o_masks = c.take((size_t)(nb > 0 ? nb : 1) * 4 * sizeof(unsigned long long));
+    P->o_masks = c.take((size_t)(nb > 0 ? nb : 1) * 256);
     P->o_ecnt = c.take((size_t)(nb + 1) * sizeof(int));
     P->o_eoff = c.take((size_t)(nb + 1) * sizeof(int));
     P->n_slots = P->n_oct * P->capc;
@@ -866,32 +953,32 @@ int pyramid_view(msift_handle* h, int H, int W, void* ws, int o, int* Ho, int* W
     return MSIFT_OK;
 }
 
+static inline int grid_cap(int n, int cap) { return n < 1 ? 1 : (n < cap ? n : cap); }
+
 // extrema -> refine -> compaction, once (gate NULL) or as the ct/2 retry
 static int keypoint_pass(const Plan& P, char* ws, const OctInfo& oi, msift_handle* h, int thr_idx,
                          const int* gate, int* total_a, int* total_b, hipStream_t s) {
     const int gb = h->p.retry_below;
     float* thr = (float*)(ws + P.o_thr);
     if (P.n_ext_blk > 0) {
-        hipLaunchKernelGGL(k_ext_count, dim3(P.n_ext_blk), dim3(256), 0, s, oi, thr, thr_idx, gate, gb,
-                           (unsigned long long*)(ws + P.o_masks), (int*)(ws + P.o_ecnt));
+        hipLaunchKernelGGL(k_ext_count, dim3(grid_cap(P.n_ext_blk, 4096)), dim3(256), 0, s, oi, P.n_ext_blk, thr,
+                           thr_idx, gate, gb, (uint8_t*)(ws + P.o_masks), (int*)(ws + P.o_ecnt));
     }
     launch_scan((int*)(ws + P.o_ecnt), (int*)(ws + P.o_eoff), P.n_ext_blk, nullptr, nullptr, gate, gb, s);
     if (P.n_ext_blk > 0) {
-        hipLaunchKernelGGL(k_ext_write, dim3(P.n_ext_blk), dim3(256), 0, s, oi,
-                           (const unsigned long long*)(ws + P.o_masks), (const int*)(ws + P.o_eoff), P.capc,
+        hipLaunchKernelGGL(k_ext_write, dim3(grid_cap(P.n_ext_blk, 4096)), dim3(256), 0, s, oi, P.n_ext_blk,
+                           (const uint8_t*)(ws + P.o_masks), (const int*)(ws + P.o_eoff), P.capc,
                            (int4*)(ws + P.o_cands), gate, gb);
     }
-    hipLaunchKernelGGL(k_oct_counts, dim3(1), dim3(64), 0, s, oi, (const int*)(ws + P.o_eoff), P.capc,
-                       (int*)(ws + P.o_ccount), gate, gb);
-    hipLaunchKernelGGL(k_refine, dim3((P.n_slots + 255) / 256), dim3(256), 0, s, oi,
-                       (const int4*)(ws + P.o_cands), (const int*)(ws + P.o_ccount), P.capc, thr, thr_idx,
+    hipLaunchKernelGGL(k_refine, dim3(grid_cap((P.n_slots + 255) / 256, 1024)), dim3(256), 0, s, oi,
+                       (const int4*)(ws + P.o_cands), (const int*)(ws + P.o_eoff), P.capc, thr, thr_idx,
                        h->edge32, (uint8_t*)(ws + P.o_valid), (Kp*)(ws + P.o_kslot), gate, gb);
-    hipLaunchKernelGGL(k_flag_count, dim3(P.n_cmp_blk), dim3(256), 0, s, (const uint8_t*)(ws + P.o_valid),
-                       P.n_slots, (int*)(ws + P.o_ccnt), gate, gb);
+    hipLaunchKernelGGL(k_flag_count, dim3(grid_cap(P.n_cmp_blk, 1024)), dim3(256), 0, s,
+                       (const uint8_t*)(ws + P.o_valid), P.n_slots, P.n_cmp_blk, (int*)(ws + P.o_ccnt), gate, gb);
     launch_scan((int*)(ws + P.o_ccnt), (int*)(ws + P.o_coff), P.n_cmp_blk, total_a, total_b, gate, gb, s);
-    hipLaunchKernelGGL(k_flag_write, dim3(P.n_cmp_blk), dim3(256), 0, s, (const uint8_t*)(ws + P.o_valid),
-                       (const Kp*)(ws + P.o_kslot), (const int*)(ws + P.o_coff), P.n_slots, (Kp*)(ws + P.o_k1),
-                       gate, gb);
+    hipLaunchKernelGGL(k_flag_write, dim3(grid_cap(P.n_cmp_blk, 1024)), dim3(256), 0, s,
+                       (const uint8_t*)(ws + P.o_valid), (const Kp*)(ws + P.o_kslot), (const int*)(ws + P.o_coff),
+                       P.n_slots, P.n_cmp_blk, (Kp*)(ws + P.o_k1), gate, gb);
     MSIFT_LAUNCH_CHECK();
     return MSIFT_OK;
 }
@@ -922,11 +1009,10 @@ int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch
     if (prof) MSIFT_HIP(hipEventRecord(h->ev[0], s));
 
     // A1: adaptive threshold
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(256), 0, s, (int*)(ws + P.o_hist), 256);
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, ints, 16);
+    hipLaunchKernelGGL(k_zero_i32, dim3(2), dim3(256), 0, s, (int*)(ws + P.o_hist), 256 + 16);
     {
-        dim3 g((W + 255) / 256 < 4 ? (W + 255) / 256 : 4, H);
-        hipLaunchKernelGGL(k_hist256, g, dim3(256), 0, s, img, H, W, pitch, (int*)(ws + P.o_hist));
+        hipLaunchKernelGGL(k_hist256, dim3(grid_cap(H, 128)), dim3(256), 0, s, img, H, W, pitch,
+                           (int*)(ws + P.o_hist));
         hipLaunchKernelGGL(k_thresh, dim3(1), dim3(64), 0, s, (const int*)(ws + P.o_hist), (double)H * (double)W,
                            p.contrast_thresh, S, p.adaptive_thresh, thr);
     }
@@ -955,7 +1041,9 @@ int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch
         size_t n = (size_t)Ho * Wo;
         for (int lv = 0; lv < S + 2; ++lv) {
             float* dec = (lv + 1 == S && o + 1 < P.n_oct) ? oi.G[o + 1] : nullptr;
-            if ((rc = blur(oi.G[o] + n * lv, oi.G[o] + n * (lv + 1), oi.D[o] + n * lv, dec, Ho, Wo, h->inc[lv],
+            // the top level is only needed through its DoG
+            float* gnext = (lv + 1 == S + 2) ? nullptr : oi.G[o] + n * (lv + 1);
+            if ((rc = blur(oi.G[o] + n * lv, gnext, oi.D[o] + n * lv, dec, Ho, Wo, h->inc[lv],
                            h->taps_inc[lv], s)))
                 return rc;
         }
@@ -968,31 +1056,28 @@ int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch
     if ((rc = keypoint_pass(P, ws, oi, h, 1, &ints[0], &ints[1], nullptr, s))) return rc;
     if (prof) MSIFT_HIP(hipEventRecord(h->ev[3], s));
     // A7: cap
-    hipLaunchKernelGGL(k_select, dim3((P.n_slots + 255) / 256), dim3(256), 0, s, (const Kp*)(ws + P.o_k1),
+    hipLaunchKernelGGL(k_select, dim3(grid_cap((P.n_slots + 255) / 256, 512)), dim3(256), 0, s, (const Kp*)(ws + P.o_k1),
                        (const int*)&ints[1], P.maxkp, (Kp*)(ws + P.o_k2), &ints[2]);
     if (prof) MSIFT_HIP(hipEventRecord(h->ev[4], s));
     // A8: orientation
-    hipLaunchKernelGGL(k_orient, dim3((P.maxkp + 3) / 4), dim3(256), 0, s, oi, (const Kp*)(ws + P.o_k2),
-                       (const int*)&ints[2], h->tab, (float4*)(ws + P.o_ang), (int*)(ws + P.o_nang));
+    hipLaunchKernelGGL(k_orient, dim3(grid_cap((P.maxkp + 3) / 4, 1024)), dim3(256), 0, s, oi,
+                       (const Kp*)(ws + P.o_k2), (const int*)&ints[2], h->tab, (float4*)(ws + P.o_ang),
+                       (int*)(ws + P.o_nang), P.maxkp);
     MSIFT_LAUNCH_CHECK();
     if (prof) MSIFT_HIP(hipEventRecord(h->ev[5], s));
     // expansion + canonical order
     {
-        // nang beyond n2 must read as zero for the scan
-        hipLaunchKernelGGL(k_zero_tail, dim3((P.maxkp + 255) / 256), dim3(256), 0, s, (int*)(ws + P.o_nang),
-                           (const int*)&ints[2], P.maxkp);
         launch_scan((int*)(ws + P.o_nang), (int*)(ws + P.o_aoff), P.maxkp, &ints[3], nullptr, nullptr, 0, s);
         double cs = P.doubled ? 0.5 : 1.0;
         hipLaunchKernelGGL(k_expand, dim3((P.maxkp + 255) / 256), dim3(256), 0, s, (const Kp*)(ws + P.o_k2),
                            (const int*)&ints[2], (const float4*)(ws + P.o_ang), (const int*)(ws + P.o_nang),
                            (const int*)(ws + P.o_aoff), cs, (Ent*)(ws + P.o_E));
-        hipLaunchKernelGGL(k_rank_xy, dim3((P.cap_e + 255) / 256), dim3(256), 0, s, (const Ent*)(ws + P.o_E),
+        hipLaunchKernelGGL(k_rank_xy, dim3(grid_cap((P.cap_e + 255) / 256, 256)), dim3(256), 0, s, (const Ent*)(ws + P.o_E),
                            (const int*)&ints[3], (int*)(ws + P.o_rank));
         // A9: descriptors written at their sorted position
-        hipLaunchKernelGGL(k_descriptor, dim3(P.cap_e), dim3(64), 0, s, oi, (const Kp*)(ws + P.o_k2),
+        hipLaunchKernelGGL(k_descriptor, dim3(grid_cap(P.cap_e, 2048)), dim3(64), 0, s, oi, (const Kp*)(ws + P.o_k2),
                            (const Ent*)(ws + P.o_E), (const int*)&ints[3], (const int*)(ws + P.o_rank), cap, h->tab,
-                           cs, p.rootsift, kp, df, du);
-        hipLaunchKernelGGL(k_final_count, dim3(1), dim3(64), 0, s, (const int*)&ints[3], count);
+                           cs, p.rootsift, kp, df, du, count);
     }
     MSIFT_LAUNCH_CHECK();
     if (prof) {

@@ -1,7 +1,474 @@
-// RANSAC (placeholder).
+// GPU-batched RANSAC with LO refit (restates _ransac / _ransac_batch_eval /
+// _compute_homography / _normalize_points, pysift/core.py:3011-3097,
+// 2272-2313, 2946-2981), all in FP64.
+//
+// Hypothesis b draws Philox4x32-10(counter = (b,0,0,0), key = seed) and maps
+// each 32-bit word u to the point index (u * N) >> 32 (with replacement).
+// The minimal solve is Gaussian elimination with partial pivoting (first
+// maximum on ties) of the 8x9 DLT system with h8 = 1; the least-squares refit
+// is the smallest-eigenvalue eigenvector of A^T A by cyclic Jacobi.  Sums over
+// points use 64 lane partials (point k -> lane k % 64) and the xor tree, so
+// every result is reproduced bit for bit by the CPU oracle.
 #include "msift_internal.hpp"
+
+#pragma clang fp contract(off)
+
 namespace msift {
-size_t ransac_workspace(int N, int B) { (void)N; (void)B; return 256; }
-int ransac(const double*, const double*, int, uint64_t, int, double, double, int, int, int, void*, size_t, double*,
-           uint8_t*, int*, int*, hipStream_t) { set_error("ransac: not available"); return MSIFT_ESTATE; }
+
+constexpr int kHypPerBlock = 64;
+
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
 }
+
+__device__ __forceinline__ void dlt_rows(double x1, double y1, double x2, double y2, double* re, double* ro) {
+    re[0] = -x1; re[1] = -y1; re[2] = -1.0; re[3] = 0.0; re[4] = 0.0; re[5] = 0.0;
+    re[6] = x2 * x1; re[7] = x2 * y1; re[8] = x2;
+    ro[0] = 0.0; ro[1] = 0.0; ro[2] = 0.0; ro[3] = -x1; ro[4] = -y1; ro[5] = -1.0;
+    ro[6] = y2 * x1; ro[7] = y2 * y1; ro[8] = y2;
+}
+
+// null vector of the 8x9 system, h8 = 1; static indices only (registers)
+__device__ __forceinline__ bool solve8(double A[8][9], double* h) {
+    double amax = 0.0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int c = 0; c < 9; ++c) amax = fabs(A[r][c]) > amax ? fabs(A[r][c]) : amax;
+    const double tiny = amax * 1e-12;
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        int p = c;
+        double best = fabs(A[c][c]);
+#pragma unroll
+        for (int r = c + 1; r < 8; ++r)
+            if (fabs(A[r][c]) > best) { best = fabs(A[r][c]); p = r; }
+        if (!(best > tiny)) ok = false;
+#pragma unroll
+        for (int r = c + 1; r < 8; ++r) {
+            if (p == r) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) { double t = A[c][k]; A[c][k] = A[r][k]; A[r][k] = t; }
+            }
+        }
+#pragma unroll
+        for (int r = c + 1; r < 8; ++r) {
+            double f = A[r][c] / A[c][c];
+#pragma unroll
+            for (int k = c; k < 9; ++k) A[r][k] = A[r][k] - f * A[c][k];
+        }
+    }
+    h[8] = 1.0;
+#pragma unroll
+    for (int c = 7; c >= 0; --c) {
+        double s = A[c][8];
+#pragma unroll
+        for (int k = c + 1; k < 8; ++k) s = s + A[c][k] * h[k];
+        h[c] = -s / A[c][c];
+    }
+    return ok;
+}
+
+__device__ __forceinline__ void denorm(const double* Hn, const double* T1, const double* T2, double* H) {
+    double s2 = T2[0], inv_s2 = 1.0 / s2;
+    double cx2 = -T2[2] / s2, cy2 = -T2[5] / s2;
+    double Ti[9] = { inv_s2, 0.0, cx2, 0.0, inv_s2, cy2, 0.0, 0.0, 1.0 };
+    double A[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            A[3 * r + c] = (Ti[3 * r] * Hn[c] + Ti[3 * r + 1] * Hn[3 + c]) + Ti[3 * r + 2] * Hn[6 + c];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            H[3 * r + c] = (A[3 * r] * T1[c] + A[3 * r + 1] * T1[3 + c]) + A[3 * r + 2] * T1[6 + c];
+    double h22 = H[8];
+    if (fabs(h22) > 1e-10) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) H[i] = H[i] / h22;
+    }
+}
+
+__device__ __forceinline__ void inv3(const double* a, double* o) {
+    double det = (a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6])) + a[2] * (a[3] * a[7] - a[4] * a[6]);
+    if (!(fabs(det) > 1e-12)) det = 1.0;
+    o[0] = (a[4] * a[8] - a[5] * a[7]) / det; o[1] = (a[2] * a[7] - a[1] * a[8]) / det; o[2] = (a[1] * a[5] - a[2] * a[4]) / det;
+    o[3] = (a[5] * a[6] - a[3] * a[8]) / det; o[4] = (a[0] * a[8] - a[2] * a[6]) / det; o[5] = (a[2] * a[3] - a[0] * a[5]) / det;
+    o[6] = (a[3] * a[7] - a[4] * a[6]) / det; o[7] = (a[1] * a[6] - a[0] * a[7]) / det; o[8] = (a[0] * a[4] - a[1] * a[3]) / det;
+}
+
+__device__ __forceinline__ double sym_err(const double* H, const double* Hi, double x1, double y1, double x2, double y2,
+                                          bool clampz) {
+    double fx = (H[0] * x1 + H[1] * y1) + H[2];
+    double fy = (H[3] * x1 + H[4] * y1) + H[5];
+    double fz = (H[6] * x1 + H[7] * y1) + H[8];
+    if (clampz && fz < 1e-10) fz = 1e-10;
+    double ex = fx / fz - x2, ey = fy / fz - y2;
+    double bx = (Hi[0] * x2 + Hi[1] * y2) + Hi[2];
+    double by = (Hi[3] * x2 + Hi[4] * y2) + Hi[5];
+    double bz = (Hi[6] * x2 + Hi[7] * y2) + Hi[8];
+    if (clampz && bz < 1e-10) bz = 1e-10;
+    double gx = bx / bz - x1, gy = by / bz - y1;
+    return (ex * ex + ey * ey) + (gx * gx + gy * gy);
+}
+
+// Hartley normalisation over the points listed in idx[0..m) (core.py:2946-2957)
+__device__ void hartley_wave(const double* __restrict__ pts, const int* __restrict__ idx, int m, double* T) {
+    const int lane = threadIdx.x & 63;
+    double sx = 0.0, sy = 0.0;
+    for (int k = lane; k < m; k += 64) {
+        int i = idx ? idx[k] : k;
+        sx = sx + pts[2 * i];
+        sy = sy + pts[2 * i + 1];
+    }
+    double cx = wave_tree_d(sx) / (double)m;
+    double cy = wave_tree_d(sy) / (double)m;
+    double sd = 0.0;
+    for (int k = lane; k < m; k += 64) {
+        int i = idx ? idx[k] : k;
+        double dx = pts[2 * i] - cx, dy = pts[2 * i + 1] - cy;
+        sd = sd + sqrt(dx * dx + dy * dy);
+    }
+    double md = wave_tree_d(sd) / (double)m;
+    double s = 0x1.6a09e667f3bcdp+0 / (md + 1e-10);
+    T[0] = s; T[1] = 0.0; T[2] = -s * cx;
+    T[3] = 0.0; T[4] = s; T[5] = -s * cy;
+    T[6] = 0.0; T[7] = 0.0; T[8] = 1.0;
+}
+
+// ws layout: scores[B] | Hs[B*9] | T[18] | best(int) ninl(int) | idx[N] | mask2[N]
+struct RansacWs {
+    double* scores;
+    double* Hs;
+    double* T;
+    int* ints;
+    int* idx;
+    uint8_t* mask2;
+};
+
+__global__ __launch_bounds__(64) void k_normalize(const double* __restrict__ p1, const double* __restrict__ p2, int N,
+                                                   double* __restrict__ T) {
+    double T1[9], T2[9];
+    hartley_wave(p1, nullptr, N, T1);
+    hartley_wave(p2, nullptr, N, T2);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 9; ++k) { T[k] = T1[k]; T[9 + k] = T2[k]; }
+}
+
+__global__ __launch_bounds__(64) void k_hypotheses(const double* __restrict__ p1, const double* __restrict__ p2, int N,
+                                                    int B, uint32_t k0, uint32_t k1, const double* __restrict__ T,
+                                                    int method, double den, double tq, double* __restrict__ scores,
+                                                    double* __restrict__ Hs) {
+    __shared__ double sH[kHypPerBlock][9], sHi[kHypPerBlock][9];
+    __shared__ int sOk[kHypPerBlock];
+    const int lane = threadIdx.x;
+    const int b = blockIdx.x * kHypPerBlock + lane;
+    double T1[9], T2[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) { T1[k] = T[k]; T2[k] = T[9 + k]; }
+    {
+        // phase 1: one hypothesis per lane
+        uint32_t c[4] = { (uint32_t)b, 0u, 0u, 0u };
+        philox4x32_10(c, k0, k1);
+        double A[8][9];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            int i = (int)(((uint64_t)c[q] * (uint64_t)N) >> 32);
+            double x1 = T1[0] * p1[2 * i] + T1[2], y1 = T1[4] * p1[2 * i + 1] + T1[5];
+            double x2 = T2[0] * p2[2 * i] + T2[2], y2 = T2[4] * p2[2 * i + 1] + T2[5];
+            dlt_rows(x1, y1, x2, y2, A[2 * q], A[2 * q + 1]);
+        }
+        double h[9], H[9], Hi[9];
+        bool ok = solve8(A, h) && b < B;
+        if (ok) {
+            denorm(h, T1, T2, H);
+            inv3(H, Hi);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) { H[k] = (k % 4 == 0) ? 1.0 : 0.0; Hi[k] = H[k]; }
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) { sH[lane][k] = H[k]; sHi[lane][k] = Hi[k]; }
+        sOk[lane] = ok;
+        if (b < B) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) Hs[(size_t)b * 9 + k] = H[k];
+        }
+    }
+    __syncthreads();
+    // phase 2: score each hypothesis with lanes over points
+    const int nh = min(kHypPerBlock, B - blockIdx.x * kHypPerBlock);
+    for (int hq = 0; hq < nh; ++hq) {
+        const int bb = blockIdx.x * kHypPerBlock + hq;
+        if (!sOk[hq]) {
+            if (lane == 0) scores[bb] = -1.0;
+            continue;
+        }
+        double H[9], Hi[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) { H[k] = sH[hq][k]; Hi[k] = sHi[hq][k]; }
+        double part = 0.0;
+        for (int i = lane; i < N; i += 64) {
+            double e = sym_err(H, Hi, p1[2 * i], p1[2 * i + 1], p2[2 * i], p2[2 * i + 1], true);
+            double w = (method == 0) ? exp_d(-e / den) : (e < tq ? 1.0 : 0.0);
+            part = part + w;
+        }
+        double sc = wave_tree_d(part);
+        if (lane == 0) scores[bb] = sc;
+    }
+}
+
+// argmax with the smallest index on ties (a total order: any reduction order)
+__global__ __launch_bounds__(1024) void k_argmax(const double* __restrict__ scores, int B, int* __restrict__ best) {
+    __shared__ double ss[1024];
+    __shared__ int si[1024];
+    double bs = -__builtin_huge_val();
+    int bi = 0x7fffffff;
+    for (int b = threadIdx.x; b < B; b += 1024) {
+        double v = scores[b];
+        if (v > bs || (v == bs && b < bi)) { bs = v; bi = b; }
+    }
+    ss[threadIdx.x] = bs; si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int off = 512; off >= 1; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            double v = ss[threadIdx.x + off];
+            int i = si[threadIdx.x + off];
+            if (v > ss[threadIdx.x] || (v == ss[threadIdx.x] && i < si[threadIdx.x])) { ss[threadIdx.x] = v; si[threadIdx.x] = i; }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *best = si[0];
+}
+
+// compacts mask (in index order) into idx, returns the count (all lanes)
+__device__ int compact_wave(const uint8_t* __restrict__ mask, int N, int* __restrict__ idx) {
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    for (int i0 = 0; i0 < N; i0 += 64) {
+        int i = i0 + lane;
+        bool f = i < N && mask[i];
+        unsigned long long m = __ballot(f);
+        if (f) idx[base + __popcll(m & lanemask_lt())] = i;
+        base += __popcll(m);
+    }
+    return base;
+}
+
+__device__ void jacobi9(double* M, double* vec) {
+    double V[81];
+#pragma unroll
+    for (int i = 0; i < 81; ++i) V[i] = (i % 10 == 0) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        double off = 0.0, diag = 0.0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+#pragma unroll
+            for (int j = 0; j < 9; ++j) {
+                if (i != j) off = off + M[9 * i + j] * M[9 * i + j];
+                else diag = diag + M[9 * i + j] * M[9 * i + j];
+            }
+        if (off <= diag * 1e-30) break;
+#pragma unroll
+        for (int p = 0; p < 8; ++p)
+#pragma unroll
+            for (int q = p + 1; q < 9; ++q) {
+                double apq = M[9 * p + q];
+                if (apq != 0.0) {
+                    double app = M[9 * p + p], aqq = M[9 * q + q];
+                    double theta = (aqq - app) / (2.0 * apq);
+                    double t = 1.0 / (fabs(theta) + sqrt(theta * theta + 1.0));
+                    if (theta < 0.0) t = -t;
+                    double c = 1.0 / sqrt(t * t + 1.0);
+                    double s = t * c;
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) {
+                        double mkp = M[9 * k + p], mkq = M[9 * k + q];
+                        M[9 * k + p] = c * mkp - s * mkq;
+                        M[9 * k + q] = s * mkp + c * mkq;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) {
+                        double mpk = M[9 * p + k], mqk = M[9 * q + k];
+                        M[9 * p + k] = c * mpk - s * mqk;
+                        M[9 * q + k] = s * mpk + c * mqk;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) {
+                        double vkp = V[9 * k + p], vkq = V[9 * k + q];
+                        V[9 * k + p] = c * vkp - s * vkq;
+                        V[9 * k + q] = s * vkp + c * vkq;
+                    }
+                }
+            }
+    }
+    int best = 0;
+    double bv = M[0];
+#pragma unroll
+    for (int i = 1; i < 9; ++i)
+        if (M[9 * i + i] < bv) { bv = M[9 * i + i]; best = i; }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        double v = V[9 * k];
+#pragma unroll
+        for (int i = 1; i < 9; ++i) v = (best == i) ? V[9 * k + i] : v;
+        vec[k] = v;
+    }
+}
+
+// least-squares DLT over the points in idx[0..m) (core.py:2959-2981)
+__device__ void dlt_ls_wave(const double* __restrict__ p1, const double* __restrict__ p2, const int* __restrict__ idx,
+                            int m, double* __restrict__ Hout, double* __restrict__ sM) {
+    const int lane = threadIdx.x & 63;
+    double T1[9], T2[9];
+    hartley_wave(p1, idx, m, T1);
+    hartley_wave(p2, idx, m, T2);
+    double part[45];
+#pragma unroll
+    for (int e = 0; e < 45; ++e) part[e] = 0.0;
+    for (int k = lane; k < m; k += 64) {
+        int i = idx[k];
+        double x1 = T1[0] * p1[2 * i] + T1[2], y1 = T1[4] * p1[2 * i + 1] + T1[5];
+        double x2 = T2[0] * p2[2 * i] + T2[2], y2 = T2[4] * p2[2 * i + 1] + T2[5];
+        double re[9], ro[9];
+        dlt_rows(x1, y1, x2, y2, re, ro);
+        int e = 0;
+#pragma unroll
+        for (int a = 0; a < 9; ++a)
+#pragma unroll
+            for (int b = a; b < 9; ++b) {
+                part[e] = part[e] + (re[a] * re[b] + ro[a] * ro[b]);
+                e++;
+            }
+    }
+    int e = 0;
+#pragma unroll
+    for (int a = 0; a < 9; ++a)
+#pragma unroll
+        for (int b = a; b < 9; ++b) {
+            double v = wave_tree_d(part[e]);
+            if (lane == 0) { sM[9 * a + b] = v; sM[9 * b + a] = v; }
+            e++;
+        }
+    __syncthreads();
+    if (lane == 0) {
+        double M[81], h[9], H[9];
+#pragma unroll
+        for (int k = 0; k < 81; ++k) M[k] = sM[k];
+        jacobi9(M, h);
+        denorm(h, T1, T2, H);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Hout[k] = H[k];
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(64) void k_finalize(const double* __restrict__ p1, const double* __restrict__ p2, int N,
+                                                  double tq, int lo, int min_inl, const double* __restrict__ Hs,
+                                                  const int* __restrict__ bestp, int* __restrict__ idx,
+                                                  uint8_t* __restrict__ mask2, double* __restrict__ Hout,
+                                                  uint8_t* __restrict__ mask, int* __restrict__ ninl,
+                                                  int* __restrict__ best_out) {
+    __shared__ double sM[81];
+    __shared__ double sH1[9], sH2[9];
+    const int lane = threadIdx.x;
+    const int best = *bestp;
+    double Hb[9], Hbi[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Hb[k] = Hs[(size_t)best * 9 + k];
+    inv3(Hb, Hbi);
+    int nb = 0;
+    for (int i = lane; i < N; i += 64) {
+        double e = sym_err(Hb, Hbi, p1[2 * i], p1[2 * i + 1], p2[2 * i], p2[2 * i + 1], true);
+        uint8_t f = e < tq;
+        mask[i] = f;
+        nb += f;
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) nb += __shfl_xor(nb, m);
+    if (lane == 0) {
+        for (int k = 0; k < 9; ++k) Hout[k] = Hb[k];
+        *best_out = best;
+    }
+    int nfinal = nb;
+    if (lo && nb >= min_inl) {
+        __syncthreads();
+        int m = compact_wave(mask, N, idx);
+        __syncthreads();
+        dlt_ls_wave(p1, p2, idx, m, sH1, sM);
+        double H1[9], H1i[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) H1[k] = sH1[k];
+        inv3(H1, H1i);
+        int n2 = 0;
+        for (int i = lane; i < N; i += 64) {
+            double e = sym_err(H1, H1i, p1[2 * i], p1[2 * i + 1], p2[2 * i], p2[2 * i + 1], false);
+            uint8_t f = e < tq;
+            mask2[i] = f;
+            n2 += f;
+        }
+#pragma unroll
+        for (int mm = 32; mm >= 1; mm >>= 1) n2 += __shfl_xor(n2, mm);
+        __syncthreads();
+        if (n2 >= min_inl) {
+            int m2 = compact_wave(mask2, N, idx);
+            __syncthreads();
+            dlt_ls_wave(p1, p2, idx, m2, sH2, sM);
+            for (int i = lane; i < N; i += 64) mask[i] = mask2[i];
+            if (lane == 0)
+                for (int k = 0; k < 9; ++k) Hout[k] = sH2[k];
+            nfinal = n2;
+        } else {
+            if (lane == 0)
+                for (int k = 0; k < 9; ++k) Hout[k] = H1[k];
+        }
+    }
+    if (lane == 0) *ninl = nfinal;
+}
+
+// ---------------------------------------------------------------- host ----
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t ransac_workspace(int N, int B) {
+    return align256((size_t)B * 8) + align256((size_t)B * 72) + align256(18 * 8) + align256(16) +
+           align256((size_t)(N > 0 ? N : 1) * 4) + align256((size_t)(N > 0 ? N : 1));
+}
+
+int ransac(const double* p1, const double* p2, int N, uint64_t seed, int B, double thr, double sigma_max, int method,
+           int lo, int min_inl, void* ws_, size_t ws_bytes, double* H, uint8_t* mask, int* ninl, int* best,
+           hipStream_t s) {
+    if (ws_bytes < ransac_workspace(N, B)) { set_error("ransac: workspace too small"); return MSIFT_EWORKSPACE; }
+    if (!p1 || !p2 || !H || !mask || !ninl) { set_error("ransac: null buffer"); return MSIFT_EINVAL; }
+    char* ws = (char*)ws_;
+    size_t off = 0;
+    double* scores = (double*)(ws + off); off += align256((size_t)B * 8);
+    double* Hs = (double*)(ws + off); off += align256((size_t)B * 72);
+    double* T = (double*)(ws + off); off += align256(18 * 8);
+    int* ints = (int*)(ws + off); off += align256(16);
+    int* idx = (int*)(ws + off); off += align256((size_t)N * 4);
+    uint8_t* mask2 = (uint8_t*)(ws + off);
+    const double tq = thr * thr;
+    const double den = 2.0 * (sigma_max * sigma_max);
+    hipLaunchKernelGGL(k_normalize, dim3(1), dim3(64), 0, s, p1, p2, N, T);
+    hipLaunchKernelGGL(k_hypotheses, dim3((B + kHypPerBlock - 1) / kHypPerBlock), dim3(64), 0, s, p1, p2, N, B,
+                       (uint32_t)seed, (uint32_t)(seed >> 32), (const double*)T, method, den, tq, scores, Hs);
+    hipLaunchKernelGGL(k_argmax, dim3(1), dim3(1024), 0, s, (const double*)scores, B, &ints[0]);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, p1, p2, N, tq, lo, min_inl, (const double*)Hs,
+                       (const int*)&ints[0], idx, mask2, H, mask, ninl, best ? best : &ints[1]);
+    MSIFT_LAUNCH_CHECK();
+    return MSIFT_OK;
+}
+
+}  // namespace msift

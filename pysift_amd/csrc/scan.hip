@@ -1,5 +1,5 @@
-// Device-wide exclusive scan of int32 counts in one workgroup (counts are
-// small: extrema blocks, compaction blocks, per-keypoint angle counts).
+// Device-wide exclusive scan of int32 counts in one workgroup: tiles of
+// 4096 (4 per thread), wave scans by shuffles, a carry across tiles.
 #include "msift_internal.hpp"
 
 namespace msift {
@@ -9,32 +9,49 @@ __global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ in, int* 
                                                 int* __restrict__ total_a, int* __restrict__ total_b,
                                                 const int* __restrict__ gate, int gate_below) {
     if (gate && *gate >= gate_below) return;
-    __shared__ int sm[1024];
-    const int tid = threadIdx.x;
-    // blocked layout: 1024 contiguous chunks, read with a strided (coalesced) sweep
-    const int chunk = (n + 1023) / 1024;
-    const int beg = tid * chunk;
-    const int end = min(n, beg + chunk);
-    int s = 0;
-    for (int i = beg; i < end; ++i) s += in[i];
-    sm[tid] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int t = tid >= off ? sm[tid - off] : 0;
+    __shared__ int wsum[16];
+    __shared__ int tile_tot;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int carry = 0;
+    for (int base = 0; base < n; base += 4096) {
+        const int i0 = base + 4 * tid;
+        int v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = (i0 + k < n) ? in[i0 + k] : 0;
+        const int c = (v[0] + v[1]) + (v[2] + v[3]);
+        int x = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            int y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wsum[wv] = x;
         __syncthreads();
-        sm[tid] += t;
+        if (wv == 0) {
+            int t = lane < 16 ? wsum[lane] : 0;
+            int xs = t;
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                int y = __shfl_up(xs, off);
+                if (lane >= off) xs += y;
+            }
+            if (lane < 16) wsum[lane] = xs - t;
+            if (lane == 15) tile_tot = xs;
+        }
+        __syncthreads();
+        int e = carry + wsum[wv] + (x - c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (i0 + k < n) out[i0 + k] = e;
+            e += v[k];
+        }
+        carry += tile_tot;
         __syncthreads();
     }
-    int run = sm[tid] - s;
-    for (int i = beg; i < end; ++i) {
-        int v = in[i];
-        out[i] = run;
-        run += v;
-    }
-    if (tid == 1023) {
-        out[n] = sm[1023];
-        if (total_a) *total_a = sm[1023];
-        if (total_b) *total_b = sm[1023];
+    if (tid == 0) {
+        out[n] = carry;
+        if (total_a) *total_a = carry;
+        if (total_b) *total_b = carry;
     }
 }
 

@@ -157,27 +157,46 @@ class PySIFT:
                          torch.empty((cap, 128), dtype=torch.uint8, device=dev),
                          torch.zeros(1, dtype=torch.int32, device=dev))
 
+    def launch(self, img, out=None, profile=False):
+        """Enqueue the device pipeline on the current stream without waiting.
+        img: (H, W) uint8 CUDA tensor.  out: optional (kp6, f32, u8, count)
+        buffers with room for `capacity` keypoints.  Returns those buffers;
+        the count stays on the device."""
+        torch = _torch()
+        H, W = img.shape
+        dev = img.device
+        self._ensure_buffers(H, W, dev)
+        kp, df, du, cnt = out if out is not None else self._out
+        phase = (ctypes.c_float * 8)() if profile else None
+        rc = _lib.lib().msift_detect_describe(
+            self._h, _lib.ptr(img), H, W, img.stride(0), _lib.ptr(self._ws), self._ws.numel(), self.capacity,
+            _lib.ptr(kp), _lib.ptr(df), _lib.ptr(du), _lib.ptr(cnt),
+            ctypes.cast(phase, ctypes.c_void_p) if profile else None, _lib.stream_ptr(dev))
+        _lib.check(rc, "detectAndCompute")
+        if profile:
+            self._phase_timings = {k: float(phase[i]) for i, k in enumerate(PHASE_KEYS)}
+        self._n_calls += 1
+        return kp, df, du, cnt
+
+    def new_outputs(self, device=None):
+        torch = _torch()
+        dev = device or self._device_of()
+        cap = self.capacity
+        return (torch.empty((cap, 6), dtype=torch.float32, device=dev),
+                torch.empty((cap, 128), dtype=torch.float32, device=dev),
+                torch.empty((cap, 128), dtype=torch.uint8, device=dev),
+                torch.zeros(1, dtype=torch.int32, device=dev))
+
     def detect_raw(self, gray, profile=False):
         """Run the device pipeline; returns (kp6, desc_f32, desc_u8) views of
         length N on the device: kp6 columns x, y, size, angle, response, octave."""
         torch = _torch()
         img = self._to_device_u8(gray)
-        H, W = img.shape
-        dev = img.device
-        with torch.cuda.device(dev):
-            self._ensure_buffers(H, W, dev)
-            kp, df, du, cnt = self._out
-            phase = (ctypes.c_float * 8)() if profile else None
-            rc = _lib.lib().msift_detect_describe(
-                self._h, _lib.ptr(img), H, W, W, _lib.ptr(self._ws), self._ws.numel(), self.capacity,
-                _lib.ptr(kp), _lib.ptr(df), _lib.ptr(du), _lib.ptr(cnt),
-                ctypes.cast(phase, ctypes.c_void_p) if profile else None, _lib.stream_ptr(dev))
-            _lib.check(rc, "detectAndCompute")
+        with torch.cuda.device(img.device):
+            kp, df, du, cnt = self.launch(img, profile=profile)
             n = min(int(cnt.item()), self.capacity)
-        if profile:
-            self._phase_timings = {k: float(phase[i]) for i, k in enumerate(PHASE_KEYS)}
-        self._n_calls += 1
-        return kp[:n], df[:n], du[:n]
+        # fresh tensors: the launch buffers are reused by the next call
+        return kp[:n].clone(), df[:n].clone(), du[:n].clone()
 
     def detectAndCompute(self, gray, mask=None, profile=False, gpu_output=False, u8=False):
         """Detect keypoints and compute descriptors (core.py:2021-2172).

@@ -47,7 +47,8 @@ def test_pyramid_bitwise(H, W, dbl):
         g_ref, d_ref = O.pyramid_octave(img, p, o)
         g, d = _pyramid_from_ws(s, H, W, o)
         assert g.shape == g_ref.shape
-        np.testing.assert_array_equal(g, g_ref, err_msg=f"gauss octave {o}")
+        # the top Gaussian level is consumed only through its DoG and is not stored
+        np.testing.assert_array_equal(g[:-1], g_ref[:-1], err_msg=f"gauss octave {o}")
         np.testing.assert_array_equal(d, d_ref, err_msg=f"dog octave {o}")
 
 
