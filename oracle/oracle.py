@@ -80,6 +80,14 @@ def _declare(L):
     L.or_warp_feather.argtypes = [_P, _I, _I, _P, _I, _I, _P, _I, _I, _D, _D, _P, _P]
     L.or_bgr_to_gray.restype = None; L.or_bgr_to_gray.argtypes = [_P, _I, _I, _P]
     L.or_set_threads.restype = _I; L.or_set_threads.argtypes = [_I]
+    L.or_extrema_octave.restype = _I
+    L.or_extrema_octave.argtypes = [_P, _I, _I, _I, ctypes.c_float, _I, _I, _P]
+    L.or_refine_octave.restype = _I
+    L.or_refine_octave.argtypes = [_P, _I, _I, _I, _I, ctypes.c_float, ctypes.c_float, _P, _I, _P, _P]
+    L.or_orientation.restype = _I
+    L.or_orientation.argtypes = [_P, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P]
+    L.or_descriptor.restype = None
+    L.or_descriptor.argtypes = [_P, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, _I, _I, _P, _P]
 
 
 def _ptr(a):
@@ -252,3 +260,42 @@ def bgr_to_gray(bgr):
     g = np.empty(bgr.shape[:2], np.uint8)
     lib().or_bgr_to_gray(_ptr(bgr), bgr.shape[0], bgr.shape[1], _ptr(g))
     return g
+
+
+# ---- single-stage hooks -----------------------------------------------------
+def extrema_octave(D, thr, border=5, cap=1 << 20):
+    """find_extrema_fused on one DoG stack (core.py:176-217): (n,3) s,r,c."""
+    D = np.ascontiguousarray(D, np.float32)
+    S2, H, W = D.shape
+    out = np.empty((cap, 3), np.int32)
+    n = lib().or_extrema_octave(_ptr(D), S2 - 2, H, W, float(thr), border, cap, _ptr(out))
+    return out[:n].copy()
+
+
+def refine_octave(D, cands, thr, edge=12.1, border=5):
+    """refine_keypoints on one DoG stack (core.py:343-463), FP64 solve.
+    Returns valid (n,) bool and out (n,4) ri, ci, si, response."""
+    D = np.ascontiguousarray(D, np.float32)
+    c = np.ascontiguousarray(cands, np.int32).reshape(-1, 3)
+    S2, H, W = D.shape
+    valid = np.zeros(len(c), np.int32); out = np.zeros((len(c), 4), np.float32)
+    lib().or_refine_octave(_ptr(D), S2 - 2, H, W, border, float(thr), float(edge), _ptr(c), len(c),
+                           _ptr(valid), _ptr(out))
+    return valid.astype(bool), out
+
+
+def orientation(G, ri, ci, sigma):
+    """orientation_hist for one keypoint on one Gaussian level (core.py:228-333)."""
+    G = np.ascontiguousarray(G, np.float32)
+    a = np.empty(4, np.float32)
+    n = lib().or_orientation(_ptr(G), G.shape[0], G.shape[1], float(ri), float(ci), float(sigma), _ptr(a))
+    return a[:n].copy()
+
+
+def descriptor(G, ri, ci, ang_rad, ph, rootsift=True):
+    """sift_descriptor_coop + RootSIFT for one keypoint (core.py:693-835, 1870-1873)."""
+    G = np.ascontiguousarray(G, np.float32)
+    d = np.empty(128, np.float32); u = np.empty(128, np.uint8)
+    lib().or_descriptor(_ptr(G), G.shape[0], G.shape[1], float(ri), float(ci), float(ang_rad), int(ph),
+                        int(rootsift), _ptr(d), _ptr(u))
+    return d, u

@@ -1338,3 +1338,48 @@ int or_set_threads(int n)
     return 1;
 #endif
 }
+
+/* ========================================================================
+ * Single-stage hooks used by the oracle's own pinning tests.
+ * ======================================================================== */
+int or_extrema_octave(const float* D, int S, int H, int W, float thr, int border, int cap, int* out)
+{
+    pyramid_t P;
+    memset(&P, 0, sizeof(P));
+    P.n_oct = 1; P.S = S; P.Ho[0] = H; P.Wo[0] = W; P.D[0] = (float*)D;
+    cand_t* c = NULL;
+    int n = find_extrema(&P, thr, border, cap, &c);
+    for (int i = 0; i < n; ++i) { out[3 * i] = c[i].s; out[3 * i + 1] = c[i].r; out[3 * i + 2] = c[i].c; }
+    free(c);
+    return n;
+}
+
+int or_refine_octave(const float* D, int S, int H, int W, int border, float thr, float edge,
+                     const int* cands, int n, int* valid, float* out)
+{
+    pyramid_t P;
+    memset(&P, 0, sizeof(P));
+    P.n_oct = 1; P.S = S; P.Ho[0] = H; P.Wo[0] = W; P.D[0] = (float*)D;
+    int m = 0;
+    for (int i = 0; i < n; ++i) {
+        cand_t cd = { 0, cands[3 * i], cands[3 * i + 1], cands[3 * i + 2] };
+        kpt_t k;
+        valid[i] = refine_one(&P, cd, border, (double)thr, (double)edge, &k);
+        if (valid[i]) {
+            out[4 * i] = k.ri; out[4 * i + 1] = k.ci; out[4 * i + 2] = (float)k.si; out[4 * i + 3] = k.resp;
+            m++;
+        }
+    }
+    return m;
+}
+
+int or_orientation(const float* G, int H, int W, float ri, float ci, float sigma, float* angles)
+{
+    return orientation_one(G, H, W, ri, ci, sigma, angles);
+}
+
+void or_descriptor(const float* G, int H, int W, float ri, float ci, float ang_rad, int ph, int rootsift,
+                   float* out, uint8_t* out_u8)
+{
+    descriptor_one(G, H, W, ri, ci, ang_rad, ph, rootsift, out, out_u8);
+}

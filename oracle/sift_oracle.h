@@ -83,6 +83,15 @@ void or_warp_feather(const uint8_t* img1, int h1, int w1, const uint8_t* img2, i
                      const double* H, int cw, int ch, double ox, double oy,
                      uint8_t* pano /*ch*cw*3*/, uint8_t* valid /*ch*cw*/);
 void or_bgr_to_gray(const uint8_t* bgr, int H, int W, uint8_t* gray);
+int  or_set_threads(int n);
+
+/* ---- single-stage hooks (one octave / one keypoint) ---- */
+int  or_extrema_octave(const float* D, int S, int H, int W, float thr, int border, int cap, int* out);
+int  or_refine_octave(const float* D, int S, int H, int W, int border, float thr, float edge,
+                      const int* cands, int n, int* valid, float* out);
+int  or_orientation(const float* G, int H, int W, float ri, float ci, float sigma, float* angles);
+void or_descriptor(const float* G, int H, int W, float ri, float ci, float ang_rad, int ph, int rootsift,
+                   float* out, uint8_t* out_u8);
 
 #ifdef __cplusplus
 }
