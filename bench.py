@@ -60,6 +60,39 @@ def cpu_baseline(views, seconds=12.0):
             "sample": f"oracle detect_describe on {n} x 1024x768 views ({dt:.1f} s, {threads} OpenMP threads)"}
 
 
+def bench_match(torch, dev, n, method, iters=5):
+    """Exact INT8-MFMA matching of two n x 128 uint8 descriptor sets; returns
+    (TOPS counted once as 2 n^2 128 / t, ms, mutual pairs)."""
+    from pysift_amd.matcher import match_u8, match_workspace_bytes
+    g = torch.Generator(device="cpu").manual_seed(4)
+    a = torch.randint(0, 256, (n, 128), dtype=torch.uint8, generator=g)
+    b = a.clone()
+    b[: n // 2] = torch.clamp(a[: n // 2].int() + torch.randint(-4, 5, (n // 2, 128), generator=g), 0, 255).to(torch.uint8)
+    b[n // 2:] = torch.randint(0, 256, (n - n // 2, 128), dtype=torch.uint8, generator=g)
+    a, b = a.to(dev), b.to(dev)
+    ws = torch.empty(match_workspace_bytes(n, n), dtype=torch.uint8, device=dev)
+    out = match_u8(a, b, 0.75, method, sync=False, ws=ws)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(iters):
+        out = match_u8(a, b, 0.75, method, sync=False, ws=ws)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / iters
+    return 2.0 * n * n * 128 / (ms * 1e-3) / 1e12, ms, int(out[3].item())
+
+
+def bench_stitch(va, vb, iters=5):
+    from pysift_amd import GPUPyStitch
+    st = GPUPyStitch(inlier_thresh=3.0, ransac_iters=2000, verbose=False, save_output=False)
+    st.stitch(va, vb)
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        st.stitch(va, vb)
+    return (time.perf_counter() - t0) * 1e3 / iters, st.last_metrics
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +100,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--pyramid-only", action="store_true", help="(profiling) run only detect on view a")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="(counter calibration) one 256 MiB device-to-device copy, then exit")
+    ap.add_argument("--match-n", type=int, default=40000, help="descriptors per side for the match TOPS leg")
+    ap.add_argument("--no-extra", action="store_true", help="skip the match / stitch legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -81,8 +118,17 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.calib_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes": src.numel() * 4}))
+        return
     va, vb, Hgt = syn.synth_pair(seed=1000)
-    gray = [(0.114 * v[..., 0] + 0.587 * v[..., 1] + 0.299 * v[..., 2]).round().astype(np.uint8) for v in (va, vb)]
+    # BGR -> gray, fixed-point BT.601 as cv2.cvtColor (the stitcher's first kernel)
+    gray = [((v[..., 0].astype(np.int32) * 1868 + v[..., 1].astype(np.int32) * 9617
+              + v[..., 2].astype(np.int32) * 4899 + 8192) >> 14).astype(np.uint8) for v in (va, vb)]
     imgs = [torch.from_numpy(g).to(dev) for g in gray]
     H, W = gray[0].shape
     s = PySIFT(device=dev)
@@ -136,6 +182,16 @@ def main():
                      "algorithmic_bytes": B, "avg_ms": round(t_pyr * 1e3, 4)},
         "phase_ms": {k: round(v, 4) for k, v in s._phase_timings.items()},
     }
+    if not args.no_extra and not args.pyramid_only:
+        extra = {}
+        for method in ("hilo", "shift"):
+            tops, ms, npairs = bench_match(torch, dev, args.match_n, method)
+            extra[f"match_{method}"] = {"n": args.match_n, "tops": round(tops, 2), "ms": round(ms, 4),
+                                        "mfma_int8_peak_tops": PEAK_INT8_TOPS, "pairs": npairs}
+        st_ms, met = bench_stitch(va, vb)
+        extra["stitch_pair_ms"] = round(st_ms, 3)
+        extra["stitch_metrics"] = {k: met[k] for k in ("kp1", "kp2", "matches", "inliers")}
+        line["extra"] = extra
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(gray)
     print(json.dumps(line))
