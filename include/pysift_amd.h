@@ -78,6 +78,26 @@ int msift_detect_describe(msift_handle* h, const uint8_t* d_img, int H, int W, i
                           float* d_kp, float* d_desc_f32, uint8_t* d_desc_u8, int* d_count,
                           float* phase_ms, void* stream);
 
+/* Batched detectAndCompute over n images of uneven size (no reference
+ * counterpart: the reference processes one image per call, core.py:2021).
+ * A plan fixes the image sizes, builds the per-image / per-octave tile tables
+ * in device memory (the only allocation, done here) and the workspace layout.
+ * Inputs are packed back to back (row pitch = width) at the offsets that
+ * msift_batch_sizes reports.  Every stage runs as one launch over all images
+ * and the call never synchronises, so it can be captured in a HIP graph.
+ * Outputs: images concatenated in order, each sorted by (x, y) exactly as
+ * msift_detect_describe would return it; d_offsets[n+1], d_counts[n]. */
+typedef struct msift_batch msift_batch;
+int msift_batch_create(msift_handle* h, int n, const int* H, const int* W, msift_batch** plan);
+int msift_batch_destroy(msift_batch* plan);
+int msift_batch_sizes(msift_batch* plan, size_t* ws_bytes, size_t* input_bytes, size_t* input_offsets);
+/* tile height of the blur kernels (0 = default, 16, 32, 64); results do not
+ * depend on it (set by the autotuner). */
+int msift_batch_set_blur_tile(msift_batch* plan, int tile_h);
+int msift_detect_describe_batched(msift_batch* plan, const uint8_t* d_packed, void* d_ws, size_t ws_bytes, int cap,
+                                  float* d_kp, float* d_desc_f32, uint8_t* d_desc_u8, int* d_counts, int* d_offsets,
+                                  void* stream);
+
 /* Octave-level access for tests: Gaussian (S+3) and DoG (S+2) stacks of
  * octave `o` inside a workspace filled by msift_detect_describe. */
 int msift_pyramid_view(msift_handle* h, int H, int W, void* d_ws, int o,

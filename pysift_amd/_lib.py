@@ -20,7 +20,8 @@ EXPORTS = [
     "msift_abi_version", "msift_last_error", "msift_create", "msift_destroy", "msift_workspace_bytes",
     "msift_detect_describe", "msift_pyramid_view", "msift_match_workspace_bytes", "msift_match_u8",
     "msift_match_f16", "msift_ransac_workspace_bytes", "msift_ransac_homography", "msift_warp_feather",
-    "msift_bgr_to_gray",
+    "msift_bgr_to_gray", "msift_batch_create", "msift_batch_destroy", "msift_batch_sizes",
+    "msift_batch_set_blur_tile", "msift_detect_describe_batched",
 ]
 
 
@@ -68,6 +69,13 @@ def _declare(L):
     L.msift_warp_feather.argtypes = [_P, _I, _I, _P, _I, _I, _P, _I, _I, _D, _D, _P, _P, _P]
     L.msift_bgr_to_gray.restype = _I
     L.msift_bgr_to_gray.argtypes = [_P, _I, _I, _P, _P]
+    L.msift_batch_create.restype = _I
+    L.msift_batch_create.argtypes = [_P, _I, _P, _P, ctypes.POINTER(_P)]
+    L.msift_batch_destroy.restype = _I; L.msift_batch_destroy.argtypes = [_P]
+    L.msift_batch_sizes.restype = _I; L.msift_batch_sizes.argtypes = [_P, _P, _P, _P]
+    L.msift_batch_set_blur_tile.restype = _I; L.msift_batch_set_blur_tile.argtypes = [_P, _I]
+    L.msift_detect_describe_batched.restype = _I
+    L.msift_detect_describe_batched.argtypes = [_P, _P, _P, _SZ, _I, _P, _P, _P, _P, _P, _P]
 
 
 def lib():
