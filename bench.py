@@ -144,6 +144,8 @@ def main():
                     help="(counter calibration) one 256 MiB device-to-device copy, then exit")
     ap.add_argument("--match-n", type=int, default=40000, help="descriptors per side for the match TOPS leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the match / stitch legs")
+    ap.add_argument("--only", choices=["match", "batch", "stitch"], default=None,
+                    help="(profiling) run a single extra leg and print its numbers")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -166,6 +168,14 @@ def main():
         print(json.dumps({"calib_copy_bytes": src.numel() * 4}))
         return
     va, vb, Hgt = syn.synth_pair(seed=1000)
+    if args.only:
+        if args.only == "match":
+            print(json.dumps({m: bench_match(torch, dev, args.match_n, m) for m in ("hilo", "shift")}))
+        elif args.only == "batch":
+            print(json.dumps(bench_batch(torch, dev)))
+        else:
+            print(json.dumps(bench_stitch(va, vb)))
+        return
     # BGR -> gray, fixed-point BT.601 as cv2.cvtColor (the stitcher's first kernel)
     gray = [((v[..., 0].astype(np.int32) * 1868 + v[..., 1].astype(np.int32) * 9617
               + v[..., 2].astype(np.int32) * 4899 + 8192) >> 14).astype(np.uint8) for v in (va, vb)]

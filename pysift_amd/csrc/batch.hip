@@ -25,7 +25,6 @@ struct BImg {
 struct BOct {
     int img, o, H, W;
     size_t G_off, D_off;
-    int blk0, nseg;     // first extrema logical block, 1024-column segments per row
 };
 
 struct BJob {           // one image's share of one pyramid step
@@ -49,12 +48,14 @@ struct msift_batch {
     std::vector<Step> steps;
     std::vector<msift::BJob> jobs;
     std::vector<msift::PixJob> ups_cols, ups_interp, unit;   // pixel-range tables
+    std::vector<msift::ExtOct> ext;                          // extrema geometry per (image, octave)
     int ups_cols_px, ups_interp_px, unit_px;
-    int n_ext_blk;
+    int n_ext_jobs, n_ext_rows;
+    long long n_ext_units;
     size_t in_bytes;
     // workspace carve
     size_t o_hist, o_thr, o_cnt, o_masks, o_ecnt, o_eoff, o_cbase, o_cands, o_valid, o_kslot, o_fcnt, o_foff,
-        o_k1a, o_k1b, o_kstart, o_k1, o_k2, o_ang, o_nang, o_aoff, o_E, o_rank, total;
+        o_k1a, o_k1b, o_kstart, o_k1, o_k2, o_ang, o_nang, o_aoff, o_E, o_rank, o_stmp, total;
     int n_cand_cap, n_cmp_blk;
     // device copies of the tables
     void* d_tables;
@@ -62,6 +63,7 @@ struct msift_batch {
     msift::BOct* d_oct;
     msift::BJob* d_jobs;
     msift::PixJob *d_ups_cols, *d_ups_interp, *d_unit;
+    msift::ExtOct* d_ext;
     int blur_th;        // tile height (autotuned): 0 = default per radius
 };
 
@@ -126,52 +128,43 @@ __global__ __launch_bounds__(64) void k_bthresh(const BImg* __restrict__ im, con
 }
 
 // ---------------------------------------------------------------- spline --
+// spline / unit kernels: blockIdx.y = image job, x strides over its pixels
 __global__ __launch_bounds__(256) void k_bups_cols(const BImg* __restrict__ im, const PixJob* __restrict__ jobs,
-                                                    int njobs, int total, const uint8_t* __restrict__ in, char* ws,
+                                                    const uint8_t* __restrict__ in, char* ws, UpsTaps h) {
+    const BImg I = im[jobs[blockIdx.y].img];
+    const int n = I.H * I.W;
+    double* out = (double*)(ws + I.pf0_off);
+    for (int y = blockIdx.x; y < I.H; y += gridDim.x)
+        for (int x = threadIdx.x; x < I.W; x += 256) out[(size_t)y * I.W + x] = ups_pf_col(in + I.in_off, I.W, I.H, x, y, h);
+    (void)n;
+}
+
+__global__ __launch_bounds__(256) void k_bups_rows(const BImg* __restrict__ im, const PixJob* __restrict__ jobs, char* ws,
                                                     UpsTaps h) {
-    for (int p = blockIdx.x * 256 + threadIdx.x; p < total; p += gridDim.x * 256) {
-        int j = bsearch_last(jobs, njobs, p, [](const PixJob& q) { return q.px0; });
-        const BImg I = im[jobs[j].img];
-        int lp = p - jobs[j].px0;
-        int y = lp / I.W, x = lp - y * I.W;
-        ((double*)(ws + I.pf0_off))[lp] = ups_pf_col(in + I.in_off, I.W, I.H, x, y, h);
-    }
+    const BImg I = im[jobs[blockIdx.y].img];
+    const double* src = (const double*)(ws + I.pf0_off);
+    double* out = (double*)(ws + I.pf1_off);
+    for (int y = blockIdx.x; y < I.H; y += gridDim.x)
+        for (int x = threadIdx.x; x < I.W; x += 256) out[(size_t)y * I.W + x] = ups_pf_row(src + (size_t)y * I.W, I.W, x, h);
 }
 
-__global__ __launch_bounds__(256) void k_bups_rows(const BImg* __restrict__ im, const PixJob* __restrict__ jobs,
-                                                    int njobs, int total, char* ws, UpsTaps h) {
-    for (int p = blockIdx.x * 256 + threadIdx.x; p < total; p += gridDim.x * 256) {
-        int j = bsearch_last(jobs, njobs, p, [](const PixJob& q) { return q.px0; });
-        const BImg I = im[jobs[j].img];
-        int lp = p - jobs[j].px0;
-        int y = lp / I.W, x = lp - y * I.W;
-        const double* row = (const double*)(ws + I.pf0_off) + (size_t)y * I.W;
-        ((double*)(ws + I.pf1_off))[lp] = ups_pf_row(row, I.W, x, h);
-    }
+__global__ __launch_bounds__(256) void k_bups_interp(const BImg* __restrict__ im, const PixJob* __restrict__ jobs, char* ws) {
+    const BImg I = im[jobs[blockIdx.y].img];
+    const double zy = (2 * I.H > 1) ? (double)(I.H - 1) / (double)(2 * I.H - 1) : 0.0;
+    const double zx = (2 * I.W > 1) ? (double)(I.W - 1) / (double)(2 * I.W - 1) : 0.0;
+    const double* cf = (const double*)(ws + I.pf1_off);
+    float* out = (float*)(ws + I.base_off);
+    const int Wo = 2 * I.W;
+    for (int oy = blockIdx.x; oy < 2 * I.H; oy += gridDim.x)
+        for (int ox = threadIdx.x; ox < Wo; ox += 256) out[(size_t)oy * Wo + ox] = ups_interp(cf, I.H, I.W, zy, zx, oy, ox);
 }
 
-__global__ __launch_bounds__(256) void k_bups_interp(const BImg* __restrict__ im, const PixJob* __restrict__ jobs,
-                                                      int njobs, int total, char* ws) {
-    for (int p = blockIdx.x * 256 + threadIdx.x; p < total; p += gridDim.x * 256) {
-        int j = bsearch_last(jobs, njobs, p, [](const PixJob& q) { return q.px0; });
-        const BImg I = im[jobs[j].img];
-        int lp = p - jobs[j].px0;
-        int Wo = 2 * I.W;
-        int oy = lp / Wo, ox = lp - oy * Wo;
-        double zy = (2 * I.H > 1) ? (double)(I.H - 1) / (double)(2 * I.H - 1) : 0.0;
-        double zx = (2 * I.W > 1) ? (double)(I.W - 1) / (double)(2 * I.W - 1) : 0.0;
-        ((float*)(ws + I.base_off))[lp] = ups_interp((const double*)(ws + I.pf1_off), I.H, I.W, zy, zx, oy, ox);
-    }
-}
-
-__global__ __launch_bounds__(256) void k_bunit(const BImg* __restrict__ im, const PixJob* __restrict__ jobs, int njobs,
-                                                int total, const uint8_t* __restrict__ in, char* ws) {
-    for (int p = blockIdx.x * 256 + threadIdx.x; p < total; p += gridDim.x * 256) {
-        int j = bsearch_last(jobs, njobs, p, [](const PixJob& q) { return q.px0; });
-        const BImg I = im[jobs[j].img];
-        int lp = p - jobs[j].px0;
-        ((float*)(ws + I.base_off))[lp] = (float)in[I.in_off + lp] / 255.0f;
-    }
+__global__ __launch_bounds__(256) void k_bunit(const BImg* __restrict__ im, const PixJob* __restrict__ jobs,
+                                                const uint8_t* __restrict__ in, char* ws) {
+    const BImg I = im[jobs[blockIdx.y].img];
+    float* out = (float*)(ws + I.base_off);
+    const int n = I.H * I.W;
+    for (int p = blockIdx.x * 256 + threadIdx.x; p < n; p += gridDim.x * 256) out[p] = (float)in[I.in_off + p] / 255.0f;
 }
 
 // ------------------------------------------------------------------ blur --
@@ -179,7 +172,21 @@ template <int R, int TH>
 __global__ __launch_bounds__(256) void k_bblur(const BJob* __restrict__ jobs, int njobs, char* ws, Taps taps) {
     __shared__ __attribute__((aligned(16))) float sIn[BlurSmem<R, TH>::nIn];
     __shared__ __attribute__((aligned(16))) float sH[BlurSmem<R, TH>::nH];
-    const int j = bsearch_last(jobs, njobs, (int)blockIdx.x, [](const BJob& q) { return q.tile0; });
+    __shared__ int s_j;
+    if (threadIdx.x < 64) {
+        // the wave scans the job starts 64 at a time (no dependent loads)
+        int last = -1;
+        for (int b0 = 0; b0 < njobs; b0 += 64) {
+            int k = b0 + (int)threadIdx.x;
+            bool le = k < njobs && jobs[k].tile0 <= (int)blockIdx.x;
+            unsigned long long m = __ballot(le);
+            if (m) last = b0 + 63 - __clzll(m);
+            if (m != ~0ull) break;
+        }
+        if (threadIdx.x == 0) s_j = last;
+    }
+    __syncthreads();
+    const int j = s_j;
     const BJob J = jobs[j];
     const int t = blockIdx.x - J.tile0;
     const int ty = t / J.tiles_x, tx = t - ty * J.tiles_x;
@@ -203,64 +210,35 @@ __global__ __launch_bounds__(256) void k_bcopy_level(const BJob* __restrict__ jo
 }
 
 // --------------------------------------------------------------- extrema --
-constexpr int kSegB = 1024;
-
-struct BExt { int io, s, r, c0; };
-
-__device__ __forceinline__ BExt bext_decode(const BOct* __restrict__ oct, int n_io, int S, int border, int b) {
-    int io = bsearch_last(oct, n_io, b, [](const BOct& q) { return q.blk0; });
-    // octaves without interior blocks share blk0 with the next one: walk forward
-    while (io + 1 < n_io && oct[io + 1].blk0 <= b) ++io;
-    const BOct O = oct[io];
-    int lb = b - O.blk0;
-    int nrows = O.H - 2 * border;
-    int per_s = nrows * O.nseg;
-    int s = 1 + lb / per_s;
-    int rem = lb - (s - 1) * per_s;
-    int r = border + rem / O.nseg;
-    int seg = rem - (r - border) * O.nseg;
-    BExt e;
-    e.io = io; e.s = s; e.r = r; e.c0 = border + seg * kSegB + 4 * (int)threadIdx.x;
-    return e;
-}
+struct ExtTabPtr {
+    const ExtOct* p;
+    __device__ __forceinline__ ExtOct get(int i) const { return p[i]; }
+};
 
 // pass 1 (gate == NULL) or the ct/2 retry (only images with gate[i] < below)
-__global__ __launch_bounds__(256) void k_bext_count(const BOct* __restrict__ oct, int n_io, int S, int border, int nblk,
-                                                     char* ws, const float* __restrict__ thr, int thr_idx,
-                                                     const int* __restrict__ gate, int below,
-                                                     uint8_t* __restrict__ masks, int* __restrict__ counts) {
-    __shared__ int wsum[4];
-    for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
-        BExt p = bext_decode(oct, n_io, S, border, b);
-        const BOct O = oct[p.io];
-        const bool run = !gate || gate[O.img] < below;
-        int bits = 0;
-        if (run) {
-            const float* D = (const float*)(ws + O.D_off);
-            const float t = thr[2 * O.img + thr_idx];
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if (p.c0 + j < O.W - border && is_extremum(D, O.H, O.W, p.s, p.r, p.c0 + j, t)) bits |= 1 << j;
-        }
-        masks[(size_t)b * 256 + threadIdx.x] = (uint8_t)bits;
-        int tot;
-        block_excl_scan256(__popc(bits), wsum, &tot);
-        if (threadIdx.x == 0) counts[b] = tot;
-    }
+__global__ __launch_bounds__(256) void k_bext_count(const ExtOct* __restrict__ ext, int n_io, int njobs, int S,
+                                                     int border, const char* ws, const float* __restrict__ thr,
+                                                     int thr_idx, const int* __restrict__ gate, int below,
+                                                     unsigned long long* __restrict__ masks, int* __restrict__ rowcnt,
+                                                     const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    ext_count_waves(ExtTabPtr{ext}, n_io, njobs, ws, S, border, thr, thr_idx, gate, below, masks, rowcnt);
 }
 
 // truncated per-(image, octave) candidate counts and their exclusive scan
-__global__ __launch_bounds__(1024) void k_bcand_base(const BOct* __restrict__ oct, int n_io, int nblk,
-                                                      const int* __restrict__ offs, int capc, int* __restrict__ cbase) {
+__global__ __launch_bounds__(1024) void k_bcand_base(const ExtOct* __restrict__ ext, int n_io, int nrows,
+                                                      const int* __restrict__ rowoff, int capc, int* __restrict__ cbase,
+                                                      const int* __restrict__ skip) {
     __shared__ int sm[1024];
+    if (skip && *skip) return;
     int carry = 0;
     for (int base = 0; base < n_io; base += 1024) {
         int io = base + threadIdx.x;
         int c = 0;
         if (io < n_io) {
-            int b0 = oct[io].blk0;
-            int b1 = io + 1 < n_io ? oct[io + 1].blk0 : nblk;
-            int t = offs[b1] - offs[b0];
+            int r0 = ext[io].row0;
+            int r1 = io + 1 < n_io ? ext[io + 1].row0 : nrows;
+            int t = rowoff[r1] - rowoff[r0];
             c = t < capc ? t : capc;
         }
         sm[threadIdx.x] = c;
@@ -278,32 +256,21 @@ __global__ __launch_bounds__(1024) void k_bcand_base(const BOct* __restrict__ oc
     if (threadIdx.x == 0) cbase[n_io] = carry;
 }
 
-__global__ __launch_bounds__(256) void k_bext_write(const BOct* __restrict__ oct, int n_io, int S, int border, int nblk,
-                                                     const uint8_t* __restrict__ masks, const int* __restrict__ offs,
-                                                     const int* __restrict__ cbase, int capc, int4* __restrict__ cands) {
-    __shared__ int wsum[4];
-    for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
-        int bits = masks[(size_t)b * 256 + threadIdx.x];
-        int tot;
-        int pos = offs[b] + block_excl_scan256(__popc(bits), wsum, &tot);
-        if (!bits) continue;
-        BExt p = bext_decode(oct, n_io, S, border, b);
-        int q = pos - offs[oct[p.io].blk0];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if ((bits >> j) & 1) {
-                if (q < capc) cands[cbase[p.io] + q] = make_int4(p.s, p.r, p.c0 + j, p.io);
-                q++;
-            }
-        }
-    }
+__global__ __launch_bounds__(256) void k_bext_write(const ExtOct* __restrict__ ext, int n_io, int nrows, int border,
+                                                     const unsigned long long* __restrict__ masks,
+                                                     const int* __restrict__ rowcnt, const int* __restrict__ rowoff,
+                                                     const int* __restrict__ cbase, int capc, int4* __restrict__ cands,
+                                                     const int* __restrict__ skip) {
+    if (skip && *skip) return;
+    ext_write_waves(ExtTabPtr{ext}, n_io, nrows, border, masks, rowcnt, rowoff, cbase, capc, cands);
 }
 
 __global__ __launch_bounds__(256) void k_brefine(const BOct* __restrict__ oct, int n_io, int S, int border, char* ws,
                                                   const int4* __restrict__ cands, const int* __restrict__ cbase,
                                                   const float* __restrict__ thr, int thr_idx, float edge32,
                                                   uint8_t* __restrict__ valid, Kp* __restrict__ kslot,
-                                                  int* __restrict__ img_count) {
+                                                  int* __restrict__ img_count, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     const int n = cbase[n_io];
     for (int t = blockIdx.x * 256 + threadIdx.x; t < n; t += gridDim.x * 256) {
         int4 cd = cands[t];
@@ -323,8 +290,10 @@ __global__ __launch_bounds__(256) void k_brefine(const BOct* __restrict__ oct, i
 }
 
 __global__ __launch_bounds__(256) void k_bflag_count(const uint8_t* __restrict__ flags, const int* __restrict__ np,
-                                                      int nblk_cap, int* __restrict__ counts) {
+                                                      int nblk_cap, int* __restrict__ counts,
+                                                      const int* __restrict__ skip) {
     __shared__ int wc[4];
+    if (skip && *skip) return;
     const int n = *np;
     const int nblk = (n + 255) / 256;
     // blocks beyond this call's candidates count zero (the scan covers the capacity)
@@ -343,8 +312,9 @@ __global__ __launch_bounds__(256) void k_bflag_count(const uint8_t* __restrict__
 
 __global__ __launch_bounds__(256) void k_bflag_write(const uint8_t* __restrict__ flags, const Kp* __restrict__ src,
                                                       const int* __restrict__ np, const int* __restrict__ offs,
-                                                      Kp* __restrict__ dst) {
+                                                      Kp* __restrict__ dst, const int* __restrict__ skip) {
     __shared__ int wc[4];
+    if (skip && *skip) return;
     const int n = *np;
     const int nblk = (n + 255) / 256;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -361,6 +331,18 @@ __global__ __launch_bounds__(256) void k_bflag_write(const uint8_t* __restrict__
         }
         __syncthreads();
     }
+}
+
+// skip = 1 when no image's pass-1 count is below the retry gate (the whole
+// retry pass is then a no-op and every kernel of it returns at once)
+__global__ __launch_bounds__(256) void k_bneed(int n, const int* __restrict__ cnt1, int below, int* __restrict__ skip) {
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 256)
+        if (cnt1[i] < below) any = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) *skip = any ? 0 : 1;
 }
 
 // per-image keypoint list: pass 1, or the retry for images below the gate
@@ -636,15 +618,15 @@ static void plan_steps(msift_batch* B) {
 
 static int upload_tables(msift_batch* B) {
     auto al = [](size_t x) { return ((x + 255) & ~(size_t)255) + 256; };
-    const void* src[6] = { B->img.data(), B->oct.data(), B->jobs.data(), B->ups_cols.data(), B->ups_interp.data(),
-                           B->unit.data() };
-    size_t sz[6] = { B->img.size() * sizeof(BImg), B->oct.size() * sizeof(BOct), B->jobs.size() * sizeof(BJob),
+    const void* src[7] = { B->img.data(), B->oct.data(), B->jobs.data(), B->ups_cols.data(), B->ups_interp.data(),
+                           B->unit.data(), B->ext.data() };
+    size_t sz[7] = { B->img.size() * sizeof(BImg), B->oct.size() * sizeof(BOct), B->jobs.size() * sizeof(BJob),
                      B->ups_cols.size() * sizeof(PixJob), B->ups_interp.size() * sizeof(PixJob),
-                     B->unit.size() * sizeof(PixJob) };
-    size_t off[6], total = 0;
-    for (int k = 0; k < 6; ++k) { off[k] = total; total += al(sz[k]); }
+                     B->unit.size() * sizeof(PixJob), B->ext.size() * sizeof(ExtOct) };
+    size_t off[7], total = 0;
+    for (int k = 0; k < 7; ++k) { off[k] = total; total += al(sz[k]); }
     std::vector<char> host(total, 0);
-    for (int k = 0; k < 6; ++k) if (sz[k]) memcpy(host.data() + off[k], src[k], sz[k]);
+    for (int k = 0; k < 7; ++k) if (sz[k]) memcpy(host.data() + off[k], src[k], sz[k]);
     if (B->d_tables) { (void)hipFree(B->d_tables); B->d_tables = nullptr; }
     MSIFT_HIP(hipMalloc(&B->d_tables, total));
     MSIFT_HIP(hipMemcpy(B->d_tables, host.data(), total, hipMemcpyHostToDevice));
@@ -655,6 +637,7 @@ static int upload_tables(msift_batch* B) {
     B->d_ups_cols = (PixJob*)(base + off[3]);
     B->d_ups_interp = (PixJob*)(base + off[4]);
     B->d_unit = (PixJob*)(base + off[5]);
+    B->d_ext = (ExtOct*)(base + off[6]);
     return MSIFT_OK;
 }
 
@@ -669,7 +652,8 @@ int batch_create(msift_handle* h, int n, const int* Hs, const int* Ws, msift_bat
     auto take = [&](size_t b) { size_t o = off; off += (b + 255) & ~(size_t)255; return o; };
     // packed input layout
     size_t in_off = 0;
-    int nb = 0;
+    int nj = 0, nr = 0;
+    long long nu = 0;
     for (int i = 0; i < n; ++i) {
         if (Hs[i] < 1 || Ws[i] < 1) { set_error("batch_create: image sizes must be positive"); delete B; return MSIFT_EINVAL; }
         BImg I;
@@ -688,10 +672,7 @@ int batch_create(msift_handle* h, int n, const int* Hs, const int* Ws, msift_bat
             size_t hw = (size_t)bh * bw;
             O.G_off = take(hw * 4 * (B->S + 3));
             O.D_off = take(hw * 4 * (B->S + 2));
-            O.blk0 = nb;
-            int nrows = bh - 2 * p.border, ncols = bw - 2 * p.border;
-            if (bh < 3 || bw < 3 || nrows <= 0 || ncols <= 0) O.nseg = 0;
-            else { O.nseg = (ncols + kSegB - 1) / kSegB; nb += B->S * nrows * O.nseg; }
+            B->ext.push_back(ext_geometry(O.D_off, bh, bw, p.border, B->S, i, &nj, &nr, &nu));
             B->oct.push_back(O);
             bh = (bh + 1) / 2; bw = (bw + 1) / 2;
         }
@@ -705,7 +686,9 @@ int batch_create(msift_handle* h, int n, const int* Hs, const int* Ws, msift_bat
         B->img.push_back(I);
     }
     B->in_bytes = in_off;
-    B->n_ext_blk = nb;
+    B->n_ext_jobs = nj;
+    B->n_ext_rows = nr;
+    B->n_ext_units = nu;
     // pixel-range tables for the spline / unit kernels
     int pc = 0, pi = 0, pu = 0;
     for (int i = 0; i < n; ++i) {
@@ -723,9 +706,9 @@ int batch_create(msift_handle* h, int n, const int* Hs, const int* Ws, msift_bat
     B->o_hist = take((size_t)n * 256 * 4 + (size_t)n * 8 * 4);   // hist + 8 counters per image
     B->o_cnt = B->o_hist + (size_t)n * 256 * 4;
     B->o_thr = take((size_t)n * 2 * 4);
-    B->o_masks = take((size_t)(nb > 0 ? nb : 1) * 256);
-    B->o_ecnt = take((size_t)(nb + 1) * 4);
-    B->o_eoff = take((size_t)(nb + 1) * 4);
+    B->o_masks = take((size_t)(nu > 0 ? nu : 1) * 8);
+    B->o_ecnt = take((size_t)(nr + 1) * 4);
+    B->o_eoff = take((size_t)(nr + 1) * 4);
     B->o_cbase = take((size_t)(n_io + 1) * 4);
     B->n_cand_cap = n_io * B->capc;
     B->o_cands = take((size_t)B->n_cand_cap * 16);
@@ -744,6 +727,11 @@ int batch_create(msift_handle* h, int n, const int* Hs, const int* Ws, msift_bat
     B->o_aoff = take(((size_t)n * B->maxkp + 1) * 4);
     B->o_E = take((size_t)n * 4 * B->maxkp * sizeof(Ent));
     B->o_rank = take((size_t)n * 4 * B->maxkp * 4);
+    {
+        long long big = std::max<long long>((long long)nr, (long long)n * B->maxkp);
+        big = std::max<long long>(big, (long long)B->n_cmp_blk);
+        B->o_stmp = take((size_t)scan_tmp_ints((int)big) * 4);
+    }
     B->total = off;
     plan_steps(B);
     int rc = upload_tables(B);
@@ -767,7 +755,7 @@ int batch_destroy(msift_batch* B) {
 }
 
 static int bkeypoint_pass(msift_batch* B, char* ws, const uint8_t* in, int thr_idx, const int* gate, int* img_count,
-                          Kp* dst, hipStream_t s) {
+                          Kp* dst, const int* skip, hipStream_t s) {
     (void)in;
     msift_handle* h = B->h;
     const msift_params& p = h->p;
@@ -776,22 +764,29 @@ static int bkeypoint_pass(msift_batch* B, char* ws, const uint8_t* in, int thr_i
     int* eoff = (int*)(ws + B->o_eoff);
     int* cbase = (int*)(ws + B->o_cbase);
     float* thr = (float*)(ws + B->o_thr);
-    if (B->n_ext_blk > 0)
-        hipLaunchKernelGGL(k_bext_count, dim3(gcap(B->n_ext_blk, 8192)), dim3(256), 0, s, B->d_oct, n_io, B->S, p.border,
-                           B->n_ext_blk, ws, thr, thr_idx, gate, p.retry_below, (uint8_t*)(ws + B->o_masks), ecnt);
-    launch_scan(ecnt, eoff, B->n_ext_blk, nullptr, nullptr, nullptr, 0, s);
-    hipLaunchKernelGGL(k_bcand_base, dim3(1), dim3(1024), 0, s, B->d_oct, n_io, B->n_ext_blk, eoff, B->capc, cbase);
-    if (B->n_ext_blk > 0)
-        hipLaunchKernelGGL(k_bext_write, dim3(gcap(B->n_ext_blk, 8192)), dim3(256), 0, s, B->d_oct, n_io, B->S, p.border,
-                           B->n_ext_blk, (const uint8_t*)(ws + B->o_masks), eoff, cbase, B->capc, (int4*)(ws + B->o_cands));
+    int* stmp = (int*)(ws + B->o_stmp);
+    const int nr = B->n_ext_rows;
+    if (nr > 0) {
+        hipLaunchKernelGGL(k_bzero, dim3(gcap((nr + 255) / 256, 1024)), dim3(256), 0, s, ecnt, nr);
+        hipLaunchKernelGGL(k_bext_count, dim3(gcap((B->n_ext_jobs + 3) / 4, 16384)), dim3(256), 0, s, B->d_ext, n_io,
+                           B->n_ext_jobs, B->S, p.border, (const char*)ws, thr, thr_idx, gate, p.retry_below,
+                           (unsigned long long*)(ws + B->o_masks), ecnt, skip);
+    }
+    launch_scan(ecnt, eoff, nr, nullptr, nullptr, skip, 1, s, stmp);
+    hipLaunchKernelGGL(k_bcand_base, dim3(1), dim3(1024), 0, s, B->d_ext, n_io, nr, eoff, B->capc, cbase, skip);
+    if (nr > 0)
+        hipLaunchKernelGGL(k_bext_write, dim3(gcap((nr + 3) / 4, 16384)), dim3(256), 0, s, B->d_ext, n_io, nr, p.border,
+                           (const unsigned long long*)(ws + B->o_masks), (const int*)ecnt, (const int*)eoff, cbase,
+                           B->capc, (int4*)(ws + B->o_cands), skip);
     hipLaunchKernelGGL(k_brefine, dim3(gcap(B->n_cmp_blk, 2048)), dim3(256), 0, s, B->d_oct, n_io, B->S, p.border, ws,
                        (const int4*)(ws + B->o_cands), cbase, thr, thr_idx, h->edge32, (uint8_t*)(ws + B->o_valid),
-                       (Kp*)(ws + B->o_kslot), img_count);
+                       (Kp*)(ws + B->o_kslot), img_count, skip);
     hipLaunchKernelGGL(k_bflag_count, dim3(gcap(B->n_cmp_blk, 2048)), dim3(256), 0, s, (const uint8_t*)(ws + B->o_valid),
-                       (const int*)(cbase + n_io), B->n_cmp_blk, (int*)(ws + B->o_fcnt));
-    launch_scan((int*)(ws + B->o_fcnt), (int*)(ws + B->o_foff), B->n_cmp_blk, nullptr, nullptr, nullptr, 0, s);
+                       (const int*)(cbase + n_io), B->n_cmp_blk, (int*)(ws + B->o_fcnt), skip);
+    launch_scan((int*)(ws + B->o_fcnt), (int*)(ws + B->o_foff), B->n_cmp_blk, nullptr, nullptr, skip, 1, s, stmp);
     hipLaunchKernelGGL(k_bflag_write, dim3(gcap(B->n_cmp_blk, 2048)), dim3(256), 0, s, (const uint8_t*)(ws + B->o_valid),
-                       (const Kp*)(ws + B->o_kslot), (const int*)(cbase + n_io), (const int*)(ws + B->o_foff), dst);
+                       (const Kp*)(ws + B->o_kslot), (const int*)(cbase + n_io), (const int*)(ws + B->o_foff), dst,
+                       skip);
     MSIFT_LAUNCH_CHECK();
     return MSIFT_OK;
 }
@@ -814,16 +809,14 @@ int batch_run(msift_batch* B, const uint8_t* in, void* ws_, size_t ws_bytes, int
                        p.adaptive_thresh, thr);
     // A3: base images
     if (B->ups_cols_px > 0) {
-        hipLaunchKernelGGL(k_bups_cols, dim3(gcap((B->ups_cols_px + 255) / 256, 16384)), dim3(256), 0, s, B->d_img,
-                           B->d_ups_cols, (int)B->ups_cols.size(), B->ups_cols_px, in, ws, h->ups);
-        hipLaunchKernelGGL(k_bups_rows, dim3(gcap((B->ups_cols_px + 255) / 256, 16384)), dim3(256), 0, s, B->d_img,
-                           B->d_ups_cols, (int)B->ups_cols.size(), B->ups_cols_px, ws, h->ups);
-        hipLaunchKernelGGL(k_bups_interp, dim3(gcap((B->ups_interp_px + 255) / 256, 16384)), dim3(256), 0, s, B->d_img,
-                           B->d_ups_interp, (int)B->ups_interp.size(), B->ups_interp_px, ws);
+        dim3 g(256, (unsigned)B->ups_cols.size());
+        hipLaunchKernelGGL(k_bups_cols, g, dim3(256), 0, s, B->d_img, B->d_ups_cols, in, ws, h->ups);
+        hipLaunchKernelGGL(k_bups_rows, g, dim3(256), 0, s, B->d_img, B->d_ups_cols, ws, h->ups);
+        hipLaunchKernelGGL(k_bups_interp, dim3(512, (unsigned)B->ups_cols.size()), dim3(256), 0, s, B->d_img,
+                           B->d_ups_cols, ws);
     }
     if (B->unit_px > 0)
-        hipLaunchKernelGGL(k_bunit, dim3(gcap((B->unit_px + 255) / 256, 16384)), dim3(256), 0, s, B->d_img, B->d_unit,
-                           (int)B->unit.size(), B->unit_px, in, ws);
+        hipLaunchKernelGGL(k_bunit, dim3(256, (unsigned)B->unit.size()), dim3(256), 0, s, B->d_img, B->d_unit, in, ws);
     MSIFT_LAUNCH_CHECK();
     // A2-A4: every pyramid step is one launch over all images
     for (const auto& st : B->steps) {
@@ -834,9 +827,11 @@ int batch_run(msift_batch* B, const uint8_t* in, void* ws_, size_t ws_bytes, int
     // A5-A7
     int* k_start = (int*)(ws + B->o_kstart);    // start1[n], start2[n], fstart[n+1], fcount[n]
     int* start1 = k_start, *start2 = k_start + n, *fstart = k_start + 2 * n, *fcount = k_start + 3 * n + 1;
-    int rc = bkeypoint_pass(B, ws, in, 0, nullptr, cnt, (Kp*)(ws + B->o_k1a), s);
+    int* skip2 = cnt + 3 * n;
+    int rc = bkeypoint_pass(B, ws, in, 0, nullptr, cnt, (Kp*)(ws + B->o_k1a), nullptr, s);
     if (rc) return rc;
-    rc = bkeypoint_pass(B, ws, in, 1, cnt, cnt + n, (Kp*)(ws + B->o_k1b), s);
+    hipLaunchKernelGGL(k_bneed, dim3(1), dim3(256), 0, s, n, (const int*)cnt, p.retry_below, skip2);
+    rc = bkeypoint_pass(B, ws, in, 1, cnt, cnt + n, (Kp*)(ws + B->o_k1b), skip2, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_bmerge_counts, dim3(1), dim3(64), 0, s, n, (const int*)cnt, (const int*)(cnt + n),
                        p.retry_below, start1, start2, fstart, fcount);
@@ -850,7 +845,8 @@ int batch_run(msift_batch* B, const uint8_t* in, void* ws_, size_t ws_bytes, int
     hipLaunchKernelGGL(k_borient, dim3(gcap(((long long)n * B->maxkp + 3) / 4, 4096)), dim3(256), 0, s, B->d_img,
                        B->d_oct, n, B->maxkp, ws, (const Kp*)(ws + B->o_k2), (const int*)n2, h->tab,
                        (float4*)(ws + B->o_ang), (int*)(ws + B->o_nang));
-    launch_scan((int*)(ws + B->o_nang), (int*)(ws + B->o_aoff), n * B->maxkp, nullptr, nullptr, nullptr, 0, s);
+    launch_scan((int*)(ws + B->o_nang), (int*)(ws + B->o_aoff), n * B->maxkp, nullptr, nullptr, nullptr, 0, s,
+                (int*)(ws + B->o_stmp));
     hipLaunchKernelGGL(k_bexpand, dim3(gcap(((long long)n * B->maxkp + 255) / 256, 4096)), dim3(256), 0, s, B->d_img, n,
                        B->maxkp, (const Kp*)(ws + B->o_k2), (const int*)n2, (const float4*)(ws + B->o_ang),
                        (const int*)(ws + B->o_nang), (const int*)(ws + B->o_aoff), (Ent*)(ws + B->o_E));

@@ -173,76 +173,31 @@ static int blur(const float* in, float* gout, float* dout, float* dec, int H, in
 }
 
 // ======================================================================
-// A5: 3x3x3 extrema (core.py:176-217).  Logical blocks walk (octave, s, row,
-// 1024-column segment) in raster order, 4 columns per thread.  Pass 1 stores
-// each thread's 4-bit predicate mask and the block count; a scan turns counts
-// into offsets; pass 2 writes candidates at their raster-order position
-// (first `capc` per octave kept).  Grid-stride loops keep a gated (skipped)
-// launch cheap.
+// A5: 3x3x3 extrema (core.py:176-217).  Wave jobs (octave, 16-row strip,
+// 62-column segment) mark extrema in per-(scale, row, segment) lane masks
+// and count them per (scale, row); a scan of the row counts gives each row's
+// raster-order position; the write pass keeps the first `capc` per octave.
 // ======================================================================
-constexpr int kSeg = 1024;
+struct ExtTabArg {
+    ExtOct o[kMaxOct];
+    __device__ __forceinline__ const ExtOct& get(int i) const { return o[i]; }
+};
 
-struct ExtPos { int o, s, r, c0; };
-
-__device__ __forceinline__ ExtPos ext_decode(const OctInfo& oi, int b) {
-    int o = 0;
-    while (o + 1 < oi.n_oct && b >= oi.blk0[o + 1]) ++o;
-    int lb = b - oi.blk0[o];
-    int bd = oi.border;
-    int nrows = oi.H[o] - 2 * bd, nseg = oi.nseg[o];
-    int per_s = nrows * nseg;
-    int s = 1 + lb / per_s;
-    int rem = lb - (s - 1) * per_s;
-    int r = bd + rem / nseg;
-    int seg = rem - (r - bd) * nseg;
-    ExtPos p;
-    p.o = o; p.s = s; p.r = r; p.c0 = bd + seg * kSeg + 4 * (int)threadIdx.x;
-    return p;
-}
-
-__global__ __launch_bounds__(256) void k_ext_count(OctInfo oi, int nblk, const float* __restrict__ thr, int thr_idx,
+__global__ __launch_bounds__(256) void k_ext_count(ExtTabArg tab, int n_oct, int njobs, const char* ws, int S,
+                                                    int border, const float* __restrict__ thr, int thr_idx,
                                                     const int* __restrict__ gate, int gate_below,
-                                                    uint8_t* __restrict__ masks, int* __restrict__ counts) {
-    if (gate && *gate >= gate_below) return;
-    __shared__ int wsum[4];
-    const float t = thr[thr_idx];
-    for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
-        ExtPos p = ext_decode(oi, b);
-        const int H = oi.H[p.o], W = oi.W[p.o];
-        const float* D = oi.D[p.o];
-        int bits = 0;
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            if (p.c0 + j < W - oi.border && is_extremum(D, H, W, p.s, p.r, p.c0 + j, t)) bits |= 1 << j;
-        masks[(size_t)b * 256 + threadIdx.x] = (uint8_t)bits;
-        int tot;
-        block_excl_scan256(__popc(bits), wsum, &tot);
-        if (threadIdx.x == 0) counts[b] = tot;
-    }
+                                                    unsigned long long* __restrict__ masks,
+                                                    int* __restrict__ rowcnt) {
+    ext_count_waves(tab, n_oct, njobs, ws, S, border, thr, thr_idx, gate, gate_below, masks, rowcnt);
 }
 
-__global__ __launch_bounds__(256) void k_ext_write(OctInfo oi, int nblk, const uint8_t* __restrict__ masks,
-                                                    const int* __restrict__ offs, int capc,
-                                                    int4* __restrict__ cands, const int* __restrict__ gate,
-                                                    int gate_below) {
+__global__ __launch_bounds__(256) void k_ext_write(ExtTabArg tab, int n_oct, int nrows, int border,
+                                                    const unsigned long long* __restrict__ masks,
+                                                    const int* __restrict__ rowcnt, const int* __restrict__ rowoff,
+                                                    int capc, int4* __restrict__ cands,
+                                                    const int* __restrict__ gate, int gate_below) {
     if (gate && *gate >= gate_below) return;
-    __shared__ int wsum[4];
-    for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
-        int bits = masks[(size_t)b * 256 + threadIdx.x];
-        int tot;
-        int pos = offs[b] + block_excl_scan256(__popc(bits), wsum, &tot);
-        if (!bits) continue;
-        ExtPos p = ext_decode(oi, b);
-        int base = offs[oi.blk0[p.o]];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if ((bits >> j) & 1) {
-                int q = pos - base;
-                if (q < capc) cands[(size_t)p.o * capc + q] = make_int4(p.s, p.r, p.c0 + j, 0);
-                pos++;
-            }
-        }
-    }
+    ext_write_waves(tab, n_oct, nrows, border, masks, rowcnt, rowoff, nullptr, capc, cands);
 }
 
 // ======================================================================
@@ -476,9 +431,9 @@ struct Plan {
     int Ho[kMaxOct], Wo[kMaxOct];
     size_t oG[kMaxOct], oD[kMaxOct];
     size_t o_unit, o_pf0, o_pf1, o_hist, o_thr, o_ints, o_masks, o_ecnt, o_eoff, o_cands, o_ccount,
-        o_valid, o_kslot, o_ccnt, o_coff, o_k1, o_k2, o_ang, o_nang, o_aoff, o_E, o_rank;
-    int n_ext_blk, n_cmp_blk;
-    int blk0[kMaxOct + 1], nseg[kMaxOct];
+        o_valid, o_kslot, o_ccnt, o_coff, o_k1, o_k2, o_ang, o_nang, o_aoff, o_E, o_rank, o_stmp;
+    int n_ext_jobs, n_ext_rows, n_cmp_blk;
+    ExtOct ext[kMaxOct];
     size_t total;
 };
 
@@ -520,20 +475,15 @@ static void make_plan(const msift_handle* h, int H, int W, Plan* P) {
     P->o_hist = c.take((256 + 16) * sizeof(int));
     P->o_ints = P->o_hist + 256 * sizeof(int);
     P->o_thr = c.take(4 * sizeof(float));
-    // extrema blocks
-    int nb = 0;
-    for (int o = 0; o < P->n_oct; ++o) {
-        P->blk0[o] = nb;
-        int nrows = P->Ho[o] - 2 * p.border, ncols = P->Wo[o] - 2 * p.border;
-        if (P->Ho[o] < 3 || P->Wo[o] < 3 || nrows <= 0 || ncols <= 0) { P->nseg[o] = 0; continue; }
-        P->nseg[o] = (ncols + kSeg - 1) / kSeg;
-        nb += P->S * nrows * P->nseg[o];
-    }
-    P->blk0[P->n_oct] = nb;
-    P->n_ext_blk = nb;
-    P->o_masks = c.take((size_t)(nb > 0 ? nb : 1) * 256);
-    P->o_ecnt = c.take((size_t)(nb + 1) * sizeof(int));
-    P->o_eoff = c.take((size_t)(nb + 1) * sizeof(int));
+    // extrema: wave jobs, (scale, row) slots, lane-mask units
+    int nj = 0, nr = 0;
+    long long nu = 0;
+    for (int o = 0; o < P->n_oct; ++o) P->ext[o] = ext_geometry(P->oD[o], P->Ho[o], P->Wo[o], p.border, P->S, 0, &nj, &nr, &nu);
+    P->n_ext_jobs = nj;
+    P->n_ext_rows = nr;
+    P->o_masks = c.take((size_t)(nu > 0 ? nu : 1) * 8);
+    P->o_ecnt = c.take((size_t)(nr + 1) * sizeof(int));
+    P->o_eoff = c.take((size_t)(nr + 1) * sizeof(int));
     P->n_slots = P->n_oct * P->capc;
     P->o_cands = c.take((size_t)P->n_slots * sizeof(int4));
     P->o_ccount = c.take(kMaxOct * sizeof(int));
@@ -549,6 +499,7 @@ static void make_plan(const msift_handle* h, int H, int W, Plan* P) {
     P->o_aoff = c.take((size_t)(P->maxkp + 1) * sizeof(int));
     P->o_E = c.take((size_t)P->cap_e * sizeof(Ent));
     P->o_rank = c.take((size_t)P->cap_e * sizeof(int));
+    P->o_stmp = c.take((size_t)scan_tmp_ints(std::max(P->n_ext_rows, P->maxkp + 1)) * sizeof(int));
     P->total = c.off;
 }
 
@@ -560,10 +511,9 @@ static OctInfo oct_info(const Plan& P, char* ws, int border) {
         oi.H[o] = P.Ho[o]; oi.W[o] = P.Wo[o];
         oi.G[o] = (float*)(ws + P.oG[o]);
         oi.D[o] = (float*)(ws + P.oD[o]);
-        oi.blk0[o] = P.blk0[o];
-        oi.nseg[o] = P.nseg[o];
+        oi.blk0[o] = P.ext[o].row0;
     }
-    oi.blk0[P.n_oct] = P.blk0[P.n_oct];
+    oi.blk0[P.n_oct] = P.n_ext_rows;
     return oi;
 }
 
@@ -591,15 +541,21 @@ static int keypoint_pass(const Plan& P, char* ws, const OctInfo& oi, msift_handl
                          const int* gate, int* total_a, int* total_b, hipStream_t s) {
     const int gb = h->p.retry_below;
     float* thr = (float*)(ws + P.o_thr);
-    if (P.n_ext_blk > 0) {
-        hipLaunchKernelGGL(k_ext_count, dim3(grid_cap(P.n_ext_blk, 4096)), dim3(256), 0, s, oi, P.n_ext_blk, thr,
-                           thr_idx, gate, gb, (uint8_t*)(ws + P.o_masks), (int*)(ws + P.o_ecnt));
+    ExtTabArg tab;
+    memcpy(tab.o, P.ext, sizeof(P.ext));
+    int* rowcnt = (int*)(ws + P.o_ecnt);
+    int* rowoff = (int*)(ws + P.o_eoff);
+    if (P.n_ext_rows > 0) {
+        hipLaunchKernelGGL(k_zero_i32, dim3((P.n_ext_rows + 255) / 256), dim3(256), 0, s, rowcnt, P.n_ext_rows);
+        hipLaunchKernelGGL(k_ext_count, dim3(grid_cap((P.n_ext_jobs + 3) / 4, 4096)), dim3(256), 0, s, tab, P.n_oct,
+                           P.n_ext_jobs, (const char*)ws, P.S, h->p.border, thr, thr_idx, gate, gb,
+                           (unsigned long long*)(ws + P.o_masks), rowcnt);
     }
-    launch_scan((int*)(ws + P.o_ecnt), (int*)(ws + P.o_eoff), P.n_ext_blk, nullptr, nullptr, gate, gb, s);
-    if (P.n_ext_blk > 0) {
-        hipLaunchKernelGGL(k_ext_write, dim3(grid_cap(P.n_ext_blk, 4096)), dim3(256), 0, s, oi, P.n_ext_blk,
-                           (const uint8_t*)(ws + P.o_masks), (const int*)(ws + P.o_eoff), P.capc,
-                           (int4*)(ws + P.o_cands), gate, gb);
+    launch_scan(rowcnt, rowoff, P.n_ext_rows, nullptr, nullptr, gate, gb, s, (int*)(ws + P.o_stmp));
+    if (P.n_ext_rows > 0) {
+        hipLaunchKernelGGL(k_ext_write, dim3(grid_cap((P.n_ext_rows + 3) / 4, 4096)), dim3(256), 0, s, tab, P.n_oct,
+                           P.n_ext_rows, h->p.border, (const unsigned long long*)(ws + P.o_masks),
+                           (const int*)rowcnt, (const int*)rowoff, P.capc, (int4*)(ws + P.o_cands), gate, gb);
     }
     hipLaunchKernelGGL(k_refine, dim3(grid_cap((P.n_slots + 255) / 256, 1024)), dim3(256), 0, s, oi,
                        (const int4*)(ws + P.o_cands), (const int*)(ws + P.o_eoff), P.capc, thr, thr_idx,

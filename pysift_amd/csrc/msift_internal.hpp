@@ -78,7 +78,9 @@ int warp_feather(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h
 int bgr_to_gray(const uint8_t* bgr, int H, int W, uint8_t* gray, hipStream_t s);
 
 // scan.hip helpers (device-wide exclusive scan of int32, single workgroup)
+// with tmp (scan_tmp_ints(n) ints) large inputs use a multi-workgroup scan
 void launch_scan(const int* in, int* out, int n, int* total_a, int* total_b, const int* gate,
-                 int gate_below, hipStream_t s);
+                 int gate_below, hipStream_t s, int* tmp = nullptr);
+int scan_tmp_ints(int n);
 
 }  // namespace msift

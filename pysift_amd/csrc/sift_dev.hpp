@@ -64,13 +64,17 @@ __device__ __forceinline__ float ups_interp(const double* __restrict__ cf, int H
     int ix = (int)fx;
     double wx[4];
     bspline_w(xx - fx, wx);
-    int xi[4];
+    int xi[4], yi[4];
+    const bool inner = ix >= 1 && ix + 2 < W && iy >= 1 && iy + 2 < H;
 #pragma unroll
-    for (int b = 0; b < 4; ++b) xi[b] = mirror_idx(ix - 1 + b, W);
+    for (int b = 0; b < 4; ++b) {
+        xi[b] = inner ? ix - 1 + b : mirror_idx(ix - 1 + b, W);
+        yi[b] = inner ? iy - 1 + b : mirror_idx(iy - 1 + b, H);
+    }
     double v = 0.0;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-        const double* row = cf + (size_t)mirror_idx(iy - 1 + a, H) * W;
+        const double* row = cf + (size_t)yi[a] * W;
         double rv = 0.0;
 #pragma unroll
         for (int b = 0; b < 4; ++b) rv = fma(wx[b], row[xi[b]], rv);
@@ -180,43 +184,181 @@ struct BlurSmem {
 };
 
 // --------------------------------------------------------------- extrema --
-__device__ __forceinline__ bool is_extremum(const float* __restrict__ D, int H, int W, int s, int r, int c,
-                                            float thr) {
-    size_t HW = (size_t)H * W;
-    float v = D[(size_t)s * HW + (size_t)r * W + c];
-    if (fabsf(v) <= thr) return false;
-    bool mx = true, mn = true;
+// 3x3x3 extrema (core.py:260-340): D[s][r][c] with |v| > thr that is >= (or
+// <=) all 26 neighbours at scales 1..S, rows/cols in [border, dim-border).
+// One lane per column: a wave covers 62 interior columns (lanes 1..62; lanes
+// 0 and 63 load the halo columns) and walks kExtRB rows of up to 3 scales,
+// keeping each level's row maxima / minima over (c-1, c, c+1) for the rows
+// r-1, r, r+1 in registers, so every DoG value is loaded once per strip.
+// v >= every neighbour  <=>  v >= the max over the 3x3x3 cube including v.
+// Output per (scale, row, 62-column segment): a 64-bit lane mask, and the
+// per-(scale, row) candidate count (integer atomics, order-free).  The write
+// pass enumerates the rows in (s, r) order and each row's bits in column
+// order, which is the raster order of the reference's nonzero().
+constexpr int kExtSeg = 62;
+constexpr int kExtRB = 16;
+
+struct ExtOct {          // extrema geometry of one (image, octave)
+    size_t D_off;        // DoG stack (S+2 levels, H x W) in the workspace
+    int H, W, Hi, Wi;    // size, interior size (minus 2*border)
+    int nseg, nstrip;    // 62-column segments, kExtRB-row strips (0 = no interior)
+    int img;             // image index (threshold and retry gate)
+    int job0, row0;      // first wave job; first (scale, row) slot
+    long long unit0;     // first (scale, row, segment) mask unit
+};
+
+__device__ __forceinline__ float max3f(float a, float b, float c) { return fmaxf(a, fmaxf(b, c)); }
+__device__ __forceinline__ float min3f(float a, float b, float c) { return fminf(a, fminf(b, c)); }
+
+// one wave job: rows [strip*kExtRB, +kExtRB) x segment `seg` of an octave,
+// all scales; must be called by the whole wave (uniform control flow)
+__device__ __forceinline__ void ext_strip(const float* __restrict__ D, const ExtOct& O, int S, int border,
+                                          int strip, int seg, float thr, unsigned long long* __restrict__ masks,
+                                          int* __restrict__ rowcnt) {
+    const int lane = threadIdx.x & 63;
+    const size_t HW = (size_t)O.H * O.W;
+    const int ri0 = strip * kExtRB;
+    const int nr = O.Hi - ri0 < kExtRB ? O.Hi - ri0 : kExtRB;
+    const int cx = seg * kExtSeg + lane - 1;            // interior column of this lane
+    const bool out_lane = lane >= 1 && lane <= kExtSeg && cx < O.Wi;
+    int col = border + cx;
+    if (col > O.W - 1) col = O.W - 1;                   // halo lane past the edge: unused value
+    for (int s0 = 1; s0 <= S; s0 += 3) {
+        const int gs = S - s0 + 1 < 3 ? S - s0 + 1 : 3;  // scales s0 .. s0+gs-1, levels s0-1 .. s0+gs
+        const float* lv[5];
 #pragma unroll
-    for (int ds = -1; ds <= 1; ++ds)
+        for (int l = 0; l < 5; ++l) {
+            int L = s0 - 1 + (l < gs + 1 ? l : gs + 1);
+            lv[l] = D + (size_t)L * HW + col;
+        }
+        float a0[5], a1[5], a2[5], b0[5], b1[5], b2[5], c1[5], c2[5], x[5];
 #pragma unroll
-        for (int dr = -1; dr <= 1; ++dr)
+        for (int l = 0; l < 5; ++l) { a1[l] = a2[l] = b1[l] = b2[l] = c2[l] = 0.0f; }
+        auto load = [&](int r, float* v) {
 #pragma unroll
-            for (int dc = -1; dc <= 1; ++dc) {
-                if (ds == 0 && dr == 0 && dc == 0) continue;
-                float nb = D[(size_t)(s + ds) * HW + (size_t)(r + dr) * W + (c + dc)];
-                mx = mx && (v >= nb);
-                mn = mn && (v <= nb);
+            for (int l = 0; l < 5; ++l) v[l] = lv[l][(size_t)r * O.W];
+        };
+        auto push = [&](const float* v) {
+#pragma unroll
+            for (int l = 0; l < 5; ++l) {
+                float xl = __shfl_up(v[l], 1), xr = __shfl_down(v[l], 1);
+                a0[l] = a1[l]; a1[l] = a2[l]; a2[l] = max3f(xl, v[l], xr);
+                b0[l] = b1[l]; b1[l] = b2[l]; b2[l] = min3f(xl, v[l], xr);
+                c1[l] = c2[l]; c2[l] = v[l];
             }
-    return mx || mn;
+        };
+        const int rA = border + ri0 - 1;
+        load(rA, x); push(x);
+        load(rA + 1, x); push(x);
+        load(rA + 2, x); push(x);
+        for (int i = 0; i < nr; ++i) {
+            const bool more = i + 1 < nr;
+            if (more) load(rA + 3 + i, x);
+            float M[5], m[5];
+#pragma unroll
+            for (int l = 0; l < 5; ++l) { M[l] = max3f(a0[l], a1[l], a2[l]); m[l] = min3f(b0[l], b1[l], b2[l]); }
+            const int ri = ri0 + i;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (k < gs) {
+                    const float v = c1[k + 1];
+                    const bool pass = out_lane && fabsf(v) > thr &&
+                                      (v >= max3f(M[k], M[k + 1], M[k + 2]) || v <= min3f(m[k], m[k + 1], m[k + 2]));
+                    const unsigned long long bits = __ballot(pass);
+                    if (lane == 0) {
+                        const int rs = (s0 + k - 1) * O.Hi + ri;
+                        masks[O.unit0 + (long long)rs * O.nseg + seg] = bits;
+                        if (bits) atomicAdd(&rowcnt[O.row0 + rs], __popcll(bits));
+                    }
+                }
+            }
+            if (more) push(x);
+        }
+    }
 }
 
-// block-wide exclusive scan of one int per thread (256 threads); returns the
-// exclusive prefix and stores the block total in *tot
-__device__ __forceinline__ int block_excl_scan256(int c, int* wsum, int* tot) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    int x = c;
+// candidates of (scale, row) slot `rs` of an octave, in column order, at
+// dst[q] for q = first + 0, 1, ... while q < capc; whole wave
+__device__ __forceinline__ void ext_write_row(const ExtOct& O, int io, int rs, int border,
+                                              const unsigned long long* __restrict__ masks, int first, int capc,
+                                              int4* __restrict__ dst) {
+    const int lane = threadIdx.x & 63;
+    const int s = 1 + rs / O.Hi, ri = rs - (s - 1) * O.Hi;
+    const long long ub = O.unit0 + (long long)rs * O.nseg;
+    int carry = first;
+    for (int j0 = 0; j0 < O.nseg && carry < capc; j0 += 64) {
+        const int j = j0 + lane;
+        unsigned long long m = j < O.nseg ? masks[ub + j] : 0ull;
+        const int c = __popcll(m);
+        int x = c;
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        int y = __shfl_up(x, off);
-        if (lane >= off) x += y;
+        for (int off = 1; off < 64; off <<= 1) {
+            int y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        int q = carry + x - c;
+        while (m && q < capc) {
+            const int b = __builtin_ctzll(m);
+            m &= m - 1;
+            dst[q] = make_int4(s, border + ri, border + j * kExtSeg + b - 1, io);
+            ++q;
+        }
+        carry += __shfl(x, 63);
     }
-    if (lane == 63) wsum[wv] = x;
-    __syncthreads();
-    int pre = 0;
-    for (int w = 0; w < wv; ++w) pre += wsum[w];
-    *tot = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
-    __syncthreads();
-    return pre + x - c;
+}
+
+// host-side geometry of one octave's extrema jobs (counters advanced)
+inline ExtOct ext_geometry(size_t D_off, int H, int W, int border, int S, int img, int* jobs, int* rows,
+                           long long* units) {
+    ExtOct e;
+    e.D_off = D_off; e.H = H; e.W = W; e.img = img;
+    e.Hi = H - 2 * border; e.Wi = W - 2 * border;
+    if (H < 3 || W < 3 || e.Hi <= 0 || e.Wi <= 0) { e.Hi = 0; e.Wi = 0; e.nseg = 0; e.nstrip = 0; }
+    else { e.nseg = (e.Wi + kExtSeg - 1) / kExtSeg; e.nstrip = (e.Hi + kExtRB - 1) / kExtRB; }
+    e.job0 = *jobs; e.row0 = *rows; e.unit0 = *units;
+    *jobs += e.nseg * e.nstrip;
+    *rows += S * e.Hi;
+    *units += (long long)S * e.Hi * e.nseg;
+    return e;
+}
+
+// count pass over a table of octaves: Tab::get(i) -> ExtOct; wave jobs are
+// strided over the grid and each wave walks the table forward
+template <class Tab>
+__device__ __forceinline__ void ext_count_waves(const Tab& tab, int n_io, int njobs, const char* ws, int S,
+                                                int border, const float* __restrict__ thr, int thr_idx,
+                                                const int* __restrict__ gate, int below,
+                                                unsigned long long* __restrict__ masks, int* __restrict__ rowcnt) {
+    const int nw = gridDim.x * (blockDim.x >> 6);
+    int io = 0;
+    for (int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); j < njobs; j += nw) {
+        while (io + 1 < n_io && tab.get(io + 1).job0 <= j) ++io;
+        const ExtOct O = tab.get(io);
+        if (gate && gate[O.img] >= below) continue;
+        const int jj = j - O.job0;
+        const int strip = jj / O.nseg, seg = jj - strip * O.nseg;
+        ext_strip((const float*)(ws + O.D_off), O, S, border, strip, seg, thr[2 * O.img + thr_idx], masks, rowcnt);
+    }
+}
+
+// write pass: one wave per (octave, scale, row) slot with candidates.
+// rowoff = exclusive scan of rowcnt; cbase (NULL: io * capc) = output base
+template <class Tab>
+__device__ __forceinline__ void ext_write_waves(const Tab& tab, int n_io, int nrows, int border,
+                                                const unsigned long long* __restrict__ masks,
+                                                const int* __restrict__ rowcnt, const int* __restrict__ rowoff,
+                                                const int* __restrict__ cbase, int capc, int4* __restrict__ cands) {
+    const int nw = gridDim.x * (blockDim.x >> 6);
+    int io = 0;
+    for (int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); g < nrows; g += nw) {
+        if (rowcnt[g] == 0) continue;
+        while (io + 1 < n_io && tab.get(io + 1).row0 <= g) ++io;
+        const ExtOct O = tab.get(io);
+        const int first = rowoff[g] - rowoff[O.row0];
+        if (first >= capc) continue;
+        int4* dst = cands + (cbase ? (size_t)cbase[io] : (size_t)io * capc);
+        ext_write_row(O, io, g - O.row0, border, masks, first, capc, dst);
+    }
 }
 
 // ---------------------------------------------------------------- refine --
@@ -301,23 +443,42 @@ __device__ __forceinline__ int orient_keypoint(const float* __restrict__ G, int 
     int n_cols = c_max - c_min + 1; if (n_cols < 1) n_cols = 1;
     int n_rows = r_max - r_min + 1; if (n_rows < 0) n_rows = 0;
     int total = n_rows * n_cols;
-    for (int p = lane; p < total; p += 64) {
-        int rq = p / n_cols;
-        int r = r_min + rq;
-        int c = c_min + (p - rq * n_cols);
-        const float* g = G + (size_t)r * W + c;
-        float dx = g[1] - g[-1];
-        float dy = g[W] - g[-W];
-        float mag = sqrtf(dx * dx + dy * dy);
-        float ori = fmod360(atan2_f(dy, dx) * 57.29577951f + 360.0f);
-        float dr = (float)r - ri, dc = (float)c - ci;
-        float wt = exp_f(-(dr * dr + dc * dc) * inv_2sw2);
-        float bin_f = ori * 0.1f;
-        int bin_i = ((int)bin_f) % 36;
-        int bin_r = (bin_i + 1) % 36;
-        float frac = bin_f - floorf(bin_f);
-        P[bin_i * 64 + lane] += mag * wt * (1.0f - frac);
-        P[bin_r * 64 + lane] += mag * wt * frac;
+    // chunks of kPf pixels per lane: all gradient taps of a chunk are loaded
+    // before any is used (memory-level parallelism); accumulation order per
+    // lane stays p = lane, lane + 64, ...
+    constexpr int kPf = 8;
+    for (int p0 = lane; p0 < total; p0 += 64 * kPf) {
+        float gl[kPf], gr[kPf], gu[kPf], gd[kPf];
+#pragma unroll
+        for (int k = 0; k < kPf; ++k) {
+            const int p = p0 + 64 * k;
+            gl[k] = gr[k] = gu[k] = gd[k] = 0.0f;
+            if (p < total) {
+                int rq = p / n_cols;
+                const float* g = G + (size_t)(r_min + rq) * W + (c_min + (p - rq * n_cols));
+                gl[k] = g[-1]; gr[k] = g[1]; gu[k] = g[-W]; gd[k] = g[W];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kPf; ++k) {
+            const int p = p0 + 64 * k;
+            if (p >= total) break;
+            int rq = p / n_cols;
+            int r = r_min + rq;
+            int c = c_min + (p - rq * n_cols);
+            float dx = gr[k] - gl[k];
+            float dy = gd[k] - gu[k];
+            float mag = sqrtf(dx * dx + dy * dy);
+            float ori = fmod360(atan2_f(dy, dx) * 57.29577951f + 360.0f);
+            float dr = (float)r - ri, dc = (float)c - ci;
+            float wt = exp_f(-(dr * dr + dc * dc) * inv_2sw2);
+            float bin_f = ori * 0.1f;
+            int bin_i = ((int)bin_f) % 36;
+            int bin_r = (bin_i + 1) % 36;
+            float frac = bin_f - floorf(bin_f);
+            P[bin_i * 64 + lane] += mag * wt * (1.0f - frac);
+            P[bin_r * 64 + lane] += mag * wt * frac;
+        }
     }
     float hist[36];
 #pragma unroll
@@ -370,45 +531,66 @@ __device__ __forceinline__ void describe_keypoint(const float* __restrict__ G, i
     const int side = 2 * ph + 1;
     const int total_px = side * side;
     const int iri = (int)ri, ici = (int)ci;
-    for (int flat = lane; flat < total_px; flat += 64) {
-        int rq = flat / side;
-        int r_off = rq - ph;
-        int c_off = (flat - rq * side) - ph;
-        int rr = iri + r_off, cc = ici + c_off;
-        if (rr < 1 || rr >= H - 1 || cc < 1 || cc >= W - 1) continue;
-        float xr = cos_a * (float)c_off - sin_a * (float)r_off;
-        float yr = sin_a * (float)c_off + cos_a * (float)r_off;
-        float xb = xr * scale_sp + 1.5f;
-        float yb = yr * scale_sp + 1.5f;
-        if (xb < -0.5f || xb >= 3.5f || yb < -0.5f || yb >= 3.5f) continue;
-        const float* g = G + (size_t)rr * W + cc;
-        float dx = g[1] - g[-1];
-        float dy = g[W] - g[-W];
-        float mag = sqrtf(dx * dx + dy * dy);
-        float ori_deg = fmod360((atan2_f(dy, dx) - ang_rad) * 57.29577951f + 360.0f);
-        float gw = exp_f(-(xr * xr + yr * yr) * inv_2s2);
-        float wmag = mag * gw;
-        float ob = ori_deg * 0.02222222f;
-        int oi0 = (int)floorf(ob);
-        float df = ob - (float)oi0;
-        int xi0 = (int)floorf(xb), yi0 = (int)floorf(yb);
-        float dxb = xb - (float)xi0, dyb = yb - (float)yi0;
-        int xi0c = xi0 < 0 ? 0 : (xi0 > 3 ? 3 : xi0);
-        int yi0c = yi0 < 0 ? 0 : (yi0 > 3 ? 3 : yi0);
-        int xi1c = xi0 + 1 < 0 ? 0 : (xi0 + 1 > 3 ? 3 : xi0 + 1);
-        int yi1c = yi0 + 1 < 0 ? 0 : (yi0 + 1 > 3 ? 3 : yi0 + 1);
-        int oi0w = ((oi0 % 8) + 8) % 8;
-        int oi1w = ((oi0 + 1) % 8 + 8) % 8;
-        float wy0 = 1.0f - dyb, wy1 = dyb, wx0 = 1.0f - dxb, wx1 = dxb, wo0 = 1.0f - df, wo1 = df;
-        float* h = P + lane;
-        h[(yi0c * 32 + xi0c * 8 + oi0w) * 64] += wy0 * wx0 * wo0 * wmag;
-        h[(yi0c * 32 + xi0c * 8 + oi1w) * 64] += wy0 * wx0 * wo1 * wmag;
-        h[(yi0c * 32 + xi1c * 8 + oi0w) * 64] += wy0 * wx1 * wo0 * wmag;
-        h[(yi0c * 32 + xi1c * 8 + oi1w) * 64] += wy0 * wx1 * wo1 * wmag;
-        h[(yi1c * 32 + xi0c * 8 + oi0w) * 64] += wy1 * wx0 * wo0 * wmag;
-        h[(yi1c * 32 + xi0c * 8 + oi1w) * 64] += wy1 * wx0 * wo1 * wmag;
-        h[(yi1c * 32 + xi1c * 8 + oi0w) * 64] += wy1 * wx1 * wo0 * wmag;
-        h[(yi1c * 32 + xi1c * 8 + oi1w) * 64] += wy1 * wx1 * wo1 * wmag;
+    // chunks of kPf pixels per lane, taps loaded before use (see orientation)
+    constexpr int kPf = 8;
+    for (int f0 = lane; f0 < total_px; f0 += 64 * kPf) {
+        float gl[kPf], gr[kPf], gu[kPf], gd[kPf], xrs[kPf], yrs[kPf];
+        bool okk[kPf];
+#pragma unroll
+        for (int k = 0; k < kPf; ++k) {
+            const int flat = f0 + 64 * k;
+            int rq = flat / side;
+            int r_off = rq - ph;
+            int c_off = (flat - rq * side) - ph;
+            int rr = iri + r_off, cc = ici + c_off;
+            float xr = cos_a * (float)c_off - sin_a * (float)r_off;
+            float yr = sin_a * (float)c_off + cos_a * (float)r_off;
+            float xb = xr * scale_sp + 1.5f;
+            float yb = yr * scale_sp + 1.5f;
+            bool ok = flat < total_px && !(rr < 1 || rr >= H - 1 || cc < 1 || cc >= W - 1) &&
+                      !(xb < -0.5f || xb >= 3.5f || yb < -0.5f || yb >= 3.5f);
+            okk[k] = ok;
+            xrs[k] = xr; yrs[k] = yr;
+            gl[k] = gr[k] = gu[k] = gd[k] = 0.0f;
+            if (ok) {
+                const float* g = G + (size_t)rr * W + cc;
+                gl[k] = g[-1]; gr[k] = g[1]; gu[k] = g[-W]; gd[k] = g[W];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kPf; ++k) {
+            if (!okk[k]) continue;
+            const float xr = xrs[k], yr = yrs[k];
+            float xb = xr * scale_sp + 1.5f;
+            float yb = yr * scale_sp + 1.5f;
+            float dx = gr[k] - gl[k];
+            float dy = gd[k] - gu[k];
+            float mag = sqrtf(dx * dx + dy * dy);
+            float ori_deg = fmod360((atan2_f(dy, dx) - ang_rad) * 57.29577951f + 360.0f);
+            float gw = exp_f(-(xr * xr + yr * yr) * inv_2s2);
+            float wmag = mag * gw;
+            float ob = ori_deg * 0.02222222f;
+            int oi0 = (int)floorf(ob);
+            float df = ob - (float)oi0;
+            int xi0 = (int)floorf(xb), yi0 = (int)floorf(yb);
+            float dxb = xb - (float)xi0, dyb = yb - (float)yi0;
+            int xi0c = xi0 < 0 ? 0 : (xi0 > 3 ? 3 : xi0);
+            int yi0c = yi0 < 0 ? 0 : (yi0 > 3 ? 3 : yi0);
+            int xi1c = xi0 + 1 < 0 ? 0 : (xi0 + 1 > 3 ? 3 : xi0 + 1);
+            int yi1c = yi0 + 1 < 0 ? 0 : (yi0 + 1 > 3 ? 3 : yi0 + 1);
+            int oi0w = ((oi0 % 8) + 8) % 8;
+            int oi1w = ((oi0 + 1) % 8 + 8) % 8;
+            float wy0 = 1.0f - dyb, wy1 = dyb, wx0 = 1.0f - dxb, wx1 = dxb, wo0 = 1.0f - df, wo1 = df;
+            float* h = P + lane;
+            h[(yi0c * 32 + xi0c * 8 + oi0w) * 64] += wy0 * wx0 * wo0 * wmag;
+            h[(yi0c * 32 + xi0c * 8 + oi1w) * 64] += wy0 * wx0 * wo1 * wmag;
+            h[(yi0c * 32 + xi1c * 8 + oi0w) * 64] += wy0 * wx1 * wo0 * wmag;
+            h[(yi0c * 32 + xi1c * 8 + oi1w) * 64] += wy0 * wx1 * wo1 * wmag;
+            h[(yi1c * 32 + xi0c * 8 + oi0w) * 64] += wy1 * wx0 * wo0 * wmag;
+            h[(yi1c * 32 + xi0c * 8 + oi1w) * 64] += wy1 * wx0 * wo1 * wmag;
+            h[(yi1c * 32 + xi1c * 8 + oi0w) * 64] += wy1 * wx1 * wo0 * wmag;
+            h[(yi1c * 32 + xi1c * 8 + oi1w) * 64] += wy1 * wx1 * wo1 * wmag;
+        }
     }
     float v0 = 0.0f, v1 = 0.0f;
 #pragma unroll 8
