@@ -58,6 +58,10 @@ def _declare(L):
     L.or_exp.restype = _D; L.or_exp.argtypes = [_D]
     L.or_log2.restype = _D; L.or_log2.argtypes = [_D]
     L.or_atan2.restype = _D; L.or_atan2.argtypes = [_D, _D]
+    L.or_exp_f.restype = ctypes.c_float; L.or_exp_f.argtypes = [ctypes.c_float]
+    L.or_atan2_f.restype = ctypes.c_float; L.or_atan2_f.argtypes = [ctypes.c_float, ctypes.c_float]
+    L.or_exp_f_arr.restype = None; L.or_exp_f_arr.argtypes = [_P, _P, ctypes.c_int]
+    L.or_atan2_f_arr.restype = None; L.or_atan2_f_arr.argtypes = [_P, _P, _P, ctypes.c_int]
     L.or_sincos.restype = None; L.or_sincos.argtypes = [_D, _P, _P]
     L.or_radius.restype = _I; L.or_radius.argtypes = [_D]
     L.or_taps.restype = None; L.or_taps.argtypes = [_D, _I, _P]
@@ -109,6 +113,23 @@ def log2(x):
 
 def atan2(y, x):
     return lib().or_atan2(float(y), float(x))
+
+
+def exp_f(x):
+    """Single-precision exp of the orientation / descriptor weights (array)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.empty_like(x)
+    lib().or_exp_f_arr(_ptr(x), _ptr(y), x.size)
+    return y
+
+
+def atan2_f(y, x):
+    """Single-precision atan2 of the orientation / descriptor gradients (arrays)."""
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    a = np.empty_like(y)
+    lib().or_atan2_f_arr(_ptr(y), _ptr(x), _ptr(a), y.size)
+    return a
 
 
 def sincos(x):

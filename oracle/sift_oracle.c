@@ -174,8 +174,83 @@ void or_sincos(double x, double* s, double* c)
     }
 }
 
-static float f_exp(float x) { return (float)or_exp((double)x); }
-static float f_atan2(float y, float x) { return (float)or_atan2((double)y, (double)x); }
+/* Single-precision exp / atan2 for the orientation and descriptor weights,
+ * restating atan2f / __expf / expf of the reference's CUDA kernels
+ * (core.py:272-273 orientation, 758-760 descriptor) as a fixed sequence of IEEE float operations with explicit fmaf,
+ * so that the device code can reproduce every bit.
+ * exp: x = n ln2 + r, ln2 = 0x1.62e4p-1 + 0x1.7f7d1cp-20 (n ln2_hi exact for
+ * |n| < 256), e^r = degree-7 Taylor polynomial, times 2^n (<= 1 ulp on
+ * [-87, 88]; 0 below -87).
+ * atan2: t = min(|x|,|y|)/max(|x|,|y|) (IEEE division), atan(t) = t P(t^2),
+ * P of degree 9 fitted to atan(t)/t on [0, 1] by iteratively reweighted
+ * least squares on the relative error (coefficients rounded to float); then
+ * pi/2 - a, pi - a and the sign (<= 2 ulp).  Pinned by test_oracle against
+ * numpy's float64 exp / arctan2 on dense sweeps. */
+float or_exp_f(float x)
+{
+    if (x != x) return x;
+    if (x > 88.0f) return HUGE_VALF;
+    if (x < -87.0f) return 0.0f;
+    const float n = floorf(x * 0x1.715476p+0f + 0.5f);
+    float r = fmaf(n, -0x1.62e4p-1f, x);
+    r = fmaf(n, -0x1.7f7d1cp-20f, r);
+    float p = 0x1.a01a02p-13f;
+    p = fmaf(p, r, 0x1.6c16c2p-10f);
+    p = fmaf(p, r, 0x1.111112p-7f);
+    p = fmaf(p, r, 0x1.555556p-5f);
+    p = fmaf(p, r, 0x1.555556p-3f);
+    p = fmaf(p, r, 0.5f);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
+    return ldexpf(p, (int)n);
+}
+
+float or_atan2_f(float y, float x)
+{
+    const float PI = 0x1.921fb6p+1f, PIO2 = 0x1.921fb6p+0f;
+    if (x != x || y != y) return x + y;
+    int ys = signbit(y) != 0, xs = signbit(x) != 0;
+    float ax = fabsf(x), ay = fabsf(y), a;
+    if (ay == 0.0f) {
+        a = xs ? PI : 0.0f;
+    } else if (ax == 0.0f) {
+        a = PIO2;
+    } else if (isinf(ax) || isinf(ay)) {
+        if (isinf(ax) && isinf(ay)) a = xs ? 0x1.2d97c8p+1f : 0x1.921fb6p-1f;
+        else if (isinf(ax)) a = xs ? PI : 0.0f;
+        else a = PIO2;
+    } else {
+        float t = (ay > ax ? ax : ay) / (ay > ax ? ay : ax);
+        float s2 = t * t;
+        float p = -0x1.cbd234p-10f;
+        p = fmaf(p, s2, 0x1.5f85d2p-7f);
+        p = fmaf(p, s2, -0x1.f8b94ep-6f);
+        p = fmaf(p, s2, 0x1.d77a5cp-5f);
+        p = fmaf(p, s2, -0x1.572366p-4f);
+        p = fmaf(p, s2, 0x1.c02fd6p-4f);
+        p = fmaf(p, s2, -0x1.2415a4p-3f);
+        p = fmaf(p, s2, 0x1.999084p-3f);
+        p = fmaf(p, s2, -0x1.555534p-2f);
+        p = fmaf(p, s2, 1.0f);
+        a = t * p;
+        if (ay > ax) a = PIO2 - a;
+        if (xs) a = PI - a;
+    }
+    return ys ? -a : a;
+}
+
+void or_exp_f_arr(const float* x, float* y, int n)
+{
+    for (int i = 0; i < n; ++i) y[i] = or_exp_f(x[i]);
+}
+
+void or_atan2_f_arr(const float* y, const float* x, float* a, int n)
+{
+    for (int i = 0; i < n; ++i) a[i] = or_atan2_f(y[i], x[i]);
+}
+
+static float f_exp(float x) { return or_exp_f(x); }
+static float f_atan2(float y, float x) { return or_atan2_f(y, x); }
 
 /* fmodf(x, 360) for |x| < 1440, exact (each subtraction is exact) */
 static float fmod360(float x)

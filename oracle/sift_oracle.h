@@ -38,6 +38,10 @@ typedef struct {
 double or_exp(double x);
 double or_log2(double x);
 double or_atan2(double y, double x);
+float or_exp_f(float x);
+float or_atan2_f(float y, float x);
+void or_exp_f_arr(const float* x, float* y, int n);
+void or_atan2_f_arr(const float* y, const float* x, float* a, int n);
 void   or_sincos(double x, double* s, double* c);
 
 /* ---- pyramid pieces ---- */

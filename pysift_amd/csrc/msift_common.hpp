@@ -149,8 +149,67 @@ MSIFT_HD void sincos_d(double x, double* s, double* c) {
     }
 }
 
-MSIFT_HD float exp_f(float x) { return (float)exp_d((double)x); }
-MSIFT_HD float atan2_f(float y, float x) { return (float)atan2_d((double)y, (double)x); }
+// Single-precision exp and atan2 of the orientation / descriptor weights.
+// Plain IEEE float operations with explicit fmaf only, so the CPU oracle
+// reproduces every bit.  exp: x = n ln2 + r (Cody-Waite, ln2 split 16+24
+// bits, n ln2_hi exact), e^r by its degree-7 Taylor polynomial (Horner),
+// times 2^n; <= 1 ulp on [-87, 88], 0 below -87.  atan2: t = min/max (IEEE
+// division), atan(t) = t P(t^2) with a degree-9 polynomial fitted to
+// atan(t)/t on [0, 1] (iteratively reweighted least squares on relative
+// error, coefficients rounded to float), then the octant fix-ups; <= 2 ulp.
+MSIFT_HD float exp_f(float x) {
+    if (x != x) return x;
+    if (x > 88.0f) return __builtin_huge_valf();
+    if (x < -87.0f) return 0.0f;
+    const float n = floorf(x * 0x1.715476p+0f + 0.5f);
+    float r = fmaf(n, -0x1.62e4p-1f, x);
+    r = fmaf(n, -0x1.7f7d1cp-20f, r);
+    float p = 0x1.a01a02p-13f;
+    p = fmaf(p, r, 0x1.6c16c2p-10f);
+    p = fmaf(p, r, 0x1.111112p-7f);
+    p = fmaf(p, r, 0x1.555556p-5f);
+    p = fmaf(p, r, 0x1.555556p-3f);
+    p = fmaf(p, r, 0.5f);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
+    union { float f; uint32_t u; } v;
+    v.u = (uint32_t)((int)n + 127) << 23;
+    return p * v.f;
+}
+
+MSIFT_HD float atan2_f(float y, float x) {
+    const float PI = 0x1.921fb6p+1f, PIO2 = 0x1.921fb6p+0f;
+    if (x != x || y != y) return x + y;
+    const bool ys = signbit(y), xs = signbit(x);
+    const float ax = fabsf(x), ay = fabsf(y);
+    float a;
+    if (ay == 0.0f) {
+        a = xs ? PI : 0.0f;
+    } else if (ax == 0.0f) {
+        a = PIO2;
+    } else if (isinf(ax) || isinf(ay)) {
+        if (isinf(ax) && isinf(ay)) a = xs ? 0x1.2d97c8p+1f : 0x1.921fb6p-1f;
+        else if (isinf(ax)) a = xs ? PI : 0.0f;
+        else a = PIO2;
+    } else {
+        const float t = (ay > ax ? ax : ay) / (ay > ax ? ay : ax);
+        const float s2 = t * t;
+        float p = -0x1.cbd234p-10f;
+        p = fmaf(p, s2, 0x1.5f85d2p-7f);
+        p = fmaf(p, s2, -0x1.f8b94ep-6f);
+        p = fmaf(p, s2, 0x1.d77a5cp-5f);
+        p = fmaf(p, s2, -0x1.572366p-4f);
+        p = fmaf(p, s2, 0x1.c02fd6p-4f);
+        p = fmaf(p, s2, -0x1.2415a4p-3f);
+        p = fmaf(p, s2, 0x1.999084p-3f);
+        p = fmaf(p, s2, -0x1.555534p-2f);
+        p = fmaf(p, s2, 1.0f);
+        a = t * p;
+        if (ay > ax) a = PIO2 - a;
+        if (xs) a = PI - a;
+    }
+    return ys ? -a : a;
+}
 
 // C fmodf(x, 360) for |x| < 1440; every subtraction is exact.
 MSIFT_HD float fmod360(float x) {

@@ -207,6 +207,15 @@ struct ExtOct {          // extrema geometry of one (image, octave)
     long long unit0;     // first (scale, row, segment) mask unit
 };
 
+// whole-wave DPP shifts: lane i receives lane i-1 (wave_shr:1) or lane i+1
+// (wave_shl:1); the lane with no source receives 0
+__device__ __forceinline__ float dpp_from_left(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float dpp_from_right(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x130, 0xf, 0xf, false));
+}
+
 __device__ __forceinline__ float max3f(float a, float b, float c) { return fmaxf(a, fmaxf(b, c)); }
 __device__ __forceinline__ float min3f(float a, float b, float c) { return fminf(a, fminf(b, c)); }
 
@@ -241,7 +250,7 @@ __device__ __forceinline__ void ext_strip(const float* __restrict__ D, const Ext
         auto push = [&](const float* v) {
 #pragma unroll
             for (int l = 0; l < 5; ++l) {
-                float xl = __shfl_up(v[l], 1), xr = __shfl_down(v[l], 1);
+                const float xl = dpp_from_left(v[l]), xr = dpp_from_right(v[l]);
                 a0[l] = a1[l]; a1[l] = a2[l]; a2[l] = max3f(xl, v[l], xr);
                 b0[l] = b1[l]; b1[l] = b2[l]; b2[l] = min3f(xl, v[l], xr);
                 c1[l] = c2[l]; c2[l] = v[l];
@@ -476,8 +485,10 @@ __device__ __forceinline__ int orient_keypoint(const float* __restrict__ G, int 
             int bin_i = ((int)bin_f) % 36;
             int bin_r = (bin_i + 1) % 36;
             float frac = bin_f - floorf(bin_f);
-            P[bin_i * 64 + lane] += mag * wt * (1.0f - frac);
-            P[bin_r * 64 + lane] += mag * wt * frac;
+            // two distinct bins: both reads issued before the stores
+            const float q0 = P[bin_i * 64 + lane], q1 = P[bin_r * 64 + lane];
+            P[bin_i * 64 + lane] = q0 + mag * wt * (1.0f - frac);
+            P[bin_r * 64 + lane] = q1 + mag * wt * frac;
         }
     }
     float hist[36];
@@ -581,15 +592,30 @@ __device__ __forceinline__ void describe_keypoint(const float* __restrict__ G, i
             int oi0w = ((oi0 % 8) + 8) % 8;
             int oi1w = ((oi0 + 1) % 8 + 8) % 8;
             float wy0 = 1.0f - dyb, wy1 = dyb, wx0 = 1.0f - dxb, wx1 = dxb, wo0 = 1.0f - df, wo1 = df;
+            // the 8 trilinear updates in the reference's order.  Targets
+            // coincide only through the clamps (x pair: xi0c == xi1c, y pair:
+            // yi0c == yi1c); all 8 reads are issued together and coinciding
+            // targets continue from the running sum, so every bin receives
+            // exactly the sequential chain of float additions
             float* h = P + lane;
-            h[(yi0c * 32 + xi0c * 8 + oi0w) * 64] += wy0 * wx0 * wo0 * wmag;
-            h[(yi0c * 32 + xi0c * 8 + oi1w) * 64] += wy0 * wx0 * wo1 * wmag;
-            h[(yi0c * 32 + xi1c * 8 + oi0w) * 64] += wy0 * wx1 * wo0 * wmag;
-            h[(yi0c * 32 + xi1c * 8 + oi1w) * 64] += wy0 * wx1 * wo1 * wmag;
-            h[(yi1c * 32 + xi0c * 8 + oi0w) * 64] += wy1 * wx0 * wo0 * wmag;
-            h[(yi1c * 32 + xi0c * 8 + oi1w) * 64] += wy1 * wx0 * wo1 * wmag;
-            h[(yi1c * 32 + xi1c * 8 + oi0w) * 64] += wy1 * wx1 * wo0 * wmag;
-            h[(yi1c * 32 + xi1c * 8 + oi1w) * 64] += wy1 * wx1 * wo1 * wmag;
+            const int t0 = (yi0c * 32 + xi0c * 8 + oi0w) * 64, t1 = (yi0c * 32 + xi0c * 8 + oi1w) * 64;
+            const int t2 = (yi0c * 32 + xi1c * 8 + oi0w) * 64, t3 = (yi0c * 32 + xi1c * 8 + oi1w) * 64;
+            const int t4 = (yi1c * 32 + xi0c * 8 + oi0w) * 64, t5 = (yi1c * 32 + xi0c * 8 + oi1w) * 64;
+            const int t6 = (yi1c * 32 + xi1c * 8 + oi0w) * 64, t7 = (yi1c * 32 + xi1c * 8 + oi1w) * 64;
+            const bool xd = xi0c == xi1c, yd = yi0c == yi1c;
+            const float p0 = h[t0], p1 = h[t1], p2 = h[t2], p3 = h[t3];
+            const float p4 = h[t4], p5 = h[t5], p6 = h[t6], p7 = h[t7];
+            const float r0 = p0 + wy0 * wx0 * wo0 * wmag;
+            const float r1 = p1 + wy0 * wx0 * wo1 * wmag;
+            const float r2 = (xd ? r0 : p2) + wy0 * wx1 * wo0 * wmag;
+            const float r3 = (xd ? r1 : p3) + wy0 * wx1 * wo1 * wmag;
+            const float r4 = (yd ? (xd ? r2 : r0) : p4) + wy1 * wx0 * wo0 * wmag;
+            const float r5 = (yd ? (xd ? r3 : r1) : p5) + wy1 * wx0 * wo1 * wmag;
+            const float r6 = (xd ? r4 : (yd ? r2 : p6)) + wy1 * wx1 * wo0 * wmag;
+            const float r7 = (xd ? r5 : (yd ? r3 : p7)) + wy1 * wx1 * wo1 * wmag;
+            // in-order stores: a coinciding target ends with its last value
+            h[t0] = r0; h[t1] = r1; h[t2] = r2; h[t3] = r3;
+            h[t4] = r4; h[t5] = r5; h[t6] = r6; h[t7] = r7;
         }
     }
     float v0 = 0.0f, v1 = 0.0f;

@@ -39,6 +39,43 @@ def test_exp_log2_atan2_sincos_accuracy():
         assert abs(s - math.sin(x)) <= 4e-16 and abs(c - math.cos(x)) <= 4e-16
 
 
+def _ulp_err(a, ref):
+    ref32 = np.abs(ref).astype(np.float32)
+    return np.abs(a.astype(np.float64) - ref) / np.spacing(ref32).astype(np.float64)
+
+
+def test_float_exp_within_one_ulp():
+    rng = np.random.default_rng(5)
+    x = np.concatenate([np.linspace(-87, 88, 1_000_001, dtype=np.float32),
+                        rng.uniform(-10, 0, 500_000).astype(np.float32),
+                        np.float32([0.0, -0.0, 1e-30, -1e-30, 88.0, -87.0])])
+    y = O.exp_f(x)
+    assert _ulp_err(y, np.exp(x.astype(np.float64))).max() <= 1.0
+    assert O.exp_f(np.float32([0.0]))[0] == 1.0
+    assert O.exp_f(np.float32([-87.5, -1e4]))[0] == 0.0
+    assert np.isinf(O.exp_f(np.float32([89.0]))[0])
+    # e^(a+b) = e^a e^b up to rounding: catches a wrong ln2 split or 2^n scale
+    a = rng.uniform(-20, 20, 10000).astype(np.float32)
+    assert np.allclose(O.exp_f(a) * O.exp_f(-a), 1.0, rtol=4e-7)
+
+
+def test_float_atan2_within_two_ulp_and_octants():
+    rng = np.random.default_rng(6)
+    n = 1_000_000
+    y = (rng.standard_normal(n) * rng.choice([1e-4, 1.0, 300.0], n)).astype(np.float32)
+    x = (rng.standard_normal(n) * rng.choice([1e-4, 1.0, 300.0], n)).astype(np.float32)
+    a = O.atan2_f(y, x)
+    ref = np.arctan2(y.astype(np.float64), x.astype(np.float64))
+    assert _ulp_err(a, ref).max() <= 2.0
+    # axes, signed zeros, the |x| == |y| diagonals and infinities
+    sy = np.float32([0.0, -0.0, 0.0, -0.0, 1.0, -1.0, 2.0, -2.0, 2.0, -2.0, np.inf, -np.inf, 1.0, np.inf])
+    sx = np.float32([1.0, 1.0, -1.0, -1.0, 0.0, 0.0, 2.0, 2.0, -2.0, -2.0, 1.0, 1.0, -np.inf, -np.inf])
+    got = O.atan2_f(sy, sx)
+    want = np.arctan2(sy.astype(np.float64), sx.astype(np.float64))
+    assert np.all(np.signbit(got) == np.signbit(want))
+    assert _ulp_err(got, want).max() <= 2.0
+
+
 # ---------------------------------------------------------------- pyramid --
 def test_taps_match_reference_formula():
     # core.py:561-567: radius int(4 sigma + 0.5); float32 exp taps normalised
