@@ -128,35 +128,34 @@ __global__ __launch_bounds__(64) void k_bthresh(const BImg* __restrict__ im, con
 }
 
 // ---------------------------------------------------------------- spline --
-// spline / unit kernels: blockIdx.y = image job, x strides over its pixels
+// spline / unit kernels: blockIdx.y = image job, x strides over its tiles
 __global__ __launch_bounds__(256) void k_bups_cols(const BImg* __restrict__ im, const PixJob* __restrict__ jobs,
                                                     const uint8_t* __restrict__ in, char* ws, UpsTaps h) {
     const BImg I = im[jobs[blockIdx.y].img];
-    const int n = I.H * I.W;
     double* out = (double*)(ws + I.pf0_off);
-    for (int y = blockIdx.x; y < I.H; y += gridDim.x)
-        for (int x = threadIdx.x; x < I.W; x += 256) out[(size_t)y * I.W + x] = ups_pf_col(in + I.in_off, I.W, I.H, x, y, h);
-    (void)n;
+    const int ntx = (I.W + kUpsColTW - 1) / kUpsColTW, nt = ntx * ((I.H + kUpsColTH - 1) / kUpsColTH);
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) ups_cols_tile(in + I.in_off, I.W, I.H, I.W, h, out, t % ntx, t / ntx);
 }
 
 __global__ __launch_bounds__(256) void k_bups_rows(const BImg* __restrict__ im, const PixJob* __restrict__ jobs, char* ws,
                                                     UpsTaps h) {
+    __shared__ double s[kUpsRowTW + 2 * kUpsK];
     const BImg I = im[jobs[blockIdx.y].img];
     const double* src = (const double*)(ws + I.pf0_off);
     double* out = (double*)(ws + I.pf1_off);
-    for (int y = blockIdx.x; y < I.H; y += gridDim.x)
-        for (int x = threadIdx.x; x < I.W; x += 256) out[(size_t)y * I.W + x] = ups_pf_row(src + (size_t)y * I.W, I.W, x, h);
+    const int ntx = (I.W + kUpsRowTW - 1) / kUpsRowTW, nt = ntx * I.H;
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) ups_rows_tile(src, I.W, h, out, t / ntx, t % ntx, s);
 }
 
 __global__ __launch_bounds__(256) void k_bups_interp(const BImg* __restrict__ im, const PixJob* __restrict__ jobs, char* ws) {
+    __shared__ double s[kUpsInSH * kUpsInSW];
     const BImg I = im[jobs[blockIdx.y].img];
     const double zy = (2 * I.H > 1) ? (double)(I.H - 1) / (double)(2 * I.H - 1) : 0.0;
     const double zx = (2 * I.W > 1) ? (double)(I.W - 1) / (double)(2 * I.W - 1) : 0.0;
     const double* cf = (const double*)(ws + I.pf1_off);
     float* out = (float*)(ws + I.base_off);
-    const int Wo = 2 * I.W;
-    for (int oy = blockIdx.x; oy < 2 * I.H; oy += gridDim.x)
-        for (int ox = threadIdx.x; ox < Wo; ox += 256) out[(size_t)oy * Wo + ox] = ups_interp(cf, I.H, I.W, zy, zx, oy, ox);
+    const int ntx = (2 * I.W + kUpsInTW - 1) / kUpsInTW, nt = ntx * ((2 * I.H + kUpsInTH - 1) / kUpsInTH);
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) ups_interp_tile(cf, I.H, I.W, zy, zx, out, t % ntx, t / ntx, s);
 }
 
 __global__ __launch_bounds__(256) void k_bunit(const BImg* __restrict__ im, const PixJob* __restrict__ jobs,

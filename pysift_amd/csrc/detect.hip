@@ -67,26 +67,24 @@ __global__ __launch_bounds__(64) void k_thresh(const int* __restrict__ hist, dou
 // A3: cubic B-spline x2 zoom (core.py:1274), mirror-boundary prefilter as a
 // 61-tap symmetric FIR in FP64, then 4x4 tensor-product interpolation.
 // ======================================================================
-__global__ __launch_bounds__(256) void k_ups_pf_cols(const uint8_t* __restrict__ img, int pitch,
-                                                      int H, int W, UpsTaps h,
-                                                      double* __restrict__ out) {
-    int x = blockIdx.x * 256 + threadIdx.x;
-    int y = blockIdx.y;
-    if (x < W) out[(size_t)y * W + x] = ups_pf_col(img, pitch, H, x, y, h);
-}
-
-__global__ __launch_bounds__(256) void k_ups_pf_rows(const double* __restrict__ in, int H, int W,
+__global__ __launch_bounds__(256) void k_ups_pf_cols(const uint8_t* __restrict__ img, int pitch, int H, int W,
                                                       UpsTaps h, double* __restrict__ out) {
-    int x = blockIdx.x * 256 + threadIdx.x;
-    int y = blockIdx.y;
-    if (x < W) out[(size_t)y * W + x] = ups_pf_row(in + (size_t)y * W, W, x, h);
+    const int ntx = (W + kUpsColTW - 1) / kUpsColTW, nt = ntx * ((H + kUpsColTH - 1) / kUpsColTH);
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) ups_cols_tile(img, pitch, H, W, h, out, t % ntx, t / ntx);
 }
 
-__global__ __launch_bounds__(256) void k_ups_interp(const double* __restrict__ cf, int H, int W,
-                                                     double zy, double zx, float* __restrict__ out) {
-    int ox = blockIdx.x * 256 + threadIdx.x;
-    int oy = blockIdx.y;
-    if (ox < 2 * W) out[(size_t)oy * (2 * W) + ox] = ups_interp(cf, H, W, zy, zx, oy, ox);
+__global__ __launch_bounds__(256) void k_ups_pf_rows(const double* __restrict__ in, int H, int W, UpsTaps h,
+                                                      double* __restrict__ out) {
+    __shared__ double s[kUpsRowTW + 2 * kUpsK];
+    const int ntx = (W + kUpsRowTW - 1) / kUpsRowTW, nt = ntx * H;
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) ups_rows_tile(in, W, h, out, t / ntx, t % ntx, s);
+}
+
+__global__ __launch_bounds__(256) void k_ups_interp(const double* __restrict__ cf, int H, int W, double zy, double zx,
+                                                     float* __restrict__ out) {
+    __shared__ double s[kUpsInSH * kUpsInSW];
+    const int ntx = (2 * W + kUpsInTW - 1) / kUpsInTW, nt = ntx * ((2 * H + kUpsInTH - 1) / kUpsInTH);
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) ups_interp_tile(cf, H, W, zy, zx, out, t % ntx, t / ntx, s);
 }
 
 __global__ __launch_bounds__(256) void k_u8_unit(const uint8_t* __restrict__ img, int pitch, int H,
@@ -610,11 +608,14 @@ int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch
         double* pf0 = (double*)(ws + P.o_pf0);
         double* pf1 = (double*)(ws + P.o_pf1);
         float* up = (float*)(ws + P.o_unit);
-        hipLaunchKernelGGL(k_ups_pf_cols, dim3((W + 255) / 256, H), dim3(256), 0, s, img, pitch, H, W, h->ups, pf0);
-        hipLaunchKernelGGL(k_ups_pf_rows, dim3((W + 255) / 256, H), dim3(256), 0, s, pf0, H, W, h->ups, pf1);
+        const int nt_c = ((W + kUpsColTW - 1) / kUpsColTW) * ((H + kUpsColTH - 1) / kUpsColTH);
+        const int nt_r = ((W + kUpsRowTW - 1) / kUpsRowTW) * H;
+        hipLaunchKernelGGL(k_ups_pf_cols, dim3(grid_cap(nt_c, 8192)), dim3(256), 0, s, img, pitch, H, W, h->ups, pf0);
+        hipLaunchKernelGGL(k_ups_pf_rows, dim3(grid_cap(nt_r, 8192)), dim3(256), 0, s, pf0, H, W, h->ups, pf1);
         double zy = (2 * H > 1) ? (double)(H - 1) / (double)(2 * H - 1) : 0.0;
         double zx = (2 * W > 1) ? (double)(W - 1) / (double)(2 * W - 1) : 0.0;
-        hipLaunchKernelGGL(k_ups_interp, dim3((2 * W + 255) / 256, 2 * H), dim3(256), 0, s, pf1, H, W, zy, zx, up);
+        const int nt_i = ((2 * W + kUpsInTW - 1) / kUpsInTW) * ((2 * H + kUpsInTH - 1) / kUpsInTH);
+        hipLaunchKernelGGL(k_ups_interp, dim3(grid_cap(nt_i, 8192)), dim3(256), 0, s, pf1, H, W, zy, zx, up);
         if ((rc = blur(up, G0, nullptr, nullptr, 2 * H, 2 * W, h->sd0[1], h->taps_base[1], s))) return rc;
     } else {
         float* unit = (float*)(ws + P.o_unit);

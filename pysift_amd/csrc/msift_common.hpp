@@ -211,10 +211,13 @@ MSIFT_HD float atan2_f(float y, float x) {
     return ys ? -a : a;
 }
 
-// C fmodf(x, 360) for |x| < 1440; every subtraction is exact.
+// C fmodf(x, 360) for |x| < 1440 (every subtraction is exact): the same
+// subtractions as a loop while |x| >= 360, unrolled to three selects
 MSIFT_HD float fmod360(float x) {
     float a = fabsf(x);
-    while (a >= 360.0f) a = a - 360.0f;
+    a = a >= 360.0f ? a - 360.0f : a;
+    a = a >= 360.0f ? a - 360.0f : a;
+    a = a >= 360.0f ? a - 360.0f : a;
     return copysignf(a, x);
 }
 

@@ -83,6 +83,103 @@ __device__ __forceinline__ float ups_interp(const double* __restrict__ cf, int H
     return (float)v;
 }
 
+// tiled forms of the three spline kernels (whole block, 256 threads); every
+// output is the same fma chain as ups_pf_col / ups_pf_row / ups_interp
+constexpr int kUpsColRB = 8;              // column prefilter: rows per thread
+constexpr int kUpsColTW = 64, kUpsColTH = 4 * kUpsColRB;
+constexpr int kUpsRowTW = 256;            // row prefilter: outputs per block
+constexpr int kUpsInTW = 64, kUpsInTH = 16;   // interpolation output tile
+constexpr int kUpsInSW = 36, kUpsInSH = 12;   // its input tile (see ups_interp_tile)
+
+// column prefilter of a 64 x 32 tile: each thread walks kUpsColRB rows of one
+// column with the 8 + 60 input values (u8 / 255 as float, then double) held
+// in registers
+__device__ __forceinline__ void ups_cols_tile(const uint8_t* __restrict__ img, int pitch, int H, int W,
+                                              const UpsTaps& h, double* __restrict__ out, int tx, int ty) {
+    const int x = tx * kUpsColTW + (int)(threadIdx.x & 63);
+    const int y0 = ty * kUpsColTH + (int)(threadIdx.x >> 6) * kUpsColRB;
+    if (x >= W || y0 >= H) return;
+    double u[kUpsColRB + 2 * kUpsK];
+    const bool inner = y0 >= kUpsK && y0 + kUpsColRB - 1 + kUpsK < H;
+#pragma unroll
+    for (int i = 0; i < kUpsColRB + 2 * kUpsK; ++i) {
+        int yy = y0 - kUpsK + i;
+        if (!inner) yy = mirror_idx(yy, H);
+        u[i] = (double)((float)img[(size_t)yy * pitch + x] / 255.0f);
+    }
+#pragma unroll
+    for (int j = 0; j < kUpsColRB; ++j) {
+        if (y0 + j < H) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = -kUpsK; k <= kUpsK; ++k) acc = fma(h.h[k < 0 ? -k : k], u[j + k + kUpsK], acc);
+            out[(size_t)(y0 + j) * W + x] = acc;
+        }
+    }
+}
+
+// row prefilter of 256 outputs of row y: the mirrored segment is staged in
+// LDS s[kUpsRowTW + 2 kUpsK]; ends with a barrier (s reusable)
+__device__ __forceinline__ void ups_rows_tile(const double* __restrict__ in, int W, const UpsTaps& h,
+                                              double* __restrict__ out, int y, int tx, double* s) {
+    const double* row = in + (size_t)y * W;
+    const int x0 = tx * kUpsRowTW;
+    for (int i = threadIdx.x; i < kUpsRowTW + 2 * kUpsK; i += 256) s[i] = row[mirror_idx(x0 - kUpsK + i, W)];
+    __syncthreads();
+    const int x = x0 + (int)threadIdx.x;
+    if (x < W) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = -kUpsK; k <= kUpsK; ++k) acc = fma(h.h[k < 0 ? -k : k], s[threadIdx.x + kUpsK + k], acc);
+        out[(size_t)y * W + x] = acc;
+    }
+    __syncthreads();
+}
+
+// interpolation of a 64 x 16 output tile.  Its outputs read input rows
+// floor(oy zy) - 1 .. + 2 and columns floor(ox zx) - 1 .. + 2; with zy, zx
+// < 1/2 these span at most 12 rows and 36 columns from (cy0, cx0), staged
+// (mirrored) in LDS s[kUpsInSH * kUpsInSW]; ends with a barrier
+__device__ __forceinline__ void ups_interp_tile(const double* __restrict__ cf, int H, int W, double zy, double zx,
+                                                float* __restrict__ out, int tx, int ty, double* s) {
+    const int Wo = 2 * W, Ho = 2 * H;
+    const int ox0 = tx * kUpsInTW, oy0 = ty * kUpsInTH;
+    const int cy0 = (int)floor((double)oy0 * zy) - 1, cx0 = (int)floor((double)ox0 * zx) - 1;
+    for (int i = threadIdx.x; i < kUpsInSH * kUpsInSW; i += 256) {
+        const int r = i / kUpsInSW, c = i - r * kUpsInSW;
+        s[i] = cf[(size_t)mirror_idx(cy0 + r, H) * W + mirror_idx(cx0 + c, W)];
+    }
+    __syncthreads();
+    const int ox = ox0 + (int)(threadIdx.x & 63);
+    double xx = (double)ox * zx;
+    double fx = floor(xx);
+    const int ix = (int)fx;
+    double wx[4];
+    bspline_w(xx - fx, wx);
+#pragma unroll
+    for (int j = 0; j < kUpsInTH / 4; ++j) {
+        const int oy = oy0 + (int)(threadIdx.x >> 6) * (kUpsInTH / 4) + j;
+        if (ox < Wo && oy < Ho) {
+            double yy = (double)oy * zy;
+            double fy = floor(yy);
+            const int iy = (int)fy;
+            double wy[4];
+            bspline_w(yy - fy, wy);
+            double v = 0.0;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const double* srow = s + (iy - 1 + a - cy0) * kUpsInSW + (ix - 1 - cx0);
+                double rv = 0.0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) rv = fma(wx[b], srow[b], rv);
+                v = fma(wy[a], rv, v);
+            }
+            out[(size_t)oy * Wo + ox] = (float)v;
+        }
+    }
+    __syncthreads();
+}
+
 // ------------------------------------------------------------------ blur --
 // One 64 x TH output tile of G[s+1] = blur(G[s]) (H pass then V pass, fused
 // multiply-add chains over the taps in ascending order), D[s] = G[s+1] - G[s]
@@ -456,25 +553,30 @@ __device__ __forceinline__ int orient_keypoint(const float* __restrict__ G, int 
     // before any is used (memory-level parallelism); accumulation order per
     // lane stays p = lane, lane + 64, ...
     constexpr int kPf = 8;
+    const int q64 = 64 / n_cols, r64 = 64 - q64 * n_cols;
+    int rq = lane / n_cols, cq = lane - rq * n_cols;
     for (int p0 = lane; p0 < total; p0 += 64 * kPf) {
         float gl[kPf], gr[kPf], gu[kPf], gd[kPf];
+        int rk[kPf], ck[kPf];
 #pragma unroll
         for (int k = 0; k < kPf; ++k) {
             const int p = p0 + 64 * k;
+            rk[k] = r_min + rq; ck[k] = c_min + cq;
+            cq += r64; rq += q64;
+            if (cq >= n_cols) { cq -= n_cols; rq += 1; }
             gl[k] = gr[k] = gu[k] = gd[k] = 0.0f;
             if (p < total) {
-                int rq = p / n_cols;
-                const float* g = G + (size_t)(r_min + rq) * W + (c_min + (p - rq * n_cols));
+                const float* g = G + (size_t)rk[k] * W + ck[k];
                 gl[k] = g[-1]; gr[k] = g[1]; gu[k] = g[-W]; gd[k] = g[W];
             }
         }
+        // branch-free: pixels past the end add +0.0f (bins are sums of
+        // non-negative terms, x + 0 == x bitwise)
 #pragma unroll
         for (int k = 0; k < kPf; ++k) {
             const int p = p0 + 64 * k;
-            if (p >= total) break;
-            int rq = p / n_cols;
-            int r = r_min + rq;
-            int c = c_min + (p - rq * n_cols);
+            int r = rk[k];
+            int c = ck[k];
             float dx = gr[k] - gl[k];
             float dy = gd[k] - gu[k];
             float mag = sqrtf(dx * dx + dy * dy);
@@ -486,9 +588,11 @@ __device__ __forceinline__ int orient_keypoint(const float* __restrict__ G, int 
             int bin_r = (bin_i + 1) % 36;
             float frac = bin_f - floorf(bin_f);
             // two distinct bins: both reads issued before the stores
+            float w0 = mag * wt * (1.0f - frac), w1 = mag * wt * frac;
+            if (p >= total) { w0 = 0.0f; w1 = 0.0f; }
             const float q0 = P[bin_i * 64 + lane], q1 = P[bin_r * 64 + lane];
-            P[bin_i * 64 + lane] = q0 + mag * wt * (1.0f - frac);
-            P[bin_r * 64 + lane] = q1 + mag * wt * frac;
+            P[bin_i * 64 + lane] = q0 + w0;
+            P[bin_r * 64 + lane] = q1 + w1;
         }
     }
     float hist[36];
@@ -544,15 +648,20 @@ __device__ __forceinline__ void describe_keypoint(const float* __restrict__ G, i
     const int iri = (int)ri, ici = (int)ci;
     // chunks of kPf pixels per lane, taps loaded before use (see orientation)
     constexpr int kPf = 8;
+    // (row, col) of flat = lane + 64 i in the side x side patch, stepped by
+    // 64 = q64 rows + r64 columns without a division per pixel
+    const int q64 = 64 / side, r64 = 64 - q64 * side;
+    int rq = lane / side, cq = lane - rq * side;
     for (int f0 = lane; f0 < total_px; f0 += 64 * kPf) {
         float gl[kPf], gr[kPf], gu[kPf], gd[kPf], xrs[kPf], yrs[kPf];
         bool okk[kPf];
 #pragma unroll
         for (int k = 0; k < kPf; ++k) {
             const int flat = f0 + 64 * k;
-            int rq = flat / side;
             int r_off = rq - ph;
-            int c_off = (flat - rq * side) - ph;
+            int c_off = cq - ph;
+            cq += r64; rq += q64;
+            if (cq >= side) { cq -= side; rq += 1; }
             int rr = iri + r_off, cc = ici + c_off;
             float xr = cos_a * (float)c_off - sin_a * (float)r_off;
             float yr = sin_a * (float)c_off + cos_a * (float)r_off;
@@ -568,9 +677,12 @@ __device__ __forceinline__ void describe_keypoint(const float* __restrict__ G, i
                 gl[k] = g[-1]; gr[k] = g[1]; gu[k] = g[-W]; gd[k] = g[W];
             }
         }
+        // branch-free: a pixel outside the patch / window adds +0.0f to
+        // in-range bins (every bin holds a sum of products of non-negative
+        // factors, so x + 0 == x bitwise), which keeps the 8 pixels of a
+        // chunk in one basic block for the scheduler
 #pragma unroll
         for (int k = 0; k < kPf; ++k) {
-            if (!okk[k]) continue;
             const float xr = xrs[k], yr = yrs[k];
             float xb = xr * scale_sp + 1.5f;
             float yb = yr * scale_sp + 1.5f;
@@ -579,7 +691,7 @@ __device__ __forceinline__ void describe_keypoint(const float* __restrict__ G, i
             float mag = sqrtf(dx * dx + dy * dy);
             float ori_deg = fmod360((atan2_f(dy, dx) - ang_rad) * 57.29577951f + 360.0f);
             float gw = exp_f(-(xr * xr + yr * yr) * inv_2s2);
-            float wmag = mag * gw;
+            float wmag = okk[k] ? mag * gw : 0.0f;
             float ob = ori_deg * 0.02222222f;
             int oi0 = (int)floorf(ob);
             float df = ob - (float)oi0;
