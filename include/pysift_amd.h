@@ -98,6 +98,29 @@ int msift_detect_describe_batched(msift_batch* plan, const uint8_t* d_packed, vo
                                   float* d_kp, float* d_desc_f32, uint8_t* d_desc_u8, int* d_counts, int* d_offsets,
                                   void* stream);
 
+/* Differentiable descriptor (no reference counterpart: the reference has no
+ * gradient path; the forward is its pyramid, core.py:1234-1352, and
+ * descriptor, core.py:693-835 + RootSIFT core.py:1870-1873).
+ * B float32 images H x W in [0, 1] (row-major, back to back), single-size
+ * base (the handle must have double_image = 0).  d_kps[B*K*5] = (octave,
+ * scale index 1..S, row, col in octave pixels, angle in degrees) per slot;
+ * an octave < 0 marks an empty slot.  Forward: d_desc[B*K*128].  Backward:
+ * given d_gdesc = dL/d(desc) it recomputes the pyramid and writes
+ * d_gimg[B*H*W] = dL/d(image); bitwise reproducible (fixed-point integer
+ * accumulation of the per-keypoint scatters).  Workspace from
+ * msift_grad_workspace_bytes (both calls). */
+int msift_grad_workspace_bytes(msift_handle* h, int B, int H, int W, size_t* bytes);
+int msift_desc_forward(msift_handle* h, const float* d_imgs, int B, int H, int W, const float* d_kps, int K,
+                       void* d_ws, size_t ws_bytes, float* d_desc, void* stream);
+int msift_desc_backward(msift_handle* h, const float* d_imgs, int B, int H, int W, const float* d_kps, int K,
+                        const float* d_gdesc, void* d_ws, size_t ws_bytes, float* d_gimg, void* stream);
+/* Level access for tests, inside a workspace filled by msift_desc_backward:
+ * image b, octave o: Gaussian levels 0..S (float, Ho x Wo each) and the
+ * descriptor stage's gradient of levels 1..S before the blur adjoints (int64,
+ * fixed point: value * 2^32). */
+int msift_grad_level_view(msift_handle* h, int B, int H, int W, void* d_ws, int b, int o, int* Ho, int* Wo,
+                          float** d_gauss, long long** d_grad_fixed);
+
 /* Octave-level access for tests: Gaussian (S+3) and DoG (S+2) stacks of
  * octave `o` inside a workspace filled by msift_detect_describe. */
 int msift_pyramid_view(msift_handle* h, int H, int W, void* d_ws, int o,

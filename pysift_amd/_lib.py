@@ -21,7 +21,8 @@ EXPORTS = [
     "msift_detect_describe", "msift_pyramid_view", "msift_match_workspace_bytes", "msift_match_u8",
     "msift_match_f16", "msift_ransac_workspace_bytes", "msift_ransac_homography", "msift_warp_feather",
     "msift_bgr_to_gray", "msift_batch_create", "msift_batch_destroy", "msift_batch_sizes",
-    "msift_batch_set_blur_tile", "msift_detect_describe_batched",
+    "msift_batch_set_blur_tile", "msift_detect_describe_batched", "msift_grad_workspace_bytes",
+    "msift_desc_forward", "msift_desc_backward", "msift_grad_level_view",
 ]
 
 
@@ -76,6 +77,15 @@ def _declare(L):
     L.msift_batch_set_blur_tile.restype = _I; L.msift_batch_set_blur_tile.argtypes = [_P, _I]
     L.msift_detect_describe_batched.restype = _I
     L.msift_detect_describe_batched.argtypes = [_P, _P, _P, _SZ, _I, _P, _P, _P, _P, _P, _P]
+    L.msift_grad_workspace_bytes.restype = _I
+    L.msift_grad_workspace_bytes.argtypes = [_P, _I, _I, _I, ctypes.POINTER(_SZ)]
+    L.msift_desc_forward.restype = _I
+    L.msift_desc_forward.argtypes = [_P, _P, _I, _I, _I, _P, _I, _P, _SZ, _P, _P]
+    L.msift_desc_backward.restype = _I
+    L.msift_desc_backward.argtypes = [_P, _P, _I, _I, _I, _P, _I, _P, _P, _SZ, _P, _P]
+    L.msift_grad_level_view.restype = _I
+    L.msift_grad_level_view.argtypes = [_P, _I, _I, _I, _P, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                        ctypes.POINTER(_P), ctypes.POINTER(_P)]
 
 
 def lib():

@@ -97,7 +97,12 @@ __global__ __launch_bounds__(256) void k_u8_unit(const uint8_t* __restrict__ img
 // a level whose sigma is below 0.01 equals its input (core.py:559-560)
 __global__ __launch_bounds__(256) void k_copy_level(const float* __restrict__ in, float* __restrict__ gout,
                                                      float* __restrict__ dout, float* __restrict__ dec, int H, int W,
-                                                     int decW) {
+                                                     int decW, size_t zin, size_t zout) {
+    const size_t z = blockIdx.z;
+    in += z * zin;
+    if (gout) gout += z * zout;
+    if (dout) dout += z * zout;
+    if (dec) dec += z * zout;
     size_t n = (size_t)H * W;
     size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
@@ -115,46 +120,48 @@ __global__ __launch_bounds__(256) void k_copy_level(const float* __restrict__ in
 // (core.py:146), and optionally the next octave's base G[S][::2, ::2].
 // A 64 x TH output tile is staged through LDS with its reflected halo.
 // ======================================================================
+// gridDim.z images of one size: pointers advance by zin (input) and zout
+// (outputs) floats per image
 template <int R, int TH>
 __global__ __launch_bounds__(256) void k_blur(const float* __restrict__ in, float* __restrict__ gout,
                                                float* __restrict__ dout, float* __restrict__ dec,
-                                               int H, int W, int decW, Taps taps) {
+                                               int H, int W, int decW, Taps taps, size_t zin, size_t zout) {
     __shared__ __attribute__((aligned(16))) float sIn[BlurSmem<R, TH>::nIn];
     __shared__ __attribute__((aligned(16))) float sH[BlurSmem<R, TH>::nH];
-    blur_tile<R, TH>(in, gout, dout, dec, H, W, decW, taps, blockIdx.x, blockIdx.y, sIn, sH);
+    const size_t z = blockIdx.z;
+    blur_tile<R, TH>(in + z * zin, gout ? gout + z * zout : nullptr, dout ? dout + z * zout : nullptr,
+                     dec ? dec + z * zout : nullptr, H, W, decW, taps, blockIdx.x, blockIdx.y, sIn, sH);
 }
-
-typedef void (*blur_fn)(const float*, float*, float*, float*, int, int, int, Taps);
 
 template <int R>
 static void blur_launch(const float* in, float* gout, float* dout, float* dec, int H, int W, int decW,
-                        const Taps& t, hipStream_t s) {
+                        const Taps& t, hipStream_t s, int nb, size_t zin, size_t zout) {
     // tile height keeps LDS under 64 KB: 64 rows up to R=12, 32 up to R=24, else 16
     if (R <= 12) {
-        dim3 grid((W + 63) / 64, (H + 63) / 64);
-        hipLaunchKernelGGL((k_blur<R, 64>), grid, dim3(256), 0, s, in, gout, dout, dec, H, W, decW, t);
+        dim3 grid((W + 63) / 64, (H + 63) / 64, nb);
+        hipLaunchKernelGGL((k_blur<R, 64>), grid, dim3(256), 0, s, in, gout, dout, dec, H, W, decW, t, zin, zout);
     } else if (R <= 24) {
-        dim3 grid((W + 63) / 64, (H + 31) / 32);
+        dim3 grid((W + 63) / 64, (H + 31) / 32, nb);
         hipLaunchKernelGGL((k_blur<R, (R <= 24 ? 32 : 16)>), grid, dim3(256), 0, s, in, gout, dout, dec, H, W,
-                           decW, t);
+                           decW, t, zin, zout);
     } else {
-        dim3 grid((W + 63) / 64, (H + 15) / 16);
-        hipLaunchKernelGGL((k_blur<R, 16>), grid, dim3(256), 0, s, in, gout, dout, dec, H, W, decW, t);
+        dim3 grid((W + 63) / 64, (H + 15) / 16, nb);
+        hipLaunchKernelGGL((k_blur<R, 16>), grid, dim3(256), 0, s, in, gout, dout, dec, H, W, decW, t, zin, zout);
     }
 }
 
-static int blur(const float* in, float* gout, float* dout, float* dec, int H, int W, double sigma,
-                const Taps& t, hipStream_t s) {
+int blur(const float* in, float* gout, float* dout, float* dec, int H, int W, double sigma, const Taps& t,
+         hipStream_t s, int nb, size_t zin, size_t zout) {
     int decW = (W + 1) / 2;
     if (sigma < 0.01) {
         // _fast_gaussian returns its input unchanged (core.py:559-560)
         size_t n = (size_t)H * W;
-        hipLaunchKernelGGL(k_copy_level, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, gout, dout, dec, H,
-                           W, decW);
+        hipLaunchKernelGGL(k_copy_level, dim3((unsigned)((n + 255) / 256), 1, nb), dim3(256), 0, s, in, gout, dout,
+                           dec, H, W, decW, zin, zout);
         return MSIFT_OK;
     }
     switch (t.r) {
-#define MSIFT_BLUR_CASE(RR) case RR: blur_launch<RR>(in, gout, dout, dec, H, W, decW, t, s); break;
+#define MSIFT_BLUR_CASE(RR) case RR: blur_launch<RR>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout); break;
         MSIFT_BLUR_CASE(0) MSIFT_BLUR_CASE(1) MSIFT_BLUR_CASE(2) MSIFT_BLUR_CASE(3) MSIFT_BLUR_CASE(4)
         MSIFT_BLUR_CASE(5) MSIFT_BLUR_CASE(6) MSIFT_BLUR_CASE(7) MSIFT_BLUR_CASE(8) MSIFT_BLUR_CASE(9)
         MSIFT_BLUR_CASE(10) MSIFT_BLUR_CASE(11) MSIFT_BLUR_CASE(12) MSIFT_BLUR_CASE(13) MSIFT_BLUR_CASE(14)

@@ -629,11 +629,10 @@ __device__ __forceinline__ int orient_keypoint(const float* __restrict__ G, int 
 }
 
 // ------------------------------------------------------------ descriptor --
-// sift_descriptor_coop (core.py:693-835) + RootSIFT (core.py:1870-1873) + u8
-// for one keypoint on one wave; lane l holds bins 2l, 2l+1 on return.
-__device__ __forceinline__ void describe_keypoint(const float* __restrict__ G, int H, int W, float ri, float ci,
-                                                  float angle_deg, int ph, int rootsift, float* P, int lane,
-                                                  float* v0o, float* v1o) {
+// raw 4x4x8 histogram of sift_descriptor_coop (core.py:693-792) for one
+// keypoint on one wave; lane l returns bins 2l, 2l+1
+__device__ __forceinline__ void desc_hist(const float* __restrict__ G, int H, int W, float ri, float ci,
+                                          float angle_deg, int ph, float* P, int lane, float* h0o, float* h1o) {
 #pragma unroll
     for (int b = 0; b < 128; ++b) P[b * 64 + lane] = 0.0f;
     const float ang_rad = angle_deg * 0.0174532924f;
@@ -736,6 +735,13 @@ __device__ __forceinline__ void describe_keypoint(const float* __restrict__ G, i
         float t = wave_tree_f(P[b * 64 + lane]);
         if ((b >> 1) == lane) { if (b & 1) v1 = t; else v0 = t; }
     }
+    *h0o = v0;
+    *h1o = v1;
+}
+
+// L2 -> clip 0.2 -> L2 (core.py:793-834) and RootSIFT (core.py:1870-1873)
+// of a histogram held as (bins 2l, 2l+1) in lane l
+__device__ __forceinline__ void desc_normalize(float v0, float v1, int rootsift, float* v0o, float* v1o) {
     float inv = 1.0f / sqrtf(wave_tree_f(v0 * v0 + v1 * v1) + 1e-14f);
     float t0 = v0 * inv, t1 = v1 * inv;
     v0 = t0 > 0.2f ? 0.2f : t0;
@@ -751,6 +757,16 @@ __device__ __forceinline__ void describe_keypoint(const float* __restrict__ G, i
     }
     *v0o = v0;
     *v1o = v1;
+}
+
+// sift_descriptor_coop (core.py:693-835) + RootSIFT for one keypoint on one
+// wave; lane l holds bins 2l, 2l+1 on return
+__device__ __forceinline__ void describe_keypoint(const float* __restrict__ G, int H, int W, float ri, float ci,
+                                                  float angle_deg, int ph, int rootsift, float* P, int lane,
+                                                  float* v0o, float* v1o) {
+    float h0, h1;
+    desc_hist(G, H, W, ri, ci, angle_deg, ph, P, lane, &h0, &h1);
+    desc_normalize(h0, h1, rootsift, v0o, v1o);
 }
 
 __device__ __forceinline__ void store_descriptor(float v0, float v1, int lane, float* out_f, uint8_t* out_u8) {

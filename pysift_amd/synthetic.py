@@ -142,3 +142,27 @@ def corner_error(H_est, H_gt, w, h):
     a = H_est @ c; a = a[:2] / a[2]
     b = H_gt @ c; b = b[:2] / b[2]
     return float(np.mean(np.linalg.norm(a - b, axis=0)))
+
+
+def synth_float_batch(B=16, H=512, W=512, seed=3000):
+    """(B, H, W) float32 images in [0, 1]: the synthetic content of synth_gray
+    (seed + image index), quantised, then scaled by 1/255."""
+    return np.stack([synth_gray(H, W, seed + b).astype(np.float32) / np.float32(255.0) for b in range(B)])
+
+
+def random_keypoints(B, K, H, W, n_oct=5, S=3, seed=3100, margin=2, max_octave=None):
+    """(B, K, 5) float32 fixed keypoints (octave, si, row, col, angle): octave
+    uniform in [0, min(n_oct, max_octave + 1)), si in [1, S], position uniform
+    over the octave image inside `margin`, angle uniform in [0, 360)."""
+    rng = np.random.default_rng(seed)
+    top = n_oct if max_octave is None else min(n_oct, max_octave + 1)
+    out = np.empty((B, K, 5), np.float32)
+    o = rng.integers(0, top, (B, K))
+    Ho = np.ceil(H / 2.0 ** o)
+    Wo = np.ceil(W / 2.0 ** o)
+    out[..., 0] = o
+    out[..., 1] = rng.integers(1, S + 1, (B, K))
+    out[..., 2] = margin + rng.random((B, K)) * np.maximum(Ho - 2 * margin - 1, 1)
+    out[..., 3] = margin + rng.random((B, K)) * np.maximum(Wo - 2 * margin - 1, 1)
+    out[..., 4] = rng.random((B, K)) * 360.0
+    return out
