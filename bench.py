@@ -133,6 +133,43 @@ def bench_batch(torch, dev, n=64, iters=5):
             "keypoints": int(plan.counts.sum().item())}
 
 
+def bench_grad(torch, dev, B=16, H=512, W=512, K=1024, iters=5):
+    """16 float 512x512 images, 1024 fixed keypoints each, loss sum(W * desc):
+    HIP forward and hand-written backward (pyramid recomputed), and one
+    torch.autograd step through them."""
+    import numpy as np
+    from pysift_amd import PySIFT
+    from pysift_amd import synthetic as syn
+    from pysift_amd.autograd import sift_descriptors
+    s = PySIFT(double_image=False, device=dev)
+    x = torch.from_numpy(syn.synth_float_batch(B, H, W, seed=3000)).to(dev)
+    kps = torch.from_numpy(syn.random_keypoints(B, K, H, W, n_oct=5, seed=3100)).to(dev)
+    w = torch.from_numpy(np.random.default_rng(3200).standard_normal((B, K, 128)).astype(np.float32)).to(dev)
+
+    def step():
+        xx = x.clone().requires_grad_(True)
+        (sift_descriptors(xx, kps, s) * w).sum().backward()
+        return xx.grad
+
+    step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(iters):
+        d = sift_descriptors(x, kps, s)
+    e1.record()
+    torch.cuda.synchronize()
+    t_fwd = e0.elapsed_time(e1) / iters
+    e0.record()
+    for _ in range(iters):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    t_step = e0.elapsed_time(e1) / iters
+    return {"images": B, "hw": [H, W], "keypoints_per_image": K, "forward_ms": round(t_fwd, 3),
+            "forward_backward_ms": round(t_step, 3), "backward_ms": round(t_step - t_fwd, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,7 +181,7 @@ def main():
                     help="(counter calibration) one 256 MiB device-to-device copy, then exit")
     ap.add_argument("--match-n", type=int, default=40000, help="descriptors per side for the match TOPS leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the match / stitch legs")
-    ap.add_argument("--only", choices=["match", "batch", "stitch"], default=None,
+    ap.add_argument("--only", choices=["match", "batch", "stitch", "grad"], default=None,
                     help="(profiling) run a single extra leg and print its numbers")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -173,6 +210,8 @@ def main():
             print(json.dumps({m: bench_match(torch, dev, args.match_n, m) for m in ("hilo", "shift")}))
         elif args.only == "batch":
             print(json.dumps(bench_batch(torch, dev)))
+        elif args.only == "grad":
+            print(json.dumps(bench_grad(torch, dev)))
         else:
             print(json.dumps(bench_stitch(va, vb)))
         return
@@ -239,6 +278,7 @@ def main():
             extra[f"match_{method}"] = {"n": args.match_n, "tops": round(tops, 2), "ms": round(ms, 4),
                                         "mfma_int8_peak_tops": PEAK_INT8_TOPS, "pairs": npairs}
         extra["batch64"] = bench_batch(torch, dev)
+        extra["grad16x512"] = bench_grad(torch, dev)
         st_ms, met = bench_stitch(va, vb)
         extra["stitch_pair_ms"] = round(st_ms, 3)
         extra["stitch_metrics"] = {k: met[k] for k in ("kp1", "kp2", "matches", "inliers")}
