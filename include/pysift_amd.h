@@ -139,6 +139,28 @@ int msift_match_u8(const uint8_t* d_a, int N, const uint8_t* d_b, int M, double 
                    void* d_ws, size_t ws_bytes, int32_t* d_fwd, int32_t* d_bwd,
                    int32_t* d_pairs, int* d_npairs, void* stream);
 
+/* Batched matcher (no reference counterpart: the reference matches one pair
+ * per call, core.py:2840-2940): n_pairs descriptor-set pairs (a_p: N[p] rows,
+ * b_p: M[p] rows of 128 B), packed back to back at the row offsets that
+ * msift_match_batch_sizes reports.  One launch computes every tile of every
+ * pair; per pair the outputs are exactly msift_match_u8's:
+ *   d_fwd[3 * (a_off + i)], d_bwd[3 * (b_off + j)],
+ *   d_pairs[2 * (a_off + k)] for k < d_npairs[p] (room for N[p] pairs).
+ * The plan (device pair table) is the only allocation; run is capturable. */
+typedef struct msift_match_batch msift_match_batch;
+int msift_match_batch_create(int n_pairs, const int* N, const int* M, msift_match_batch** plan);
+int msift_match_batch_destroy(msift_match_batch* plan);
+int msift_match_batch_sizes(msift_match_batch* plan, size_t* ws_bytes, int* a_offsets, int* b_offsets);
+int msift_match_u8_batched(msift_match_batch* plan, const uint8_t* d_a, const uint8_t* d_b, double ratio, int method,
+                           void* d_ws, size_t ws_bytes, int32_t* d_fwd, int32_t* d_bwd, int32_t* d_pairs,
+                           int* d_npairs, void* stream);
+/* Matched point coordinates for batched RANSAC: keypoint rows (x, y, ...) of
+ * stride_a / stride_b floats in the same packed order as the descriptors;
+ * d_p1/d_p2[2 * (a_off + k)] = (x, y) of match k of pair p (doubles). */
+int msift_match_batch_gather_points(msift_match_batch* plan, const float* d_kpa, int stride_a, const float* d_kpb,
+                                    int stride_b, const int32_t* d_pairs, const int* d_npairs, double* d_p1,
+                                    double* d_p2, void* stream);
+
 /* Float descriptor path: d2 = sum (a-b)^2 via FP16 MFMA with FP32
  * accumulation of a.b on fp16-rounded inputs; same outputs (distances as
  * float bits in the int32 slots). */
@@ -156,6 +178,22 @@ int msift_ransac_homography(const double* d_p1, const double* d_p2, int N, uint6
                             double thr_px, double sigma_max, int method, int lo, int min_inliers,
                             void* d_ws, size_t ws_bytes, double* d_H, uint8_t* d_mask,
                             int* d_ninl, int* d_best, void* stream);
+
+/* Batched RANSAC (no reference counterpart: _ransac runs per pair,
+ * core.py:3011-3097): n_pairs point sets packed back to back, pair p at the
+ * point offset msift_ransac_batch_sizes reports with room for cap[p] points;
+ * the counts d_npts[p] are read on the device (pairs below 4 points get
+ * H = 0, no inliers, best = -1).  Pair p uses key seed + p, so it equals
+ * msift_ransac_homography with seed + p.  Outputs d_H[9 * p], d_mask at the
+ * pair's point offset, d_ninl[p], d_best[p]. */
+typedef struct msift_ransac_batch msift_ransac_batch;
+int msift_ransac_batch_create(int n_pairs, const int* cap, int n_hyp, msift_ransac_batch** plan);
+int msift_ransac_batch_destroy(msift_ransac_batch* plan);
+int msift_ransac_batch_sizes(msift_ransac_batch* plan, size_t* ws_bytes, long long* point_offsets);
+int msift_ransac_homography_batched(msift_ransac_batch* plan, const double* d_p1, const double* d_p2,
+                                    const int* d_npts, uint64_t seed, double thr_px, double sigma_max, int method,
+                                    int lo, int min_inliers, void* d_ws, size_t ws_bytes, double* d_H,
+                                    uint8_t* d_mask, int* d_ninl, int* d_best, void* stream);
 
 /* _warp_image + compositing (core.py:3210-3239, 3794-3802) with a feather
  * blend: img1 (BGR u8, h1 x w1) warped by H into a cw x ch canvas,

@@ -22,7 +22,10 @@ EXPORTS = [
     "msift_match_f16", "msift_ransac_workspace_bytes", "msift_ransac_homography", "msift_warp_feather",
     "msift_bgr_to_gray", "msift_batch_create", "msift_batch_destroy", "msift_batch_sizes",
     "msift_batch_set_blur_tile", "msift_detect_describe_batched", "msift_grad_workspace_bytes",
-    "msift_desc_forward", "msift_desc_backward", "msift_grad_level_view",
+    "msift_desc_forward", "msift_desc_backward", "msift_grad_level_view", "msift_match_batch_create",
+    "msift_match_batch_destroy", "msift_match_batch_sizes", "msift_match_u8_batched",
+    "msift_match_batch_gather_points", "msift_ransac_batch_create", "msift_ransac_batch_destroy",
+    "msift_ransac_batch_sizes", "msift_ransac_homography_batched",
 ]
 
 
@@ -83,6 +86,21 @@ def _declare(L):
     L.msift_desc_forward.argtypes = [_P, _P, _I, _I, _I, _P, _I, _P, _SZ, _P, _P]
     L.msift_desc_backward.restype = _I
     L.msift_desc_backward.argtypes = [_P, _P, _I, _I, _I, _P, _I, _P, _P, _SZ, _P, _P]
+    L.msift_match_batch_create.restype = _I
+    L.msift_match_batch_create.argtypes = [_I, _P, _P, ctypes.POINTER(_P)]
+    L.msift_match_batch_destroy.restype = _I; L.msift_match_batch_destroy.argtypes = [_P]
+    L.msift_match_batch_sizes.restype = _I; L.msift_match_batch_sizes.argtypes = [_P, _P, _P, _P]
+    L.msift_match_u8_batched.restype = _I
+    L.msift_match_u8_batched.argtypes = [_P, _P, _P, _D, _I, _P, _SZ, _P, _P, _P, _P, _P]
+    L.msift_match_batch_gather_points.restype = _I
+    L.msift_match_batch_gather_points.argtypes = [_P, _P, _I, _P, _I, _P, _P, _P, _P, _P]
+    L.msift_ransac_batch_create.restype = _I
+    L.msift_ransac_batch_create.argtypes = [_I, _P, _I, ctypes.POINTER(_P)]
+    L.msift_ransac_batch_destroy.restype = _I; L.msift_ransac_batch_destroy.argtypes = [_P]
+    L.msift_ransac_batch_sizes.restype = _I; L.msift_ransac_batch_sizes.argtypes = [_P, _P, _P]
+    L.msift_ransac_homography_batched.restype = _I
+    L.msift_ransac_homography_batched.argtypes = [_P, _P, _P, _P, ctypes.c_uint64, _D, _D, _I, _I, _I, _P, _SZ,
+                                                  _P, _P, _P, _P, _P]
     L.msift_grad_level_view.restype = _I
     L.msift_grad_level_view.argtypes = [_P, _I, _I, _I, _P, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I),
                                         ctypes.POINTER(_P), ctypes.POINTER(_P)]

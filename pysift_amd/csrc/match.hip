@@ -12,6 +12,9 @@
 // (d << bits | local index), which are unique, so the result does not depend
 // on evaluation order; small reduce kernels merge the per-chunk partials.
 #include "msift_internal.hpp"
+#include <algorithm>
+#include <cstring>
+#include <vector>
 
 #pragma clang fp contract(off)
 
@@ -56,15 +59,15 @@ __global__ __launch_bounds__(256) void k_norms(const uint8_t* __restrict__ a, in
     if (lane == 0) out[i] = s;
 }
 
+// one 256-row x 512-column tile (row block rb, column chunk cc) of a
+// workgroup of 8 waves; colk / s_na are the workgroup's LDS
 template <int METHOD>
-__global__ __launch_bounds__(512) void k_match_tiles(const uint8_t* __restrict__ A, int N, const uint8_t* __restrict__ B,
-                                                      int M, const int* __restrict__ na, const int* __restrict__ nb,
-                                                      uint32_t* __restrict__ rowpart, uint32_t* __restrict__ colpart) {
-    __shared__ uint32_t colk[8][64][2];
-    __shared__ int s_na[kRowBlk];
+__device__ __forceinline__ void match_tile(const uint8_t* __restrict__ A, int N, const uint8_t* __restrict__ B, int M,
+                                           const int* __restrict__ na, const int* __restrict__ nb,
+                                           uint32_t* __restrict__ rowpart, uint32_t* __restrict__ colpart, int rb,
+                                           int cc, uint32_t (*colk)[64][2], int* s_na) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int l32 = lane & 31, hi = lane >> 5;
-    const int rb = blockIdx.x, cc = blockIdx.y;
     const int row0 = rb * kRowBlk + w * 32;     // this wave's 32 rows
     if (threadIdx.x < kRowBlk) {
         int i = rb * kRowBlk + threadIdx.x;
@@ -178,11 +181,60 @@ __global__ __launch_bounds__(512) void k_match_tiles(const uint8_t* __restrict__
     }
 }
 
+template <int METHOD>
+__global__ __launch_bounds__(512) void k_match_tiles(const uint8_t* __restrict__ A, int N, const uint8_t* __restrict__ B,
+                                                      int M, const int* __restrict__ na, const int* __restrict__ nb,
+                                                      uint32_t* __restrict__ rowpart, uint32_t* __restrict__ colpart) {
+    __shared__ uint32_t colk[8][64][2];
+    __shared__ int s_na[kRowBlk];
+    match_tile<METHOD>(A, N, B, M, na, nb, rowpart, colpart, blockIdx.x, blockIdx.y, colk, s_na);
+}
+
+// ------------------------------------------------------------- batched ----
+// Pairs (a_p, b_p) of descriptor sets packed back to back; one launch covers
+// every tile of every pair, a block finds its pair in the table by a ballot
+// over the pairs' first tile.
+struct MPair {
+    int a_off, N, b_off, M;   // rows of a / b in the packed sets
+    int nrb, ncc, tile0;      // row blocks, column chunks, first tile
+    int blk0, nblk;           // first 256-row block of the mutual pass, its count
+    size_t o_rowpart, o_colpart;   // workspace offsets of the partial top-2 keys
+};
+
+__device__ __forceinline__ int pair_of_tile(const MPair* __restrict__ pairs, int np, int t) {
+    int last = 0;
+    for (int b0 = 0; b0 < np; b0 += 64) {
+        const int k = b0 + (int)(threadIdx.x & 63);
+        const bool le = k < np && pairs[k].tile0 <= t;
+        const unsigned long long m = __ballot(le);
+        if (m) last = b0 + 63 - __clzll(m);
+        if (m != ~0ull) break;
+    }
+    return last;
+}
+
+template <int METHOD>
+__global__ __launch_bounds__(512) void k_match_tiles_b(const MPair* __restrict__ pairs, int np,
+                                                        const uint8_t* __restrict__ A, const uint8_t* __restrict__ B,
+                                                        const int* __restrict__ na, const int* __restrict__ nb,
+                                                        char* __restrict__ ws) {
+    __shared__ uint32_t colk[8][64][2];
+    __shared__ int s_na[kRowBlk];
+    __shared__ int s_p;
+    if (threadIdx.x < 64) {
+        const int p = pair_of_tile(pairs, np, (int)blockIdx.x);
+        if (threadIdx.x == 0) s_p = p;
+    }
+    __syncthreads();
+    const MPair P = pairs[s_p];
+    const int t = (int)blockIdx.x - P.tile0;
+    match_tile<METHOD>(A + (size_t)P.a_off * 128, P.N, B + (size_t)P.b_off * 128, P.M, na + P.a_off, nb + P.b_off,
+                       (uint32_t*)(ws + P.o_rowpart), (uint32_t*)(ws + P.o_colpart), t % P.nrb, t / P.nrb, colk, s_na);
+}
+
 // merge per-chunk partials into (best index, d1, d2); key 0xffffffff = none
-__global__ __launch_bounds__(256) void k_reduce_top2(const uint32_t* __restrict__ part, int n, int nparts,
-                                                      int shift, int span, int32_t* __restrict__ out) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void reduce_top2_row(const uint32_t* __restrict__ part, int n, int nparts, int shift,
+                                                int span, int32_t* __restrict__ out, int i) {
     uint64_t k1 = ~0ull, k2 = ~0ull;
     for (int p = 0; p < nparts; ++p) {
 #pragma unroll
@@ -197,6 +249,25 @@ __global__ __launch_bounds__(256) void k_reduce_top2(const uint32_t* __restrict_
     out[3 * i] = k1 == ~0ull ? -1 : (int32_t)(k1 & 0xffffffffu);
     out[3 * i + 1] = k1 == ~0ull ? 0x7fffffff : (int32_t)(k1 >> 32);
     out[3 * i + 2] = k2 == ~0ull ? 0x7fffffff : (int32_t)(k2 >> 32);
+}
+
+__global__ __launch_bounds__(256) void k_reduce_top2(const uint32_t* __restrict__ part, int n, int nparts,
+                                                      int shift, int span, int32_t* __restrict__ out) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) reduce_top2_row(part, n, nparts, shift, span, out, i);
+}
+
+// rows (cols == 0: top-2 over b, into fwd) or columns (top-2 over a, into
+// bwd) of pair blockIdx.y; no partials (an empty side) gives "none"
+__global__ __launch_bounds__(256) void k_reduce_top2_b(const MPair* __restrict__ pairs, const char* __restrict__ ws,
+                                                        int cols, int32_t* __restrict__ fwd, int32_t* __restrict__ bwd) {
+    const MPair P = pairs[blockIdx.y];
+    const int n = cols ? P.M : P.N;
+    const int nparts = (P.N > 0 && P.M > 0) ? (cols ? P.nrb : P.ncc) : 0;
+    const uint32_t* part = (const uint32_t*)(ws + (cols ? P.o_colpart : P.o_rowpart));
+    int32_t* out = cols ? bwd + 3 * (size_t)P.b_off : fwd + 3 * (size_t)P.a_off;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+        reduce_top2_row(part, n, nparts, cols ? 8 : 9, cols ? kRowBlk : kColChunk, out, i);
 }
 
 __global__ void k_fill_top2_none(int32_t* __restrict__ out, int n) {
@@ -215,33 +286,98 @@ __device__ __forceinline__ bool mutual_ok(const int32_t* fwd, const int32_t* bwd
     return (double)bwd[3 * j + 1] < r2 * (double)bwd[3 * j + 2];
 }
 
+// block b (256 rows): number of mutual matches; whole block
+__device__ __forceinline__ int mutual_block_count(const int32_t* __restrict__ fwd, const int32_t* __restrict__ bwd,
+                                                  int N, int M, double r2, int b, int* wc) {
+    const int i = b * 256 + (int)threadIdx.x;
+    const bool ok = i < N && mutual_ok(fwd, bwd, N, M, r2, i);
+    const unsigned long long m = __ballot(ok);
+    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    const int c = (wc[0] + wc[1]) + (wc[2] + wc[3]);
+    __syncthreads();
+    return c;
+}
+
+// block b: write its mutual pairs (i, fwd j) from position `pos0` on; whole block
+__device__ __forceinline__ void mutual_block_write(const int32_t* __restrict__ fwd, const int32_t* __restrict__ bwd,
+                                                   int N, int M, double r2, int b, int pos0, int32_t* __restrict__ pairs,
+                                                   int* wc) {
+    const int i = b * 256 + (int)threadIdx.x;
+    const bool ok = i < N && mutual_ok(fwd, bwd, N, M, r2, i);
+    const unsigned long long m = __ballot(ok);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) wc[wave] = __popcll(m);
+    __syncthreads();
+    if (ok) {
+        int pre = 0;
+        for (int w = 0; w < wave; ++w) pre += wc[w];
+        const int pos = pos0 + pre + __popcll(m & lanemask_lt());
+        pairs[2 * pos] = i;
+        pairs[2 * pos + 1] = fwd[3 * i];
+    }
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(256) void k_mutual_count(const int32_t* __restrict__ fwd, const int32_t* __restrict__ bwd,
                                                        int N, int M, double r2, int* __restrict__ counts) {
     __shared__ int wc[4];
-    int i = blockIdx.x * 256 + threadIdx.x;
-    bool ok = i < N && mutual_ok(fwd, bwd, N, M, r2, i);
-    unsigned long long m = __ballot(ok);
-    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(m);
-    __syncthreads();
-    if (threadIdx.x == 0) counts[blockIdx.x] = (wc[0] + wc[1]) + (wc[2] + wc[3]);
+    const int c = mutual_block_count(fwd, bwd, N, M, r2, blockIdx.x, wc);
+    if (threadIdx.x == 0) counts[blockIdx.x] = c;
 }
 
 __global__ __launch_bounds__(256) void k_mutual_write(const int32_t* __restrict__ fwd, const int32_t* __restrict__ bwd,
                                                        int N, int M, double r2, const int* __restrict__ offs,
                                                        int32_t* __restrict__ pairs) {
     __shared__ int wc[4];
-    int i = blockIdx.x * 256 + threadIdx.x;
-    bool ok = i < N && mutual_ok(fwd, bwd, N, M, r2, i);
-    unsigned long long m = __ballot(ok);
-    int wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) wc[wave] = __popcll(m);
-    __syncthreads();
-    if (!ok) return;
-    int pre = 0;
-    for (int w = 0; w < wave; ++w) pre += wc[w];
-    int pos = offs[blockIdx.x] + pre + __popcll(m & lanemask_lt());
-    pairs[2 * pos] = i;
-    pairs[2 * pos + 1] = fwd[3 * i];
+    mutual_block_write(fwd, bwd, N, M, r2, blockIdx.x, offs[blockIdx.x], pairs, wc);
+}
+
+__global__ __launch_bounds__(256) void k_mutual_count_b(const MPair* __restrict__ pairs, const int32_t* __restrict__ fwd,
+                                                         const int32_t* __restrict__ bwd, double r2,
+                                                         int* __restrict__ counts) {
+    __shared__ int wc[4];
+    const MPair P = pairs[blockIdx.y];
+    for (int b = blockIdx.x; b < P.nblk; b += gridDim.x) {
+        const int c = mutual_block_count(fwd + 3 * (size_t)P.a_off, bwd + 3 * (size_t)P.b_off, P.N, P.M, r2, b, wc);
+        if (threadIdx.x == 0) counts[P.blk0 + b] = c;
+    }
+}
+
+// pair p's mutual pairs go to out[2 * a_off ...] (at most N of them)
+__global__ __launch_bounds__(256) void k_mutual_write_b(const MPair* __restrict__ pairs, const int32_t* __restrict__ fwd,
+                                                         const int32_t* __restrict__ bwd, double r2,
+                                                         const int* __restrict__ offs, int32_t* __restrict__ out) {
+    __shared__ int wc[4];
+    const MPair P = pairs[blockIdx.y];
+    for (int b = blockIdx.x; b < P.nblk; b += gridDim.x)
+        mutual_block_write(fwd + 3 * (size_t)P.a_off, bwd + 3 * (size_t)P.b_off, P.N, P.M, r2, b,
+                           offs[P.blk0 + b] - offs[P.blk0], out + 2 * (size_t)P.a_off, wc);
+}
+
+__global__ __launch_bounds__(64) void k_npairs_b(const MPair* __restrict__ pairs, int np, const int* __restrict__ offs,
+                                                  int* __restrict__ npairs) {
+    for (int p = threadIdx.x; p < np; p += 64) npairs[p] = offs[pairs[p].blk0 + pairs[p].nblk] - offs[pairs[p].blk0];
+}
+
+// matched point coordinates of every pair: p1[off + k] = (x, y) of a's row,
+// p2[off + k] = (x, y) of b's row, for k < npairs[p]; off = a_off (room for N)
+__global__ __launch_bounds__(256) void k_gather_points_b(const MPair* __restrict__ pairs,
+                                                          const float* __restrict__ kpa, int sa,
+                                                          const float* __restrict__ kpb, int sb,
+                                                          const int32_t* __restrict__ mp, const int* __restrict__ npairs,
+                                                          double* __restrict__ p1, double* __restrict__ p2) {
+    const MPair P = pairs[blockIdx.y];
+    const int n = npairs[blockIdx.y];
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
+        const int i = mp[2 * ((size_t)P.a_off + k)], j = mp[2 * ((size_t)P.a_off + k) + 1];
+        const float* a = kpa + (size_t)(P.a_off + i) * sa;
+        const float* b = kpb + (size_t)(P.b_off + j) * sb;
+        p1[2 * ((size_t)P.a_off + k)] = (double)a[0];
+        p1[2 * ((size_t)P.a_off + k) + 1] = (double)a[1];
+        p2[2 * ((size_t)P.a_off + k)] = (double)b[0];
+        p2[2 * ((size_t)P.a_off + k) + 1] = (double)b[1];
+    }
 }
 
 // ---------------------------------------------------------------- host ----
@@ -308,6 +444,131 @@ int match_u8(const uint8_t* a, int N, const uint8_t* b, int M, double ratio, int
     return MSIFT_OK;
 }
 
+}  // namespace msift
+
+// plan of a batched match: pair table in device memory, workspace layout
+struct msift_match_batch {
+    int np;
+    std::vector<msift::MPair> pairs;
+    int sumN, sumM, tiles, blks, max_rows, max_cols, max_blk;
+    size_t o_na, o_nb, o_cnt, o_off, o_stmp, total;
+    msift::MPair* d_pairs;
+};
+
+namespace msift {
+
+static int match_batch_create(int np, const int* N, const int* M, msift_match_batch** out) {
+    if (np < 1 || np > 65535 || !N || !M || !out) { set_error("match batch: bad arguments"); return MSIFT_EINVAL; }
+    msift_match_batch* B = new (std::nothrow) msift_match_batch();
+    if (!B) { set_error("out of host memory"); return MSIFT_EINVAL; }
+    B->np = np;
+    B->d_pairs = nullptr;
+    size_t off = 0;
+    auto take = [&](size_t b) { size_t o = off; off += (b + 255) & ~(size_t)255; return o; };
+    long long sa = 0, sb = 0;
+    int tiles = 0, blks = 0;
+    B->max_rows = B->max_cols = B->max_blk = 0;
+    for (int p = 0; p < np; ++p) {
+        if (N[p] < 0 || M[p] < 0) { set_error("match batch: negative count"); delete B; return MSIFT_EINVAL; }
+        MPair P;
+        memset(&P, 0, sizeof(P));
+        P.a_off = (int)sa; P.N = N[p]; P.b_off = (int)sb; P.M = M[p];
+        const bool both = N[p] > 0 && M[p] > 0;
+        P.nrb = both ? (N[p] + kRowBlk - 1) / kRowBlk : 0;
+        P.ncc = both ? (M[p] + kColChunk - 1) / kColChunk : 0;
+        P.tile0 = tiles;
+        tiles += P.nrb * P.ncc;
+        P.nblk = (N[p] + 255) / 256;
+        P.blk0 = blks;
+        blks += P.nblk;
+        P.o_rowpart = take((size_t)P.ncc * (N[p] > 0 ? N[p] : 1) * 8);
+        P.o_colpart = take((size_t)P.nrb * (M[p] > 0 ? M[p] : 1) * 8);
+        B->pairs.push_back(P);
+        sa += N[p]; sb += M[p];
+        B->max_rows = std::max(B->max_rows, N[p]);
+        B->max_cols = std::max(B->max_cols, M[p]);
+        B->max_blk = std::max(B->max_blk, P.nblk);
+    }
+    if (sa > (1LL << 30) || sb > (1LL << 30)) { set_error("match batch: too many descriptors"); delete B; return MSIFT_EINVAL; }
+    B->sumN = (int)sa; B->sumM = (int)sb; B->tiles = tiles; B->blks = blks;
+    B->o_na = take((size_t)(sa > 0 ? sa : 1) * 4);
+    B->o_nb = take((size_t)(sb > 0 ? sb : 1) * 4);
+    B->o_cnt = take((size_t)(blks + 1) * 4);
+    B->o_off = take((size_t)(blks + 1) * 4);
+    B->o_stmp = take((size_t)scan_tmp_ints(blks) * 4);
+    B->total = off;
+    if (hipMalloc(&B->d_pairs, sizeof(MPair) * np) != hipSuccess ||
+        hipMemcpy(B->d_pairs, B->pairs.data(), sizeof(MPair) * np, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("match batch: device table allocation failed");
+        if (B->d_pairs) (void)hipFree(B->d_pairs);
+        delete B;
+        return MSIFT_EHIP;
+    }
+    *out = B;
+    return MSIFT_OK;
+}
+
+static inline unsigned gcap_u(long long n, unsigned cap) { return n < 1 ? 1u : (n < cap ? (unsigned)n : cap); }
+
+static int match_batch_run(msift_match_batch* B, const uint8_t* a, const uint8_t* b, double ratio, int method,
+                           void* ws_, size_t ws_bytes, int32_t* fwd, int32_t* bwd, int32_t* pairs, int* npairs,
+                           hipStream_t s) {
+    if (ws_bytes < B->total) { set_error("match_u8_batched: workspace too small"); return MSIFT_EWORKSPACE; }
+    if ((B->sumN > 0 && (!a || !fwd || !pairs)) || (B->sumM > 0 && (!b || !bwd)) || !npairs || !ws_) {
+        set_error("match_u8_batched: null buffer");
+        return MSIFT_EINVAL;
+    }
+    if (((uintptr_t)a & 15) || ((uintptr_t)b & 15)) { set_error("match_u8_batched: descriptors must be 16-byte aligned"); return MSIFT_EINVAL; }
+    if (method != 0 && method != 1) { set_error("match_u8_batched: method must be 0 or 1"); return MSIFT_EINVAL; }
+    char* ws = (char*)ws_;
+    int* na = (int*)(ws + B->o_na);
+    int* nb = (int*)(ws + B->o_nb);
+    const int np = B->np;
+    if (B->sumN > 0) hipLaunchKernelGGL(k_norms, dim3((B->sumN + 3) / 4), dim3(256), 0, s, a, B->sumN, method, na);
+    if (B->sumM > 0) hipLaunchKernelGGL(k_norms, dim3((B->sumM + 3) / 4), dim3(256), 0, s, b, B->sumM, method, nb);
+    if (B->tiles > 0) {
+        if (method == 0)
+            hipLaunchKernelGGL(k_match_tiles_b<0>, dim3(B->tiles), dim3(512), 0, s, (const MPair*)B->d_pairs, np, a, b,
+                               (const int*)na, (const int*)nb, ws);
+        else
+            hipLaunchKernelGGL(k_match_tiles_b<1>, dim3(B->tiles), dim3(512), 0, s, (const MPair*)B->d_pairs, np, a, b,
+                               (const int*)na, (const int*)nb, ws);
+    }
+    if (B->max_rows > 0)
+        hipLaunchKernelGGL(k_reduce_top2_b, dim3(gcap_u((B->max_rows + 255) / 256, 1024), np), dim3(256), 0, s,
+                           (const MPair*)B->d_pairs, (const char*)ws, 0, fwd, bwd);
+    if (B->max_cols > 0)
+        hipLaunchKernelGGL(k_reduce_top2_b, dim3(gcap_u((B->max_cols + 255) / 256, 1024), np), dim3(256), 0, s,
+                           (const MPair*)B->d_pairs, (const char*)ws, 1, fwd, bwd);
+    const double r2 = ratio * ratio;
+    int* cnt = (int*)(ws + B->o_cnt);
+    int* offs = (int*)(ws + B->o_off);
+    if (B->max_blk > 0)
+        hipLaunchKernelGGL(k_mutual_count_b, dim3(gcap_u(B->max_blk, 1024), np), dim3(256), 0, s,
+                           (const MPair*)B->d_pairs, (const int32_t*)fwd, (const int32_t*)bwd, r2, cnt);
+    launch_scan(cnt, offs, B->blks, nullptr, nullptr, nullptr, 0, s, (int*)(ws + B->o_stmp));
+    if (B->max_blk > 0)
+        hipLaunchKernelGGL(k_mutual_write_b, dim3(gcap_u(B->max_blk, 1024), np), dim3(256), 0, s,
+                           (const MPair*)B->d_pairs, (const int32_t*)fwd, (const int32_t*)bwd, r2, (const int*)offs,
+                           pairs);
+    hipLaunchKernelGGL(k_npairs_b, dim3(1), dim3(64), 0, s, (const MPair*)B->d_pairs, np, (const int*)offs, npairs);
+    MSIFT_LAUNCH_CHECK();
+    return MSIFT_OK;
+}
+
+static int match_batch_gather(msift_match_batch* B, const float* kpa, int sa, const float* kpb, int sb,
+                              const int32_t* mp, const int* npairs, double* p1, double* p2, hipStream_t s) {
+    if (!kpa || !kpb || !mp || !npairs || !p1 || !p2 || sa < 2 || sb < 2) {
+        set_error("match batch gather: bad arguments");
+        return MSIFT_EINVAL;
+    }
+    if (B->max_rows > 0)
+        hipLaunchKernelGGL(k_gather_points_b, dim3(gcap_u((B->max_rows + 255) / 256, 256), B->np), dim3(256), 0, s,
+                           (const MPair*)B->d_pairs, kpa, sa, kpb, sb, mp, npairs, p1, p2);
+    MSIFT_LAUNCH_CHECK();
+    return MSIFT_OK;
+}
+
 int match_f16(const float*, int, const float*, int, double, void*, size_t, int32_t*, int32_t*, int32_t*, int*,
               hipStream_t) {
     set_error("match_f16: float descriptor path is not built");
@@ -315,3 +576,46 @@ int match_f16(const float*, int, const float*, int, double, void*, size_t, int32
 }
 
 }  // namespace msift
+
+using namespace msift;
+
+extern "C" {
+
+int msift_match_batch_create(int n_pairs, const int* N, const int* M, msift_match_batch** plan) {
+    return match_batch_create(n_pairs, N, M, plan);
+}
+
+int msift_match_batch_destroy(msift_match_batch* plan) {
+    if (!plan) return MSIFT_OK;
+    if (plan->d_pairs) (void)hipFree(plan->d_pairs);
+    delete plan;
+    return MSIFT_OK;
+}
+
+int msift_match_batch_sizes(msift_match_batch* plan, size_t* ws_bytes, int* a_offsets, int* b_offsets) {
+    if (!plan) { set_error("null plan"); return MSIFT_EINVAL; }
+    if (ws_bytes) *ws_bytes = plan->total;
+    for (int p = 0; p < plan->np; ++p) {
+        if (a_offsets) a_offsets[p] = plan->pairs[p].a_off;
+        if (b_offsets) b_offsets[p] = plan->pairs[p].b_off;
+    }
+    return MSIFT_OK;
+}
+
+int msift_match_u8_batched(msift_match_batch* plan, const uint8_t* d_a, const uint8_t* d_b, double ratio, int method,
+                           void* d_ws, size_t ws_bytes, int32_t* d_fwd, int32_t* d_bwd, int32_t* d_pairs,
+                           int* d_npairs, void* stream) {
+    if (!plan) { set_error("null plan"); return MSIFT_EINVAL; }
+    return match_batch_run(plan, d_a, d_b, ratio, method, d_ws, ws_bytes, d_fwd, d_bwd, d_pairs, d_npairs,
+                           (hipStream_t)stream);
+}
+
+int msift_match_batch_gather_points(msift_match_batch* plan, const float* d_kpa, int stride_a, const float* d_kpb,
+                                    int stride_b, const int32_t* d_pairs, const int* d_npairs, double* d_p1,
+                                    double* d_p2, void* stream) {
+    if (!plan) { set_error("null plan"); return MSIFT_EINVAL; }
+    return match_batch_gather(plan, d_kpa, stride_a, d_kpb, stride_b, d_pairs, d_npairs, d_p1, d_p2,
+                              (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -10,6 +10,7 @@
 // points use 64 lane partials (point k -> lane k % 64) and the xor tree, so
 // every result is reproduced bit for bit by the CPU oracle.
 #include "msift_internal.hpp"
+#include <vector>
 
 #pragma clang fp contract(off)
 
@@ -158,8 +159,8 @@ struct RansacWs {
     uint8_t* mask2;
 };
 
-__global__ __launch_bounds__(64) void k_normalize(const double* __restrict__ p1, const double* __restrict__ p2, int N,
-                                                   double* __restrict__ T) {
+__device__ __forceinline__ void normalize_body(const double* __restrict__ p1, const double* __restrict__ p2, int N,
+                                               double* __restrict__ T) {
     double T1[9], T2[9];
     hartley_wave(p1, nullptr, N, T1);
     hartley_wave(p2, nullptr, N, T2);
@@ -167,14 +168,19 @@ __global__ __launch_bounds__(64) void k_normalize(const double* __restrict__ p1,
         for (int k = 0; k < 9; ++k) { T[k] = T1[k]; T[9 + k] = T2[k]; }
 }
 
-__global__ __launch_bounds__(64) void k_hypotheses(const double* __restrict__ p1, const double* __restrict__ p2, int N,
-                                                    int B, uint32_t k0, uint32_t k1, const double* __restrict__ T,
-                                                    int method, double den, double tq, double* __restrict__ scores,
-                                                    double* __restrict__ Hs) {
-    __shared__ double sH[kHypPerBlock][9], sHi[kHypPerBlock][9];
-    __shared__ int sOk[kHypPerBlock];
+__global__ __launch_bounds__(64) void k_normalize(const double* __restrict__ p1, const double* __restrict__ p2, int N,
+                                                   double* __restrict__ T) {
+    normalize_body(p1, p2, N, T);
+}
+
+// hypotheses blk*64 .. blk*64+63 (one wave): solve, then score
+__device__ __forceinline__ void hypotheses_body(const double* __restrict__ p1, const double* __restrict__ p2, int N,
+                                                int B, uint32_t k0, uint32_t k1, const double* __restrict__ T,
+                                                int method, double den, double tq, double* __restrict__ scores,
+                                                double* __restrict__ Hs, int blk, double (*sH)[9], double (*sHi)[9],
+                                                int* sOk) {
     const int lane = threadIdx.x;
-    const int b = blockIdx.x * kHypPerBlock + lane;
+    const int b = blk * kHypPerBlock + lane;
     double T1[9], T2[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) { T1[k] = T[k]; T2[k] = T[9 + k]; }
@@ -209,9 +215,9 @@ __global__ __launch_bounds__(64) void k_hypotheses(const double* __restrict__ p1
     }
     __syncthreads();
     // phase 2: score each hypothesis with lanes over points
-    const int nh = min(kHypPerBlock, B - blockIdx.x * kHypPerBlock);
+    const int nh = min(kHypPerBlock, B - blk * kHypPerBlock);
     for (int hq = 0; hq < nh; ++hq) {
-        const int bb = blockIdx.x * kHypPerBlock + hq;
+        const int bb = blk * kHypPerBlock + hq;
         if (!sOk[hq]) {
             if (lane == 0) scores[bb] = -1.0;
             continue;
@@ -230,10 +236,18 @@ __global__ __launch_bounds__(64) void k_hypotheses(const double* __restrict__ p1
     }
 }
 
+__global__ __launch_bounds__(64) void k_hypotheses(const double* __restrict__ p1, const double* __restrict__ p2, int N,
+                                                    int B, uint32_t k0, uint32_t k1, const double* __restrict__ T,
+                                                    int method, double den, double tq, double* __restrict__ scores,
+                                                    double* __restrict__ Hs) {
+    __shared__ double sH[kHypPerBlock][9], sHi[kHypPerBlock][9];
+    __shared__ int sOk[kHypPerBlock];
+    hypotheses_body(p1, p2, N, B, k0, k1, T, method, den, tq, scores, Hs, blockIdx.x, sH, sHi, sOk);
+}
+
 // argmax with the smallest index on ties (a total order: any reduction order)
-__global__ __launch_bounds__(1024) void k_argmax(const double* __restrict__ scores, int B, int* __restrict__ best) {
-    __shared__ double ss[1024];
-    __shared__ int si[1024];
+__device__ __forceinline__ void argmax_body(const double* __restrict__ scores, int B, int* __restrict__ best,
+                                            double* ss, int* si) {
     double bs = -__builtin_huge_val();
     int bi = 0x7fffffff;
     for (int b = threadIdx.x; b < B; b += 1024) {
@@ -251,6 +265,12 @@ __global__ __launch_bounds__(1024) void k_argmax(const double* __restrict__ scor
         __syncthreads();
     }
     if (threadIdx.x == 0) *best = si[0];
+}
+
+__global__ __launch_bounds__(1024) void k_argmax(const double* __restrict__ scores, int B, int* __restrict__ best) {
+    __shared__ double ss[1024];
+    __shared__ int si[1024];
+    argmax_body(scores, B, best, ss, si);
 }
 
 // compacts mask (in index order) into idx, returns the count (all lanes)
@@ -375,14 +395,12 @@ __device__ void dlt_ls_wave(const double* __restrict__ p1, const double* __restr
     __syncthreads();
 }
 
-__global__ __launch_bounds__(64) void k_finalize(const double* __restrict__ p1, const double* __restrict__ p2, int N,
-                                                  double tq, int lo, int min_inl, const double* __restrict__ Hs,
-                                                  const int* __restrict__ bestp, int* __restrict__ idx,
-                                                  uint8_t* __restrict__ mask2, double* __restrict__ Hout,
-                                                  uint8_t* __restrict__ mask, int* __restrict__ ninl,
-                                                  int* __restrict__ best_out) {
-    __shared__ double sM[81];
-    __shared__ double sH1[9], sH2[9];
+__device__ __forceinline__ void finalize_body(const double* __restrict__ p1, const double* __restrict__ p2, int N,
+                                              double tq, int lo, int min_inl, const double* __restrict__ Hs,
+                                              const int* __restrict__ bestp, int* __restrict__ idx,
+                                              uint8_t* __restrict__ mask2, double* __restrict__ Hout,
+                                              uint8_t* __restrict__ mask, int* __restrict__ ninl,
+                                              int* __restrict__ best_out, double* sM, double* sH1, double* sH2) {
     const int lane = threadIdx.x;
     const int best = *bestp;
     double Hb[9], Hbi[9];
@@ -438,6 +456,88 @@ __global__ __launch_bounds__(64) void k_finalize(const double* __restrict__ p1, 
     if (lane == 0) *ninl = nfinal;
 }
 
+__global__ __launch_bounds__(64) void k_finalize(const double* __restrict__ p1, const double* __restrict__ p2, int N,
+                                                  double tq, int lo, int min_inl, const double* __restrict__ Hs,
+                                                  const int* __restrict__ bestp, int* __restrict__ idx,
+                                                  uint8_t* __restrict__ mask2, double* __restrict__ Hout,
+                                                  uint8_t* __restrict__ mask, int* __restrict__ ninl,
+                                                  int* __restrict__ best_out) {
+    __shared__ double sM[81];
+    __shared__ double sH1[9], sH2[9];
+    finalize_body(p1, p2, N, tq, lo, min_inl, Hs, bestp, idx, mask2, Hout, mask, ninl, best_out, sM, sH1, sH2);
+}
+
+// ------------------------------------------------------------- batched ----
+// Pairs of point sets packed back to back (pair p at point offset poff, room
+// for cap points); the point count of each pair is read from device memory,
+// so a caller can chain the batched matcher without a host round trip.
+// Pair p uses key = seed + p: its result equals a single call with that seed.
+struct RPair {
+    size_t poff, cap;
+    size_t o_scores, o_Hs, o_T, o_ints, o_idx, o_mask2;
+};
+
+__device__ __forceinline__ int pair_npts(const RPair& P, const int* npts) {
+    int n = npts[blockIdx.y];
+    return n < 0 ? 0 : (n > (int)P.cap ? (int)P.cap : n);
+}
+
+__global__ __launch_bounds__(64) void k_bnormalize(const RPair* __restrict__ rp, const double* __restrict__ p1,
+                                                    const double* __restrict__ p2, const int* __restrict__ npts,
+                                                    char* __restrict__ ws) {
+    const RPair P = rp[blockIdx.y];
+    const int N = pair_npts(P, npts);
+    if (N < 4) return;
+    normalize_body(p1 + 2 * P.poff, p2 + 2 * P.poff, N, (double*)(ws + P.o_T));
+}
+
+__global__ __launch_bounds__(64) void k_bhypotheses(const RPair* __restrict__ rp, const double* __restrict__ p1,
+                                                     const double* __restrict__ p2, const int* __restrict__ npts, int B,
+                                                     uint64_t seed, int method, double den, double tq,
+                                                     char* __restrict__ ws) {
+    __shared__ double sH[kHypPerBlock][9], sHi[kHypPerBlock][9];
+    __shared__ int sOk[kHypPerBlock];
+    const RPair P = rp[blockIdx.y];
+    const int N = pair_npts(P, npts);
+    if (N < 4) return;
+    const uint64_t key = seed + (uint64_t)blockIdx.y;
+    hypotheses_body(p1 + 2 * P.poff, p2 + 2 * P.poff, N, B, (uint32_t)key, (uint32_t)(key >> 32),
+                    (const double*)(ws + P.o_T), method, den, tq, (double*)(ws + P.o_scores), (double*)(ws + P.o_Hs),
+                    blockIdx.x, sH, sHi, sOk);
+}
+
+__global__ __launch_bounds__(1024) void k_bargmax(const RPair* __restrict__ rp, const int* __restrict__ npts, int B,
+                                                   char* __restrict__ ws) {
+    __shared__ double ss[1024];
+    __shared__ int si[1024];
+    const RPair P = rp[blockIdx.y];
+    if (pair_npts(P, npts) < 4) return;
+    argmax_body((const double*)(ws + P.o_scores), B, (int*)(ws + P.o_ints), ss, si);
+}
+
+// pairs with fewer than 4 points: H = 0, no inliers, best = -1
+__global__ __launch_bounds__(64) void k_bfinalize(const RPair* __restrict__ rp, const double* __restrict__ p1,
+                                                   const double* __restrict__ p2, const int* __restrict__ npts, double tq,
+                                                   int lo, int min_inl, char* __restrict__ ws, double* __restrict__ Hout,
+                                                   uint8_t* __restrict__ mask, int* __restrict__ ninl,
+                                                   int* __restrict__ best) {
+    __shared__ double sM[81];
+    __shared__ double sH1[9], sH2[9];
+    const RPair P = rp[blockIdx.y];
+    const int N = pair_npts(P, npts);
+    const int p = blockIdx.y;
+    if (N < 4) {
+        if (threadIdx.x < 9) Hout[9 * p + threadIdx.x] = 0.0;
+        for (int i = threadIdx.x; i < N; i += 64) mask[P.poff + i] = 0;
+        if (threadIdx.x == 0) { ninl[p] = 0; if (best) best[p] = -1; }
+        return;
+    }
+    int* ints = (int*)(ws + P.o_ints);
+    finalize_body(p1 + 2 * P.poff, p2 + 2 * P.poff, N, tq, lo, min_inl, (const double*)(ws + P.o_Hs), ints,
+                  (int*)(ws + P.o_idx), (uint8_t*)(ws + P.o_mask2), Hout + 9 * p, mask + P.poff, ninl + p,
+                  best ? best + p : ints + 1, sM, sH1, sH2);
+}
+
 // ---------------------------------------------------------------- host ----
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
@@ -472,3 +572,107 @@ int ransac(const double* p1, const double* p2, int N, uint64_t seed, int B, doub
 }
 
 }  // namespace msift
+
+struct msift_ransac_batch {
+    int np, B;
+    std::vector<msift::RPair> pairs;
+    size_t total;
+    msift::RPair* d_pairs;
+};
+
+namespace msift {
+
+static int ransac_batch_create(int np, const int* cap, int B, msift_ransac_batch** out) {
+    if (np < 1 || np > 65535 || !cap || B < 1 || !out) { set_error("ransac batch: bad arguments"); return MSIFT_EINVAL; }
+    msift_ransac_batch* R = new (std::nothrow) msift_ransac_batch();
+    if (!R) { set_error("out of host memory"); return MSIFT_EINVAL; }
+    R->np = np; R->B = B; R->d_pairs = nullptr;
+    size_t off = 0, poff = 0;
+    auto take = [&](size_t b) { size_t o = off; off += align256(b); return o; };
+    for (int p = 0; p < np; ++p) {
+        if (cap[p] < 0) { set_error("ransac batch: negative capacity"); delete R; return MSIFT_EINVAL; }
+        RPair P;
+        P.poff = poff; P.cap = (size_t)cap[p];
+        poff += (size_t)cap[p];
+        P.o_scores = take((size_t)B * 8);
+        P.o_Hs = take((size_t)B * 72);
+        P.o_T = take(18 * 8);
+        P.o_ints = take(16);
+        P.o_idx = take((size_t)(cap[p] > 0 ? cap[p] : 1) * 4);
+        P.o_mask2 = take((size_t)(cap[p] > 0 ? cap[p] : 1));
+        R->pairs.push_back(P);
+    }
+    R->total = off;
+    if (hipMalloc(&R->d_pairs, sizeof(RPair) * np) != hipSuccess ||
+        hipMemcpy(R->d_pairs, R->pairs.data(), sizeof(RPair) * np, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("ransac batch: device table allocation failed");
+        if (R->d_pairs) (void)hipFree(R->d_pairs);
+        delete R;
+        return MSIFT_EHIP;
+    }
+    *out = R;
+    return MSIFT_OK;
+}
+
+static int ransac_batch_run(msift_ransac_batch* R, const double* p1, const double* p2, const int* npts, uint64_t seed,
+                            double thr, double sigma_max, int method, int lo, int min_inl, void* ws_, size_t ws_bytes,
+                            double* H, uint8_t* mask, int* ninl, int* best, hipStream_t s) {
+    if (ws_bytes < R->total) { set_error("ransac_homography_batched: workspace too small"); return MSIFT_EWORKSPACE; }
+    if (!p1 || !p2 || !npts || !H || !mask || !ninl || !ws_) {
+        set_error("ransac_homography_batched: null buffer");
+        return MSIFT_EINVAL;
+    }
+    if (!(thr > 0.0) || !(sigma_max > 0.0) || (method != 0 && method != 1)) {
+        set_error("ransac_homography_batched: bad arguments");
+        return MSIFT_EINVAL;
+    }
+    char* ws = (char*)ws_;
+    const int np = R->np, B = R->B;
+    const double tq = thr * thr;
+    const double den = 2.0 * (sigma_max * sigma_max);
+    const RPair* rp = R->d_pairs;
+    hipLaunchKernelGGL(k_bnormalize, dim3(1, np), dim3(64), 0, s, rp, p1, p2, npts, ws);
+    hipLaunchKernelGGL(k_bhypotheses, dim3((B + kHypPerBlock - 1) / kHypPerBlock, np), dim3(64), 0, s, rp, p1, p2,
+                       npts, B, seed, method, den, tq, ws);
+    hipLaunchKernelGGL(k_bargmax, dim3(1, np), dim3(1024), 0, s, rp, npts, B, ws);
+    hipLaunchKernelGGL(k_bfinalize, dim3(1, np), dim3(64), 0, s, rp, p1, p2, npts, tq, lo, min_inl, ws, H, mask, ninl,
+                       best);
+    MSIFT_LAUNCH_CHECK();
+    return MSIFT_OK;
+}
+
+}  // namespace msift
+
+using namespace msift;
+
+extern "C" {
+
+int msift_ransac_batch_create(int n_pairs, const int* cap, int n_hyp, msift_ransac_batch** plan) {
+    return ransac_batch_create(n_pairs, cap, n_hyp, plan);
+}
+
+int msift_ransac_batch_destroy(msift_ransac_batch* plan) {
+    if (!plan) return MSIFT_OK;
+    if (plan->d_pairs) (void)hipFree(plan->d_pairs);
+    delete plan;
+    return MSIFT_OK;
+}
+
+int msift_ransac_batch_sizes(msift_ransac_batch* plan, size_t* ws_bytes, long long* point_offsets) {
+    if (!plan) { set_error("null plan"); return MSIFT_EINVAL; }
+    if (ws_bytes) *ws_bytes = plan->total;
+    if (point_offsets)
+        for (int p = 0; p < plan->np; ++p) point_offsets[p] = (long long)plan->pairs[p].poff;
+    return MSIFT_OK;
+}
+
+int msift_ransac_homography_batched(msift_ransac_batch* plan, const double* d_p1, const double* d_p2,
+                                    const int* d_npts, uint64_t seed, double thr_px, double sigma_max, int method,
+                                    int lo, int min_inliers, void* d_ws, size_t ws_bytes, double* d_H,
+                                    uint8_t* d_mask, int* d_ninl, int* d_best, void* stream) {
+    if (!plan) { set_error("null plan"); return MSIFT_EINVAL; }
+    return ransac_batch_run(plan, d_p1, d_p2, d_npts, seed, thr_px, sigma_max, method, lo, min_inliers, d_ws,
+                            ws_bytes, d_H, d_mask, d_ninl, d_best, (hipStream_t)stream);
+}
+
+}  // extern "C"

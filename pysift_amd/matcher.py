@@ -42,3 +42,95 @@ def match_u8(a, b, ratio=0.75, method="hilo", sync=True, ws=None):
         return fwd[:N], bwd[:M], pairs, npairs
     k = int(npairs.item())
     return fwd[:N], bwd[:M], pairs[:k]
+
+
+class MatchBatchPlan:
+    """Plan of a batched match over pairs (a_p: N[p] x 128, b_p: M[p] x 128
+    uint8): one launch over every tile of every pair; per pair the results
+    equal match_u8 on that pair."""
+
+    def __init__(self, Ns, Ms, device=None):
+        import torch
+        self.Ns = [int(n) for n in Ns]
+        self.Ms = [int(m) for m in Ms]
+        if len(self.Ns) != len(self.Ms) or not self.Ns:
+            raise ValueError("need equal, non-empty lists of pair sizes")
+        self.np = len(self.Ns)
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        L = _lib.lib()
+        arrN = (ctypes.c_int * self.np)(*self.Ns)
+        arrM = (ctypes.c_int * self.np)(*self.Ms)
+        p = ctypes.c_void_p()
+        _lib.check(L.msift_match_batch_create(self.np, arrN, arrM, ctypes.byref(p)), "match batch")
+        self._p = p
+        ws = ctypes.c_size_t()
+        ao = (ctypes.c_int * self.np)()
+        bo = (ctypes.c_int * self.np)()
+        _lib.check(L.msift_match_batch_sizes(p, ctypes.byref(ws), ao, bo), "match batch sizes")
+        self.a_off = list(ao)
+        self.b_off = list(bo)
+        dev = self.device
+        sN, sM = sum(self.Ns), sum(self.Ms)
+        self.ws = torch.empty(max(ws.value, 1), dtype=torch.uint8, device=dev)
+        self.fwd = torch.empty((max(sN, 1), 3), dtype=torch.int32, device=dev)
+        self.bwd = torch.empty((max(sM, 1), 3), dtype=torch.int32, device=dev)
+        self.pairs = torch.empty((max(sN, 1), 2), dtype=torch.int32, device=dev)
+        self.npairs = torch.zeros(self.np, dtype=torch.int32, device=dev)
+
+    def __del__(self):
+        p = getattr(self, "_p", None)
+        if p is not None and _lib._lib is not None:
+            _lib._lib.msift_match_batch_destroy(p)
+            self._p = None
+
+    def pack(self, sets, counts):
+        """Concatenate (n_p, 128) uint8 tensors into one (sum n_p, 128) tensor."""
+        import torch
+        for t, n in zip(sets, counts):
+            if t.dtype != torch.uint8 or t.dim() != 2 or t.shape[1] != 128 or t.shape[0] != n:
+                raise ValueError("descriptor sets must be (n,128) uint8 matching the plan")
+        if sum(counts) == 0:
+            return torch.zeros((1, 128), dtype=torch.uint8, device=self.device)
+        return torch.cat([t.to(self.device) for t in sets]).contiguous()
+
+    def run(self, A, B, ratio=0.75, method="hilo"):
+        """A, B packed (sum N, 128) / (sum M, 128) uint8 device tensors."""
+        rc = _lib.lib().msift_match_u8_batched(
+            self._p, _lib.ptr(A), _lib.ptr(B), float(ratio), METHODS[method], _lib.ptr(self.ws), self.ws.numel(),
+            _lib.ptr(self.fwd), _lib.ptr(self.bwd), _lib.ptr(self.pairs), _lib.ptr(self.npairs),
+            _lib.stream_ptr(self.device))
+        _lib.check(rc, "match_u8_batched")
+
+    def results(self):
+        """Per pair (fwd (N,3), bwd (M,3), pairs (K,2)) device tensors (synchronises)."""
+        k = self.npairs.cpu().tolist()
+        out = []
+        for p in range(self.np):
+            a, b, n, m = self.a_off[p], self.b_off[p], self.Ns[p], self.Ms[p]
+            out.append((self.fwd[a:a + n], self.bwd[b:b + m], self.pairs[a:a + k[p]]))
+        return out
+
+    def gather_points(self, kpa, kpb):
+        """Keypoint rows (x, y, ...) packed like the descriptors -> (p1, p2)
+        (sum N, 2) float64 with pair p's matches at rows a_off[p] ..."""
+        import torch
+        kpa = kpa.contiguous()
+        kpb = kpb.contiguous()
+        sN = max(sum(self.Ns), 1)
+        p1 = torch.zeros((sN, 2), dtype=torch.float64, device=self.device)
+        p2 = torch.zeros((sN, 2), dtype=torch.float64, device=self.device)
+        rc = _lib.lib().msift_match_batch_gather_points(
+            self._p, _lib.ptr(kpa), kpa.shape[1], _lib.ptr(kpb), kpb.shape[1], _lib.ptr(self.pairs),
+            _lib.ptr(self.npairs), _lib.ptr(p1), _lib.ptr(p2), _lib.stream_ptr(self.device))
+        _lib.check(rc, "gather points")
+        return p1, p2
+
+
+def match_u8_batched(sets_a, sets_b, ratio=0.75, method="hilo"):
+    """Match pairs (sets_a[p], sets_b[p]) in one launch; returns per pair
+    (fwd, bwd, pairs) as match_u8 would."""
+    plan = MatchBatchPlan([t.shape[0] for t in sets_a], [t.shape[0] for t in sets_b], device=sets_a[0].device)
+    A = plan.pack(sets_a, plan.Ns)
+    B = plan.pack(sets_b, plan.Ms)
+    plan.run(A, B, ratio, method)
+    return plan.results()
