@@ -133,6 +133,90 @@ def bench_batch(torch, dev, n=64, iters=5):
             "keypoints": int(plan.counts.sum().item())}
 
 
+def bench_pairs(torch, dev, n_pairs=32, n_hyp=1024, iters=5):
+    """32 homography-warped pairs at the uneven batch sizes: the 64 images are
+    detected by one batched graph replay, then every pair is matched in one
+    launch (exact INT8 MFMA, hi/lo split) and verified by batched RANSAC
+    (1024 Philox hypotheses per pair, 3 px), against the per-pair loop
+    (match_u8 + RANSAC per pair) on the same descriptors."""
+    import numpy as np
+    from pysift_amd import PySIFT
+    from pysift_amd import synthetic as syn
+    from pysift_amd.batch import BatchPlan
+    from pysift_amd.matcher import MatchBatchPlan, match_u8
+    from pysift_amd.ransac import RansacBatchPlan, ransac_homography
+    pairs = syn.synth_pairs_batch(n_pairs, 2100)
+    imgs = [im for a, b, _ in pairs for im in (a, b)]
+    s = PySIFT(device=dev)
+    plan = BatchPlan(s, [i.shape for i in imgs], device=dev)
+    plan.pack(imgs)
+    plan.autotune()
+    plan.capture()
+    plan.run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(iters):
+        plan.run()
+    e1.record()
+    torch.cuda.synchronize()
+    t_detect = e0.elapsed_time(e1) / iters
+    res = plan.results(u8=True)
+    A, B = res[0::2], res[1::2]
+    mp = MatchBatchPlan([k.shape[0] for k, _ in A], [k.shape[0] for k, _ in B], device=dev)
+    DA = mp.pack([d for _, d in A], mp.Ns)
+    DB = mp.pack([d for _, d in B], mp.Ms)
+    KA = torch.cat([k for k, _ in A]).contiguous()
+    KB = torch.cat([k for k, _ in B]).contiguous()
+    rp = RansacBatchPlan(mp.Ns, n_hyp, dev)
+
+    def batched():
+        mp.run(DA, DB, 0.75, "hilo")
+        p1, p2 = mp.gather_points(KA, KB)
+        return rp.run(p1, p2, mp.npairs, seed=42, thr=3.0)
+
+    batched()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(iters):
+        H, mask, ninl, _ = batched()
+    e1.record()
+    torch.cuda.synchronize()
+    t_batched = e0.elapsed_time(e1) / iters
+
+    def per_pair():
+        out = []
+        for p in range(n_pairs):
+            (ka, da), (kb, db) = A[p], B[p]
+            _, _, pr = match_u8(da, db, 0.75, "hilo")
+            if pr.shape[0] < 4:
+                out.append(None)
+                continue
+            p1 = ka[pr[:, 0].long(), :2].double()
+            p2 = kb[pr[:, 1].long(), :2].double()
+            out.append(ransac_homography(p1, p2, seed=42 + p, n_hyp=n_hyp, thr=3.0, return_device=True)[0])
+        return out
+
+    per_pair()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        single = per_pair()
+    torch.cuda.synchronize()
+    t_single = (time.perf_counter() - t0) * 1e3 / iters
+    Hn = H.cpu().numpy()
+    same = all(sh is not None and np.array_equal(Hn[p], sh.cpu().numpy()) for p, sh in enumerate(single))
+    errs = [syn.corner_error(Hn[p], pairs[p][2], pairs[p][0].shape[1], pairs[p][0].shape[0]) for p in range(n_pairs)]
+    kp = [k.shape[0] for k, _ in res]
+    return {"pairs": n_pairs, "hypotheses_per_pair": n_hyp, "detect_64_images_ms": round(t_detect, 3),
+            "keypoints_per_image": {"min": min(kp), "median": int(np.median(kp)), "max": max(kp)},
+            "matches": int(mp.npairs.sum().item()), "inliers": int(ninl.sum().item()),
+            "match_ransac_batched_ms": round(t_batched, 3), "match_ransac_per_pair_ms": round(t_single, 3),
+            "speedup": round(t_single / t_batched, 2), "batched_equals_per_pair": bool(same),
+            "corner_error_px": {"median": round(float(np.median(errs)), 3), "max": round(float(np.max(errs)), 3),
+                                "pairs_below_2px": int(np.sum(np.array(errs) < 2.0))}}
+
+
 def bench_grad(torch, dev, B=16, H=512, W=512, K=1024, iters=5):
     """16 float 512x512 images, 1024 fixed keypoints each, loss sum(W * desc):
     HIP forward and hand-written backward (pyramid recomputed), and one
@@ -181,7 +265,7 @@ def main():
                     help="(counter calibration) one 256 MiB device-to-device copy, then exit")
     ap.add_argument("--match-n", type=int, default=40000, help="descriptors per side for the match TOPS leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the match / stitch legs")
-    ap.add_argument("--only", choices=["match", "batch", "stitch", "grad"], default=None,
+    ap.add_argument("--only", choices=["match", "batch", "stitch", "grad", "pairs"], default=None,
                     help="(profiling) run a single extra leg and print its numbers")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -212,6 +296,8 @@ def main():
             print(json.dumps(bench_batch(torch, dev)))
         elif args.only == "grad":
             print(json.dumps(bench_grad(torch, dev)))
+        elif args.only == "pairs":
+            print(json.dumps(bench_pairs(torch, dev)))
         else:
             print(json.dumps(bench_stitch(va, vb)))
         return
@@ -278,6 +364,7 @@ def main():
             extra[f"match_{method}"] = {"n": args.match_n, "tops": round(tops, 2), "ms": round(ms, 4),
                                         "mfma_int8_peak_tops": PEAK_INT8_TOPS, "pairs": npairs}
         extra["batch64"] = bench_batch(torch, dev)
+        extra["pairs32"] = bench_pairs(torch, dev)
         extra["grad16x512"] = bench_grad(torch, dev)
         st_ms, met = bench_stitch(va, vb)
         extra["stitch_pair_ms"] = round(st_ms, 3)
