@@ -24,6 +24,11 @@ PEAK_INT8_TOPS = 5000.0
 REF_MS_PER_MP = 72.8 / (768 * 1024 / 1e6)   # reference RTX 3050, 768x1024
 
 
+# HBM bytes the pyramid kernels move per 1024x768 view (FETCH_SIZE + WRITE_SIZE
+# counter runs, scaled by a 256 MiB copy kernel): profiles/pyramid_1024x768_traffic.md
+PYRAMID_TRAFFIC_BYTES = 411_685_893
+
+
 def octave_pixels(H, W, n_oct, doubled):
     h, w = (2 * H, 2 * W) if doubled else (H, W)
     tot = 0
@@ -352,7 +357,7 @@ def main():
                                "(5 octaves, S=3, doubled, 8000 kp cap)", "images_per_step": len(views),
                    "keypoints": counts},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": PYRAMID_TRAFFIC_BYTES,
                      "kernel": "pyramid (k_ups_* + 26 x k_blur) per 1024x768 view",
                      "algorithmic_bytes": B, "avg_ms": round(t_pyr * 1e3, 4)},
         "phase_ms": {k: round(v, 4) for k, v in s._phase_timings.items()},

@@ -172,12 +172,13 @@ __global__ __launch_bounds__(256) void k_bblur(const BJob* __restrict__ jobs, in
     __shared__ __attribute__((aligned(16))) float sIn[BlurSmem<R, TH>::nIn];
     __shared__ __attribute__((aligned(16))) float sH[BlurSmem<R, TH>::nH];
     __shared__ int s_j;
+    const int tile = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     if (threadIdx.x < 64) {
         // the wave scans the job starts 64 at a time (no dependent loads)
         int last = -1;
         for (int b0 = 0; b0 < njobs; b0 += 64) {
             int k = b0 + (int)threadIdx.x;
-            bool le = k < njobs && jobs[k].tile0 <= (int)blockIdx.x;
+            bool le = k < njobs && jobs[k].tile0 <= tile;
             unsigned long long m = __ballot(le);
             if (m) last = b0 + 63 - __clzll(m);
             if (m != ~0ull) break;
@@ -187,7 +188,7 @@ __global__ __launch_bounds__(256) void k_bblur(const BJob* __restrict__ jobs, in
     __syncthreads();
     const int j = s_j;
     const BJob J = jobs[j];
-    const int t = blockIdx.x - J.tile0;
+    const int t = tile - J.tile0;
     const int ty = t / J.tiles_x, tx = t - ty * J.tiles_x;
     blur_tile<R, TH>((const float*)(ws + J.in_off), J.g_off == kNone ? nullptr : (float*)(ws + J.g_off),
                      J.d_off == kNone ? nullptr : (float*)(ws + J.d_off),

@@ -129,8 +129,9 @@ __global__ __launch_bounds__(256) void k_blur(const float* __restrict__ in, floa
     __shared__ __attribute__((aligned(16))) float sIn[BlurSmem<R, TH>::nIn];
     __shared__ __attribute__((aligned(16))) float sH[BlurSmem<R, TH>::nH];
     const size_t z = blockIdx.z;
+    const int t = xcd_remap((int)(blockIdx.x + blockIdx.y * gridDim.x), (int)(gridDim.x * gridDim.y));
     blur_tile<R, TH>(in + z * zin, gout ? gout + z * zout : nullptr, dout ? dout + z * zout : nullptr,
-                     dec ? dec + z * zout : nullptr, H, W, decW, taps, blockIdx.x, blockIdx.y, sIn, sH);
+                     dec ? dec + z * zout : nullptr, H, W, decW, taps, t % (int)gridDim.x, t / (int)gridDim.x, sIn, sH);
 }
 
 template <int R>

@@ -180,6 +180,17 @@ __device__ __forceinline__ void ups_interp_tile(const double* __restrict__ cf, i
     __syncthreads();
 }
 
+// ------------------------------------------------------------ placement --
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2:
+// linear block b runs on XCD b % 8.  xcd_remap hands every XCD a contiguous
+// range of logical tiles (a bijection of [0, n)), so the halo rows that
+// neighbouring tiles share are fetched into one L2 instead of eight.
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+    const int x = b & 7, i = b >> 3;
+    const int q = n >> 3, r = n & 7;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 // ------------------------------------------------------------------ blur --
 // One 64 x TH output tile of G[s+1] = blur(G[s]) (H pass then V pass, fused
 // multiply-add chains over the taps in ascending order), D[s] = G[s+1] - G[s]
