@@ -88,6 +88,43 @@ def bench_match(torch, dev, n, method, iters=5):
     return 2.0 * n * n * 128 / (ms * 1e-3) / 1e12, ms, int(out[3].item())
 
 
+def bench_match_f16(torch, dev, n, iters=5, sample=256):
+    """FP16-MFMA / FP32-accumulate matching of two n x 128 unit float
+    descriptor sets; TOPS counted once (2 n^2 128 / t) and the squared-distance
+    error of the returned neighbours against FP64 on `sample` rows."""
+    import numpy as np
+    from pysift_amd.matcher import match_f16, match_workspace_bytes
+    g = torch.Generator(device="cpu").manual_seed(5)
+    a = torch.rand((n, 128), generator=g) ** 3
+    a = a / a.norm(dim=1, keepdim=True)
+    b = a.clone()
+    b[: n // 2] = a[: n // 2] + 0.01 * torch.randn((n // 2, 128), generator=g)
+    b[n // 2:] = torch.rand((n - n // 2, 128), generator=g) ** 3
+    b = b / b.norm(dim=1, keepdim=True)
+    ta, tb = a.to(dev), b.to(dev)
+    ws = torch.empty(match_workspace_bytes(n, n), dtype=torch.uint8, device=dev)
+    fwd, _, pairs = match_f16(ta, tb, 0.75, ws=ws)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(iters):
+        fwd, _, pairs = match_f16(ta, tb, 0.75, ws=ws)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / iters
+    rows = np.random.default_rng(6).choice(n, sample, replace=False)
+    A = a[rows].double()
+    D = (A * A).sum(1)[:, None] + (b.double() ** 2).sum(1)[None, :] - 2 * A @ b.double().T
+    f = fwd.cpu().numpy()[rows]
+    d1 = f[:, 1].view(np.float32).astype(np.float64)
+    Dn = D.numpy()
+    err = np.abs(d1 - Dn[np.arange(sample), f[:, 0]])
+    same = float(np.mean(f[:, 0] == Dn.argmin(1)))
+    return {"n": n, "tops": round(2.0 * n * n * 128 / (ms * 1e-3) / 1e12, 2), "ms": round(ms, 4),
+            "pairs": int(pairs.shape[0]), "d2_abs_err_vs_fp64": {"max": float(err.max()), "mean": float(err.mean())},
+            "nn_equal_fp64": same, "sample_rows": sample}
+
+
 def bench_stitch(va, vb, iters=5):
     from pysift_amd import GPUPyStitch
     st = GPUPyStitch(inlier_thresh=3.0, ransac_iters=2000, verbose=False, save_output=False)
@@ -296,7 +333,9 @@ def main():
     va, vb, Hgt = syn.synth_pair(seed=1000)
     if args.only:
         if args.only == "match":
-            print(json.dumps({m: bench_match(torch, dev, args.match_n, m) for m in ("hilo", "shift")}))
+            out = {m: bench_match(torch, dev, args.match_n, m) for m in ("hilo", "shift")}
+            out["f16"] = bench_match_f16(torch, dev, args.match_n)
+            print(json.dumps(out))
         elif args.only == "batch":
             print(json.dumps(bench_batch(torch, dev)))
         elif args.only == "grad":
@@ -368,6 +407,7 @@ def main():
             tops, ms, npairs = bench_match(torch, dev, args.match_n, method)
             extra[f"match_{method}"] = {"n": args.match_n, "tops": round(tops, 2), "ms": round(ms, 4),
                                         "mfma_int8_peak_tops": PEAK_INT8_TOPS, "pairs": npairs}
+        extra["match_f16"] = bench_match_f16(torch, dev, args.match_n)
         extra["batch64"] = bench_batch(torch, dev)
         extra["pairs32"] = bench_pairs(torch, dev)
         extra["grad16x512"] = bench_grad(torch, dev)

@@ -161,9 +161,13 @@ int msift_match_batch_gather_points(msift_match_batch* plan, const float* d_kpa,
                                     int stride_b, const int32_t* d_pairs, const int* d_npairs, double* d_p1,
                                     double* d_p2, void* stream);
 
-/* Float descriptor path: d2 = sum (a-b)^2 via FP16 MFMA with FP32
- * accumulation of a.b on fp16-rounded inputs; same outputs (distances as
- * float bits in the int32 slots). */
+/* Float descriptor path (the reference's approximate float matcher, core.py:
+ * 2574-2743, 2840-2940, as an FP16 GEMM): inputs rounded to fp16, a.b on FP16
+ * MFMA with FP32 accumulation, norms in FP32 from the rounded values,
+ * d2 = |a|^2 + |b|^2 - 2 a.b clamped to [0, 4] (unit-norm descriptors) and
+ * quantised to 2^-21 for the top-2 keys.  Same outputs as msift_match_u8 with
+ * the distance slots holding float bits (+inf = none).  Workspace from
+ * msift_match_workspace_bytes. */
 int msift_match_f16(const float* d_a, int N, const float* d_b, int M, double ratio,
                     void* d_ws, size_t ws_bytes, int32_t* d_fwd, int32_t* d_bwd,
                     int32_t* d_pairs, int* d_npairs, void* stream);

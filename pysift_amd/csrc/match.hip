@@ -190,6 +190,147 @@ __global__ __launch_bounds__(512) void k_match_tiles(const uint8_t* __restrict__
     match_tile<METHOD>(A, N, B, M, na, nb, rowpart, colpart, blockIdx.x, blockIdx.y, colk, s_na);
 }
 
+// ---------------------------------------------------------------- float ----
+// Float descriptor path: inputs rounded to fp16, a.b on v_mfma_f32_32x32x16_f16
+// with fp32 accumulation, norms in fp32 from the rounded values.  For unit
+// descriptors d = |a|^2 + |b|^2 - 2 a.b lies in [0, 4]; it is clamped there and
+// quantised to q = round(d 2^21) < 2^23, so the exact path's 32-bit top-2 keys
+// (q << 9 | j), reduce and mutual kernels apply unchanged (the quantum 4.8e-7
+// is far below the fp16 rounding of the product).
+constexpr float kF16Q = 2097152.0f;   // 2^21
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+__global__ __launch_bounds__(256) void k_to_f16(const float* __restrict__ in, size_t n, _Float16* __restrict__ out) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        out[i] = (_Float16)in[i];
+}
+
+// squared norms of fp16 rows (128 halves), fp32, lane pairs + xor tree
+__global__ __launch_bounds__(256) void k_norms_f16(const _Float16* __restrict__ a, int n, float* __restrict__ out) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= n) return;
+    const float v0 = (float)a[(size_t)i * 128 + 2 * lane], v1 = (float)a[(size_t)i * 128 + 2 * lane + 1];
+    float s = v0 * v0 + v1 * v1;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) s = s + __shfl_xor(s, m);
+    if (lane == 0) out[i] = s;
+}
+
+__device__ __forceinline__ uint32_t f16_q(float d) {
+    d = d < 0.0f ? 0.0f : (d > 4.0f ? 4.0f : d);
+    uint32_t q = (uint32_t)(d * kF16Q + 0.5f);
+    return q > 0x7fffffu ? 0x7fffffu : q;
+}
+
+__global__ __launch_bounds__(512) void k_match_tiles_f16(const _Float16* __restrict__ A, int N,
+                                                          const _Float16* __restrict__ B, int M,
+                                                          const float* __restrict__ na, const float* __restrict__ nb,
+                                                          uint32_t* __restrict__ rowpart, uint32_t* __restrict__ colpart) {
+    __shared__ uint32_t colk[8][64][2];
+    __shared__ float s_na[kRowBlk];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int l32 = lane & 31, hi = lane >> 5;
+    const int rb = blockIdx.x, cc = blockIdx.y;
+    const int row0 = rb * kRowBlk + w * 32;
+    if (threadIdx.x < kRowBlk) {
+        int i = rb * kRowBlk + threadIdx.x;
+        s_na[threadIdx.x] = i < N ? na[i] : 0.0f;
+    }
+    // A fragment: lane holds row (row0 + l32), halves [16 ks + 8 hi, +8) for ks = 0..7
+    v8h af[8];
+    {
+        int i = row0 + l32;
+        int ic = i < N ? i : N - 1;
+        const v8h* src = reinterpret_cast<const v8h*>(A + (size_t)ic * 128);
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) af[ks] = src[2 * ks + hi];
+    }
+    uint32_t rk1[16], rk2[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) { rk1[r] = 0xffffffffu; rk2[r] = 0xffffffffu; }
+    __syncthreads();
+    for (int ct = 0; ct < kColChunk / 64; ++ct) {
+        const int col0 = cc * kColChunk + ct * 64;
+        if (col0 >= M) break;
+        v16f acc[2];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            int j = col0 + 32 * b + l32;
+            int jc = j < M ? j : M - 1;
+            const v8h* src = reinterpret_cast<const v8h*>(B + (size_t)jc * 128);
+            acc[b] = (v16f)0.0f;
+#pragma unroll
+            for (int ks = 0; ks < 8; ++ks)
+                acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af[ks], src[2 * ks + hi], acc[b], 0, 0, 0);
+        }
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int j = col0 + 32 * b + l32;
+            const bool jv = j < M;
+            const float nbj = jv ? nb[j] : 0.0f;
+            const uint32_t lj = (uint32_t)(ct * 64 + 32 * b + l32);
+            uint32_t c1 = 0xffffffffu, c2 = 0xffffffffu;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int il = 32 * w + 8 * (r >> 2) + 4 * hi + (r & 3);
+                const bool v = jv && (rb * kRowBlk + il < N);
+                const uint32_t q = f16_q((s_na[il] + nbj) - 2.0f * acc[b][r]);
+                const uint32_t kc = v ? ((q << 8) | (uint32_t)il) : 0xffffffffu;
+                top2_insert(c1, c2, kc);
+                const uint32_t kr = v ? ((q << 9) | lj) : 0xffffffffu;
+                top2_insert(rk1[r], rk2[r], kr);
+            }
+            uint32_t o1 = __shfl_xor(c1, 32), o2 = __shfl_xor(c2, 32);
+            top2_merge(c1, c2, o1, o2);
+            if (hi == 0) {
+                colk[w][32 * b + l32][0] = c1;
+                colk[w][32 * b + l32][1] = c2;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const int c = threadIdx.x, j = col0 + c;
+            uint32_t k1 = colk[0][c][0], k2 = colk[0][c][1];
+#pragma unroll
+            for (int ww = 1; ww < 8; ++ww) top2_merge(k1, k2, colk[ww][c][0], colk[ww][c][1]);
+            if (j < M) {
+                colpart[((size_t)rb * M + j) * 2] = k1;
+                colpart[((size_t)rb * M + j) * 2 + 1] = k2;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        uint32_t k1 = rk1[r], k2 = rk2[r];
+#pragma unroll
+        for (int m = 1; m < 32; m <<= 1) {
+            uint32_t o1 = __shfl_xor(k1, m), o2 = __shfl_xor(k2, m);
+            top2_merge(k1, k2, o1, o2);
+        }
+        const int i = row0 + 8 * (r >> 2) + 4 * hi + (r & 3);
+        if (l32 == 0 && i < N) {
+            rowpart[((size_t)cc * N + i) * 2] = k1;
+            rowpart[((size_t)cc * N + i) * 2 + 1] = k2;
+        }
+    }
+}
+
+// distance slots of the top-2 triples: quantised q -> float bits q / 2^21
+// ("none", 0x7fffffff, -> +inf)
+__global__ __launch_bounds__(256) void k_q_to_float(int32_t* __restrict__ t, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int k = 1; k <= 2; ++k) {
+        const int32_t q = t[3 * i + k];
+        const float d = q == 0x7fffffff ? __builtin_huge_valf() : (float)q * (1.0f / kF16Q);
+        t[3 * i + k] = __builtin_bit_cast(int32_t, d);
+    }
+}
+
 // ------------------------------------------------------------- batched ----
 // Pairs (a_p, b_p) of descriptor sets packed back to back; one launch covers
 // every tile of every pair, a block finds its pair in the table by a ballot
@@ -383,7 +524,7 @@ __global__ __launch_bounds__(256) void k_gather_points_b(const MPair* __restrict
 // ---------------------------------------------------------------- host ----
 struct MatchPlan {
     int nrb_a, ncc_a;   // row blocks of a, column chunks of b
-    size_t o_na, o_nb, o_rowpart, o_colpart, o_cnt, o_off, total;
+    size_t o_na, o_nb, o_rowpart, o_colpart, o_cnt, o_off, o_ha, o_hb, total;
 };
 
 static MatchPlan match_plan(int N, int M) {
@@ -399,6 +540,9 @@ static MatchPlan match_plan(int N, int M) {
     int nblk = (N + 255) / 256;
     P.o_cnt = take((size_t)(nblk + 1) * 4);
     P.o_off = take((size_t)(nblk + 1) * 4);
+    // fp16 copies for the float path (the integer path leaves them unused)
+    P.o_ha = take((size_t)(N > 0 ? N : 1) * 256);
+    P.o_hb = take((size_t)(M > 0 ? M : 1) * 256);
     P.total = off;
     return P;
 }
@@ -569,10 +713,47 @@ static int match_batch_gather(msift_match_batch* B, const float* kpa, int sa, co
     return MSIFT_OK;
 }
 
-int match_f16(const float*, int, const float*, int, double, void*, size_t, int32_t*, int32_t*, int32_t*, int*,
-              hipStream_t) {
-    set_error("match_f16: float descriptor path is not built");
-    return MSIFT_ESTATE;
+int match_f16(const float* a, int N, const float* b, int M, double ratio, void* ws_, size_t ws_bytes, int32_t* fwd,
+              int32_t* bwd, int32_t* pairs, int* npairs, hipStream_t s) {
+    MatchPlan P = match_plan(N, M);
+    if (ws_bytes < P.total) { set_error("match_f16: workspace too small"); return MSIFT_EWORKSPACE; }
+    if ((N > 0 && (!a || !fwd)) || (M > 0 && (!b || !bwd)) || (!pairs && N > 0 && M > 0)) {
+        set_error("match_f16: null buffer");
+        return MSIFT_EINVAL;
+    }
+    char* ws = (char*)ws_;
+    float* na = (float*)(ws + P.o_na);
+    float* nb = (float*)(ws + P.o_nb);
+    _Float16* ha = (_Float16*)(ws + P.o_ha);
+    _Float16* hb = (_Float16*)(ws + P.o_hb);
+    uint32_t* rowpart = (uint32_t*)(ws + P.o_rowpart);
+    uint32_t* colpart = (uint32_t*)(ws + P.o_colpart);
+    if (N > 0 && M > 0) {
+        const size_t ea = (size_t)N * 128, eb = (size_t)M * 128;
+        hipLaunchKernelGGL(k_to_f16, dim3((unsigned)std::min<size_t>((ea + 255) / 256, 16384)), dim3(256), 0, s, a, ea, ha);
+        hipLaunchKernelGGL(k_to_f16, dim3((unsigned)std::min<size_t>((eb + 255) / 256, 16384)), dim3(256), 0, s, b, eb, hb);
+        hipLaunchKernelGGL(k_norms_f16, dim3((N + 3) / 4), dim3(256), 0, s, (const _Float16*)ha, N, na);
+        hipLaunchKernelGGL(k_norms_f16, dim3((M + 3) / 4), dim3(256), 0, s, (const _Float16*)hb, M, nb);
+        hipLaunchKernelGGL(k_match_tiles_f16, dim3(P.nrb_a, P.ncc_a), dim3(512), 0, s, (const _Float16*)ha, N,
+                           (const _Float16*)hb, M, (const float*)na, (const float*)nb, rowpart, colpart);
+        hipLaunchKernelGGL(k_reduce_top2, dim3((N + 255) / 256), dim3(256), 0, s, rowpart, N, P.ncc_a, 9, kColChunk, fwd);
+        hipLaunchKernelGGL(k_reduce_top2, dim3((M + 255) / 256), dim3(256), 0, s, colpart, M, P.nrb_a, 8, kRowBlk, bwd);
+    } else {
+        if (N > 0) hipLaunchKernelGGL(k_fill_top2_none, dim3((N + 255) / 256), dim3(256), 0, s, fwd, N);
+        if (M > 0) hipLaunchKernelGGL(k_fill_top2_none, dim3((M + 255) / 256), dim3(256), 0, s, bwd, M);
+    }
+    // ratio and mutual tests on the quantised distances, then float bits
+    const int nblk = (N + 255) / 256;
+    const double r2 = ratio * ratio;
+    int* cnt = (int*)(ws + P.o_cnt);
+    int* off = (int*)(ws + P.o_off);
+    if (nblk > 0) hipLaunchKernelGGL(k_mutual_count, dim3(nblk), dim3(256), 0, s, fwd, bwd, N, M, r2, cnt);
+    launch_scan(cnt, off, nblk, npairs, nullptr, nullptr, 0, s);
+    if (nblk > 0) hipLaunchKernelGGL(k_mutual_write, dim3(nblk), dim3(256), 0, s, fwd, bwd, N, M, r2, off, pairs);
+    if (N > 0) hipLaunchKernelGGL(k_q_to_float, dim3((N + 255) / 256), dim3(256), 0, s, fwd, N);
+    if (M > 0) hipLaunchKernelGGL(k_q_to_float, dim3((M + 255) / 256), dim3(256), 0, s, bwd, M);
+    MSIFT_LAUNCH_CHECK();
+    return MSIFT_OK;
 }
 
 }  // namespace msift

@@ -44,6 +44,33 @@ def match_u8(a, b, ratio=0.75, method="hilo", sync=True, ws=None):
     return fwd[:N], bwd[:M], pairs[:k]
 
 
+def match_f16(a, b, ratio=0.75, ws=None):
+    """Float descriptors a: (N,128), b: (M,128) float32 CUDA tensors (unit
+    norm, e.g. RootSIFT) matched on FP16 MFMA with FP32 accumulation.
+    Returns (fwd (N,3), bwd (M,3), pairs (K,2)); the distance columns are the
+    float32 squared distances viewed as int32 (use .view(torch.float32))."""
+    import torch
+    if a.dtype != torch.float32 or b.dtype != torch.float32 or a.dim() != 2 or b.dim() != 2 \
+            or a.shape[1] != 128 or b.shape[1] != 128:
+        raise ValueError("descriptors must be (N,128) float32")
+    a = a.contiguous(); b = b.contiguous()
+    dev = a.device
+    N, M = a.shape[0], b.shape[0]
+    need = match_workspace_bytes(N, M)
+    if ws is None or ws.numel() < need:
+        ws = torch.empty(max(need, 1), dtype=torch.uint8, device=dev)
+    fwd = torch.empty((max(N, 1), 3), dtype=torch.int32, device=dev)
+    bwd = torch.empty((max(M, 1), 3), dtype=torch.int32, device=dev)
+    pairs = torch.empty((max(min(N, M), 1), 2), dtype=torch.int32, device=dev)
+    npairs = torch.zeros(1, dtype=torch.int32, device=dev)
+    rc = _lib.lib().msift_match_f16(_lib.ptr(a), N, _lib.ptr(b), M, float(ratio), _lib.ptr(ws), ws.numel(),
+                                    _lib.ptr(fwd), _lib.ptr(bwd), _lib.ptr(pairs), _lib.ptr(npairs),
+                                    _lib.stream_ptr(dev))
+    _lib.check(rc, "match_f16")
+    k = int(npairs.item())
+    return fwd[:N], bwd[:M], pairs[:k]
+
+
 class MatchBatchPlan:
     """Plan of a batched match over pairs (a_p: N[p] x 128, b_p: M[p] x 128
     uint8): one launch over every tile of every pair; per pair the results
