@@ -259,6 +259,111 @@ def bench_pairs(torch, dev, n_pairs=32, n_hyp=1024, iters=5):
                                 "pairs_below_2px": int(np.sum(np.array(errs) < 2.0))}}
 
 
+def bench_config4(torch, dev, iters=3, check=True):
+    """4000x3000 grayscale pair (the second warped by a known homography):
+    detect with lifted caps (40 000 keypoints, 200 000 candidates per octave),
+    exact INT8 match (checked against the integer CPU oracle), the FP16 float
+    path with its error against FP64, 16 384-hypothesis FP64 RANSAC and the
+    feather-blended panorama."""
+    import numpy as np
+    from pysift_amd import GPUPyStitch, PySIFT
+    from pysift_amd import synthetic as syn
+    from pysift_amd.matcher import match_f16, match_u8, match_workspace_bytes
+    from pysift_amd.ransac import ransac_homography
+    from pysift_amd.stitch import compute_canvas_size
+    a, b, Hgt = syn.synth_warped_pair(3000, 4000, 4000)
+    H, W = a.shape
+    s = PySIFT(max_keypoints=40000, max_cands_per_octave=200000, device=dev)
+    ta, tb = torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev)
+    outs = s.new_outputs(dev)
+    s.launch(ta, outs, profile=True)
+    torch.cuda.synchronize()
+    pyr = []
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(iters):
+        s.launch(ta, outs)
+    e1.record()
+    torch.cuda.synchronize()
+    t_det = e0.elapsed_time(e1) / iters
+    for _ in range(iters):
+        s.launch(ta, outs, profile=True)
+        pyr.append(s._phase_timings["gauss_pyr_ms"])
+    t_pyr = float(np.median(pyr))
+    kpa, dfa, dua = s.detect_raw(ta)
+    kpb, dfb, dub = s.detect_raw(tb)
+    N, M = kpa.shape[0], kpb.shape[0]
+    ws = torch.empty(match_workspace_bytes(N, M), dtype=torch.uint8, device=dev)
+    res = {"image": [H, W], "keypoints": [N, M], "detect_ms": round(t_det, 3), "ms_per_mp": round(t_det / (H * W / 1e6), 4),
+           "pyramid_ms": round(t_pyr, 4)}
+    Bp = pyramid_bytes(H, W, 5)
+    res["pyramid_roofline"] = {"algorithmic_bytes": Bp, "achieved_gbs": round(Bp / (t_pyr * 1e-3) / 1e9, 1),
+                               "frac": round(Bp / (t_pyr * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+    for method in ("hilo", "shift"):
+        match_u8(dua, dub, 0.75, method, sync=False, ws=ws)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(iters):
+            fwd, bwd, pairs, npairs = match_u8(dua, dub, 0.75, method, sync=False, ws=ws)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / iters
+        res[f"match_{method}"] = {"ms": round(ms, 3), "tops": round(2.0 * N * M * 128 / (ms * 1e-3) / 1e12, 2),
+                                  "pairs": int(npairs.item())}
+    fwd, bwd, pairs = match_u8(dua, dub, 0.75, "hilo", ws=ws)
+    if check:
+        from oracle import oracle as O
+        O.set_threads(min(os.cpu_count() or 8, 16))
+        t0 = time.perf_counter()
+        rf, rb, rp = O.match_u8(dua.cpu().numpy(), dub.cpu().numpy(), 0.75)
+        res["match_exact_vs_cpu_oracle"] = bool(np.array_equal(fwd.cpu().numpy(), rf) and
+                                                np.array_equal(bwd.cpu().numpy(), rb) and
+                                                np.array_equal(pairs.cpu().numpy(), rp))
+        res["cpu_oracle_match_s"] = round(time.perf_counter() - t0, 2)
+    f16 = match_f16(dfa, dfb, 0.75, ws=ws)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(iters):
+        f16 = match_f16(dfa, dfb, 0.75, ws=ws)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / iters
+    rows = np.random.default_rng(8).choice(N, min(256, N), replace=False)
+    A64 = dfa[rows].double().cpu()
+    B64 = dfb.double().cpu()
+    D = ((A64 * A64).sum(1)[:, None] + (B64 * B64).sum(1)[None, :] - 2 * A64 @ B64.T).numpy()
+    ff = f16[0].cpu().numpy()[rows]
+    err = np.abs(ff[:, 1].view(np.float32) - D[np.arange(len(rows)), ff[:, 0]])
+    res["match_f16"] = {"ms": round(ms, 3), "tops": round(2.0 * N * M * 128 / (ms * 1e-3) / 1e12, 2),
+                        "pairs": int(f16[2].shape[0]), "d2_abs_err_vs_fp64_max": float(err.max()),
+                        "nn_equal_fp64": float(np.mean(ff[:, 0] == D.argmin(1)))}
+    p1 = kpa[pairs[:, 0].long(), :2].double().contiguous()
+    p2 = kpb[pairs[:, 1].long(), :2].double().contiguous()
+    ransac_homography(p1, p2, seed=42, n_hyp=16384, thr=3.0, return_device=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(iters):
+        Hd, mask, ninl, best, _ = ransac_homography(p1, p2, seed=42, n_hyp=16384, thr=3.0, return_device=True)
+    e1.record()
+    torch.cuda.synchronize()
+    Hn = Hd.cpu().numpy()
+    res["ransac_16384"] = {"ms": round(e0.elapsed_time(e1) / iters, 3), "matches": int(pairs.shape[0]),
+                           "inliers": int(ninl.item()), "corner_error_px": round(syn.corner_error(Hn, Hgt, W, H), 3)}
+    st = GPUPyStitch(save_output=False, verbose=False)
+    b1 = torch.from_numpy(np.repeat(a[..., None], 3, 2)).to(dev)
+    b2 = torch.from_numpy(np.repeat(b[..., None], 3, 2)).to(dev)
+    cw, ch, ox, oy = compute_canvas_size(H, W, H, W, Hn)
+    st.warp_feather(b1, b2, Hn, cw, ch, ox, oy)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(iters):
+        pano, valid = st.warp_feather(b1, b2, Hn, cw, ch, ox, oy)
+    e1.record()
+    torch.cuda.synchronize()
+    res["panorama"] = {"ms": round(e0.elapsed_time(e1) / iters, 3), "canvas": [ch, cw]}
+    return res
+
+
 def bench_grad(torch, dev, B=16, H=512, W=512, K=1024, iters=5):
     """16 float 512x512 images, 1024 fixed keypoints each, loss sum(W * desc):
     HIP forward and hand-written backward (pyramid recomputed), and one
@@ -307,7 +412,7 @@ def main():
                     help="(counter calibration) one 256 MiB device-to-device copy, then exit")
     ap.add_argument("--match-n", type=int, default=40000, help="descriptors per side for the match TOPS leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the match / stitch legs")
-    ap.add_argument("--only", choices=["match", "batch", "stitch", "grad", "pairs"], default=None,
+    ap.add_argument("--only", choices=["match", "batch", "stitch", "grad", "pairs", "config4"], default=None,
                     help="(profiling) run a single extra leg and print its numbers")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -342,6 +447,8 @@ def main():
             print(json.dumps(bench_grad(torch, dev)))
         elif args.only == "pairs":
             print(json.dumps(bench_pairs(torch, dev)))
+        elif args.only == "config4":
+            print(json.dumps(bench_config4(torch, dev)))
         else:
             print(json.dumps(bench_stitch(va, vb)))
         return

@@ -60,18 +60,32 @@ __global__ __launch_bounds__(256) void k_norms(const uint8_t* __restrict__ a, in
 }
 
 // one 256-row x 512-column tile (row block rb, column chunk cc) of a
-// workgroup of 8 waves; colk / s_na are the workgroup's LDS
+// workgroup of 8 waves.  Each 64-column step of b is staged once in LDS
+// (rows padded to 144 B: conflict-free 16-byte reads), double-buffered: the
+// next step's global load is issued before the current step's MFMA and top-2
+// work.  Per-wave column partials stay in LDS for the whole chunk and are
+// merged once at the end.  Results do not depend on this schedule (unique
+// keys).
+constexpr int kBStride = 144;                     // bytes per staged column
+constexpr int kBTile = 64 * kBStride;             // one 64-column step
+
+struct MatchSmem {
+    uint8_t bt[2][kBTile];                        // staged b columns, double buffer
+    uint32_t colk[8][kColChunk][2];               // per-wave column top-2 for the chunk
+    int s_na[kRowBlk];
+};
+
 template <int METHOD>
 __device__ __forceinline__ void match_tile(const uint8_t* __restrict__ A, int N, const uint8_t* __restrict__ B, int M,
                                            const int* __restrict__ na, const int* __restrict__ nb,
                                            uint32_t* __restrict__ rowpart, uint32_t* __restrict__ colpart, int rb,
-                                           int cc, uint32_t (*colk)[64][2], int* s_na) {
+                                           int cc, MatchSmem& sm) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int l32 = lane & 31, hi = lane >> 5;
     const int row0 = rb * kRowBlk + w * 32;     // this wave's 32 rows
     if (threadIdx.x < kRowBlk) {
         int i = rb * kRowBlk + threadIdx.x;
-        s_na[threadIdx.x] = i < N ? na[i] : 0;
+        sm.s_na[threadIdx.x] = i < N ? na[i] : 0;
     }
     // A fragment: lane holds row (row0 + l32), bytes [32 ks + 16 hi, +16) for ks = 0..3
     v4i al[4], ah[4];
@@ -93,22 +107,34 @@ __device__ __forceinline__ void match_tile(const uint8_t* __restrict__ A, int N,
     uint32_t rk1[16], rk2[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) { rk1[r] = 0xffffffffu; rk2[r] = 0xffffffffu; }
+    // staging of step ct: thread t moves bytes [16 (t % 8), +16) of column t / 8
+    const int n_ct = min(kColChunk / 64, (M - cc * kColChunk + 63) / 64);
+    const int sc = threadIdx.x >> 3, sq = threadIdx.x & 7;
+    auto gload = [&](int ct) -> v4i {
+        int j = cc * kColChunk + ct * 64 + sc;
+        j = j < M ? j : M - 1;
+        return *reinterpret_cast<const v4i*>(B + (size_t)j * 128 + 16 * sq);
+    };
+    auto lstore = [&](int buf, v4i v) {
+        *reinterpret_cast<v4i*>(&sm.bt[buf][sc * kBStride + 16 * sq]) = v;
+    };
+    lstore(0, gload(0));
     __syncthreads();
-
-    for (int ct = 0; ct < kColChunk / 64; ++ct) {
+    for (int ct = 0; ct < n_ct; ++ct) {
         const int col0 = cc * kColChunk + ct * 64;
-        if (col0 >= M) break;
+        const bool more = ct + 1 < n_ct;
+        v4i nxt;
+        if (more) nxt = gload(ct + 1);
+        const uint8_t* bt = sm.bt[ct & 1];
         v16i acc[2], accx[2];
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
-            int j = col0 + 32 * b + l32;
-            int jc = j < M ? j : M - 1;
-            const v4i* src = reinterpret_cast<const v4i*>(B + (size_t)jc * 128);
+            const uint8_t* src = bt + (32 * b + l32) * kBStride + 16 * hi;
             acc[b] = (v16i)0;
             if (METHOD == 0) accx[b] = (v16i)0;
 #pragma unroll
             for (int ks = 0; ks < 4; ++ks) {
-                v4i v = src[2 * ks + hi];
+                v4i v = *reinterpret_cast<const v4i*>(src + 32 * ks);
                 if (METHOD == 1) {
                     acc[b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(al[ks], v ^ (v4i)(int)0x80808080, acc[b], 0, 0, 0);
                 } else {
@@ -138,7 +164,7 @@ __device__ __forceinline__ void match_tile(const uint8_t* __restrict__ A, int N,
                 const bool v = jv && (rb * kRowBlk + il < N);
                 int dot = acc[b][r];
                 if (METHOD == 0) dot = dot + 128 * accx[b][r];
-                const int d = (s_na[il] + nbj) - 2 * dot;
+                const int d = (sm.s_na[il] + nbj) - 2 * dot;
                 const uint32_t kc = v ? (((uint32_t)d << 8) | (uint32_t)il) : 0xffffffffu;
                 top2_insert(c1, c2, kc);
                 const uint32_t kr = v ? (((uint32_t)d << 9) | lj) : 0xffffffffu;
@@ -147,22 +173,23 @@ __device__ __forceinline__ void match_tile(const uint8_t* __restrict__ A, int N,
             uint32_t o1 = __shfl_xor(c1, 32), o2 = __shfl_xor(c2, 32);
             top2_merge(c1, c2, o1, o2);
             if (hi == 0) {
-                colk[w][32 * b + l32][0] = c1;
-                colk[w][32 * b + l32][1] = c2;
+                sm.colk[w][ct * 64 + 32 * b + l32][0] = c1;
+                sm.colk[w][ct * 64 + 32 * b + l32][1] = c2;
             }
         }
+        if (more) lstore((ct + 1) & 1, nxt);
         __syncthreads();
-        if (threadIdx.x < 64) {
-            const int c = threadIdx.x, j = col0 + c;
-            uint32_t k1 = colk[0][c][0], k2 = colk[0][c][1];
+    }
+    // columns: one merge over the 8 waves per column of the chunk
+    {
+        const int c = threadIdx.x, j = cc * kColChunk + c;
+        if (c < n_ct * 64 && j < M) {
+            uint32_t k1 = sm.colk[0][c][0], k2 = sm.colk[0][c][1];
 #pragma unroll
-            for (int ww = 1; ww < 8; ++ww) top2_merge(k1, k2, colk[ww][c][0], colk[ww][c][1]);
-            if (j < M) {
-                colpart[((size_t)rb * M + j) * 2] = k1;
-                colpart[((size_t)rb * M + j) * 2 + 1] = k2;
-            }
+            for (int ww = 1; ww < 8; ++ww) top2_merge(k1, k2, sm.colk[ww][c][0], sm.colk[ww][c][1]);
+            colpart[((size_t)rb * M + j) * 2] = k1;
+            colpart[((size_t)rb * M + j) * 2 + 1] = k2;
         }
-        __syncthreads();
     }
     // rows: merge across the 32 lanes that share each row slot
 #pragma unroll
@@ -185,9 +212,8 @@ template <int METHOD>
 __global__ __launch_bounds__(512) void k_match_tiles(const uint8_t* __restrict__ A, int N, const uint8_t* __restrict__ B,
                                                       int M, const int* __restrict__ na, const int* __restrict__ nb,
                                                       uint32_t* __restrict__ rowpart, uint32_t* __restrict__ colpart) {
-    __shared__ uint32_t colk[8][64][2];
-    __shared__ int s_na[kRowBlk];
-    match_tile<METHOD>(A, N, B, M, na, nb, rowpart, colpart, blockIdx.x, blockIdx.y, colk, s_na);
+    __shared__ MatchSmem sm;
+    match_tile<METHOD>(A, N, B, M, na, nb, rowpart, colpart, blockIdx.x, blockIdx.y, sm);
 }
 
 // ---------------------------------------------------------------- float ----
@@ -359,8 +385,7 @@ __global__ __launch_bounds__(512) void k_match_tiles_b(const MPair* __restrict__
                                                         const uint8_t* __restrict__ A, const uint8_t* __restrict__ B,
                                                         const int* __restrict__ na, const int* __restrict__ nb,
                                                         char* __restrict__ ws) {
-    __shared__ uint32_t colk[8][64][2];
-    __shared__ int s_na[kRowBlk];
+    __shared__ MatchSmem sm;
     __shared__ int s_p;
     if (threadIdx.x < 64) {
         const int p = pair_of_tile(pairs, np, (int)blockIdx.x);
@@ -370,7 +395,7 @@ __global__ __launch_bounds__(512) void k_match_tiles_b(const MPair* __restrict__
     const MPair P = pairs[s_p];
     const int t = (int)blockIdx.x - P.tile0;
     match_tile<METHOD>(A + (size_t)P.a_off * 128, P.N, B + (size_t)P.b_off * 128, P.M, na + P.a_off, nb + P.b_off,
-                       (uint32_t*)(ws + P.o_rowpart), (uint32_t*)(ws + P.o_colpart), t % P.nrb, t / P.nrb, colk, s_na);
+                       (uint32_t*)(ws + P.o_rowpart), (uint32_t*)(ws + P.o_colpart), t % P.nrb, t / P.nrb, sm);
 }
 
 // merge per-chunk partials into (best index, d1, d2); key 0xffffffff = none

@@ -168,32 +168,36 @@ def random_keypoints(B, K, H, W, n_oct=5, S=3, seed=3100, margin=2, max_octave=N
     return out
 
 
-def synth_pairs_batch(n_pairs=32, seed=2100):
-    """n_pairs (A, B, H_gt) uint8 grayscale pairs at the uneven batch sizes:
-    A = synth_gray, B = A warped about its centre by a random homography
-    (rotation +-15 deg, scale 0.8-1.25, perspective ~U(-2e-4, 2e-4), shift
-    ~U(-10 %, 10 %) of the size), bilinear, zeros outside; H_gt maps A pixel
-    coordinates to B pixel coordinates."""
-    sizes = synth_batch_sizes(2 * n_pairs, seed)
-    out = []
-    for p in range(n_pairs):
-        h, w = sizes[2 * p]
-        rng = np.random.default_rng(seed + 7 * p)
-        a = synth_gray(h, w, seed + 1000 + p, as_float=True)
-        th = math.radians(rng.uniform(-15, 15))
-        sc = rng.uniform(0.8, 1.25)
-        px, py = rng.uniform(-2e-4, 2e-4, 2)
-        tx, ty = rng.uniform(-0.1, 0.1) * w, rng.uniform(-0.1, 0.1) * h
-        T0 = np.array([[1, 0, -w / 2.0], [0, 1, -h / 2.0], [0, 0, 1]], float)
-        P = np.array([[1, 0, 0], [0, 1, 0], [px, py, 1]], float)
-        R = np.array([[sc * math.cos(th), -sc * math.sin(th), 0], [sc * math.sin(th), sc * math.cos(th), 0], [0, 0, 1]])
-        T1 = np.array([[1, 0, w / 2.0 + tx], [0, 1, h / 2.0 + ty], [0, 0, 1]], float)
-        G = T1 @ R @ P @ T0                       # B pixel -> A pixel
-        ys, xs = np.mgrid[0:h, 0:w].astype(np.float64)
+def synth_warped_pair(h, w, seed, content_seed=None):
+    """(A, B, H_gt) uint8 grayscale: A = synth_gray(h, w), B = A warped about
+    its centre by a random homography (rotation +-15 deg, scale 0.8-1.25,
+    perspective ~U(-2e-4, 2e-4) scaled to the image size, shift ~U(-10 %, 10 %)),
+    bilinear, zeros outside; H_gt maps A pixel coordinates to B's."""
+    rng = np.random.default_rng(seed)
+    a = synth_gray(h, w, seed + 1000 if content_seed is None else content_seed, as_float=True)
+    th = math.radians(rng.uniform(-15, 15))
+    sc = rng.uniform(0.8, 1.25)
+    px, py = rng.uniform(-2e-4, 2e-4, 2) * (1024.0 / max(h, w))
+    tx, ty = rng.uniform(-0.1, 0.1) * w, rng.uniform(-0.1, 0.1) * h
+    T0 = np.array([[1, 0, -w / 2.0], [0, 1, -h / 2.0], [0, 0, 1]], float)
+    P = np.array([[1, 0, 0], [0, 1, 0], [px, py, 1]], float)
+    R = np.array([[sc * math.cos(th), -sc * math.sin(th), 0], [sc * math.sin(th), sc * math.cos(th), 0], [0, 0, 1]])
+    T1 = np.array([[1, 0, w / 2.0 + tx], [0, 1, h / 2.0 + ty], [0, 0, 1]], float)
+    G = T1 @ R @ P @ T0                       # B pixel -> A pixel
+    b = np.empty((h, w), np.float64)
+    for y0 in range(0, h, 512):               # row bands keep the temporaries small
+        y1 = min(h, y0 + 512)
+        ys, xs = np.mgrid[y0:y1, 0:w].astype(np.float64)
         q = G @ np.stack([xs.ravel(), ys.ravel(), np.ones(xs.size)])
-        sx = (q[0] / q[2]).reshape(h, w)
-        sy = (q[1] / q[2]).reshape(h, w)
-        b = np.clip(_bilinear(a[..., None], sx, sy)[..., 0], 0, 255)
-        Hgt = np.linalg.inv(G)
-        out.append((np.round(a).astype(np.uint8), np.round(b).astype(np.uint8), Hgt / Hgt[2, 2]))
-    return out
+        sx = (q[0] / q[2]).reshape(y1 - y0, w)
+        sy = (q[1] / q[2]).reshape(y1 - y0, w)
+        b[y0:y1] = np.clip(_bilinear(a[..., None], sx, sy)[..., 0], 0, 255)
+    Hgt = np.linalg.inv(G)
+    return np.round(a).astype(np.uint8), np.round(b).astype(np.uint8), Hgt / Hgt[2, 2]
+
+
+def synth_pairs_batch(n_pairs=32, seed=2100):
+    """n_pairs warped pairs (synth_warped_pair) at the uneven batch sizes."""
+    sizes = synth_batch_sizes(2 * n_pairs, seed)
+    return [synth_warped_pair(sizes[2 * p][0], sizes[2 * p][1], seed + 7 * p, seed + 1000 + p)
+            for p in range(n_pairs)]
