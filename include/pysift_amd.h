@@ -59,6 +59,10 @@ int msift_last_error(char* buf, size_t n);
 /* PySIFT.__init__ (core.py:1184): validates parameters, precomputes taps. */
 int msift_create(const msift_params* p, msift_handle** h);
 int msift_destroy(msift_handle* h);
+/* No counterpart (the reference's Gaussian kernels have one launch shape):
+ * tile height of the pyramid blur kernels, 0 = chosen per level size (the
+ * default), or 8, 16, 32, 64.  Results do not depend on it. */
+int msift_set_blur_tile(msift_handle* h, int tile_h);
 
 /* Workspace for detect_describe on an H x W image (pyramid, candidates,
  * keypoint lists).  Replaces the CuPy memory-pool allocations of

@@ -21,7 +21,7 @@ EXPORTS = [
     "msift_detect_describe", "msift_pyramid_view", "msift_match_workspace_bytes", "msift_match_u8",
     "msift_match_f16", "msift_ransac_workspace_bytes", "msift_ransac_homography", "msift_warp_feather",
     "msift_bgr_to_gray", "msift_batch_create", "msift_batch_destroy", "msift_batch_sizes",
-    "msift_batch_set_blur_tile", "msift_detect_describe_batched", "msift_grad_workspace_bytes",
+    "msift_batch_set_blur_tile", "msift_set_blur_tile", "msift_detect_describe_batched", "msift_grad_workspace_bytes",
     "msift_desc_forward", "msift_desc_backward", "msift_grad_level_view", "msift_match_batch_create",
     "msift_match_batch_destroy", "msift_match_batch_sizes", "msift_match_u8_batched",
     "msift_match_batch_gather_points", "msift_ransac_batch_create", "msift_ransac_batch_destroy",
@@ -78,6 +78,7 @@ def _declare(L):
     L.msift_batch_destroy.restype = _I; L.msift_batch_destroy.argtypes = [_P]
     L.msift_batch_sizes.restype = _I; L.msift_batch_sizes.argtypes = [_P, _P, _P, _P]
     L.msift_batch_set_blur_tile.restype = _I; L.msift_batch_set_blur_tile.argtypes = [_P, _I]
+    L.msift_set_blur_tile.restype = _I; L.msift_set_blur_tile.argtypes = [_P, _I]
     L.msift_detect_describe_batched.restype = _I
     L.msift_detect_describe_batched.argtypes = [_P, _P, _P, _SZ, _I, _P, _P, _P, _P, _P, _P]
     L.msift_grad_workspace_bytes.restype = _I

@@ -453,29 +453,13 @@ __global__ __launch_bounds__(256) void k_bexpand(const BImg* __restrict__ im, in
 }
 
 // per-image stable (x, y) rank (core.py:2130)
+// stable (x, y, index) rank within image z (blockIdx.z), added into rank[]
+// (zero on entry) at the image's entry offset
 __global__ __launch_bounds__(256) void k_brank_xy(const Ent* __restrict__ E, const int* __restrict__ aoff, int maxkp,
                                                    int* __restrict__ rank) {
-    __shared__ float sx[256], sy[256];
-    const int i = blockIdx.y;
+    const int i = blockIdx.z;
     const int e0 = aoff[(size_t)i * maxkp], m = aoff[(size_t)(i + 1) * maxkp] - e0;
-    const int nblk = (m + 255) / 256;
-    for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
-        const int l = b * 256 + threadIdx.x;
-        float xi = 0.0f, yi = 0.0f;
-        if (l < m) { xi = E[e0 + l].x; yi = E[e0 + l].y; }
-        int r = 0;
-        for (int t0 = 0; t0 < m; t0 += 256) {
-            __syncthreads();
-            if (t0 + threadIdx.x < m) { sx[threadIdx.x] = E[e0 + t0 + threadIdx.x].x; sy[threadIdx.x] = E[e0 + t0 + threadIdx.x].y; }
-            __syncthreads();
-            int lim = min(256, m - t0);
-            for (int j = 0; j < lim; ++j) {
-                float xj = sx[j], yj = sy[j];
-                r += (xj < xi) || (xj == xi && (yj < yi || (yj == yi && t0 + j < l)));
-            }
-        }
-        if (l < m) rank[e0 + l] = r;
-    }
+    rank_xy_body(E + e0, m, rank + e0);
 }
 
 __global__ __launch_bounds__(64) void k_bdescriptor(const BImg* __restrict__ im, const BOct* __restrict__ oct,
@@ -850,7 +834,9 @@ int batch_run(msift_batch* B, const uint8_t* in, void* ws_, size_t ws_bytes, int
     hipLaunchKernelGGL(k_bexpand, dim3(gcap(((long long)n * B->maxkp + 255) / 256, 4096)), dim3(256), 0, s, B->d_img, n,
                        B->maxkp, (const Kp*)(ws + B->o_k2), (const int*)n2, (const float4*)(ws + B->o_ang),
                        (const int*)(ws + B->o_nang), (const int*)(ws + B->o_aoff), (Ent*)(ws + B->o_E));
-    hipLaunchKernelGGL(k_brank_xy, dim3(32, n), dim3(256), 0, s, (const Ent*)(ws + B->o_E), (const int*)(ws + B->o_aoff),
+    hipLaunchKernelGGL(k_bzero, dim3(gcap(((long long)n * 4 * B->maxkp + 255) / 256, 2048)), dim3(256), 0, s,
+                       (int*)(ws + B->o_rank), n * 4 * B->maxkp);
+    hipLaunchKernelGGL(k_brank_xy, dim3(8, 32, n), dim3(256), 0, s, (const Ent*)(ws + B->o_E), (const int*)(ws + B->o_aoff),
                        B->maxkp, (int*)(ws + B->o_rank));
     // A9 + outputs (images concatenated, each sorted by (x, y))
     hipLaunchKernelGGL(k_bdescriptor, dim3(gcap((long long)n * B->maxkp * 4, 8192)), dim3(64), 0, s, B->d_img, B->d_oct,

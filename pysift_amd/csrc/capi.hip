@@ -105,7 +105,17 @@ int msift_destroy(msift_handle* h) {
     if (!h) return MSIFT_OK;
     if (h->ev_init)
         for (int i = 0; i < 10; ++i) (void)hipEventDestroy(h->ev[i]);
+    msift::release_side_stream(h);
     delete h;
+    return MSIFT_OK;
+}
+
+int msift_set_blur_tile(msift_handle* h, int tile_h) {
+    if (!h || !(tile_h == 0 || tile_h == 8 || tile_h == 16 || tile_h == 32 || tile_h == 64)) {
+        set_error("msift_set_blur_tile: tile_h must be 0, 8, 16, 32 or 64");
+        return MSIFT_EINVAL;
+    }
+    h->blur_th = tile_h;
     return MSIFT_OK;
 }
 

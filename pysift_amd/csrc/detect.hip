@@ -134,25 +134,43 @@ __global__ __launch_bounds__(256) void k_blur(const float* __restrict__ in, floa
                      dec ? dec + z * zout : nullptr, H, W, decW, taps, t % (int)gridDim.x, t / (int)gridDim.x, sIn, sH);
 }
 
+// Tile height for a level: the tallest tile (least halo re-read) whose tile
+// count still gives every CU four workgroups; small levels use short tiles
+// (down to 8 rows below 256 tiles) so their latency is spread over more CUs.
+// LDS bounds it by the radius (64 rows up to R=12, 32 up to R=24, else 16).
+constexpr int kBlurMinTiles = 4 * 256;
+static int blur_tile_h(int R, int H, int W, int nb, int th) {
+    const int cap = R <= 12 ? 64 : (R <= 24 ? 32 : 16);
+    if (th) return th < cap ? th : cap;
+    const long long cols = (long long)((W + 63) / 64) * nb;
+    for (int t = cap; t > 16; t >>= 1)
+        if (cols * ((H + t - 1) / t) >= kBlurMinTiles) return t;
+    return cols * ((H + 15) / 16) >= kBlurMinTiles / 4 ? 16 : 8;
+}
+
+template <int R, int TH>
+static void blur_launch_th(const float* in, float* gout, float* dout, float* dec, int H, int W, int decW,
+                           const Taps& t, hipStream_t s, int nb, size_t zin, size_t zout) {
+    dim3 grid((W + 63) / 64, (H + TH - 1) / TH, nb);
+    hipLaunchKernelGGL((k_blur<R, TH>), grid, dim3(256), 0, s, in, gout, dout, dec, H, W, decW, t, zin, zout);
+}
+
 template <int R>
 static void blur_launch(const float* in, float* gout, float* dout, float* dec, int H, int W, int decW,
-                        const Taps& t, hipStream_t s, int nb, size_t zin, size_t zout) {
-    // tile height keeps LDS under 64 KB: 64 rows up to R=12, 32 up to R=24, else 16
-    if (R <= 12) {
-        dim3 grid((W + 63) / 64, (H + 63) / 64, nb);
-        hipLaunchKernelGGL((k_blur<R, 64>), grid, dim3(256), 0, s, in, gout, dout, dec, H, W, decW, t, zin, zout);
-    } else if (R <= 24) {
-        dim3 grid((W + 63) / 64, (H + 31) / 32, nb);
-        hipLaunchKernelGGL((k_blur<R, (R <= 24 ? 32 : 16)>), grid, dim3(256), 0, s, in, gout, dout, dec, H, W,
-                           decW, t, zin, zout);
-    } else {
-        dim3 grid((W + 63) / 64, (H + 15) / 16, nb);
-        hipLaunchKernelGGL((k_blur<R, 16>), grid, dim3(256), 0, s, in, gout, dout, dec, H, W, decW, t, zin, zout);
+                        const Taps& t, hipStream_t s, int nb, size_t zin, size_t zout, int th) {
+    const int TH = blur_tile_h(R, H, W, nb, th);
+    if constexpr (R <= 12) {
+        if (TH == 64) { blur_launch_th<R, 64>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout); return; }
     }
+    if constexpr (R <= 24) {
+        if (TH == 32) { blur_launch_th<R, 32>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout); return; }
+    }
+    if (TH == 16) { blur_launch_th<R, 16>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout); return; }
+    blur_launch_th<R, 8>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout);
 }
 
 int blur(const float* in, float* gout, float* dout, float* dec, int H, int W, double sigma, const Taps& t,
-         hipStream_t s, int nb, size_t zin, size_t zout) {
+         hipStream_t s, int nb, size_t zin, size_t zout, int th) {
     int decW = (W + 1) / 2;
     if (sigma < 0.01) {
         // _fast_gaussian returns its input unchanged (core.py:559-560)
@@ -162,7 +180,7 @@ int blur(const float* in, float* gout, float* dout, float* dec, int H, int W, do
         return MSIFT_OK;
     }
     switch (t.r) {
-#define MSIFT_BLUR_CASE(RR) case RR: blur_launch<RR>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout); break;
+#define MSIFT_BLUR_CASE(RR) case RR: blur_launch<RR>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout, th); break;
         MSIFT_BLUR_CASE(0) MSIFT_BLUR_CASE(1) MSIFT_BLUR_CASE(2) MSIFT_BLUR_CASE(3) MSIFT_BLUR_CASE(4)
         MSIFT_BLUR_CASE(5) MSIFT_BLUR_CASE(6) MSIFT_BLUR_CASE(7) MSIFT_BLUR_CASE(8) MSIFT_BLUR_CASE(9)
         MSIFT_BLUR_CASE(10) MSIFT_BLUR_CASE(11) MSIFT_BLUR_CASE(12) MSIFT_BLUR_CASE(13) MSIFT_BLUR_CASE(14)
@@ -355,29 +373,15 @@ __global__ __launch_bounds__(256) void k_expand(const Kp* __restrict__ k2, const
     }
 }
 
-// canonical order: np.lexsort((ys, xs)) (core.py:2130), stable
+// canonical order: np.lexsort((ys, xs)) (core.py:2130), stable.  rank[i] =
+// #{j : (x_j, y_j, j) < (x_i, y_i, i)}.  Block (bx, by) counts the 256
+// entries i of i-block bx against the kRankJ entries j of chunk by (chunks
+// past m are padded with +inf, which no finite coordinate exceeds) and adds
+// its partial with an integer atomic (order-free); rank must be zero on entry
 __global__ __launch_bounds__(256) void k_rank_xy(const Ent* __restrict__ E, const int* __restrict__ mp,
                                                   int* __restrict__ rank) {
-    __shared__ float sx[256], sy[256];
     const int m = *mp;
-    const int nblk = (m + 255) / 256;
-    for (int b = blockIdx.x; b < nblk; b += gridDim.x) {
-        int i = b * 256 + threadIdx.x;
-        float xi = 0.0f, yi = 0.0f;
-        if (i < m) { xi = E[i].x; yi = E[i].y; }
-        int r = 0;
-        for (int t0 = 0; t0 < m; t0 += 256) {
-            __syncthreads();
-            if (t0 + threadIdx.x < m) { sx[threadIdx.x] = E[t0 + threadIdx.x].x; sy[threadIdx.x] = E[t0 + threadIdx.x].y; }
-            __syncthreads();
-            int lim = min(256, m - t0);
-            for (int j = 0; j < lim; ++j) {
-                float xj = sx[j], yj = sy[j];
-                r += (xj < xi) || (xj == xi && (yj < yi || (yj == yi && t0 + j < i)));
-            }
-        }
-        if (i < m) rank[i] = r;
-    }
+    rank_xy_body(E, m, rank);
 }
 
 // ======================================================================
@@ -576,6 +580,39 @@ static int keypoint_pass(const Plan& P, char* ws, const OctInfo& oi, msift_handl
     return MSIFT_OK;
 }
 
+void release_side_stream(msift_handle* h) {
+    if (!h->side_init) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(h->side_dev);
+    (void)hipStreamDestroy(h->side);
+    for (int i = 0; i <= kMaxOct; ++i) (void)hipEventDestroy(h->fork[i]);
+    (void)hipSetDevice(cur);
+    h->side_init = false;
+}
+
+int side_stream(msift_handle* h, hipStream_t s, hipStream_t* out) {
+    int dev = 0;
+    MSIFT_HIP(hipStreamGetDevice(s, &dev));
+    if (h->side_init && h->side_dev == dev) { *out = h->side; return MSIFT_OK; }
+    release_side_stream(h);
+    int cur = 0;
+    MSIFT_HIP(hipGetDevice(&cur));
+    MSIFT_HIP(hipSetDevice(dev));
+    hipError_t e = hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking);
+    int ne = 0;
+    for (; e == hipSuccess && ne <= kMaxOct; ++ne) e = hipEventCreateWithFlags(&h->fork[ne], hipEventDisableTiming);
+    (void)hipSetDevice(cur);
+    if (e != hipSuccess) {
+        set_error(std::string("side stream: ") + hipGetErrorString(e));
+        return MSIFT_EHIP;
+    }
+    h->side_dev = dev;
+    h->side_init = true;
+    *out = h->side;
+    return MSIFT_OK;
+}
+
 int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch, void* ws_, size_t ws_bytes,
                     int cap, float* kp, float* df, uint8_t* du, int* count, float* phase_ms, hipStream_t s) {
     if (H < 1 || W < 1 || pitch < W || !img || !ws_ || !count || cap < 0) {
@@ -624,27 +661,48 @@ int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch
         double zx = (2 * W > 1) ? (double)(W - 1) / (double)(2 * W - 1) : 0.0;
         const int nt_i = ((2 * W + kUpsInTW - 1) / kUpsInTW) * ((2 * H + kUpsInTH - 1) / kUpsInTH);
         hipLaunchKernelGGL(k_ups_interp, dim3(grid_cap(nt_i, 8192)), dim3(256), 0, s, pf1, H, W, zy, zx, up);
-        if ((rc = blur(up, G0, nullptr, nullptr, 2 * H, 2 * W, h->sd0[1], h->taps_base[1], s))) return rc;
+        if ((rc = blur(up, G0, nullptr, nullptr, 2 * H, 2 * W, h->sd0[1], h->taps_base[1], s, 1, 0, 0, h->blur_th)))
+            return rc;
     } else {
         float* unit = (float*)(ws + P.o_unit);
         hipLaunchKernelGGL(k_u8_unit, dim3((W + 255) / 256, H), dim3(256), 0, s, img, pitch, H, W, unit);
-        if ((rc = blur(unit, G0, nullptr, nullptr, H, W, h->sd0[0], h->taps_base[0], s))) return rc;
+        if ((rc = blur(unit, G0, nullptr, nullptr, H, W, h->sd0[0], h->taps_base[0], s, 1, 0, 0, h->blur_th)))
+            return rc;
     }
     MSIFT_LAUNCH_CHECK();
-    // A3/A4: octaves
+    // A3/A4: octaves.  The next octave's base is G[S][::2, ::2], so levels
+    // S and S+1 of octave o are off the octave-to-octave chain: they run on
+    // the side stream (after a fork event recorded once G[S] exists) while the
+    // launch stream goes on to octave o+1; one join event brings them back
+    // before the extrema.
+    hipStream_t side = nullptr;
+    if (P.n_oct > 1 && (rc = side_stream(h, s, &side))) return rc;
     for (int o = 0; o < P.n_oct; ++o) {
         int Ho = P.Ho[o], Wo = P.Wo[o];
         size_t n = (size_t)Ho * Wo;
+        const bool fork = o + 1 < P.n_oct;
         for (int lv = 0; lv < S + 2; ++lv) {
-            float* dec = (lv + 1 == S && o + 1 < P.n_oct) ? oi.G[o + 1] : nullptr;
+            float* dec = (lv + 1 == S && fork) ? oi.G[o + 1] : nullptr;
             // the top level is only needed through its DoG
             float* gnext = (lv + 1 == S + 2) ? nullptr : oi.G[o] + n * (lv + 1);
+            hipStream_t st = s;
+            if (fork && lv >= S) {
+                if (lv == S) {
+                    MSIFT_HIP(hipEventRecord(h->fork[o], s));
+                    MSIFT_HIP(hipStreamWaitEvent(side, h->fork[o], 0));
+                }
+                st = side;
+            }
             if ((rc = blur(oi.G[o] + n * lv, gnext, oi.D[o] + n * lv, dec, Ho, Wo, h->inc[lv],
-                           h->taps_inc[lv], s)))
+                           h->taps_inc[lv], st, 1, 0, 0, h->blur_th)))
                 return rc;
         }
     }
     MSIFT_LAUNCH_CHECK();
+    if (side) {
+        MSIFT_HIP(hipEventRecord(h->fork[kMaxOct], side));
+        MSIFT_HIP(hipStreamWaitEvent(s, h->fork[kMaxOct], 0));
+    }
     if (prof) MSIFT_HIP(hipEventRecord(h->ev[1], s));
     if (prof) MSIFT_HIP(hipEventRecord(h->ev[2], s));
     // A5/A6 (+ retry at ct/2 when fewer than retry_below keypoints, core.py:2064-2071)
@@ -668,8 +726,12 @@ int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch
         hipLaunchKernelGGL(k_expand, dim3((P.maxkp + 255) / 256), dim3(256), 0, s, (const Kp*)(ws + P.o_k2),
                            (const int*)&ints[2], (const float4*)(ws + P.o_ang), (const int*)(ws + P.o_nang),
                            (const int*)(ws + P.o_aoff), cs, (Ent*)(ws + P.o_E));
-        hipLaunchKernelGGL(k_rank_xy, dim3(grid_cap((P.cap_e + 255) / 256, 256)), dim3(256), 0, s, (const Ent*)(ws + P.o_E),
-                           (const int*)&ints[3], (int*)(ws + P.o_rank));
+        hipLaunchKernelGGL(k_zero_i32, dim3((P.cap_e + 255) / 256), dim3(256), 0, s, (int*)(ws + P.o_rank), P.cap_e);
+        {
+            hipLaunchKernelGGL(k_rank_xy, dim3(grid_cap((P.cap_e + 255) / 256, 8), grid_cap((P.cap_e + kRankJ - 1) / kRankJ, 128)),
+                               dim3(256), 0, s, (const Ent*)(ws + P.o_E), (const int*)&ints[3],
+                               (int*)(ws + P.o_rank));
+        }
         // A9: descriptors written at their sorted position
         hipLaunchKernelGGL(k_descriptor, dim3(grid_cap(P.cap_e, 2048)), dim3(64), 0, s, oi, (const Kp*)(ws + P.o_k2),
                            (const Ent*)(ws + P.o_E), (const int*)&ints[3], (const int*)(ws + P.o_rank), cap, h->tab,

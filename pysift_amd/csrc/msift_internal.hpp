@@ -22,6 +22,13 @@ struct msift_handle {
     float edge32;
     hipEvent_t ev[10];
     bool ev_init;
+    int blur_th;                     // blur tile height: 0 = by level size, else 8/16/32/64
+    // second stream for the pyramid levels off the octave-to-octave chain,
+    // with its fork/join events; created on first use on a device
+    hipStream_t side;
+    hipEvent_t fork[msift::kMaxOct + 1];
+    int side_dev;
+    bool side_init;
 };
 
 namespace msift {
@@ -49,15 +56,20 @@ void set_error(const std::string& msg);
 void make_taps(double sigma, Taps* t);
 
 // detect.hip
+// the handle's side stream on the device of `s` (created on first use, or
+// re-created when the device changes); release frees it
+int side_stream(msift_handle* h, hipStream_t s, hipStream_t* out);
+void release_side_stream(msift_handle* h);
 int detect_workspace(msift_handle* h, int H, int W, size_t* bytes);
 int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch, void* ws,
                     size_t ws_bytes, int cap, float* kp, float* df, uint8_t* du, int* count,
                     float* phase_ms, hipStream_t s);
 // one pyramid level: gout = blur(in) (and dout = gout - in, dec = gout[::2, ::2]
 // when non-NULL); sigma < 0.01 copies.  nb images of one size, pointers
-// advancing by zin / zout floats per image
+// advancing by zin / zout floats per image; th = tile height (0: chosen from
+// the level size; results do not depend on it)
 int blur(const float* in, float* gout, float* dout, float* dec, int H, int W, double sigma, const Taps& t,
-         hipStream_t s, int nb = 1, size_t zin = 0, size_t zout = 0);
+         hipStream_t s, int nb = 1, size_t zin = 0, size_t zout = 0, int th = 0);
 int pyramid_view(msift_handle* h, int H, int W, void* ws, int o, int* Ho, int* Wo, float** g,
                  float** d);
 

@@ -185,6 +185,45 @@ __device__ __forceinline__ void ups_interp_tile(const double* __restrict__ cf, i
 // linear block b runs on XCD b % 8.  xcd_remap hands every XCD a contiguous
 // range of logical tiles (a bijection of [0, n)), so the halo rows that
 // neighbouring tiles share are fetched into one L2 instead of eight.
+// Stable lexicographic rank of entries by (x, y, index): the rank of entry i
+// is #{j : (x_j, y_j, j) < (x_i, y_i, i)}, added into rank[] (zero on entry)
+// by integer atomics.  Grid x strides over 256-entry i-blocks, grid y over
+// kRankJ-entry j-chunks staged in LDS; the chunk is read as float4 and the
+// loop is fully unrolled so the 64 comparisons overlap their LDS latency.
+constexpr int kRankJ = 64;
+__device__ __forceinline__ void rank_xy_body(const Ent* __restrict__ E, int m, int* __restrict__ rank) {
+    __shared__ float4 sxy[kRankJ / 2];                 // (x_j, y_j, x_j+1, y_j+1)
+    const int nbi = (m + 255) / 256, nbj = (m + kRankJ - 1) / kRankJ;
+    for (int jb = blockIdx.y; jb < nbj; jb += gridDim.y) {
+        const int t0 = jb * kRankJ;
+        __syncthreads();
+        if (threadIdx.x < kRankJ) {
+            const int j = t0 + threadIdx.x;
+            float x = __builtin_inff(), y = __builtin_inff();
+            if (j < m) { x = E[j].x; y = E[j].y; }
+            float* f = reinterpret_cast<float*>(sxy);
+            f[2 * threadIdx.x] = x;
+            f[2 * threadIdx.x + 1] = y;
+        }
+        __syncthreads();
+        for (int b = blockIdx.x; b < nbi; b += gridDim.x) {
+            const int i = b * 256 + threadIdx.x;
+            if (i >= m) continue;
+            const float xi = E[i].x, yi = E[i].y;
+            const int li = i - t0;                      // j < i  <=>  local j < li
+            int r = 0;
+#pragma unroll
+            for (int q = 0; q < kRankJ / 2; ++q) {
+                const float4 v = sxy[q];
+                const int j = 2 * q;
+                r += (v.x < xi) || (v.x == xi && (v.y < yi || (v.y == yi && j < li)));
+                r += (v.z < xi) || (v.z == xi && (v.w < yi || (v.w == yi && j + 1 < li)));
+            }
+            if (r) atomicAdd(&rank[i], r);
+        }
+    }
+}
+
 __device__ __forceinline__ int xcd_remap(int b, int n) {
     const int x = b & 7, i = b >> 3;
     const int q = n >> 3, r = n & 7;
