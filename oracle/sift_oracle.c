@@ -7,11 +7,17 @@
  * library must reproduce every one of them bit for bit:
  *   - all float arithmetic is IEEE single, no FMA contraction (-ffp-contract=off);
  *     explicit fmaf()/fma() where an accumulation is specified as fused;
- *   - exp/atan2/sincos/log2 are this file's own double-precision routines
- *     (Cody-Waite reduction + Taylor/atanh series), rounded to float at the use;
- *   - per-keypoint histograms accumulate in 64 lane-private partials (pixel p
- *     goes to partial p % 64, in increasing p), then each bin combines its 64
- *     partials with the 64-lane xor tree of wave_tree_f();
+ *   - sincos/log2 are this file's own double-precision routines (Cody-Waite
+ *     reduction + Taylor/atanh series), rounded to float at the use; the
+ *     per-pixel exp and atan2 of the orientation and descriptor histograms are
+ *     its own single-precision routines (or_exp_f, or_atan2_f);
+ *   - the orientation histogram accumulates in 64 lane-private partials (pixel
+ *     p goes to partial p % 64, in increasing p), then each bin combines its
+ *     64 partials with the 64-lane xor tree of wave_tree_f();
+ *   - the descriptor histogram sums its contributions as integers: each,
+ *     scaled by 2^30 (after clamping the pixel weight below 2^32), is
+ *     truncated to an integer (desc_fx), the 64-bit sums are exact and so
+ *     independent of order, and bin = (float)(sum * 2^-30);
  *   - 128-bin and per-point sums use the 64-lane xor tree of wave_tree_*();
  *   - keypoint refinement and every homography solve run in FP64.
  */
@@ -733,11 +739,19 @@ static int orientation_one(const float* G, int H, int W, float ri, float ci, flo
 /* ========================================================================
  * A9: 4x4x8 descriptor (core.py:693-835) + RootSIFT (core.py:1870-1873)
  * ======================================================================== */
+/* one trilinear contribution c in [0, 2^32), already scaled by 2^30, as the
+ * integer floor(c) */
+static uint64_t desc_fx(float c)
+{
+    return (uint64_t)(uint32_t)c;
+}
+
 static void descriptor_one(const float* G, int H, int W, float ri, float ci, float ang_rad, int ph,
                            int rootsift, float* out, uint8_t* out_u8)
 {
-    static __thread float part[LANES][128];
-    memset(part, 0, sizeof(part));
+    /* bins are exact integer sums of the rounded contributions: order-free */
+    uint64_t hist[128];
+    memset(hist, 0, sizeof(hist));
     double sd, cd;
     or_sincos((double)ang_rad, &sd, &cd);
     float cos_a = (float)cd, sin_a = (float)sd;
@@ -748,7 +762,6 @@ static void descriptor_one(const float* G, int H, int W, float ri, float ci, flo
     int total_px = side * side;
     int iri = (int)ri, ici = (int)ci;
     for (int flat = 0; flat < total_px; ++flat) {
-        int lane = flat % LANES;
         int r_off = flat / side - ph;
         int c_off = flat % side - ph;
         int rr = iri + r_off, cc = ici + c_off;
@@ -763,7 +776,8 @@ static void descriptor_one(const float* G, int H, int W, float ri, float ci, flo
         float mag = sqrtf(dx * dx + dy * dy);
         float ori_deg = fmod360((f_atan2(dy, dx) - ang_rad) * 57.29577951f + 360.0f);
         float gw = f_exp(-(xr * xr + yr * yr) * inv_2s2);
-        float wmag = mag * gw;
+        /* the pixel weight bounds its 8 contributions: one clamp below 2^32 */
+        float wms = fminf((mag * gw) * 0x1p30f, 0x1.fffffep31f);
         float ob = ori_deg * 0.02222222f;
         int oi0 = (int)floorf(ob);
         float df = ob - (float)oi0;
@@ -776,22 +790,17 @@ static void descriptor_one(const float* G, int H, int W, float ri, float ci, flo
         int oi0w = ((oi0 % 8) + 8) % 8;
         int oi1w = ((oi0 + 1) % 8 + 8) % 8;
         float wy0 = 1.0f - dyb, wy1 = dyb, wx0 = 1.0f - dxb, wx1 = dxb, wo0 = 1.0f - df, wo1 = df;
-        float* h = part[lane];
-        h[yi0c * 32 + xi0c * 8 + oi0w] += wy0 * wx0 * wo0 * wmag;
-        h[yi0c * 32 + xi0c * 8 + oi1w] += wy0 * wx0 * wo1 * wmag;
-        h[yi0c * 32 + xi1c * 8 + oi0w] += wy0 * wx1 * wo0 * wmag;
-        h[yi0c * 32 + xi1c * 8 + oi1w] += wy0 * wx1 * wo1 * wmag;
-        h[yi1c * 32 + xi0c * 8 + oi0w] += wy1 * wx0 * wo0 * wmag;
-        h[yi1c * 32 + xi0c * 8 + oi1w] += wy1 * wx0 * wo1 * wmag;
-        h[yi1c * 32 + xi1c * 8 + oi0w] += wy1 * wx1 * wo0 * wmag;
-        h[yi1c * 32 + xi1c * 8 + oi1w] += wy1 * wx1 * wo1 * wmag;
+        hist[yi0c * 32 + xi0c * 8 + oi0w] += desc_fx(wy0 * wx0 * wo0 * wms);
+        hist[yi0c * 32 + xi0c * 8 + oi1w] += desc_fx(wy0 * wx0 * wo1 * wms);
+        hist[yi0c * 32 + xi1c * 8 + oi0w] += desc_fx(wy0 * wx1 * wo0 * wms);
+        hist[yi0c * 32 + xi1c * 8 + oi1w] += desc_fx(wy0 * wx1 * wo1 * wms);
+        hist[yi1c * 32 + xi0c * 8 + oi0w] += desc_fx(wy1 * wx0 * wo0 * wms);
+        hist[yi1c * 32 + xi0c * 8 + oi1w] += desc_fx(wy1 * wx0 * wo1 * wms);
+        hist[yi1c * 32 + xi1c * 8 + oi0w] += desc_fx(wy1 * wx1 * wo0 * wms);
+        hist[yi1c * 32 + xi1c * 8 + oi1w] += desc_fx(wy1 * wx1 * wo1 * wms);
     }
     float v[128];
-    for (int b = 0; b < 128; ++b) {
-        float t[LANES];
-        for (int l = 0; l < LANES; ++l) t[l] = part[l][b];
-        v[b] = wave_tree_f(t);
-    }
+    for (int b = 0; b < 128; ++b) v[b] = (float)((double)hist[b] * 0x1p-30);
     /* L2 -> clip 0.2 -> L2 (core.py:793-834); lane l holds bins 2l, 2l+1 */
     float pp[LANES];
     for (int l = 0; l < LANES; ++l) pp[l] = v[2 * l] * v[2 * l] + v[2 * l + 1] * v[2 * l + 1];

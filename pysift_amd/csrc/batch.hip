@@ -462,18 +462,18 @@ __global__ __launch_bounds__(256) void k_brank_xy(const Ent* __restrict__ E, con
     rank_xy_body(E + e0, m, rank + e0);
 }
 
-__global__ __launch_bounds__(64) void k_bdescriptor(const BImg* __restrict__ im, const BOct* __restrict__ oct,
+__global__ __launch_bounds__(kDescT) void k_bdescriptor(const BImg* __restrict__ im, const BOct* __restrict__ oct,
                                                      int n, int maxkp, char* ws, const Kp* __restrict__ k2,
                                                      const Ent* __restrict__ E, const int* __restrict__ aoff,
                                                      const int* __restrict__ rank, int cap, KpTables tab, int rootsift,
                                                      float* __restrict__ out_kp, float* __restrict__ out_f,
                                                      uint8_t* __restrict__ out_u8, int* __restrict__ counts,
                                                      int* __restrict__ offsets) {
-    __shared__ float P[128 * 64];
+    __shared__ unsigned long long hist[kDescCopies * 128];
     const int lane = threadIdx.x;
     const int m = aoff[(size_t)n * maxkp];
     if (blockIdx.x == 0)
-        for (int i = lane; i <= n; i += 64) {
+        for (int i = lane; i <= n; i += kDescT) {
             if (offsets) offsets[i] = aoff[(size_t)i * maxkp];
             if (counts && i < n) counts[i] = aoff[(size_t)(i + 1) * maxkp] - aoff[(size_t)i * maxkp];
         }
@@ -486,10 +486,11 @@ __global__ __launch_bounds__(64) void k_bdescriptor(const BImg* __restrict__ im,
         const int o = (k.o_si >> 8) & 255, si = k.o_si & 255;
         const BOct O = oct[im[i].oct0 + o];
         float v0, v1;
-        describe_keypoint((const float*)(ws + O.G_off) + (size_t)si * O.H * O.W, O.H, O.W, k.ri, k.ci, en.angle,
-                          tab.ph[si], rootsift, P, lane, &v0, &v1);
-        store_descriptor(v0, v1, lane, out_f ? out_f + (size_t)row * 128 : nullptr,
-                         out_u8 ? out_u8 + (size_t)row * 128 : nullptr);
+        describe_keypoint_block((const float*)(ws + O.G_off) + (size_t)si * O.H * O.W, O.H, O.W, k.ri, k.ci,
+                                en.angle, tab.ph[si], rootsift, hist, &v0, &v1);
+        if (lane < 64)
+            store_descriptor(v0, v1, lane, out_f ? out_f + (size_t)row * 128 : nullptr,
+                             out_u8 ? out_u8 + (size_t)row * 128 : nullptr);
         if (lane == 0 && out_kp) {
             double sf = (double)(1 << o);
             float* ko = out_kp + (size_t)row * 6;
@@ -839,7 +840,7 @@ int batch_run(msift_batch* B, const uint8_t* in, void* ws_, size_t ws_bytes, int
     hipLaunchKernelGGL(k_brank_xy, dim3(8, 32, n), dim3(256), 0, s, (const Ent*)(ws + B->o_E), (const int*)(ws + B->o_aoff),
                        B->maxkp, (int*)(ws + B->o_rank));
     // A9 + outputs (images concatenated, each sorted by (x, y))
-    hipLaunchKernelGGL(k_bdescriptor, dim3(gcap((long long)n * B->maxkp * 4, 8192)), dim3(64), 0, s, B->d_img, B->d_oct,
+    hipLaunchKernelGGL(k_bdescriptor, dim3(gcap((long long)n * B->maxkp * 4, 8192)), dim3(kDescT), 0, s, B->d_img, B->d_oct,
                        n, B->maxkp, ws, (const Kp*)(ws + B->o_k2), (const Ent*)(ws + B->o_E),
                        (const int*)(ws + B->o_aoff), (const int*)(ws + B->o_rank), cap, h->tab, p.rootsift, kp, df, du,
                        counts, offsets);

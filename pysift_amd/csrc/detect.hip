@@ -389,13 +389,13 @@ __global__ __launch_bounds__(256) void k_rank_xy(const Ent* __restrict__ E, cons
 // (core.py:1870-1873), uint8 quantisation.  One wave per entry; pixel p of
 // the patch goes to lane p % 64's private 128-bin column in LDS.
 // ======================================================================
-__global__ __launch_bounds__(64) void k_descriptor(OctInfo oi, const Kp* __restrict__ k2,
+__global__ __launch_bounds__(kDescT) void k_descriptor(OctInfo oi, const Kp* __restrict__ k2,
                                                     const Ent* __restrict__ E, const int* __restrict__ mp,
                                                     const int* __restrict__ rank, int cap, KpTables tab,
                                                     double cs, int rootsift, float* __restrict__ out_kp,
                                                     float* __restrict__ out_f, uint8_t* __restrict__ out_u8,
                                                     int* __restrict__ count) {
-    __shared__ float P[128 * 64];
+    __shared__ unsigned long long hist[kDescCopies * 128];
     const int m = *mp;
     const int lane = threadIdx.x;
     if (blockIdx.x == 0 && lane == 0) *count = m;
@@ -407,10 +407,11 @@ __global__ __launch_bounds__(64) void k_descriptor(OctInfo oi, const Kp* __restr
         const int o = k.o_si >> 8, si = k.o_si & 255;
         const int H = oi.H[o], W = oi.W[o];
         float v0, v1;
-        describe_keypoint(oi.G[o] + (size_t)si * H * W, H, W, k.ri, k.ci, en.angle, tab.ph[si], rootsift, P, lane,
-                          &v0, &v1);
-        store_descriptor(v0, v1, lane, out_f ? out_f + (size_t)rk * 128 : nullptr,
-                         out_u8 ? out_u8 + (size_t)rk * 128 : nullptr);
+        describe_keypoint_block(oi.G[o] + (size_t)si * H * W, H, W, k.ri, k.ci, en.angle, tab.ph[si], rootsift, hist,
+                                &v0, &v1);
+        if (lane < 64)
+            store_descriptor(v0, v1, lane, out_f ? out_f + (size_t)rk * 128 : nullptr,
+                             out_u8 ? out_u8 + (size_t)rk * 128 : nullptr);
         if (lane == 0 && out_kp) {
             double sf = (double)(1 << o);
             float* ko = out_kp + (size_t)rk * 6;
@@ -733,7 +734,7 @@ int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch
                                (int*)(ws + P.o_rank));
         }
         // A9: descriptors written at their sorted position
-        hipLaunchKernelGGL(k_descriptor, dim3(grid_cap(P.cap_e, 2048)), dim3(64), 0, s, oi, (const Kp*)(ws + P.o_k2),
+        hipLaunchKernelGGL(k_descriptor, dim3(grid_cap(P.cap_e, 2048)), dim3(kDescT), 0, s, oi, (const Kp*)(ws + P.o_k2),
                            (const Ent*)(ws + P.o_E), (const int*)&ints[3], (const int*)(ws + P.o_rank), cap, h->tab,
                            cs, p.rootsift, kp, df, du, count);
     }

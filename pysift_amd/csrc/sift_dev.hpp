@@ -819,6 +819,143 @@ __device__ __forceinline__ void describe_keypoint(const float* __restrict__ G, i
     desc_normalize(h0, h1, rootsift, v0o, v1o);
 }
 
+// ----------------------------------------- descriptor, block per keypoint --
+// The 4x4x8 histogram of sift_descriptor_coop (core.py:693-792) summed
+// order-free.  A pixel's weight w = min(mag * gw * 2^30, 2^32 - 256) bounds
+// each of its 8 trilinear contributions c = wy * wx * wo * w (the other
+// factors lie in [0, 1]); c becomes the integer floor(c) and the bins are
+// exact 64-bit sums, so every assignment of pixels to threads and every
+// atomic order give the same bins; bin = (float)(sum 2^-30).  (The reference
+// adds float contributions with atomics in no fixed order; the integer sum is
+// within 2^-30 per contribution of the exact one.)  mag * gw stays below 2
+// for images in [0, 1], so the clamp never acts on the detector's levels.
+constexpr int kDescT = 256;            // threads per keypoint
+constexpr int kDescCopies = kDescT / 64;   // one histogram copy per wave
+constexpr float kDescFx = 0x1p30f;
+constexpr float kDescFxMax = 0x1.fffffep31f;
+
+// Pixel p of the side x side patch is visited as p = (f K) mod side^2 for
+// f = tid + kDescT i, K = 97 (101 when 97 divides side): a bijection that
+// sends the 64 lanes of a wave to pixels spread over the patch, so their
+// atomics fall on different bins.  p advances by (kDescT K) mod side^2 as a
+// (row, col) step.  hw = this wave's bins.
+template <bool kInterior>
+__device__ __forceinline__ void desc_hist_fx(const float* __restrict__ G, int H, int W, float ri, float ci,
+                                             float angle_deg, int ph, unsigned long long* hw, int tid) {
+    const float ang_rad = angle_deg * 0.0174532924f;
+    double sd, cd;
+    sincos_d((double)ang_rad, &sd, &cd);
+    const float cos_a = (float)cd, sin_a = (float)sd;
+    const float scale_sp = 4.0f / (2.0f * (float)ph);
+    const float sigma_d = (float)ph * 0.5657f;
+    const float inv_2s2 = 1.0f / (2.0f * sigma_d * sigma_d + 1e-12f);
+    const int side = 2 * ph + 1;
+    const int total_px = side * side;
+    const int iri = (int)ri, ici = (int)ci;
+    constexpr int kPf = 4;
+    const int K = (side % 97) ? 97 : 101;
+    const int step = (kDescT * K) % total_px;
+    const int srow = step / side, scol = step - srow * side;
+    const int p0 = (tid * K) % total_px;
+    int rq = p0 / side, cq = p0 - rq * side;
+    for (int f0 = tid; f0 < total_px; f0 += kDescT * kPf) {
+        float gl[kPf], gr[kPf], gu[kPf], gd[kPf], xrs[kPf], yrs[kPf];
+        bool okk[kPf];
+#pragma unroll
+        for (int k = 0; k < kPf; ++k) {
+            const int flat = f0 + kDescT * k;
+            const int r_off = rq - ph;
+            const int c_off = cq - ph;
+            cq += scol; rq += srow;
+            if (cq >= side) { cq -= side; rq += 1; }
+            if (rq >= side) rq -= side;
+            const int rr = iri + r_off, cc = ici + c_off;
+            const float xr = cos_a * (float)c_off - sin_a * (float)r_off;
+            const float yr = sin_a * (float)c_off + cos_a * (float)r_off;
+            const float xb = xr * scale_sp + 1.5f;
+            const float yb = yr * scale_sp + 1.5f;
+            bool ok = flat < total_px && !(xb < -0.5f || xb >= 3.5f || yb < -0.5f || yb >= 3.5f);
+            if (!kInterior) ok = ok && !(rr < 1 || rr >= H - 1 || cc < 1 || cc >= W - 1);
+            okk[k] = ok;
+            xrs[k] = xr; yrs[k] = yr;
+            gl[k] = gr[k] = gu[k] = gd[k] = 0.0f;
+            if (ok) {
+                const float* g = G + (size_t)rr * W + cc;
+                gl[k] = g[-1]; gr[k] = g[1]; gu[k] = g[-W]; gd[k] = g[W];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kPf; ++k) {
+            const float xr = xrs[k], yr = yrs[k];
+            const float xb = xr * scale_sp + 1.5f;
+            const float yb = yr * scale_sp + 1.5f;
+            const float dx = gr[k] - gl[k];
+            const float dy = gd[k] - gu[k];
+            const float mag = sqrtf(dx * dx + dy * dy);
+            const float ori_deg = fmod360((atan2_f(dy, dx) - ang_rad) * 57.29577951f + 360.0f);
+            const float gw = exp_f(-(xr * xr + yr * yr) * inv_2s2);
+            // a pixel outside the patch / window adds 0 to every target
+            const float wms = fminf((okk[k] ? mag * gw : 0.0f) * kDescFx, kDescFxMax);
+            const float ob = ori_deg * 0.02222222f;
+            const int oi0 = (int)floorf(ob);
+            const float df = ob - (float)oi0;
+            const int xi0 = (int)floorf(xb), yi0 = (int)floorf(yb);
+            const float dxb = xb - (float)xi0, dyb = yb - (float)yi0;
+            const int xi0c = xi0 < 0 ? 0 : (xi0 > 3 ? 3 : xi0);
+            const int yi0c = yi0 < 0 ? 0 : (yi0 > 3 ? 3 : yi0);
+            const int xi1c = xi0 + 1 < 0 ? 0 : (xi0 + 1 > 3 ? 3 : xi0 + 1);
+            const int yi1c = yi0 + 1 < 0 ? 0 : (yi0 + 1 > 3 ? 3 : yi0 + 1);
+            // & 7 is the residue mod 8 in [0, 8), for negative oi0 too
+            const int oi0w = oi0 & 7;
+            const int oi1w = (oi0 + 1) & 7;
+            const float wy0 = 1.0f - dyb, wy1 = dyb, wx0 = 1.0f - dxb, wx1 = dxb, wo0 = 1.0f - df, wo1 = df;
+            unsigned long long* h00 = hw + yi0c * 32 + xi0c * 8;
+            unsigned long long* h01 = hw + yi0c * 32 + xi1c * 8;
+            unsigned long long* h10 = hw + yi1c * 32 + xi0c * 8;
+            unsigned long long* h11 = hw + yi1c * 32 + xi1c * 8;
+            atomicAdd(h00 + oi0w, (unsigned long long)(unsigned)(wy0 * wx0 * wo0 * wms));
+            atomicAdd(h00 + oi1w, (unsigned long long)(unsigned)(wy0 * wx0 * wo1 * wms));
+            atomicAdd(h01 + oi0w, (unsigned long long)(unsigned)(wy0 * wx1 * wo0 * wms));
+            atomicAdd(h01 + oi1w, (unsigned long long)(unsigned)(wy0 * wx1 * wo1 * wms));
+            atomicAdd(h10 + oi0w, (unsigned long long)(unsigned)(wy1 * wx0 * wo0 * wms));
+            atomicAdd(h10 + oi1w, (unsigned long long)(unsigned)(wy1 * wx0 * wo1 * wms));
+            atomicAdd(h11 + oi0w, (unsigned long long)(unsigned)(wy1 * wx1 * wo0 * wms));
+            atomicAdd(h11 + oi1w, (unsigned long long)(unsigned)(wy1 * wx1 * wo1 * wms));
+        }
+    }
+}
+
+// sift_descriptor_coop (core.py:693-835) + RootSIFT for one keypoint on a
+// kDescT-thread block (every thread calls it): integer histogram in hist
+// (kDescCopies x 128, zeroed here), then wave 0 converts and normalises;
+// thread l < 64 returns bins 2l, 2l+1
+__device__ __forceinline__ void describe_keypoint_block(const float* __restrict__ G, int H, int W, float ri,
+                                                        float ci, float angle_deg, int ph, int rootsift,
+                                                        unsigned long long* hist, float* v0o, float* v1o) {
+    const int tid = threadIdx.x;
+    for (int i = tid; i < kDescCopies * 128; i += kDescT) hist[i] = 0ull;
+    __syncthreads();
+    const int iri = (int)ri, ici = (int)ci;
+    if (iri - ph >= 2 && iri + ph < H - 2 && ici - ph >= 2 && ici + ph < W - 2)
+        desc_hist_fx<true>(G, H, W, ri, ci, angle_deg, ph, hist + (tid >> 6) * 128, tid);
+    else
+        desc_hist_fx<false>(G, H, W, ri, ci, angle_deg, ph, hist + (tid >> 6) * 128, tid);
+    __syncthreads();
+    float v0 = 0.0f, v1 = 0.0f;
+    if (tid < 64) {
+        unsigned long long s0 = 0, s1 = 0;
+#pragma unroll
+        for (int w = 0; w < kDescCopies; ++w) {
+            s0 += hist[w * 128 + 2 * tid];
+            s1 += hist[w * 128 + 2 * tid + 1];
+        }
+        desc_normalize((float)((double)s0 * 0x1p-30), (float)((double)s1 * 0x1p-30), rootsift, &v0, &v1);
+    }
+    *v0o = v0;
+    *v1o = v1;
+    __syncthreads();                       // hist is reused by the caller's next keypoint
+}
+
 __device__ __forceinline__ void store_descriptor(float v0, float v1, int lane, float* out_f, uint8_t* out_u8) {
     if (out_f) {
         out_f[2 * lane] = v0;
