@@ -248,34 +248,53 @@ __device__ __forceinline__ void blur_tile(const float* __restrict__ in, float* _
     const int tid = threadIdx.x;
     const int x0 = bx * TW, y0 = by * TH;
     {
-        // all loads issued before the LDS stores; one reflection suffices
-        // unless the image is narrower than the halo
+        // all loads issued before the LDS stores.  Element i = tid + 256 j of
+        // the IH x IW input tile sits at (i / IW, i % IW); it advances by 256 =
+        // DY rows + DX columns per j, tracked as running offsets.
         constexpr int NL = (IH * IW + 255) / 256;
-        const bool small = (H <= R) || (W <= R);
+        constexpr int DY = 256 / IW, DX = 256 - DY * IW;
         float tv[NL];
+        const int ly0 = tid / IW, lx0 = tid - ly0 * IW;
+        if (y0 >= R && y0 + TH + R <= H && x0 >= R && x0 + TW + R <= W) {
+            // interior tile: no reflection
+            const float* base = in + (size_t)(y0 - R) * W + (x0 - R);
+            int lx = lx0;
+            size_t off = (size_t)ly0 * W + lx0;
 #pragma unroll
-        for (int j = 0; j < NL; ++j) {
-            int i = tid + 256 * j;
-            if (i < IH * IW) {
-                int ly = i / IW, lx = i - ly * IW;
-                int gy = y0 - R + ly, gx = x0 - R + lx;
-                if (small) {
-                    gy = reflect_idx(gy, H);
-                    gx = reflect_idx(gx, W);
-                } else {
-                    gy = gy < 0 ? -1 - gy : (gy >= H ? 2 * H - 1 - gy : gy);
-                    gx = gx < 0 ? -1 - gx : (gx >= W ? 2 * W - 1 - gx : gx);
+            for (int j = 0; j < NL; ++j) {
+                if (j + 1 < NL || tid + 256 * j < IH * IW) tv[j] = base[off];
+                off += (size_t)DY * W + DX;
+                lx += DX;
+                if (lx >= IW) { lx -= IW; off += W - IW; }
+            }
+        } else {
+            // one reflection suffices unless the image is narrower than the halo
+            const bool small = (H <= R) || (W <= R);
+            int ly = ly0, lx = lx0;
+#pragma unroll
+            for (int j = 0; j < NL; ++j) {
+                if (j + 1 < NL || tid + 256 * j < IH * IW) {
+                    int gy = y0 - R + ly, gx = x0 - R + lx;
+                    if (small) {
+                        gy = reflect_idx(gy, H);
+                        gx = reflect_idx(gx, W);
+                    } else {
+                        gy = gy < 0 ? -1 - gy : (gy >= H ? 2 * H - 1 - gy : gy);
+                        gx = gx < 0 ? -1 - gx : (gx >= W ? 2 * W - 1 - gx : gx);
+                    }
+                    tv[j] = in[(size_t)gy * W + gx];
                 }
-                tv[j] = in[(size_t)gy * W + gx];
+                ly += DY; lx += DX;
+                if (lx >= IW) { lx -= IW; ly += 1; }
             }
         }
+        int soff = ly0 * IWP + lx0, lx = lx0;
 #pragma unroll
         for (int j = 0; j < NL; ++j) {
-            int i = tid + 256 * j;
-            if (i < IH * IW) {
-                int ly = i / IW, lx = i - ly * IW;
-                sIn[ly * IWP + lx] = tv[j];
-            }
+            if (j + 1 < NL || tid + 256 * j < IH * IW) sIn[soff] = tv[j];
+            soff += DY * IWP + DX;
+            lx += DX;
+            if (lx >= IW) { lx -= IW; soff += IWP - IW; }
         }
     }
     __syncthreads();
