@@ -88,18 +88,9 @@ __global__ void k_bzero(int* p, int n) {
 // ------------------------------------------------------------- threshold --
 __global__ __launch_bounds__(256) void k_bhist(const BImg* __restrict__ im, const uint8_t* __restrict__ in,
                                                 int* __restrict__ hist) {
-    __shared__ int sh[4][256];
+    __shared__ int sh[kHistCopies * kHistStride];
     const BImg I = im[blockIdx.y];
-    const int wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) sh[j][threadIdx.x] = 0;
-    __syncthreads();
-    const uint8_t* img = in + I.in_off;
-    for (int y = blockIdx.x; y < I.H; y += gridDim.x)
-        for (int x = threadIdx.x; x < I.W; x += 256) atomicAdd(&sh[wv][img[(size_t)y * I.W + x]], 1);
-    __syncthreads();
-    int t = (sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
-    if (t) atomicAdd(&hist[blockIdx.y * 256 + threadIdx.x], t);
+    hist256_body(in + I.in_off, I.H, I.W, (size_t)I.W, hist + blockIdx.y * 256, sh);
 }
 
 __global__ __launch_bounds__(64) void k_bthresh(const BImg* __restrict__ im, const int* __restrict__ hist, double base,
@@ -139,16 +130,16 @@ __global__ __launch_bounds__(256) void k_bups_cols(const BImg* __restrict__ im, 
 
 __global__ __launch_bounds__(256) void k_bups_rows(const BImg* __restrict__ im, const PixJob* __restrict__ jobs, char* ws,
                                                     UpsTaps h) {
-    __shared__ double s[kUpsRowTW + 2 * kUpsK];
+    __shared__ double s[kUpsRowS];
     const BImg I = im[jobs[blockIdx.y].img];
     const double* src = (const double*)(ws + I.pf0_off);
     double* out = (double*)(ws + I.pf1_off);
-    const int ntx = (I.W + kUpsRowTW - 1) / kUpsRowTW, nt = ntx * I.H;
-    for (int t = blockIdx.x; t < nt; t += gridDim.x) ups_rows_tile(src, I.W, h, out, t / ntx, t % ntx, s);
+    const int ntx = (I.W + kUpsRowTW - 1) / kUpsRowTW, nt = ntx * ((I.H + kUpsRowTH - 1) / kUpsRowTH);
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) ups_rows_tile(src, I.H, I.W, h, out, t % ntx, t / ntx, s);
 }
 
 __global__ __launch_bounds__(256) void k_bups_interp(const BImg* __restrict__ im, const PixJob* __restrict__ jobs, char* ws) {
-    __shared__ double s[kUpsInSH * kUpsInSW];
+    __shared__ double s[kUpsInS];
     const BImg I = im[jobs[blockIdx.y].img];
     const double zy = (2 * I.H > 1) ? (double)(I.H - 1) / (double)(2 * I.H - 1) : 0.0;
     const double zx = (2 * I.W > 1) ? (double)(I.W - 1) / (double)(2 * I.W - 1) : 0.0;

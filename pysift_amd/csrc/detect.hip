@@ -23,18 +23,8 @@ __global__ void k_zero_i32(int* p, int n) {
 // per-wave LDS sub-histograms, rows strided over the grid
 __global__ __launch_bounds__(256) void k_hist256(const uint8_t* __restrict__ img, int H, int W,
                                                   int pitch, int* __restrict__ hist) {
-    __shared__ int sh[4][256];
-    const int wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) sh[j][threadIdx.x] = 0;
-    __syncthreads();
-    for (int y = blockIdx.x; y < H; y += gridDim.x) {
-        const uint8_t* row = img + (size_t)y * pitch;
-        for (int x = threadIdx.x; x < W; x += 256) atomicAdd(&sh[wv][row[x]], 1);
-    }
-    __syncthreads();
-    int t = (sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
-    if (t) atomicAdd(&hist[threadIdx.x], t);
+    __shared__ int sh[kHistCopies * kHistStride];
+    hist256_body(img, H, W, (size_t)pitch, hist, sh);
 }
 
 // _adaptive_contrast_thresh (core.py:1129-1148): entropy in FP64; lane l
@@ -75,14 +65,14 @@ __global__ __launch_bounds__(256) void k_ups_pf_cols(const uint8_t* __restrict__
 
 __global__ __launch_bounds__(256) void k_ups_pf_rows(const double* __restrict__ in, int H, int W, UpsTaps h,
                                                       double* __restrict__ out) {
-    __shared__ double s[kUpsRowTW + 2 * kUpsK];
-    const int ntx = (W + kUpsRowTW - 1) / kUpsRowTW, nt = ntx * H;
-    for (int t = blockIdx.x; t < nt; t += gridDim.x) ups_rows_tile(in, W, h, out, t / ntx, t % ntx, s);
+    __shared__ double s[kUpsRowS];
+    const int ntx = (W + kUpsRowTW - 1) / kUpsRowTW, nt = ntx * ((H + kUpsRowTH - 1) / kUpsRowTH);
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) ups_rows_tile(in, H, W, h, out, t % ntx, t / ntx, s);
 }
 
 __global__ __launch_bounds__(256) void k_ups_interp(const double* __restrict__ cf, int H, int W, double zy, double zx,
                                                      float* __restrict__ out) {
-    __shared__ double s[kUpsInSH * kUpsInSW];
+    __shared__ double s[kUpsInS];
     const int ntx = (2 * W + kUpsInTW - 1) / kUpsInTW, nt = ntx * ((2 * H + kUpsInTH - 1) / kUpsInTH);
     for (int t = blockIdx.x; t < nt; t += gridDim.x) ups_interp_tile(cf, H, W, zy, zx, out, t % ntx, t / ntx, s);
 }
@@ -655,7 +645,7 @@ int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch
         double* pf1 = (double*)(ws + P.o_pf1);
         float* up = (float*)(ws + P.o_unit);
         const int nt_c = ((W + kUpsColTW - 1) / kUpsColTW) * ((H + kUpsColTH - 1) / kUpsColTH);
-        const int nt_r = ((W + kUpsRowTW - 1) / kUpsRowTW) * H;
+        const int nt_r = ((W + kUpsRowTW - 1) / kUpsRowTW) * ((H + kUpsRowTH - 1) / kUpsRowTH);
         hipLaunchKernelGGL(k_ups_pf_cols, dim3(grid_cap(nt_c, 8192)), dim3(256), 0, s, img, pitch, H, W, h->ups, pf0);
         hipLaunchKernelGGL(k_ups_pf_rows, dim3(grid_cap(nt_r, 8192)), dim3(256), 0, s, pf0, H, W, h->ups, pf1);
         double zy = (2 * H > 1) ? (double)(H - 1) / (double)(2 * H - 1) : 0.0;

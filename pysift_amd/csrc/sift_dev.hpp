@@ -87,9 +87,13 @@ __device__ __forceinline__ float ups_interp(const double* __restrict__ cf, int H
 // output is the same fma chain as ups_pf_col / ups_pf_row / ups_interp
 constexpr int kUpsColRB = 8;              // column prefilter: rows per thread
 constexpr int kUpsColTW = 64, kUpsColTH = 4 * kUpsColRB;
-constexpr int kUpsRowTW = 256;            // row prefilter: outputs per block
+constexpr int kUpsRowPT = 8;              // row prefilter: outputs per thread
+constexpr int kUpsRowTH = 32, kUpsRowTW = 8 * kUpsRowPT;   // its 32 x 64 tile
+constexpr int kUpsRowSP = kUpsRowTW + 2 * kUpsK + 1;       // odd LDS row stride
+constexpr int kUpsRowS = kUpsRowTH * kUpsRowSP;            // its LDS doubles
 constexpr int kUpsInTW = 64, kUpsInTH = 16;   // interpolation output tile
 constexpr int kUpsInSW = 36, kUpsInSH = 12;   // its input tile (see ups_interp_tile)
+constexpr int kUpsInS = kUpsInSH * (kUpsInSW + kUpsInTW);  // its LDS doubles
 
 // column prefilter of a 64 x 32 tile: each thread walks kUpsColRB rows of one
 // column with the 8 + 60 input values (u8 / 255 as float, then double) held
@@ -118,20 +122,54 @@ __device__ __forceinline__ void ups_cols_tile(const uint8_t* __restrict__ img, i
     }
 }
 
-// row prefilter of 256 outputs of row y: the mirrored segment is staged in
-// LDS s[kUpsRowTW + 2 kUpsK]; ends with a barrier (s reusable)
-__device__ __forceinline__ void ups_rows_tile(const double* __restrict__ in, int W, const UpsTaps& h,
-                                              double* __restrict__ out, int y, int tx, double* s) {
-    const double* row = in + (size_t)y * W;
-    const int x0 = tx * kUpsRowTW;
-    for (int i = threadIdx.x; i < kUpsRowTW + 2 * kUpsK; i += 256) s[i] = row[mirror_idx(x0 - kUpsK + i, W)];
-    __syncthreads();
-    const int x = x0 + (int)threadIdx.x;
-    if (x < W) {
-        double acc = 0.0;
+// row prefilter of a 32-row x 64-column tile.  The mirrored input rows are
+// staged in LDS with an odd row stride, so the lanes of a wave that read one
+// column of different rows hit different banks.  Thread (r, g) = (tid % 32,
+// tid / 32) walks the 8 + 2 kUpsK inputs of row r from column 8 g once,
+// advancing the 8 outputs' fma chains, each in ascending tap order (the
+// chain of ups_pf_row), with the taps held in registers.  s holds kUpsRowS
+// doubles; ends with a barrier.
+__device__ __forceinline__ void ups_rows_tile(const double* __restrict__ in, int H, int W, const UpsTaps& h,
+                                              double* __restrict__ out, int tx, int ty, double* s) {
+    const int x0 = tx * kUpsRowTW, y0 = ty * kUpsRowTH;
+    constexpr int IW = kUpsRowTW + 2 * kUpsK;
+    static_assert(IW <= 256, "one staged column per thread");
+    if ((int)threadIdx.x < IW) {
+        // column c of all 32 rows: every load issued before the LDS stores
+        const int c = (int)threadIdx.x;
+        const double* src = in + (size_t)y0 * W + mirror_idx(x0 - kUpsK + c, W);
+        const int nr = H - y0 < kUpsRowTH ? H - y0 : kUpsRowTH;
+        double v[kUpsRowTH];
 #pragma unroll
-        for (int k = -kUpsK; k <= kUpsK; ++k) acc = fma(h.h[k < 0 ? -k : k], s[threadIdx.x + kUpsK + k], acc);
-        out[(size_t)y * W + x] = acc;
+        for (int r = 0; r < kUpsRowTH; ++r) v[r] = r < nr ? src[(size_t)r * W] : 0.0;
+#pragma unroll
+        for (int r = 0; r < kUpsRowTH; ++r) s[r * kUpsRowSP + c] = v[r];
+    }
+    __syncthreads();
+    const int r = threadIdx.x & 31, g = threadIdx.x >> 5;
+    const int y = y0 + r;
+    if (y < H) {
+        const double* sr = s + r * kUpsRowSP + g * kUpsRowPT;
+        double hv[kUpsK + 1];
+#pragma unroll
+        for (int k = 0; k <= kUpsK; ++k) hv[k] = h.h[k];
+        double acc[kUpsRowPT];
+#pragma unroll
+        for (int u = 0; u < kUpsRowPT; ++u) acc[u] = 0.0;
+#pragma unroll
+        for (int m = 0; m < kUpsRowPT + 2 * kUpsK; ++m) {
+            const double v = sr[m];
+#pragma unroll
+            for (int u = 0; u < kUpsRowPT; ++u) {
+                const int k = m - u - kUpsK;             // tap of input m for output u
+                if (k >= -kUpsK && k <= kUpsK) acc[u] = fma(hv[k < 0 ? -k : k], v, acc[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kUpsRowPT; ++u) {
+            const int x = x0 + g * kUpsRowPT + u;
+            if (x < W) out[(size_t)y * W + x] = acc[u];
+        }
     }
     __syncthreads();
 }
@@ -139,9 +177,14 @@ __device__ __forceinline__ void ups_rows_tile(const double* __restrict__ in, int
 // interpolation of a 64 x 16 output tile.  Its outputs read input rows
 // floor(oy zy) - 1 .. + 2 and columns floor(ox zx) - 1 .. + 2; with zy, zx
 // < 1/2 these span at most 12 rows and 36 columns from (cy0, cx0), staged
-// (mirrored) in LDS s[kUpsInSH * kUpsInSW]; ends with a barrier
+// (mirrored) in LDS.  The interpolation is separable: an output's inner chain
+// over its 4 columns depends only on (input row, output column), so it is
+// formed once per pair in tmp[kUpsInSH][64], and each output then runs its
+// 4-term chain over rows -- the same fma chains as ups_interp.  s holds
+// kUpsInS doubles; ends with a barrier.
 __device__ __forceinline__ void ups_interp_tile(const double* __restrict__ cf, int H, int W, double zy, double zx,
                                                 float* __restrict__ out, int tx, int ty, double* s) {
+    double* tmp = s + kUpsInSH * kUpsInSW;
     const int Wo = 2 * W, Ho = 2 * H;
     const int ox0 = tx * kUpsInTW, oy0 = ty * kUpsInTH;
     const int cy0 = (int)floor((double)oy0 * zy) - 1, cx0 = (int)floor((double)ox0 * zx) - 1;
@@ -150,12 +193,23 @@ __device__ __forceinline__ void ups_interp_tile(const double* __restrict__ cf, i
         s[i] = cf[(size_t)mirror_idx(cy0 + r, H) * W + mirror_idx(cx0 + c, W)];
     }
     __syncthreads();
-    const int ox = ox0 + (int)(threadIdx.x & 63);
-    double xx = (double)ox * zx;
-    double fx = floor(xx);
-    const int ix = (int)fx;
-    double wx[4];
-    bspline_w(xx - fx, wx);
+    const int oxl = (int)(threadIdx.x & 63);
+    const int ox = ox0 + oxl;
+    {
+        double xx = (double)ox * zx;
+        double fx = floor(xx);
+        const int ix = (int)fx;
+        double wx[4];
+        bspline_w(xx - fx, wx);
+        for (int r = (int)(threadIdx.x >> 6); r < kUpsInSH; r += 4) {
+            const double* srow = s + r * kUpsInSW + (ix - 1 - cx0);
+            double rv = 0.0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) rv = fma(wx[b], srow[b], rv);
+            tmp[r * kUpsInTW + oxl] = rv;
+        }
+    }
+    __syncthreads();
 #pragma unroll
     for (int j = 0; j < kUpsInTH / 4; ++j) {
         const int oy = oy0 + (int)(threadIdx.x >> 6) * (kUpsInTH / 4) + j;
@@ -167,17 +221,38 @@ __device__ __forceinline__ void ups_interp_tile(const double* __restrict__ cf, i
             bspline_w(yy - fy, wy);
             double v = 0.0;
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const double* srow = s + (iy - 1 + a - cy0) * kUpsInSW + (ix - 1 - cx0);
-                double rv = 0.0;
-#pragma unroll
-                for (int b = 0; b < 4; ++b) rv = fma(wx[b], srow[b], rv);
-                v = fma(wy[a], rv, v);
-            }
+            for (int a = 0; a < 4; ++a) v = fma(wy[a], tmp[(iy - 1 + a - cy0) * kUpsInTW + oxl], v);
             out[(size_t)oy * Wo + ox] = (float)v;
         }
     }
     __syncthreads();
+}
+
+// ----------------------------------------------------------- histogram --
+// 256-bin histogram of rows blockIdx.x, + gridDim.x, ... of a u8 image, added
+// into hist[256] with integer atomics (order-free, exact).  Lane l counts into
+// LDS copy l % 32 (row stride 257, so lanes that see equal values hit
+// different banks); the block sums the copies per bin at the end.
+constexpr int kHistCopies = 32, kHistStride = 257;
+__device__ __forceinline__ void hist256_body(const uint8_t* __restrict__ img, int H, int W, size_t pitch,
+                                             int* __restrict__ hist, int* sh /* kHistCopies * kHistStride */) {
+    const int tid = (int)threadIdx.x;
+    for (int i = tid; i < kHistCopies * kHistStride; i += 256) sh[i] = 0;
+    __syncthreads();
+    int* mine = sh + (tid & (kHistCopies - 1)) * kHistStride;
+    for (int y = blockIdx.x; y < H; y += gridDim.x) {
+        const uint8_t* row = img + (size_t)y * pitch;
+        int x = tid;
+        for (; x + 768 < W; x += 1024) {
+            const int v0 = row[x], v1 = row[x + 256], v2 = row[x + 512], v3 = row[x + 768];
+            atomicAdd(mine + v0, 1); atomicAdd(mine + v1, 1); atomicAdd(mine + v2, 1); atomicAdd(mine + v3, 1);
+        }
+        for (; x < W; x += 256) atomicAdd(mine + row[x], 1);
+    }
+    __syncthreads();
+    int t = 0;
+    for (int c = 0; c < kHistCopies; ++c) t += sh[c * kHistStride + tid];
+    if (t) atomicAdd(&hist[tid], t);
 }
 
 // ------------------------------------------------------------ placement --
