@@ -204,6 +204,12 @@ __global__ __launch_bounds__(256) void k_bcopy_level(const BJob* __restrict__ jo
 struct ExtTabPtr {
     const ExtOct* p;
     __device__ __forceinline__ ExtOct get(int i) const { return p[i]; }
+    __device__ __forceinline__ int find_job(int v, int from, int n) const {
+        return wave_find_last_le([&](int k) { return p[k].job0; }, v, from, n);
+    }
+    __device__ __forceinline__ int find_row(int v, int from, int n) const {
+        return wave_find_last_le([&](int k) { return p[k].row0; }, v, from, n);
+    }
 };
 
 // pass 1 (gate == NULL) or the ct/2 retry (only images with gate[i] < below)

@@ -195,6 +195,8 @@ int blur(const float* in, float* gout, float* dout, float* dec, int H, int W, do
 struct ExtTabArg {
     ExtOct o[kMaxOct];
     __device__ __forceinline__ const ExtOct& get(int i) const { return o[i]; }
+    __device__ __forceinline__ int find_job(int v, int from, int n) const { return tab_scan_job(*this, v, from, n); }
+    __device__ __forceinline__ int find_row(int v, int from, int n) const { return tab_scan_row(*this, v, from, n); }
 };
 
 __global__ __launch_bounds__(256) void k_ext_count(ExtTabArg tab, int n_oct, int njobs, const char* ws, int S,

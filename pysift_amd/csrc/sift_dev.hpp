@@ -255,6 +255,35 @@ __device__ __forceinline__ void hist256_body(const uint8_t* __restrict__ img, in
     if (t) atomicAdd(&hist[tid], t);
 }
 
+// ------------------------------------------------------- table lookups --
+// Octave tables hold ExtOct entries in increasing job0 / row0.  For a key v
+// and a start entry `from` (whose key is <= v) the entry that owns v is the
+// last one with key <= v.  A short table in kernel arguments is scanned
+// linearly; a long one in device memory is searched by the whole wave, 64
+// entries per step, with a ballot.
+template <class Tab>
+__device__ __forceinline__ int tab_scan_job(const Tab& tab, int v, int from, int n) {
+    while (from + 1 < n && tab.get(from + 1).job0 <= v) ++from;
+    return from;
+}
+template <class Tab>
+__device__ __forceinline__ int tab_scan_row(const Tab& tab, int v, int from, int n) {
+    while (from + 1 < n && tab.get(from + 1).row0 <= v) ++from;
+    return from;
+}
+template <class KeyAt>
+__device__ __forceinline__ int wave_find_last_le(KeyAt key_at, int v, int from, int n) {
+    const int lane = (int)(threadIdx.x & 63);
+    int last = from;
+    for (int b0 = from; b0 < n; b0 += 64) {
+        const int k = b0 + lane;
+        const unsigned long long m = __ballot(k < n && key_at(k) <= v);
+        if (m) last = b0 + 63 - __clzll(m);
+        if (m != ~0ull) break;
+    }
+    return last;
+}
+
 // ------------------------------------------------------------ placement --
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2:
 // linear block b runs on XCD b % 8.  xcd_remap hands every XCD a contiguous
@@ -582,7 +611,7 @@ __device__ __forceinline__ void ext_count_waves(const Tab& tab, int n_io, int nj
     const int nw = gridDim.x * (blockDim.x >> 6);
     int io = 0;
     for (int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); j < njobs; j += nw) {
-        while (io + 1 < n_io && tab.get(io + 1).job0 <= j) ++io;
+        io = tab.find_job(j, io, n_io);
         const ExtOct O = tab.get(io);
         if (gate && gate[O.img] >= below) continue;
         const int jj = j - O.job0;
@@ -602,7 +631,7 @@ __device__ __forceinline__ void ext_write_waves(const Tab& tab, int n_io, int nr
     int io = 0;
     for (int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); g < nrows; g += nw) {
         if (rowcnt[g] == 0) continue;
-        while (io + 1 < n_io && tab.get(io + 1).row0 <= g) ++io;
+        io = tab.find_row(g, io, n_io);
         const ExtOct O = tab.get(io);
         const int first = rowoff[g] - rowoff[O.row0];
         if (first >= capc) continue;
