@@ -93,7 +93,7 @@ constexpr int kUpsRowSP = kUpsRowTW + 2 * kUpsK + 1;       // odd LDS row stride
 constexpr int kUpsRowS = kUpsRowTH * kUpsRowSP;            // its LDS doubles
 constexpr int kUpsInTW = 64, kUpsInTH = 16;   // interpolation output tile
 constexpr int kUpsInSW = 36, kUpsInSH = 12;   // its input tile (see ups_interp_tile)
-constexpr int kUpsInS = kUpsInSH * (kUpsInSW + kUpsInTW);  // its LDS doubles
+constexpr int kUpsInS = kUpsInSH * (kUpsInSW + kUpsInTW) + 5 * kUpsInTH;  // its LDS doubles
 
 // column prefilter of a 64 x 32 tile: each thread walks kUpsColRB rows of one
 // column with the 8 + 60 input values (u8 / 255 as float, then double) held
@@ -180,17 +180,26 @@ __device__ __forceinline__ void ups_rows_tile(const double* __restrict__ in, int
 // (mirrored) in LDS.  The interpolation is separable: an output's inner chain
 // over its 4 columns depends only on (input row, output column), so it is
 // formed once per pair in tmp[kUpsInSH][64], and each output then runs its
-// 4-term chain over rows -- the same fma chains as ups_interp.  s holds
-// kUpsInS doubles; ends with a barrier.
+// 4-term chain over rows -- the same fma chains as ups_interp.  The row
+// weights and first input row of each output row are formed once, in wyt.
+// s holds kUpsInS doubles; ends with a barrier.
 __device__ __forceinline__ void ups_interp_tile(const double* __restrict__ cf, int H, int W, double zy, double zx,
                                                 float* __restrict__ out, int tx, int ty, double* s) {
     double* tmp = s + kUpsInSH * kUpsInSW;
+    double* wyt = tmp + kUpsInSH * kUpsInTW;        // per output row: wy[4], row offset
     const int Wo = 2 * W, Ho = 2 * H;
     const int ox0 = tx * kUpsInTW, oy0 = ty * kUpsInTH;
     const int cy0 = (int)floor((double)oy0 * zy) - 1, cx0 = (int)floor((double)ox0 * zx) - 1;
     for (int i = threadIdx.x; i < kUpsInSH * kUpsInSW; i += 256) {
         const int r = i / kUpsInSW, c = i - r * kUpsInSW;
         s[i] = cf[(size_t)mirror_idx(cy0 + r, H) * W + mirror_idx(cx0 + c, W)];
+    }
+    if (threadIdx.x < kUpsInTH) {
+        const int oy = oy0 + (int)threadIdx.x;
+        double yy = (double)oy * zy;
+        double fy = floor(yy);
+        bspline_w(yy - fy, wyt + 5 * threadIdx.x);
+        wyt[5 * threadIdx.x + 4] = (double)(((int)fy - 1 - cy0) * kUpsInTW);
     }
     __syncthreads();
     const int oxl = (int)(threadIdx.x & 63);
@@ -212,16 +221,14 @@ __device__ __forceinline__ void ups_interp_tile(const double* __restrict__ cf, i
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kUpsInTH / 4; ++j) {
-        const int oy = oy0 + (int)(threadIdx.x >> 6) * (kUpsInTH / 4) + j;
+        const int ly = (int)(threadIdx.x >> 6) * (kUpsInTH / 4) + j;
+        const int oy = oy0 + ly;
         if (ox < Wo && oy < Ho) {
-            double yy = (double)oy * zy;
-            double fy = floor(yy);
-            const int iy = (int)fy;
-            double wy[4];
-            bspline_w(yy - fy, wy);
+            const double* w = wyt + 5 * ly;
+            const double* t = tmp + (int)w[4] + oxl;
             double v = 0.0;
 #pragma unroll
-            for (int a = 0; a < 4; ++a) v = fma(wy[a], tmp[(iy - 1 + a - cy0) * kUpsInTW + oxl], v);
+            for (int a = 0; a < 4; ++a) v = fma(w[a], t[a * kUpsInTW], v);
             out[(size_t)oy * Wo + ox] = (float)v;
         }
     }
