@@ -716,6 +716,7 @@ __device__ __forceinline__ bool refine_candidate(const float* __restrict__ D, in
 // Every lane returns; lane 0's out[]/n are the result.
 __device__ __forceinline__ int orient_keypoint(const float* __restrict__ G, int H, int W, float ri, float ci,
                                                float sigma, float* P, int lane, float* out) {
+    __builtin_amdgcn_wave_barrier();     // the previous keypoint's reads of P are done
 #pragma unroll
     for (int b = 0; b < 36; ++b) P[b * 64 + lane] = 0.0f;
     int radius = (int)(4.5f * sigma + 0.5f);
@@ -767,42 +768,66 @@ __device__ __forceinline__ int orient_keypoint(const float* __restrict__ G, int 
             int bin_i = ((int)bin_f) % 36;
             int bin_r = (bin_i + 1) % 36;
             float frac = bin_f - floorf(bin_f);
-            // two distinct bins: both reads issued before the stores
+            // two distinct bins: both reads issued before the stores.  The
+            // partial of (bin, lane) sits at bin * 64 + ((lane + bin) & 63).
             float w0 = mag * wt * (1.0f - frac), w1 = mag * wt * frac;
             if (p >= total) { w0 = 0.0f; w1 = 0.0f; }
-            const float q0 = P[bin_i * 64 + lane], q1 = P[bin_r * 64 + lane];
-            P[bin_i * 64 + lane] = q0 + w0;
-            P[bin_r * 64 + lane] = q1 + w1;
+            float* s0 = P + bin_i * 64 + ((lane + bin_i) & 63);
+            float* s1 = P + bin_r * 64 + ((lane + bin_r) & 63);
+            const float q0 = *s0, q1 = *s1;
+            *s0 = q0 + w0;
+            *s1 = q1 + w1;
         }
     }
-    float hist[36];
+    // lane b < 36 owns bin b: it combines the 64 partials in the pairing of
+    // the xor-butterfly tree (t[l] += t[l + off], off = 32 .. 1; wave_tree_f
+    // gives every lane that value); the lane rotation spreads these reads
+    // over the banks
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    float h = 0.0f;
+    if (lane < 36) {
+        const float* row = P + lane * 64;
+        float t[32];
 #pragma unroll
-    for (int b = 0; b < 36; ++b) hist[b] = wave_tree_f(P[b * 64 + lane]);
-    out[0] = out[1] = out[2] = out[3] = -1.0f;
-    if (lane != 0) return 0;
-    float smooth[36];
-    for (int it = 0; it < 2; ++it) {
+        for (int l = 0; l < 32; ++l) t[l] = row[(l + lane) & 63] + row[(l + 32 + lane) & 63];
 #pragma unroll
-        for (int b = 0; b < 36; ++b)
-            smooth[b] = (0.25f * hist[(b + 35) % 36] + 0.5f * hist[b]) + 0.25f * hist[(b + 1) % 36];
+        for (int off = 16; off >= 1; off >>= 1)
 #pragma unroll
-        for (int b = 0; b < 36; ++b) hist[b] = smooth[b];
+            for (int l = 0; l < off; ++l) t[l] = t[l] + t[l + off];
+        h = t[0];
     }
-    float peak = 0.0f;
+    // two circular [1/4, 1/2, 1/4] passes over bins (lane b = bin b)
+    const int bl = lane < 36 ? (lane + 35) % 36 : lane, br = lane < 36 ? (lane + 1) % 36 : lane;
+    for (int it = 0; it < 2; ++it) {
+        const float hl = __shfl(h, bl), hr = __shfl(h, br);
+        h = (0.25f * hl + 0.5f * h) + 0.25f * hr;
+    }
+    if (lane >= 36) h = 0.0f;
+    float peak = h;                      // max over bins and 0 (bins are >= 0)
 #pragma unroll
-    for (int b = 0; b < 36; ++b) if (hist[b] > peak) peak = hist[b];
-    float thresh = 0.8f * peak;
+    for (int off = 32; off >= 1; off >>= 1) peak = fmaxf(peak, __shfl_xor(peak, off));
+    const float thresh = 0.8f * peak;
+    const float l = __shfl(h, bl), rv = __shfl(h, br);
+    const bool is_peak = lane < 36 && h >= thresh && h >= l && h >= rv;
+    float angle = -1.0f;
+    if (is_peak) {
+        const float denom = (l - 2.0f * h) + rv;
+        const float fine = (float)lane + (fabsf(denom) > 1e-12f ? (0.5f * (l - rv)) / denom : 0.0f);
+        angle = fmod360((fine / 36.0f) * 360.0f + 360.0f);
+    }
+    // the first four peaks in bin order
+    unsigned long long m = __ballot(is_peak);
     int n = 0;
 #pragma unroll
-    for (int b = 0; b < 36; ++b) {
-        if (hist[b] >= thresh) {
-            float l = hist[(b + 35) % 36], rv = hist[(b + 1) % 36];
-            if (hist[b] >= l && hist[b] >= rv) {
-                float denom = (l - 2.0f * hist[b]) + rv;
-                float fine = (float)b + (fabsf(denom) > 1e-12f ? (0.5f * (l - rv)) / denom : 0.0f);
-                float angle = fmod360((fine / 36.0f) * 360.0f + 360.0f);
-                if (n < 4) out[n++] = angle;
-            }
+    for (int k = 0; k < 4; ++k) {
+        out[k] = -1.0f;
+        if (m) {
+            const int b = __builtin_ctzll(m);
+            m &= m - 1;
+            out[k] = __shfl(angle, b);
+            ++n;
         }
     }
     return n;
