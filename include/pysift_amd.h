@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define MSIFT_ABI_VERSION 1
+#define MSIFT_ABI_VERSION 2
 
 #define MSIFT_OK          0
 #define MSIFT_EINVAL     -1   /* bad argument (Python: ValueError) */
@@ -49,6 +49,9 @@ typedef struct msift_params {
     int    border;               /* core.py:1381: 5 */
     int    retry_below;          /* core.py:2064: 100 */
     int    adaptive_thresh;      /* core.py:2045: 1 */
+    int    dsp_n_scales;         /* core.py:1196-1206: 0 = plain SIFT descriptor, 3/4/5 = DSP-SIFT preset */
+    int    fp16_pyramid;         /* core.py:1207: 1 = binary16 base and octaves >= 2 (the reference's
+                                    low-VRAM numerics; storage stays float32) */
 } msift_params;
 
 typedef struct msift_handle msift_handle;

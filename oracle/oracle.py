@@ -37,16 +37,28 @@ class Params(ctypes.Structure):
         ("double_image", ctypes.c_int), ("rootsift", ctypes.c_int), ("double_px_cap", ctypes.c_int),
         ("max_cands_per_octave", ctypes.c_int), ("max_keypoints", ctypes.c_int), ("border", ctypes.c_int),
         ("retry_below", ctypes.c_int), ("adaptive_thresh", ctypes.c_int),
+        ("dsp_n_scales", ctypes.c_int), ("fp16_pyramid", ctypes.c_int),
     ]
 
 
 def params(n_octaves=5, n_scales=3, sigma0=1.6, contrast_thresh=0.04, edge_thresh=10.0,
            double_image=True, rootsift=True, double_px_cap=16_000_000, max_cands_per_octave=25000,
-           max_keypoints=8000, border=5, retry_below=100, adaptive_thresh=True):
-    """Defaults of PySIFT.__init__ on a >=12 GB device (core.py:1184-1270)."""
+           max_keypoints=8000, border=5, retry_below=100, adaptive_thresh=True, dsp_n_scales=0,
+           fp16_pyramid=False):
+    """Defaults of PySIFT.__init__ on a >=12 GB device (core.py:1184-1270);
+    dsp_n_scales 3/4/5 selects DSP-SIFT (0: plain), fp16_pyramid the
+    binary16 storage of the low-VRAM mode."""
     return Params(n_octaves, n_scales, sigma0, contrast_thresh, edge_thresh, int(double_image),
                   int(rootsift), double_px_cap, max_cands_per_octave, max_keypoints, border,
-                  retry_below, int(adaptive_thresh))
+                  retry_below, int(adaptive_thresh), int(dsp_n_scales), int(fp16_pyramid))
+
+
+def params_low_vram(**kw):
+    """The reference's defaults below 12 GB of device memory (core.py:1191,
+    1207, 1270): 4 octaves, binary16 pyramid storage, doubling below 4.19 MP."""
+    d = dict(n_octaves=4, double_px_cap=4_194_304, fp16_pyramid=True)
+    d.update(kw)
+    return params(**d)
 
 
 _P = ctypes.c_void_p
@@ -61,6 +73,9 @@ def _declare(L):
     L.or_exp_f.restype = ctypes.c_float; L.or_exp_f.argtypes = [ctypes.c_float]
     L.or_atan2_f.restype = ctypes.c_float; L.or_atan2_f.argtypes = [ctypes.c_float, ctypes.c_float]
     L.or_exp_f_arr.restype = None; L.or_exp_f_arr.argtypes = [_P, _P, ctypes.c_int]
+    L.or_f16_round_arr.restype = None; L.or_f16_round_arr.argtypes = [_P, _P, ctypes.c_int]
+    L.or_descriptor_dsp.restype = None
+    L.or_descriptor_dsp.argtypes = [_P, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, _I, _I, _I, _P, _P]
     L.or_atan2_f_arr.restype = None; L.or_atan2_f_arr.argtypes = [_P, _P, _P, ctypes.c_int]
     L.or_sincos.restype = None; L.or_sincos.argtypes = [_D, _P, _P]
     L.or_radius.restype = _I; L.or_radius.argtypes = [_D]
@@ -313,10 +328,19 @@ def orientation(G, ri, ci, sigma):
     return a[:n].copy()
 
 
-def descriptor(G, ri, ci, ang_rad, ph, rootsift=True):
-    """sift_descriptor_coop + RootSIFT for one keypoint (core.py:693-835, 1870-1873)."""
+def descriptor(G, ri, ci, ang_rad, ph, rootsift=True, dsp_n_scales=0):
+    """sift_descriptor_coop (or, with dsp_n_scales 3/4/5, sift_descriptor_dsp_coop)
+    + RootSIFT for one keypoint (core.py:693-835, 845-997, 1870-1873)."""
     G = np.ascontiguousarray(G, np.float32)
     d = np.empty(128, np.float32); u = np.empty(128, np.uint8)
-    lib().or_descriptor(_ptr(G), G.shape[0], G.shape[1], float(ri), float(ci), float(ang_rad), int(ph),
-                        int(rootsift), _ptr(d), _ptr(u))
+    lib().or_descriptor_dsp(_ptr(G), G.shape[0], G.shape[1], float(ri), float(ci), float(ang_rad), int(ph),
+                            int(rootsift), int(dsp_n_scales), _ptr(d), _ptr(u))
     return d, u
+
+
+def f16_round(x):
+    """x (float32) rounded to binary16 and back, as a float16 array stores it."""
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(x)
+    lib().or_f16_round_arr(_ptr(x), _ptr(y), x.size)
+    return y

@@ -245,6 +245,35 @@ float or_atan2_f(float y, float x)
     return ys ? -a : a;
 }
 
+/* x rounded to IEEE binary16 (round to nearest, ties to even; subnormals
+ * kept; beyond the largest half, infinity) and back to float: the value a
+ * float16 array stores (core.py:1244, 1308) */
+float or_f16_round(float x)
+{
+    union { float f; uint32_t u; } v;
+    v.f = x;
+    const uint32_t sign = v.u & 0x80000000u, a = v.u & 0x7fffffffu;
+    if (a >= 0x7f800000u) return x;
+    union { uint32_t u; float f; } m;
+    m.u = a;
+    if (m.f >= 65520.0f) { v.u = sign | 0x7f800000u; return v.f; }
+    if (m.f < 0x1p-14f) {
+        /* half subnormals are the multiples of 2^-24 */
+        m.f = rintf(m.f * 0x1p24f) * 0x1p-24f;
+        v.u = sign | m.u;
+        return v.f;
+    }
+    /* 23 -> 10 mantissa bits, ties to even; a carry moves into the exponent */
+    const uint32_t r = (a + 0x0fffu + ((a >> 13) & 1u)) & ~0x1fffu;
+    v.u = sign | r;
+    return v.f;
+}
+
+void or_f16_round_arr(const float* x, float* y, int n)
+{
+    for (int i = 0; i < n; ++i) y[i] = or_f16_round(x[i]);
+}
+
 void or_exp_f_arr(const float* x, float* y, int n)
 {
     for (int i = 0; i < n; ++i) y[i] = or_exp_f(x[i]);
@@ -498,6 +527,10 @@ static int build_pyramid(const uint8_t* img, int H, int W, int pitch, const or_p
     P->doubled = p->double_image && ((long long)H * W < (long long)p->double_px_cap);
     float* unit = (float*)malloc(sizeof(float) * (size_t)H * W);
     or_u8_to_unit(img, H, W, pitch, unit);
+    /* fp16 mode: the u8 / 255 base is a float16 array (core.py:1256), and the
+     * zoomed base is cast back to float16 (core.py:1274) */
+    if (p->fp16_pyramid)
+        for (size_t i = 0; i < (size_t)H * W; ++i) unit[i] = or_f16_round(unit[i]);
     int bh = H, bw = W;
     float* base = unit;
     if (P->doubled) {
@@ -505,6 +538,8 @@ static int build_pyramid(const uint8_t* img, int H, int W, int pitch, const or_p
         or_upsample2x(unit, H, W, base);
         free(unit);
         bh = 2 * H; bw = 2 * W;
+        if (p->fp16_pyramid)
+            for (size_t i = 0; i < (size_t)bh * bw; ++i) base[i] = or_f16_round(base[i]);
     }
     double sd0, inc[64];
     sigma_schedule(p, P->doubled, &sd0, inc);
@@ -521,15 +556,25 @@ static int build_pyramid(const uint8_t* img, int H, int W, int pitch, const or_p
             float* src = P->G[o] + n * s;
             float* dst = P->G[o] + n * (s + 1);
             or_blur(src, dst, bh, bw, inc[s]);
-            float* d = P->D[o] + n * s;
-            for (size_t i = 0; i < n; ++i) d[i] = dst[i] - src[i];
         }
-        /* next base = level S decimated [::2, ::2] (core.py:1350) */
+        /* next base = level S decimated [::2, ::2] (core.py:1350), from the
+         * float32 chain */
         int nh = (bh + 1) / 2, nw = (bw + 1) / 2;
         float* nb = (float*)malloc(sizeof(float) * (size_t)nh * nw);
         const float* gs = P->G[o] + n * S;
         for (int y = 0; y < nh; ++y)
             for (int x = 0; x < nw; ++x) nb[(size_t)y * nw + x] = gs[(size_t)(2 * y) * bw + 2 * x];
+        /* fp16 mode: octaves >= 2 are stored as float16 (core.py:1306-1308);
+         * the DoG, orientation and descriptor read the stored levels, the
+         * blur chain does not (core.py:1313-1318) */
+        if (p->fp16_pyramid && o >= 2)
+            for (size_t i = 0; i < n * (S + 3); ++i) P->G[o][i] = or_f16_round(P->G[o][i]);
+        for (int s = 0; s < S + 2; ++s) {
+            const float* src = P->G[o] + n * s;
+            const float* dst = P->G[o] + n * (s + 1);
+            float* d = P->D[o] + n * s;
+            for (size_t i = 0; i < n; ++i) d[i] = dst[i] - src[i];
+        }
         free(b0);
         b0 = nb; bh = nh; bw = nw;
     }
@@ -746,15 +791,11 @@ static uint64_t desc_fx(float c)
     return (uint64_t)(uint32_t)c;
 }
 
-static void descriptor_one(const float* G, int H, int W, float ri, float ci, float ang_rad, int ph,
-                           int rootsift, float* out, uint8_t* out_u8)
+/* the 4x4x8 histogram of one patch of half-size ph, added into hist
+ * (core.py:693-792; one scale of the DSP loop, core.py:883-950) */
+static void desc_hist_one(const float* G, int H, int W, float ri, float ci, float ang_rad, float cos_a,
+                          float sin_a, int ph, uint64_t* hist)
 {
-    /* bins are exact integer sums of the rounded contributions: order-free */
-    uint64_t hist[128];
-    memset(hist, 0, sizeof(hist));
-    double sd, cd;
-    or_sincos((double)ang_rad, &sd, &cd);
-    float cos_a = (float)cd, sin_a = (float)sd;
     float scale_sp = 4.0f / (2.0f * (float)ph);
     float sigma_d = (float)ph * 0.5657f;
     float inv_2s2 = 1.0f / (2.0f * sigma_d * sigma_d + 1e-12f);
@@ -799,8 +840,48 @@ static void descriptor_one(const float* G, int H, int W, float ri, float ci, flo
         hist[yi1c * 32 + xi1c * 8 + oi0w] += desc_fx(wy1 * wx1 * wo0 * wms);
         hist[yi1c * 32 + xi1c * 8 + oi1w] += desc_fx(wy1 * wx1 * wo1 * wms);
     }
+}
+
+/* DSP-SIFT scale multipliers (core.py:1198-1202), float32 */
+static int dsp_scales(int n, float* sm)
+{
+    const float r2 = (float)(1.0 / sqrt(2.0)), s2 = (float)sqrt(2.0);
+    if (n == 3) { sm[0] = r2; sm[1] = 1.0f; sm[2] = s2; return 3; }
+    if (n == 4) { sm[0] = 0.5f; sm[1] = r2; sm[2] = 1.0f; sm[3] = s2; return 4; }
+    if (n == 5) { sm[0] = 0.5f; sm[1] = r2; sm[2] = 1.0f; sm[3] = s2; sm[4] = 2.0f; return 5; }
+    sm[0] = 1.0f;
+    return 0;
+}
+
+/* sift_descriptor_coop (core.py:693-835) or, with dsp_n_scales 3/4/5,
+ * sift_descriptor_dsp_coop (core.py:845-997): one histogram over the patch
+ * half-sizes max(4, (int)(ph * m + 0.5f)), divided by the number of scales;
+ * then L2 -> clip 0.2 -> L2 and RootSIFT (core.py:1870-1873) */
+static void descriptor_one(const float* G, int H, int W, float ri, float ci, float ang_rad, int ph,
+                           int rootsift, int dsp_n_scales, float* out, uint8_t* out_u8)
+{
+    /* bins are exact integer sums of the rounded contributions: order-free */
+    uint64_t hist[128];
+    memset(hist, 0, sizeof(hist));
+    double sd, cd;
+    or_sincos((double)ang_rad, &sd, &cd);
+    float cos_a = (float)cd, sin_a = (float)sd;
+    float sm[5];
+    int n_dsp = dsp_scales(dsp_n_scales, sm);
+    if (n_dsp == 0) {
+        desc_hist_one(G, H, W, ri, ci, ang_rad, cos_a, sin_a, ph, hist);
+    } else {
+        for (int d = 0; d < n_dsp; ++d) {
+            int phd = (int)((float)ph * sm[d] + 0.5f);
+            desc_hist_one(G, H, W, ri, ci, ang_rad, cos_a, sin_a, phd < 4 ? 4 : phd, hist);
+        }
+    }
     float v[128];
     for (int b = 0; b < 128; ++b) v[b] = (float)((double)hist[b] * 0x1p-30);
+    if (n_dsp > 0) {
+        const float inv_n = 1.0f / (float)n_dsp;
+        for (int b = 0; b < 128; ++b) v[b] = v[b] * inv_n;
+    }
     /* L2 -> clip 0.2 -> L2 (core.py:793-834); lane l holds bins 2l, 2l+1 */
     float pp[LANES];
     for (int l = 0; l < LANES; ++l) pp[l] = v[2 * l] * v[2 * l] + v[2 * l + 1] * v[2 * l + 1];
@@ -933,7 +1014,7 @@ int or_detect_describe(const uint8_t* img, int H, int W, int pitch, const or_par
         float dtmp[128];
         uint8_t utmp[128];
         descriptor_one(P.G[k->o] + HW * k->si, P.Ho[k->o], P.Wo[k->o], k->ri, k->ci, rad, ph,
-                       p->rootsift, dtmp, utmp);
+                       p->rootsift, p->dsp_n_scales, dtmp, utmp);
         if (desc_out) memcpy(desc_out + 128 * (size_t)t, dtmp, sizeof(dtmp));
         if (desc_u8) memcpy(desc_u8 + 128 * (size_t)t, utmp, sizeof(utmp));
     }
@@ -1465,5 +1546,11 @@ int or_orientation(const float* G, int H, int W, float ri, float ci, float sigma
 void or_descriptor(const float* G, int H, int W, float ri, float ci, float ang_rad, int ph, int rootsift,
                    float* out, uint8_t* out_u8)
 {
-    descriptor_one(G, H, W, ri, ci, ang_rad, ph, rootsift, out, out_u8);
+    descriptor_one(G, H, W, ri, ci, ang_rad, ph, rootsift, 0, out, out_u8);
+}
+
+void or_descriptor_dsp(const float* G, int H, int W, float ri, float ci, float ang_rad, int ph, int rootsift,
+                       int dsp_n_scales, float* out, uint8_t* out_u8)
+{
+    descriptor_one(G, H, W, ri, ci, ang_rad, ph, rootsift, dsp_n_scales, out, out_u8);
 }

@@ -32,6 +32,8 @@ typedef struct {
     int    border;               /* core.py:1381 */
     int    retry_below;          /* core.py:2064: retry at ct/2 when fewer */
     int    adaptive_thresh;      /* core.py:2045 */
+    int    dsp_n_scales;         /* core.py:1196-1206: 0 = plain SIFT, 3/4/5 = DSP-SIFT preset */
+    int    fp16_pyramid;         /* core.py:1207, 1244, 1308: binary16 base and octaves >= 2 */
 } or_params;
 
 /* ---- elementary functions (double, IEEE basic operations only) ---- */
@@ -39,6 +41,8 @@ double or_exp(double x);
 double or_log2(double x);
 double or_atan2(double y, double x);
 float or_exp_f(float x);
+float or_f16_round(float x);
+void or_f16_round_arr(const float* x, float* y, int n);
 float or_atan2_f(float y, float x);
 void or_exp_f_arr(const float* x, float* y, int n);
 void or_atan2_f_arr(const float* y, const float* x, float* a, int n);
@@ -94,6 +98,8 @@ int  or_extrema_octave(const float* D, int S, int H, int W, float thr, int borde
 int  or_refine_octave(const float* D, int S, int H, int W, int border, float thr, float edge,
                       const int* cands, int n, int* valid, float* out);
 int  or_orientation(const float* G, int H, int W, float ri, float ci, float sigma, float* angles);
+void or_descriptor_dsp(const float* G, int H, int W, float ri, float ci, float ang_rad, int ph, int rootsift,
+                       int dsp_n_scales, float* out, uint8_t* out_u8);
 void or_descriptor(const float* G, int H, int W, float ri, float ci, float ang_rad, int ph, int rootsift,
                    float* out, uint8_t* out_u8);
 

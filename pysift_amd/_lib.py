@@ -36,6 +36,7 @@ class MsiftParams(ctypes.Structure):
         ("double_image", ctypes.c_int), ("rootsift", ctypes.c_int), ("double_px_cap", ctypes.c_int),
         ("max_cands_per_octave", ctypes.c_int), ("max_keypoints", ctypes.c_int), ("border", ctypes.c_int),
         ("retry_below", ctypes.c_int), ("adaptive_thresh", ctypes.c_int),
+        ("dsp_n_scales", ctypes.c_int), ("fp16_pyramid", ctypes.c_int),
     ]
 
 
@@ -118,7 +119,7 @@ def lib():
         except OSError as e:
             raise ImportError(f"cannot load {LIB_PATH}: {e}") from e
         _declare(L)
-        if L.msift_abi_version() != 1:
+        if L.msift_abi_version() != 2:
             raise ImportError("libpysift_amd ABI version mismatch")
         _lib = L
     return _lib

@@ -30,6 +30,7 @@ struct BOct {
 struct BJob {           // one image's share of one pyramid step
     size_t in_off, g_off, d_off, dec_off;   // SIZE_MAX = not written
     int H, W, decW, tiles_x, tile0;
+    int q16;            // DoG between binary16 values (fp16 mode, octaves >= 2)
 };
 
 struct PixJob { int img; int px0; };          // per-image pixel ranges for the spline kernels
@@ -121,11 +122,12 @@ __global__ __launch_bounds__(64) void k_bthresh(const BImg* __restrict__ im, con
 // ---------------------------------------------------------------- spline --
 // spline / unit kernels: blockIdx.y = image job, x strides over its tiles
 __global__ __launch_bounds__(256) void k_bups_cols(const BImg* __restrict__ im, const PixJob* __restrict__ jobs,
-                                                    const uint8_t* __restrict__ in, char* ws, UpsTaps h) {
+                                                    const uint8_t* __restrict__ in, char* ws, UpsTaps h, int q16) {
     const BImg I = im[jobs[blockIdx.y].img];
     double* out = (double*)(ws + I.pf0_off);
     const int ntx = (I.W + kUpsColTW - 1) / kUpsColTW, nt = ntx * ((I.H + kUpsColTH - 1) / kUpsColTH);
-    for (int t = blockIdx.x; t < nt; t += gridDim.x) ups_cols_tile(in + I.in_off, I.W, I.H, I.W, h, out, t % ntx, t / ntx);
+    for (int t = blockIdx.x; t < nt; t += gridDim.x)
+        ups_cols_tile(in + I.in_off, I.W, I.H, I.W, h, out, t % ntx, t / ntx, q16 != 0);
 }
 
 __global__ __launch_bounds__(256) void k_bups_rows(const BImg* __restrict__ im, const PixJob* __restrict__ jobs, char* ws,
@@ -138,7 +140,8 @@ __global__ __launch_bounds__(256) void k_bups_rows(const BImg* __restrict__ im, 
     for (int t = blockIdx.x; t < nt; t += gridDim.x) ups_rows_tile(src, I.H, I.W, h, out, t % ntx, t / ntx, s);
 }
 
-__global__ __launch_bounds__(256) void k_bups_interp(const BImg* __restrict__ im, const PixJob* __restrict__ jobs, char* ws) {
+__global__ __launch_bounds__(256) void k_bups_interp(const BImg* __restrict__ im, const PixJob* __restrict__ jobs, char* ws,
+                                                      int q16) {
     __shared__ double s[kUpsInS];
     const BImg I = im[jobs[blockIdx.y].img];
     const double zy = (2 * I.H > 1) ? (double)(I.H - 1) / (double)(2 * I.H - 1) : 0.0;
@@ -146,15 +149,19 @@ __global__ __launch_bounds__(256) void k_bups_interp(const BImg* __restrict__ im
     const double* cf = (const double*)(ws + I.pf1_off);
     float* out = (float*)(ws + I.base_off);
     const int ntx = (2 * I.W + kUpsInTW - 1) / kUpsInTW, nt = ntx * ((2 * I.H + kUpsInTH - 1) / kUpsInTH);
-    for (int t = blockIdx.x; t < nt; t += gridDim.x) ups_interp_tile(cf, I.H, I.W, zy, zx, out, t % ntx, t / ntx, s);
+    for (int t = blockIdx.x; t < nt; t += gridDim.x)
+        ups_interp_tile(cf, I.H, I.W, zy, zx, out, t % ntx, t / ntx, s, q16 != 0);
 }
 
 __global__ __launch_bounds__(256) void k_bunit(const BImg* __restrict__ im, const PixJob* __restrict__ jobs,
-                                                const uint8_t* __restrict__ in, char* ws) {
+                                                const uint8_t* __restrict__ in, char* ws, int q16) {
     const BImg I = im[jobs[blockIdx.y].img];
     float* out = (float*)(ws + I.base_off);
     const int n = I.H * I.W;
-    for (int p = blockIdx.x * 256 + threadIdx.x; p < n; p += gridDim.x * 256) out[p] = (float)in[I.in_off + p] / 255.0f;
+    for (int p = blockIdx.x * 256 + threadIdx.x; p < n; p += gridDim.x * 256) {
+        const float v = (float)in[I.in_off + p] / 255.0f;
+        out[p] = q16 ? f16_round(v) : v;
+    }
 }
 
 // ------------------------------------------------------------------ blur --
@@ -183,7 +190,8 @@ __global__ __launch_bounds__(256) void k_bblur(const BJob* __restrict__ jobs, in
     const int ty = t / J.tiles_x, tx = t - ty * J.tiles_x;
     blur_tile<R, TH>((const float*)(ws + J.in_off), J.g_off == kNone ? nullptr : (float*)(ws + J.g_off),
                      J.d_off == kNone ? nullptr : (float*)(ws + J.d_off),
-                     J.dec_off == kNone ? nullptr : (float*)(ws + J.dec_off), J.H, J.W, J.decW, taps, tx, ty, sIn, sH);
+                     J.dec_off == kNone ? nullptr : (float*)(ws + J.dec_off), J.H, J.W, J.decW, taps, tx, ty, sIn, sH,
+                     J.q16);
 }
 
 __global__ __launch_bounds__(256) void k_bcopy_level(const BJob* __restrict__ jobs, int njobs, char* ws) {
@@ -417,8 +425,9 @@ __global__ __launch_bounds__(256) void k_borient(const BImg* __restrict__ im, co
         const int o = (k.o_si >> 8) & 255, si = k.o_si & 255;
         const BOct O = oct[im[i].oct0 + o];
         float out[4];
-        int c = orient_keypoint((const float*)(ws + O.G_off) + (size_t)si * O.H * O.W, O.H, O.W, k.ri, k.ci,
-                                tab.sig_ori[si], part[wave], lane, out);
+        const float* Gl = (const float*)(ws + O.G_off) + (size_t)si * O.H * O.W;
+        int c = o >= tab.q16_oct ? orient_keypoint<true>(Gl, O.H, O.W, k.ri, k.ci, tab.sig_ori[si], part[wave], lane, out)
+                                 : orient_keypoint<false>(Gl, O.H, O.W, k.ri, k.ci, tab.sig_ori[si], part[wave], lane, out);
         if (lane == 0) {
             ang[slot] = make_float4(out[0], out[1], out[2], out[3]);
             nang[slot] = c;
@@ -484,7 +493,7 @@ __global__ __launch_bounds__(kDescT) void k_bdescriptor(const BImg* __restrict__
         const BOct O = oct[im[i].oct0 + o];
         float v0, v1;
         describe_keypoint_block((const float*)(ws + O.G_off) + (size_t)si * O.H * O.W, O.H, O.W, k.ri, k.ci,
-                                en.angle, tab.ph[si], rootsift, hist, &v0, &v1);
+                                en.angle, tab.ph[si], rootsift, tab, o >= tab.q16_oct, hist, &v0, &v1);
         if (lane < 64)
             store_descriptor(v0, v1, lane, out_f ? out_f + (size_t)row * 128 : nullptr,
                              out_u8 ? out_u8 + (size_t)row * 128 : nullptr);
@@ -575,7 +584,7 @@ static void plan_steps(msift_batch* B) {
             const BOct& O = B->oct[I.oct0];
             BJob j;
             j.in_off = I.base_off; j.g_off = O.G_off; j.d_off = kNone; j.dec_off = kNone;
-            j.H = O.H; j.W = O.W; j.decW = (O.W + 1) / 2;
+            j.H = O.H; j.W = O.W; j.decW = (O.W + 1) / 2; j.q16 = 0;
             js.push_back(j);
         }
         if (!js.empty()) add_step(js, &h->taps_base[d], h->sd0[d]);
@@ -592,6 +601,7 @@ static void plan_steps(msift_batch* B) {
                 j.d_off = O.D_off + HW4 * lv;
                 j.dec_off = (lv + 1 == S && o + 1 < B->n_oct) ? B->oct[B->img[i].oct0 + o + 1].G_off : kNone;
                 j.H = O.H; j.W = O.W; j.decW = (O.W + 1) / 2;
+                j.q16 = o >= h->tab.q16_oct;
                 js.push_back(j);
             }
             add_step(js, &h->taps_inc[lv], h->inc[lv]);
@@ -792,13 +802,14 @@ int batch_run(msift_batch* B, const uint8_t* in, void* ws_, size_t ws_bytes, int
     // A3: base images
     if (B->ups_cols_px > 0) {
         dim3 g(256, (unsigned)B->ups_cols.size());
-        hipLaunchKernelGGL(k_bups_cols, g, dim3(256), 0, s, B->d_img, B->d_ups_cols, in, ws, h->ups);
+        hipLaunchKernelGGL(k_bups_cols, g, dim3(256), 0, s, B->d_img, B->d_ups_cols, in, ws, h->ups, p.fp16_pyramid);
         hipLaunchKernelGGL(k_bups_rows, g, dim3(256), 0, s, B->d_img, B->d_ups_cols, ws, h->ups);
         hipLaunchKernelGGL(k_bups_interp, dim3(512, (unsigned)B->ups_cols.size()), dim3(256), 0, s, B->d_img,
-                           B->d_ups_cols, ws);
+                           B->d_ups_cols, ws, p.fp16_pyramid);
     }
     if (B->unit_px > 0)
-        hipLaunchKernelGGL(k_bunit, dim3(256, (unsigned)B->unit.size()), dim3(256), 0, s, B->d_img, B->d_unit, in, ws);
+        hipLaunchKernelGGL(k_bunit, dim3(256, (unsigned)B->unit.size()), dim3(256), 0, s, B->d_img, B->d_unit, in, ws,
+                           p.fp16_pyramid);
     MSIFT_LAUNCH_CHECK();
     // A2-A4: every pyramid step is one launch over all images
     for (const auto& st : B->steps) {

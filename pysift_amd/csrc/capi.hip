@@ -57,6 +57,14 @@ int msift_create(const msift_params* p, msift_handle** out) {
         set_error("caps and border must be positive");
         return MSIFT_EINVAL;
     }
+    if (!(p->dsp_n_scales == 0 || p->dsp_n_scales == 3 || p->dsp_n_scales == 4 || p->dsp_n_scales == 5)) {
+        set_error("dsp_n_scales must be 3, 4, or 5 (or 0 for the plain descriptor)");
+        return MSIFT_EINVAL;
+    }
+    if (p->fp16_pyramid != 0 && p->fp16_pyramid != 1) {
+        set_error("fp16_pyramid must be 0 or 1");
+        return MSIFT_EINVAL;
+    }
     msift_handle* h = new (std::nothrow) msift_handle();
     if (!h) { set_error("out of host memory"); return MSIFT_EINVAL; }
     memset(h, 0, sizeof(*h));
@@ -97,6 +105,15 @@ int msift_create(const msift_params* p, msift_handle** out) {
         h->tab.base_size[si] = p->sigma0 * pow(k, (double)si);
     }
     h->edge32 = (float)((p->edge_thresh + 1.0) * (p->edge_thresh + 1.0) / p->edge_thresh);
+    // DSP-SIFT presets (core.py:1198-1202) as float32
+    {
+        const float r2 = (float)(1.0 / sqrt(2.0)), s2 = (float)sqrt(2.0);
+        const float p3[3] = {r2, 1.0f, s2}, p4[4] = {0.5f, r2, 1.0f, s2}, p5[5] = {0.5f, r2, 1.0f, s2, 2.0f};
+        const float* src = p->dsp_n_scales == 3 ? p3 : (p->dsp_n_scales == 4 ? p4 : p5);
+        h->tab.n_dsp = p->dsp_n_scales;
+        for (int i = 0; i < 5; ++i) h->tab.dsp[i] = i < p->dsp_n_scales ? src[i] : 0.0f;
+    }
+    h->tab.q16_oct = p->fp16_pyramid ? 2 : (1 << 30);
     *out = h;
     return MSIFT_OK;
 }

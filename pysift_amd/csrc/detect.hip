@@ -58,9 +58,10 @@ __global__ __launch_bounds__(64) void k_thresh(const int* __restrict__ hist, dou
 // 61-tap symmetric FIR in FP64, then 4x4 tensor-product interpolation.
 // ======================================================================
 __global__ __launch_bounds__(256) void k_ups_pf_cols(const uint8_t* __restrict__ img, int pitch, int H, int W,
-                                                      UpsTaps h, double* __restrict__ out) {
+                                                      UpsTaps h, double* __restrict__ out, int q16) {
     const int ntx = (W + kUpsColTW - 1) / kUpsColTW, nt = ntx * ((H + kUpsColTH - 1) / kUpsColTH);
-    for (int t = blockIdx.x; t < nt; t += gridDim.x) ups_cols_tile(img, pitch, H, W, h, out, t % ntx, t / ntx);
+    for (int t = blockIdx.x; t < nt; t += gridDim.x)
+        ups_cols_tile(img, pitch, H, W, h, out, t % ntx, t / ntx, q16 != 0);
 }
 
 __global__ __launch_bounds__(256) void k_ups_pf_rows(const double* __restrict__ in, int H, int W, UpsTaps h,
@@ -71,17 +72,22 @@ __global__ __launch_bounds__(256) void k_ups_pf_rows(const double* __restrict__ 
 }
 
 __global__ __launch_bounds__(256) void k_ups_interp(const double* __restrict__ cf, int H, int W, double zy, double zx,
-                                                     float* __restrict__ out) {
+                                                     float* __restrict__ out, int q16) {
     __shared__ double s[kUpsInS];
     const int ntx = (2 * W + kUpsInTW - 1) / kUpsInTW, nt = ntx * ((2 * H + kUpsInTH - 1) / kUpsInTH);
-    for (int t = blockIdx.x; t < nt; t += gridDim.x) ups_interp_tile(cf, H, W, zy, zx, out, t % ntx, t / ntx, s);
+    for (int t = blockIdx.x; t < nt; t += gridDim.x)
+        ups_interp_tile(cf, H, W, zy, zx, out, t % ntx, t / ntx, s, q16 != 0);
 }
 
+// u8 / 255 (core.py:1256); fp16 mode: a float16 array
 __global__ __launch_bounds__(256) void k_u8_unit(const uint8_t* __restrict__ img, int pitch, int H,
-                                                  int W, float* __restrict__ out) {
+                                                  int W, float* __restrict__ out, int q16) {
     int x = blockIdx.x * 256 + threadIdx.x;
     int y = blockIdx.y;
-    if (x < W) out[(size_t)y * W + x] = (float)img[(size_t)y * pitch + x] / 255.0f;
+    if (x < W) {
+        const float v = (float)img[(size_t)y * pitch + x] / 255.0f;
+        out[(size_t)y * W + x] = q16 ? f16_round(v) : v;
+    }
 }
 
 // a level whose sigma is below 0.01 equals its input (core.py:559-560)
@@ -115,13 +121,14 @@ __global__ __launch_bounds__(256) void k_copy_level(const float* __restrict__ in
 template <int R, int TH>
 __global__ __launch_bounds__(256) void k_blur(const float* __restrict__ in, float* __restrict__ gout,
                                                float* __restrict__ dout, float* __restrict__ dec,
-                                               int H, int W, int decW, Taps taps, size_t zin, size_t zout) {
+                                               int H, int W, int decW, Taps taps, size_t zin, size_t zout, int q16) {
     __shared__ __attribute__((aligned(16))) float sIn[BlurSmem<R, TH>::nIn];
     __shared__ __attribute__((aligned(16))) float sH[BlurSmem<R, TH>::nH];
     const size_t z = blockIdx.z;
     const int t = xcd_remap((int)(blockIdx.x + blockIdx.y * gridDim.x), (int)(gridDim.x * gridDim.y));
     blur_tile<R, TH>(in + z * zin, gout ? gout + z * zout : nullptr, dout ? dout + z * zout : nullptr,
-                     dec ? dec + z * zout : nullptr, H, W, decW, taps, t % (int)gridDim.x, t / (int)gridDim.x, sIn, sH);
+                     dec ? dec + z * zout : nullptr, H, W, decW, taps, t % (int)gridDim.x, t / (int)gridDim.x, sIn, sH,
+                     q16);
 }
 
 // Tile height for a level: the tallest tile (least halo re-read) whose tile
@@ -140,27 +147,27 @@ static int blur_tile_h(int R, int H, int W, int nb, int th) {
 
 template <int R, int TH>
 static void blur_launch_th(const float* in, float* gout, float* dout, float* dec, int H, int W, int decW,
-                           const Taps& t, hipStream_t s, int nb, size_t zin, size_t zout) {
+                           const Taps& t, hipStream_t s, int nb, size_t zin, size_t zout, int q16) {
     dim3 grid((W + 63) / 64, (H + TH - 1) / TH, nb);
-    hipLaunchKernelGGL((k_blur<R, TH>), grid, dim3(256), 0, s, in, gout, dout, dec, H, W, decW, t, zin, zout);
+    hipLaunchKernelGGL((k_blur<R, TH>), grid, dim3(256), 0, s, in, gout, dout, dec, H, W, decW, t, zin, zout, q16);
 }
 
 template <int R>
 static void blur_launch(const float* in, float* gout, float* dout, float* dec, int H, int W, int decW,
-                        const Taps& t, hipStream_t s, int nb, size_t zin, size_t zout, int th) {
+                        const Taps& t, hipStream_t s, int nb, size_t zin, size_t zout, int th, int q16) {
     const int TH = blur_tile_h(R, H, W, nb, th);
     if constexpr (R <= 12) {
-        if (TH == 64) { blur_launch_th<R, 64>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout); return; }
+        if (TH == 64) { blur_launch_th<R, 64>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout, q16); return; }
     }
     if constexpr (R <= 24) {
-        if (TH == 32) { blur_launch_th<R, 32>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout); return; }
+        if (TH == 32) { blur_launch_th<R, 32>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout, q16); return; }
     }
-    if (TH == 16) { blur_launch_th<R, 16>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout); return; }
-    blur_launch_th<R, 8>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout);
+    if (TH == 16) { blur_launch_th<R, 16>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout, q16); return; }
+    blur_launch_th<R, 8>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout, q16);
 }
 
 int blur(const float* in, float* gout, float* dout, float* dec, int H, int W, double sigma, const Taps& t,
-         hipStream_t s, int nb, size_t zin, size_t zout, int th) {
+         hipStream_t s, int nb, size_t zin, size_t zout, int th, int q16) {
     int decW = (W + 1) / 2;
     if (sigma < 0.01) {
         // _fast_gaussian returns its input unchanged (core.py:559-560)
@@ -170,7 +177,7 @@ int blur(const float* in, float* gout, float* dout, float* dec, int H, int W, do
         return MSIFT_OK;
     }
     switch (t.r) {
-#define MSIFT_BLUR_CASE(RR) case RR: blur_launch<RR>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout, th); break;
+#define MSIFT_BLUR_CASE(RR) case RR: blur_launch<RR>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout, th, q16); break;
         MSIFT_BLUR_CASE(0) MSIFT_BLUR_CASE(1) MSIFT_BLUR_CASE(2) MSIFT_BLUR_CASE(3) MSIFT_BLUR_CASE(4)
         MSIFT_BLUR_CASE(5) MSIFT_BLUR_CASE(6) MSIFT_BLUR_CASE(7) MSIFT_BLUR_CASE(8) MSIFT_BLUR_CASE(9)
         MSIFT_BLUR_CASE(10) MSIFT_BLUR_CASE(11) MSIFT_BLUR_CASE(12) MSIFT_BLUR_CASE(13) MSIFT_BLUR_CASE(14)
@@ -334,8 +341,9 @@ __global__ __launch_bounds__(256) void k_orient(OctInfo oi, const Kp* __restrict
         int o = k.o_si >> 8, si = k.o_si & 255;
         int H = oi.H[o], W = oi.W[o];
         float out[4];
-        int n = orient_keypoint(oi.G[o] + (size_t)si * H * W, H, W, k.ri, k.ci, tab.sig_ori[si], part[wave], lane,
-                                out);
+        const float* Gl = oi.G[o] + (size_t)si * H * W;
+        int n = o >= tab.q16_oct ? orient_keypoint<true>(Gl, H, W, k.ri, k.ci, tab.sig_ori[si], part[wave], lane, out)
+                                 : orient_keypoint<false>(Gl, H, W, k.ri, k.ci, tab.sig_ori[si], part[wave], lane, out);
         if (lane == 0) {
             ang[i] = make_float4(out[0], out[1], out[2], out[3]);
             nang[i] = n;
@@ -399,8 +407,8 @@ __global__ __launch_bounds__(kDescT) void k_descriptor(OctInfo oi, const Kp* __r
         const int o = k.o_si >> 8, si = k.o_si & 255;
         const int H = oi.H[o], W = oi.W[o];
         float v0, v1;
-        describe_keypoint_block(oi.G[o] + (size_t)si * H * W, H, W, k.ri, k.ci, en.angle, tab.ph[si], rootsift, hist,
-                                &v0, &v1);
+        describe_keypoint_block(oi.G[o] + (size_t)si * H * W, H, W, k.ri, k.ci, en.angle, tab.ph[si], rootsift, tab,
+                                o >= tab.q16_oct, hist, &v0, &v1);
         if (lane < 64)
             store_descriptor(v0, v1, lane, out_f ? out_f + (size_t)rk * 128 : nullptr,
                              out_u8 ? out_u8 + (size_t)rk * 128 : nullptr);
@@ -648,17 +656,20 @@ int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch
         float* up = (float*)(ws + P.o_unit);
         const int nt_c = ((W + kUpsColTW - 1) / kUpsColTW) * ((H + kUpsColTH - 1) / kUpsColTH);
         const int nt_r = ((W + kUpsRowTW - 1) / kUpsRowTW) * ((H + kUpsRowTH - 1) / kUpsRowTH);
-        hipLaunchKernelGGL(k_ups_pf_cols, dim3(grid_cap(nt_c, 8192)), dim3(256), 0, s, img, pitch, H, W, h->ups, pf0);
+        hipLaunchKernelGGL(k_ups_pf_cols, dim3(grid_cap(nt_c, 8192)), dim3(256), 0, s, img, pitch, H, W, h->ups, pf0,
+                           p.fp16_pyramid);
         hipLaunchKernelGGL(k_ups_pf_rows, dim3(grid_cap(nt_r, 8192)), dim3(256), 0, s, pf0, H, W, h->ups, pf1);
         double zy = (2 * H > 1) ? (double)(H - 1) / (double)(2 * H - 1) : 0.0;
         double zx = (2 * W > 1) ? (double)(W - 1) / (double)(2 * W - 1) : 0.0;
         const int nt_i = ((2 * W + kUpsInTW - 1) / kUpsInTW) * ((2 * H + kUpsInTH - 1) / kUpsInTH);
-        hipLaunchKernelGGL(k_ups_interp, dim3(grid_cap(nt_i, 8192)), dim3(256), 0, s, pf1, H, W, zy, zx, up);
+        hipLaunchKernelGGL(k_ups_interp, dim3(grid_cap(nt_i, 8192)), dim3(256), 0, s, pf1, H, W, zy, zx, up,
+                           p.fp16_pyramid);
         if ((rc = blur(up, G0, nullptr, nullptr, 2 * H, 2 * W, h->sd0[1], h->taps_base[1], s, 1, 0, 0, h->blur_th)))
             return rc;
     } else {
         float* unit = (float*)(ws + P.o_unit);
-        hipLaunchKernelGGL(k_u8_unit, dim3((W + 255) / 256, H), dim3(256), 0, s, img, pitch, H, W, unit);
+        hipLaunchKernelGGL(k_u8_unit, dim3((W + 255) / 256, H), dim3(256), 0, s, img, pitch, H, W, unit,
+                           p.fp16_pyramid);
         if ((rc = blur(unit, G0, nullptr, nullptr, H, W, h->sd0[0], h->taps_base[0], s, 1, 0, 0, h->blur_th)))
             return rc;
     }
@@ -687,7 +698,7 @@ int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch
                 st = side;
             }
             if ((rc = blur(oi.G[o] + n * lv, gnext, oi.D[o] + n * lv, dec, Ho, Wo, h->inc[lv],
-                           h->taps_inc[lv], st, 1, 0, 0, h->blur_th)))
+                           h->taps_inc[lv], st, 1, 0, 0, h->blur_th, o >= h->tab.q16_oct)))
                 return rc;
         }
     }

@@ -309,6 +309,10 @@ static int check_args(msift_handle* h, int B, int H, int W, int K, const void* i
         set_error("descriptor gradient: needs double_image = 0 (the spline-upsample adjoint is not built)");
         return MSIFT_EINVAL;
     }
+    if (h->p.dsp_n_scales || h->p.fp16_pyramid) {
+        set_error("descriptor gradient: the plain float32 descriptor only (dsp_n_scales = 0, fp16_pyramid = 0)");
+        return MSIFT_EINVAL;
+    }
     if (B > 65535) { set_error("descriptor gradient: at most 65535 images"); return MSIFT_EINVAL; }
     return MSIFT_OK;
 }

@@ -8,6 +8,7 @@
 // the GPU reproduces a CPU evaluation of the same expressions bit for bit.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
 #include <stdint.h>
 #include <stddef.h>
 
@@ -213,6 +214,10 @@ MSIFT_HD float atan2_f(float y, float x) {
 
 // C fmodf(x, 360) for |x| < 1440 (every subtraction is exact): the same
 // subtractions as a loop while |x| >= 360, unrolled to three selects
+// x rounded to binary16 (nearest, ties to even) and back: the value a
+// float16 array stores (core.py:1244, 1308)
+__device__ __forceinline__ float f16_round(float x) { return __half2float(__float2half_rn(x)); }
+
 MSIFT_HD float fmod360(float x) {
     float a = fabsf(x);
     a = a >= 360.0f ? a - 360.0f : a;

@@ -8,6 +8,9 @@ struct KpTables {
     float sig_ori[32];     // orientation sigma per scale index (float32)
     int ph[32];            // descriptor patch half-size per scale index
     double base_size[32];  // sigma0 * k^si (double)
+    float dsp[5];          // DSP-SIFT patch multipliers (core.py:1198-1202)
+    int n_dsp;             // 0: plain descriptor
+    int q16_oct;           // first octave read as binary16 (fp16 mode: 2, else past the last)
 };
 
 struct msift_handle {
@@ -67,9 +70,10 @@ int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch
 // one pyramid level: gout = blur(in) (and dout = gout - in, dec = gout[::2, ::2]
 // when non-NULL); sigma < 0.01 copies.  nb images of one size, pointers
 // advancing by zin / zout floats per image; th = tile height (0: chosen from
-// the level size; results do not depend on it)
+// the level size; results do not depend on it); q16: DoG between binary16
+// values (fp16 mode, octaves >= 2)
 int blur(const float* in, float* gout, float* dout, float* dec, int H, int W, double sigma, const Taps& t,
-         hipStream_t s, int nb = 1, size_t zin = 0, size_t zout = 0, int th = 0);
+         hipStream_t s, int nb = 1, size_t zin = 0, size_t zout = 0, int th = 0, int q16 = 0);
 int pyramid_view(msift_handle* h, int H, int W, void* ws, int o, int* Ho, int* Wo, float** g,
                  float** d);
 

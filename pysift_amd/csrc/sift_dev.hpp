@@ -96,10 +96,11 @@ constexpr int kUpsInSW = 36, kUpsInSH = 12;   // its input tile (see ups_interp_
 constexpr int kUpsInS = kUpsInSH * (kUpsInSW + kUpsInTW) + 5 * kUpsInTH;  // its LDS doubles
 
 // column prefilter of a 64 x 32 tile: each thread walks kUpsColRB rows of one
-// column with the 8 + 60 input values (u8 / 255 as float, then double) held
-// in registers
+// column with the 8 + 60 input values (u8 / 255 as float, rounded to binary16
+// when q16, then double) held in registers
 __device__ __forceinline__ void ups_cols_tile(const uint8_t* __restrict__ img, int pitch, int H, int W,
-                                              const UpsTaps& h, double* __restrict__ out, int tx, int ty) {
+                                              const UpsTaps& h, double* __restrict__ out, int tx, int ty,
+                                              bool q16) {
     const int x = tx * kUpsColTW + (int)(threadIdx.x & 63);
     const int y0 = ty * kUpsColTH + (int)(threadIdx.x >> 6) * kUpsColRB;
     if (x >= W || y0 >= H) return;
@@ -109,7 +110,8 @@ __device__ __forceinline__ void ups_cols_tile(const uint8_t* __restrict__ img, i
     for (int i = 0; i < kUpsColRB + 2 * kUpsK; ++i) {
         int yy = y0 - kUpsK + i;
         if (!inner) yy = mirror_idx(yy, H);
-        u[i] = (double)((float)img[(size_t)yy * pitch + x] / 255.0f);
+        const float v = (float)img[(size_t)yy * pitch + x] / 255.0f;
+        u[i] = (double)(q16 ? f16_round(v) : v);    // fp16 mode: the u8 / 255 base is float16
     }
 #pragma unroll
     for (int j = 0; j < kUpsColRB; ++j) {
@@ -182,9 +184,10 @@ __device__ __forceinline__ void ups_rows_tile(const double* __restrict__ in, int
 // formed once per pair in tmp[kUpsInSH][64], and each output then runs its
 // 4-term chain over rows -- the same fma chains as ups_interp.  The row
 // weights and first input row of each output row are formed once, in wyt.
-// s holds kUpsInS doubles; ends with a barrier.
+// q16 rounds the output to binary16.  s holds kUpsInS doubles; ends with a
+// barrier.
 __device__ __forceinline__ void ups_interp_tile(const double* __restrict__ cf, int H, int W, double zy, double zx,
-                                                float* __restrict__ out, int tx, int ty, double* s) {
+                                                float* __restrict__ out, int tx, int ty, double* s, bool q16) {
     double* tmp = s + kUpsInSH * kUpsInSW;
     double* wyt = tmp + kUpsInSH * kUpsInTW;        // per output row: wy[4], row offset
     const int Wo = 2 * W, Ho = 2 * H;
@@ -229,7 +232,8 @@ __device__ __forceinline__ void ups_interp_tile(const double* __restrict__ cf, i
             double v = 0.0;
 #pragma unroll
             for (int a = 0; a < 4; ++a) v = fma(w[a], t[a * kUpsInTW], v);
-            out[(size_t)oy * Wo + ox] = (float)v;
+            // fp16 mode: the zoomed base is cast to float16 (core.py:1274)
+            out[(size_t)oy * Wo + ox] = q16 ? f16_round((float)v) : (float)v;
         }
     }
     __syncthreads();
@@ -345,11 +349,13 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
 // One 64 x TH output tile of G[s+1] = blur(G[s]) (H pass then V pass, fused
 // multiply-add chains over the taps in ascending order), D[s] = G[s+1] - G[s]
 // and the next octave's base G[S][::2, ::2].  The tile and its reflected
-// halo are staged through LDS.
+// halo are staged through LDS.  q16 (fp16 mode, octaves >= 2): the DoG is
+// taken between the binary16 values the reference stores (core.py:1306-1308,
+// 1362-1364); the chain itself stays float32.
 template <int R, int TH>
 __device__ __forceinline__ void blur_tile(const float* __restrict__ in, float* __restrict__ gout,
                                           float* __restrict__ dout, float* __restrict__ dec, int H, int W, int decW,
-                                          const Taps& taps, int bx, int by, float* sIn, float* sH) {
+                                          const Taps& taps, int bx, int by, float* sIn, float* sH, int q16) {
     constexpr int TW = 64;
     constexpr int IW = TW + 2 * R;
     constexpr int IWP = (IW + 4 + 3) & ~3;
@@ -445,7 +451,10 @@ __device__ __forceinline__ void blur_tile(const float* __restrict__ in, float* _
         if (gy < H && gx < W) {
             size_t o = (size_t)gy * W + gx;
             if (gout) gout[o] = s;
-            if (dout) dout[o] = s - sIn[(g * RG + j + R) * IWP + lx + R];
+            if (dout) {
+                const float g0 = sIn[(g * RG + j + R) * IWP + lx + R];
+                dout[o] = q16 ? f16_round(s) - f16_round(g0) : s - g0;
+            }
             if (dec && !(gy & 1) && !(gx & 1)) dec[(size_t)(gy >> 1) * decW + (gx >> 1)] = s;
         }
     }
@@ -713,7 +722,9 @@ __device__ __forceinline__ bool refine_candidate(const float* __restrict__ D, in
 // ----------------------------------------------------------- orientation --
 // orientation_hist (core.py:228-333) for one keypoint on one wave: pixel p of
 // the window goes to lane p % 64's private 36-bin column P[bin*64 + lane].
-// Every lane returns; lane 0's out[]/n are the result.
+// Every lane returns; lane 0's out[]/n are the result.  Q16: samples are read
+// as the binary16 values of the fp16 mode's stored octaves.
+template <bool Q16>
 __device__ __forceinline__ int orient_keypoint(const float* __restrict__ G, int H, int W, float ri, float ci,
                                                float sigma, float* P, int lane, float* out) {
     __builtin_amdgcn_wave_barrier();     // the previous keypoint's reads of P are done
@@ -749,6 +760,7 @@ __device__ __forceinline__ int orient_keypoint(const float* __restrict__ G, int 
             if (p < total) {
                 const float* g = G + (size_t)rk[k] * W + ck[k];
                 gl[k] = g[-1]; gr[k] = g[1]; gu[k] = g[-W]; gd[k] = g[W];
+                if (Q16) { gl[k] = f16_round(gl[k]); gr[k] = f16_round(gr[k]); gu[k] = f16_round(gu[k]); gd[k] = f16_round(gd[k]); }
             }
         }
         // branch-free: pixels past the end add +0.0f (bins are sums of
@@ -994,7 +1006,7 @@ constexpr float kDescFxMax = 0x1.fffffep31f;
 // sends the 64 lanes of a wave to pixels spread over the patch, so their
 // atomics fall on different bins.  p advances by (kDescT K) mod side^2 as a
 // (row, col) step.  hw = this wave's bins.
-template <bool kInterior>
+template <bool kInterior, bool Q16>
 __device__ __forceinline__ void desc_hist_fx(const float* __restrict__ G, int H, int W, float ri, float ci,
                                              float angle_deg, int ph, unsigned long long* hw, int tid) {
     const float ang_rad = angle_deg * 0.0174532924f;
@@ -1037,6 +1049,7 @@ __device__ __forceinline__ void desc_hist_fx(const float* __restrict__ G, int H,
             if (ok) {
                 const float* g = G + (size_t)rr * W + cc;
                 gl[k] = g[-1]; gr[k] = g[1]; gu[k] = g[-W]; gd[k] = g[W];
+                if (Q16) { gl[k] = f16_round(gl[k]); gr[k] = f16_round(gr[k]); gu[k] = f16_round(gu[k]); gd[k] = f16_round(gd[k]); }
             }
         }
 #pragma unroll
@@ -1080,21 +1093,41 @@ __device__ __forceinline__ void desc_hist_fx(const float* __restrict__ G, int H,
     }
 }
 
-// sift_descriptor_coop (core.py:693-835) + RootSIFT for one keypoint on a
-// kDescT-thread block (every thread calls it): integer histogram in hist
-// (kDescCopies x 128, zeroed here), then wave 0 converts and normalises;
-// thread l < 64 returns bins 2l, 2l+1
+// sift_descriptor_coop (core.py:693-835), or with tab.n_dsp scales
+// sift_descriptor_dsp_coop (core.py:845-997: one histogram over the patch
+// half-sizes max(4, (int)(ph m + 0.5f)), divided by the number of scales),
+// + RootSIFT, for one keypoint on a kDescT-thread block (every thread calls
+// it): integer histogram in hist (kDescCopies x 128, zeroed here), then wave
+// 0 converts and normalises; thread l < 64 returns bins 2l, 2l+1.  q16: the
+// level is read as binary16 (fp16 mode, octaves >= 2).
+template <bool Q16>
+__device__ __forceinline__ void desc_hist_scale(const float* __restrict__ G, int H, int W, float ri, float ci,
+                                                float angle_deg, int ph, unsigned long long* hw, int tid) {
+    const int iri = (int)ri, ici = (int)ci;
+    if (iri - ph >= 2 && iri + ph < H - 2 && ici - ph >= 2 && ici + ph < W - 2)
+        desc_hist_fx<true, Q16>(G, H, W, ri, ci, angle_deg, ph, hw, tid);
+    else
+        desc_hist_fx<false, Q16>(G, H, W, ri, ci, angle_deg, ph, hw, tid);
+}
+
 __device__ __forceinline__ void describe_keypoint_block(const float* __restrict__ G, int H, int W, float ri,
                                                         float ci, float angle_deg, int ph, int rootsift,
+                                                        const KpTables& tab, bool q16,
                                                         unsigned long long* hist, float* v0o, float* v1o) {
     const int tid = threadIdx.x;
     for (int i = tid; i < kDescCopies * 128; i += kDescT) hist[i] = 0ull;
     __syncthreads();
-    const int iri = (int)ri, ici = (int)ci;
-    if (iri - ph >= 2 && iri + ph < H - 2 && ici - ph >= 2 && ici + ph < W - 2)
-        desc_hist_fx<true>(G, H, W, ri, ci, angle_deg, ph, hist + (tid >> 6) * 128, tid);
-    else
-        desc_hist_fx<false>(G, H, W, ri, ci, angle_deg, ph, hist + (tid >> 6) * 128, tid);
+    unsigned long long* hw = hist + (tid >> 6) * 128;
+    const int nd = tab.n_dsp > 0 ? tab.n_dsp : 1;
+    for (int d = 0; d < nd; ++d) {
+        int phd = ph;
+        if (tab.n_dsp > 0) {
+            phd = (int)((float)ph * tab.dsp[d] + 0.5f);
+            phd = phd < 4 ? 4 : phd;
+        }
+        if (q16) desc_hist_scale<true>(G, H, W, ri, ci, angle_deg, phd, hw, tid);
+        else desc_hist_scale<false>(G, H, W, ri, ci, angle_deg, phd, hw, tid);
+    }
     __syncthreads();
     float v0 = 0.0f, v1 = 0.0f;
     if (tid < 64) {
@@ -1104,7 +1137,13 @@ __device__ __forceinline__ void describe_keypoint_block(const float* __restrict_
             s0 += hist[w * 128 + 2 * tid];
             s1 += hist[w * 128 + 2 * tid + 1];
         }
-        desc_normalize((float)((double)s0 * 0x1p-30), (float)((double)s1 * 0x1p-30), rootsift, &v0, &v1);
+        float h0 = (float)((double)s0 * 0x1p-30), h1 = (float)((double)s1 * 0x1p-30);
+        if (tab.n_dsp > 0) {
+            const float inv_n = 1.0f / (float)tab.n_dsp;
+            h0 = h0 * inv_n;
+            h1 = h1 * inv_n;
+        }
+        desc_normalize(h0, h1, rootsift, &v0, &v1);
     }
     *v0o = v0;
     *v1o = v1;

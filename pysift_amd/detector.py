@@ -73,10 +73,6 @@ class PySIFT:
                  double_px_cap=16_000_000, device=None):
         if dsp_n_scales not in _DSP_PRESETS:
             raise ValueError(f"dsp_n_scales must be 3, 4, or 5 (got {dsp_n_scales})")
-        if dsp:
-            raise NotImplementedError("DSP-SIFT pooling (dsp=True) is not part of this build")
-        if fp16_pyramid:
-            raise NotImplementedError("fp16 pyramid storage (low-VRAM mode) is not part of this build")
         if orientation != "histogram" or descriptor != "sift":
             raise NotImplementedError("learned orientation/descriptor backends are not part of this build")
         self.n_octaves = 5 if n_octaves is None else int(n_octaves)
@@ -84,10 +80,12 @@ class PySIFT:
         self.sigma0 = float(sigma0)
         self.contrast_thresh = float(contrast_thresh)
         self.edge_thresh = float(edge_thresh)
-        self.dsp = False
+        self.dsp = bool(dsp)
         self.dsp_n_scales = dsp_n_scales
         self._dsp_scales_np = np.array(_DSP_PRESETS[dsp_n_scales], np.float32)
-        self.fp16_pyramid = False
+        # an MI355X is the reference's high-VRAM device (core.py:1207): float32
+        # storage unless fp16_pyramid=True asks for the low-VRAM numerics
+        self.fp16_pyramid = bool(fp16_pyramid) if fp16_pyramid is not None else False
         self.orientation = orientation
         self.descriptor = descriptor
         self.double_image = bool(double_image)
@@ -102,7 +100,7 @@ class PySIFT:
         self._params = _lib.MsiftParams(self.n_octaves, self.n_scales, self.sigma0, self.contrast_thresh,
                                         self.edge_thresh, int(self.double_image), int(self.rootsift),
                                         self.double_px_cap, self.max_cands_per_octave, self.max_keypoints,
-                                        5, 100, 1)
+                                        5, 100, 1, self.dsp_n_scales if self.dsp else 0, int(self.fp16_pyramid))
         L = _lib.lib()
         h = ctypes.c_void_p()
         _lib.check(L.msift_create(ctypes.byref(self._params), ctypes.byref(h)), "PySIFT")

@@ -1,8 +1,10 @@
 """Copies the reference's IMC determinism fixtures (data only) into
-tests/golden/imc: the keypoint positions/angles/sizes recorded on an RTX 3050
-(determinism_reference/imc_*.npz, manifest.json: n_octaves 4, S 3, sigma 1.6,
-ct 0.04, edge 10, fp16 pyramid, DSP-5 descriptors) and the JPEG inputs they
-were computed from.  Descriptors are not copied (DSP-5 mode)."""
+tests/golden/imc: the keypoint positions/angles/sizes and RootSIFT
+descriptors recorded on an RTX 3050 (determinism_reference/imc_*.npz,
+manifest.json: n_octaves 4, S 3, sigma 1.6, ct 0.04, edge 10, fp16 pyramid,
+DSP-5 descriptors) and the JPEG inputs they were computed from.  Descriptors
+are stored as float16 (relative step 2^-11, far below the comparison's
+cosine tolerance)."""
 import os
 import shutil
 import sys
@@ -24,6 +26,6 @@ if __name__ == "__main__":
         z = np.load(os.path.join(REF, "determinism_reference", name + ".npz"))
         np.savez_compressed(os.path.join(OUT, name + ".npz"), points=z["points"].astype(np.float32),
                             angles=z["angles"].astype(np.float32), sizes=z["sizes"].astype(np.float32),
-                            image_shape=z["image_shape"])
+                            descriptors=z["descriptors"].astype(np.float16), image_shape=z["image_shape"])
         shutil.copyfile(os.path.join(REF, img), os.path.join(OUT, name + ".jpg"))
         print(name, len(z["points"]))

@@ -221,10 +221,10 @@ def test_orientation_two_directions_give_two_peaks():
 
 
 # ------------------------------------------------------------- descriptor --
-def _desc_reference(G, ri, ci, ang_rad, ph):
-    """Float64 NumPy restatement of sift_descriptor_coop + RootSIFT."""
+def _desc_hist64(G, ri, ci, ang_rad, ph):
+    """Float64 NumPy restatement of one patch histogram of sift_descriptor_coop
+    (core.py:693-792; one scale of sift_descriptor_dsp_coop, core.py:883-950)."""
     H, W = G.shape
-    side = 2 * ph + 1
     ro, co = np.mgrid[-ph:ph + 1, -ph:ph + 1]
     rr, cc = int(ri) + ro, int(ci) + co
     inb = (rr >= 1) & (rr < H - 1) & (cc >= 1) & (cc < W - 1)
@@ -252,6 +252,23 @@ def _desc_reference(G, ri, ci, ang_rad, ph):
             for oo, wo in ((0, 1 - df), (1, df)):
                 idx = np.clip(yi0 + yo, 0, 3) * 32 + np.clip(xi0 + xo, 0, 3) * 8 + (oi0 + oo) % 8
                 np.add.at(hist, idx, wy * wx * wo * wmag)
+    return hist
+
+
+DSP_PRESETS = {3: [1 / math.sqrt(2), 1.0, math.sqrt(2)], 4: [0.5, 1 / math.sqrt(2), 1.0, math.sqrt(2)],
+               5: [0.5, 1 / math.sqrt(2), 1.0, math.sqrt(2), 2.0]}
+
+
+def _desc_reference(G, ri, ci, ang_rad, ph, dsp_n_scales=0):
+    """Float64 restatement of sift_descriptor_coop, or of
+    sift_descriptor_dsp_coop (the patch half-sizes max(4, (int)(ph m + 0.5)) in
+    float32, one histogram divided by the scale count; core.py:1039-1040,
+    884-885, 953-957), + RootSIFT."""
+    if dsp_n_scales:
+        phs = [max(4, int(np.float32(ph) * np.float32(m) + np.float32(0.5))) for m in DSP_PRESETS[dsp_n_scales]]
+        hist = sum(_desc_hist64(G, ri, ci, ang_rad, q) for q in phs) / len(phs)
+    else:
+        hist = _desc_hist64(G, ri, ci, ang_rad, ph)
     v = hist / math.sqrt((hist ** 2).sum() + 1e-14)
     v = np.minimum(v, 0.2)
     v = v / math.sqrt((v ** 2).sum() + 1e-14)
@@ -277,6 +294,85 @@ def test_descriptor_of_horizontal_ramp_uses_orientation_bin_zero():
     assert h[:, :, 1:].max() == 0.0 and h[:, :, 0].min() > 0.0
     z, _ = O.descriptor(np.zeros((80, 80), np.float32), 40.0, 40.0, np.float32(0.0), 8)
     assert not z.any()
+
+
+@pytest.mark.parametrize("ph,ang,n", [(21, 0.7, 5), (9, 2.5, 3), (27, 4.0, 4), (4, 1.0, 5)])
+def test_dsp_descriptor_matches_float64_restatement(ph, ang, n):
+    G = nd.gaussian_filter(np.random.default_rng(ph + n).random((300, 320)), 2.0).astype(np.float32)
+    d, u = O.descriptor(G, 150.3, 160.8, np.float32(ang), ph, dsp_n_scales=n)
+    ref = _desc_reference(G, 150.3, 160.8, float(np.float32(ang)), ph, dsp_n_scales=n)
+    np.testing.assert_allclose(d, ref, atol=2e-4)
+    assert abs(float((d.astype(np.float64) ** 2).sum()) - 1.0) < 1e-4
+    plain, _ = O.descriptor(G, 150.3, 160.8, np.float32(ang), ph)
+    assert np.abs(plain - d).max() > 1e-3          # the pooling changes the descriptor
+
+
+def test_f16_round_matches_numpy():
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.standard_normal(100000).astype(np.float32) * np.float32(s)
+                        for s in (1e-8, 1e-6, 1e-4, 1e-2, 1, 1e2, 3e4)])
+    bits = np.arange(0, 0x7bff, 7, dtype=np.uint16)
+    h = bits.view(np.float16).astype(np.float32)
+    h1 = (bits + 1).view(np.float16).astype(np.float32)
+    ties = ((h.astype(np.float64) + h1) / 2).astype(np.float32)        # exact midpoints: ties to even
+    x = np.concatenate([x, h, -h, np.nextafter(h, np.float32(np.inf)), ties, -ties,
+                        np.float32([0.0, -0.0, 65504, 65519.996, 65520, 1e9, -1e9, 2.9e-8, 3e-8, 6e-8])])
+    with np.errstate(over="ignore"):
+        ref = x.astype(np.float16).astype(np.float32)
+    np.testing.assert_array_equal(O.f16_round(x), ref)
+
+
+def test_fp16_pyramid_levels_and_base():
+    img = syn.synth_gray(120, 160, 5)
+    p = O.params(n_octaves=4, fp16_pyramid=True)
+    # octave 0 base: the binary16 u8/255 array, zoomed, cast to binary16 again,
+    # then blurred in float32 (core.py:1256, 1274, 1286-1288)
+    g0, _ = O.pyramid_octave(img, p, 0)
+    up = O.f16_round(O.upsample2x(O.f16_round(img.astype(np.float32) / np.float32(255.0))))
+    np.testing.assert_array_equal(g0[0], O.blur(up, math.sqrt(1.6 ** 2 - 1.0)))
+    for o in range(4):
+        g, d = O.pyramid_octave(img, p, o)
+        np.testing.assert_array_equal(d, g[1:] - g[:-1])
+        # octaves 0-1 stay float32, octaves >= 2 hold binary16 values
+        assert np.array_equal(O.f16_round(g), g) == (o >= 2)
+    # the chain is float32: octave 2 of the plain pyramid with the same
+    # binary16 base, rounded, gives the fp16 mode's stored levels
+    g2, _ = O.pyramid_octave(img, p, 2)
+    g1, _ = O.pyramid_octave(img, p, 1)
+    np.testing.assert_array_equal(g2[0], O.f16_round(g1[3][::2, ::2]))
+
+
+@pytest.mark.parametrize("name", ["imc_british_museum", "imc_florence_cathedral_side", "imc_st_pauls_cathedral"])
+def test_reference_fixture_dsp5_fp16_all_octaves(name):
+    """The reference's RTX 3050 output in the mode it was recorded in (DSP-5
+    descriptors, binary16 pyramid, 4 octaves; manifest.json): all octaves'
+    keypoints within 0.5 px, and the descriptors of keypoints matched by
+    position and angle (a location with several orientations holds several
+    keypoints) at cosine >= 0.9999 (the reference's EQUIVALENT bar is 95 % and
+    0.99, tests/cross_determinism_test.py:112-113)."""
+    from PIL import Image
+    from scipy.spatial import cKDTree
+    z = np.load(os.path.join(GOLDEN, "imc", name + ".npz"))
+    im = Image.open(os.path.join(GOLDEN, "imc", name + ".jpg"))
+    im.draft("L", im.size)
+    g = np.asarray(im.convert("L"))
+    kp, d, u = O.detect_describe(g, O.params_low_vram(dsp_n_scales=5))
+    dist, _ = cKDTree(kp[:, :2]).query(z["points"])
+    assert (dist <= 0.5).mean() > 0.995
+    assert abs(len(kp) - len(z["points"])) <= 0.002 * len(z["points"])
+    a = np.radians(kp[:, 3]); ar = np.radians(z["angles"])
+    X = np.c_[kp[:, :2], 50 * np.cos(a), 50 * np.sin(a)]
+    Y = np.c_[z["points"], 50 * np.cos(ar), 50 * np.sin(ar)]
+    dist2, idx = cKDTree(X).query(Y)
+    m = dist2 < 0.5
+    assert m.mean() > 0.995
+    rd = z["descriptors"][m].astype(np.float64)
+    nd = d[idx[m]].astype(np.float64)
+    cos = (rd * nd).sum(1) / (np.linalg.norm(rd, axis=1) * np.linalg.norm(nd, axis=1) + 1e-12)
+    assert cos.mean() > 0.9999 and cos.min() > 0.999
+    # uint8 descriptors (512 scale) within 1 LSB of the reference's values
+    ref_u8 = np.minimum(255, (z["descriptors"][m].astype(np.float32) * 512 + 0.5).astype(np.int64))
+    assert np.mean(np.abs(ref_u8 - u[idx[m]].astype(np.int64)) <= 1) > 0.999
 
 
 # --------------------------------------------------------- full detection --
