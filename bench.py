@@ -364,6 +364,31 @@ def bench_config4(torch, dev, iters=3, check=True):
     return res
 
 
+def bench_clahe(torch, dev, gray, iters=50):
+    """CLAHE(2.0, 8 x 8) as the stitcher runs it ahead of detection, on one
+    1024x768 view and on a 4000x3000 image, timed with HIP events on the
+    launch stream.  Algorithmic HBM bytes: 3 per pixel (the histogram pass and
+    the apply pass each read the image once, the apply pass writes it)."""
+    from pysift_amd import synthetic as syn
+    from pysift_amd.clahe import CLAHE
+    c = CLAHE(2.0, (8, 8))
+    res = {}
+    for name, arr in (("1024x768", gray), ("4000x3000", syn.synth_gray(3000, 4000, 5))):
+        img = torch.from_numpy(arr).to(dev)
+        out = torch.empty_like(img)
+        c.apply(img, dst=out)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(iters):
+            c.apply(img, dst=out)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / iters
+        res[name] = {"ms": round(ms, 4), "achieved_gbs": round(3 * img.numel() / (ms * 1e-3) / 1e9, 1)}
+    return res
+
+
 def bench_grad(torch, dev, B=16, H=512, W=512, K=1024, iters=5):
     """16 float 512x512 images, 1024 fixed keypoints each, loss sum(W * desc):
     HIP forward and hand-written backward (pyramid recomputed), and one
@@ -412,7 +437,7 @@ def main():
                     help="(counter calibration) one 256 MiB device-to-device copy, then exit")
     ap.add_argument("--match-n", type=int, default=40000, help="descriptors per side for the match TOPS leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the match / stitch legs")
-    ap.add_argument("--only", choices=["match", "batch", "stitch", "grad", "pairs", "config4"], default=None,
+    ap.add_argument("--only", choices=["match", "batch", "stitch", "grad", "pairs", "config4", "clahe"], default=None,
                     help="(profiling) run a single extra leg and print its numbers")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -449,6 +474,10 @@ def main():
             print(json.dumps(bench_pairs(torch, dev)))
         elif args.only == "config4":
             print(json.dumps(bench_config4(torch, dev)))
+        elif args.only == "clahe":
+            g = ((va[..., 0].astype(np.int32) * 1868 + va[..., 1].astype(np.int32) * 9617
+                  + va[..., 2].astype(np.int32) * 4899 + 8192) >> 14).astype(np.uint8)
+            print(json.dumps(bench_clahe(torch, dev, g)))
         else:
             print(json.dumps(bench_stitch(va, vb)))
         return
@@ -518,6 +547,7 @@ def main():
         extra["batch64"] = bench_batch(torch, dev)
         extra["pairs32"] = bench_pairs(torch, dev)
         extra["grad16x512"] = bench_grad(torch, dev)
+        extra["clahe"] = bench_clahe(torch, dev, gray[0])
         st_ms, met = bench_stitch(va, vb)
         extra["stitch_pair_ms"] = round(st_ms, 3)
         extra["stitch_metrics"] = {k: met[k] for k in ("kp1", "kp2", "matches", "inliers")}

@@ -216,6 +216,18 @@ int msift_warp_feather(const uint8_t* d_img1, int h1, int w1, const uint8_t* d_i
 /* cv2.cvtColor(BGR2GRAY) fixed-point BT.601 (core.py:2548). */
 int msift_bgr_to_gray(const uint8_t* d_bgr, int H, int W, uint8_t* d_gray, void* stream);
 
+/* cv2.createCLAHE(clipLimit, (tiles_x, tiles_y)).apply(gray) (core.py:2562-2565,
+ * applied to every image ahead of detectAndCompute) with OpenCV's 8-bit
+ * semantics: BORDER_REFLECT_101 tiling of ragged images, clip limit
+ * max(1, (int)(clip * tile_area / 256)) (clip_limit <= 0: no clipping),
+ * redistribution of the clipped count, bilinear blend of the 4 nearest tile
+ * LUTs.  d_src is H x W with row pitch `pitch` bytes; d_dst is dense H x W
+ * (d_dst == d_src with pitch == W runs in place).  1 <= tiles <= 128; d_ws is
+ * 4-byte aligned. */
+int msift_clahe_workspace_bytes(int tiles_x, int tiles_y, size_t* bytes);
+int msift_clahe(const uint8_t* d_src, int H, int W, int pitch, double clip_limit, int tiles_x, int tiles_y,
+                void* d_ws, size_t ws_bytes, uint8_t* d_dst, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,7 @@ def _declare(L):
     L.or_atan2_f.restype = ctypes.c_float; L.or_atan2_f.argtypes = [ctypes.c_float, ctypes.c_float]
     L.or_exp_f_arr.restype = None; L.or_exp_f_arr.argtypes = [_P, _P, ctypes.c_int]
     L.or_f16_round_arr.restype = None; L.or_f16_round_arr.argtypes = [_P, _P, ctypes.c_int]
+    L.or_clahe.restype = _I; L.or_clahe.argtypes = [_P, _I, _I, _I, _D, _I, _I, _P]
     L.or_descriptor_dsp.restype = None
     L.or_descriptor_dsp.argtypes = [_P, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, _I, _I, _I, _P, _P]
     L.or_atan2_f_arr.restype = None; L.or_atan2_f_arr.argtypes = [_P, _P, _P, ctypes.c_int]
@@ -344,3 +345,13 @@ def f16_round(x):
     y = np.empty_like(x)
     lib().or_f16_round_arr(_ptr(x), _ptr(y), x.size)
     return y
+
+
+def clahe(img, clip=2.0, tiles=(8, 8)):
+    """CLAHE with OpenCV semantics on a u8 image (core.py:2562-2565); tiles = (x, y)."""
+    img = np.ascontiguousarray(img, np.uint8)
+    H, W = img.shape
+    out = np.empty_like(img)
+    if lib().or_clahe(_ptr(img), H, W, W, float(clip), int(tiles[0]), int(tiles[1]), _ptr(out)) != 0:
+        raise ValueError("bad CLAHE arguments")
+    return out

@@ -26,6 +26,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <float.h>
+#include <limits.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -1021,6 +1022,95 @@ int or_detect_describe(const uint8_t* img, int H, int W, int pitch, const or_par
     free(order); free(ok); free(ang); free(sorted); free(kps);
     pyr_free(&P);
     return nout;
+}
+
+/* ========================================================================
+ * A12: CLAHE (core.py:2562-2565: cv2.createCLAHE(clipLimit, (t, t)).apply on
+ * the u8 gray image), restated from OpenCV's published algorithm:
+ *   - when H or W is not a multiple of the tile count, the image is padded
+ *     at the bottom by tiles_y - H % tiles_y and at the right by
+ *     tiles_x - W % tiles_x (BORDER_REFLECT_101) before it is tiled;
+ *   - per tile a 256-bin histogram, clipped at max(1, (int)(clip * area /
+ *     256)); the clipped count is spread evenly, and the remainder adds one
+ *     to every max(1, 256 / remainder)-th bin from bin 0;
+ *   - LUT[v] = saturate(round_even((float)cumsum[v] * (255.0f / area)));
+ *   - each pixel blends the LUTs of its 4 nearest tile centres bilinearly in
+ *     float: ((l11 xa1 + l12 xa) ya1 + (l21 xa1 + l22 xa) ya), rounded.
+ * ======================================================================== */
+static int reflect101(int p, int len)
+{
+    if (len == 1) return 0;
+    while (p < 0 || p >= len) p = p < 0 ? -p : 2 * len - 2 - p;
+    return p;
+}
+
+static uint8_t sat_round_u8(float v)
+{
+    int r = (int)lrintf(v);
+    return (uint8_t)(r < 0 ? 0 : (r > 255 ? 255 : r));
+}
+
+int or_clahe(const uint8_t* src, int H, int W, int pitch, double clip, int tiles_x, int tiles_y, uint8_t* dst)
+{
+    if (H < 1 || W < 1 || tiles_x < 1 || tiles_y < 1) return -1;
+    int He = H, We = W;
+    if (H % tiles_y != 0 || W % tiles_x != 0) { He = H + tiles_y - H % tiles_y; We = W + tiles_x - W % tiles_x; }
+    const int tw = We / tiles_x, th = He / tiles_y, area = tw * th;
+    const float lut_scale = 255.0f / (float)area;
+    int limit = 0;
+    if (clip > 0.0) {
+        const double l = clip * area / 256;
+        limit = l >= (double)INT_MAX ? INT_MAX : (int)l;
+        if (limit < 1) limit = 1;
+    }
+    uint8_t* lut = (uint8_t*)malloc((size_t)tiles_x * tiles_y * 256);
+    for (int t = 0; t < tiles_x * tiles_y; ++t) {
+        const int ty = t / tiles_x, tx = t % tiles_x;
+        int hist[256];
+        memset(hist, 0, sizeof(hist));
+        for (int y = ty * th; y < (ty + 1) * th; ++y)
+            for (int x = tx * tw; x < (tx + 1) * tw; ++x)
+                hist[src[(size_t)reflect101(y, H) * pitch + reflect101(x, W)]]++;
+        if (limit > 0) {
+            int clipped = 0;
+            for (int i = 0; i < 256; ++i)
+                if (hist[i] > limit) { clipped += hist[i] - limit; hist[i] = limit; }
+            const int batch = clipped / 256;
+            int residual = clipped - batch * 256;
+            for (int i = 0; i < 256; ++i) hist[i] += batch;
+            if (residual != 0) {
+                const int step = 256 / residual > 1 ? 256 / residual : 1;
+                for (int i = 0; i < 256 && residual > 0; i += step, residual--) hist[i]++;
+            }
+        }
+        int sum = 0;
+        for (int i = 0; i < 256; ++i) {
+            sum += hist[i];
+            lut[(size_t)t * 256 + i] = sat_round_u8((float)sum * lut_scale);
+        }
+    }
+    const float inv_tw = 1.0f / (float)tw, inv_th = 1.0f / (float)th;
+    for (int y = 0; y < H; ++y) {
+        const float tyf = (float)y * inv_th - 0.5f;
+        int ty1 = (int)floorf(tyf), ty2 = ty1 + 1;
+        const float ya = tyf - (float)ty1, ya1 = 1.0f - ya;
+        if (ty1 < 0) ty1 = 0;
+        if (ty2 > tiles_y - 1) ty2 = tiles_y - 1;
+        for (int x = 0; x < W; ++x) {
+            const float txf = (float)x * inv_tw - 0.5f;
+            int tx1 = (int)floorf(txf), tx2 = tx1 + 1;
+            const float xa = txf - (float)tx1, xa1 = 1.0f - xa;
+            if (tx1 < 0) tx1 = 0;
+            if (tx2 > tiles_x - 1) tx2 = tiles_x - 1;
+            const int v = src[(size_t)y * pitch + x];
+            const float l11 = lut[((size_t)ty1 * tiles_x + tx1) * 256 + v], l12 = lut[((size_t)ty1 * tiles_x + tx2) * 256 + v];
+            const float l21 = lut[((size_t)ty2 * tiles_x + tx1) * 256 + v], l22 = lut[((size_t)ty2 * tiles_x + tx2) * 256 + v];
+            const float res = (l11 * xa1 + l12 * xa) * ya1 + (l21 * xa1 + l22 * xa) * ya;
+            dst[(size_t)y * W + x] = sat_round_u8(res);
+        }
+    }
+    free(lut);
+    return 0;
 }
 
 /* ========================================================================

@@ -98,6 +98,8 @@ int  or_extrema_octave(const float* D, int S, int H, int W, float thr, int borde
 int  or_refine_octave(const float* D, int S, int H, int W, int border, float thr, float edge,
                       const int* cands, int n, int* valid, float* out);
 int  or_orientation(const float* G, int H, int W, float ri, float ci, float sigma, float* angles);
+/* CLAHE with OpenCV semantics on a u8 image (core.py:2562-2565) */
+int  or_clahe(const uint8_t* src, int H, int W, int pitch, double clip, int tiles_x, int tiles_y, uint8_t* dst);
 void or_descriptor_dsp(const float* G, int H, int W, float ri, float ci, float ang_rad, int ph, int rootsift,
                        int dsp_n_scales, float* out, uint8_t* out_u8);
 void or_descriptor(const float* G, int H, int W, float ri, float ci, float ang_rad, int ph, int rootsift,

@@ -25,7 +25,7 @@ EXPORTS = [
     "msift_desc_forward", "msift_desc_backward", "msift_grad_level_view", "msift_match_batch_create",
     "msift_match_batch_destroy", "msift_match_batch_sizes", "msift_match_u8_batched",
     "msift_match_batch_gather_points", "msift_ransac_batch_create", "msift_ransac_batch_destroy",
-    "msift_ransac_batch_sizes", "msift_ransac_homography_batched",
+    "msift_ransac_batch_sizes", "msift_ransac_homography_batched", "msift_clahe_workspace_bytes", "msift_clahe",
 ]
 
 
@@ -74,6 +74,10 @@ def _declare(L):
     L.msift_warp_feather.argtypes = [_P, _I, _I, _P, _I, _I, _P, _I, _I, _D, _D, _P, _P, _P]
     L.msift_bgr_to_gray.restype = _I
     L.msift_bgr_to_gray.argtypes = [_P, _I, _I, _P, _P]
+    L.msift_clahe_workspace_bytes.restype = _I
+    L.msift_clahe_workspace_bytes.argtypes = [_I, _I, ctypes.POINTER(_SZ)]
+    L.msift_clahe.restype = _I
+    L.msift_clahe.argtypes = [_P, _I, _I, _I, _D, _I, _I, _P, _SZ, _P, _P]
     L.msift_batch_create.restype = _I
     L.msift_batch_create.argtypes = [_P, _I, _P, _P, ctypes.POINTER(_P)]
     L.msift_batch_destroy.restype = _I; L.msift_batch_destroy.argtypes = [_P]

@@ -212,4 +212,18 @@ int msift_bgr_to_gray(const uint8_t* d_bgr, int H, int W, uint8_t* d_gray, void*
     return bgr_to_gray(d_bgr, H, W, d_gray, (hipStream_t)stream);
 }
 
+int msift_clahe_workspace_bytes(int tiles_x, int tiles_y, size_t* bytes) {
+    if (tiles_x < 1 || tiles_y < 1 || tiles_x > 128 || tiles_y > 128 || !bytes) {
+        set_error("msift_clahe_workspace_bytes: bad arguments (1 <= tiles <= 128)");
+        return MSIFT_EINVAL;
+    }
+    *bytes = clahe_workspace(tiles_x, tiles_y);
+    return MSIFT_OK;
+}
+
+int msift_clahe(const uint8_t* d_src, int H, int W, int pitch, double clip_limit, int tiles_x, int tiles_y,
+                void* d_ws, size_t ws_bytes, uint8_t* d_dst, void* stream) {
+    return clahe(d_src, H, W, pitch, clip_limit, tiles_x, tiles_y, d_ws, ws_bytes, d_dst, (hipStream_t)stream);
+}
+
 }  // extern "C"

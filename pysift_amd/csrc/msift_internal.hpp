@@ -97,6 +97,9 @@ int warp_feather(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h
                  const double* H, int cw, int ch, double ox, double oy, uint8_t* pano,
                  uint8_t* valid, hipStream_t s);
 int bgr_to_gray(const uint8_t* bgr, int H, int W, uint8_t* gray, hipStream_t s);
+size_t clahe_workspace(int tiles_x, int tiles_y);
+int clahe(const uint8_t* src, int H, int W, int pitch, double clip, int tiles_x, int tiles_y, void* ws,
+          size_t ws_bytes, uint8_t* dst, hipStream_t s);
 
 // scan.hip helpers (device-wide exclusive scan of int32, single workgroup)
 // with tmp (scan_tmp_ints(n) ints) large inputs use a multi-workgroup scan

@@ -15,6 +15,7 @@ from datetime import datetime
 import numpy as np
 
 from . import _lib
+from .clahe import CLAHE
 from .detector import DMatch, PySIFT
 from .matcher import match_u8
 from .ransac import ransac_homography
@@ -94,6 +95,7 @@ class GPUPyStitch:
         self.color_match = color_match
         self.clahe_clip = clahe_clip
         self.clahe_tile = clahe_tile
+        self._clahe = CLAHE(clipLimit=clahe_clip, tileGridSize=(clahe_tile, clahe_tile))
         self.output_format = output_format.lower()
         self.output_quality = output_quality
         self.match_method = match_method
@@ -134,10 +136,13 @@ class GPUPyStitch:
         return g
 
     def _extract_features(self, bgr):
-        """_extract_features (core.py:2535-2572) without CLAHE: gray ->
-        detectAndCompute -> keep the max_keypoints strongest (stable)."""
+        """_extract_features (core.py:2535-2572): gray -> CLAHE(clahe_clip,
+        clahe_tile x clahe_tile) in place -> detectAndCompute -> keep the
+        max_keypoints strongest (stable)."""
         torch = _torch()
-        kp, _, du = self._detector.detect_raw(self._gray(bgr))
+        gray = self._gray(bgr)
+        self._clahe.apply(gray, dst=gray)
+        kp, _, du = self._detector.detect_raw(gray)
         if kp.shape[0] > self.max_keypoints:
             order = torch.sort(kp[:, 4], descending=True, stable=True).indices[:self.max_keypoints]
             kp, du = kp[order], du[order]

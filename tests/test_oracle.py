@@ -546,6 +546,100 @@ def test_warp_identity_and_translation():
     np.testing.assert_array_equal(pano[3:43, 50:57], img[0:40, 43:50])
 
 
+# ---- CLAHE (core.py:2562-2565, cv2.createCLAHE(...).apply) ----
+
+def test_clahe_constant_image_known_answers():
+    # 64x64, 8x8 tiles: tile area 64, limit max(1, (int)(2 * 64 / 256)) = 1.
+    # Constant 100: clipped 63 -> batch 0, residual 63, step 256 // 63 = 4, so
+    # bins 0, 4, ..., 248 get one each; cumsum[100] = 25 (multiples of 4
+    # below 100) + 1 (limit) + 1 (100 is one of them) = 27 -> round(27 * 255 / 64) = 108.
+    assert (O.clahe(np.full((64, 64), 100, np.uint8), 2.0, (8, 8)) == 108).all()
+    # constant 90 (not a multiple of 4): 23 + 1 = 24 -> 95.625 -> 96
+    assert (O.clahe(np.full((64, 64), 90, np.uint8), 2.0, (8, 8)) == 96).all()
+    # one 64x64 tile: area 4096, limit 32; clipped 4064 -> batch 15,
+    # residual 224, step 1 (bins 0..223 get one more).
+    # v = 100: 100 * 16 + (32 + 15 + 1) = 1648 -> 102.598 -> 103
+    # v = 230: 230 * 15 + 224 + (32 + 15) = 3721 -> 231.65 -> 232
+    assert (O.clahe(np.full((64, 64), 100, np.uint8), 2.0, (1, 1)) == 103).all()
+    assert (O.clahe(np.full((64, 64), 230, np.uint8), 2.0, (1, 1)) == 232).all()
+
+
+def test_clahe_two_level_tile_known_answer():
+    # one 64x64 tile, half 10 and half 240: each bin holds 2048 > limit 32,
+    # clipped 2 * 2016 = 4032 -> batch 15, residual 192, step 1.
+    # LUT[10]  = 10 * 16 + (32 + 15 + 1) = 208 -> 12.95 -> 13
+    # LUT[240] = 240 * 15 + 192 + (48 - 16) + (32 + 15) = 3871 -> 240.99 -> 241
+    img = np.full((64, 64), 10, np.uint8)
+    img[:, 32:] = 240
+    out = O.clahe(img, 2.0, (1, 1))
+    assert (out[:, :32] == 13).all() and (out[:, 32:] == 241).all()
+
+
+def test_clahe_one_tile_no_clip_is_histogram_equalisation():
+    # clip 0 and a single tile reduce to textbook equalisation,
+    # round_even(cdf[v] * 255 / N) (float32 product as OpenCV's LUT)
+    rng = np.random.default_rng(5)
+    img = np.clip(rng.normal(90, 30, (57, 83)), 0, 255).astype(np.uint8)
+    cdf = np.cumsum(np.bincount(img.ravel(), minlength=256))
+    lut = np.rint(cdf.astype(np.float32) * np.float32(255.0 / img.size)).astype(np.uint8)
+    np.testing.assert_array_equal(O.clahe(img, 0.0, (1, 1)), lut[img])
+    # with clipping the one-tile map stays monotone in the input value
+    out = O.clahe(img, 2.0, (1, 1))
+    order = np.argsort(img.ravel(), kind="stable")
+    assert (np.diff(out.ravel()[order].astype(int)) >= 0).all()
+
+
+def test_clahe_bilinear_ramp_between_tile_centres():
+    # two 32-wide tiles, left all 50, right all 200, no clipping:
+    # LUT_left[v] = 255 for v >= 50 else 0, LUT_right[v] = 255 for v >= 200.
+    # Right pixels map to 255 through both; left pixels get 255 times the
+    # left tile's weight: 1 up to its centre (x = 15.5), then 1.5 - x / 32.
+    img = np.full((8, 64), 50, np.uint8)
+    img[:, 32:] = 200
+    x = np.arange(64)
+    exp = np.where((x >= 16) & (x < 32), np.rint(255.0 * (1.5 - x / 32.0)), 255).astype(np.uint8)
+    out = O.clahe(img, 0.0, (2, 1))
+    np.testing.assert_array_equal(out, np.broadcast_to(exp, (8, 64)))
+    # the same along y (tiles (1, 2) on the transposed image)
+    np.testing.assert_array_equal(O.clahe(np.ascontiguousarray(img.T), 0.0, (1, 2)), np.broadcast_to(exp, (8, 64)).T)
+    # right tile: 200 pixels of 30 (columns 32..56) and 56 of 50, so
+    # LUT_right[30] = round(200 * 255 / 256) = 199 and LUT_left[30] = 0; the
+    # 30s left of the right centre blend to 199 (x / 32 - 0.5), fractions
+    # above one half (rounded, not truncated) and a tie 99.5 -> 100 at x = 32
+    img = np.full((8, 64), 50, np.uint8)
+    img[:, 32:57] = 30
+    exp = np.full(64, 255.0)
+    exp[32:48] = np.rint(199.0 * (x[32:48] / 32.0 - 0.5))
+    exp[48:57] = 199
+    assert exp[32] == 100 and exp[33] == 106
+    np.testing.assert_array_equal(O.clahe(img, 0.0, (2, 1)), np.broadcast_to(exp.astype(np.uint8), (8, 64)))
+
+
+def test_clahe_ragged_image_tiles_reflect101_padding():
+    # H = 61, W = 60, 4 x 4 tiles: H is ragged, so the image is padded by 3
+    # rows AND by 4 - 60 % 4 = 4 columns (OpenCV pads both axes), giving
+    # 16 x 16 tiles of the 64 x 64 BORDER_REFLECT_101 extension (numpy's
+    # "reflect").  At a tile centre (16 m + 8, 16 k + 8) the bilinear weights
+    # are exactly (1, 0), so the output is that tile's own equalisation LUT.
+    rng = np.random.default_rng(9)
+    img = rng.integers(0, 256, (61, 60)).astype(np.uint8)
+    img[:30, :30] //= 3
+    ext = np.pad(img, ((0, 3), (0, 4)), mode="reflect")
+    out = O.clahe(img, 0.0, (4, 4))
+    for m in range(4):
+        for k in range(4):
+            tile = ext[16 * m:16 * m + 16, 16 * k:16 * k + 16]
+            cdf = np.cumsum(np.bincount(tile.ravel(), minlength=256))
+            lut = np.rint(cdf * 255.0 / 256.0).astype(np.uint8)
+            y, x = 16 * m + 8, 16 * k + 8
+            assert out[y, x] == lut[img[y, x]], (m, k)
+
+
+def test_clahe_rejects_bad_arguments():
+    with pytest.raises(ValueError):
+        O.clahe(np.zeros((8, 8), np.uint8), 2.0, (0, 8))
+
+
 def test_bgr_to_gray_formula():
     rng = np.random.default_rng(11)
     bgr = rng.integers(0, 256, (13, 17, 3)).astype(np.uint8)
