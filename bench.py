@@ -364,6 +364,52 @@ def bench_config4(torch, dev, iters=3, check=True):
     return res
 
 
+def bench_match_float(torch, dev, sizes=(8000, 40000), iters=5):
+    """The reference's float matching path (PCA-64 + binary16 similarity +
+    symmetric ratio test) on two n x 128 unit descriptor sets: the PCA fit
+    (once per stitcher in the reference), the projection of both sets, and
+    the similarity match; TOPS of the match counted once (2 n^2 64 / t)."""
+    from pysift_amd.matcher import (l2_normalize, match_sim_f16, match_workspace_bytes, pca_fit,
+                                    pca_project)
+    res = {}
+    for n in sizes:
+        g = torch.Generator(device="cpu").manual_seed(5)
+        a = torch.rand((n, 128), generator=g) ** 3
+        b = a.clone()
+        b[: n // 2] = a[: n // 2] + 0.01 * torch.randn((n // 2, 128), generator=g)
+        b[n // 2:] = torch.rand((n - n // 2, 128), generator=g) ** 3
+        ta, tb = a.to(dev), b.to(dev)
+        ws = torch.empty(match_workspace_bytes(n, n), dtype=torch.uint8, device=dev)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        mean, proj, _, sweeps = pca_fit(ta, tb, 64)
+        torch.cuda.synchronize()
+        e0.record()
+        mean, proj, _, sweeps = pca_fit(ta, tb, 64)
+        e1.record()
+        torch.cuda.synchronize()
+        t_fit = e0.elapsed_time(e1)
+        na, nb = l2_normalize(ta), l2_normalize(tb)
+        e0.record()
+        for _ in range(iters):
+            pa, pb = pca_project(na, mean, proj), pca_project(nb, mean, proj)
+        e1.record()
+        torch.cuda.synchronize()
+        t_proj = e0.elapsed_time(e1) / iters
+        _, _, pairs = match_sim_f16(pa, pb, 0.75, ws=ws)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(iters):
+            _, _, pairs = match_sim_f16(pa, pb, 0.75, ws=ws)
+        e1.record()
+        torch.cuda.synchronize()
+        t_match = e0.elapsed_time(e1) / iters
+        res[str(n)] = {"pca_fit_ms": round(t_fit, 3), "jacobi_sweeps": int(sweeps.item()),
+                       "project_both_ms": round(t_proj, 4), "match_ms": round(t_match, 4),
+                       "match_tops": round(2.0 * n * n * 64 / (t_match * 1e-3) / 1e12, 2),
+                       "fp16_mfma_peak_tops": 2500.0, "pairs": int(pairs.shape[0])}
+    return res
+
+
 def bench_clahe(torch, dev, gray, iters=50):
     """CLAHE(2.0, 8 x 8) as the stitcher runs it ahead of detection, on one
     1024x768 view and on a 4000x3000 image, timed with HIP events on the
@@ -437,7 +483,7 @@ def main():
                     help="(counter calibration) one 256 MiB device-to-device copy, then exit")
     ap.add_argument("--match-n", type=int, default=40000, help="descriptors per side for the match TOPS leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the match / stitch legs")
-    ap.add_argument("--only", choices=["match", "batch", "stitch", "grad", "pairs", "config4", "clahe"], default=None,
+    ap.add_argument("--only", choices=["match", "batch", "stitch", "grad", "pairs", "config4", "clahe", "match_float"], default=None,
                     help="(profiling) run a single extra leg and print its numbers")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -474,6 +520,8 @@ def main():
             print(json.dumps(bench_pairs(torch, dev)))
         elif args.only == "config4":
             print(json.dumps(bench_config4(torch, dev)))
+        elif args.only == "match_float":
+            print(json.dumps(bench_match_float(torch, dev)))
         elif args.only == "clahe":
             g = ((va[..., 0].astype(np.int32) * 1868 + va[..., 1].astype(np.int32) * 9617
                   + va[..., 2].astype(np.int32) * 4899 + 8192) >> 14).astype(np.uint8)
@@ -548,6 +596,7 @@ def main():
         extra["pairs32"] = bench_pairs(torch, dev)
         extra["grad16x512"] = bench_grad(torch, dev)
         extra["clahe"] = bench_clahe(torch, dev, gray[0])
+        extra["match_float_pca64"] = bench_match_float(torch, dev)
         st_ms, met = bench_stitch(va, vb)
         extra["stitch_pair_ms"] = round(st_ms, 3)
         extra["stitch_metrics"] = {k: met[k] for k in ("kp1", "kp2", "matches", "inliers")}

@@ -216,6 +216,36 @@ int msift_warp_feather(const uint8_t* d_img1, int h1, int w1, const uint8_t* d_i
 /* cv2.cvtColor(BGR2GRAY) fixed-point BT.601 (core.py:2548). */
 int msift_bgr_to_gray(const uint8_t* d_bgr, int H, int W, uint8_t* d_gray, void* stream);
 
+/* The reference's float matcher (core.py:2840-2940 with int8_matching off,
+ * _match_symmetric core.py:2652-2701, _pca_compress core.py:2703-2743):
+ *   msift_l2_normalize   rows of 128 floats -> x / (||x|| + 1e-7)
+ *                        (core.py:2879-2880);
+ *   msift_pca_fit        PCA of the normalised rows of a and b stacked
+ *                        (core.py:2728-2736): d_mean[128], d_proj[128 * dims]
+ *                        (columns = eigenvectors of the dims largest
+ *                        eigenvalues, ascending, as eigh(cov)[:, -dims:]),
+ *                        d_evals[128] descending (double), *d_sweeps =
+ *                        Jacobi sweeps; FP64, deterministic; workspace from
+ *                        msift_pca_workspace_bytes, 8-byte aligned;
+ *   msift_pca_project    unit rows -> ((x - mean) @ proj) / (||.|| + 1e-7)
+ *                        (core.py:2737-2742), n x dims;
+ *   msift_match_sim_f16  unit rows a (N x D), b (M x D), 1 <= D <= 128:
+ *                        binary16 similarity on FP16 MFMA (fp32 accumulation,
+ *                        result rounded to binary16), top-2 by similarity in
+ *                        both directions, ratio test on d = max(2 - 2 s, 0),
+ *                        mutual pairs; fwd/bwd triples (index, d1, d2) hold d
+ *                        as float32 bits; workspace from
+ *                        msift_match_workspace_bytes(N, M). */
+int msift_l2_normalize(const float* d_x, int n, float* d_y, void* stream);
+int msift_pca_workspace_bytes(int N, int M, size_t* bytes);
+int msift_pca_fit(const float* d_a, int N, const float* d_b, int M, int dims, void* d_ws, size_t ws_bytes,
+                  float* d_mean, float* d_proj, double* d_evals, int* d_sweeps, void* stream);
+int msift_pca_project(const float* d_x, int n, const float* d_mean, const float* d_proj, int dims, float* d_y,
+                      void* stream);
+int msift_match_sim_f16(const float* d_a, int N, const float* d_b, int M, int D, double ratio, void* d_ws,
+                        size_t ws_bytes, int32_t* d_fwd, int32_t* d_bwd, int32_t* d_pairs, int* d_npairs,
+                        void* stream);
+
 /* cv2.createCLAHE(clipLimit, (tiles_x, tiles_y)).apply(gray) (core.py:2562-2565,
  * applied to every image ahead of detectAndCompute) with OpenCV's 8-bit
  * semantics: BORDER_REFLECT_101 tiling of ragged images, clip limit

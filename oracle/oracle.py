@@ -75,6 +75,11 @@ def _declare(L):
     L.or_exp_f_arr.restype = None; L.or_exp_f_arr.argtypes = [_P, _P, ctypes.c_int]
     L.or_f16_round_arr.restype = None; L.or_f16_round_arr.argtypes = [_P, _P, ctypes.c_int]
     L.or_clahe.restype = _I; L.or_clahe.argtypes = [_P, _I, _I, _I, _D, _I, _I, _P]
+    L.or_l2n_rows.restype = None; L.or_l2n_rows.argtypes = [_P, _I, _P]
+    L.or_pca_fit.restype = _I; L.or_pca_fit.argtypes = [_P, _I, _P, _I, _I, _P, _P, _P, _P]
+    L.or_pca_project.restype = None; L.or_pca_project.argtypes = [_P, _I, _P, _P, _I, _P]
+    L.or_sim16.restype = None; L.or_sim16.argtypes = [_P, _I, _P, _I, _I, _P]
+    L.or_match_sim16.restype = None; L.or_match_sim16.argtypes = [_P, _I, _P, _I, _I, _D, _P, _P, _P, _P]
     L.or_descriptor_dsp.restype = None
     L.or_descriptor_dsp.argtypes = [_P, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, _I, _I, _I, _P, _P]
     L.or_atan2_f_arr.restype = None; L.or_atan2_f_arr.argtypes = [_P, _P, _P, ctypes.c_int]
@@ -355,3 +360,72 @@ def clahe(img, clip=2.0, tiles=(8, 8)):
     if lib().or_clahe(_ptr(img), H, W, W, float(clip), int(tiles[0]), int(tiles[1]), _ptr(out)) != 0:
         raise ValueError("bad CLAHE arguments")
     return out
+
+
+# ---- the reference's float matcher (core.py:2574-2743, 2840-2940) ------------
+def l2n_rows(x):
+    """x / (||x|| + 1e-7) per row (core.py:2879-2880), float32, (n, 128)."""
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(x)
+    lib().or_l2n_rows(_ptr(x), x.shape[0], _ptr(y))
+    return y
+
+
+def pca_fit(xa, xb, dims=64):
+    """PCA of the stacked L2-normalised rows (core.py:2728-2736): returns
+    (mean (128,) f32, proj (128, dims) f32, eigenvalues descending (128,) f64,
+    Jacobi sweeps)."""
+    xa = np.ascontiguousarray(xa, np.float32); xb = np.ascontiguousarray(xb, np.float32)
+    mean = np.empty(128, np.float32); proj = np.empty((128, dims), np.float32); ev = np.empty(128, np.float64)
+    sw = ctypes.c_int()
+    if lib().or_pca_fit(_ptr(xa), xa.shape[0], _ptr(xb), xb.shape[0], dims, _ptr(mean), _ptr(proj), _ptr(ev),
+                        ctypes.byref(sw)) != 0:
+        raise ValueError("bad PCA arguments")
+    return mean, proj, ev, sw.value
+
+
+def pca_project(x, mean, proj):
+    """(x - mean) @ proj, rows re-normalised with +1e-7 (core.py:2737-2742)."""
+    x = np.ascontiguousarray(x, np.float32)
+    mean = np.ascontiguousarray(mean, np.float32); proj = np.ascontiguousarray(proj, np.float32)
+    dims = proj.shape[1]
+    y = np.empty((x.shape[0], dims), np.float32)
+    lib().or_pca_project(_ptr(x), x.shape[0], _ptr(mean), _ptr(proj), dims, _ptr(y))
+    return y
+
+
+def sim16(a, b):
+    """binary16 similarity matrix (the fp16-autocast mm output, core.py:2668-2670)."""
+    a = np.ascontiguousarray(a, np.float32); b = np.ascontiguousarray(b, np.float32)
+    out = np.empty((a.shape[0], b.shape[0]), np.float32)
+    lib().or_sim16(_ptr(a), a.shape[0], _ptr(b), b.shape[0], a.shape[1], _ptr(out))
+    return out
+
+
+def match_sim16(a, b, ratio=0.75):
+    """_match_symmetric on unit rows a (N, D), b (M, D): returns fwd (N, 3),
+    bwd (M, 3) (index, d1, d2 with the squared distances as float32 bits) and
+    the mutual pairs."""
+    a = np.ascontiguousarray(a, np.float32); b = np.ascontiguousarray(b, np.float32)
+    N, M = a.shape[0], b.shape[0]
+    fwd = np.empty((max(N, 1), 3), np.int32); bwd = np.empty((max(M, 1), 3), np.int32)
+    pairs = np.empty((max(min(N, M), 1), 2), np.int32)
+    npairs = ctypes.c_int()
+    lib().or_match_sim16(_ptr(a), N, _ptr(b), M, a.shape[1], float(ratio), _ptr(fwd), _ptr(bwd), _ptr(pairs),
+                         ctypes.byref(npairs))
+    return fwd[:N], bwd[:M], pairs[:npairs.value].copy()
+
+
+def match_float(da, db, ratio=0.75, dims=64, pca=None):
+    """The reference's float path end to end (core.py:2875-2939 with
+    int8_matching off): normalise, PCA to dims (fitted on this pair unless
+    `pca` = (mean, proj) is given; dims >= 128 or 0 skips it), fp16 similarity,
+    symmetric ratio test, mutual pairs.  Returns (fwd, bwd, pairs, pca)."""
+    a, b = l2n_rows(da), l2n_rows(db)
+    if 0 < dims < 128:
+        if pca is None:
+            m, p, _, _ = pca_fit(a, b, dims)
+            pca = (m, p)
+        a, b = pca_project(a, *pca), pca_project(b, *pca)
+    fwd, bwd, pairs = match_sim16(a, b, ratio)
+    return fwd, bwd, pairs, pca

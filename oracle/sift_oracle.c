@@ -1163,6 +1163,240 @@ void or_match_u8(const uint8_t* a, int N, const uint8_t* b, int M, double ratio,
 }
 
 /* ========================================================================
+ * F4: the reference's float matcher (core.py:2840-2940 with int8_matching
+ * off): rows L2-normalised with +1e-7 (core.py:2879-2880), PCA to `dims`
+ * fitted on both sets (core.py:2703-2743), fp16-autocast similarity GEMM,
+ * top-2 by similarity in both directions, ratio test on d = max(2 - 2 s, 0)
+ * (core.py:2662-2690), mutual check (core.py:2933-2939).
+ *
+ * Arithmetic (defined here; the reference's CuPy/cuBLAS orders are not
+ * reproducible):
+ *   - norms: float sum of squares in index order, x / (sqrtf(s) + 1e-7f);
+ *   - PCA: double mean and centred Gram, rows summed in index order within
+ *     chunks of OR_PCA_CHUNK rows and the chunk partials summed in chunk
+ *     order; covariance = Gram / n; eigenvectors by the cyclic parallel
+ *     Jacobi method: each sweep runs 127 rounds of the circle-method pairing
+ *     of the 128 indices (round r: (r, 127) and ((r + k) % 127, (r - k) %
+ *     127) for k = 1..63), all 64 rotations of a round computed from the
+ *     same matrix, then applied to rows, then to columns (then a_pq = a_qp =
+ *     0), and to the eigenvector matrix; a pair rotates when |a_pq| >
+ *     1e-15 sqrt(|a_pp a_qq|), sweeps stop after one without a rotation (at
+ *     most OR_PCA_MAX_SWEEPS);
+ *   - the `dims` largest eigenvalues (ties: lower index) give the columns of
+ *     proj in ascending eigenvalue order (as eigh(cov)[:, -dims:]), each
+ *     signed so that its largest-magnitude entry (first on ties) is positive;
+ *   - projection: float c_j = x_j - mean_j, p_k = sum_j c_j proj_jk in j
+ *     order (separate multiply and add), then p / (sqrtf(sum p_k^2) + 1e-7f);
+ *   - similarity: inputs rounded to binary16, the exact sum of their
+ *     products rounded to float, then to binary16 (the fp16 GEMM output).
+ * ======================================================================== */
+void or_l2n_rows(const float* x, int n, float* y)
+{
+    for (int i = 0; i < n; ++i) {
+        const float* r = x + (size_t)i * 128;
+        float s = 0.0f;
+        for (int k = 0; k < 128; ++k) s = s + r[k] * r[k];
+        const float den = sqrtf(s) + 1e-7f;
+        for (int k = 0; k < 128; ++k) y[(size_t)i * 128 + k] = r[k] / den;
+    }
+}
+
+static void jacobi_cs(double app, double aqq, double apq, double* c, double* s)
+{
+    const double theta = (aqq - app) / (2.0 * apq);
+    double t = 1.0 / (fabs(theta) + sqrt(theta * theta + 1.0));
+    if (theta < 0.0) t = -t;
+    *c = 1.0 / sqrt(t * t + 1.0);
+    *s = t * *c;
+}
+
+int or_pca_fit(const float* xa, int N, const float* xb, int M, int dims, float* mean32, float* proj,
+               double* evals, int* sweeps_out)
+{
+    const int n = N + M;
+    if (n < 1 || dims < 1 || dims > 128) return -1;
+    const int nch = (n + OR_PCA_CHUNK - 1) / OR_PCA_CHUNK;
+    double* part = (double*)calloc((size_t)nch * 128 * 128, sizeof(double));
+    double mean[128];
+#define ROW(r) ((r) < N ? xa + (size_t)(r) * 128 : xb + (size_t)((r) - N) * 128)
+    for (int c = 0; c < nch; ++c)
+        for (int r = c * OR_PCA_CHUNK; r < n && r < (c + 1) * OR_PCA_CHUNK; ++r)
+            for (int j = 0; j < 128; ++j) part[(size_t)c * 128 + j] += (double)ROW(r)[j];
+    for (int j = 0; j < 128; ++j) {
+        double t = 0.0;
+        for (int c = 0; c < nch; ++c) t += part[(size_t)c * 128 + j];
+        mean[j] = t / (double)n;
+    }
+    memset(part, 0, (size_t)nch * 128 * 128 * sizeof(double));
+    for (int c = 0; c < nch; ++c) {
+        double* g = part + (size_t)c * 128 * 128;
+        for (int r = c * OR_PCA_CHUNK; r < n && r < (c + 1) * OR_PCA_CHUNK; ++r) {
+            double cr[128];
+            for (int j = 0; j < 128; ++j) cr[j] = (double)ROW(r)[j] - mean[j];
+            for (int j = 0; j < 128; ++j)
+                for (int k = 0; k < 128; ++k) g[j * 128 + k] += cr[j] * cr[k];
+        }
+    }
+#undef ROW
+    double* A = (double*)malloc(128 * 128 * sizeof(double));
+    double* V = (double*)malloc(128 * 128 * sizeof(double));
+    for (int e = 0; e < 128 * 128; ++e) {
+        double t = 0.0;
+        for (int c = 0; c < nch; ++c) t += part[(size_t)c * 128 * 128 + e];
+        A[e] = t / (double)n;
+        V[e] = (e / 128 == e % 128) ? 1.0 : 0.0;
+    }
+    free(part);
+    int sweeps = 0;
+    for (; sweeps < OR_PCA_MAX_SWEEPS; ++sweeps) {
+        int rotated = 0;
+        for (int r = 0; r < 127; ++r) {
+            int P[64], Q[64], rot[64];
+            double C[64], S[64];
+            for (int k = 0; k < 64; ++k) {
+                int p, q;
+                if (k == 0) { p = r; q = 127; }
+                else { p = (r + k) % 127; q = (r - k + 127) % 127; }
+                if (p > q) { int t = p; p = q; q = t; }
+                P[k] = p; Q[k] = q;
+                const double app = A[p * 128 + p], aqq = A[q * 128 + q], apq = A[p * 128 + q];
+                rot[k] = fabs(apq) > 1e-15 * sqrt(fabs(app * aqq));
+                if (rot[k]) jacobi_cs(app, aqq, apq, &C[k], &S[k]);
+                rotated |= rot[k];
+            }
+            for (int k = 0; k < 64; ++k) {
+                if (!rot[k]) continue;
+                const int p = P[k], q = Q[k];
+                for (int j = 0; j < 128; ++j) {
+                    const double apj = A[p * 128 + j], aqj = A[q * 128 + j];
+                    A[p * 128 + j] = C[k] * apj - S[k] * aqj;
+                    A[q * 128 + j] = S[k] * apj + C[k] * aqj;
+                }
+            }
+            for (int k = 0; k < 64; ++k) {
+                if (!rot[k]) continue;
+                const int p = P[k], q = Q[k];
+                for (int i = 0; i < 128; ++i) {
+                    const double aip = A[i * 128 + p], aiq = A[i * 128 + q];
+                    A[i * 128 + p] = C[k] * aip - S[k] * aiq;
+                    A[i * 128 + q] = S[k] * aip + C[k] * aiq;
+                    const double vip = V[i * 128 + p], viq = V[i * 128 + q];
+                    V[i * 128 + p] = C[k] * vip - S[k] * viq;
+                    V[i * 128 + q] = S[k] * vip + C[k] * viq;
+                }
+            }
+            for (int k = 0; k < 64; ++k)
+                if (rot[k]) { A[P[k] * 128 + Q[k]] = 0.0; A[Q[k] * 128 + P[k]] = 0.0; }
+        }
+        if (!rotated) break;
+    }
+    /* eigenvalues in descending order (ties: lower index first) */
+    int order[128];
+    for (int j = 0; j < 128; ++j) order[j] = j;
+    for (int a = 0; a < 128; ++a) {
+        int best = a;
+        for (int b = a + 1; b < 128; ++b) {
+            const double vb = A[order[b] * 129], vbest = A[order[best] * 129];
+            if (vb > vbest || (vb == vbest && order[b] < order[best])) best = b;
+        }
+        int t = order[a]; order[a] = order[best]; order[best] = t;
+    }
+    if (evals) for (int j = 0; j < 128; ++j) evals[j] = A[order[j] * 129];
+    for (int k = 0; k < dims; ++k) {
+        const int col = order[dims - 1 - k];
+        int arg = 0;
+        for (int j = 1; j < 128; ++j) if (fabs(V[j * 128 + col]) > fabs(V[arg * 128 + col])) arg = j;
+        const double sg = V[arg * 128 + col] < 0.0 ? -1.0 : 1.0;
+        for (int j = 0; j < 128; ++j) proj[j * dims + k] = (float)(sg * V[j * 128 + col]);
+    }
+    for (int j = 0; j < 128; ++j) mean32[j] = (float)mean[j];
+    if (sweeps_out) *sweeps_out = sweeps;
+    free(A); free(V);
+    return 0;
+}
+
+void or_pca_project(const float* x, int n, const float* mean, const float* proj, int dims, float* y)
+{
+    for (int i = 0; i < n; ++i) {
+        float c[128], p[128];
+        for (int j = 0; j < 128; ++j) c[j] = x[(size_t)i * 128 + j] - mean[j];
+        for (int k = 0; k < dims; ++k) {
+            float acc = 0.0f;
+            for (int j = 0; j < 128; ++j) acc = acc + c[j] * proj[j * dims + k];
+            p[k] = acc;
+        }
+        float s = 0.0f;
+        for (int k = 0; k < dims; ++k) s = s + p[k] * p[k];
+        const float den = sqrtf(s) + 1e-7f;
+        for (int k = 0; k < dims; ++k) y[(size_t)i * dims + k] = p[k] / den;
+    }
+}
+
+void or_sim16(const float* a, int N, const float* b, int M, int D, float* s16)
+{
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+        double ha[128];
+        for (int k = 0; k < D; ++k) ha[k] = (double)or_f16_round(a[(size_t)i * D + k]);
+        for (int j = 0; j < M; ++j) {
+            double t = 0.0;   /* exact: products of binary16 values carry <= 22 bits */
+            for (int k = 0; k < D; ++k) t += ha[k] * (double)or_f16_round(b[(size_t)j * D + k]);
+            s16[(size_t)i * M + j] = or_f16_round((float)t);
+        }
+    }
+}
+
+static int32_t f_bits(float f) { int32_t u; memcpy(&u, &f, 4); return u; }
+
+/* top-2 by similarity (ties: lower index), distances max(2 - 2 s, 0) */
+static void sim_top2(const float* s16, int N, int M, int transposed, int32_t* out)
+{
+    const int rows = transposed ? M : N, cols = transposed ? N : M;
+    for (int i = 0; i < rows; ++i) {
+        float b1 = -INFINITY, b2 = -INFINITY;
+        int j1 = -1;
+        for (int j = 0; j < cols; ++j) {
+            const float s = transposed ? s16[(size_t)j * M + i] : s16[(size_t)i * M + j];
+            if (s > b1) { b2 = b1; b1 = s; j1 = j; }
+            else if (s > b2) b2 = s;
+        }
+        out[3 * i] = j1;
+        float d1 = 2.0f - 2.0f * b1, d2 = 2.0f - 2.0f * b2;
+        d1 = d1 > 0.0f ? d1 : 0.0f;
+        d2 = d2 > 0.0f ? d2 : 0.0f;
+        out[3 * i + 1] = j1 < 0 ? 0x7fffffff : f_bits(d1);
+        out[3 * i + 2] = cols < 2 ? 0x7fffffff : f_bits(d2);
+    }
+}
+
+static int sim_pass(const int32_t* t, float r2)
+{
+    if (t[0] < 0 || t[2] == 0x7fffffff) return 0;
+    float d1, d2;
+    memcpy(&d1, &t[1], 4); memcpy(&d2, &t[2], 4);
+    return d2 > 1e-10f && d1 < r2 * d2;
+}
+
+void or_match_sim16(const float* a, int N, const float* b, int M, int D, double ratio,
+                    int32_t* fwd, int32_t* bwd, int32_t* pairs, int* npairs)
+{
+    float* s16 = (float*)malloc((size_t)(N > 0 ? N : 1) * (M > 0 ? M : 1) * sizeof(float));
+    or_sim16(a, N, b, M, D, s16);
+    sim_top2(s16, N, M, 0, fwd);
+    sim_top2(s16, N, M, 1, bwd);
+    free(s16);
+    const float r2 = (float)(ratio * ratio);
+    int np = 0;
+    for (int i = 0; i < N; ++i) {
+        if (!sim_pass(fwd + 3 * i, r2)) continue;
+        const int j = fwd[3 * i];
+        if (bwd[3 * j] != i || !sim_pass(bwd + 3 * j, r2)) continue;
+        pairs[2 * np] = i; pairs[2 * np + 1] = j; np++;
+    }
+    *npairs = np;
+}
+
+/* ========================================================================
  * Philox4x32-10 (Salmon et al., SC'11)
  * ======================================================================== */
 void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])

@@ -73,6 +73,17 @@ void or_match_u8(const uint8_t* a, int N, const uint8_t* b, int M, double ratio,
                  int32_t* fwd /*N*3: j,d1,d2*/, int32_t* bwd /*M*3: i,d1,d2*/,
                  int32_t* pairs /*min(N,M)*2*/, int* npairs);
 
+/* ---- the reference's float matcher: L2-normalise, PCA, fp16 similarity ---- */
+#define OR_PCA_CHUNK 128
+#define OR_PCA_MAX_SWEEPS 40
+void or_l2n_rows(const float* x, int n, float* y /*n*128*/);
+int  or_pca_fit(const float* xa, int N, const float* xb, int M, int dims, float* mean /*128*/,
+                float* proj /*128*dims*/, double* evals /*128, descending*/, int* sweeps);
+void or_pca_project(const float* x, int n, const float* mean, const float* proj, int dims, float* y /*n*dims*/);
+void or_sim16(const float* a, int N, const float* b, int M, int D, float* s16 /*N*M*/);
+void or_match_sim16(const float* a, int N, const float* b, int M, int D, double ratio,
+                    int32_t* fwd /*N*3: j, d1 bits, d2 bits*/, int32_t* bwd, int32_t* pairs, int* npairs);
+
 /* ---- Philox4x32-10 ---- */
 void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void or_ransac_sample(int B, int N, uint64_t seed, int32_t* idx /*B*4*/);

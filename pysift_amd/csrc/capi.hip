@@ -212,6 +212,32 @@ int msift_bgr_to_gray(const uint8_t* d_bgr, int H, int W, uint8_t* d_gray, void*
     return bgr_to_gray(d_bgr, H, W, d_gray, (hipStream_t)stream);
 }
 
+int msift_l2_normalize(const float* d_x, int n, float* d_y, void* stream) {
+    return l2_normalize_rows(d_x, n, d_y, (hipStream_t)stream);
+}
+
+int msift_pca_workspace_bytes(int N, int M, size_t* bytes) {
+    if (N < 0 || M < 0 || !bytes) { set_error("msift_pca_workspace_bytes: bad arguments"); return MSIFT_EINVAL; }
+    *bytes = pca_workspace(N, M);
+    return MSIFT_OK;
+}
+
+int msift_pca_fit(const float* d_a, int N, const float* d_b, int M, int dims, void* d_ws, size_t ws_bytes,
+                  float* d_mean, float* d_proj, double* d_evals, int* d_sweeps, void* stream) {
+    return pca_fit(d_a, N, d_b, M, dims, d_ws, ws_bytes, d_mean, d_proj, d_evals, d_sweeps, (hipStream_t)stream);
+}
+
+int msift_pca_project(const float* d_x, int n, const float* d_mean, const float* d_proj, int dims, float* d_y,
+                      void* stream) {
+    return pca_project(d_x, n, d_mean, d_proj, dims, d_y, (hipStream_t)stream);
+}
+
+int msift_match_sim_f16(const float* d_a, int N, const float* d_b, int M, int D, double ratio, void* d_ws,
+                        size_t ws_bytes, int32_t* d_fwd, int32_t* d_bwd, int32_t* d_pairs, int* d_npairs,
+                        void* stream) {
+    return match_sim16(d_a, N, d_b, M, D, ratio, d_ws, ws_bytes, d_fwd, d_bwd, d_pairs, d_npairs, (hipStream_t)stream);
+}
+
 int msift_clahe_workspace_bytes(int tiles_x, int tiles_y, size_t* bytes) {
     if (tiles_x < 1 || tiles_y < 1 || tiles_x > 128 || tiles_y > 128 || !bytes) {
         set_error("msift_clahe_workspace_bytes: bad arguments (1 <= tiles <= 128)");

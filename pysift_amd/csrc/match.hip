@@ -250,28 +250,50 @@ __device__ __forceinline__ uint32_t f16_q(float d) {
     return q > 0x7fffffu ? 0x7fffffu : q;
 }
 
+// Similarity keys of the reference's fp16 path (core.py:2668-2677): s is
+// rounded to binary16 (the GEMM's output type) and ordered largest first,
+// ties by index; -0 counts as +0.  key = 0xffff - (order-preserving map of
+// the binary16 bits), so a smaller key is a larger similarity.
+__device__ __forceinline__ uint32_t sim16_key(float s) {
+    uint32_t h = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)s);
+    if (h == 0x8000u) h = 0;
+    const uint32_t u = (h & 0x8000u) ? (~h & 0xffffu) : (h | 0x8000u);
+    return 0xffffu - u;
+}
+
+__device__ __forceinline__ float sim16_of_key(uint32_t key) {
+    const uint32_t u = 0xffffu - key;
+    const uint32_t h = (u & 0x8000u) ? (u & 0x7fffu) : (~u & 0xffffu);
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)h);
+}
+
+// KS = 16-element MFMA steps per row (rows are 16 KS halves); SIM selects the
+// epilogue: false = quantised squared distance from the norms (128-D rows),
+// true = the binary16 similarity key above.
+template <int KS, bool SIM>
 __global__ __launch_bounds__(512) void k_match_tiles_f16(const _Float16* __restrict__ A, int N,
                                                           const _Float16* __restrict__ B, int M,
                                                           const float* __restrict__ na, const float* __restrict__ nb,
                                                           uint32_t* __restrict__ rowpart, uint32_t* __restrict__ colpart) {
+    constexpr int D = 16 * KS;
     __shared__ uint32_t colk[8][64][2];
     __shared__ float s_na[kRowBlk];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int l32 = lane & 31, hi = lane >> 5;
     const int rb = blockIdx.x, cc = blockIdx.y;
     const int row0 = rb * kRowBlk + w * 32;
-    if (threadIdx.x < kRowBlk) {
+    if (!SIM && threadIdx.x < kRowBlk) {
         int i = rb * kRowBlk + threadIdx.x;
         s_na[threadIdx.x] = i < N ? na[i] : 0.0f;
     }
-    // A fragment: lane holds row (row0 + l32), halves [16 ks + 8 hi, +8) for ks = 0..7
-    v8h af[8];
+    // A fragment: lane holds row (row0 + l32), halves [16 ks + 8 hi, +8) for ks < KS
+    v8h af[KS];
     {
         int i = row0 + l32;
         int ic = i < N ? i : N - 1;
-        const v8h* src = reinterpret_cast<const v8h*>(A + (size_t)ic * 128);
+        const v8h* src = reinterpret_cast<const v8h*>(A + (size_t)ic * D);
 #pragma unroll
-        for (int ks = 0; ks < 8; ++ks) af[ks] = src[2 * ks + hi];
+        for (int ks = 0; ks < KS; ++ks) af[ks] = src[2 * ks + hi];
     }
     uint32_t rk1[16], rk2[16];
 #pragma unroll
@@ -285,24 +307,24 @@ __global__ __launch_bounds__(512) void k_match_tiles_f16(const _Float16* __restr
         for (int b = 0; b < 2; ++b) {
             int j = col0 + 32 * b + l32;
             int jc = j < M ? j : M - 1;
-            const v8h* src = reinterpret_cast<const v8h*>(B + (size_t)jc * 128);
+            const v8h* src = reinterpret_cast<const v8h*>(B + (size_t)jc * D);
             acc[b] = (v16f)0.0f;
 #pragma unroll
-            for (int ks = 0; ks < 8; ++ks)
+            for (int ks = 0; ks < KS; ++ks)
                 acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(af[ks], src[2 * ks + hi], acc[b], 0, 0, 0);
         }
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
             const int j = col0 + 32 * b + l32;
             const bool jv = j < M;
-            const float nbj = jv ? nb[j] : 0.0f;
+            const float nbj = (!SIM && jv) ? nb[j] : 0.0f;
             const uint32_t lj = (uint32_t)(ct * 64 + 32 * b + l32);
             uint32_t c1 = 0xffffffffu, c2 = 0xffffffffu;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int il = 32 * w + 8 * (r >> 2) + 4 * hi + (r & 3);
                 const bool v = jv && (rb * kRowBlk + il < N);
-                const uint32_t q = f16_q((s_na[il] + nbj) - 2.0f * acc[b][r]);
+                const uint32_t q = SIM ? sim16_key(acc[b][r]) : f16_q((s_na[il] + nbj) - 2.0f * acc[b][r]);
                 const uint32_t kc = v ? ((q << 8) | (uint32_t)il) : 0xffffffffu;
                 top2_insert(c1, c2, kc);
                 const uint32_t kr = v ? ((q << 9) | lj) : 0xffffffffu;
@@ -354,6 +376,32 @@ __global__ __launch_bounds__(256) void k_q_to_float(int32_t* __restrict__ t, int
         const int32_t q = t[3 * i + k];
         const float d = q == 0x7fffffff ? __builtin_huge_valf() : (float)q * (1.0f / kF16Q);
         t[3 * i + k] = __builtin_bit_cast(int32_t, d);
+    }
+}
+
+// similarity path: key slots -> float bits of d = max(2 - 2 s, 0)
+// (core.py:2674-2675); "none" stays 0x7fffffff
+__global__ __launch_bounds__(256) void k_sim_to_dist(int32_t* __restrict__ t, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int k = 1; k <= 2; ++k) {
+        const int32_t key = t[3 * i + k];
+        if (key == 0x7fffffff) continue;
+        float d = 2.0f - 2.0f * sim16_of_key((uint32_t)key);
+        d = d > 0.0f ? d : 0.0f;
+        t[3 * i + k] = __builtin_bit_cast(int32_t, d);
+    }
+}
+
+// binary16 copies of n rows of D floats, zero-padded to Dp halves
+__global__ __launch_bounds__(256) void k_to_f16_pad(const float* __restrict__ in, int n, int D, int Dp,
+                                                    _Float16* __restrict__ out) {
+    const size_t tot = (size_t)n * Dp;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < tot; e += (size_t)gridDim.x * 256) {
+        const size_t i = e / Dp;
+        const int k = (int)(e - i * Dp);
+        out[e] = k < D ? (_Float16)in[i * D + k] : (_Float16)0.0f;
     }
 }
 
@@ -443,20 +491,30 @@ __global__ void k_fill_top2_none(int32_t* __restrict__ out, int n) {
 }
 
 // mutual ratio test (core.py:2935-2939), then raster-order compaction
-__device__ __forceinline__ bool mutual_ok(const int32_t* fwd, const int32_t* bwd, int N, int M, double r2, int i) {
+// fmode 0: integer distances, d1 < r2 d2 in double; fmode 1: float bits of the
+// similarity path, d2 > 1e-10 and d1 < (float)r2 d2 in float (core.py:2676)
+__device__ __forceinline__ bool ratio_ok(const int32_t* t, double r2, int fmode) {
+    if (!fmode) return (double)t[1] < r2 * (double)t[2];
+    if (t[2] == 0x7fffffff) return false;
+    const float d1 = __builtin_bit_cast(float, t[1]), d2 = __builtin_bit_cast(float, t[2]);
+    return d2 > 1e-10f && d1 < (float)r2 * d2;
+}
+
+__device__ __forceinline__ bool mutual_ok(const int32_t* fwd, const int32_t* bwd, int N, int M, double r2, int i,
+                                          int fmode = 0) {
     if (N < 2 || M < 2) return false;
     int j = fwd[3 * i];
     if (j < 0) return false;
-    if (!((double)fwd[3 * i + 1] < r2 * (double)fwd[3 * i + 2])) return false;
+    if (!ratio_ok(fwd + 3 * i, r2, fmode)) return false;
     if (bwd[3 * j] != i) return false;
-    return (double)bwd[3 * j + 1] < r2 * (double)bwd[3 * j + 2];
+    return ratio_ok(bwd + 3 * j, r2, fmode);
 }
 
 // block b (256 rows): number of mutual matches; whole block
 __device__ __forceinline__ int mutual_block_count(const int32_t* __restrict__ fwd, const int32_t* __restrict__ bwd,
-                                                  int N, int M, double r2, int b, int* wc) {
+                                                  int N, int M, double r2, int b, int* wc, int fmode = 0) {
     const int i = b * 256 + (int)threadIdx.x;
-    const bool ok = i < N && mutual_ok(fwd, bwd, N, M, r2, i);
+    const bool ok = i < N && mutual_ok(fwd, bwd, N, M, r2, i, fmode);
     const unsigned long long m = __ballot(ok);
     if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(m);
     __syncthreads();
@@ -468,9 +526,9 @@ __device__ __forceinline__ int mutual_block_count(const int32_t* __restrict__ fw
 // block b: write its mutual pairs (i, fwd j) from position `pos0` on; whole block
 __device__ __forceinline__ void mutual_block_write(const int32_t* __restrict__ fwd, const int32_t* __restrict__ bwd,
                                                    int N, int M, double r2, int b, int pos0, int32_t* __restrict__ pairs,
-                                                   int* wc) {
+                                                   int* wc, int fmode = 0) {
     const int i = b * 256 + (int)threadIdx.x;
-    const bool ok = i < N && mutual_ok(fwd, bwd, N, M, r2, i);
+    const bool ok = i < N && mutual_ok(fwd, bwd, N, M, r2, i, fmode);
     const unsigned long long m = __ballot(ok);
     const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) wc[wave] = __popcll(m);
@@ -486,17 +544,18 @@ __device__ __forceinline__ void mutual_block_write(const int32_t* __restrict__ f
 }
 
 __global__ __launch_bounds__(256) void k_mutual_count(const int32_t* __restrict__ fwd, const int32_t* __restrict__ bwd,
-                                                       int N, int M, double r2, int* __restrict__ counts) {
+                                                       int N, int M, double r2, int* __restrict__ counts,
+                                                       int fmode) {
     __shared__ int wc[4];
-    const int c = mutual_block_count(fwd, bwd, N, M, r2, blockIdx.x, wc);
+    const int c = mutual_block_count(fwd, bwd, N, M, r2, blockIdx.x, wc, fmode);
     if (threadIdx.x == 0) counts[blockIdx.x] = c;
 }
 
 __global__ __launch_bounds__(256) void k_mutual_write(const int32_t* __restrict__ fwd, const int32_t* __restrict__ bwd,
                                                        int N, int M, double r2, const int* __restrict__ offs,
-                                                       int32_t* __restrict__ pairs) {
+                                                       int32_t* __restrict__ pairs, int fmode) {
     __shared__ int wc[4];
-    mutual_block_write(fwd, bwd, N, M, r2, blockIdx.x, offs[blockIdx.x], pairs, wc);
+    mutual_block_write(fwd, bwd, N, M, r2, blockIdx.x, offs[blockIdx.x], pairs, wc, fmode);
 }
 
 __global__ __launch_bounds__(256) void k_mutual_count_b(const MPair* __restrict__ pairs, const int32_t* __restrict__ fwd,
@@ -606,9 +665,9 @@ int match_u8(const uint8_t* a, int N, const uint8_t* b, int M, double ratio, int
     double r2 = ratio * ratio;
     int* cnt = (int*)(ws + P.o_cnt);
     int* off = (int*)(ws + P.o_off);
-    if (nblk > 0) hipLaunchKernelGGL(k_mutual_count, dim3(nblk), dim3(256), 0, s, fwd, bwd, N, M, r2, cnt);
+    if (nblk > 0) hipLaunchKernelGGL(k_mutual_count, dim3(nblk), dim3(256), 0, s, fwd, bwd, N, M, r2, cnt, 0);
     launch_scan(cnt, off, nblk, npairs, nullptr, nullptr, 0, s);
-    if (nblk > 0) hipLaunchKernelGGL(k_mutual_write, dim3(nblk), dim3(256), 0, s, fwd, bwd, N, M, r2, off, pairs);
+    if (nblk > 0) hipLaunchKernelGGL(k_mutual_write, dim3(nblk), dim3(256), 0, s, fwd, bwd, N, M, r2, off, pairs, 0);
     MSIFT_LAUNCH_CHECK();
     return MSIFT_OK;
 }
@@ -759,7 +818,7 @@ int match_f16(const float* a, int N, const float* b, int M, double ratio, void* 
         hipLaunchKernelGGL(k_to_f16, dim3((unsigned)std::min<size_t>((eb + 255) / 256, 16384)), dim3(256), 0, s, b, eb, hb);
         hipLaunchKernelGGL(k_norms_f16, dim3((N + 3) / 4), dim3(256), 0, s, (const _Float16*)ha, N, na);
         hipLaunchKernelGGL(k_norms_f16, dim3((M + 3) / 4), dim3(256), 0, s, (const _Float16*)hb, M, nb);
-        hipLaunchKernelGGL(k_match_tiles_f16, dim3(P.nrb_a, P.ncc_a), dim3(512), 0, s, (const _Float16*)ha, N,
+        hipLaunchKernelGGL((k_match_tiles_f16<8, false>), dim3(P.nrb_a, P.ncc_a), dim3(512), 0, s, (const _Float16*)ha, N,
                            (const _Float16*)hb, M, (const float*)na, (const float*)nb, rowpart, colpart);
         hipLaunchKernelGGL(k_reduce_top2, dim3((N + 255) / 256), dim3(256), 0, s, rowpart, N, P.ncc_a, 9, kColChunk, fwd);
         hipLaunchKernelGGL(k_reduce_top2, dim3((M + 255) / 256), dim3(256), 0, s, colpart, M, P.nrb_a, 8, kRowBlk, bwd);
@@ -772,9 +831,9 @@ int match_f16(const float* a, int N, const float* b, int M, double ratio, void* 
     const double r2 = ratio * ratio;
     int* cnt = (int*)(ws + P.o_cnt);
     int* off = (int*)(ws + P.o_off);
-    if (nblk > 0) hipLaunchKernelGGL(k_mutual_count, dim3(nblk), dim3(256), 0, s, fwd, bwd, N, M, r2, cnt);
+    if (nblk > 0) hipLaunchKernelGGL(k_mutual_count, dim3(nblk), dim3(256), 0, s, fwd, bwd, N, M, r2, cnt, 0);
     launch_scan(cnt, off, nblk, npairs, nullptr, nullptr, 0, s);
-    if (nblk > 0) hipLaunchKernelGGL(k_mutual_write, dim3(nblk), dim3(256), 0, s, fwd, bwd, N, M, r2, off, pairs);
+    if (nblk > 0) hipLaunchKernelGGL(k_mutual_write, dim3(nblk), dim3(256), 0, s, fwd, bwd, N, M, r2, off, pairs, 0);
     if (N > 0) hipLaunchKernelGGL(k_q_to_float, dim3((N + 255) / 256), dim3(256), 0, s, fwd, N);
     if (M > 0) hipLaunchKernelGGL(k_q_to_float, dim3((M + 255) / 256), dim3(256), 0, s, bwd, M);
     MSIFT_LAUNCH_CHECK();
@@ -825,3 +884,63 @@ int msift_match_batch_gather_points(msift_match_batch* plan, const float* d_kpa,
 }
 
 }  // extern "C"
+
+namespace msift {
+
+// the reference's similarity matcher (core.py:2652-2701) on unit rows of D
+// floats: binary16 inputs, a.b on FP16 MFMA with fp32 accumulation, the
+// result rounded to binary16, top-2 by similarity in both directions, ratio
+// test on d = max(2 - 2 s, 0), mutual pairs
+int match_sim16(const float* a, int N, const float* b, int M, int D, double ratio, void* ws_, size_t ws_bytes,
+                int32_t* fwd, int32_t* bwd, int32_t* pairs, int* npairs, hipStream_t s) {
+    if (N < 0 || M < 0 || D < 1 || D > 128) { set_error("msift_match_sim_f16: bad arguments (1 <= D <= 128)"); return MSIFT_EINVAL; }
+    MatchPlan P = match_plan(N, M);
+    if (ws_bytes < P.total) { set_error("msift_match_sim_f16: workspace too small"); return MSIFT_EWORKSPACE; }
+    if ((N > 0 && (!a || !fwd)) || (M > 0 && (!b || !bwd)) || !npairs || (!pairs && N > 0 && M > 0)) {
+        set_error("msift_match_sim_f16: null buffer");
+        return MSIFT_EINVAL;
+    }
+    char* ws = (char*)ws_;
+    _Float16* ha = (_Float16*)(ws + P.o_ha);
+    _Float16* hb = (_Float16*)(ws + P.o_hb);
+    uint32_t* rowpart = (uint32_t*)(ws + P.o_rowpart);
+    uint32_t* colpart = (uint32_t*)(ws + P.o_colpart);
+    const int KS = (D + 15) / 16, Dp = 16 * KS;
+    if (N > 0 && M > 0) {
+        const size_t ea = (size_t)N * Dp, eb = (size_t)M * Dp;
+        hipLaunchKernelGGL(k_to_f16_pad, dim3((unsigned)std::min<size_t>((ea + 255) / 256, 16384)), dim3(256), 0, s, a,
+                           N, D, Dp, ha);
+        hipLaunchKernelGGL(k_to_f16_pad, dim3((unsigned)std::min<size_t>((eb + 255) / 256, 16384)), dim3(256), 0, s, b,
+                           M, D, Dp, hb);
+        const dim3 g(P.nrb_a, P.ncc_a);
+        const _Float16 *cha = ha, *chb = hb;
+        switch (KS) {
+        case 1: hipLaunchKernelGGL((k_match_tiles_f16<1, true>), g, dim3(512), 0, s, cha, N, chb, M, nullptr, nullptr, rowpart, colpart); break;
+        case 2: hipLaunchKernelGGL((k_match_tiles_f16<2, true>), g, dim3(512), 0, s, cha, N, chb, M, nullptr, nullptr, rowpart, colpart); break;
+        case 3: hipLaunchKernelGGL((k_match_tiles_f16<3, true>), g, dim3(512), 0, s, cha, N, chb, M, nullptr, nullptr, rowpart, colpart); break;
+        case 4: hipLaunchKernelGGL((k_match_tiles_f16<4, true>), g, dim3(512), 0, s, cha, N, chb, M, nullptr, nullptr, rowpart, colpart); break;
+        case 5: hipLaunchKernelGGL((k_match_tiles_f16<5, true>), g, dim3(512), 0, s, cha, N, chb, M, nullptr, nullptr, rowpart, colpart); break;
+        case 6: hipLaunchKernelGGL((k_match_tiles_f16<6, true>), g, dim3(512), 0, s, cha, N, chb, M, nullptr, nullptr, rowpart, colpart); break;
+        case 7: hipLaunchKernelGGL((k_match_tiles_f16<7, true>), g, dim3(512), 0, s, cha, N, chb, M, nullptr, nullptr, rowpart, colpart); break;
+        default: hipLaunchKernelGGL((k_match_tiles_f16<8, true>), g, dim3(512), 0, s, cha, N, chb, M, nullptr, nullptr, rowpart, colpart); break;
+        }
+        hipLaunchKernelGGL(k_reduce_top2, dim3((N + 255) / 256), dim3(256), 0, s, rowpart, N, P.ncc_a, 9, kColChunk, fwd);
+        hipLaunchKernelGGL(k_reduce_top2, dim3((M + 255) / 256), dim3(256), 0, s, colpart, M, P.nrb_a, 8, kRowBlk, bwd);
+        hipLaunchKernelGGL(k_sim_to_dist, dim3((N + 255) / 256), dim3(256), 0, s, fwd, N);
+        hipLaunchKernelGGL(k_sim_to_dist, dim3((M + 255) / 256), dim3(256), 0, s, bwd, M);
+    } else {
+        if (N > 0) hipLaunchKernelGGL(k_fill_top2_none, dim3((N + 255) / 256), dim3(256), 0, s, fwd, N);
+        if (M > 0) hipLaunchKernelGGL(k_fill_top2_none, dim3((M + 255) / 256), dim3(256), 0, s, bwd, M);
+    }
+    const int nblk = (N + 255) / 256;
+    const double r2 = ratio * ratio;
+    int* cnt = (int*)(ws + P.o_cnt);
+    int* off = (int*)(ws + P.o_off);
+    if (nblk > 0) hipLaunchKernelGGL(k_mutual_count, dim3(nblk), dim3(256), 0, s, fwd, bwd, N, M, r2, cnt, 1);
+    launch_scan(cnt, off, nblk, npairs, nullptr, nullptr, 0, s);
+    if (nblk > 0) hipLaunchKernelGGL(k_mutual_write, dim3(nblk), dim3(256), 0, s, fwd, bwd, N, M, r2, off, pairs, 1);
+    MSIFT_LAUNCH_CHECK();
+    return MSIFT_OK;
+}
+
+}  // namespace msift

@@ -98,6 +98,13 @@ int warp_feather(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h
                  uint8_t* valid, hipStream_t s);
 int bgr_to_gray(const uint8_t* bgr, int H, int W, uint8_t* gray, hipStream_t s);
 size_t clahe_workspace(int tiles_x, int tiles_y);
+size_t pca_workspace(int N, int M);
+int l2_normalize_rows(const float* x, int n, float* y, hipStream_t s);
+int pca_fit(const float* a, int N, const float* b, int M, int dims, void* ws, size_t ws_bytes, float* mean,
+            float* proj, double* evals, int* sweeps, hipStream_t s);
+int pca_project(const float* x, int n, const float* mean, const float* proj, int dims, float* y, hipStream_t s);
+int match_sim16(const float* a, int N, const float* b, int M, int D, double ratio, void* ws, size_t ws_bytes,
+                int32_t* fwd, int32_t* bwd, int32_t* pairs, int* npairs, hipStream_t s);
 int clahe(const uint8_t* src, int H, int W, int pitch, double clip, int tiles_x, int tiles_y, void* ws,
           size_t ws_bytes, uint8_t* dst, hipStream_t s);
 

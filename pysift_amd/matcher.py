@@ -161,3 +161,97 @@ def match_u8_batched(sets_a, sets_b, ratio=0.75, method="hilo"):
     B = plan.pack(sets_b, plan.Ms)
     plan.run(A, B, ratio, method)
     return plan.results()
+
+
+# ---- the reference's float matcher (core.py:2840-2940, int8_matching off) ----
+
+def l2_normalize(x):
+    """(n, 128) float32 CUDA tensor -> x / (||x|| + 1e-7) per row (core.py:2879-2880)."""
+    import torch
+    if x.dtype != torch.float32 or x.dim() != 2 or x.shape[1] != 128:
+        raise ValueError("descriptors must be (n,128) float32")
+    x = x.contiguous()
+    y = torch.empty_like(x)
+    _lib.check(_lib.lib().msift_l2_normalize(_lib.ptr(x), x.shape[0], _lib.ptr(y), _lib.stream_ptr(x.device)),
+               "l2_normalize")
+    return y
+
+
+def pca_fit(a, b, dims=64):
+    """_pca_compress's fit (core.py:2728-2736) on the L2-normalised rows of
+    a (N,128) and b (M,128) float32 CUDA tensors stacked: FP64 mean and
+    covariance, Jacobi eigenvectors.  Returns (mean (128,) f32, proj (128,
+    dims) f32, eigenvalues (128,) f64 descending, sweeps (1,) int32), all on
+    the device."""
+    import torch
+    for t in (a, b):
+        if t.dtype != torch.float32 or t.dim() != 2 or t.shape[1] != 128:
+            raise ValueError("descriptors must be (n,128) float32")
+    a = a.contiguous(); b = b.contiguous()
+    dev = a.device
+    N, M = a.shape[0], b.shape[0]
+    n = ctypes.c_size_t()
+    _lib.check(_lib.lib().msift_pca_workspace_bytes(N, M, ctypes.byref(n)), "pca workspace")
+    ws = torch.empty(n.value, dtype=torch.uint8, device=dev)
+    mean = torch.empty(128, dtype=torch.float32, device=dev)
+    proj = torch.empty((128, int(dims)), dtype=torch.float32, device=dev)
+    evals = torch.empty(128, dtype=torch.float64, device=dev)
+    sweeps = torch.empty(1, dtype=torch.int32, device=dev)
+    _lib.check(_lib.lib().msift_pca_fit(_lib.ptr(a), N, _lib.ptr(b), M, int(dims), _lib.ptr(ws), ws.numel(),
+                                        _lib.ptr(mean), _lib.ptr(proj), _lib.ptr(evals), _lib.ptr(sweeps),
+                                        _lib.stream_ptr(dev)), "pca_fit")
+    return mean, proj, evals, sweeps
+
+
+def pca_project(x, mean, proj):
+    """Unit rows (n,128) -> ((x - mean) @ proj) re-normalised with +1e-7 (core.py:2737-2742)."""
+    import torch
+    x = x.contiguous()
+    dims = proj.shape[1]
+    y = torch.empty((x.shape[0], dims), dtype=torch.float32, device=x.device)
+    _lib.check(_lib.lib().msift_pca_project(_lib.ptr(x), x.shape[0], _lib.ptr(mean), _lib.ptr(proj), dims,
+                                            _lib.ptr(y), _lib.stream_ptr(x.device)), "pca_project")
+    return y
+
+
+def match_sim_f16(a, b, ratio=0.75, ws=None):
+    """_match_symmetric (core.py:2652-2701) on unit rows a (N,D), b (M,D)
+    float32, D <= 128: binary16 similarity on FP16 MFMA, top-2 by similarity
+    both ways, ratio test on d = max(2 - 2 s, 0), mutual pairs.  Returns
+    (fwd (N,3), bwd (M,3), pairs (K,2)); the distance columns hold d as
+    float32 bits (view them with .view(torch.float32))."""
+    import torch
+    if a.dtype != torch.float32 or b.dtype != torch.float32 or a.dim() != 2 or b.dim() != 2 \
+            or a.shape[1] != b.shape[1] or not 1 <= a.shape[1] <= 128:
+        raise ValueError("descriptors must be (N,D) and (M,D) float32 with D <= 128")
+    a = a.contiguous(); b = b.contiguous()
+    dev = a.device
+    N, M, D = a.shape[0], b.shape[0], a.shape[1]
+    need = match_workspace_bytes(N, M)
+    if ws is None or ws.numel() < need:
+        ws = torch.empty(max(need, 1), dtype=torch.uint8, device=dev)
+    fwd = torch.empty((max(N, 1), 3), dtype=torch.int32, device=dev)
+    bwd = torch.empty((max(M, 1), 3), dtype=torch.int32, device=dev)
+    pairs = torch.empty((max(min(N, M), 1), 2), dtype=torch.int32, device=dev)
+    npairs = torch.zeros(1, dtype=torch.int32, device=dev)
+    _lib.check(_lib.lib().msift_match_sim_f16(_lib.ptr(a), N, _lib.ptr(b), M, D, float(ratio), _lib.ptr(ws),
+                                              ws.numel(), _lib.ptr(fwd), _lib.ptr(bwd), _lib.ptr(pairs),
+                                              _lib.ptr(npairs), _lib.stream_ptr(dev)), "match_sim_f16")
+    k = int(npairs.item())
+    return fwd[:N], bwd[:M], pairs[:k]
+
+
+def match_float(da, db, ratio=0.75, pca_dims=64, pca=None):
+    """The reference's float matching path end to end (core.py:2875-2939 with
+    int8_matching off): normalise, PCA to pca_dims (0 or >= 128: none; fitted
+    on this pair unless pca = (mean, proj) is given, as the reference caches
+    _pca_matrix after the first pair), binary16 similarity, symmetric ratio
+    test, mutual pairs.  Returns (fwd, bwd, pairs, pca)."""
+    a, b = l2_normalize(da), l2_normalize(db)
+    if 0 < pca_dims < 128:
+        if pca is None:
+            mean, proj, _, _ = pca_fit(da, db, pca_dims)
+            pca = (mean, proj)
+        a, b = pca_project(a, *pca), pca_project(b, *pca)
+    fwd, bwd, pairs = match_sim_f16(a, b, ratio)
+    return fwd, bwd, pairs, pca

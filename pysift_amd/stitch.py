@@ -17,7 +17,7 @@ import numpy as np
 from . import _lib
 from .clahe import CLAHE
 from .detector import DMatch, PySIFT
-from .matcher import match_u8
+from .matcher import match_float, match_u8
 from .ransac import ransac_homography
 
 
@@ -98,7 +98,10 @@ class GPUPyStitch:
         self._clahe = CLAHE(clipLimit=clahe_clip, tileGridSize=(clahe_tile, clahe_tile))
         self.output_format = output_format.lower()
         self.output_quality = output_quality
+        if match_method not in ("hilo", "shift", "float"):
+            raise ValueError(f"match_method must be 'hilo', 'shift' or 'float' (got {match_method!r})")
         self.match_method = match_method
+        self._pca = None   # (mean, proj) of the float path, fitted on the first pair (core.py:2729-2736)
         self.ransac_seed = int(ransac_seed)
         self.save_output = bool(save_output)
         self.verbose = bool(verbose)
@@ -138,26 +141,46 @@ class GPUPyStitch:
     def _extract_features(self, bgr):
         """_extract_features (core.py:2535-2572): gray -> CLAHE(clahe_clip,
         clahe_tile x clahe_tile) in place -> detectAndCompute -> keep the
-        max_keypoints strongest (stable)."""
+        max_keypoints strongest (stable).  Descriptors are uint8 for the exact
+        matcher, float32 for match_method "float"."""
         torch = _torch()
         gray = self._gray(bgr)
         self._clahe.apply(gray, dst=gray)
-        kp, _, du = self._detector.detect_raw(gray)
+        kp, df, du = self._detector.detect_raw(gray)
+        d = df if self.match_method == "float" else du
         if kp.shape[0] > self.max_keypoints:
             order = torch.sort(kp[:, 4], descending=True, stable=True).indices[:self.max_keypoints]
-            kp, du = kp[order], du[order]
-        return kp, du, tuple(bgr.shape[:2])
+            kp, d = kp[order], d[order]
+        return kp, d, tuple(bgr.shape[:2])
+
+    def _match_float(self, d1, d2):
+        """The reference's float path (core.py:2875-2939): PCA to pca_dims for
+        SIFT descriptors (fitted on the first pair, then cached), binary16
+        similarity, symmetric ratio test."""
+        dims = int(self.pca_dims) if (self.descriptor == "sift" and self.pca_dims) else 0
+        fwd, bwd, pairs, pca = match_float(d1, d2, self.match_ratio, dims, self._pca)
+        if pca is not None:
+            self._pca = pca
+        return fwd, bwd, pairs
 
     def _match_features(self, d1, d2):
-        """Exact mutual ratio test; returns (N,2) int32 device pairs (i, j)."""
+        """Mutual ratio test; returns (N,2) int32 device pairs (i, j)."""
         if d1.shape[0] == 0 or d2.shape[0] < 2:
             torch = _torch()
             return torch.zeros((0, 2), dtype=torch.int32, device=d1.device)
+        if self.match_method == "float":
+            return self._match_float(d1, d2)[2]
         _, _, pairs = match_u8(d1, d2, self.match_ratio, self.match_method)
         return pairs
 
     def match(self, d1, d2):
-        """cv2.DMatch list (distance = sqrt of the integer squared distance)."""
+        """cv2.DMatch list (distance = sqrt of the squared distance: integer
+        for the exact matcher, 2 - 2 s for the float path)."""
+        if self.match_method == "float":
+            fwd, _, pairs = self._match_float(d1, d2)
+            p = pairs.cpu().numpy()
+            f = fwd.cpu().numpy()[:, 1].view(np.float32)
+            return [DMatch(int(i), int(j), float(np.sqrt(f[i]))) for i, j in p]
         fwd, _, pairs = match_u8(d1, d2, self.match_ratio, self.match_method)
         p = pairs.cpu().numpy()
         f = fwd.cpu().numpy()

@@ -546,6 +546,135 @@ def test_warp_identity_and_translation():
     np.testing.assert_array_equal(pano[3:43, 50:57], img[0:40, 43:50])
 
 
+# ---- the reference's float matcher (core.py:2574-2743, 2840-2940) ----
+
+def _gamma_desc(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.gamma(0.6, 1.0, (n, 128)).astype(np.float32)
+
+
+def test_l2n_rows_against_float64():
+    x = _gamma_desc(300, 1) * 7.0
+    x[:20] *= 1e-7 / 7.0   # norms near 1e-6, where the +1e-7 outside the root shows
+    y = O.l2n_rows(x)
+    X = x.astype(np.float64)
+    ref = X / (np.linalg.norm(X, axis=1, keepdims=True) + 1e-7)
+    assert np.abs(y - ref).max() < 2e-7
+    assert np.abs(np.linalg.norm(y[20:].astype(np.float64), axis=1) - 1.0).max() < 1e-6
+    n = np.linalg.norm(X[:20], axis=1)
+    assert np.abs(np.linalg.norm(y[:20].astype(np.float64), axis=1) - n / (n + 1e-7)).max() < 1e-6
+
+
+@pytest.mark.parametrize("N,M,dims", [(700, 900, 64), (150, 20, 32), (1, 300, 100), (400, 0, 128)])
+def test_pca_fit_against_numpy_eigh(N, M, dims):
+    xa, xb = O.l2n_rows(_gamma_desc(N, N + 3)), O.l2n_rows(_gamma_desc(M, M + 5))
+    mean, proj, ev, sweeps = O.pca_fit(xa, xb, dims)
+    X = np.vstack([xa, xb]).astype(np.float64)
+    mu = X.mean(0)
+    C = (X - mu).T @ (X - mu) / len(X)
+    w, V = np.linalg.eigh(C)
+    assert 3 <= sweeps < 40
+    np.testing.assert_allclose(mean, mu.astype(np.float32), rtol=0, atol=1e-8)
+    scale = w.max()
+    assert np.abs(ev - w[::-1]).max() < 1e-12 * scale
+    P = proj.astype(np.float64)
+    # orthonormal columns, in ascending eigenvalue order, each an eigenvector
+    assert np.abs(P.T @ P - np.eye(dims)).max() < 1e-6
+    lam = ev[:dims][::-1]
+    assert np.abs(C @ P - P * lam[None, :]).max() < 1e-6 * scale + 1e-9
+    # the same subspace as numpy's top-dims eigenvectors (where the gap allows)
+    if dims < 128 and w[-dims] - w[-dims - 1] > 1e-3 * scale:
+        Pn = V[:, -dims:]
+        assert np.abs(P @ P.T - Pn @ Pn.T).max() < 1e-5
+    # sign rule: the largest-magnitude entry of every column is positive
+    am = np.abs(proj).argmax(0)
+    assert (proj[am, np.arange(dims)] > 0).all()
+
+
+def test_pca_fit_known_spectrum():
+    # rows = 0.25 + 2.5 e1 u + e2 v with e1 = (+ + - -)..., e2 = (+ - + -)...:
+    # zero-mean, exactly uncorrelated, every entry exact in float32, so the
+    # covariance is 6.25 u u^T + v v^T: eigenvalues 6.25, 1, then 0
+    u = np.zeros(128); u[3] = 0.6; u[70] = 0.8
+    v = np.zeros(128); v[10] = 1.0
+    e1 = np.tile([1.0, 1.0, -1.0, -1.0], 128); e2 = np.tile([1.0, -1.0, 1.0, -1.0], 128)
+    X = (0.25 + 2.5 * e1[:, None] * u[None] + e2[:, None] * v[None]).astype(np.float32)
+    mean, proj, ev, _ = O.pca_fit(X[:300], X[300:], 2)
+    assert abs(ev[0] - 6.25) < 1e-12 and abs(ev[1] - 1.0) < 1e-12 and np.abs(ev[2:]).max() < 1e-12
+    assert (mean == np.float32(0.25)).all()
+    # ascending order: column 0 is v (smaller), column 1 is u, both sign-normalised
+    assert np.abs(proj[:, 1] - u).max() < 1e-6 and np.abs(proj[:, 0] - v).max() < 1e-6
+
+
+def test_pca_project_against_float64():
+    xa, xb = O.l2n_rows(_gamma_desc(200, 1)), O.l2n_rows(_gamma_desc(250, 2))
+    mean, proj, _, _ = O.pca_fit(xa, xb, 64)
+    y = O.pca_project(xa, mean, proj)
+    Y = (xa.astype(np.float64) - mean) @ proj.astype(np.float64)
+    Y /= np.linalg.norm(Y, axis=1, keepdims=True) + 1e-7
+    assert np.abs(y - Y).max() < 2e-6
+
+
+def test_sim16_against_numpy_binary16():
+    # products of binary16 values are exact in float64 and partial sums of
+    # unit rows stay below 2, so a float64 matmul is exact; then one rounding
+    # to float32 and one to binary16
+    a = O.l2n_rows(_gamma_desc(60, 3))[:, :64].copy()
+    b = O.l2n_rows(_gamma_desc(70, 4))[:, :64].copy()
+    a /= np.linalg.norm(a, axis=1, keepdims=True); b /= np.linalg.norm(b, axis=1, keepdims=True)
+    A = a.astype(np.float16).astype(np.float64); B = b.astype(np.float16).astype(np.float64)
+    ref = (A @ B.T).astype(np.float32).astype(np.float16).astype(np.float32)
+    np.testing.assert_array_equal(O.sim16(a, b), ref)
+
+
+def _unit2(deg):
+    t = np.radians(deg)
+    return np.array([np.cos(t), np.sin(t)], np.float32)
+
+
+def test_match_sim16_ratio_closed_forms():
+    # query e0 against directions at theta: d = 2 - 2 cos(theta) (binary16 cos)
+    a = np.array([[1.0, 0.0]], np.float32)
+    # best 30 deg (d 0.268), second 60 deg (d 1): 0.268 < 0.5625 -> passes
+    fwd, bwd, pairs = O.match_sim16(a, np.stack([_unit2(60), _unit2(30)]), 0.75)
+    assert fwd[0, 0] == 1 and len(pairs) == 0   # b has one query only: the backward test needs 2 rows
+    d1 = fwd[0, 1:].view(np.float32)
+    c30 = np.float32(np.float16(np.cos(np.radians(30))))
+    assert d1[0] == np.float32(2.0) - np.float32(2.0) * c30
+    # mutual with a second query far away: the pair passes both ways
+    a2 = np.stack([a[0], _unit2(200)])
+    fwd, bwd, pairs = O.match_sim16(a2, np.stack([_unit2(60), _unit2(30)]), 0.75)
+    assert pairs.tolist() == [[0, 1]]
+    # best 50 deg (d 0.714), second 60 deg (d 1): 0.714 > 0.5625 -> rejected
+    fwd, bwd, pairs = O.match_sim16(a2, np.stack([_unit2(60), _unit2(50)]), 0.75)
+    assert fwd[0, 0] == 1 and len(pairs) == 0
+    # a row above unit length: its binary16 self-similarity 1.00098^2 rounds
+    # to 1.00195, so 2 - 2 s < 0 and d clamps to 0
+    big = np.array([[1.001, 0.0], [0.0, 1.0]], np.float32)
+    fwd, _, _ = O.match_sim16(big, big, 0.75)
+    assert fwd[0, 1].view(np.float32) == 0.0 and fwd[0, 0] == 0
+    # the same query twice in b: best == second -> rejected
+    fwd, bwd, pairs = O.match_sim16(a2, np.stack([_unit2(30), _unit2(30), _unit2(100)]), 0.75)
+    assert fwd[0, 0] == 0 and len(pairs) == 0
+
+
+def test_match_sim16_permutation_and_symmetry():
+    a = O.l2n_rows(_gamma_desc(300, 6))
+    perm = np.random.default_rng(1).permutation(300)
+    b = a[perm]
+    fwd, bwd, pairs = O.match_sim16(a, b, 0.75)
+    inv = np.argsort(perm)
+    assert len(pairs) == 300 and (pairs[:, 1] == inv[pairs[:, 0]]).all()
+    # swapping the sets swaps the pairs
+    c = O.l2n_rows(_gamma_desc(200, 7))
+    c[:100] = a[:100] + 0.02 * O.l2n_rows(_gamma_desc(100, 8))
+    c = O.l2n_rows(c)
+    _, _, p1 = O.match_sim16(a, c, 0.75)
+    _, _, p2 = O.match_sim16(c, a, 0.75)
+    assert sorted(map(tuple, p1.tolist())) == sorted((j, i) for i, j in p2.tolist())
+    assert len(p1) > 80
+
+
 # ---- CLAHE (core.py:2562-2565, cv2.createCLAHE(...).apply) ----
 
 def test_clahe_constant_image_known_answers():
