@@ -410,6 +410,42 @@ def bench_match_float(torch, dev, sizes=(8000, 40000), iters=5):
     return res
 
 
+def bench_composite(torch, dev, va, vb, Hgt, iters=5):
+    """The reference's composition (gain + DP seam + 6-level multiband blend)
+    on the stitch pair's canvas and on two 4000x3000 views overlapping by a
+    third (a 6667x3000 canvas), beside the feather blend on the same canvas."""
+    import numpy as np
+    from pysift_amd import GPUPyStitch
+    from pysift_amd import synthetic as syn
+    from pysift_amd.stitch import compute_canvas_size
+    st = GPUPyStitch(save_output=False, verbose=False, blend="multiband")
+    g = syn.synth_gray(3000, 6667, 9)
+    big = np.repeat(g[..., None], 3, 2)
+    Hb = np.array([[1.0, 0.0, -2667.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    res = {}
+    for name, a, b, H in (("stitch pair", va, vb, Hgt), ("2x 4000x3000", big[:, :4000].copy(), big[:, 2667:].copy(), Hb)):
+        b1, b2 = torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev)
+        cw, ch, ox, oy = compute_canvas_size(a.shape[0], a.shape[1], b.shape[0], b.shape[1], H)
+        st.composite_multiband(b1, b2, H, cw, ch, ox, oy)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(iters):
+            _, _, _, stats = st.composite_multiband(b1, b2, H, cw, ch, ox, oy)
+        e1.record()
+        torch.cuda.synchronize()
+        t_mb = e0.elapsed_time(e1) / iters
+        e0.record()
+        for _ in range(iters):
+            st.warp_feather(b1, b2, H, cw, ch, ox, oy)
+        e1.record()
+        torch.cuda.synchronize()
+        res[name] = {"canvas": [ch, cw], "multiband_ms": round(t_mb, 3),
+                     "feather_ms": round(e0.elapsed_time(e1) / iters, 3),
+                     "seam": "vertical" if stats["vertical"] == 1 else "horizontal", "seam_rmse": round(stats["rmse"], 3)}
+    return res
+
+
 def bench_clahe(torch, dev, gray, iters=50):
     """CLAHE(2.0, 8 x 8) as the stitcher runs it ahead of detection, on one
     1024x768 view and on a 4000x3000 image, timed with HIP events on the
@@ -483,7 +519,7 @@ def main():
                     help="(counter calibration) one 256 MiB device-to-device copy, then exit")
     ap.add_argument("--match-n", type=int, default=40000, help="descriptors per side for the match TOPS leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the match / stitch legs")
-    ap.add_argument("--only", choices=["match", "batch", "stitch", "grad", "pairs", "config4", "clahe", "match_float"], default=None,
+    ap.add_argument("--only", choices=["match", "batch", "stitch", "grad", "pairs", "config4", "clahe", "match_float", "composite"], default=None,
                     help="(profiling) run a single extra leg and print its numbers")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -520,6 +556,8 @@ def main():
             print(json.dumps(bench_pairs(torch, dev)))
         elif args.only == "config4":
             print(json.dumps(bench_config4(torch, dev)))
+        elif args.only == "composite":
+            print(json.dumps(bench_composite(torch, dev, va, vb, Hgt)))
         elif args.only == "match_float":
             print(json.dumps(bench_match_float(torch, dev)))
         elif args.only == "clahe":
@@ -597,6 +635,7 @@ def main():
         extra["grad16x512"] = bench_grad(torch, dev)
         extra["clahe"] = bench_clahe(torch, dev, gray[0])
         extra["match_float_pca64"] = bench_match_float(torch, dev)
+        extra["composite_multiband"] = bench_composite(torch, dev, va, vb, Hgt)
         st_ms, met = bench_stitch(va, vb)
         extra["stitch_pair_ms"] = round(st_ms, 3)
         extra["stitch_metrics"] = {k: met[k] for k in ("kp1", "kp2", "matches", "inliers")}

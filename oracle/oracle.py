@@ -76,6 +76,16 @@ def _declare(L):
     L.or_f16_round_arr.restype = None; L.or_f16_round_arr.argtypes = [_P, _P, ctypes.c_int]
     L.or_clahe.restype = _I; L.or_clahe.argtypes = [_P, _I, _I, _I, _D, _I, _I, _P]
     L.or_l2n_rows.restype = None; L.or_l2n_rows.argtypes = [_P, _I, _P]
+    L.or_grid_warp.restype = None; L.or_grid_warp.argtypes = [_P, _I, _I, _P, _I, _I, _D, _D, _P, _P]
+    L.or_gauss_weights_sp.restype = None; L.or_gauss_weights_sp.argtypes = [_P]
+    L.or_gauss_weights_cv.restype = None; L.or_gauss_weights_cv.argtypes = [_P]
+    L.or_gauss_down.restype = None; L.or_gauss_down.argtypes = [_P, _I, _I, _P, _P]
+    L.or_zoom_up.restype = None; L.or_zoom_up.argtypes = [_P, _I, _I, _I, _I, _P]
+    L.or_sobel.restype = None; L.or_sobel.argtypes = [_P, _I, _I, _I, _P]
+    L.or_cv_blur25.restype = None; L.or_cv_blur25.argtypes = [_P, _I, _I, _P, _P]
+    L.or_dp_seam.restype = None; L.or_dp_seam.argtypes = [_P, _P, _I, _I, _I, _P]
+    L.or_composite.restype = _I
+    L.or_composite.argtypes = [_P, _I, _I, _P, _I, _I, _P, _I, _I, _D, _D, _I, _P, _P, _P, _P, _P, _P]
     L.or_pca_fit.restype = _I; L.or_pca_fit.argtypes = [_P, _I, _P, _I, _I, _P, _P, _P, _P]
     L.or_pca_project.restype = None; L.or_pca_project.argtypes = [_P, _I, _P, _P, _I, _P]
     L.or_sim16.restype = None; L.or_sim16.argtypes = [_P, _I, _P, _I, _I, _P]
@@ -429,3 +439,73 @@ def match_float(da, db, ratio=0.75, dims=64, pca=None):
         a, b = pca_project(a, *pca), pca_project(b, *pca)
     fwd, bwd, pairs = match_sim16(a, b, ratio)
     return fwd, bwd, pairs, pca
+
+
+# ---- compositing: gain + DP seam + multiband (core.py:3211-3239, 3287-3553, 3686-3714) ----
+def grid_warp(img, H, cw, ch, ox, oy):
+    """_warp_image (core.py:3211-3239): (warped uint8 (ch, cw, 3), mask bool)."""
+    img = np.ascontiguousarray(img, np.uint8); H = np.ascontiguousarray(H, np.float64)
+    out = np.empty((ch, cw, 3), np.uint8); m = np.empty((ch, cw), np.uint8)
+    lib().or_grid_warp(_ptr(img), img.shape[0], img.shape[1], _ptr(H), cw, ch, float(ox), float(oy), _ptr(out), _ptr(m))
+    return out, m.astype(bool)
+
+
+def gauss_weights_sp():
+    w = np.empty(9, np.float32); lib().or_gauss_weights_sp(_ptr(w)); return w
+
+
+def gauss_weights_cv():
+    w = np.empty(25, np.float32); lib().or_gauss_weights_cv(_ptr(w)); return w
+
+
+def gauss_down(x):
+    x = np.ascontiguousarray(x, np.float32)
+    H, W = x.shape
+    out = np.empty(((H + 1) // 2, (W + 1) // 2), np.float32)
+    lib().or_gauss_down(_ptr(x), H, W, _ptr(gauss_weights_sp()), _ptr(out))
+    return out
+
+
+def zoom_up(x, th, tw):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty((th, tw), np.float32)
+    lib().or_zoom_up(_ptr(x), x.shape[0], x.shape[1], th, tw, _ptr(out))
+    return out
+
+
+def sobel(x, axis):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(x)
+    lib().or_sobel(_ptr(x), x.shape[0], x.shape[1], int(axis), _ptr(out))
+    return out
+
+
+def cv_blur25(x):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(x)
+    lib().or_cv_blur25(_ptr(x), x.shape[0], x.shape[1], _ptr(gauss_weights_cv()), _ptr(out))
+    return out
+
+
+def dp_seam(cost, ov, vertical=True):
+    cost = np.ascontiguousarray(cost, np.float32); ov = np.ascontiguousarray(ov, np.uint8)
+    H, W = cost.shape
+    seam = np.empty(H if vertical else W, np.int32)
+    lib().or_dp_seam(_ptr(cost), _ptr(ov), H, W, int(bool(vertical)), _ptr(seam))
+    return seam
+
+
+def composite(img1, img2, H, cw, ch, ox, oy, n_levels=6):
+    """_blend_pair with color_match 'gain' plus the warp and paste of
+    _stitch_pair (core.py:3686-3714, 3783-3802): returns (pano uint8 (ch, cw,
+    3), valid bool, seam mask float32, gain (3,), seam RMSE, vertical)."""
+    img1 = np.ascontiguousarray(img1, np.uint8); img2 = np.ascontiguousarray(img2, np.uint8)
+    H = np.ascontiguousarray(H, np.float64)
+    pano = np.empty((ch, cw, 3), np.uint8); valid = np.empty((ch, cw), np.uint8)
+    seam = np.empty((ch, cw), np.float32); gain = np.empty(3, np.float32)
+    rmse = ctypes.c_double(); vert = ctypes.c_int()
+    if lib().or_composite(_ptr(img1), img1.shape[0], img1.shape[1], _ptr(img2), img2.shape[0], img2.shape[1],
+                          _ptr(H), cw, ch, float(ox), float(oy), int(n_levels), _ptr(pano), _ptr(valid), _ptr(seam),
+                          _ptr(gain), ctypes.byref(rmse), ctypes.byref(vert)) != 0:
+        raise ValueError("bad composite arguments")
+    return pano, valid.astype(bool), seam, gain, rmse.value, vert.value

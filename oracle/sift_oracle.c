@@ -1397,6 +1397,451 @@ void or_match_sim16(const float* a, int N, const float* b, int M, int D, double 
 }
 
 /* ========================================================================
+ * F3: the reference's composition of a pair (core.py:3686-3714, 3783-3802):
+ * grid_sample warp, gain compensation, DP seam, multiband blend.
+ *
+ * Arithmetic (defined here; the reference runs it in torch, NumPy, CuPy and
+ * OpenCV):
+ *   warp (core.py:3211-3239): X = u - (float)ox, source = hf (X, Y, 1) as
+ *     ((h0 X + h1 Y) + h2), z clamped at 1e-10, g = 2 s / (n - 1) - 1, valid
+ *     when -1 <= g <= 1; grid_sample bilinear (align_corners, zeros):
+ *     i = ((g + 1) / 2)(n - 1), corner weights nw = (x_se - x)(y_se - y), ne =
+ *     (x - x_sw)(y_sw - y), sw = (x_ne - x)(y - y_ne), se = (x - x_nw)(y - y_nw),
+ *     out = ((nw v + ne v) + sw v) + se v over in-bounds corners (none when
+ *     |i| >= 2^30), truncated;
+ *   gain (core.py:3696-3697): exact integer sums over the overlap, float
+ *     means m = (float)(sum / n), g = (m2 + 1e-3f) / (m1 + 1e-3f) in [0.5, 2],
+ *     w1 = min(max(w1 g, 0), 255);
+ *   seam cost (core.py:3411-3417): sqrtf((d0^2 + d1^2) + d2^2) + 0.05f *
+ *     sqrtf(gx^2 + gy^2) on the channel mean of (w1 + c2) * 0.5; Sobel as
+ *     SciPy (derivative [-1, 0, 1] then smoothing [1, 2, 1], mode 'reflect',
+ *     double accumulation); 1e9 outside the overlap;
+ *   DP seam (core.py:3287-3384) as the reference, in double, first minimum
+ *     on ties; orientation and side from the overlap and solo extents;
+ *   seam mask blur: cv2.GaussianBlur(sigma 3): 25 float32 taps as
+ *     getGaussianKernel, rows then columns, BORDER_REFLECT_101, float32 sums
+ *     in tap order; written back only inside the overlap;
+ *   pyramid (core.py:3475-3553): down = SciPy/CuPy gaussian_filter(sigma 1,
+ *     'reflect', radius 4; float32 weights, fmaf in tap order, rows then
+ *     columns) then [::2, ::2]; up = zoom(order 1) with align-corners
+ *     coordinates in double, clamped to the input, double weights; blend
+ *     g lp1 + (1 - g) lp2 and reconstruction in float32; output truncated.
+ * ======================================================================== */
+static void inv3(const double* a, double* o);
+
+static int refl_half(int p, int n)
+{
+    if (n == 1) return 0;
+    const int period = 2 * n;
+    p %= period;
+    if (p < 0) p += period;
+    return p < n ? p : period - 1 - p;
+}
+
+static int refl_101(int p, int n)
+{
+    if (n == 1) return 0;
+    const int period = 2 * n - 2;
+    p %= period;
+    if (p < 0) p += period;
+    return p < n ? p : period - p;
+}
+
+void or_grid_warp(const uint8_t* img, int h, int w, const double* H, int cw, int ch, double ox, double oy,
+                  uint8_t* out, uint8_t* mask)
+{
+    double Hi[9];
+    inv3(H, Hi);
+    float hf[9];
+    for (int i = 0; i < 9; ++i) hf[i] = (float)Hi[i];
+    const float oxf = (float)ox, oyf = (float)oy, wm = (float)(w - 1), hm = (float)(h - 1);
+    #pragma omp parallel for schedule(static)
+    for (int v = 0; v < ch; ++v)
+        for (int u = 0; u < cw; ++u) {
+            const float X = (float)u - oxf, Y = (float)v - oyf;
+            const float sxn = (hf[0] * X + hf[1] * Y) + hf[2];
+            const float syn = (hf[3] * X + hf[4] * Y) + hf[5];
+            float szn = (hf[6] * X + hf[7] * Y) + hf[8];
+            if (szn < 1e-10f) szn = 1e-10f;
+            const float sx = sxn / szn, sy = syn / szn;
+            const float gx = (2.0f * sx) / wm - 1.0f, gy = (2.0f * sy) / hm - 1.0f;
+            const size_t o = (size_t)v * cw + u;
+            mask[o] = gx >= -1.0f && gx <= 1.0f && gy >= -1.0f && gy <= 1.0f;
+            const float ix = ((gx + 1.0f) / 2.0f) * wm, iy = ((gy + 1.0f) / 2.0f) * hm;
+            const float fx = floorf(ix), fy = floorf(iy);
+            const float ix_nw = fx, iy_nw = fy, ix_ne = fx + 1.0f, iy_ne = fy, ix_sw = fx, iy_sw = fy + 1.0f;
+            const float ix_se = fx + 1.0f, iy_se = fy + 1.0f;
+            const float nw = (ix_se - ix) * (iy_se - iy), ne = (ix - ix_sw) * (iy_sw - iy);
+            const float sw = (ix_ne - ix) * (iy - iy_ne), se = (ix - ix_nw) * (iy - iy_nw);
+            const int fin = fabsf(ix) < 1073741824.0f && fabsf(iy) < 1073741824.0f;
+            const int x0 = fin ? (int)fx : -2, y0 = fin ? (int)fy : -2;
+            const int in_nw = x0 >= 0 && x0 < w && y0 >= 0 && y0 < h, in_ne = x0 + 1 >= 0 && x0 + 1 < w && y0 >= 0 && y0 < h;
+            const int in_sw = x0 >= 0 && x0 < w && y0 + 1 >= 0 && y0 + 1 < h;
+            const int in_se = x0 + 1 >= 0 && x0 + 1 < w && y0 + 1 >= 0 && y0 + 1 < h;
+            for (int c = 0; c < 3; ++c) {
+                float acc = 0.0f;
+                if (fin) {
+                    if (in_nw) acc = acc + (float)img[((size_t)y0 * w + x0) * 3 + c] * nw;
+                    if (in_ne) acc = acc + (float)img[((size_t)y0 * w + x0 + 1) * 3 + c] * ne;
+                    if (in_sw) acc = acc + (float)img[((size_t)(y0 + 1) * w + x0) * 3 + c] * sw;
+                    if (in_se) acc = acc + (float)img[((size_t)(y0 + 1) * w + x0 + 1) * 3 + c] * se;
+                }
+                acc = acc < 0.0f ? 0.0f : (acc > 255.0f ? 255.0f : acc);
+                out[o * 3 + c] = (uint8_t)acc;
+            }
+        }
+}
+
+void or_gauss_weights_sp(float* w9)
+{
+    double phi[9];
+    for (int k = 0; k < 9; ++k) { const double x = (double)(k - 4); phi[k] = exp(-0.5 * (x * x)); }
+    /* numpy's pairwise sum of 9 terms */
+    const double sum = (((phi[0] + phi[1]) + (phi[2] + phi[3])) + ((phi[4] + phi[5]) + (phi[6] + phi[7]))) + phi[8];
+    for (int k = 0; k < 9; ++k) w9[k] = (float)(phi[k] / sum);
+}
+
+void or_gauss_weights_cv(float* w25)
+{
+    double sum = 0.0;
+    for (int i = 0; i < 25; ++i) {
+        const double x = (double)i - 12.0;
+        w25[i] = (float)exp((-0.5 / 9.0) * x * x);
+        sum += w25[i];
+    }
+    sum = 1.0 / sum;
+    for (int i = 0; i < 25; ++i) w25[i] = (float)(w25[i] * sum);
+}
+
+void or_gauss_down(const float* in, int H, int W, const float* w9, float* out)
+{
+    const int Ho = (H + 1) / 2, Wo = (W + 1) / 2;
+    float* t = (float*)malloc((size_t)Ho * W * sizeof(float));
+    for (int yo = 0; yo < Ho; ++yo)
+        for (int x = 0; x < W; ++x) {
+            float acc = 0.0f;
+            for (int k = 0; k < 9; ++k) acc = fmaf(in[(size_t)refl_half(2 * yo + k - 4, H) * W + x], w9[k], acc);
+            t[(size_t)yo * W + x] = acc;
+        }
+    for (int yo = 0; yo < Ho; ++yo)
+        for (int xo = 0; xo < Wo; ++xo) {
+            float acc = 0.0f;
+            for (int k = 0; k < 9; ++k) acc = fmaf(t[(size_t)yo * W + refl_half(2 * xo + k - 4, W)], w9[k], acc);
+            out[(size_t)yo * Wo + xo] = acc;
+        }
+    free(t);
+}
+
+static void zoom_coord(int o, int n_in, int n_out, int* i0, int* i1, double* w0, double* w1)
+{
+    double c = n_out > 1 ? (double)o * ((double)(n_in - 1) / (double)(n_out - 1)) : 0.0;
+    if (c > (double)(n_in - 1)) c = (double)(n_in - 1);
+    const double f = floor(c);
+    *i0 = (int)f;
+    *i1 = *i0 + 1 < n_in ? *i0 + 1 : n_in - 1;
+    *w1 = c - f;
+    *w0 = 1.0 - *w1;
+}
+
+void or_zoom_up(const float* in, int h, int w, int th, int tw, float* out)
+{
+    for (int y = 0; y < th; ++y) {
+        int y0, y1; double wy0, wy1;
+        zoom_coord(y, h, th, &y0, &y1, &wy0, &wy1);
+        for (int x = 0; x < tw; ++x) {
+            int x0, x1; double wx0, wx1;
+            zoom_coord(x, w, tw, &x0, &x1, &wx0, &wx1);
+            double acc = (double)in[(size_t)y0 * w + x0] * (wy0 * wx0);
+            acc = acc + (double)in[(size_t)y0 * w + x1] * (wy0 * wx1);
+            acc = acc + (double)in[(size_t)y1 * w + x0] * (wy1 * wx0);
+            acc = acc + (double)in[(size_t)y1 * w + x1] * (wy1 * wx1);
+            out[(size_t)y * tw + x] = (float)acc;
+        }
+    }
+}
+
+/* SciPy sobel: derivative along `axis`, then [1, 2, 1] along the other,
+   mode 'reflect', double accumulation in tap order, float32 between passes */
+void or_sobel(const float* in, int H, int W, int axis, float* out)
+{
+    float* t = (float*)malloc((size_t)H * W * sizeof(float));
+    const double dk[3] = { -1.0, 0.0, 1.0 }, sk[3] = { 1.0, 2.0, 1.0 };
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            double acc = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                const float v = axis == 1 ? in[(size_t)y * W + refl_half(x + k - 1, W)] : in[(size_t)refl_half(y + k - 1, H) * W + x];
+                acc += (double)v * dk[k];
+            }
+            t[(size_t)y * W + x] = (float)acc;
+        }
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            double acc = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                const float v = axis == 1 ? t[(size_t)refl_half(y + k - 1, H) * W + x] : t[(size_t)y * W + refl_half(x + k - 1, W)];
+                acc += (double)v * sk[k];
+            }
+            out[(size_t)y * W + x] = (float)acc;
+        }
+    free(t);
+}
+
+void or_cv_blur25(const float* in, int H, int W, const float* w25, float* out)
+{
+    float* t = (float*)malloc((size_t)H * W * sizeof(float));
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            float acc = 0.0f;
+            for (int k = 0; k < 25; ++k) acc = acc + w25[k] * in[(size_t)y * W + refl_101(x + k - 12, W)];
+            t[(size_t)y * W + x] = acc;
+        }
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            float acc = 0.0f;
+            for (int k = 0; k < 25; ++k) acc = acc + w25[k] * t[(size_t)refl_101(y + k - 12, H) * W + x];
+            out[(size_t)y * W + x] = acc;
+        }
+    free(t);
+}
+
+/* vertical: one column per row over the rows that hold overlap; horizontal:
+   the transpose.  seam[H] (or seam[W]); rows without overlap keep the last
+   value the reference leaves there (the final seam position). */
+void or_dp_seam(const float* cost, const uint8_t* ov, int H, int W, int vertical, int32_t* seam)
+{
+    const int L = vertical ? H : W, N = vertical ? W : H;     /* L lines of N cells */
+#define CST(l, n) (vertical ? cost[(size_t)(l) * W + (n)] : cost[(size_t)(n) * W + (l)])
+#define OVM(l, n) (vertical ? ov[(size_t)(l) * W + (n)] : ov[(size_t)(n) * W + (l)])
+    const double INF = 1e18;
+    int* lines = (int*)malloc((size_t)L * sizeof(int));
+    int nl = 0;
+    for (int l = 0; l < L; ++l) {
+        int any = 0;
+        for (int n = 0; n < N && !any; ++n) any = OVM(l, n) != 0;
+        if (any) lines[nl++] = l;
+    }
+    if (nl == 0) { for (int l = 0; l < L; ++l) seam[l] = 0; free(lines); return; }
+    double* dp = (double*)malloc((size_t)N * sizeof(double));
+    double* nd = (double*)malloc((size_t)N * sizeof(double));
+    int32_t* bt = (int32_t*)calloc((size_t)L * N, sizeof(int32_t));
+    for (int n = 0; n < N; ++n) dp[n] = OVM(lines[0], n) ? (double)CST(lines[0], n) : INF;
+    for (int i = 1; i < nl; ++i) {
+        const int l = lines[i];
+        for (int n = 0; n < N; ++n) {
+            const double pl = n > 0 ? dp[n - 1] : INF, pm = dp[n], pr = n < N - 1 ? dp[n + 1] : INF;
+            int a = 0; double best = pl;
+            if (pm < best) { best = pm; a = 1; }
+            if (pr < best) { best = pr; a = 2; }
+            nd[n] = OVM(l, n) ? (double)CST(l, n) + best : INF;
+            int b = n + a - 1;
+            bt[(size_t)l * N + n] = b < 0 ? 0 : (b > N - 1 ? N - 1 : b);
+        }
+        double* t = dp; dp = nd; nd = t;
+    }
+    const int last = lines[nl - 1];
+    int sc = 0; double bc = INF; int first = 1;
+    for (int n = 0; n < N; ++n) {
+        const double c = OVM(last, n) ? dp[n] : INF;
+        if (first || c < bc) { bc = c; sc = n; first = 0; }
+    }
+    for (int l = 0; l < L; ++l) seam[l] = sc;
+    for (int i = nl - 1; i > 0; --i) seam[lines[i - 1]] = bt[(size_t)lines[i] * N + seam[lines[i]]];
+#undef CST
+#undef OVM
+    free(lines); free(dp); free(nd); free(bt);
+}
+
+int or_composite(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h2, int w2, const double* H,
+                 int cw, int ch, double ox, double oy, int n_levels,
+                 uint8_t* pano, uint8_t* valid, float* seam_out, float* gain_out, double* rmse, int* vertical_out)
+{
+    if (h1 < 2 || w1 < 2 || h2 < 1 || w2 < 1 || cw < 1 || ch < 1 || n_levels < 1) return -1;
+    const size_t P = (size_t)cw * ch;
+    uint8_t* wu = (uint8_t*)malloc(P * 3);
+    uint8_t* m1 = (uint8_t*)malloc(P);
+    uint8_t* c2 = (uint8_t*)calloc(P * 3, 1);
+    uint8_t* m2 = (uint8_t*)calloc(P, 1);
+    or_grid_warp(img1, h1, w1, H, cw, ch, ox, oy, wu, m1);
+    const int iox = (int)rint(ox), ioy = (int)rint(oy);
+    for (int y = 0; y < h2; ++y)
+        for (int x = 0; x < w2; ++x) {
+            const int u = x + iox, v = y + ioy;
+            if (u < 0 || u >= cw || v < 0 || v >= ch) continue;
+            for (int c = 0; c < 3; ++c) c2[((size_t)v * cw + u) * 3 + c] = img2[((size_t)y * w2 + x) * 3 + c];
+            m2[(size_t)v * cw + u] = 1;
+        }
+    /* gain */
+    uint64_t s1[3] = { 0, 0, 0 }, s2[3] = { 0, 0, 0 }, nov = 0;
+    for (size_t p = 0; p < P; ++p)
+        if (m1[p] && m2[p]) {
+            nov++;
+            for (int c = 0; c < 3; ++c) { s1[c] += wu[p * 3 + c]; s2[c] += c2[p * 3 + c]; }
+        }
+    float g[3] = { 1.0f, 1.0f, 1.0f };
+    if (nov) for (int c = 0; c < 3; ++c) {
+        const float a = (float)((double)s1[c] / (double)nov), b = (float)((double)s2[c] / (double)nov);
+        float t = (b + 1e-3f) / (a + 1e-3f);
+        g[c] = t < 0.5f ? 0.5f : (t > 2.0f ? 2.0f : t);
+    }
+    if (gain_out) for (int c = 0; c < 3; ++c) gain_out[c] = g[c];
+    float* w1g = (float*)malloc(P * 3 * sizeof(float));
+    for (size_t p = 0; p < P; ++p)
+        for (int c = 0; c < 3; ++c) {
+            float t = (float)wu[p * 3 + c];
+            if (nov) { t = t * g[c]; t = t < 0.0f ? 0.0f : (t > 255.0f ? 255.0f : t); }
+            w1g[p * 3 + c] = t;
+        }
+    /* seam mask */
+    float* seam = (float*)calloc(P, sizeof(float));
+    uint8_t* ov = (uint8_t*)malloc(P);
+    for (size_t p = 0; p < P; ++p) { ov[p] = m1[p] && m2[p]; seam[p] = (m1[p] && !m2[p]) ? 1.0f : 0.0f; }
+    int vertical = 1;
+    if (nov) {
+        float* avg = (float*)malloc(P * sizeof(float));
+        float* cost = (float*)malloc(P * sizeof(float));
+        float* gx = (float*)malloc(P * sizeof(float));
+        float* gy = (float*)malloc(P * sizeof(float));
+        for (size_t p = 0; p < P; ++p) {
+            float d2[3], hv[3];
+            for (int c = 0; c < 3; ++c) {
+                const float a = w1g[p * 3 + c], b = (float)c2[p * 3 + c];
+                const float d = a - b;
+                d2[c] = d * d;
+                hv[c] = (a + b) * 0.5f;
+            }
+            cost[p] = sqrtf((d2[0] + d2[1]) + d2[2]);
+            avg[p] = ((hv[0] + hv[1]) + hv[2]) / 3.0f;
+        }
+        or_sobel(avg, ch, cw, 1, gx);
+        or_sobel(avg, ch, cw, 0, gy);
+        for (size_t p = 0; p < P; ++p) {
+            const float gr = sqrtf(gx[p] * gx[p] + gy[p] * gy[p]);
+            cost[p] = ov[p] ? cost[p] + 0.05f * gr : 1e9f;
+        }
+        /* orientation and side */
+        long long nr = 0, nc = 0, s1c = 0, n1c = 0, s2c = 0, n2c = 0, s1r = 0, n1r = 0, s2r = 0, n2r = 0;
+        for (int y = 0; y < ch; ++y) {
+            int a = 0, b = 0, d = 0;
+            for (int x = 0; x < cw; ++x) {
+                const size_t p = (size_t)y * cw + x;
+                a |= ov[p]; b |= m1[p] && !m2[p]; d |= m2[p] && !m1[p];
+            }
+            nr += a; if (b) { s1r += y; n1r++; } if (d) { s2r += y; n2r++; }
+        }
+        for (int x = 0; x < cw; ++x) {
+            int a = 0, b = 0, d = 0;
+            for (int y = 0; y < ch; ++y) {
+                const size_t p = (size_t)y * cw + x;
+                a |= ov[p]; b |= m1[p] && !m2[p]; d |= m2[p] && !m1[p];
+            }
+            nc += a; if (b) { s1c += x; n1c++; } if (d) { s2c += x; n2c++; }
+        }
+        const int left = n1c == 0 || n2c == 0 || (double)s1c / (double)n1c <= (double)s2c / (double)n2c;
+        const int top = n1r == 0 || n2r == 0 || (double)s1r / (double)n1r <= (double)s2r / (double)n2r;
+        vertical = nr >= nc;
+        const int L = vertical ? ch : cw;
+        int32_t* sl = (int32_t*)malloc((size_t)L * sizeof(int32_t));
+        or_dp_seam(cost, ov, ch, cw, vertical, sl);
+        const int first = vertical ? left : top;
+        for (int l = 0; l < L; ++l) {
+            const int N = vertical ? cw : ch;
+            int any = 0;
+            for (int n = 0; n < N && !any; ++n) any = vertical ? ov[(size_t)l * cw + n] : ov[(size_t)n * cw + l];
+            if (!any) continue;
+            for (int n = 0; n < N; ++n) {
+                const size_t p = vertical ? (size_t)l * cw + n : (size_t)n * cw + l;
+                if (!ov[p]) continue;
+                seam[p] = (n <= sl[l]) == first ? 1.0f : 0.0f;
+            }
+        }
+        free(sl);
+        float w25[25];
+        or_gauss_weights_cv(w25);
+        float* bl = (float*)malloc(P * sizeof(float));
+        or_cv_blur25(seam, ch, cw, w25, bl);
+        for (size_t p = 0; p < P; ++p) if (ov[p]) seam[p] = bl[p];
+        free(bl); free(avg); free(cost); free(gx); free(gy);
+    }
+    if (seam_out) memcpy(seam_out, seam, P * sizeof(float));
+    /* multiband: planes 0-2 image 1, 3-5 image 2, 6 mask */
+    float w9[9];
+    or_gauss_weights_sp(w9);
+    int hs[16], ws[16];
+    float* gp[7][16];
+    hs[0] = ch; ws[0] = cw;
+    for (int k = 1; k < n_levels; ++k) { hs[k] = (hs[k - 1] + 1) / 2; ws[k] = (ws[k - 1] + 1) / 2; }
+    for (int q = 0; q < 7; ++q) {
+        gp[q][0] = (float*)malloc(P * sizeof(float));
+        for (size_t p = 0; p < P; ++p) {
+            float v;
+            if (q == 6) v = seam[p];
+            else {
+                const int c = q % 3;
+                const float a = w1g[p * 3 + c], b = (float)c2[p * 3 + c];
+                if (q < 3) v = m1[p] ? a : (m2[p] ? b : 0.0f);
+                else v = m2[p] ? b : (m1[p] ? a : 0.0f);
+            }
+            gp[q][0][p] = v;
+        }
+        for (int k = 1; k < n_levels; ++k) {
+            gp[q][k] = (float*)malloc((size_t)hs[k] * ws[k] * sizeof(float));
+            or_gauss_down(gp[q][k - 1], hs[k - 1], ws[k - 1], w9, gp[q][k]);
+        }
+    }
+    float* res[3];
+    for (int c = 0; c < 3; ++c) {
+        const int kL = n_levels - 1;
+        const size_t nL = (size_t)hs[kL] * ws[kL];
+        res[c] = (float*)malloc(nL * sizeof(float));
+        for (size_t p = 0; p < nL; ++p) {
+            const float m = gp[6][kL][p];
+            res[c][p] = m * gp[c][kL][p] + (1.0f - m) * gp[c + 3][kL][p];
+        }
+        for (int k = n_levels - 2; k >= 0; --k) {
+            const size_t n = (size_t)hs[k] * ws[k];
+            float* u1 = (float*)malloc(n * sizeof(float));
+            float* u2 = (float*)malloc(n * sizeof(float));
+            float* ur = (float*)malloc(n * sizeof(float));
+            or_zoom_up(gp[c][k + 1], hs[k + 1], ws[k + 1], hs[k], ws[k], u1);
+            or_zoom_up(gp[c + 3][k + 1], hs[k + 1], ws[k + 1], hs[k], ws[k], u2);
+            or_zoom_up(res[c], hs[k + 1], ws[k + 1], hs[k], ws[k], ur);
+            for (size_t p = 0; p < n; ++p) {
+                const float m = gp[6][k][p];
+                const float l1 = gp[c][k][p] - u1[p], l2 = gp[c + 3][k][p] - u2[p];
+                const float lb = m * l1 + (1.0f - m) * l2;
+                ur[p] = ur[p] + lb;
+            }
+            free(u1); free(u2); free(res[c]);
+            res[c] = ur;
+        }
+    }
+    for (size_t p = 0; p < P; ++p) {
+        valid[p] = m1[p] || m2[p];
+        for (int c = 0; c < 3; ++c) {
+            float v = res[c][p];
+            v = v < 0.0f ? 0.0f : (v > 255.0f ? 255.0f : v);
+            pano[p * 3 + c] = valid[p] ? (uint8_t)v : 0;
+        }
+    }
+    /* seam RMSE over the overlap of the (gained) uint8 images */
+    if (rmse) {
+        uint64_t sq = 0;
+        for (size_t p = 0; p < P; ++p)
+            if (ov[p]) for (int c = 0; c < 3; ++c) {
+                const int d = (int)(uint8_t)w1g[p * 3 + c] - (int)c2[p * 3 + c];
+                sq += (uint64_t)(d * d);
+            }
+        *rmse = nov ? sqrt((double)sq / (double)(3 * nov)) : NAN;
+    }
+    if (vertical_out) *vertical_out = nov ? vertical : -1;
+    for (int q = 0; q < 7; ++q) for (int k = 0; k < n_levels; ++k) free(gp[q][k]);
+    for (int c = 0; c < 3; ++c) free(res[c]);
+    free(wu); free(m1); free(c2); free(m2); free(w1g); free(seam); free(ov);
+    return 0;
+}
+
+/* ========================================================================
  * Philox4x32-10 (Salmon et al., SC'11)
  * ======================================================================== */
 void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])

@@ -84,6 +84,21 @@ void or_sim16(const float* a, int N, const float* b, int M, int D, float* s16 /*
 void or_match_sim16(const float* a, int N, const float* b, int M, int D, double ratio,
                     int32_t* fwd /*N*3: j, d1 bits, d2 bits*/, int32_t* bwd, int32_t* pairs, int* npairs);
 
+/* ---- F3: gain + DP seam + multiband compositing ---- */
+void or_grid_warp(const uint8_t* img, int h, int w, const double* H, int cw, int ch, double ox, double oy,
+                  uint8_t* out /*ch*cw*3*/, uint8_t* mask /*ch*cw*/);
+void or_gauss_weights_sp(float* w9);                  /* sigma 1, radius 4, float32 */
+void or_gauss_weights_cv(float* w25);                 /* cv2 getGaussianKernel(25, 3), float32 */
+void or_gauss_down(const float* in, int H, int W, const float* w9, float* out /*((H+1)/2)*((W+1)/2)*/);
+void or_zoom_up(const float* in, int h, int w, int th, int tw, float* out);
+void or_sobel(const float* in, int H, int W, int axis, float* out);
+void or_cv_blur25(const float* in, int H, int W, const float* w25, float* out);
+void or_dp_seam(const float* cost, const uint8_t* ov, int H, int W, int vertical, int32_t* seam);
+int  or_composite(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h2, int w2, const double* H,
+                  int cw, int ch, double ox, double oy, int n_levels,
+                  uint8_t* pano /*ch*cw*3*/, uint8_t* valid, float* seam_mask /*ch*cw, may be NULL*/,
+                  float* gain /*3*/, double* rmse, int* vertical);
+
 /* ---- Philox4x32-10 ---- */
 void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void or_ransac_sample(int B, int N, uint64_t seed, int32_t* idx /*B*4*/);

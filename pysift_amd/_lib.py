@@ -27,6 +27,7 @@ EXPORTS = [
     "msift_match_batch_gather_points", "msift_ransac_batch_create", "msift_ransac_batch_destroy",
     "msift_ransac_batch_sizes", "msift_ransac_homography_batched", "msift_clahe_workspace_bytes", "msift_clahe",
     "msift_l2_normalize", "msift_pca_workspace_bytes", "msift_pca_fit", "msift_pca_project", "msift_match_sim_f16",
+    "msift_composite_workspace_bytes", "msift_composite_multiband",
 ]
 
 
@@ -85,6 +86,10 @@ def _declare(L):
     L.msift_pca_fit.restype = _I
     L.msift_pca_fit.argtypes = [_P, _I, _P, _I, _I, _P, _SZ, _P, _P, _P, _P, _P]
     L.msift_pca_project.restype = _I; L.msift_pca_project.argtypes = [_P, _I, _P, _P, _I, _P, _P]
+    L.msift_composite_workspace_bytes.restype = _I
+    L.msift_composite_workspace_bytes.argtypes = [_I, _I, _I, ctypes.POINTER(_SZ)]
+    L.msift_composite_multiband.restype = _I
+    L.msift_composite_multiband.argtypes = [_P, _I, _I, _P, _I, _I, _P, _I, _I, _D, _D, _I, _P, _SZ, _P, _P, _P, _P, _P]
     L.msift_match_sim_f16.restype = _I
     L.msift_match_sim_f16.argtypes = [_P, _I, _P, _I, _I, _D, _P, _SZ, _P, _P, _P, _P, _P]
     L.msift_batch_create.restype = _I

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 LIB = os.path.join(HERE, "libpysift_amd.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["capi.hip", "scan.hip", "detect.hip", "batch.hip", "match.hip", "ransac.hip", "warp.hip", "grad.hip", "clahe.hip", "pca.hip"]
+SOURCES = ["capi.hip", "scan.hip", "detect.hip", "batch.hip", "match.hip", "ransac.hip", "warp.hip", "grad.hip", "clahe.hip", "pca.hip", "composite.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PYSIFT_AMD_ARCH", "gfx950")

@@ -212,6 +212,23 @@ int msift_bgr_to_gray(const uint8_t* d_bgr, int H, int W, uint8_t* d_gray, void*
     return bgr_to_gray(d_bgr, H, W, d_gray, (hipStream_t)stream);
 }
 
+int msift_composite_workspace_bytes(int cw, int ch, int n_levels, size_t* bytes) {
+    if (cw < 1 || ch < 1 || n_levels < 1 || n_levels > 12 || !bytes) {
+        set_error("msift_composite_workspace_bytes: bad arguments (1 <= n_levels <= 12)");
+        return MSIFT_EINVAL;
+    }
+    *bytes = composite_workspace(cw, ch, n_levels);
+    return MSIFT_OK;
+}
+
+int msift_composite_multiband(const uint8_t* d_img1, int h1, int w1, const uint8_t* d_img2, int h2, int w2,
+                              const double* H, int cw, int ch, double ox, double oy, int n_levels, void* d_ws,
+                              size_t ws_bytes, uint8_t* d_pano, uint8_t* d_valid, float* d_seam_mask,
+                              msift_composite_stats* d_stats, void* stream) {
+    return composite(d_img1, h1, w1, d_img2, h2, w2, H, cw, ch, ox, oy, n_levels, d_ws, ws_bytes, d_pano, d_valid,
+                     d_seam_mask, d_stats, (hipStream_t)stream);
+}
+
 int msift_l2_normalize(const float* d_x, int n, float* d_y, void* stream) {
     return l2_normalize_rows(d_x, n, d_y, (hipStream_t)stream);
 }

@@ -1,0 +1,742 @@
+// The reference's composition of a stitched pair (core.py:3686-3714,
+// 3783-3802): grid_sample warp of image 1 (core.py:3211-3239), paste of
+// image 2, gain compensation over the overlap, a DP seam through a colour +
+// gradient cost (core.py:3287-3472), a Gaussian-softened seam mask, and a
+// Laplacian-pyramid (multiband) blend (core.py:3475-3553).  Every stage runs
+// on the device; the decisions that depend on whole-image reductions (gain,
+// seam orientation and side) are made by a one-block kernel and read by the
+// later kernels, so the host never waits in the middle.
+//
+// Arithmetic (the CPU oracle reproduces it bit for bit):
+//   warp: X = u - (float)ox; source = ((h0 X + h1 Y) + h2) / max(z, 1e-10);
+//     g = 2 s / (n - 1) - 1, valid when -1 <= g <= 1; i = ((g + 1) / 2)(n - 1);
+//     grid_sample corner weights nw = (x_se - x)(y_se - y), ne = (x - x_sw)
+//     (y_sw - y), sw = (x_ne - x)(y - y_ne), se = (x - x_nw)(y - y_nw), summed
+//     nw, ne, sw, se over in-bounds corners; clipped and truncated to u8;
+//   gain: exact integer sums over the overlap, m = (float)(sum / n),
+//     g = clamp((m2 + 1e-3f) / (m1 + 1e-3f), 0.5, 2), w1 = clamp(w1 g, 0, 255);
+//   cost: sqrtf((d0^2 + d1^2) + d2^2) + 0.05f sqrtf(gx^2 + gy^2), Sobel of the
+//     channel mean of (w1 + c2) * 0.5 with SciPy's passes (derivative, then
+//     [1, 2, 1]; 'reflect'; double accumulation); 1e9 outside the overlap;
+//   seam: dynamic programme in double over the lines holding overlap, first
+//     minimum on ties; vertical when the overlap spans at least as many rows
+//     as columns; the mask is 1 on image 1's side; then a 25-tap float32
+//     Gaussian (sigma 3, BORDER_REFLECT_101, rows then columns) replaces it
+//     inside the overlap;
+//   pyramid: sigma-1 Gaussian (float32 taps, fmaf in tap order, 'reflect',
+//     rows then columns) evaluated only at the kept [::2, ::2] samples;
+//     bilinear up-sampling with align-corners coordinates and weights in
+//     double; blend m l1 + (1 - m) l2 and reconstruction in float32;
+//     output clipped and truncated inside the union of the two images.
+#include "msift_internal.hpp"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#pragma clang fp contract(off)
+
+namespace msift {
+
+namespace {
+
+constexpr int kCT = 256;
+constexpr int kMaxLevels = 12;
+
+// device-side accumulators and decisions; the layout of msift_composite_stats
+typedef msift_composite_stats CmpStats;
+static_assert(sizeof(CmpStats) == 104, "msift_composite_stats layout");
+
+struct Hf9 { float h[9]; };
+struct W9 { float w[9]; };
+struct W25 { float w[25]; };
+
+__device__ __forceinline__ int refl_half(int p, int n) {
+    if (n == 1) return 0;
+    const int period = 2 * n;
+    p %= period;
+    if (p < 0) p += period;
+    return p < n ? p : period - 1 - p;
+}
+
+__device__ __forceinline__ int refl_101(int p, int n) {
+    if (n == 1) return 0;
+    const int period = 2 * n - 2;
+    p %= period;
+    if (p < 0) p += period;
+    return p < n ? p : period - p;
+}
+
+__device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v, unsigned long long* sh) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    unsigned long long t = 0;
+    for (int k = 0; k < kCT / 64; ++k) t += sh[k];
+    __syncthreads();
+    return t;
+}
+
+// grid (ceil(cw / 256), ch): warp image 1, paste image 2; each block adds
+// its overlap sums to its row's partials (rowacc[v][0..6]: n, s1[3], s2[3])
+// and ORs its flags into rowflags[v] (bit 0 overlap, bit 1 image 1 only,
+// bit 2 image 2 only)
+__global__ __launch_bounds__(kCT) void k_cmp_warp(const uint8_t* __restrict__ img1, int h1, int w1,
+                                                  const uint8_t* __restrict__ img2, int h2, int w2, Hf9 hf, int cw,
+                                                  float oxf, float oyf, int iox, int ioy, uint8_t* __restrict__ wu,
+                                                  uint8_t* __restrict__ c2, uint8_t* __restrict__ m1,
+                                                  uint8_t* __restrict__ m2, unsigned long long* __restrict__ rowacc,
+                                                  uint32_t* __restrict__ rowflags) {
+    __shared__ unsigned long long sh[kCT / 64];
+    const int u = blockIdx.x * kCT + threadIdx.x, v = blockIdx.y;
+    unsigned long long acc7[7] = {0, 0, 0, 0, 0, 0, 0};
+    uint32_t bits = 0;
+    if (u < cw) {
+        const size_t o = (size_t)v * cw + u;
+        const float wm = (float)(w1 - 1), hm = (float)(h1 - 1);
+        const float X = (float)u - oxf, Y = (float)v - oyf;
+        const float sxn = (hf.h[0] * X + hf.h[1] * Y) + hf.h[2];
+        const float syn = (hf.h[3] * X + hf.h[4] * Y) + hf.h[5];
+        float szn = (hf.h[6] * X + hf.h[7] * Y) + hf.h[8];
+        if (szn < 1e-10f) szn = 1e-10f;
+        const float sx = sxn / szn, sy = syn / szn;
+        const float gx = (2.0f * sx) / wm - 1.0f, gy = (2.0f * sy) / hm - 1.0f;
+        const bool in1 = gx >= -1.0f && gx <= 1.0f && gy >= -1.0f && gy <= 1.0f;
+        m1[o] = in1;
+        const float ix = ((gx + 1.0f) / 2.0f) * wm, iy = ((gy + 1.0f) / 2.0f) * hm;
+        const float fx = floorf(ix), fy = floorf(iy);
+        const float nw = ((fx + 1.0f) - ix) * ((fy + 1.0f) - iy), ne = (ix - fx) * ((fy + 1.0f) - iy);
+        const float sw = ((fx + 1.0f) - ix) * (iy - fy), se = (ix - fx) * (iy - fy);
+        const bool finite = fabsf(ix) < 1073741824.0f && fabsf(iy) < 1073741824.0f;
+        const int x0 = finite ? (int)fx : -2, y0 = finite ? (int)fy : -2;
+        const bool bx0 = x0 >= 0 && x0 < w1, bx1 = x0 + 1 >= 0 && x0 + 1 < w1;
+        const bool by0 = y0 >= 0 && y0 < h1, by1 = y0 + 1 >= 0 && y0 + 1 < h1;
+        const int px = u - iox, py = v - ioy;
+        const bool in2 = px >= 0 && px < w2 && py >= 0 && py < h2;
+        m2[o] = in2;
+        bits = (in1 && in2 ? 1u : 0u) | (in1 && !in2 ? 2u : 0u) | (!in1 && in2 ? 4u : 0u);
+        acc7[0] = in1 && in2;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float acc = 0.0f;
+            if (finite) {
+                if (bx0 && by0) acc = acc + (float)img1[((size_t)y0 * w1 + x0) * 3 + c] * nw;
+                if (bx1 && by0) acc = acc + (float)img1[((size_t)y0 * w1 + x0 + 1) * 3 + c] * ne;
+                if (bx0 && by1) acc = acc + (float)img1[((size_t)(y0 + 1) * w1 + x0) * 3 + c] * sw;
+                if (bx1 && by1) acc = acc + (float)img1[((size_t)(y0 + 1) * w1 + x0 + 1) * 3 + c] * se;
+            }
+            acc = acc < 0.0f ? 0.0f : (acc > 255.0f ? 255.0f : acc);
+            const uint8_t w8 = (uint8_t)acc;
+            const uint8_t q8 = in2 ? img2[((size_t)py * w2 + px) * 3 + c] : 0;
+            wu[o * 3 + c] = w8;
+            c2[o * 3 + c] = q8;
+            if (in1 && in2) { acc7[1 + c] = w8; acc7[4 + c] = q8; }
+        }
+    }
+    const unsigned long long n = block_sum_u64(acc7[0], sh);
+    if (n) {
+        if (threadIdx.x == 0) atomicAdd(&rowacc[(size_t)v * 8], n);
+        for (int k = 1; k < 7; ++k) {
+            const unsigned long long t = block_sum_u64(acc7[k], sh);
+            if (threadIdx.x == 0) atomicAdd(&rowacc[(size_t)v * 8 + k], t);
+        }
+    }
+    uint32_t wb = bits;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) wb |= __shfl_xor(wb, off, 64);
+    if ((threadIdx.x & 63) == 0 && wb) atomicOr(&rowflags[v], wb);
+}
+
+// grid (ceil(cw / 256), ceil(ch / 64)): a thread ORs 64 rows of its column's
+// flags into colflags[u]
+__global__ __launch_bounds__(kCT) void k_cmp_colflags(const uint8_t* __restrict__ m1, const uint8_t* __restrict__ m2,
+                                                      int cw, int ch, uint32_t* __restrict__ colflags) {
+    const int u = blockIdx.x * kCT + threadIdx.x;
+    if (u >= cw) return;
+    const int v0 = blockIdx.y * 64, v1 = min(ch, v0 + 64);
+    uint32_t bits = 0;
+    for (int v = v0; v < v1; ++v) {
+        const size_t o = (size_t)v * cw + u;
+        const bool a = m1[o], b = m2[o];
+        bits |= (a && b ? 1u : 0u) | (a && !b ? 2u : 0u) | (!a && b ? 4u : 0u);
+    }
+    if (bits) atomicOr(&colflags[u], bits);
+}
+
+// one block: gain, seam orientation and side (core.py:3419-3431, 3696)
+__global__ __launch_bounds__(kCT) void k_cmp_decide(const uint32_t* __restrict__ rowflags, int ch,
+                                                    const uint32_t* __restrict__ colflags, int cw,
+                                                    const unsigned long long* __restrict__ rowacc,
+                                                    CmpStats* __restrict__ st) {
+    __shared__ unsigned long long sh[kCT / 64];
+    {
+        unsigned long long t7[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (int v = threadIdx.x; v < ch; v += kCT)
+            for (int k = 0; k < 7; ++k) t7[k] += rowacc[(size_t)v * 8 + k];
+        for (int k = 0; k < 7; ++k) t7[k] = block_sum_u64(t7[k], sh);
+        if (threadIdx.x == 0) {
+            st->n_overlap = t7[0];
+            for (int c = 0; c < 3; ++c) { st->sum1[c] = t7[1 + c]; st->sum2[c] = t7[4 + c]; }
+        }
+        __syncthreads();
+    }
+    unsigned long long r[5] = {0, 0, 0, 0, 0}, c[5] = {0, 0, 0, 0, 0};   // n_ov, sum1, n1, sum2, n2
+    for (int v = threadIdx.x; v < ch; v += kCT) {
+        const uint32_t f = rowflags[v];
+        r[0] += f & 1u;
+        if (f & 2u) { r[1] += v; r[2]++; }
+        if (f & 4u) { r[3] += v; r[4]++; }
+    }
+    for (int u = threadIdx.x; u < cw; u += kCT) {
+        const uint32_t f = colflags[u];
+        c[0] += f & 1u;
+        if (f & 2u) { c[1] += u; c[2]++; }
+        if (f & 4u) { c[3] += u; c[4]++; }
+    }
+    for (int k = 0; k < 5; ++k) { r[k] = block_sum_u64(r[k], sh); c[k] = block_sum_u64(c[k], sh); }
+    if (threadIdx.x == 0) {
+        const unsigned long long n = st->n_overlap;
+        st->has_overlap = n > 0;
+        for (int k = 0; k < 3; ++k) {
+            float g = 1.0f;
+            if (n) {
+                const float a = (float)((double)st->sum1[k] / (double)n), b = (float)((double)st->sum2[k] / (double)n);
+                g = (b + 1e-3f) / (a + 1e-3f);
+                g = g < 0.5f ? 0.5f : (g > 2.0f ? 2.0f : g);
+            }
+            st->gain[k] = g;
+        }
+        const bool left = c[2] == 0 || c[4] == 0 || (double)c[1] / (double)c[2] <= (double)c[3] / (double)c[4];
+        const bool top = r[2] == 0 || r[4] == 0 || (double)r[1] / (double)r[2] <= (double)r[3] / (double)r[4];
+        st->vertical = r[0] >= c[0];
+        st->img1_first = st->vertical ? left : top;
+        st->sq_diff = 0;
+    }
+}
+
+// gained image 1, the level-0 planes of both images (each filled with the
+// other outside its own region), the colour cost, the channel mean and the
+// seam RMSE sums; planes: [I1 b, g, r, I2 b, g, r, mask] of P floats
+__global__ __launch_bounds__(kCT) void k_cmp_prepare(const uint8_t* __restrict__ wu, const uint8_t* __restrict__ c2,
+                                                     const uint8_t* __restrict__ m1, const uint8_t* __restrict__ m2,
+                                                     size_t P, CmpStats* __restrict__ st, float* __restrict__ planes,
+                                                     float* __restrict__ costc, float* __restrict__ avg,
+                                                     float* __restrict__ seam0) {
+    __shared__ unsigned long long sh[kCT / 64];
+    const size_t p = (size_t)blockIdx.x * kCT + threadIdx.x;
+    unsigned long long sq = 0;
+    if (p < P) {
+        const bool a = m1[p], b = m2[p], any = st->has_overlap;
+        float d2[3], hv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float w = (float)wu[p * 3 + c];
+            if (any) { w = w * st->gain[c]; w = w < 0.0f ? 0.0f : (w > 255.0f ? 255.0f : w); }
+            const float q = (float)c2[p * 3 + c];
+            const float d = w - q;
+            d2[c] = d * d;
+            hv[c] = (w + q) * 0.5f;
+            planes[(size_t)c * P + p] = a ? w : (b ? q : 0.0f);
+            planes[(size_t)(c + 3) * P + p] = b ? q : (a ? w : 0.0f);
+            if (a && b) { const int e = (int)(uint8_t)w - (int)c2[p * 3 + c]; sq += (unsigned long long)(e * e); }
+        }
+        costc[p] = sqrtf((d2[0] + d2[1]) + d2[2]);
+        avg[p] = ((hv[0] + hv[1]) + hv[2]) / 3.0f;
+        seam0[p] = (a && !b) ? 1.0f : 0.0f;
+    }
+    sq = block_sum_u64(sq, sh);
+    if (threadIdx.x == 0 && sq) atomicAdd(&st->sq_diff, sq);
+}
+
+// Sobel first passes: dx = [-1, 0, 1] along x, dy = [-1, 0, 1] along y
+__global__ __launch_bounds__(kCT) void k_cmp_sobel1(const float* __restrict__ avg, int cw, int ch,
+                                                    float* __restrict__ dx, float* __restrict__ dy) {
+    const int u = blockIdx.x * kCT + threadIdx.x, v = blockIdx.y;
+    if (u >= cw) return;
+    const double k3[3] = { -1.0, 0.0, 1.0 };
+    double ax = 0.0, ay = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        ax += (double)avg[(size_t)v * cw + refl_half(u + k - 1, cw)] * k3[k];
+        ay += (double)avg[(size_t)refl_half(v + k - 1, ch) * cw + u] * k3[k];
+    }
+    dx[(size_t)v * cw + u] = (float)ax;
+    dy[(size_t)v * cw + u] = (float)ay;
+}
+
+// Sobel second passes ([1, 2, 1] across), gradient magnitude, total cost;
+// the cost is stored along the seam's lines (row-major for a vertical seam,
+// column-major for a horizontal one) for the dynamic programme's reads
+__global__ __launch_bounds__(kCT) void k_cmp_sobel2(const float* __restrict__ dx, const float* __restrict__ dy,
+                                                    const float* __restrict__ costc, const uint8_t* __restrict__ m1,
+                                                    const uint8_t* __restrict__ m2, int cw, int ch,
+                                                    const CmpStats* __restrict__ st, float* __restrict__ cost) {
+    const int u = blockIdx.x * kCT + threadIdx.x, v = blockIdx.y;
+    if (u >= cw) return;
+    const double k3[3] = { 1.0, 2.0, 1.0 };
+    double ax = 0.0, ay = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        ax += (double)dx[(size_t)refl_half(v + k - 1, ch) * cw + u] * k3[k];
+        ay += (double)dy[(size_t)v * cw + refl_half(u + k - 1, cw)] * k3[k];
+    }
+    const float gx = (float)ax, gy = (float)ay;
+    const size_t o = (size_t)v * cw + u;
+    const float gr = sqrtf(gx * gx + gy * gy);
+    const float c = (m1[o] && m2[o]) ? costc[o] + 0.05f * gr : 1e9f;
+    cost[st->vertical ? o : (size_t)u * ch + v] = c;
+}
+
+// one block: the seam's dynamic programme and backtrack.  Lines are rows
+// (vertical seam) or columns; a cell belongs to the overlap exactly when its
+// cost is below 1e9.  The two dp lines live in LDS up to kDpLds cells (in
+// global memory beyond).  The backtrack runs in windows of kWin lines: the
+// seam moves at most one cell per line, so the block first loads those
+// lines' back-pointers within kWin cells of the current position into LDS,
+// then one thread walks them.
+constexpr int kDpLds = 9088;
+constexpr int kDpJ = (kDpLds + 1023) / 1024;
+constexpr int kMaxLines = 65536;  // LDS bitmask of the lines holding overlap
+constexpr int kWin = 64;
+
+__global__ __launch_bounds__(1024) void k_cmp_dp(const float* __restrict__ cost, const uint32_t* __restrict__ rowflags,
+                                                 const uint32_t* __restrict__ colflags, int cw, int ch,
+                                                 const CmpStats* __restrict__ st, double* __restrict__ gdpa,
+                                                 double* __restrict__ gdpb, uint8_t* __restrict__ bt,
+                                                 int32_t* __restrict__ lines, int32_t* __restrict__ seam) {
+    __shared__ double ldp[2][kDpLds];
+    __shared__ uint8_t win[kWin][2 * kWin + 1];
+    __shared__ uint32_t ovl[kMaxLines / 32];
+    __shared__ double rv[16];
+    __shared__ int ri[16];
+    __shared__ int s_pos;
+    if (!st->has_overlap) return;
+    const bool vert = st->vertical;
+    const int L = vert ? ch : cw, N = vert ? cw : ch;
+    const uint32_t* lf = vert ? rowflags : colflags;
+    const double INF = 1e18;
+    const int tid = threadIdx.x;
+    const bool in_lds = N <= kDpLds;
+    for (int w = tid; w < (L + 31) / 32; w += 1024) {
+        uint32_t bits = 0;
+        for (int b = 0; b < 32; ++b) {
+            const int l = 32 * w + b;
+            if (l < L && (lf[l] & 1u)) bits |= 1u << b;
+        }
+        ovl[w] = bits;
+    }
+    __syncthreads();
+    auto has_ov = [&](int l) { return (ovl[l >> 5] >> (l & 31)) & 1u; };
+    // forward pass over the lines holding overlap.  LDS path: the dp lines
+    // are indexed in the __shared__ array directly (LDS instructions, which
+    // do not wait on the global loads in flight) and the next line's costs
+    // are loaded while the current one is computed.
+    int nl = 0;
+    int l = 0;
+    while (l < L && !has_ov(l)) ++l;
+    int pb = 0;                 // LDS path: ldp[pb] holds the previous line
+    double* dp = gdpa;          // global path
+    double* nd = gdpb;
+    float cur[kDpJ], nxt[kDpJ];
+#pragma unroll
+    for (int j = 0; j < kDpJ; ++j) {
+        const int n = tid + 1024 * j;
+        cur[j] = (in_lds && n < N) ? cost[(size_t)l * N + n] : 1e9f;
+    }
+    while (l < L) {
+        int ln = l + 1;
+        while (ln < L && !has_ov(ln)) ++ln;
+        if (tid == 0) lines[nl] = l;
+        if (in_lds) {
+#pragma unroll
+            for (int j = 0; j < kDpJ; ++j) {
+                const int n = tid + 1024 * j;
+                nxt[j] = (ln < L && n < N) ? cost[(size_t)ln * N + n] : 1e9f;
+            }
+#pragma unroll
+            for (int j = 0; j < kDpJ; ++j) {
+                const int n = tid + 1024 * j;
+                if (n >= N) continue;
+                const float c = cur[j];
+                const bool ov = c < 1e9f;
+                if (nl == 0) {
+                    ldp[pb ^ 1][n] = ov ? (double)c : INF;
+                } else {
+                    const double pl = n > 0 ? ldp[pb][n - 1] : INF, pm = ldp[pb][n];
+                    const double pr = n < N - 1 ? ldp[pb][n + 1] : INF;
+                    int a = 0;
+                    double best = pl;
+                    if (pm < best) { best = pm; a = 1; }
+                    if (pr < best) { best = pr; a = 2; }
+                    ldp[pb ^ 1][n] = ov ? (double)c + best : INF;
+                    bt[(size_t)l * N + n] = (uint8_t)a;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kDpJ; ++j) cur[j] = nxt[j];
+            pb ^= 1;
+        } else {
+            for (int n = tid; n < N; n += 1024) {
+                const float c = cost[(size_t)l * N + n];
+                const bool ov = c < 1e9f;
+                if (nl == 0) {
+                    nd[n] = ov ? (double)c : INF;
+                } else {
+                    const double pl = n > 0 ? dp[n - 1] : INF, pm = dp[n], pr = n < N - 1 ? dp[n + 1] : INF;
+                    int a = 0;
+                    double best = pl;
+                    if (pm < best) { best = pm; a = 1; }
+                    if (pr < best) { best = pr; a = 2; }
+                    nd[n] = ov ? (double)c + best : INF;
+                    bt[(size_t)l * N + n] = (uint8_t)a;
+                }
+            }
+            double* t = dp; dp = nd; nd = t;
+        }
+        __syncthreads();
+        nl++;
+        l = ln;
+    }
+    if (in_lds) dp = ldp[pb];
+    const int last = lines[nl - 1];
+    // first minimum over the last line's overlap cells
+    double bv = INF;
+    int bi = 0x7fffffff;
+    for (int n = tid; n < N; n += 1024) {
+        const float c = cost[(size_t)last * N + n];
+        const double d = c < 1e9f ? dp[n] : INF;
+        if (d < bv || (d == bv && n < bi)) { bv = d; bi = n; }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double ov = __shfl_xor(bv, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (ov < bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    }
+    if ((tid & 63) == 0) { rv[tid >> 6] = bv; ri[tid >> 6] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+        for (int k = 1; k < 16; ++k)
+            if (rv[k] < rv[0] || (rv[k] == rv[0] && ri[k] < ri[0])) { rv[0] = rv[k]; ri[0] = ri[k]; }
+        s_pos = ri[0] == 0x7fffffff ? 0 : ri[0];
+    }
+    __syncthreads();
+    const int sc = s_pos;
+    for (int l = tid; l < L; l += 1024) seam[l] = sc;
+    __syncthreads();
+    // walk back: entry i (from nl - 1 down to 1) sets seam[lines[i - 1]]
+    for (int i = nl - 1; i > 0; i -= kWin) {
+        const int cnt = min(kWin, i);           // entries i, i - 1, ..., i - cnt + 1
+        const int pos0 = s_pos;
+        for (int e = tid; e < cnt * (2 * kWin + 1); e += 1024) {
+            const int j = e / (2 * kWin + 1), d = e - j * (2 * kWin + 1);
+            int p = pos0 + d - kWin;
+            p = p < 0 ? 0 : (p > N - 1 ? N - 1 : p);
+            win[j][d] = bt[(size_t)lines[i - j] * N + p];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int pos = pos0;
+            for (int j = 0; j < cnt; ++j) {
+                int b = pos + (int)win[j][pos - pos0 + kWin] - 1;
+                b = b < 0 ? 0 : (b > N - 1 ? N - 1 : b);
+                seam[lines[i - j - 1]] = b;
+                pos = b;
+            }
+            s_pos = pos;
+        }
+        __syncthreads();
+    }
+}
+
+// seam mask inside the overlap (1 on image 1's side of the seam)
+__global__ __launch_bounds__(kCT) void k_cmp_seam_mask(const uint8_t* __restrict__ m1, const uint8_t* __restrict__ m2,
+                                                       const int32_t* __restrict__ seam, int cw,
+                                                       const CmpStats* __restrict__ st, float* __restrict__ seam0) {
+    const int u = blockIdx.x * kCT + threadIdx.x, v = blockIdx.y;
+    if (u >= cw || !st->has_overlap) return;
+    const size_t o = (size_t)v * cw + u;
+    if (!(m1[o] && m2[o])) return;
+    const bool vert = st->vertical;
+    const int l = vert ? v : u, n = vert ? u : v;
+    seam0[o] = ((n <= seam[l]) == (st->img1_first != 0)) ? 1.0f : 0.0f;
+}
+
+// 25-tap Gaussian of the seam mask: rows into tmp, then columns into the
+// mask plane (inside the overlap; the unblurred mask elsewhere)
+__global__ __launch_bounds__(kCT) void k_cmp_blur_rows(const float* __restrict__ in, int cw, W25 w,
+                                                       const CmpStats* __restrict__ st, float* __restrict__ out) {
+    const int u = blockIdx.x * kCT + threadIdx.x, v = blockIdx.y;
+    if (u >= cw || !st->has_overlap) return;
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 25; ++k) acc = acc + w.w[k] * in[(size_t)v * cw + refl_101(u + k - 12, cw)];
+    out[(size_t)v * cw + u] = acc;
+}
+
+__global__ __launch_bounds__(kCT) void k_cmp_blur_cols(const float* __restrict__ tmp, const float* __restrict__ seam0,
+                                                       const uint8_t* __restrict__ m1, const uint8_t* __restrict__ m2,
+                                                       int cw, int ch, W25 w, const CmpStats* __restrict__ st,
+                                                       float* __restrict__ mask) {
+    const int u = blockIdx.x * kCT + threadIdx.x, v = blockIdx.y;
+    if (u >= cw) return;
+    const size_t o = (size_t)v * cw + u;
+    float r = seam0[o];
+    if (st->has_overlap && m1[o] && m2[o]) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 25; ++k) acc = acc + w.w[k] * tmp[(size_t)refl_101(v + k - 12, ch) * cw + u];
+        r = acc;
+    }
+    mask[o] = r;
+}
+
+// pyramid down, rows at the kept even rows: grid (ceil(W / 256), Ho, planes)
+__global__ __launch_bounds__(kCT) void k_pyr_down_rows(const float* __restrict__ in, int H, int W, int Ho, W9 w,
+                                                       float* __restrict__ tmp) {
+    const int x = blockIdx.x * kCT + threadIdx.x, yo = blockIdx.y, q = blockIdx.z;
+    if (x >= W) return;
+    const float* src = in + (size_t)q * H * W;
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc = fmaf(src[(size_t)refl_half(2 * yo + k - 4, H) * W + x], w.w[k], acc);
+    tmp[(size_t)q * Ho * W + (size_t)yo * W + x] = acc;
+}
+
+__global__ __launch_bounds__(kCT) void k_pyr_down_cols(const float* __restrict__ tmp, int W, int Ho, int Wo, W9 w,
+                                                       float* __restrict__ out) {
+    const int xo = blockIdx.x * kCT + threadIdx.x, yo = blockIdx.y, q = blockIdx.z;
+    if (xo >= Wo) return;
+    const float* src = tmp + (size_t)q * Ho * W + (size_t)yo * W;
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc = fmaf(src[refl_half(2 * xo + k - 4, W)], w.w[k], acc);
+    out[(size_t)q * Ho * Wo + (size_t)yo * Wo + xo] = acc;
+}
+
+__device__ __forceinline__ void zoom_coord(int o, int n_in, int n_out, int& i0, int& i1, double& w0, double& w1) {
+    double c = n_out > 1 ? (double)o * ((double)(n_in - 1) / (double)(n_out - 1)) : 0.0;
+    if (c > (double)(n_in - 1)) c = (double)(n_in - 1);
+    const double f = floor(c);
+    i0 = (int)f;
+    i1 = i0 + 1 < n_in ? i0 + 1 : n_in - 1;
+    w1 = c - f;
+    w0 = 1.0 - w1;
+}
+
+__device__ __forceinline__ float zoom_at(const float* __restrict__ in, int w, int y0, int y1, int x0, int x1,
+                                         double wy0, double wy1, double wx0, double wx1) {
+    double acc = (double)in[(size_t)y0 * w + x0] * (wy0 * wx0);
+    acc = acc + (double)in[(size_t)y0 * w + x1] * (wy0 * wx1);
+    acc = acc + (double)in[(size_t)y1 * w + x0] * (wy1 * wx0);
+    acc = acc + (double)in[(size_t)y1 * w + x1] * (wy1 * wx1);
+    return (float)acc;
+}
+
+// coarsest level: res = m g1 + (1 - m) g2, grid (ceil(W / 256), H, 3)
+__global__ __launch_bounds__(kCT) void k_pyr_top(const float* __restrict__ lvl, int H, int W, float* __restrict__ res) {
+    const int x = blockIdx.x * kCT + threadIdx.x, y = blockIdx.y, c = blockIdx.z;
+    if (x >= W) return;
+    const size_t n = (size_t)H * W, o = (size_t)y * W + x;
+    const float m = lvl[6 * n + o];
+    res[c * n + o] = m * lvl[c * n + o] + (1.0f - m) * lvl[(c + 3) * n + o];
+}
+
+// level k from level k + 1: res_k = up(res_k+1) + m (g1 - up g1') + (1 - m) (g2 - up g2')
+__global__ __launch_bounds__(kCT) void k_pyr_up_blend(const float* __restrict__ lvk, int H, int W,
+                                                      const float* __restrict__ lvn, int h, int w,
+                                                      const float* __restrict__ resn, float* __restrict__ res) {
+    const int x = blockIdx.x * kCT + threadIdx.x, y = blockIdx.y, c = blockIdx.z;
+    if (x >= W) return;
+    int y0, y1, x0, x1;
+    double wy0, wy1, wx0, wx1;
+    zoom_coord(y, h, H, y0, y1, wy0, wy1);
+    zoom_coord(x, w, W, x0, x1, wx0, wx1);
+    const size_t n = (size_t)H * W, nn = (size_t)h * w, o = (size_t)y * W + x;
+    const float u1 = zoom_at(lvn + c * nn, w, y0, y1, x0, x1, wy0, wy1, wx0, wx1);
+    const float u2 = zoom_at(lvn + (c + 3) * nn, w, y0, y1, x0, x1, wy0, wy1, wx0, wx1);
+    const float ur = zoom_at(resn + c * nn, w, y0, y1, x0, x1, wy0, wy1, wx0, wx1);
+    const float m = lvk[6 * n + o];
+    const float l1 = lvk[c * n + o] - u1, l2 = lvk[(c + 3) * n + o] - u2;
+    const float lb = m * l1 + (1.0f - m) * l2;
+    res[c * n + o] = ur + lb;
+}
+
+__global__ __launch_bounds__(kCT) void k_cmp_final(const float* __restrict__ res, const uint8_t* __restrict__ m1,
+                                                   const uint8_t* __restrict__ m2, size_t P, uint8_t* __restrict__ pano,
+                                                   uint8_t* __restrict__ valid, const float* __restrict__ mask,
+                                                   float* __restrict__ seam_out, CmpStats* __restrict__ st) {
+    const size_t p = (size_t)blockIdx.x * kCT + threadIdx.x;
+    if (p == 0) st->rmse = st->n_overlap ? sqrt((double)st->sq_diff / (double)(3 * st->n_overlap)) : __builtin_nan("");
+    if (p >= P) return;
+    const bool vl = m1[p] || m2[p];
+    valid[p] = vl;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float v = res[(size_t)c * P + p];
+        v = v < 0.0f ? 0.0f : (v > 255.0f ? 255.0f : v);
+        pano[p * 3 + c] = vl ? (uint8_t)v : 0;
+    }
+    if (seam_out) seam_out[p] = mask[p];
+}
+
+__global__ void k_cmp_zero(unsigned long long* __restrict__ p, int n) {
+    const int i = blockIdx.x * kCT + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+struct CmpPlan {
+    int L, hs[kMaxLevels], ws[kMaxLevels];
+    size_t P, o_wu, o_c2, o_m1, o_m2, o_rowf, o_colf, o_st, o_costc, o_avg, o_dx, o_dy, o_cost, o_seam0, o_tmp,
+        o_lvl[kMaxLevels], o_res[kMaxLevels], o_dpa, o_dpb, o_bt, o_seam, o_lines, o_rowacc, zero_bytes, total;
+};
+
+CmpPlan cmp_plan(int cw, int ch, int n_levels) {
+    CmpPlan Pl;
+    size_t off = 0;
+    auto take = [&](size_t b) { size_t o = off; off += (b + 255) & ~(size_t)255; return o; };
+    const size_t P = (size_t)cw * ch;
+    Pl.P = P;
+    Pl.L = n_levels;
+    Pl.hs[0] = ch; Pl.ws[0] = cw;
+    for (int k = 1; k < n_levels; ++k) { Pl.hs[k] = (Pl.hs[k - 1] + 1) / 2; Pl.ws[k] = (Pl.ws[k - 1] + 1) / 2; }
+    Pl.o_wu = take(P * 3); Pl.o_c2 = take(P * 3); Pl.o_m1 = take(P); Pl.o_m2 = take(P);
+    // zeroed at the start of every call: stats, row / column flags, row partials
+    Pl.o_st = take(sizeof(CmpStats));
+    Pl.o_rowf = take((size_t)ch * 4); Pl.o_colf = take((size_t)cw * 4);
+    Pl.o_rowacc = take((size_t)ch * 8 * 8);
+    Pl.zero_bytes = off - Pl.o_st;
+    Pl.o_costc = take(P * 4); Pl.o_avg = take(P * 4); Pl.o_dx = take(P * 4); Pl.o_dy = take(P * 4);
+    Pl.o_cost = take(P * 4); Pl.o_seam0 = take(P * 4);
+    Pl.o_tmp = take(std::max(P, (size_t)7 * ((ch + 1) / 2) * cw) * 4);
+    for (int k = 0; k < n_levels; ++k) Pl.o_lvl[k] = take((size_t)7 * Pl.hs[k] * Pl.ws[k] * 4);
+    for (int k = 0; k < n_levels; ++k) Pl.o_res[k] = take((size_t)3 * Pl.hs[k] * Pl.ws[k] * 4);
+    const int N = std::max(cw, ch);
+    Pl.o_dpa = take((size_t)N * 8); Pl.o_dpb = take((size_t)N * 8);
+    Pl.o_bt = take(P);
+    Pl.o_seam = take((size_t)N * 4);
+    Pl.o_lines = take((size_t)N * 4);
+    Pl.total = off;
+    return Pl;
+}
+
+void inv3_adj(const double* a, double* o) {
+    double det = (a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6])) + a[2] * (a[3] * a[7] - a[4] * a[6]);
+    if (!(std::fabs(det) > 1e-12)) det = 1.0;
+    o[0] = (a[4] * a[8] - a[5] * a[7]) / det; o[1] = (a[2] * a[7] - a[1] * a[8]) / det; o[2] = (a[1] * a[5] - a[2] * a[4]) / det;
+    o[3] = (a[5] * a[6] - a[3] * a[8]) / det; o[4] = (a[0] * a[8] - a[2] * a[6]) / det; o[5] = (a[2] * a[3] - a[0] * a[5]) / det;
+    o[6] = (a[3] * a[7] - a[4] * a[6]) / det; o[7] = (a[1] * a[6] - a[0] * a[7]) / det; o[8] = (a[0] * a[4] - a[1] * a[3]) / det;
+}
+
+}  // namespace
+
+size_t composite_workspace(int cw, int ch, int n_levels) { return cmp_plan(cw, ch, n_levels).total; }
+
+int composite(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h2, int w2, const double* H, int cw,
+              int ch, double ox, double oy, int n_levels, void* ws_, size_t ws_bytes, uint8_t* pano, uint8_t* valid,
+              float* seam_out, void* stats_out, hipStream_t s) {
+    if (!img1 || !img2 || !H || !pano || !valid || !stats_out || !ws_) {
+        set_error("msift_composite_multiband: null buffer");
+        return MSIFT_EINVAL;
+    }
+    if (h1 < 2 || w1 < 2 || h2 < 1 || w2 < 1 || cw < 1 || ch < 1 || ch > 65535 || n_levels < 1 ||
+        n_levels > kMaxLevels || cw > 65535) {
+        set_error("msift_composite_multiband: bad arguments (image 1 at least 2x2, canvas sides <= 65535, "
+                  "1 <= n_levels <= 12)");
+        return MSIFT_EINVAL;
+    }
+    const CmpPlan Pl = cmp_plan(cw, ch, n_levels);
+    if (ws_bytes < Pl.total) { set_error("msift_composite_multiband: workspace too small"); return MSIFT_EWORKSPACE; }
+    if (((uintptr_t)ws_ & 255) != 0) { set_error("msift_composite_multiband: workspace must be 256-byte aligned"); return MSIFT_EINVAL; }
+    char* ws = (char*)ws_;
+    uint8_t* wu = (uint8_t*)(ws + Pl.o_wu);
+    uint8_t* c2 = (uint8_t*)(ws + Pl.o_c2);
+    uint8_t* m1 = (uint8_t*)(ws + Pl.o_m1);
+    uint8_t* m2 = (uint8_t*)(ws + Pl.o_m2);
+    uint32_t* rowf = (uint32_t*)(ws + Pl.o_rowf);
+    uint32_t* colf = (uint32_t*)(ws + Pl.o_colf);
+    CmpStats* st = (CmpStats*)(ws + Pl.o_st);
+    float* costc = (float*)(ws + Pl.o_costc);
+    float* avg = (float*)(ws + Pl.o_avg);
+    float* dx = (float*)(ws + Pl.o_dx);
+    float* dy = (float*)(ws + Pl.o_dy);
+    float* cost = (float*)(ws + Pl.o_cost);
+    float* seam0 = (float*)(ws + Pl.o_seam0);
+    float* tmp = (float*)(ws + Pl.o_tmp);
+    const size_t P = Pl.P;
+    double Hi[9];
+    inv3_adj(H, Hi);
+    Hf9 hf;
+    for (int i = 0; i < 9; ++i) hf.h[i] = (float)Hi[i];
+    // sigma-1 taps (SciPy's kernel: exp(-x^2 / 2) over numpy's pairwise sum)
+    W9 w9;
+    {
+        double phi[9];
+        for (int k = 0; k < 9; ++k) { const double x = (double)(k - 4); phi[k] = std::exp(-0.5 * (x * x)); }
+        const double sum = (((phi[0] + phi[1]) + (phi[2] + phi[3])) + ((phi[4] + phi[5]) + (phi[6] + phi[7]))) + phi[8];
+        for (int k = 0; k < 9; ++k) w9.w[k] = (float)(phi[k] / sum);
+    }
+    // cv2.getGaussianKernel(25, 3) in float32
+    W25 w25;
+    {
+        double sum = 0.0;
+        for (int i = 0; i < 25; ++i) {
+            const double x = (double)i - 12.0;
+            w25.w[i] = (float)std::exp((-0.5 / 9.0) * x * x);
+            sum += w25.w[i];
+        }
+        sum = 1.0 / sum;
+        for (int i = 0; i < 25; ++i) w25.w[i] = (float)(w25.w[i] * sum);
+    }
+    const dim3 g2d((cw + kCT - 1) / kCT, ch);
+    const unsigned gP = (unsigned)((P + kCT - 1) / kCT);
+    const int nz = (int)(Pl.zero_bytes / 8);
+    unsigned long long* rowacc = (unsigned long long*)(ws + Pl.o_rowacc);
+    hipLaunchKernelGGL(k_cmp_zero, dim3((nz + kCT - 1) / kCT), dim3(kCT), 0, s, (unsigned long long*)st, nz);
+    hipLaunchKernelGGL(k_cmp_warp, g2d, dim3(kCT), 0, s, img1, h1, w1, img2, h2, w2, hf, cw, (float)ox, (float)oy,
+                       (int)std::rint(ox), (int)std::rint(oy), wu, c2, m1, m2, rowacc, rowf);
+    hipLaunchKernelGGL(k_cmp_colflags, dim3((cw + kCT - 1) / kCT, (ch + 63) / 64), dim3(kCT), 0, s,
+                       (const uint8_t*)m1, (const uint8_t*)m2, cw, ch, colf);
+    hipLaunchKernelGGL(k_cmp_decide, dim3(1), dim3(kCT), 0, s, (const uint32_t*)rowf, ch, (const uint32_t*)colf, cw,
+                       (const unsigned long long*)rowacc, st);
+    float* lvl0 = (float*)(ws + Pl.o_lvl[0]);
+    hipLaunchKernelGGL(k_cmp_prepare, dim3(gP), dim3(kCT), 0, s, (const uint8_t*)wu, (const uint8_t*)c2,
+                       (const uint8_t*)m1, (const uint8_t*)m2, P, st, lvl0, costc, avg, seam0);
+    hipLaunchKernelGGL(k_cmp_sobel1, g2d, dim3(kCT), 0, s, (const float*)avg, cw, ch, dx, dy);
+    hipLaunchKernelGGL(k_cmp_sobel2, g2d, dim3(kCT), 0, s, (const float*)dx, (const float*)dy, (const float*)costc,
+                       (const uint8_t*)m1, (const uint8_t*)m2, cw, ch, (const CmpStats*)st, cost);
+    int32_t* seam = (int32_t*)(ws + Pl.o_seam);
+    hipLaunchKernelGGL(k_cmp_dp, dim3(1), dim3(1024), 0, s, (const float*)cost, (const uint32_t*)rowf,
+                       (const uint32_t*)colf, cw, ch, (const CmpStats*)st, (double*)(ws + Pl.o_dpa),
+                       (double*)(ws + Pl.o_dpb), (uint8_t*)(ws + Pl.o_bt), (int32_t*)(ws + Pl.o_lines), seam);
+    hipLaunchKernelGGL(k_cmp_seam_mask, g2d, dim3(kCT), 0, s, (const uint8_t*)m1, (const uint8_t*)m2,
+                       (const int32_t*)seam, cw, (const CmpStats*)st, seam0);
+    hipLaunchKernelGGL(k_cmp_blur_rows, g2d, dim3(kCT), 0, s, (const float*)seam0, cw, w25, (const CmpStats*)st, tmp);
+    hipLaunchKernelGGL(k_cmp_blur_cols, g2d, dim3(kCT), 0, s, (const float*)tmp, (const float*)seam0,
+                       (const uint8_t*)m1, (const uint8_t*)m2, cw, ch, w25, (const CmpStats*)st, lvl0 + 6 * P);
+    for (int k = 1; k < n_levels; ++k) {
+        const int Hk = Pl.hs[k - 1], Wk = Pl.ws[k - 1], Ho = Pl.hs[k], Wo = Pl.ws[k];
+        hipLaunchKernelGGL(k_pyr_down_rows, dim3((Wk + kCT - 1) / kCT, Ho, 7), dim3(kCT), 0, s,
+                           (const float*)(ws + Pl.o_lvl[k - 1]), Hk, Wk, Ho, w9, tmp);
+        hipLaunchKernelGGL(k_pyr_down_cols, dim3((Wo + kCT - 1) / kCT, Ho, 7), dim3(kCT), 0, s, (const float*)tmp, Wk,
+                           Ho, Wo, w9, (float*)(ws + Pl.o_lvl[k]));
+    }
+    const int kL = n_levels - 1;
+    hipLaunchKernelGGL(k_pyr_top, dim3((Pl.ws[kL] + kCT - 1) / kCT, Pl.hs[kL], 3), dim3(kCT), 0, s,
+                       (const float*)(ws + Pl.o_lvl[kL]), Pl.hs[kL], Pl.ws[kL], (float*)(ws + Pl.o_res[kL]));
+    for (int k = n_levels - 2; k >= 0; --k)
+        hipLaunchKernelGGL(k_pyr_up_blend, dim3((Pl.ws[k] + kCT - 1) / kCT, Pl.hs[k], 3), dim3(kCT), 0, s,
+                           (const float*)(ws + Pl.o_lvl[k]), Pl.hs[k], Pl.ws[k], (const float*)(ws + Pl.o_lvl[k + 1]),
+                           Pl.hs[k + 1], Pl.ws[k + 1], (const float*)(ws + Pl.o_res[k + 1]),
+                           (float*)(ws + Pl.o_res[k]));
+    hipLaunchKernelGGL(k_cmp_final, dim3(gP), dim3(kCT), 0, s, (const float*)(ws + Pl.o_res[0]), (const uint8_t*)m1,
+                       (const uint8_t*)m2, P, pano, valid, (const float*)(lvl0 + 6 * P), seam_out, st);
+    MSIFT_LAUNCH_CHECK();
+    MSIFT_HIP(hipMemcpyAsync(stats_out, st, sizeof(CmpStats), hipMemcpyDeviceToDevice, s));
+    return MSIFT_OK;
+}
+
+size_t composite_stats_bytes() { return sizeof(CmpStats); }
+
+}  // namespace msift

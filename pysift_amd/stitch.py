@@ -72,7 +72,7 @@ class GPUPyStitch:
                  edge_thresh=10.0, double_image=True, max_dimension=0, denoise=False, sharpen=False,
                  clahe_clip=2.0, clahe_tile=8, output_max_dim=0, post_sharpen=False, output_format="png",
                  output_quality=90, *, match_method="hilo", ransac_seed=42, save_output=True, verbose=True,
-                 device=None):
+                 blend="feather", device=None):
         if matcher != "ratio":
             raise NotImplementedError("only the ratio-test matcher is part of this build")
         if max_dimension or denoise or sharpen or output_max_dim or post_sharpen:
@@ -80,7 +80,12 @@ class GPUPyStitch:
         if ransac_method not in ("magsac", "classic"):
             raise ValueError(f"ransac_method must be 'magsac' or 'classic' (got {ransac_method!r})")
         self.n_depth_layers = n_depth_layers   # MiDaS is unavailable: global homography (core.py:3788-3792)
-        self.n_blend_levels = n_blend_levels   # multiband blend replaced by the feather blend
+        if blend not in ("feather", "multiband"):
+            raise ValueError(f"blend must be 'feather' or 'multiband' (got {blend!r})")
+        if blend == "multiband" and color_match != "gain":
+            raise NotImplementedError("multiband composition supports color_match='gain' only")
+        self.blend = blend                     # "multiband": the reference's gain + seam + pyramid blend
+        self.n_blend_levels = int(n_blend_levels)
         self.ransac_iters = int(ransac_iters)
         self.inlier_thresh = float(inlier_thresh)
         self.ransac_method = ransac_method
@@ -224,11 +229,16 @@ class GPUPyStitch:
         h1, w1 = b1.shape[:2]
         h2, w2 = b2.shape[:2]
         cw, ch, ox, oy = compute_canvas_size(h1, w1, h2, w2, H)
-        pano_d, valid_d = self.warp_feather(b1, b2, H, cw, ch, ox, oy)
+        seam_rmse = float("nan")
+        if self.blend == "multiband":
+            pano_d, valid_d, _, stats = self.composite_multiband(b1, b2, H, cw, ch, ox, oy)
+            seam_rmse = stats["rmse"]
+        else:
+            pano_d, valid_d = self.warp_feather(b1, b2, H, cw, ch, ox, oy)
         pano = pano_d.cpu().numpy()
         both = valid_d.cpu().numpy().astype(bool)
         timings["st5"] = time.perf_counter() - t0
-        metrics.update(seam_rmse=float("nan"), coverage=float(both.sum()) / float(ch * cw) * 100.0,
+        metrics.update(seam_rmse=seam_rmse, coverage=float(both.sum()) / float(ch * cw) * 100.0,
                        canvas_w=cw, canvas_h=ch)
         return autocrop(pano, both), timings, metrics
 
@@ -243,6 +253,35 @@ class GPUPyStitch:
                                            float(oy), _lib.ptr(pano), _lib.ptr(valid), _lib.stream_ptr(b1.device))
         _lib.check(rc, "warp_feather")
         return pano, valid
+
+    def composite_multiband(self, b1, b2, H, cw, ch, ox, oy, n_levels=None, seam_mask=False):
+        """_blend_pair with color_match 'gain' plus the warp and paste of
+        _stitch_pair (core.py:3686-3714, 3783-3802) on the device: returns
+        (pano (ch, cw, 3) u8, valid (ch, cw) u8, seam mask float32 or None,
+        stats dict with gain, seam orientation and seam RMSE)."""
+        torch = _torch()
+        import ctypes
+        L = n_levels if n_levels is not None else self.n_blend_levels
+        dev = b1.device
+        n = ctypes.c_size_t()
+        _lib.check(_lib.lib().msift_composite_workspace_bytes(cw, ch, L, ctypes.byref(n)), "composite")
+        ws = torch.empty(n.value, dtype=torch.uint8, device=dev)
+        pano = torch.empty((ch, cw, 3), dtype=torch.uint8, device=dev)
+        valid = torch.empty((ch, cw), dtype=torch.uint8, device=dev)
+        sm = torch.empty((ch, cw), dtype=torch.float32, device=dev) if seam_mask else None
+        st = torch.zeros(13, dtype=torch.int64, device=dev)   # msift_composite_stats, 104 bytes
+        Hc = np.ascontiguousarray(H, np.float64)
+        _lib.check(_lib.lib().msift_composite_multiband(
+            _lib.ptr(b1), b1.shape[0], b1.shape[1], _lib.ptr(b2), b2.shape[0], b2.shape[1],
+            Hc.ctypes.data_as(ctypes.c_void_p), cw, ch, float(ox), float(oy), L, _lib.ptr(ws), ws.numel(),
+            _lib.ptr(pano), _lib.ptr(valid), _lib.ptr(sm), _lib.ptr(st), _lib.stream_ptr(dev)), "composite")
+        raw = st.cpu().numpy().tobytes()
+        gain = np.frombuffer(raw[72:84], np.float32).copy()
+        vertical, first, has = np.frombuffer(raw[84:96], np.int32)
+        rmse = float(np.frombuffer(raw[96:104], np.float64)[0])
+        stats = {"gain": gain, "vertical": int(vertical) if has else -1, "img1_first": int(first),
+                 "n_overlap": int(np.frombuffer(raw[48:56], np.uint64)[0]), "rmse": rmse}
+        return pano, valid, sm, stats
 
     def stitch(self, *args):
         """Stitch two images (BGR uint8); returns the uint8 (H, W, 3) panorama."""

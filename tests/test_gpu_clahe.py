@@ -19,7 +19,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "imc
 def _apply(img, clip, tiles):
     import torch
     from pysift_amd.clahe import createCLAHE
-    out = createCLAHE(clip, tiles).apply(torch.from_numpy(img).cuda())
+    out = createCLAHE(clip, tiles).apply(torch.from_numpy(np.array(img)).cuda())
     return out.cpu().numpy()
 
 

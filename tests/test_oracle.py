@@ -675,6 +675,127 @@ def test_match_sim16_permutation_and_symmetry():
     assert len(p1) > 80
 
 
+# ---- compositing: gain + DP seam + multiband (core.py:3211-3239, 3287-3553, 3686-3714) ----
+
+def test_grid_warp_against_torch_grid_sample():
+    # the reference's _warp_image computed with torch on the CPU
+    import torch
+    rng = np.random.default_rng(0)
+    img = rng.integers(0, 256, (120, 160, 3)).astype(np.uint8)
+    H = np.array([[0.9, 0.1, 60.3], [-0.07, 1.05, -20.6], [2e-4, -1e-4, 1.0]])
+    cw, ch, ox, oy = O.canvas(120, 160, 100, 150, H)
+    u = torch.arange(cw, dtype=torch.float32); v = torch.arange(ch, dtype=torch.float32)
+    uu, vv = torch.meshgrid(u, v, indexing="xy")
+    flat = torch.stack([uu - ox, vv - oy, torch.ones_like(uu)], 0).reshape(3, -1)
+    src = torch.tensor(np.linalg.inv(H), dtype=torch.float32) @ flat
+    sx = src[0] / src[2].clamp(min=1e-10); sy = src[1] / src[2].clamp(min=1e-10)
+    gx = (2.0 * sx / 159) - 1.0; gy = (2.0 * sy / 119) - 1.0
+    grid = torch.stack([gx, gy], -1).reshape(1, ch, cw, 2)
+    it = torch.tensor(img.astype(np.float32)).permute(2, 0, 1).unsqueeze(0)
+    out = torch.nn.functional.grid_sample(it, grid, mode="bilinear", padding_mode="zeros", align_corners=True)
+    ref = out.squeeze(0).permute(1, 2, 0).numpy().clip(0, 255).astype(np.uint8)
+    mref = ((gx >= -1) & (gx <= 1) & (gy >= -1) & (gy <= 1)).reshape(ch, cw).numpy()
+    w, m = O.grid_warp(img, H, cw, ch, ox, oy)
+    assert (m == mref).all()
+    assert np.mean(w == ref) > 0.9995 and np.abs(w.astype(int) - ref).max() <= 1
+
+
+def test_pyramid_filters_against_scipy():
+    rng = np.random.default_rng(1)
+    x = (rng.random((123, 217)) * 255).astype(np.float32)
+    # sigma-1 blur then [::2, ::2] (float32 accumulation vs SciPy's double)
+    ref = nd.gaussian_filter(x, 1.0, mode="reflect", truncate=4.0)[::2, ::2]
+    assert np.abs(O.gauss_down(x) - ref).max() < 1e-4
+    # bilinear zoom with align-corners coordinates: exactly SciPy's order-1 zoom
+    for (h, w, th, tw) in [(62, 109, 123, 217), (31, 55, 62, 109), (1, 3, 2, 5)]:
+        y = (rng.random((h, w)) * 255).astype(np.float32)
+        np.testing.assert_array_equal(O.zoom_up(y, th, tw), nd.zoom(y, (th / h, tw / w), order=1))
+    # Sobel: exactly SciPy's
+    for ax in (0, 1):
+        np.testing.assert_array_equal(O.sobel(x, ax), nd.sobel(x, axis=ax, mode="reflect"))
+    # cv2.GaussianBlur(sigma 3): 25 taps, BORDER_REFLECT_101 (SciPy's "mirror")
+    m = (rng.random((90, 140)) > 0.5).astype(np.float32)
+    assert np.abs(O.cv_blur25(m) - nd.gaussian_filter(m, 3.0, mode="mirror", truncate=4.0)).max() < 1e-6
+    w = O.gauss_weights_cv()
+    assert abs(float(w.astype(np.float64).sum()) - 1.0) < 1e-6 and w[12] == w.max()
+
+
+def _brute_seam(cost, ov):
+    # all one-column-per-row paths over the rows that hold overlap
+    import itertools
+    rows = [r for r in range(cost.shape[0]) if ov[r].any()]
+    W = cost.shape[1]
+    best = None
+    for start in range(W):
+        for steps in itertools.product((-1, 0, 1), repeat=len(rows) - 1):
+            cols = [start]
+            for st in steps:
+                cols.append(cols[-1] + st)
+            if min(cols) < 0 or max(cols) >= W:
+                continue
+            if not all(ov[r, c] for r, c in zip(rows, cols)):
+                continue
+            t = sum(float(cost[r, c]) for r, c in zip(rows, cols))
+            if best is None or t < best:
+                best = t
+    return rows, best
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_dp_seam_against_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    H, W = 5, 5
+    cost = rng.integers(0, 9, (H, W)).astype(np.float32)
+    ov = np.ones((H, W), bool)
+    if seed % 2:
+        ov[2] = False          # a row without overlap is skipped
+        ov[:, 0] = seed == 3   # a column outside the overlap
+        cost[~ov] = 1e9
+    rows, best = _brute_seam(cost, ov)
+    seam = O.dp_seam(cost, ov, True)
+    assert sum(float(cost[r, seam[r]]) for r in rows) == best
+    # the horizontal seam is the vertical one of the transpose
+    np.testing.assert_array_equal(O.dp_seam(np.ascontiguousarray(cost.T), np.ascontiguousarray(ov.T), False), seam)
+
+
+def test_composite_identical_views_and_constant_gain():
+    rng = np.random.default_rng(2)
+    img = np.clip(nd.gaussian_filter(rng.random((80, 100, 3)) * 255, (3, 3, 0)), 0, 255).astype(np.uint8)
+    # the same view over itself.  The identity warp's normalise/unnormalise
+    # round trip lands some samples just below an integer, which truncates
+    # (as the reference's grid_sample does), so the gain is the closed form
+    # of exact sums over the warped image, and the blend stays within 2 LSB
+    pano, valid, seam, gain, rmse, vert = O.composite(img, img, np.eye(3), 100, 80, 0.0, 0.0)
+    w, m = O.grid_warp(img, np.eye(3), 100, 80, 0.0, 0.0)
+    assert valid.all() and m.all()
+    n = float(w.shape[0] * w.shape[1])
+    m1 = (w.reshape(-1, 3).astype(np.int64).sum(0) / n).astype(np.float32)
+    m2 = (img.reshape(-1, 3).astype(np.int64).sum(0) / n).astype(np.float32)
+    g = np.clip((m2 + np.float32(1e-3)) / (m1 + np.float32(1e-3)), np.float32(0.5), np.float32(2.0))
+    np.testing.assert_array_equal(gain, g)
+    wg = np.clip(w.astype(np.float32) * g, 0, 255).astype(np.uint8).astype(np.int64)
+    assert abs(rmse - np.sqrt(((wg - img) ** 2).sum() / (3 * n))) < 1e-12
+    assert np.abs(pano.astype(int) - img).max() <= 2
+    # two flat views shifted by 40 px: the gain is (c2 + 1e-3) / (m1 + 1e-3)
+    # with m1 the exact overlap mean of the warped first view; a tall overlap
+    # gives a vertical seam and the blend stays within 2 LSB of the second colour
+    a = np.empty((60, 80, 3), np.uint8); a[:] = (100, 120, 60)
+    b = np.empty((60, 80, 3), np.uint8); b[:] = (110, 126, 66)
+    Ht = np.array([[1.0, 0, 40.0], [0, 1.0, 0], [0, 0, 1.0]])
+    cw, ch, ox, oy = O.canvas(60, 80, 60, 80, Ht)
+    pano, valid, seam, gain, rmse, vert = O.composite(a, b, Ht, cw, ch, ox, oy)
+    w, m1 = O.grid_warp(a, Ht, cw, ch, ox, oy)
+    m2 = np.zeros((ch, cw), bool); m2[:60, :80] = True
+    ov = m1 & m2
+    mean1 = (w[ov].astype(np.int64).sum(0) / ov.sum()).astype(np.float32)
+    exp_gain = (np.float32([110, 126, 66]) + np.float32(1e-3)) / (mean1 + np.float32(1e-3))
+    np.testing.assert_array_equal(gain, exp_gain)
+    assert vert == 1 and valid.sum() == (m1 | m2).sum()
+    assert (np.abs(pano[valid].astype(int) - np.array([110, 126, 66])) <= 2).all()
+    wg = np.clip(w[ov].astype(np.float32) * gain, 0, 255).astype(np.uint8).astype(np.int64)
+    assert abs(rmse - np.sqrt(((wg - np.array([110, 126, 66])) ** 2).sum() / (3 * ov.sum()))) < 1e-12
+
+
 # ---- CLAHE (core.py:2562-2565, cv2.createCLAHE(...).apply) ----
 
 def test_clahe_constant_image_known_answers():
