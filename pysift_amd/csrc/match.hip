@@ -6,7 +6,10 @@
 //   method 0 (high/low split): a = 128 h + l, h in {0,1}, l in [0,127];
 //     a.b = l.l + 128 (h_a.l_b + l_a.h_b + 2 h_a.(64 h_b))  (two accumulators,
 //     every operand byte in [0,127])
-//   method 1 (shift): a' = a - 128 is int8 and d = |a'|^2 + |b'|^2 - 2 a'.b'.
+//   method 1 (shift): a' = a - 128 is int8 and d = |a'|^2 + |b'|^2 - 2 a'.b';
+//     the MFMA takes ~a' = -a' - 1 (also int8) and b', returning
+//     dn = -a'.b' - sum b', so d = |a'|^2 + (|b'|^2 + 2 sum b') + 2 dn and
+//     each key is two integer instructions away from the accumulator.
 // A workgroup (8 waves) owns 256 rows x a 512-column chunk; each wave
 // computes 32x64 tiles.  Top-2 per row and per column is tracked with 32-bit keys
 // (d << bits | local index), which are unique, so the result does not depend
@@ -27,10 +30,12 @@ typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 constexpr int kRowBlk = 256;   // rows per workgroup
 constexpr int kColChunk = 512; // columns per workgroup (9-bit local index)
 
+// top-2 of unique 32-bit keys, k1 <= k2 always: the new second is the median
+// of (k1, k2, k) (one v_med3_u32) and the new first min(k1, k)
 __device__ __forceinline__ void top2_insert(uint32_t& k1, uint32_t& k2, uint32_t k) {
-    uint32_t t = max(k1, k);
+    const uint32_t m = max(min(k1, k2), min(max(k1, k2), k));
     k1 = min(k1, k);
-    k2 = min(k2, t);
+    k2 = m;
 }
 
 __device__ __forceinline__ void top2_merge(uint32_t& k1, uint32_t& k2, uint32_t o1, uint32_t o2) {
@@ -45,15 +50,18 @@ __device__ __forceinline__ void top2_insert64(uint64_t& k1, uint64_t& k2, uint64
     k2 = k2 < t ? k2 : t;
 }
 
-// squared norms (method 0: of a, method 1: of a - 128), exact int32
+// squared norms, exact int32: method 0 |a|^2, method 1 |a - 128|^2, mode 2
+// |b'|^2 + 2 sum b' with b' = b - 128 (the column term of the shift
+// method's keys, whose A operand is fed negated)
 __global__ __launch_bounds__(256) void k_norms(const uint8_t* __restrict__ a, int n, int method,
                                                 int* __restrict__ out) {
     int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     int lane = threadIdx.x & 63;
     if (i >= n) return;
     int v0 = a[(size_t)i * 128 + 2 * lane], v1 = a[(size_t)i * 128 + 2 * lane + 1];
-    if (method == 1) { v0 -= 128; v1 -= 128; }
+    if (method != 0) { v0 -= 128; v1 -= 128; }
     int s = v0 * v0 + v1 * v1;
+    if (method == 2) s += 2 * (v0 + v1);
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m);
     if (lane == 0) out[i] = s;
@@ -72,7 +80,10 @@ constexpr int kBTile = 64 * kBStride;             // one 64-column step
 struct MatchSmem {
     uint8_t bt[2][kBTile];                        // staged b columns, double buffer
     uint32_t colk[8][kColChunk][2];               // per-wave column top-2 for the chunk
-    int s_na[kRowBlk];
+    int s_na[kRowBlk];                            // method 0: |a|^2; method 1: |a'|^2 << 9 (row keys)
+    uint32_t s_nac[kRowBlk];                      // method 1: (|a'|^2 << 8) | row (column keys)
+    int s_nb[kColChunk];                          // the chunk's column norms (no global load in
+                                                  // the epilogue, which would wait on the prefetch)
 };
 
 template <int METHOD>
@@ -83,11 +94,22 @@ __device__ __forceinline__ void match_tile(const uint8_t* __restrict__ A, int N,
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int l32 = lane & 31, hi = lane >> 5;
     const int row0 = rb * kRowBlk + w * 32;     // this wave's 32 rows
+    {
+        const int j = cc * kColChunk + threadIdx.x;
+        sm.s_nb[threadIdx.x] = j < M ? nb[j] : 0;
+    }
     if (threadIdx.x < kRowBlk) {
         int i = rb * kRowBlk + threadIdx.x;
-        sm.s_na[threadIdx.x] = i < N ? na[i] : 0;
+        const int v = i < N ? na[i] : 0;
+        if (METHOD == 1) {
+            sm.s_na[threadIdx.x] = (int)((uint32_t)v << 9);
+            sm.s_nac[threadIdx.x] = ((uint32_t)v << 8) | (uint32_t)threadIdx.x;
+        } else {
+            sm.s_na[threadIdx.x] = v;
+        }
     }
-    // A fragment: lane holds row (row0 + l32), bytes [32 ks + 16 hi, +16) for ks = 0..3
+    // A fragment: lane holds row (row0 + l32), bytes [32 ks + 16 hi, +16) for ks = 0..3;
+    // method 1 feeds ~(a - 128) = -(a - 128) - 1, so the MFMA returns -a'.b' - sum b'
     v4i al[4], ah[4];
     {
         int i = row0 + l32;
@@ -97,7 +119,7 @@ __device__ __forceinline__ void match_tile(const uint8_t* __restrict__ A, int N,
         for (int ks = 0; ks < 4; ++ks) {
             v4i v = src[2 * ks + hi];
             if (METHOD == 1) {
-                al[ks] = v ^ (v4i)(int)0x80808080;
+                al[ks] = v ^ (v4i)(int)0x7f7f7f7f;
             } else {
                 al[ks] = v & (v4i)0x7f7f7f7f;
                 ah[ks] = (v >> 7) & (v4i)0x01010101;
@@ -116,17 +138,21 @@ __device__ __forceinline__ void match_tile(const uint8_t* __restrict__ A, int N,
         return *reinterpret_cast<const v4i*>(B + (size_t)j * 128 + 16 * sq);
     };
     auto lstore = [&](int buf, v4i v) {
+        if (METHOD == 1) v = v ^ (v4i)(int)0x80808080;   // staged as b' = b - 128
         *reinterpret_cast<v4i*>(&sm.bt[buf][sc * kBStride + 16 * sq]) = v;
     };
     lstore(0, gload(0));
     __syncthreads();
-    for (int ct = 0; ct < n_ct; ++ct) {
-        const int col0 = cc * kColChunk + ct * 64;
-        const bool more = ct + 1 < n_ct;
-        v4i nxt;
-        if (more) nxt = gload(ct + 1);
-        const uint8_t* bt = sm.bt[ct & 1];
-        v16i acc[2], accx[2];
+    // the lane's 16 rows keep their norm terms in registers for the whole chunk
+    uint32_t rn[16], rc[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int il = 32 * w + 8 * (r >> 2) + 4 * hi + (r & 3);
+        rn[r] = (uint32_t)sm.s_na[il];
+        rc[r] = METHOD == 1 ? sm.s_nac[il] : 0u;
+    }
+    auto mfma_step = [&](int step, v16i* acc, v16i* accx) {
+        const uint8_t* bt = sm.bt[step & 1];
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
             const uint8_t* src = bt + (32 * b + l32) * kBStride + 16 * hi;
@@ -136,7 +162,7 @@ __device__ __forceinline__ void match_tile(const uint8_t* __restrict__ A, int N,
             for (int ks = 0; ks < 4; ++ks) {
                 v4i v = *reinterpret_cast<const v4i*>(src + 32 * ks);
                 if (METHOD == 1) {
-                    acc[b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(al[ks], v ^ (v4i)(int)0x80808080, acc[b], 0, 0, 0);
+                    acc[b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(al[ks], v, acc[b], 0, 0, 0);
                 } else {
                     v4i bl = v & (v4i)0x7f7f7f7f;
                     v4i bh = (v >> 7) & (v4i)0x01010101;
@@ -150,25 +176,65 @@ __device__ __forceinline__ void match_tile(const uint8_t* __restrict__ A, int N,
                 }
             }
         }
-        // epilogue: lane holds column (col0 + 32b + l32), rows row0 + 8(r/4) + 4 hi + r%4
+    };
+    for (int ct = 0; ct < n_ct; ++ct) {
+        const int col0 = cc * kColChunk + ct * 64;
+        const bool more = ct + 1 < n_ct;
+        v4i nxt;
+        if (more) nxt = gload(ct + 1);
+        v16i acc[2], accx[2];
+        mfma_step(ct, acc, accx);
+        // epilogue: lane holds column (col0 + 32b + l32), rows row0 + 8(r/4) + 4 hi + r%4.
+        // Keys: row (d << 9) | local column, column (d << 8) | local row.  Method 1
+        // forms them exactly (mod 2^32) as (|a'|^2 << 9) + ((nbx << 9) + lj) +
+        // (dn << 10) and ((|a'|^2 << 8) | il) + (nbx << 8) + (dn << 9), with dn the
+        // MFMA result -a'.b' - sum b' and nbx = |b'|^2 + 2 sum b', since
+        // d = |a'|^2 + nbx + 2 dn.  A step whose rows and columns are all in range
+        // skips the validity selects (the condition is uniform over the block).
+        const bool full = (rb + 1) * kRowBlk <= N && col0 + 64 <= M;
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
             const int j = col0 + 32 * b + l32;
             const bool jv = j < M;
-            const int nbj = jv ? nb[j] : 0;
             const uint32_t lj = (uint32_t)(ct * 64 + 32 * b + l32);
+            const int nbj = sm.s_nb[lj];
             uint32_t c1 = 0xffffffffu, c2 = 0xffffffffu;
+            const uint32_t nbr = ((uint32_t)nbj << 9) + lj, nbc = (uint32_t)nbj << 8;
+            if (full) {
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int il = 32 * w + 8 * (r >> 2) + 4 * hi + (r & 3);   // row within the block
-                const bool v = jv && (rb * kRowBlk + il < N);
-                int dot = acc[b][r];
-                if (METHOD == 0) dot = dot + 128 * accx[b][r];
-                const int d = (sm.s_na[il] + nbj) - 2 * dot;
-                const uint32_t kc = v ? (((uint32_t)d << 8) | (uint32_t)il) : 0xffffffffu;
-                top2_insert(c1, c2, kc);
-                const uint32_t kr = v ? (((uint32_t)d << 9) | lj) : 0xffffffffu;
-                top2_insert(rk1[r], rk2[r], kr);
+                for (int r = 0; r < 16; ++r) {
+                    const int il = 32 * w + 8 * (r >> 2) + 4 * hi + (r & 3);
+                    uint32_t kc, kr;
+                    if (METHOD == 1) {
+                        const uint32_t dn = (uint32_t)acc[b][r];
+                        kr = (rn[r] + nbr) + (dn << 10);
+                        kc = (rc[r] + nbc) + (dn << 9);
+                    } else {
+                        const int d = ((int)rn[r] + nbj) - 2 * (acc[b][r] + 128 * accx[b][r]);
+                        kc = ((uint32_t)d << 8) | (uint32_t)il;
+                        kr = ((uint32_t)d << 9) | lj;
+                    }
+                    top2_insert(c1, c2, kc);
+                    top2_insert(rk1[r], rk2[r], kr);
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int il = 32 * w + 8 * (r >> 2) + 4 * hi + (r & 3);   // row within the block
+                    const bool v = jv && (rb * kRowBlk + il < N);
+                    uint32_t kc, kr;
+                    if (METHOD == 1) {
+                        const uint32_t dn = (uint32_t)acc[b][r];
+                        kr = (rn[r] + nbr) + (dn << 10);
+                        kc = (rc[r] + nbc) + (dn << 9);
+                    } else {
+                        const int d = ((int)rn[r] + nbj) - 2 * (acc[b][r] + 128 * accx[b][r]);
+                        kc = ((uint32_t)d << 8) | (uint32_t)il;
+                        kr = ((uint32_t)d << 9) | lj;
+                    }
+                    top2_insert(c1, c2, v ? kc : 0xffffffffu);
+                    top2_insert(rk1[r], rk2[r], v ? kr : 0xffffffffu);
+                }
             }
             uint32_t o1 = __shfl_xor(c1, 32), o2 = __shfl_xor(c2, 32);
             top2_merge(c1, c2, o1, o2);
@@ -450,10 +516,14 @@ __global__ __launch_bounds__(512) void k_match_tiles_b(const MPair* __restrict__
 __device__ __forceinline__ void reduce_top2_row(const uint32_t* __restrict__ part, int n, int nparts, int shift,
                                                 int span, int32_t* __restrict__ out, int i) {
     uint64_t k1 = ~0ull, k2 = ~0ull;
+    // partials in order; 8-byte loads, unrolled so several are in flight
+    const uint2* p2 = reinterpret_cast<const uint2*>(part);
+#pragma unroll 8
     for (int p = 0; p < nparts; ++p) {
+        const uint2 kk = p2[(size_t)p * n + i];
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            uint32_t k = part[((size_t)p * n + i) * 2 + q];
+            const uint32_t k = q ? kk.y : kk.x;
             if (k == 0xffffffffu) continue;
             uint64_t d = k >> shift;
             uint64_t idx = (uint64_t)p * span + (k & ((1u << shift) - 1));
@@ -649,7 +719,7 @@ int match_u8(const uint8_t* a, int N, const uint8_t* b, int M, double ratio, int
     uint32_t* colpart = (uint32_t*)(ws + P.o_colpart);
     if (N > 0 && M > 0) {
         hipLaunchKernelGGL(k_norms, dim3((N + 3) / 4), dim3(256), 0, s, a, N, method, na);
-        hipLaunchKernelGGL(k_norms, dim3((M + 3) / 4), dim3(256), 0, s, b, M, method, nb);
+        hipLaunchKernelGGL(k_norms, dim3((M + 3) / 4), dim3(256), 0, s, b, M, method == 1 ? 2 : method, nb);
         dim3 grid(P.nrb_a, P.ncc_a);
         if (method == 0)
             hipLaunchKernelGGL(k_match_tiles<0>, grid, dim3(512), 0, s, a, N, b, M, na, nb, rowpart, colpart);
@@ -753,7 +823,7 @@ static int match_batch_run(msift_match_batch* B, const uint8_t* a, const uint8_t
     int* nb = (int*)(ws + B->o_nb);
     const int np = B->np;
     if (B->sumN > 0) hipLaunchKernelGGL(k_norms, dim3((B->sumN + 3) / 4), dim3(256), 0, s, a, B->sumN, method, na);
-    if (B->sumM > 0) hipLaunchKernelGGL(k_norms, dim3((B->sumM + 3) / 4), dim3(256), 0, s, b, B->sumM, method, nb);
+    if (B->sumM > 0) hipLaunchKernelGGL(k_norms, dim3((B->sumM + 3) / 4), dim3(256), 0, s, b, B->sumM, method == 1 ? 2 : method, nb);
     if (B->tiles > 0) {
         if (method == 0)
             hipLaunchKernelGGL(k_match_tiles_b<0>, dim3(B->tiles), dim3(512), 0, s, (const MPair*)B->d_pairs, np, a, b,
