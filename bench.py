@@ -61,8 +61,27 @@ def cpu_baseline(views, seconds=12.0):
         if time.perf_counter() - t0 > seconds:
             break
     dt = time.perf_counter() - t0
+    # the same restatement on one thread (a shorter sample: whole views until ~6 s)
+    O.set_threads(1)
+    mp1, n1, t1 = 0.0, 0, time.perf_counter()
+    while time.perf_counter() - t1 < 6.0:
+        v = views[n1 % len(views)]
+        O.detect_describe(v, p)
+        mp1 += v.size / 1e6
+        n1 += 1
+    dt1 = time.perf_counter() - t1
+    O.set_threads(threads)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(dt * 1e3 / mp, 3), "unit": "ms/MP", "cores": threads, "kind": "port",
-            "sample": f"oracle detect_describe on {n} x 1024x768 views ({dt:.1f} s, {threads} OpenMP threads)"}
+            "sample": f"oracle detect_describe on {n} x 1024x768 views ({dt:.1f} s, {threads} OpenMP threads)",
+            "single_thread": {"value": round(dt1 * 1e3 / mp1, 3), "unit": "ms/MP",
+                              "sample": f"{n1} views on 1 thread ({dt1:.1f} s)"},
+            "cpu_model": model, "host_cpus": os.cpu_count()}
 
 
 def bench_match(torch, dev, n, method, iters=5):
