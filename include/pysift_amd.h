@@ -246,11 +246,12 @@ int msift_match_sim_f16(const float* d_a, int N, const float* d_b, int M, int D,
                         size_t ws_bytes, int32_t* d_fwd, int32_t* d_bwd, int32_t* d_pairs, int* d_npairs,
                         void* stream);
 
-/* The reference's composition of a pair (_blend_pair with color_match
- * 'gain', core.py:3686-3714, plus the warp and paste of _stitch_pair,
+/* The reference's composition of a pair (_blend_pair, core.py:3686-3714;
+ * color_match 0 = 'gain', 1 = 'hist' (_hist_match, core.py:3591-3626); plus
+ * the warp and paste of _stitch_pair,
  * core.py:3783-3802): image 1 (BGR u8, h1 x w1, at least 2 x 2) warped by H
- * with grid_sample semantics, image 2 pasted at (round(ox), round(oy)), gain
- * compensation over the overlap, a DP seam through the colour + gradient
+ * with grid_sample semantics, image 2 pasted at (round(ox), round(oy)),
+ * colour matching over the overlap (gain, or histogram matching), a DP seam through the colour + gradient
  * cost, the seam mask softened by a sigma-3 Gaussian, and an n_levels
  * Laplacian-pyramid blend (1 <= n_levels <= 12).  Outputs: d_pano
  * [ch*cw*3] (0 outside both images), d_valid [ch*cw], optional d_seam_mask
@@ -269,9 +270,9 @@ typedef struct msift_composite_stats {
 } msift_composite_stats;
 int msift_composite_workspace_bytes(int cw, int ch, int n_levels, size_t* bytes);
 int msift_composite_multiband(const uint8_t* d_img1, int h1, int w1, const uint8_t* d_img2, int h2, int w2,
-                              const double* H, int cw, int ch, double ox, double oy, int n_levels, void* d_ws,
-                              size_t ws_bytes, uint8_t* d_pano, uint8_t* d_valid, float* d_seam_mask,
-                              msift_composite_stats* d_stats, void* stream);
+                              const double* H, int cw, int ch, double ox, double oy, int n_levels,
+                              int color_match, void* d_ws, size_t ws_bytes, uint8_t* d_pano, uint8_t* d_valid,
+                              float* d_seam_mask, msift_composite_stats* d_stats, void* stream);
 
 /* cv2.createCLAHE(clipLimit, (tiles_x, tiles_y)).apply(gray) (core.py:2562-2565,
  * applied to every image ahead of detectAndCompute) with OpenCV's 8-bit

@@ -85,7 +85,8 @@ def _declare(L):
     L.or_cv_blur25.restype = None; L.or_cv_blur25.argtypes = [_P, _I, _I, _P, _P]
     L.or_dp_seam.restype = None; L.or_dp_seam.argtypes = [_P, _P, _I, _I, _I, _P]
     L.or_composite.restype = _I
-    L.or_composite.argtypes = [_P, _I, _I, _P, _I, _I, _P, _I, _I, _D, _D, _I, _P, _P, _P, _P, _P, _P]
+    L.or_composite.argtypes = [_P, _I, _I, _P, _I, _I, _P, _I, _I, _D, _D, _I, _I, _P, _P, _P, _P, _P, _P]
+    L.or_hist_lut.restype = None; L.or_hist_lut.argtypes = [_P, _P, _P]
     L.or_pca_fit.restype = _I; L.or_pca_fit.argtypes = [_P, _I, _P, _I, _I, _P, _P, _P, _P]
     L.or_pca_project.restype = None; L.or_pca_project.argtypes = [_P, _I, _P, _P, _I, _P]
     L.or_sim16.restype = None; L.or_sim16.argtypes = [_P, _I, _P, _I, _I, _P]
@@ -495,8 +496,17 @@ def dp_seam(cost, ov, vertical=True):
     return seam
 
 
-def composite(img1, img2, H, cw, ch, ox, oy, n_levels=6):
-    """_blend_pair with color_match 'gain' plus the warp and paste of
+def hist_lut(hs, hr):
+    """The per-channel LUT of _hist_match (core.py:3591-3626) from the overlap
+    histograms hs, hr (3, 256) of the image to correct and of the reference."""
+    hs = np.ascontiguousarray(hs, np.uint64); hr = np.ascontiguousarray(hr, np.uint64)
+    lut = np.empty((3, 256), np.float32)
+    lib().or_hist_lut(_ptr(hs), _ptr(hr), _ptr(lut))
+    return lut
+
+
+def composite(img1, img2, H, cw, ch, ox, oy, n_levels=6, color_match="gain"):
+    """_blend_pair (color_match 'gain' or 'hist') plus the warp and paste of
     _stitch_pair (core.py:3686-3714, 3783-3802): returns (pano uint8 (ch, cw,
     3), valid bool, seam mask float32, gain (3,), seam RMSE, vertical)."""
     img1 = np.ascontiguousarray(img1, np.uint8); img2 = np.ascontiguousarray(img2, np.uint8)
@@ -505,7 +515,8 @@ def composite(img1, img2, H, cw, ch, ox, oy, n_levels=6):
     seam = np.empty((ch, cw), np.float32); gain = np.empty(3, np.float32)
     rmse = ctypes.c_double(); vert = ctypes.c_int()
     if lib().or_composite(_ptr(img1), img1.shape[0], img1.shape[1], _ptr(img2), img2.shape[0], img2.shape[1],
-                          _ptr(H), cw, ch, float(ox), float(oy), int(n_levels), _ptr(pano), _ptr(valid), _ptr(seam),
+                          _ptr(H), cw, ch, float(ox), float(oy), int(n_levels), 1 if color_match == "hist" else 0,
+                          _ptr(pano), _ptr(valid), _ptr(seam),
                           _ptr(gain), ctypes.byref(rmse), ctypes.byref(vert)) != 0:
         raise ValueError("bad composite arguments")
     return pano, valid.astype(bool), seam, gain, rmse.value, vert.value

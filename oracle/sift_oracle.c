@@ -1411,7 +1411,8 @@ void or_match_sim16(const float* a, int N, const float* b, int M, int D, double 
  *     |i| >= 2^30), truncated;
  *   gain (core.py:3696-3697): exact integer sums over the overlap, float
  *     means m = (float)(sum / n), g = (m2 + 1e-3f) / (m1 + 1e-3f) in [0.5, 2],
- *     w1 = min(max(w1 g, 0), 255);
+ *     w1 = min(max(w1 g, 0), 255); or (color_match 'hist', core.py:3694-3695)
+ *     w1 = lut[w1], the per-channel histogram-matching LUT of or_hist_lut;
  *   seam cost (core.py:3411-3417): sqrtf((d0^2 + d1^2) + d2^2) + 0.05f *
  *     sqrtf(gx^2 + gy^2) on the channel mean of (w1 + c2) * 0.5; Sobel as
  *     SciPy (derivative [-1, 0, 1] then smoothing [1, 2, 1], mode 'reflect',
@@ -1652,8 +1653,40 @@ void or_dp_seam(const float* cost, const uint8_t* ov, int H, int W, int vertical
     free(lines); free(dp); free(nd); free(bt);
 }
 
+/* np.interp(x, xp, arange(n)) with NumPy's rules: fp[0] left of xp[0],
+   fp[n-1] right of xp[n-1] and at it, fp[j] at an exact xp[j] (j the last
+   index with xp[j] <= x), else slope * (x - xp[j]) + fp[j] with slope =
+   (fp[j+1] - fp[j]) / (xp[j+1] - xp[j]) */
+static double np_interp_index(double x, const double* xp, int n)
+{
+    if (x < xp[0]) return 0.0;
+    if (x > xp[n - 1]) return (double)(n - 1);
+    int lo = 0, hi = n;                 /* first index with xp[i] > x */
+    while (lo < hi) { const int mid = (lo + hi) >> 1; if (x >= xp[mid]) lo = mid + 1; else hi = mid; }
+    const int j = lo - 1;
+    if (j == n - 1) return (double)(n - 1);
+    if (xp[j] == x) return (double)j;
+    const double slope = ((double)(j + 1) - (double)j) / (xp[j + 1] - xp[j]);
+    return slope * (x - xp[j]) + (double)j;
+}
+
+/* _hist_match (core.py:3591-3626) on integer-valued images: per channel, the
+   overlap histograms of src and ref, their CDFs cumsum / (total + 1e-8) in
+   double, lut[v] = interp(s_cdf[v], r_cdf, 0..255) rounded to float */
+void or_hist_lut(const uint64_t* hs /*3*256*/, const uint64_t* hr /*3*256*/, float* lut /*3*256*/)
+{
+    for (int c = 0; c < 3; ++c) {
+        double sc[256], rc[256];
+        uint64_t a = 0, b = 0;
+        for (int v = 0; v < 256; ++v) { a += hs[c * 256 + v]; b += hr[c * 256 + v]; sc[v] = (double)a; rc[v] = (double)b; }
+        const double st = sc[255] + 1e-8, rt = rc[255] + 1e-8;
+        for (int v = 0; v < 256; ++v) { sc[v] = sc[v] / st; rc[v] = rc[v] / rt; }
+        for (int v = 0; v < 256; ++v) lut[c * 256 + v] = (float)np_interp_index(sc[v], rc, 256);
+    }
+}
+
 int or_composite(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h2, int w2, const double* H,
-                 int cw, int ch, double ox, double oy, int n_levels,
+                 int cw, int ch, double ox, double oy, int n_levels, int color_match,
                  uint8_t* pano, uint8_t* valid, float* seam_out, float* gain_out, double* rmse, int* vertical_out)
 {
     if (h1 < 2 || w1 < 2 || h2 < 1 || w2 < 1 || cw < 1 || ch < 1 || n_levels < 1) return -1;
@@ -1679,7 +1712,18 @@ int or_composite(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h
             for (int c = 0; c < 3; ++c) { s1[c] += wu[p * 3 + c]; s2[c] += c2[p * 3 + c]; }
         }
     float g[3] = { 1.0f, 1.0f, 1.0f };
-    if (nov) for (int c = 0; c < 3; ++c) {
+    float* hlut = NULL;
+    if (nov && color_match == 1) {
+        uint64_t* hs = (uint64_t*)calloc(3 * 256, sizeof(uint64_t));
+        uint64_t* hr = (uint64_t*)calloc(3 * 256, sizeof(uint64_t));
+        for (size_t p = 0; p < P; ++p)
+            if (m1[p] && m2[p])
+                for (int c = 0; c < 3; ++c) { hs[c * 256 + wu[p * 3 + c]]++; hr[c * 256 + c2[p * 3 + c]]++; }
+        hlut = (float*)malloc(3 * 256 * sizeof(float));
+        or_hist_lut(hs, hr, hlut);
+        free(hs); free(hr);
+    }
+    if (nov && color_match != 1) for (int c = 0; c < 3; ++c) {
         const float a = (float)((double)s1[c] / (double)nov), b = (float)((double)s2[c] / (double)nov);
         float t = (b + 1e-3f) / (a + 1e-3f);
         g[c] = t < 0.5f ? 0.5f : (t > 2.0f ? 2.0f : t);
@@ -1689,7 +1733,8 @@ int or_composite(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h
     for (size_t p = 0; p < P; ++p)
         for (int c = 0; c < 3; ++c) {
             float t = (float)wu[p * 3 + c];
-            if (nov) { t = t * g[c]; t = t < 0.0f ? 0.0f : (t > 255.0f ? 255.0f : t); }
+            if (hlut) t = hlut[c * 256 + wu[p * 3 + c]];
+            else if (nov) { t = t * g[c]; t = t < 0.0f ? 0.0f : (t > 255.0f ? 255.0f : t); }
             w1g[p * 3 + c] = t;
         }
     /* seam mask */
@@ -1837,7 +1882,7 @@ int or_composite(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h
     if (vertical_out) *vertical_out = nov ? vertical : -1;
     for (int q = 0; q < 7; ++q) for (int k = 0; k < n_levels; ++k) free(gp[q][k]);
     for (int c = 0; c < 3; ++c) free(res[c]);
-    free(wu); free(m1); free(c2); free(m2); free(w1g); free(seam); free(ov);
+    free(wu); free(m1); free(c2); free(m2); free(w1g); free(seam); free(ov); free(hlut);
     return 0;
 }
 

@@ -94,8 +94,9 @@ void or_zoom_up(const float* in, int h, int w, int th, int tw, float* out);
 void or_sobel(const float* in, int H, int W, int axis, float* out);
 void or_cv_blur25(const float* in, int H, int W, const float* w25, float* out);
 void or_dp_seam(const float* cost, const uint8_t* ov, int H, int W, int vertical, int32_t* seam);
+void or_hist_lut(const uint64_t* hs, const uint64_t* hr, float* lut);
 int  or_composite(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h2, int w2, const double* H,
-                  int cw, int ch, double ox, double oy, int n_levels,
+                  int cw, int ch, double ox, double oy, int n_levels, int color_match,
                   uint8_t* pano /*ch*cw*3*/, uint8_t* valid, float* seam_mask /*ch*cw, may be NULL*/,
                   float* gain /*3*/, double* rmse, int* vertical);
 

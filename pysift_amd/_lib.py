@@ -89,7 +89,8 @@ def _declare(L):
     L.msift_composite_workspace_bytes.restype = _I
     L.msift_composite_workspace_bytes.argtypes = [_I, _I, _I, ctypes.POINTER(_SZ)]
     L.msift_composite_multiband.restype = _I
-    L.msift_composite_multiband.argtypes = [_P, _I, _I, _P, _I, _I, _P, _I, _I, _D, _D, _I, _P, _SZ, _P, _P, _P, _P, _P]
+    L.msift_composite_multiband.argtypes = [_P, _I, _I, _P, _I, _I, _P, _I, _I, _D, _D, _I, _I, _P, _SZ, _P, _P, _P, _P,
+                                            _P]
     L.msift_match_sim_f16.restype = _I
     L.msift_match_sim_f16.argtypes = [_P, _I, _P, _I, _I, _D, _P, _SZ, _P, _P, _P, _P, _P]
     L.msift_batch_create.restype = _I

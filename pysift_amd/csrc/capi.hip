@@ -222,11 +222,11 @@ int msift_composite_workspace_bytes(int cw, int ch, int n_levels, size_t* bytes)
 }
 
 int msift_composite_multiband(const uint8_t* d_img1, int h1, int w1, const uint8_t* d_img2, int h2, int w2,
-                              const double* H, int cw, int ch, double ox, double oy, int n_levels, void* d_ws,
-                              size_t ws_bytes, uint8_t* d_pano, uint8_t* d_valid, float* d_seam_mask,
-                              msift_composite_stats* d_stats, void* stream) {
-    return composite(d_img1, h1, w1, d_img2, h2, w2, H, cw, ch, ox, oy, n_levels, d_ws, ws_bytes, d_pano, d_valid,
-                     d_seam_mask, d_stats, (hipStream_t)stream);
+                              const double* H, int cw, int ch, double ox, double oy, int n_levels,
+                              int color_match, void* d_ws, size_t ws_bytes, uint8_t* d_pano, uint8_t* d_valid,
+                              float* d_seam_mask, msift_composite_stats* d_stats, void* stream) {
+    return composite(d_img1, h1, w1, d_img2, h2, w2, H, cw, ch, ox, oy, n_levels, color_match, d_ws, ws_bytes, d_pano,
+                     d_valid, d_seam_mask, d_stats, (hipStream_t)stream);
 }
 
 int msift_l2_normalize(const float* d_x, int n, float* d_y, void* stream) {

@@ -15,6 +15,7 @@
 //     nw, ne, sw, se over in-bounds corners; clipped and truncated to u8;
 //   gain: exact integer sums over the overlap, m = (float)(sum / n),
 //     g = clamp((m2 + 1e-3f) / (m1 + 1e-3f), 0.5, 2), w1 = clamp(w1 g, 0, 255);
+//     or (color_match 'hist') w1 = lut[w1] from the overlap CDFs (np.interp rules);
 //   cost: sqrtf((d0^2 + d1^2) + d2^2) + 0.05f sqrtf(gx^2 + gy^2), Sobel of the
 //     channel mean of (w1 + c2) * 0.5 with SciPy's passes (derivative, then
 //     [1, 2, 1]; 'reflect'; double accumulation); 1e9 outside the overlap;
@@ -164,12 +165,48 @@ __global__ __launch_bounds__(kCT) void k_cmp_colflags(const uint8_t* __restrict_
     if (bits) atomicOr(&colflags[u], bits);
 }
 
+// color_match 'hist': overlap histograms of the warped image 1 (hist[0..2])
+// and of image 2 (hist[3..5]); per-block LDS counts, one atomic per bin
+__global__ __launch_bounds__(kCT) void k_cmp_hist(const uint8_t* __restrict__ wu, const uint8_t* __restrict__ c2,
+                                                  const uint8_t* __restrict__ m1, const uint8_t* __restrict__ m2,
+                                                  size_t P, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[6 * 256];
+    for (int i = threadIdx.x; i < 6 * 256; i += kCT) h[i] = 0;
+    __syncthreads();
+    for (size_t p = (size_t)blockIdx.x * kCT + threadIdx.x; p < P; p += (size_t)gridDim.x * kCT) {
+        if (!(m1[p] && m2[p])) continue;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            atomicAdd(&h[c * 256 + wu[p * 3 + c]], 1u);
+            atomicAdd(&h[(3 + c) * 256 + c2[p * 3 + c]], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 6 * 256; i += kCT)
+        if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// np.interp(x, xp, 0..n-1) with NumPy's rules (see the oracle's restatement)
+__device__ double np_interp_index(double x, const double* xp, int n) {
+    if (x < xp[0]) return 0.0;
+    if (x > xp[n - 1]) return (double)(n - 1);
+    int lo = 0, hi = n;
+    while (lo < hi) { const int mid = (lo + hi) >> 1; if (x >= xp[mid]) lo = mid + 1; else hi = mid; }
+    const int j = lo - 1;
+    if (j == n - 1) return (double)(n - 1);
+    if (xp[j] == x) return (double)j;
+    const double slope = ((double)(j + 1) - (double)j) / (xp[j + 1] - xp[j]);
+    return slope * (x - xp[j]) + (double)j;
+}
+
 // one block: gain, seam orientation and side (core.py:3419-3431, 3696)
 __global__ __launch_bounds__(kCT) void k_cmp_decide(const uint32_t* __restrict__ rowflags, int ch,
                                                     const uint32_t* __restrict__ colflags, int cw,
                                                     const unsigned long long* __restrict__ rowacc,
-                                                    CmpStats* __restrict__ st) {
+                                                    int color_match, const uint32_t* __restrict__ hist,
+                                                    float* __restrict__ lut, CmpStats* __restrict__ st) {
     __shared__ unsigned long long sh[kCT / 64];
+    __shared__ double sc[256], rc[256];
     {
         unsigned long long t7[7] = {0, 0, 0, 0, 0, 0, 0};
         for (int v = threadIdx.x; v < ch; v += kCT)
@@ -199,8 +236,8 @@ __global__ __launch_bounds__(kCT) void k_cmp_decide(const uint32_t* __restrict__
         const unsigned long long n = st->n_overlap;
         st->has_overlap = n > 0;
         for (int k = 0; k < 3; ++k) {
-            float g = 1.0f;
-            if (n) {
+            float g = 1.0f;   // 1 when no gain is applied (no overlap, or 'hist')
+            if (n && color_match == 0) {
                 const float a = (float)((double)st->sum1[k] / (double)n), b = (float)((double)st->sum2[k] / (double)n);
                 g = (b + 1e-3f) / (a + 1e-3f);
                 g = g < 0.5f ? 0.5f : (g > 2.0f ? 2.0f : g);
@@ -213,6 +250,27 @@ __global__ __launch_bounds__(kCT) void k_cmp_decide(const uint32_t* __restrict__
         st->img1_first = st->vertical ? left : top;
         st->sq_diff = 0;
     }
+    // color_match 'hist' (core.py:3591-3626): per channel lut[v] =
+    // interp(s_cdf[v], r_cdf, 0..255), CDFs cumsum / (total + 1e-8) in double
+    if (color_match == 1) {
+        __syncthreads();
+        const bool any = st->n_overlap > 0;
+        for (int c = 0; c < 3; ++c) {
+            const int v = threadIdx.x;
+            unsigned long long a = 0, b = 0;
+            for (int u = 0; u <= v; ++u) { a += hist[c * 256 + u]; b += hist[(3 + c) * 256 + u]; }
+            sc[v] = (double)a;
+            rc[v] = (double)b;
+            __syncthreads();
+            const double stt = sc[255] + 1e-8, rtt = rc[255] + 1e-8;
+            __syncthreads();
+            sc[v] = sc[v] / stt;
+            rc[v] = rc[v] / rtt;
+            __syncthreads();
+            lut[c * 256 + v] = any ? (float)np_interp_index(sc[v], rc, 256) : (float)v;
+            __syncthreads();
+        }
+    }
 }
 
 // gained image 1, the level-0 planes of both images (each filled with the
@@ -220,7 +278,8 @@ __global__ __launch_bounds__(kCT) void k_cmp_decide(const uint32_t* __restrict__
 // seam RMSE sums; planes: [I1 b, g, r, I2 b, g, r, mask] of P floats
 __global__ __launch_bounds__(kCT) void k_cmp_prepare(const uint8_t* __restrict__ wu, const uint8_t* __restrict__ c2,
                                                      const uint8_t* __restrict__ m1, const uint8_t* __restrict__ m2,
-                                                     size_t P, CmpStats* __restrict__ st, float* __restrict__ planes,
+                                                     size_t P, CmpStats* __restrict__ st, int color_match,
+                                                     const float* __restrict__ lut, float* __restrict__ planes,
                                                      float* __restrict__ costc, float* __restrict__ avg,
                                                      float* __restrict__ seam0) {
     __shared__ unsigned long long sh[kCT / 64];
@@ -232,7 +291,14 @@ __global__ __launch_bounds__(kCT) void k_cmp_prepare(const uint8_t* __restrict__
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             float w = (float)wu[p * 3 + c];
-            if (any) { w = w * st->gain[c]; w = w < 0.0f ? 0.0f : (w > 255.0f ? 255.0f : w); }
+            if (any) {
+                if (color_match == 1) {
+                    w = lut[c * 256 + wu[p * 3 + c]];
+                } else {
+                    w = w * st->gain[c];
+                    w = w < 0.0f ? 0.0f : (w > 255.0f ? 255.0f : w);
+                }
+            }
             const float q = (float)c2[p * 3 + c];
             const float d = w - q;
             d2[c] = d * d;
@@ -589,7 +655,7 @@ __global__ void k_cmp_zero(unsigned long long* __restrict__ p, int n) {
 struct CmpPlan {
     int L, hs[kMaxLevels], ws[kMaxLevels];
     size_t P, o_wu, o_c2, o_m1, o_m2, o_rowf, o_colf, o_st, o_costc, o_avg, o_dx, o_dy, o_cost, o_seam0, o_tmp,
-        o_lvl[kMaxLevels], o_res[kMaxLevels], o_dpa, o_dpb, o_bt, o_seam, o_lines, o_rowacc, zero_bytes, total;
+        o_lvl[kMaxLevels], o_res[kMaxLevels], o_dpa, o_dpb, o_bt, o_seam, o_lines, o_rowacc, o_hist, o_lut, zero_bytes, total;
 };
 
 CmpPlan cmp_plan(int cw, int ch, int n_levels) {
@@ -606,7 +672,9 @@ CmpPlan cmp_plan(int cw, int ch, int n_levels) {
     Pl.o_st = take(sizeof(CmpStats));
     Pl.o_rowf = take((size_t)ch * 4); Pl.o_colf = take((size_t)cw * 4);
     Pl.o_rowacc = take((size_t)ch * 8 * 8);
+    Pl.o_hist = take(6 * 256 * 4);
     Pl.zero_bytes = off - Pl.o_st;
+    Pl.o_lut = take(3 * 256 * 4);
     Pl.o_costc = take(P * 4); Pl.o_avg = take(P * 4); Pl.o_dx = take(P * 4); Pl.o_dy = take(P * 4);
     Pl.o_cost = take(P * 4); Pl.o_seam0 = take(P * 4);
     Pl.o_tmp = take(std::max(P, (size_t)7 * ((ch + 1) / 2) * cw) * 4);
@@ -634,8 +702,8 @@ void inv3_adj(const double* a, double* o) {
 size_t composite_workspace(int cw, int ch, int n_levels) { return cmp_plan(cw, ch, n_levels).total; }
 
 int composite(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h2, int w2, const double* H, int cw,
-              int ch, double ox, double oy, int n_levels, void* ws_, size_t ws_bytes, uint8_t* pano, uint8_t* valid,
-              float* seam_out, void* stats_out, hipStream_t s) {
+              int ch, double ox, double oy, int n_levels, int color_match, void* ws_, size_t ws_bytes, uint8_t* pano,
+              uint8_t* valid, float* seam_out, void* stats_out, hipStream_t s) {
     if (!img1 || !img2 || !H || !pano || !valid || !stats_out || !ws_) {
         set_error("msift_composite_multiband: null buffer");
         return MSIFT_EINVAL;
@@ -648,6 +716,10 @@ int composite(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h2, 
     }
     const CmpPlan Pl = cmp_plan(cw, ch, n_levels);
     if (ws_bytes < Pl.total) { set_error("msift_composite_multiband: workspace too small"); return MSIFT_EWORKSPACE; }
+    if (color_match != 0 && color_match != 1) {
+        set_error("msift_composite_multiband: color_match must be 0 (gain) or 1 (hist)");
+        return MSIFT_EINVAL;
+    }
     if (((uintptr_t)ws_ & 255) != 0) { set_error("msift_composite_multiband: workspace must be 256-byte aligned"); return MSIFT_EINVAL; }
     char* ws = (char*)ws_;
     uint8_t* wu = (uint8_t*)(ws + Pl.o_wu);
@@ -698,11 +770,17 @@ int composite(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h2, 
                        (int)std::rint(ox), (int)std::rint(oy), wu, c2, m1, m2, rowacc, rowf);
     hipLaunchKernelGGL(k_cmp_colflags, dim3((cw + kCT - 1) / kCT, (ch + 63) / 64), dim3(kCT), 0, s,
                        (const uint8_t*)m1, (const uint8_t*)m2, cw, ch, colf);
+    uint32_t* hist = (uint32_t*)(ws + Pl.o_hist);
+    float* lut = (float*)(ws + Pl.o_lut);
+    if (color_match == 1)
+        hipLaunchKernelGGL(k_cmp_hist, dim3((unsigned)std::min<size_t>((P + 4095) / 4096, 1024)), dim3(kCT), 0, s,
+                           (const uint8_t*)wu, (const uint8_t*)c2, (const uint8_t*)m1, (const uint8_t*)m2, P, hist);
     hipLaunchKernelGGL(k_cmp_decide, dim3(1), dim3(kCT), 0, s, (const uint32_t*)rowf, ch, (const uint32_t*)colf, cw,
-                       (const unsigned long long*)rowacc, st);
+                       (const unsigned long long*)rowacc, color_match, (const uint32_t*)hist, lut, st);
     float* lvl0 = (float*)(ws + Pl.o_lvl[0]);
     hipLaunchKernelGGL(k_cmp_prepare, dim3(gP), dim3(kCT), 0, s, (const uint8_t*)wu, (const uint8_t*)c2,
-                       (const uint8_t*)m1, (const uint8_t*)m2, P, st, lvl0, costc, avg, seam0);
+                       (const uint8_t*)m1, (const uint8_t*)m2, P, st, color_match, (const float*)lut, lvl0, costc,
+                       avg, seam0);
     hipLaunchKernelGGL(k_cmp_sobel1, g2d, dim3(kCT), 0, s, (const float*)avg, cw, ch, dx, dy);
     hipLaunchKernelGGL(k_cmp_sobel2, g2d, dim3(kCT), 0, s, (const float*)dx, (const float*)dy, (const float*)costc,
                        (const uint8_t*)m1, (const uint8_t*)m2, cw, ch, (const CmpStats*)st, cost);

@@ -101,8 +101,8 @@ size_t clahe_workspace(int tiles_x, int tiles_y);
 size_t pca_workspace(int N, int M);
 size_t composite_workspace(int cw, int ch, int n_levels);
 int composite(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h2, int w2, const double* H, int cw,
-              int ch, double ox, double oy, int n_levels, void* ws, size_t ws_bytes, uint8_t* pano, uint8_t* valid,
-              float* seam_out, void* stats_out, hipStream_t s);
+              int ch, double ox, double oy, int n_levels, int color_match, void* ws, size_t ws_bytes, uint8_t* pano,
+              uint8_t* valid, float* seam_out, void* stats_out, hipStream_t s);
 int l2_normalize_rows(const float* x, int n, float* y, hipStream_t s);
 int pca_fit(const float* a, int N, const float* b, int M, int dims, void* ws, size_t ws_bytes, float* mean,
             float* proj, double* evals, int* sweeps, hipStream_t s);

@@ -82,8 +82,8 @@ class GPUPyStitch:
         self.n_depth_layers = n_depth_layers   # MiDaS is unavailable: global homography (core.py:3788-3792)
         if blend not in ("feather", "multiband"):
             raise ValueError(f"blend must be 'feather' or 'multiband' (got {blend!r})")
-        if blend == "multiband" and color_match != "gain":
-            raise NotImplementedError("multiband composition supports color_match='gain' only")
+        if color_match not in ("gain", "hist"):
+            raise ValueError(f"color_match must be 'gain' or 'hist' (got {color_match!r})")
         self.blend = blend                     # "multiband": the reference's gain + seam + pyramid blend
         self.n_blend_levels = int(n_blend_levels)
         self.ransac_iters = int(ransac_iters)
@@ -255,7 +255,7 @@ class GPUPyStitch:
         return pano, valid
 
     def composite_multiband(self, b1, b2, H, cw, ch, ox, oy, n_levels=None, seam_mask=False):
-        """_blend_pair with color_match 'gain' plus the warp and paste of
+        """_blend_pair (self.color_match: 'gain' or 'hist') plus the warp and paste of
         _stitch_pair (core.py:3686-3714, 3783-3802) on the device: returns
         (pano (ch, cw, 3) u8, valid (ch, cw) u8, seam mask float32 or None,
         stats dict with gain, seam orientation and seam RMSE)."""
@@ -273,7 +273,8 @@ class GPUPyStitch:
         Hc = np.ascontiguousarray(H, np.float64)
         _lib.check(_lib.lib().msift_composite_multiband(
             _lib.ptr(b1), b1.shape[0], b1.shape[1], _lib.ptr(b2), b2.shape[0], b2.shape[1],
-            Hc.ctypes.data_as(ctypes.c_void_p), cw, ch, float(ox), float(oy), L, _lib.ptr(ws), ws.numel(),
+            Hc.ctypes.data_as(ctypes.c_void_p), cw, ch, float(ox), float(oy), L, 1 if self.color_match == "hist" else 0,
+            _lib.ptr(ws), ws.numel(),
             _lib.ptr(pano), _lib.ptr(valid), _lib.ptr(sm), _lib.ptr(st), _lib.stream_ptr(dev)), "composite")
         raw = st.cpu().numpy().tobytes()
         gain = np.frombuffer(raw[72:84], np.float32).copy()

@@ -10,20 +10,21 @@ from pysift_amd import synthetic as syn
 pytestmark = pytest.mark.gpu
 
 
-def _gpu(va, vb, H, cw, ch, ox, oy, levels=6):
+def _gpu(va, vb, H, cw, ch, ox, oy, levels=6, color_match="gain"):
     import torch
     from pysift_amd import GPUPyStitch
-    st = GPUPyStitch(verbose=False, save_output=False, blend="multiband", n_blend_levels=levels)
+    st = GPUPyStitch(verbose=False, save_output=False, blend="multiband", n_blend_levels=levels,
+                     color_match=color_match)
     b1, b2 = torch.from_numpy(va).cuda(), torch.from_numpy(vb).cuda()
     pano, valid, sm, stats = st.composite_multiband(b1, b2, H, cw, ch, ox, oy, seam_mask=True)
     return pano.cpu().numpy(), valid.cpu().numpy().astype(bool), sm.cpu().numpy(), stats
 
 
-def _check(va, vb, H, levels=6, tag=""):
+def _check(va, vb, H, levels=6, tag="", color_match="gain"):
     cw, ch, ox, oy = O.canvas(va.shape[0], va.shape[1], vb.shape[0], vb.shape[1], H)
-    print(f"[case] composite {tag} canvas {cw}x{ch} levels {levels}", flush=True)
-    p, v, sm, stats = _gpu(va, vb, H, cw, ch, ox, oy, levels)
-    rp, rv, rsm, rg, rr, rvert = O.composite(va, vb, H, cw, ch, ox, oy, levels)
+    print(f"[case] composite {tag} canvas {cw}x{ch} levels {levels} {color_match}", flush=True)
+    p, v, sm, stats = _gpu(va, vb, H, cw, ch, ox, oy, levels, color_match)
+    rp, rv, rsm, rg, rr, rvert = O.composite(va, vb, H, cw, ch, ox, oy, levels, color_match)
     np.testing.assert_array_equal(v, rv)
     np.testing.assert_array_equal(stats["gain"], rg)
     assert stats["vertical"] == rvert
@@ -36,9 +37,10 @@ def _check(va, vb, H, levels=6, tag=""):
     return stats
 
 
-def test_composite_stitch_pair():
+@pytest.mark.parametrize("color_match", ["gain", "hist"])
+def test_composite_stitch_pair(color_match):
     va, vb, Hgt = syn.synth_pair(seed=1000, view_hw=(384, 512), canvas_hw=(768, 1024))
-    st = _check(va, vb, Hgt, 6, "side by side")
+    st = _check(va, vb, Hgt, 6, "side by side", color_match)
     assert st["vertical"] == 1 and st["n_overlap"] > 10000
 
 
@@ -51,6 +53,7 @@ def test_composite_stacked_views_horizontal_seam():
     H = np.array([[1.0, 0.0, 0.0], [0.0, 1.0, -180.0], [0.0, 0.0, 1.0]])   # view a's pixels in view b's frame
     st = _check(va, vb, H, 5, "stacked")
     assert st["vertical"] == 0
+    _check(va, vb, H, 5, "stacked", "hist")
 
 
 @pytest.mark.parametrize("levels", [1, 3, 8])

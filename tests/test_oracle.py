@@ -758,6 +758,50 @@ def test_dp_seam_against_brute_force(seed):
     np.testing.assert_array_equal(O.dp_seam(np.ascontiguousarray(cost.T), np.ascontiguousarray(ov.T), False), seam)
 
 
+def test_hist_lut_equals_numpy_interp():
+    # _hist_match's LUT: np.interp of the source CDF into the reference CDF,
+    # including empty bins (plateaus of the CDF, where np.interp's search rule
+    # decides), against NumPy itself
+    rng = np.random.default_rng(12)
+    for trial in range(60):
+        hs = rng.integers(0, 40, (3, 256)).astype(np.uint64)
+        hr = rng.integers(0, 40, (3, 256)).astype(np.uint64)
+        if trial % 2:
+            hs[:, rng.integers(0, 256, 120)] = 0
+            hr[:, rng.integers(0, 256, 150)] = 0
+        lut = O.hist_lut(hs, hr)
+        for c in range(3):
+            s = np.cumsum(hs[c].astype(np.int64)).astype(np.float64)
+            r = np.cumsum(hr[c].astype(np.int64)).astype(np.float64)
+            s /= s[-1] + 1e-8
+            r /= r[-1] + 1e-8
+            np.testing.assert_array_equal(lut[c], np.interp(s, r, np.arange(256)).astype(np.float32))
+
+
+def test_composite_hist_matching_uses_the_lut():
+    # two flat views shifted by 40 px.  The reference CDF of a channel is a
+    # single step at the second colour, so NumPy's rule (the last index of a
+    # plateau at or below x) sends the top source value to 255 and a value
+    # below it into the step; the composite applies exactly that LUT
+    a = np.empty((60, 80, 3), np.uint8); a[:] = (100, 120, 60)
+    b = np.empty((60, 80, 3), np.uint8); b[:] = (110, 126, 66)
+    Ht = np.array([[1.0, 0, 40.0], [0, 1.0, 0], [0, 0, 1.0]])
+    cw, ch, ox, oy = O.canvas(60, 80, 60, 80, Ht)
+    pano, valid, seam, gain, rmse, vert = O.composite(a, b, Ht, cw, ch, ox, oy, 6, "hist")
+    w, m1 = O.grid_warp(a, Ht, cw, ch, ox, oy)
+    m2 = np.zeros((ch, cw), bool); m2[:60, :80] = True
+    ov = m1 & m2
+    hs = np.stack([np.bincount(w[ov][:, c], minlength=256) for c in range(3)]).astype(np.uint64)
+    hr = np.stack([np.bincount(np.broadcast_to(b[0, 0, c], ov.sum()), minlength=256) for c in range(3)]).astype(np.uint64)
+    lut = O.hist_lut(hs, hr)
+    wg = np.stack([lut[c][w[..., c]] for c in range(3)], -1)
+    exp_rmse = np.sqrt(((wg[ov].astype(np.uint8).astype(np.int64) - np.array([110, 126, 66])) ** 2).sum() / (3 * ov.sum()))
+    assert abs(rmse - exp_rmse) < 1e-12
+    top = w[ov][:, 0].max()
+    assert lut[0][top] == 255.0 and 109.0 <= lut[0][top - 1] < 110.0
+    assert valid.sum() == (m1 | m2).sum()
+
+
 def test_composite_identical_views_and_constant_gain():
     rng = np.random.default_rng(2)
     img = np.clip(nd.gaussian_filter(rng.random((80, 100, 3)) * 255, (3, 3, 0)), 0, 255).astype(np.uint8)
