@@ -168,7 +168,6 @@ __global__ __launch_bounds__(256) void k_bunit(const BImg* __restrict__ im, cons
 template <int R, int TH>
 __global__ __launch_bounds__(256) void k_bblur(const BJob* __restrict__ jobs, int njobs, char* ws, Taps taps) {
     __shared__ __attribute__((aligned(16))) float sIn[BlurSmem<R, TH>::nIn];
-    __shared__ __attribute__((aligned(16))) float sH[BlurSmem<R, TH>::nH];
     __shared__ int s_j;
     const int tile = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     if (threadIdx.x < 64) {
@@ -190,7 +189,7 @@ __global__ __launch_bounds__(256) void k_bblur(const BJob* __restrict__ jobs, in
     const int ty = t / J.tiles_x, tx = t - ty * J.tiles_x;
     blur_tile<R, TH>((const float*)(ws + J.in_off), J.g_off == kNone ? nullptr : (float*)(ws + J.g_off),
                      J.d_off == kNone ? nullptr : (float*)(ws + J.d_off),
-                     J.dec_off == kNone ? nullptr : (float*)(ws + J.dec_off), J.H, J.W, J.decW, taps, tx, ty, sIn, sH,
+                     J.dec_off == kNone ? nullptr : (float*)(ws + J.dec_off), J.H, J.W, J.decW, taps, tx, ty, sIn,
                      J.q16);
 }
 

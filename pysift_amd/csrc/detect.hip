@@ -123,11 +123,10 @@ __global__ __launch_bounds__(256) void k_blur(const float* __restrict__ in, floa
                                                float* __restrict__ dout, float* __restrict__ dec,
                                                int H, int W, int decW, Taps taps, size_t zin, size_t zout, int q16) {
     __shared__ __attribute__((aligned(16))) float sIn[BlurSmem<R, TH>::nIn];
-    __shared__ __attribute__((aligned(16))) float sH[BlurSmem<R, TH>::nH];
     const size_t z = blockIdx.z;
     const int t = xcd_remap((int)(blockIdx.x + blockIdx.y * gridDim.x), (int)(gridDim.x * gridDim.y));
     blur_tile<R, TH>(in + z * zin, gout ? gout + z * zout : nullptr, dout ? dout + z * zout : nullptr,
-                     dec ? dec + z * zout : nullptr, H, W, decW, taps, t % (int)gridDim.x, t / (int)gridDim.x, sIn, sH,
+                     dec ? dec + z * zout : nullptr, H, W, decW, taps, t % (int)gridDim.x, t / (int)gridDim.x, sIn,
                      q16);
 }
 

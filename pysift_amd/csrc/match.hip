@@ -28,12 +28,21 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 
 constexpr int kRowBlk = 256;   // rows per workgroup
+constexpr uint32_t kKeyOff = 1u << 21;   // offset of the shift method's keys (see match_tile)
 constexpr int kColChunk = 512; // columns per workgroup (9-bit local index)
 
 // top-2 of unique 32-bit keys, k1 <= k2 always: the new second is the median
-// of (k1, k2, k) (one v_med3_u32) and the new first min(k1, k)
+// of (k1, k2, k) and the new first min(k1, k).  The median is written as the
+// instruction: from a chain that starts at (~0, ~0) the compiler folds the
+// min/max form into three operations instead of one v_med3_u32.
+__device__ __forceinline__ uint32_t med3_u32(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 __device__ __forceinline__ void top2_insert(uint32_t& k1, uint32_t& k2, uint32_t k) {
-    const uint32_t m = max(min(k1, k2), min(max(k1, k2), k));
+    const uint32_t m = med3_u32(k1, k2, k);
     k1 = min(k1, k);
     k2 = m;
 }
@@ -80,9 +89,9 @@ constexpr int kBTile = 64 * kBStride;             // one 64-column step
 struct MatchSmem {
     uint8_t bt[2][kBTile];                        // staged b columns, double buffer
     uint32_t colk[8][kColChunk][2];               // per-wave column top-2 for the chunk
-    int s_na[kRowBlk];                            // method 0: |a|^2; method 1: |a'|^2 << 9 (row keys)
-    uint32_t s_nac[kRowBlk];                      // method 1: (|a'|^2 << 8) | row (column keys)
-    int s_nb[kColChunk];                          // the chunk's column norms (no global load in
+    int s_na[kRowBlk];                            // method 0: |a|^2
+    uint32_t s_nac[kRowBlk];                      // method 1: ((|a'|^2 + 2^21) << 8) | row (column keys)
+    int s_nb[kColChunk];                          // the chunk's column terms (no global load in
                                                   // the epilogue, which would wait on the prefetch)
 };
 
@@ -96,14 +105,15 @@ __device__ __forceinline__ void match_tile(const uint8_t* __restrict__ A, int N,
     const int row0 = rb * kRowBlk + w * 32;     // this wave's 32 rows
     {
         const int j = cc * kColChunk + threadIdx.x;
-        sm.s_nb[threadIdx.x] = j < M ? nb[j] : 0;
+        // method 1 stages its row keys' whole column term ((nbx + 2^21) << 9) + local column
+        const int v = j < M ? nb[j] : 0;
+        sm.s_nb[threadIdx.x] = METHOD == 1 ? (int)((((uint32_t)v + kKeyOff) << 9) + threadIdx.x) : v;
     }
     if (threadIdx.x < kRowBlk) {
         int i = rb * kRowBlk + threadIdx.x;
         const int v = i < N ? na[i] : 0;
         if (METHOD == 1) {
-            sm.s_na[threadIdx.x] = (int)((uint32_t)v << 9);
-            sm.s_nac[threadIdx.x] = ((uint32_t)v << 8) | (uint32_t)threadIdx.x;
+            sm.s_nac[threadIdx.x] = (((uint32_t)v + kKeyOff) << 8) | (uint32_t)threadIdx.x;
         } else {
             sm.s_na[threadIdx.x] = v;
         }
@@ -148,7 +158,7 @@ __device__ __forceinline__ void match_tile(const uint8_t* __restrict__ A, int N,
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int il = 32 * w + 8 * (r >> 2) + 4 * hi + (r & 3);
-        rn[r] = (uint32_t)sm.s_na[il];
+        rn[r] = METHOD == 1 ? 0u : (uint32_t)sm.s_na[il];
         rc[r] = METHOD == 1 ? sm.s_nac[il] : 0u;
     }
     auto mfma_step = [&](int step, v16i* acc, v16i* accx) {
@@ -186,11 +196,14 @@ __device__ __forceinline__ void match_tile(const uint8_t* __restrict__ A, int N,
         mfma_step(ct, acc, accx);
         // epilogue: lane holds column (col0 + 32b + l32), rows row0 + 8(r/4) + 4 hi + r%4.
         // Keys: row (d << 9) | local column, column (d << 8) | local row.  Method 1
-        // forms them exactly (mod 2^32) as (|a'|^2 << 9) + ((nbx << 9) + lj) +
-        // (dn << 10) and ((|a'|^2 << 8) | il) + (nbx << 8) + (dn << 9), with dn the
-        // MFMA result -a'.b' - sum b' and nbx = |b'|^2 + 2 sum b', since
-        // d = |a'|^2 + nbx + 2 dn.  A step whose rows and columns are all in range
-        // skips the validity selects (the condition is uniform over the block).
+        // drops from each key the term that is constant along it: with dn the MFMA
+        // result -a'.b' - sum b', nbx = |b'|^2 + 2 sum b' and d = |a'|^2 + nbx + 2 dn,
+        // the row key is ((d - |a'|^2 + 2^21) << 9) | lj = (dn << 10) + ((nbx + 2^21) << 9) + lj
+        // and the column key ((d - nbx + 2^21) << 8) | il = (dn << 9) + ((|a'|^2 + 2^21) << 8) + il,
+        // one shift-add each, exact mod 2^32 (both offset values lie in [0, 2^23) for every
+        // pair of u8 vectors).  The reduction adds the dropped term back.  A step whose
+        // rows and columns are all in range skips the validity selects (the condition is
+        // uniform over the block).
         const bool full = (rb + 1) * kRowBlk <= N && col0 + 64 <= M;
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
@@ -199,7 +212,7 @@ __device__ __forceinline__ void match_tile(const uint8_t* __restrict__ A, int N,
             const uint32_t lj = (uint32_t)(ct * 64 + 32 * b + l32);
             const int nbj = sm.s_nb[lj];
             uint32_t c1 = 0xffffffffu, c2 = 0xffffffffu;
-            const uint32_t nbr = ((uint32_t)nbj << 9) + lj, nbc = (uint32_t)nbj << 8;
+            const uint32_t nbr = METHOD == 1 ? (uint32_t)nbj : ((uint32_t)nbj << 9) + lj;
             if (full) {
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
@@ -207,8 +220,8 @@ __device__ __forceinline__ void match_tile(const uint8_t* __restrict__ A, int N,
                     uint32_t kc, kr;
                     if (METHOD == 1) {
                         const uint32_t dn = (uint32_t)acc[b][r];
-                        kr = (rn[r] + nbr) + (dn << 10);
-                        kc = (rc[r] + nbc) + (dn << 9);
+                        kr = nbr + (dn << 10);
+                        kc = rc[r] + (dn << 9);
                     } else {
                         const int d = ((int)rn[r] + nbj) - 2 * (acc[b][r] + 128 * accx[b][r]);
                         kc = ((uint32_t)d << 8) | (uint32_t)il;
@@ -225,8 +238,8 @@ __device__ __forceinline__ void match_tile(const uint8_t* __restrict__ A, int N,
                     uint32_t kc, kr;
                     if (METHOD == 1) {
                         const uint32_t dn = (uint32_t)acc[b][r];
-                        kr = (rn[r] + nbr) + (dn << 10);
-                        kc = (rc[r] + nbc) + (dn << 9);
+                        kr = nbr + (dn << 10);
+                        kc = rc[r] + (dn << 9);
                     } else {
                         const int d = ((int)rn[r] + nbj) - 2 * (acc[b][r] + 128 * accx[b][r]);
                         kc = ((uint32_t)d << 8) | (uint32_t)il;
@@ -512,14 +525,18 @@ __global__ __launch_bounds__(512) void k_match_tiles_b(const MPair* __restrict__
                        (uint32_t*)(ws + P.o_rowpart), (uint32_t*)(ws + P.o_colpart), t % P.nrb, t / P.nrb, sm);
 }
 
-// merge per-chunk partials into (best index, d1, d2); key 0xffffffff = none
-__device__ __forceinline__ void reduce_top2_row(const uint32_t* __restrict__ part, int n, int nparts, int shift,
-                                                int span, int32_t* __restrict__ out, int i) {
-    uint64_t k1 = ~0ull, k2 = ~0ull;
-    // partials in order; 8-byte loads, unrolled so several are in flight
+// merge per-chunk partials into (best index, d1, d2); key 0xffffffff = none.
+// Partials p0, p0 + pstep, ... of entry i fold into the 64-bit top-2 (k1, k2)
+// of (d << 32) | global index: unique keys, so any split of the partials gives
+// the same pair.  adj (the shift method): the keys hold d - adj[i] + 2^21,
+// adj being |a'|^2 for rows and nbx for columns; the distances written are d.
+__device__ __forceinline__ void reduce_top2_scan(const uint32_t* __restrict__ part, int n, int p0, int pstep,
+                                                 int nparts, int shift, int span, int i, uint64_t& k1,
+                                                 uint64_t& k2) {
+    // 8-byte loads, unrolled so several are in flight
     const uint2* p2 = reinterpret_cast<const uint2*>(part);
 #pragma unroll 8
-    for (int p = 0; p < nparts; ++p) {
+    for (int p = p0; p < nparts; p += pstep) {
         const uint2 kk = p2[(size_t)p * n + i];
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
@@ -530,28 +547,64 @@ __device__ __forceinline__ void reduce_top2_row(const uint32_t* __restrict__ par
             top2_insert64(k1, k2, (d << 32) | idx);
         }
     }
-    out[3 * i] = k1 == ~0ull ? -1 : (int32_t)(k1 & 0xffffffffu);
-    out[3 * i + 1] = k1 == ~0ull ? 0x7fffffff : (int32_t)(k1 >> 32);
-    out[3 * i + 2] = k2 == ~0ull ? 0x7fffffff : (int32_t)(k2 >> 32);
 }
 
+__device__ __forceinline__ void reduce_top2_write(int32_t* __restrict__ out, int i, uint64_t k1, uint64_t k2,
+                                                  const int* __restrict__ adj) {
+    const int32_t da = adj ? adj[i] - (int32_t)kKeyOff : 0;
+    out[3 * i] = k1 == ~0ull ? -1 : (int32_t)(k1 & 0xffffffffu);
+    out[3 * i + 1] = k1 == ~0ull ? 0x7fffffff : (int32_t)(k1 >> 32) + da;
+    out[3 * i + 2] = k2 == ~0ull ? 0x7fffffff : (int32_t)(k2 >> 32) + da;
+}
+
+__device__ __forceinline__ void reduce_top2_row(const uint32_t* __restrict__ part, int n, int nparts, int shift,
+                                                int span, int32_t* __restrict__ out, int i,
+                                                const int* __restrict__ adj) {
+    uint64_t k1 = ~0ull, k2 = ~0ull;
+    reduce_top2_scan(part, n, 0, 1, nparts, shift, span, i, k1, k2);
+    reduce_top2_write(out, i, k1, k2, adj);
+}
+
+// 64 entries per block; wave w scans partials w, w + 4, ... and wave 0 merges
+// the four pairs (n / 64 blocks keep enough loads in flight to stream the
+// partials at HBM rate)
 __global__ __launch_bounds__(256) void k_reduce_top2(const uint32_t* __restrict__ part, int n, int nparts,
-                                                      int shift, int span, int32_t* __restrict__ out) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) reduce_top2_row(part, n, nparts, shift, span, out, i);
+                                                      int shift, int span, int32_t* __restrict__ out,
+                                                      const int* __restrict__ adj) {
+    __shared__ uint64_t s_k[3][64][2];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    uint64_t k1 = ~0ull, k2 = ~0ull;
+    if (i < n) reduce_top2_scan(part, n, w, 4, nparts, shift, span, i, k1, k2);
+    if (w > 0) {
+        s_k[w - 1][lane][0] = k1;
+        s_k[w - 1][lane][1] = k2;
+    }
+    __syncthreads();
+    if (w == 0 && i < n) {
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            top2_insert64(k1, k2, s_k[o][lane][0]);
+            top2_insert64(k1, k2, s_k[o][lane][1]);
+        }
+        reduce_top2_write(out, i, k1, k2, adj);
+    }
 }
 
 // rows (cols == 0: top-2 over b, into fwd) or columns (top-2 over a, into
 // bwd) of pair blockIdx.y; no partials (an empty side) gives "none"
 __global__ __launch_bounds__(256) void k_reduce_top2_b(const MPair* __restrict__ pairs, const char* __restrict__ ws,
-                                                        int cols, int32_t* __restrict__ fwd, int32_t* __restrict__ bwd) {
+                                                        int cols, int32_t* __restrict__ fwd, int32_t* __restrict__ bwd,
+                                                        const int* __restrict__ na, const int* __restrict__ nb,
+                                                        int method) {
     const MPair P = pairs[blockIdx.y];
     const int n = cols ? P.M : P.N;
     const int nparts = (P.N > 0 && P.M > 0) ? (cols ? P.nrb : P.ncc) : 0;
     const uint32_t* part = (const uint32_t*)(ws + (cols ? P.o_colpart : P.o_rowpart));
     int32_t* out = cols ? bwd + 3 * (size_t)P.b_off : fwd + 3 * (size_t)P.a_off;
+    const int* adj = method == 1 ? (cols ? nb + P.b_off : na + P.a_off) : nullptr;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
-        reduce_top2_row(part, n, nparts, cols ? 8 : 9, cols ? kRowBlk : kColChunk, out, i);
+        reduce_top2_row(part, n, nparts, cols ? 8 : 9, cols ? kRowBlk : kColChunk, out, i, adj);
 }
 
 __global__ void k_fill_top2_none(int32_t* __restrict__ out, int n) {
@@ -725,8 +778,10 @@ int match_u8(const uint8_t* a, int N, const uint8_t* b, int M, double ratio, int
             hipLaunchKernelGGL(k_match_tiles<0>, grid, dim3(512), 0, s, a, N, b, M, na, nb, rowpart, colpart);
         else
             hipLaunchKernelGGL(k_match_tiles<1>, grid, dim3(512), 0, s, a, N, b, M, na, nb, rowpart, colpart);
-        hipLaunchKernelGGL(k_reduce_top2, dim3((N + 255) / 256), dim3(256), 0, s, rowpart, N, P.ncc_a, 9, kColChunk, fwd);
-        hipLaunchKernelGGL(k_reduce_top2, dim3((M + 255) / 256), dim3(256), 0, s, colpart, M, P.nrb_a, 8, kRowBlk, bwd);
+        hipLaunchKernelGGL(k_reduce_top2, dim3((N + 63) / 64), dim3(256), 0, s, rowpart, N, P.ncc_a, 9, kColChunk, fwd,
+                           method == 1 ? (const int*)na : nullptr);
+        hipLaunchKernelGGL(k_reduce_top2, dim3((M + 63) / 64), dim3(256), 0, s, colpart, M, P.nrb_a, 8, kRowBlk, bwd,
+                           method == 1 ? (const int*)nb : nullptr);
     } else {
         if (N > 0) hipLaunchKernelGGL(k_fill_top2_none, dim3((N + 255) / 256), dim3(256), 0, s, fwd, N);
         if (M > 0) hipLaunchKernelGGL(k_fill_top2_none, dim3((M + 255) / 256), dim3(256), 0, s, bwd, M);
@@ -834,10 +889,10 @@ static int match_batch_run(msift_match_batch* B, const uint8_t* a, const uint8_t
     }
     if (B->max_rows > 0)
         hipLaunchKernelGGL(k_reduce_top2_b, dim3(gcap_u((B->max_rows + 255) / 256, 1024), np), dim3(256), 0, s,
-                           (const MPair*)B->d_pairs, (const char*)ws, 0, fwd, bwd);
+                           (const MPair*)B->d_pairs, (const char*)ws, 0, fwd, bwd, (const int*)na, (const int*)nb, method);
     if (B->max_cols > 0)
         hipLaunchKernelGGL(k_reduce_top2_b, dim3(gcap_u((B->max_cols + 255) / 256, 1024), np), dim3(256), 0, s,
-                           (const MPair*)B->d_pairs, (const char*)ws, 1, fwd, bwd);
+                           (const MPair*)B->d_pairs, (const char*)ws, 1, fwd, bwd, (const int*)na, (const int*)nb, method);
     const double r2 = ratio * ratio;
     int* cnt = (int*)(ws + B->o_cnt);
     int* offs = (int*)(ws + B->o_off);
@@ -890,8 +945,10 @@ int match_f16(const float* a, int N, const float* b, int M, double ratio, void* 
         hipLaunchKernelGGL(k_norms_f16, dim3((M + 3) / 4), dim3(256), 0, s, (const _Float16*)hb, M, nb);
         hipLaunchKernelGGL((k_match_tiles_f16<8, false>), dim3(P.nrb_a, P.ncc_a), dim3(512), 0, s, (const _Float16*)ha, N,
                            (const _Float16*)hb, M, (const float*)na, (const float*)nb, rowpart, colpart);
-        hipLaunchKernelGGL(k_reduce_top2, dim3((N + 255) / 256), dim3(256), 0, s, rowpart, N, P.ncc_a, 9, kColChunk, fwd);
-        hipLaunchKernelGGL(k_reduce_top2, dim3((M + 255) / 256), dim3(256), 0, s, colpart, M, P.nrb_a, 8, kRowBlk, bwd);
+        hipLaunchKernelGGL(k_reduce_top2, dim3((N + 63) / 64), dim3(256), 0, s, rowpart, N, P.ncc_a, 9, kColChunk, fwd,
+                           nullptr);
+        hipLaunchKernelGGL(k_reduce_top2, dim3((M + 63) / 64), dim3(256), 0, s, colpart, M, P.nrb_a, 8, kRowBlk, bwd,
+                           nullptr);
     } else {
         if (N > 0) hipLaunchKernelGGL(k_fill_top2_none, dim3((N + 255) / 256), dim3(256), 0, s, fwd, N);
         if (M > 0) hipLaunchKernelGGL(k_fill_top2_none, dim3((M + 255) / 256), dim3(256), 0, s, bwd, M);
@@ -994,8 +1051,10 @@ int match_sim16(const float* a, int N, const float* b, int M, int D, double rati
         case 7: hipLaunchKernelGGL((k_match_tiles_f16<7, true>), g, dim3(512), 0, s, cha, N, chb, M, nullptr, nullptr, rowpart, colpart); break;
         default: hipLaunchKernelGGL((k_match_tiles_f16<8, true>), g, dim3(512), 0, s, cha, N, chb, M, nullptr, nullptr, rowpart, colpart); break;
         }
-        hipLaunchKernelGGL(k_reduce_top2, dim3((N + 255) / 256), dim3(256), 0, s, rowpart, N, P.ncc_a, 9, kColChunk, fwd);
-        hipLaunchKernelGGL(k_reduce_top2, dim3((M + 255) / 256), dim3(256), 0, s, colpart, M, P.nrb_a, 8, kRowBlk, bwd);
+        hipLaunchKernelGGL(k_reduce_top2, dim3((N + 63) / 64), dim3(256), 0, s, rowpart, N, P.ncc_a, 9, kColChunk, fwd,
+                           nullptr);
+        hipLaunchKernelGGL(k_reduce_top2, dim3((M + 63) / 64), dim3(256), 0, s, colpart, M, P.nrb_a, 8, kRowBlk, bwd,
+                           nullptr);
         hipLaunchKernelGGL(k_sim_to_dist, dim3((N + 255) / 256), dim3(256), 0, s, fwd, N);
         hipLaunchKernelGGL(k_sim_to_dist, dim3((M + 255) / 256), dim3(256), 0, s, bwd, M);
     } else {

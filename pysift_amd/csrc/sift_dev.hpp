@@ -355,7 +355,7 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
 template <int R, int TH>
 __device__ __forceinline__ void blur_tile(const float* __restrict__ in, float* __restrict__ gout,
                                           float* __restrict__ dout, float* __restrict__ dec, int H, int W, int decW,
-                                          const Taps& taps, int bx, int by, float* sIn, float* sH, int q16) {
+                                          const Taps& taps, int bx, int by, float* sIn, int q16) {
     constexpr int TW = 64;
     constexpr int IW = TW + 2 * R;
     constexpr int IWP = (IW + 4 + 3) & ~3;
@@ -415,13 +415,26 @@ __device__ __forceinline__ void blur_tile(const float* __restrict__ in, float* _
         }
     }
     __syncthreads();
+    // H pass, in place: row ly's 64 outputs overwrite the first 64 floats of
+    // row ly of the input tile.  A row's 16 float4 items fall in one wave and
+    // one iteration, whose LDS reads all precede its writes, so a row is read
+    // whole before it is overwritten, and no other wave touches it.  The
+    // input values the DoG subtracts are taken into registers first.
+    constexpr int RG = TH / 4;
+    const int lx = tid & 63, g = tid >> 6;
+    float g0v[RG];
+    if (dout) {   // uniform over the block
+#pragma unroll
+        for (int j = 0; j < RG; ++j) g0v[j] = sIn[(g * RG + j + R) * IWP + lx + R];
+        __syncthreads();
+    }
     for (int i = tid; i < IH * (TW / 4); i += 256) {
         int ly = i / (TW / 4), q = i - ly * (TW / 4);
-        const float* row = sIn + ly * IWP + 4 * q;
+        float* row = sIn + ly * IWP;
         float v[NV4 * 4];
 #pragma unroll
         for (int k = 0; k < NV4; ++k) {
-            float4 t = *reinterpret_cast<const float4*>(row + 4 * k);
+            float4 t = *reinterpret_cast<const float4*>(row + 4 * q + 4 * k);
             v[4 * k] = t.x; v[4 * k + 1] = t.y; v[4 * k + 2] = t.z; v[4 * k + 3] = t.w;
         }
         float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f, o3 = 0.0f;
@@ -433,14 +446,12 @@ __device__ __forceinline__ void blur_tile(const float* __restrict__ in, float* _
             o2 = fmaf(v[k + 2], w, o2);
             o3 = fmaf(v[k + 3], w, o3);
         }
-        *reinterpret_cast<float4*>(sH + ly * TW + 4 * q) = make_float4(o0, o1, o2, o3);
+        *reinterpret_cast<float4*>(row + 4 * q) = make_float4(o0, o1, o2, o3);
     }
     __syncthreads();
-    constexpr int RG = TH / 4;
-    const int lx = tid & 63, g = tid >> 6;
     float v[RG + 2 * R];
 #pragma unroll
-    for (int k = 0; k < RG + 2 * R; ++k) v[k] = sH[(g * RG + k) * TW + lx];
+    for (int k = 0; k < RG + 2 * R; ++k) v[k] = sIn[(g * RG + k) * IWP + lx];
     const int gx = x0 + lx;
 #pragma unroll
     for (int j = 0; j < RG; ++j) {
@@ -452,7 +463,7 @@ __device__ __forceinline__ void blur_tile(const float* __restrict__ in, float* _
             size_t o = (size_t)gy * W + gx;
             if (gout) gout[o] = s;
             if (dout) {
-                const float g0 = sIn[(g * RG + j + R) * IWP + lx + R];
+                const float g0 = g0v[j];
                 dout[o] = q16 ? f16_round(s) - f16_round(g0) : s - g0;
             }
             if (dec && !(gy & 1) && !(gx & 1)) dec[(size_t)(gy >> 1) * decW + (gx >> 1)] = s;
@@ -465,8 +476,7 @@ struct BlurSmem {
     static constexpr int TW = 64;
     static constexpr int IWP = (TW + 2 * R + 4 + 3) & ~3;
     static constexpr int IH = TH + 2 * R;
-    static constexpr int nIn = IH * IWP;
-    static constexpr int nH = IH * TW;
+    static constexpr int nIn = IH * IWP;   // the input tile; the H pass overwrites it row by row
 };
 
 // --------------------------------------------------------------- extrema --
