@@ -232,7 +232,7 @@ def bench_pairs(torch, dev, n_pairs=32, n_hyp=1024, iters=5):
     rp = RansacBatchPlan(mp.Ns, n_hyp, dev)
 
     def batched():
-        mp.run(DA, DB, 0.75, "hilo")
+        mp.run(DA, DB, 0.75, "shift")
         p1, p2 = mp.gather_points(KA, KB)
         return rp.run(p1, p2, mp.npairs, seed=42, thr=3.0)
 
@@ -249,7 +249,7 @@ def bench_pairs(torch, dev, n_pairs=32, n_hyp=1024, iters=5):
         out = []
         for p in range(n_pairs):
             (ka, da), (kb, db) = A[p], B[p]
-            _, _, pr = match_u8(da, db, 0.75, "hilo")
+            _, _, pr = match_u8(da, db, 0.75, "shift")
             if pr.shape[0] < 4:
                 out.append(None)
                 continue

@@ -18,7 +18,7 @@ def match_workspace_bytes(N, M):
     return n.value
 
 
-def match_u8(a, b, ratio=0.75, method="hilo", sync=True, ws=None):
+def match_u8(a, b, ratio=0.75, method="shift", sync=True, ws=None):
     """a: (N,128) uint8 CUDA tensor, b: (M,128).  Returns (fwd (N,3), bwd (M,3),
     pairs (K,2)) int32 device tensors; fwd[i] = (best j, d1, d2)."""
     import torch
@@ -120,7 +120,7 @@ class MatchBatchPlan:
             return torch.zeros((1, 128), dtype=torch.uint8, device=self.device)
         return torch.cat([t.to(self.device) for t in sets]).contiguous()
 
-    def run(self, A, B, ratio=0.75, method="hilo"):
+    def run(self, A, B, ratio=0.75, method="shift"):
         """A, B packed (sum N, 128) / (sum M, 128) uint8 device tensors."""
         rc = _lib.lib().msift_match_u8_batched(
             self._p, _lib.ptr(A), _lib.ptr(B), float(ratio), METHODS[method], _lib.ptr(self.ws), self.ws.numel(),
@@ -153,7 +153,7 @@ class MatchBatchPlan:
         return p1, p2
 
 
-def match_u8_batched(sets_a, sets_b, ratio=0.75, method="hilo"):
+def match_u8_batched(sets_a, sets_b, ratio=0.75, method="shift"):
     """Match pairs (sets_a[p], sets_b[p]) in one launch; returns per pair
     (fwd, bwd, pairs) as match_u8 would."""
     plan = MatchBatchPlan([t.shape[0] for t in sets_a], [t.shape[0] for t in sets_b], device=sets_a[0].device)

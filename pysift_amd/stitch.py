@@ -71,7 +71,7 @@ class GPUPyStitch:
                  color_match="gain", dsp_sift=False, contrast_thresh=0.04, n_octaves=4, n_scales=3, sigma0=1.6,
                  edge_thresh=10.0, double_image=True, max_dimension=0, denoise=False, sharpen=False,
                  clahe_clip=2.0, clahe_tile=8, output_max_dim=0, post_sharpen=False, output_format="png",
-                 output_quality=90, *, match_method="hilo", ransac_seed=42, save_output=True, verbose=True,
+                 output_quality=90, *, match_method="shift", ransac_seed=42, save_output=True, verbose=True,
                  blend="feather", device=None):
         if matcher != "ratio":
             raise NotImplementedError("only the ratio-test matcher is part of this build")
