@@ -332,12 +332,15 @@ __device__ __forceinline__ uint32_t f16_q(float d) {
 // Similarity keys of the reference's fp16 path (core.py:2668-2677): s is
 // rounded to binary16 (the GEMM's output type) and ordered largest first,
 // ties by index; -0 counts as +0.  key = 0xffff - (order-preserving map of
-// the binary16 bits), so a smaller key is a larger similarity.
+// the binary16 bits), so a smaller key is a larger similarity.  With h the
+// bits after a binary16 add of +0 (which turns -0 into +0 and keeps every
+// other value), that is h for negative h and 0x7fff - h = h ^ 0x7fff for
+// the others (equal to the map for all 65536 patterns).
 __device__ __forceinline__ uint32_t sim16_key(float s) {
-    uint32_t h = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)s);
-    if (h == 0x8000u) h = 0;
-    const uint32_t u = (h & 0x8000u) ? (~h & 0xffffu) : (h | 0x8000u);
-    return 0xffffu - u;
+    const _Float16 f = (_Float16)s + (_Float16)0.0f;
+    const uint32_t h = (uint32_t)__builtin_bit_cast(uint16_t, f);
+    const uint32_t m = (uint32_t)((int32_t)(h << 16) >> 31);   // all ones for negative h
+    return h ^ (~m & 0x7fffu);
 }
 
 __device__ __forceinline__ float sim16_of_key(uint32_t key) {
@@ -381,6 +384,8 @@ __global__ __launch_bounds__(512) void k_match_tiles_f16(const _Float16* __restr
     for (int ct = 0; ct < kColChunk / 64; ++ct) {
         const int col0 = cc * kColChunk + ct * 64;
         if (col0 >= M) break;
+        // rows and columns all in range: no validity selects (uniform over the block)
+        const bool full = (rb + 1) * kRowBlk <= N && col0 + 64 <= M;
         v16f acc[2];
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
@@ -397,17 +402,28 @@ __global__ __launch_bounds__(512) void k_match_tiles_f16(const _Float16* __restr
             const int j = col0 + 32 * b + l32;
             const bool jv = j < M;
             const float nbj = (!SIM && jv) ? nb[j] : 0.0f;
-            const uint32_t lj = (uint32_t)(ct * 64 + 32 * b + l32);
+            uint32_t lj = (uint32_t)(ct * 64 + 32 * b + l32);
+            asm("" : "+v"(lj));   // one register: keeps (q << 9) | lj a single shift-or
             uint32_t c1 = 0xffffffffu, c2 = 0xffffffffu;
+            if (full) {
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int il = 32 * w + 8 * (r >> 2) + 4 * hi + (r & 3);
-                const bool v = jv && (rb * kRowBlk + il < N);
-                const uint32_t q = SIM ? sim16_key(acc[b][r]) : f16_q((s_na[il] + nbj) - 2.0f * acc[b][r]);
-                const uint32_t kc = v ? ((q << 8) | (uint32_t)il) : 0xffffffffu;
-                top2_insert(c1, c2, kc);
-                const uint32_t kr = v ? ((q << 9) | lj) : 0xffffffffu;
-                top2_insert(rk1[r], rk2[r], kr);
+                for (int r = 0; r < 16; ++r) {
+                    const int il = 32 * w + 8 * (r >> 2) + 4 * hi + (r & 3);
+                    const uint32_t q = SIM ? sim16_key(acc[b][r]) : f16_q((s_na[il] + nbj) - 2.0f * acc[b][r]);
+                    top2_insert(c1, c2, (q << 8) | (uint32_t)il);
+                    top2_insert(rk1[r], rk2[r], (q << 9) | lj);
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int il = 32 * w + 8 * (r >> 2) + 4 * hi + (r & 3);
+                    const bool v = jv && (rb * kRowBlk + il < N);
+                    const uint32_t q = SIM ? sim16_key(acc[b][r]) : f16_q((s_na[il] + nbj) - 2.0f * acc[b][r]);
+                    const uint32_t kc = v ? ((q << 8) | (uint32_t)il) : 0xffffffffu;
+                    top2_insert(c1, c2, kc);
+                    const uint32_t kr = v ? ((q << 9) | lj) : 0xffffffffu;
+                    top2_insert(rk1[r], rk2[r], kr);
+                }
             }
             uint32_t o1 = __shfl_xor(c1, 32), o2 = __shfl_xor(c2, 32);
             top2_merge(c1, c2, o1, o2);
