@@ -175,52 +175,52 @@ __global__ __launch_bounds__(1024) void k_pca_jacobi(const double* __restrict__ 
             }
             if (!__syncthreads_or(rot)) continue;
             rotated = 1;
-            // each (pair, index) item is owned by one thread and the items of a
-            // phase are disjoint, so a thread loads all its items before it
-            // stores any (8 items: 64 pairs x 128 indices over 1024 threads)
-            {
-                double x0[8], x1[8];
+            // One pass per round.  The row phase (rows p_k, q_k rotated by pair k)
+            // and then the column phase (columns p_m, q_m by pair m) touch the
+            // 2x2 block {p_k, q_k} x {p_m, q_m} only through its own four values,
+            // so one thread applies both phases to its blocks in that order (the
+            // same operations as two passes over the matrix) and zeroes a_pq of
+            // a rotated pair; 64 x 64 blocks and 64 x 128 Vt items over 1024
+            // threads, every value owned by one thread.  The Vt loads (global
+            // memory) are issued first so that their latency overlaps the blocks.
+            double v0[8], v1[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int it = tid + 1024 * u, k = it >> 7, j = it & 127;
-                    x0[u] = A[Ps[k] * kAStride + j];
-                    x1[u] = A[Qs[k] * kAStride + j];
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int it = tid + 1024 * u, k = it >> 7, j = it & 127;
-                    if (!Rs[k]) continue;
-                    const double c = Cs[k], s = Ss[k];
-                    A[Ps[k] * kAStride + j] = c * x0[u] - s * x1[u];
-                    A[Qs[k] * kAStride + j] = s * x0[u] + c * x1[u];
-                }
+            for (int u = 0; u < 8; ++u) {
+                const int it = tid + 1024 * u, k = it >> 7, i = it & 127;
+                v0[u] = Vt[Ps[k] * 128 + i];
+                v1[u] = Vt[Qs[k] * 128 + i];
             }
-            __syncthreads();
-            {
-                double x0[8], x1[8], v0[8], v1[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int it = tid + 1024 * u, k = it >> 7, i = it & 127;
-                    x0[u] = A[i * kAStride + Ps[k]];
-                    x1[u] = A[i * kAStride + Qs[k]];
-                    v0[u] = Vt[Ps[k] * 128 + i];
-                    v1[u] = Vt[Qs[k] * 128 + i];
+            for (int u = 0; u < 4; ++u) {
+                const int bi = tid + 1024 * u, k = bi >> 6, m = bi & 63;
+                const int pk = Ps[k], qk = Qs[k], pm = Ps[m], qm = Qs[m];
+                double a00 = A[pk * kAStride + pm], a01 = A[pk * kAStride + qm];
+                double a10 = A[qk * kAStride + pm], a11 = A[qk * kAStride + qm];
+                if (Rs[k]) {
+                    const double c = Cs[k], s = Ss[k];
+                    const double n00 = c * a00 - s * a10, n10 = s * a00 + c * a10;
+                    const double n01 = c * a01 - s * a11, n11 = s * a01 + c * a11;
+                    a00 = n00; a10 = n10; a01 = n01; a11 = n11;
                 }
+                if (Rs[m]) {
+                    const double c = Cs[m], s = Ss[m];
+                    const double n00 = c * a00 - s * a01, n01 = s * a00 + c * a01;
+                    const double n10 = c * a10 - s * a11, n11 = s * a10 + c * a11;
+                    a00 = n00; a01 = n01; a10 = n10; a11 = n11;
+                }
+                if (k == m && Rs[k]) { a01 = 0.0; a10 = 0.0; }
+                A[pk * kAStride + pm] = a00; A[pk * kAStride + qm] = a01;
+                A[qk * kAStride + pm] = a10; A[qk * kAStride + qm] = a11;
+            }
+            {
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     const int it = tid + 1024 * u, k = it >> 7, i = it & 127;
                     if (!Rs[k]) continue;
                     const double c = Cs[k], s = Ss[k];
-                    A[i * kAStride + Ps[k]] = c * x0[u] - s * x1[u];
-                    A[i * kAStride + Qs[k]] = s * x0[u] + c * x1[u];
                     Vt[Ps[k] * 128 + i] = c * v0[u] - s * v1[u];
                     Vt[Qs[k] * 128 + i] = s * v0[u] + c * v1[u];
                 }
-            }
-            __syncthreads();
-            if (tid < 64 && Rs[tid]) {
-                A[Ps[tid] * kAStride + Qs[tid]] = 0.0;
-                A[Qs[tid] * kAStride + Ps[tid]] = 0.0;
             }
             __syncthreads();
         }
