@@ -62,6 +62,27 @@ def autocrop(pano, both):
     return pano[r0 + ir0:r0 + ir1 + 1, c0 + ic0:c0 + ic1 + 1]
 
 
+def autocrop_box(both_d):
+    """autocrop's rectangle from a device validity mask, computed on the
+    device (the same first/last rules: argmax gives the first maximum):
+    returns (y0, y1, x0, x1), half-open, or None when no pixel is valid."""
+    torch = _torch()
+    both_d = both_d != 0
+    rows, cols = both_d.any(1), both_d.any(0)
+    ri, ci = torch.nonzero(rows).flatten(), torch.nonzero(cols).flatten()
+    if ri.numel() == 0:
+        return None
+    r0, r1, c0, c1 = (int(x) for x in torch.stack([ri[0], ri[-1], ci[0], ci[-1]]).cpu())
+    sub = both_d[r0:r1 + 1, c0:c1 + 1].to(torch.uint8)
+    nr, nc = sub.shape
+    left, right = sub.argmax(1), nc - 1 - sub.flip(1).argmax(1)
+    top, bot = sub.argmax(0), nr - 1 - sub.flip(0).argmax(0)
+    ic0, ic1, ir0, ir1 = (int(x) for x in torch.stack([left.max(), right.min(), top.max(), bot.min()]).cpu())
+    if ic0 >= ic1 or ir0 >= ir1:
+        return r0, r1 + 1, c0, c1 + 1
+    return r0 + ir0, r0 + ir1 + 1, c0 + ic0, c0 + ic1 + 1
+
+
 class GPUPyStitch:
     """Two-image panorama stitcher (reference: core.py:2400-4092)."""
 
@@ -235,12 +256,14 @@ class GPUPyStitch:
             seam_rmse = stats["rmse"]
         else:
             pano_d, valid_d = self.warp_feather(b1, b2, H, cw, ch, ox, oy)
-        pano = pano_d.cpu().numpy()
-        both = valid_d.cpu().numpy().astype(bool)
+        # autocrop on the device; only the cropped panorama crosses to the host
+        box = autocrop_box(valid_d)
+        n_valid = int(valid_d.sum().item())
+        pano = (pano_d if box is None else pano_d[box[0]:box[1], box[2]:box[3]]).cpu().numpy()
         timings["st5"] = time.perf_counter() - t0
-        metrics.update(seam_rmse=seam_rmse, coverage=float(both.sum()) / float(ch * cw) * 100.0,
+        metrics.update(seam_rmse=seam_rmse, coverage=float(n_valid) / float(ch * cw) * 100.0,
                        canvas_w=cw, canvas_h=ch)
-        return autocrop(pano, both), timings, metrics
+        return pano, timings, metrics
 
     def warp_feather(self, b1, b2, H, cw, ch, ox, oy):
         torch = _torch()

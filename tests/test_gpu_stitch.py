@@ -81,3 +81,32 @@ def test_stitch_errors():
         st.stitch(flat, flat)
     with pytest.raises(ValueError):
         st.stitch(flat)
+
+
+def test_autocrop_box_matches_host_autocrop():
+    # the device rectangle against the host restatement of _autocrop
+    # (core.py:3627-3684) on masks with ragged edges, holes, one valid row or
+    # column, an all-valid and an empty mask
+    import torch
+    from pysift_amd.stitch import autocrop, autocrop_box
+    rng = np.random.default_rng(7)
+    masks = []
+    for _ in range(6):
+        h, w = rng.integers(20, 90, 2)
+        m = np.zeros((h, w), bool)
+        y0, x0 = rng.integers(0, h // 3), rng.integers(0, w // 3)
+        y1, x1 = rng.integers(2 * h // 3, h + 1), rng.integers(2 * w // 3, w + 1)
+        m[y0:y1, x0:x1] = True
+        for r in range(y0, y1):                       # ragged left and right edges
+            m[r, x0:x0 + rng.integers(0, 4)] = False
+            m[r, max(x1 - rng.integers(0, 4), x0):x1] = False
+        m[rng.integers(0, h, 5), rng.integers(0, w, 5)] = False   # holes
+        masks.append(m)
+    one_row = np.zeros((30, 40), bool); one_row[12, 5:30] = True
+    one_col = np.zeros((30, 40), bool); one_col[3:25, 17] = True
+    masks += [one_row, one_col, np.ones((25, 35), bool), np.zeros((10, 12), bool)]
+    for m in masks:
+        pano = np.arange(m.size * 3, dtype=np.int64).reshape(m.shape + (3,))
+        box = autocrop_box(torch.from_numpy(m.astype(np.uint8)).cuda())
+        got = pano if box is None else pano[box[0]:box[1], box[2]:box[3]]
+        np.testing.assert_array_equal(got, autocrop(pano, m))
