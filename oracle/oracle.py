@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (oracle/libsift_oracle.so).
 
 TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg as the checker; the pysift_amd package never
+bench.py's cpu_baseline leg, to check results; the pysift_amd package never
 imports it.  Each wrapper names the reference function (pysift/core.py:<line>)
 that the C routine restates.
 """

@@ -1,11 +1,11 @@
 /*
  * CPU oracle for the PySIFT detect -> describe -> match -> RANSAC -> warp/feather
  * path.  TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py load this library as the checker.  The product
+ * cpu_baseline leg of bench.py load this library to check results.  The product
  * (pysift_amd) never links, imports or calls it.
  *
  * Every function restates the reference's algorithm (pysift/core.py, cited per
- * function) with the numerics this build defines where the reference leaves
+ * function) with the numerics pysift_amd defines where the reference leaves
  * them open (summation order, transcendental functions, FP64 solves).  The GPU
  * library is required to match these outputs bit for bit.
  */

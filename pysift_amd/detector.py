@@ -74,7 +74,7 @@ class PySIFT:
         if dsp_n_scales not in _DSP_PRESETS:
             raise ValueError(f"dsp_n_scales must be 3, 4, or 5 (got {dsp_n_scales})")
         if orientation != "histogram" or descriptor != "sift":
-            raise NotImplementedError("learned orientation/descriptor backends are not part of this build")
+            raise NotImplementedError("learned orientation/descriptor backends are not implemented in pysift_amd")
         self.n_octaves = 5 if n_octaves is None else int(n_octaves)
         self.n_scales = int(n_scales)
         self.sigma0 = float(sigma0)

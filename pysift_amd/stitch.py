@@ -95,9 +95,9 @@ class GPUPyStitch:
                  output_quality=90, *, match_method="shift", ransac_seed=42, save_output=True, verbose=True,
                  blend="feather", device=None):
         if matcher != "ratio":
-            raise NotImplementedError("only the ratio-test matcher is part of this build")
+            raise NotImplementedError("pysift_amd implements only the ratio-test matcher")
         if max_dimension or denoise or sharpen or output_max_dim or post_sharpen:
-            raise NotImplementedError("OpenCV pre/post-processing is not part of this build")
+            raise NotImplementedError("OpenCV pre/post-processing is not implemented in pysift_amd")
         if ransac_method not in ("magsac", "classic"):
             raise ValueError(f"ransac_method must be 'magsac' or 'classic' (got {ransac_method!r})")
         self.n_depth_layers = n_depth_layers   # MiDaS is unavailable: global homography (core.py:3788-3792)
@@ -311,7 +311,7 @@ class GPUPyStitch:
         """Stitch two images (BGR uint8); returns the uint8 (H, W, 3) panorama."""
         images = list(args[0]) if len(args) == 1 and isinstance(args[0], (list, tuple)) else list(args)
         if len(images) == 3:
-            raise NotImplementedError("3-image stitching is not part of this build")
+            raise NotImplementedError("3-image stitching is not implemented in pysift_amd")
         if len(images) != 2:
             raise ValueError(f"stitch() accepts 2 or 3 images, got {len(images)}")
         pano, timings, metrics = self._stitch_pair(images[0], images[1])
