@@ -17,6 +17,7 @@
 namespace msift {
 
 constexpr int kHypPerBlock = 64;
+constexpr int kHypWaves = 8;       // waves per hypothesis block: wave w scores hypotheses w, w + 8, ...
 
 __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
@@ -173,18 +174,20 @@ __global__ __launch_bounds__(64) void k_normalize(const double* __restrict__ p1,
     normalize_body(p1, p2, N, T);
 }
 
-// hypotheses blk*64 .. blk*64+63 (one wave): solve, then score
+// hypotheses blk*64 .. blk*64+63: wave 0 solves them (one per lane), then the
+// block's kHypWaves waves score them, each hypothesis by one wave with lanes
+// over the points (the same sum per hypothesis whichever wave computes it)
 __device__ __forceinline__ void hypotheses_body(const double* __restrict__ p1, const double* __restrict__ p2, int N,
                                                 int B, uint32_t k0, uint32_t k1, const double* __restrict__ T,
                                                 int method, double den, double tq, double* __restrict__ scores,
                                                 double* __restrict__ Hs, int blk, double (*sH)[9], double (*sHi)[9],
                                                 int* sOk) {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int b = blk * kHypPerBlock + lane;
     double T1[9], T2[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) { T1[k] = T[k]; T2[k] = T[9 + k]; }
-    {
+    if (wv == 0) {
         // phase 1: one hypothesis per lane
         uint32_t c[4] = { (uint32_t)b, 0u, 0u, 0u };
         philox4x32_10(c, k0, k1);
@@ -216,7 +219,7 @@ __device__ __forceinline__ void hypotheses_body(const double* __restrict__ p1, c
     __syncthreads();
     // phase 2: score each hypothesis with lanes over points
     const int nh = min(kHypPerBlock, B - blk * kHypPerBlock);
-    for (int hq = 0; hq < nh; ++hq) {
+    for (int hq = wv; hq < nh; hq += kHypWaves) {
         const int bb = blk * kHypPerBlock + hq;
         if (!sOk[hq]) {
             if (lane == 0) scores[bb] = -1.0;
@@ -236,7 +239,7 @@ __device__ __forceinline__ void hypotheses_body(const double* __restrict__ p1, c
     }
 }
 
-__global__ __launch_bounds__(64) void k_hypotheses(const double* __restrict__ p1, const double* __restrict__ p2, int N,
+__global__ __launch_bounds__(64 * kHypWaves) void k_hypotheses(const double* __restrict__ p1, const double* __restrict__ p2, int N,
                                                     int B, uint32_t k0, uint32_t k1, const double* __restrict__ T,
                                                     int method, double den, double tq, double* __restrict__ scores,
                                                     double* __restrict__ Hs) {
@@ -491,7 +494,7 @@ __global__ __launch_bounds__(64) void k_bnormalize(const RPair* __restrict__ rp,
     normalize_body(p1 + 2 * P.poff, p2 + 2 * P.poff, N, (double*)(ws + P.o_T));
 }
 
-__global__ __launch_bounds__(64) void k_bhypotheses(const RPair* __restrict__ rp, const double* __restrict__ p1,
+__global__ __launch_bounds__(64 * kHypWaves) void k_bhypotheses(const RPair* __restrict__ rp, const double* __restrict__ p1,
                                                      const double* __restrict__ p2, const int* __restrict__ npts, int B,
                                                      uint64_t seed, int method, double den, double tq,
                                                      char* __restrict__ ws) {
@@ -562,7 +565,7 @@ int ransac(const double* p1, const double* p2, int N, uint64_t seed, int B, doub
     const double tq = thr * thr;
     const double den = 2.0 * (sigma_max * sigma_max);
     hipLaunchKernelGGL(k_normalize, dim3(1), dim3(64), 0, s, p1, p2, N, T);
-    hipLaunchKernelGGL(k_hypotheses, dim3((B + kHypPerBlock - 1) / kHypPerBlock), dim3(64), 0, s, p1, p2, N, B,
+    hipLaunchKernelGGL(k_hypotheses, dim3((B + kHypPerBlock - 1) / kHypPerBlock), dim3(64 * kHypWaves), 0, s, p1, p2, N, B,
                        (uint32_t)seed, (uint32_t)(seed >> 32), (const double*)T, method, den, tq, scores, Hs);
     hipLaunchKernelGGL(k_argmax, dim3(1), dim3(1024), 0, s, (const double*)scores, B, &ints[0]);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, p1, p2, N, tq, lo, min_inl, (const double*)Hs,
@@ -632,7 +635,7 @@ static int ransac_batch_run(msift_ransac_batch* R, const double* p1, const doubl
     const double den = 2.0 * (sigma_max * sigma_max);
     const RPair* rp = R->d_pairs;
     hipLaunchKernelGGL(k_bnormalize, dim3(1, np), dim3(64), 0, s, rp, p1, p2, npts, ws);
-    hipLaunchKernelGGL(k_bhypotheses, dim3((B + kHypPerBlock - 1) / kHypPerBlock, np), dim3(64), 0, s, rp, p1, p2,
+    hipLaunchKernelGGL(k_bhypotheses, dim3((B + kHypPerBlock - 1) / kHypPerBlock, np), dim3(64 * kHypWaves), 0, s, rp, p1, p2,
                        npts, B, seed, method, den, tq, ws);
     hipLaunchKernelGGL(k_bargmax, dim3(1, np), dim3(1024), 0, s, rp, npts, B, ws);
     hipLaunchKernelGGL(k_bfinalize, dim3(1, np), dim3(64), 0, s, rp, p1, p2, npts, tq, lo, min_inl, ws, H, mask, ninl,
