@@ -17,7 +17,7 @@
 namespace msift {
 
 constexpr int kHypPerBlock = 64;
-constexpr int kHypWaves = 8;       // waves per hypothesis block: wave w scores hypotheses w, w + 8, ...
+constexpr int kScoreWaves = 16;    // scoring: one hypothesis per wave, 16 waves per block
 
 __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
@@ -174,60 +174,57 @@ __global__ __launch_bounds__(64) void k_normalize(const double* __restrict__ p1,
     normalize_body(p1, p2, N, T);
 }
 
-// hypotheses blk*64 .. blk*64+63: wave 0 solves them (one per lane), then the
-// block's kHypWaves waves score them, each hypothesis by one wave with lanes
-// over the points (the same sum per hypothesis whichever wave computes it)
-__device__ __forceinline__ void hypotheses_body(const double* __restrict__ p1, const double* __restrict__ p2, int N,
-                                                int B, uint32_t k0, uint32_t k1, const double* __restrict__ T,
-                                                int method, double den, double tq, double* __restrict__ scores,
-                                                double* __restrict__ Hs, int blk, double (*sH)[9], double (*sHi)[9],
-                                                int* sOk) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+// hypotheses blk*64 .. blk*64+63, one per lane: the minimal solve.  Hs[b]
+// gets H (the identity when the solve fails) and scores[b] -1 for a failed
+// solve, 0 otherwise (the scoring kernel then fills in the score).
+__device__ __forceinline__ void hyp_solve_body(const double* __restrict__ p1, const double* __restrict__ p2, int N,
+                                               int B, uint32_t k0, uint32_t k1, const double* __restrict__ T,
+                                               double* __restrict__ scores, double* __restrict__ Hs, int blk) {
+    const int lane = threadIdx.x & 63;
     const int b = blk * kHypPerBlock + lane;
     double T1[9], T2[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) { T1[k] = T[k]; T2[k] = T[9 + k]; }
-    if (wv == 0) {
-        // phase 1: one hypothesis per lane
-        uint32_t c[4] = { (uint32_t)b, 0u, 0u, 0u };
-        philox4x32_10(c, k0, k1);
-        double A[8][9];
+    uint32_t c[4] = { (uint32_t)b, 0u, 0u, 0u };
+    philox4x32_10(c, k0, k1);
+    double A[8][9];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            int i = (int)(((uint64_t)c[q] * (uint64_t)N) >> 32);
-            double x1 = T1[0] * p1[2 * i] + T1[2], y1 = T1[4] * p1[2 * i + 1] + T1[5];
-            double x2 = T2[0] * p2[2 * i] + T2[2], y2 = T2[4] * p2[2 * i + 1] + T2[5];
-            dlt_rows(x1, y1, x2, y2, A[2 * q], A[2 * q + 1]);
-        }
-        double h[9], H[9], Hi[9];
-        bool ok = solve8(A, h) && b < B;
-        if (ok) {
-            denorm(h, T1, T2, H);
-            inv3(H, Hi);
-        } else {
-#pragma unroll
-            for (int k = 0; k < 9; ++k) { H[k] = (k % 4 == 0) ? 1.0 : 0.0; Hi[k] = H[k]; }
-        }
-#pragma unroll
-        for (int k = 0; k < 9; ++k) { sH[lane][k] = H[k]; sHi[lane][k] = Hi[k]; }
-        sOk[lane] = ok;
-        if (b < B) {
-#pragma unroll
-            for (int k = 0; k < 9; ++k) Hs[(size_t)b * 9 + k] = H[k];
-        }
+    for (int q = 0; q < 4; ++q) {
+        int i = (int)(((uint64_t)c[q] * (uint64_t)N) >> 32);
+        double x1 = T1[0] * p1[2 * i] + T1[2], y1 = T1[4] * p1[2 * i + 1] + T1[5];
+        double x2 = T2[0] * p2[2 * i] + T2[2], y2 = T2[4] * p2[2 * i + 1] + T2[5];
+        dlt_rows(x1, y1, x2, y2, A[2 * q], A[2 * q + 1]);
     }
-    __syncthreads();
-    // phase 2: score each hypothesis with lanes over points
-    const int nh = min(kHypPerBlock, B - blk * kHypPerBlock);
-    for (int hq = wv; hq < nh; hq += kHypWaves) {
-        const int bb = blk * kHypPerBlock + hq;
-        if (!sOk[hq]) {
-            if (lane == 0) scores[bb] = -1.0;
-            continue;
-        }
+    double h[9], H[9];
+    const bool ok = solve8(A, h) && b < B;
+    if (ok) {
+        denorm(h, T1, T2, H);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) H[k] = (k % 4 == 0) ? 1.0 : 0.0;
+    }
+    if (b < B) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Hs[(size_t)b * 9 + k] = H[k];
+        scores[b] = ok ? 0.0 : -1.0;
+    }
+}
+
+// hypothesis blk * kScoreWaves + w scored by wave w, lanes over the points
+// (the same sum whichever wave computes it).  A kernel of its own with one
+// hypothesis per wave: its few registers and many waves keep the FP64 units
+// busy.
+__device__ __forceinline__ void hyp_score_body(const double* __restrict__ p1, const double* __restrict__ p2, int N,
+                                               int B, int method, double den, double tq,
+                                               double* __restrict__ scores, const double* __restrict__ Hs, int blk) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    {
+        const int bb = blk * kScoreWaves + wv;
+        if (bb >= B || scores[bb] < 0.0) return;   // failed solve: stays -1
         double H[9], Hi[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) { H[k] = sH[hq][k]; Hi[k] = sHi[hq][k]; }
+        for (int k = 0; k < 9; ++k) H[k] = Hs[(size_t)bb * 9 + k];
+        inv3(H, Hi);
         double part = 0.0;
         for (int i = lane; i < N; i += 64) {
             double e = sym_err(H, Hi, p1[2 * i], p1[2 * i + 1], p2[2 * i], p2[2 * i + 1], true);
@@ -239,13 +236,17 @@ __device__ __forceinline__ void hypotheses_body(const double* __restrict__ p1, c
     }
 }
 
-__global__ __launch_bounds__(64 * kHypWaves) void k_hypotheses(const double* __restrict__ p1, const double* __restrict__ p2, int N,
-                                                    int B, uint32_t k0, uint32_t k1, const double* __restrict__ T,
-                                                    int method, double den, double tq, double* __restrict__ scores,
-                                                    double* __restrict__ Hs) {
-    __shared__ double sH[kHypPerBlock][9], sHi[kHypPerBlock][9];
-    __shared__ int sOk[kHypPerBlock];
-    hypotheses_body(p1, p2, N, B, k0, k1, T, method, den, tq, scores, Hs, blockIdx.x, sH, sHi, sOk);
+__global__ __launch_bounds__(64) void k_hyp_solve(const double* __restrict__ p1, const double* __restrict__ p2, int N,
+                                                   int B, uint32_t k0, uint32_t k1, const double* __restrict__ T,
+                                                   double* __restrict__ scores, double* __restrict__ Hs) {
+    hyp_solve_body(p1, p2, N, B, k0, k1, T, scores, Hs, blockIdx.x);
+}
+
+__global__ __launch_bounds__(64 * kScoreWaves) void k_hyp_score(const double* __restrict__ p1,
+                                                               const double* __restrict__ p2, int N, int B, int method,
+                                                               double den, double tq, double* __restrict__ scores,
+                                                               const double* __restrict__ Hs) {
+    hyp_score_body(p1, p2, N, B, method, den, tq, scores, Hs, blockIdx.x);
 }
 
 // argmax with the smallest index on ties (a total order: any reduction order)
@@ -494,19 +495,27 @@ __global__ __launch_bounds__(64) void k_bnormalize(const RPair* __restrict__ rp,
     normalize_body(p1 + 2 * P.poff, p2 + 2 * P.poff, N, (double*)(ws + P.o_T));
 }
 
-__global__ __launch_bounds__(64 * kHypWaves) void k_bhypotheses(const RPair* __restrict__ rp, const double* __restrict__ p1,
-                                                     const double* __restrict__ p2, const int* __restrict__ npts, int B,
-                                                     uint64_t seed, int method, double den, double tq,
-                                                     char* __restrict__ ws) {
-    __shared__ double sH[kHypPerBlock][9], sHi[kHypPerBlock][9];
-    __shared__ int sOk[kHypPerBlock];
+__global__ __launch_bounds__(64) void k_bhyp_solve(const RPair* __restrict__ rp, const double* __restrict__ p1,
+                                                    const double* __restrict__ p2, const int* __restrict__ npts, int B,
+                                                    uint64_t seed, char* __restrict__ ws) {
     const RPair P = rp[blockIdx.y];
     const int N = pair_npts(P, npts);
     if (N < 4) return;
     const uint64_t key = seed + (uint64_t)blockIdx.y;
-    hypotheses_body(p1 + 2 * P.poff, p2 + 2 * P.poff, N, B, (uint32_t)key, (uint32_t)(key >> 32),
-                    (const double*)(ws + P.o_T), method, den, tq, (double*)(ws + P.o_scores), (double*)(ws + P.o_Hs),
-                    blockIdx.x, sH, sHi, sOk);
+    hyp_solve_body(p1 + 2 * P.poff, p2 + 2 * P.poff, N, B, (uint32_t)key, (uint32_t)(key >> 32),
+                   (const double*)(ws + P.o_T), (double*)(ws + P.o_scores), (double*)(ws + P.o_Hs), blockIdx.x);
+}
+
+__global__ __launch_bounds__(64 * kScoreWaves) void k_bhyp_score(const RPair* __restrict__ rp,
+                                                                const double* __restrict__ p1,
+                                                                const double* __restrict__ p2,
+                                                                const int* __restrict__ npts, int B, int method,
+                                                                double den, double tq, char* __restrict__ ws) {
+    const RPair P = rp[blockIdx.y];
+    const int N = pair_npts(P, npts);
+    if (N < 4) return;
+    hyp_score_body(p1 + 2 * P.poff, p2 + 2 * P.poff, N, B, method, den, tq, (double*)(ws + P.o_scores),
+                   (const double*)(ws + P.o_Hs), blockIdx.x);
 }
 
 __global__ __launch_bounds__(1024) void k_bargmax(const RPair* __restrict__ rp, const int* __restrict__ npts, int B,
@@ -565,8 +574,11 @@ int ransac(const double* p1, const double* p2, int N, uint64_t seed, int B, doub
     const double tq = thr * thr;
     const double den = 2.0 * (sigma_max * sigma_max);
     hipLaunchKernelGGL(k_normalize, dim3(1), dim3(64), 0, s, p1, p2, N, T);
-    hipLaunchKernelGGL(k_hypotheses, dim3((B + kHypPerBlock - 1) / kHypPerBlock), dim3(64 * kHypWaves), 0, s, p1, p2, N, B,
-                       (uint32_t)seed, (uint32_t)(seed >> 32), (const double*)T, method, den, tq, scores, Hs);
+    const unsigned nhb = (unsigned)((B + kHypPerBlock - 1) / kHypPerBlock);
+    hipLaunchKernelGGL(k_hyp_solve, dim3(nhb), dim3(64), 0, s, p1, p2, N, B, (uint32_t)seed, (uint32_t)(seed >> 32),
+                       (const double*)T, scores, Hs);
+    hipLaunchKernelGGL(k_hyp_score, dim3((unsigned)((B + kScoreWaves - 1) / kScoreWaves)), dim3(64 * kScoreWaves), 0, s,
+                       p1, p2, N, B, method, den, tq, scores, (const double*)Hs);
     hipLaunchKernelGGL(k_argmax, dim3(1), dim3(1024), 0, s, (const double*)scores, B, &ints[0]);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, p1, p2, N, tq, lo, min_inl, (const double*)Hs,
                        (const int*)&ints[0], idx, mask2, H, mask, ninl, best ? best : &ints[1]);
@@ -635,8 +647,10 @@ static int ransac_batch_run(msift_ransac_batch* R, const double* p1, const doubl
     const double den = 2.0 * (sigma_max * sigma_max);
     const RPair* rp = R->d_pairs;
     hipLaunchKernelGGL(k_bnormalize, dim3(1, np), dim3(64), 0, s, rp, p1, p2, npts, ws);
-    hipLaunchKernelGGL(k_bhypotheses, dim3((B + kHypPerBlock - 1) / kHypPerBlock, np), dim3(64 * kHypWaves), 0, s, rp, p1, p2,
-                       npts, B, seed, method, den, tq, ws);
+    const unsigned nhb = (unsigned)((B + kHypPerBlock - 1) / kHypPerBlock);
+    hipLaunchKernelGGL(k_bhyp_solve, dim3(nhb, np), dim3(64), 0, s, rp, p1, p2, npts, B, seed, ws);
+    hipLaunchKernelGGL(k_bhyp_score, dim3((unsigned)((B + kScoreWaves - 1) / kScoreWaves), np), dim3(64 * kScoreWaves), 0,
+                       s, rp, p1, p2, npts, B, method, den, tq, ws);
     hipLaunchKernelGGL(k_bargmax, dim3(1, np), dim3(1024), 0, s, rp, npts, B, ws);
     hipLaunchKernelGGL(k_bfinalize, dim3(1, np), dim3(64), 0, s, rp, p1, p2, npts, tq, lo, min_inl, ws, H, mask, ninl,
                        best);
