@@ -127,21 +127,48 @@ __device__ __forceinline__ double sym_err(const double* H, const double* Hi, dou
 }
 
 // Hartley normalisation over the points listed in idx[0..m) (core.py:2946-2957)
+// The lane's points are fetched kGather at a time (indices, then
+// coordinates, all loads in flight together) and summed in the same order
+// k = lane, lane + 64, ... as one at a time.
+constexpr int kGather = 8;
 __device__ void hartley_wave(const double* __restrict__ pts, const int* __restrict__ idx, int m, double* T) {
     const int lane = threadIdx.x & 63;
     double sx = 0.0, sy = 0.0;
-    for (int k = lane; k < m; k += 64) {
-        int i = idx ? idx[k] : k;
-        sx = sx + pts[2 * i];
-        sy = sy + pts[2 * i + 1];
+    for (int k0 = lane; k0 < m; k0 += 64 * kGather) {
+        int ii[kGather];
+        double xv[kGather], yv[kGather];
+#pragma unroll
+        for (int u = 0; u < kGather; ++u) {
+            const int k = k0 + 64 * u;
+            ii[u] = k < m ? (idx ? idx[k] : k) : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < kGather; ++u)
+            if (ii[u] >= 0) { xv[u] = pts[2 * ii[u]]; yv[u] = pts[2 * ii[u] + 1]; }
+#pragma unroll
+        for (int u = 0; u < kGather; ++u)
+            if (ii[u] >= 0) { sx = sx + xv[u]; sy = sy + yv[u]; }
     }
     double cx = wave_tree_d(sx) / (double)m;
     double cy = wave_tree_d(sy) / (double)m;
     double sd = 0.0;
-    for (int k = lane; k < m; k += 64) {
-        int i = idx ? idx[k] : k;
-        double dx = pts[2 * i] - cx, dy = pts[2 * i + 1] - cy;
-        sd = sd + sqrt(dx * dx + dy * dy);
+    for (int k0 = lane; k0 < m; k0 += 64 * kGather) {
+        int ii[kGather];
+        double xv[kGather], yv[kGather];
+#pragma unroll
+        for (int u = 0; u < kGather; ++u) {
+            const int k = k0 + 64 * u;
+            ii[u] = k < m ? (idx ? idx[k] : k) : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < kGather; ++u)
+            if (ii[u] >= 0) { xv[u] = pts[2 * ii[u]]; yv[u] = pts[2 * ii[u] + 1]; }
+#pragma unroll
+        for (int u = 0; u < kGather; ++u)
+            if (ii[u] >= 0) {
+                double dx = xv[u] - cx, dy = yv[u] - cy;
+                sd = sd + sqrt(dx * dx + dy * dy);
+            }
     }
     double md = wave_tree_d(sd) / (double)m;
     double s = 0x1.6a09e667f3bcdp+0 / (md + 1e-10);
@@ -362,20 +389,34 @@ __device__ void dlt_ls_wave(const double* __restrict__ p1, const double* __restr
     double part[45];
 #pragma unroll
     for (int e = 0; e < 45; ++e) part[e] = 0.0;
-    for (int k = lane; k < m; k += 64) {
-        int i = idx[k];
-        double x1 = T1[0] * p1[2 * i] + T1[2], y1 = T1[4] * p1[2 * i + 1] + T1[5];
-        double x2 = T2[0] * p2[2 * i] + T2[2], y2 = T2[4] * p2[2 * i + 1] + T2[5];
-        double re[9], ro[9];
-        dlt_rows(x1, y1, x2, y2, re, ro);
-        int e = 0;
+    // points fetched 4 at a time (as in hartley_wave), accumulated in order
+    for (int k0 = lane; k0 < m; k0 += 64 * 4) {
+        int ii[4];
+        double q[4][4];
 #pragma unroll
-        for (int a = 0; a < 9; ++a)
+        for (int u = 0; u < 4; ++u) ii[u] = k0 + 64 * u < m ? idx[k0 + 64 * u] : -1;
 #pragma unroll
-            for (int b = a; b < 9; ++b) {
-                part[e] = part[e] + (re[a] * re[b] + ro[a] * ro[b]);
-                e++;
+        for (int u = 0; u < 4; ++u)
+            if (ii[u] >= 0) {
+                q[u][0] = p1[2 * ii[u]]; q[u][1] = p1[2 * ii[u] + 1];
+                q[u][2] = p2[2 * ii[u]]; q[u][3] = p2[2 * ii[u] + 1];
             }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (ii[u] < 0) continue;
+            double x1 = T1[0] * q[u][0] + T1[2], y1 = T1[4] * q[u][1] + T1[5];
+            double x2 = T2[0] * q[u][2] + T2[2], y2 = T2[4] * q[u][3] + T2[5];
+            double re[9], ro[9];
+            dlt_rows(x1, y1, x2, y2, re, ro);
+            int e = 0;
+#pragma unroll
+            for (int a = 0; a < 9; ++a)
+#pragma unroll
+                for (int b = a; b < 9; ++b) {
+                    part[e] = part[e] + (re[a] * re[b] + ro[a] * ro[b]);
+                    e++;
+                }
+        }
     }
     int e = 0;
 #pragma unroll
