@@ -440,6 +440,36 @@ __device__ void dlt_ls_wave(const double* __restrict__ p1, const double* __restr
     __syncthreads();
 }
 
+// inlier flags e < tq of H over all N points into f (order-free: a byte per
+// point and an integer count), the coordinates of 4 points per lane fetched
+// together; returns the wave's count
+__device__ __forceinline__ int flag_inliers(const double* H, const double* Hi, const double* __restrict__ p1,
+                                            const double* __restrict__ p2, int N, double tq, bool clampz,
+                                            uint8_t* __restrict__ f) {
+    const int lane = threadIdx.x & 63;
+    int n = 0;
+    for (int i0 = lane; i0 < N; i0 += 64 * 4) {
+        double q[4][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + 64 * u;
+            if (i < N) { q[u][0] = p1[2 * i]; q[u][1] = p1[2 * i + 1]; q[u][2] = p2[2 * i]; q[u][3] = p2[2 * i + 1]; }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + 64 * u;
+            if (i < N) {
+                const uint8_t fl = sym_err(H, Hi, q[u][0], q[u][1], q[u][2], q[u][3], clampz) < tq;
+                f[i] = fl;
+                n += fl;
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) n += __shfl_xor(n, m);
+    return n;
+}
+
 __device__ __forceinline__ void finalize_body(const double* __restrict__ p1, const double* __restrict__ p2, int N,
                                               double tq, int lo, int min_inl, const double* __restrict__ Hs,
                                               const int* __restrict__ bestp, int* __restrict__ idx,
@@ -452,15 +482,7 @@ __device__ __forceinline__ void finalize_body(const double* __restrict__ p1, con
 #pragma unroll
     for (int k = 0; k < 9; ++k) Hb[k] = Hs[(size_t)best * 9 + k];
     inv3(Hb, Hbi);
-    int nb = 0;
-    for (int i = lane; i < N; i += 64) {
-        double e = sym_err(Hb, Hbi, p1[2 * i], p1[2 * i + 1], p2[2 * i], p2[2 * i + 1], true);
-        uint8_t f = e < tq;
-        mask[i] = f;
-        nb += f;
-    }
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) nb += __shfl_xor(nb, m);
+    const int nb = flag_inliers(Hb, Hbi, p1, p2, N, tq, true, mask);
     if (lane == 0) {
         for (int k = 0; k < 9; ++k) Hout[k] = Hb[k];
         *best_out = best;
@@ -475,15 +497,7 @@ __device__ __forceinline__ void finalize_body(const double* __restrict__ p1, con
 #pragma unroll
         for (int k = 0; k < 9; ++k) H1[k] = sH1[k];
         inv3(H1, H1i);
-        int n2 = 0;
-        for (int i = lane; i < N; i += 64) {
-            double e = sym_err(H1, H1i, p1[2 * i], p1[2 * i + 1], p2[2 * i], p2[2 * i + 1], false);
-            uint8_t f = e < tq;
-            mask2[i] = f;
-            n2 += f;
-        }
-#pragma unroll
-        for (int mm = 32; mm >= 1; mm >>= 1) n2 += __shfl_xor(n2, mm);
+        const int n2 = flag_inliers(H1, H1i, p1, p2, N, tq, false, mask2);
         __syncthreads();
         if (n2 >= min_inl) {
             int m2 = compact_wave(mask2, N, idx);
