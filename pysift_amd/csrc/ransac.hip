@@ -305,15 +305,25 @@ __global__ __launch_bounds__(1024) void k_argmax(const double* __restrict__ scor
 }
 
 // compacts mask (in index order) into idx, returns the count (all lanes)
+// (the flags of 8 chunks of 64 are loaded together, then compacted chunk by
+// chunk in order)
 __device__ int compact_wave(const uint8_t* __restrict__ mask, int N, int* __restrict__ idx) {
     const int lane = threadIdx.x & 63;
     int base = 0;
-    for (int i0 = 0; i0 < N; i0 += 64) {
-        int i = i0 + lane;
-        bool f = i < N && mask[i];
-        unsigned long long m = __ballot(f);
-        if (f) idx[base + __popcll(m & lanemask_lt())] = i;
-        base += __popcll(m);
+    for (int c0 = 0; c0 < N; c0 += 64 * 8) {
+        bool fl[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = c0 + 64 * u + lane;
+            fl[u] = i < N && mask[i];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = c0 + 64 * u + lane;
+            const unsigned long long m = __ballot(fl[u]);
+            if (fl[u]) idx[base + __popcll(m & lanemask_lt())] = i;
+            base += __popcll(m);
+        }
     }
     return base;
 }
