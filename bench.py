@@ -197,7 +197,7 @@ def bench_batch(torch, dev, n=64, iters=5):
 def bench_pairs(torch, dev, n_pairs=32, n_hyp=1024, iters=5):
     """32 homography-warped pairs at the uneven batch sizes: the 64 images are
     detected by one batched graph replay, then every pair is matched in one
-    launch (exact INT8 MFMA, hi/lo split) and verified by batched RANSAC
+    launch (exact INT8 MFMA, shift method: a - 128 as int8) and verified by batched RANSAC
     (1024 Philox hypotheses per pair, 3 px), against the per-pair loop
     (match_u8 + RANSAC per pair) on the same descriptors."""
     import numpy as np

@@ -308,7 +308,32 @@ def warp_feather(img1, img2, H, cw, ch, ox, oy):
     return pano, valid.astype(bool)
 
 
+def autocrop(pano, both):
+    """_autocrop (core.py:3627-3684): the bounding box of the valid pixels,
+    then inside it the column range [max first-valid, min last-valid] over the
+    rows and the row range [max first-valid, min last-valid] over the columns
+    (an all-invalid row or column counts as first 0 and last n-1, as NumPy's
+    argmax of an all-False line gives 0); a range that is empty or one wide
+    falls back to the bounding box; no valid pixel returns pano unchanged."""
+    both = np.asarray(both, bool)
+    rows = np.flatnonzero(both.any(1))
+    cols = np.flatnonzero(both.any(0))
+    if rows.size == 0:
+        return pano
+    r0, r1, c0, c1 = int(rows[0]), int(rows[-1]), int(cols[0]), int(cols[-1])
+    sub = both[r0:r1 + 1, c0:c1 + 1]
+    nr, nc = sub.shape
+    ic0 = int(np.argmax(sub, axis=1).max())
+    ic1 = int((nc - 1 - np.argmax(sub[:, ::-1], axis=1)).min())
+    ir0 = int(np.argmax(sub, axis=0).max())
+    ir1 = int((nr - 1 - np.argmax(sub[::-1, :], axis=0)).min())
+    if ic0 >= ic1 or ir0 >= ir1:
+        return pano[r0:r1 + 1, c0:c1 + 1]
+    return pano[r0 + ir0:r0 + ir1 + 1, c0 + ic0:c0 + ic1 + 1]
+
+
 def bgr_to_gray(bgr):
+    """cv2.cvtColor(BGR2GRAY) (core.py:2548): OpenCV's fixed-point BT.601."""
     bgr = np.ascontiguousarray(bgr, np.uint8)
     g = np.empty(bgr.shape[:2], np.uint8)
     lib().or_bgr_to_gray(_ptr(bgr), bgr.shape[0], bgr.shape[1], _ptr(g))

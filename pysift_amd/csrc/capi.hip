@@ -193,6 +193,12 @@ int msift_ransac_homography(const double* d_p1, const double* d_p2, int N, uint6
         set_error("msift_ransac_homography: bad arguments");
         return MSIFT_EINVAL;
     }
+    if (lo && min_inliers < 4) {
+        /* the LO refit solves a DLT on the inliers, which needs 4 points
+           (_compute_homography asserts it, core.py:2959-2961) */
+        set_error("msift_ransac_homography: min_inliers must be >= 4 when lo is set");
+        return MSIFT_EINVAL;
+    }
     return ransac(d_p1, d_p2, N, seed, n_hyp, thr_px, sigma_max, method, lo, min_inliers, d_ws, ws_bytes, d_H,
                   d_mask, d_ninl, d_best, (hipStream_t)stream);
 }

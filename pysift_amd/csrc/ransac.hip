@@ -753,6 +753,10 @@ int msift_ransac_homography_batched(msift_ransac_batch* plan, const double* d_p1
                                     int lo, int min_inliers, void* d_ws, size_t ws_bytes, double* d_H,
                                     uint8_t* d_mask, int* d_ninl, int* d_best, void* stream) {
     if (!plan) { set_error("null plan"); return MSIFT_EINVAL; }
+    if (lo && min_inliers < 4) {
+        set_error("msift_ransac_homography_batched: min_inliers must be >= 4 when lo is set");
+        return MSIFT_EINVAL;
+    }
     return ransac_batch_run(plan, d_p1, d_p2, d_npts, seed, thr_px, sigma_max, method, lo, min_inliers, d_ws,
                             ws_bytes, d_H, d_mask, d_ninl, d_best, (hipStream_t)stream);
 }

@@ -161,6 +161,10 @@ class PySIFT:
         buffers with room for `capacity` keypoints.  Returns those buffers;
         the count stays on the device."""
         torch = _torch()
+        if img.dtype != torch.uint8 or img.dim() != 2 or not img.is_cuda:
+            raise ValueError("img must be a 2-D uint8 CUDA tensor")
+        if img.stride(1) != 1:
+            raise ValueError("img rows must be contiguous (stride(1) == 1); pass img.contiguous()")
         H, W = img.shape
         dev = img.device
         self._ensure_buffers(H, W, dev)

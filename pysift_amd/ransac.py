@@ -81,6 +81,17 @@ class RansacBatchPlan:
 
     def run(self, p1, p2, npts, seed=42, thr=5.0, sigma_max=None, method="magsac", lo=True, min_inliers=10):
         """p1, p2: (sum caps, 2) float64 device tensors; npts: (n_pairs,) int32 device."""
+        import torch
+        need = max((o + c for o, c in zip(self.offsets, self.caps)), default=0)
+        for name, t in (("p1", p1), ("p2", p2)):
+            if t.dtype != torch.float64 or t.dim() != 2 or t.shape[1] != 2 or t.shape[0] < need:
+                raise ValueError(f"{name} must be a ({need}, 2) float64 tensor (got {tuple(t.shape)} {t.dtype})")
+            if t.device != self.device:
+                raise ValueError(f"{name} is on {t.device}, the plan on {self.device}")
+        if npts.dtype != torch.int32 or npts.dim() != 1 or npts.numel() != self.np or npts.device != self.device:
+            raise ValueError(f"npts must be a ({self.np},) int32 tensor on {self.device}")
+        if lo and min_inliers < 4:
+            raise ValueError("min_inliers must be >= 4 when lo is set")
         sm = thr if sigma_max is None else sigma_max
         rc = _lib.lib().msift_ransac_homography_batched(
             self._p, _lib.ptr(p1.contiguous()), _lib.ptr(p2.contiguous()), _lib.ptr(npts), int(seed) & 0xFFFFFFFFFFFFFFFF,

@@ -41,27 +41,6 @@ def compute_canvas_size(h1, w1, h2, w2, H):
     return int(np.ceil(ax.max() + ox)) + 1, int(np.ceil(ay.max() + oy)) + 1, ox, oy
 
 
-def autocrop(pano, both):
-    """_autocrop (core.py:3627-3684): largest inner rectangle of valid pixels."""
-    row_valid = np.any(both, axis=1)
-    col_valid = np.any(both, axis=0)
-    if not row_valid.any():
-        return pano
-    r0 = int(np.where(row_valid)[0][0]); r1 = int(np.where(row_valid)[0][-1])
-    c0 = int(np.where(col_valid)[0][0]); c1 = int(np.where(col_valid)[0][-1])
-    sub = both[r0:r1 + 1, c0:c1 + 1]
-    nr, nc = sub.shape
-    left = np.argmax(sub, axis=1)
-    right = nc - 1 - np.argmax(sub[:, ::-1], axis=1)
-    top = np.argmax(sub, axis=0)
-    bot = nr - 1 - np.argmax(sub[::-1, :], axis=0)
-    ic0, ic1 = int(left.max()), int(right.min())
-    ir0, ir1 = int(top.max()), int(bot.min())
-    if ic0 >= ic1 or ir0 >= ir1:
-        return pano[r0:r1 + 1, c0:c1 + 1]
-    return pano[r0 + ir0:r0 + ir1 + 1, c0 + ic0:c0 + ic1 + 1]
-
-
 def autocrop_box(both_d):
     """autocrop's rectangle from a device validity mask, computed on the
     device (the same first/last rules: argmax gives the first maximum):
@@ -126,6 +105,13 @@ class GPUPyStitch:
         self.output_quality = output_quality
         if match_method not in ("hilo", "shift", "float"):
             raise ValueError(f"match_method must be 'hilo', 'shift' or 'float' (got {match_method!r})")
+        # int8_matching (core.py:2894-2910) asks for INT8 GEMM matching.  The
+        # integer matchers ('shift', 'hilo') are INT8 MFMA GEMMs and exact, so
+        # the flag selects them; the reference's per-row-scaled approximate
+        # INT8 similarity on the PCA/float path is not reproduced.
+        if int8_matching and match_method == "float":
+            raise NotImplementedError("int8_matching with match_method='float' (the approximate per-row-scaled INT8 "
+                                      "similarity) is not implemented; match_method='shift' is exact INT8 MFMA")
         self.match_method = match_method
         self._pca = None   # (mean, proj) of the float path, fitted on the first pair (core.py:2729-2736)
         self.ransac_seed = int(ransac_seed)

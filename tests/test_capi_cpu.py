@@ -52,3 +52,23 @@ def test_pysift_constructor_errors():
     from pysift_amd import PySIFT
     with pytest.raises(ValueError):
         PySIFT(dsp_n_scales=7)
+
+
+def test_ransac_rejects_lo_with_fewer_than_four_inliers():
+    # the LO refit solves a DLT on the inliers; min_inliers < 4 would hand it
+    # 0-3 points (the reference's _compute_homography asserts >= 4)
+    from pysift_amd import _lib
+    L = _lib.lib()
+    rc = L.msift_ransac_homography(None, None, 10, 42, 16, 3.0, 3.0, 0, 1, 2, None, 0, None, None, None, None, None)
+    assert rc == _lib.MSIFT_EINVAL
+    assert "min_inliers" in _lib.last_error()
+    rc = L.msift_ransac_homography(None, None, 3, 42, 16, 3.0, 3.0, 0, 1, 10, None, 0, None, None, None, None, None)
+    assert rc == _lib.MSIFT_EINVAL and "4 correspondences" in _lib.last_error()
+
+
+def test_stitch_int8_matching_mapping():
+    # int8_matching selects the exact INT8 matchers; the approximate float-path
+    # INT8 similarity is refused rather than silently ignored
+    from pysift_amd.stitch import GPUPyStitch
+    with pytest.raises(NotImplementedError):
+        GPUPyStitch(int8_matching=True, match_method="float", verbose=False, save_output=False)

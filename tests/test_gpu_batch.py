@@ -9,10 +9,17 @@ from pysift_amd import synthetic as syn
 pytestmark = pytest.mark.gpu
 
 
+def _low_contrast(img, k=0.1):
+    return np.clip(np.round(128 + k * (img.astype(np.float64) - 128)), 0, 255).astype(np.uint8)
+
+
 def _images():
-    sizes = [(240, 320), (301, 417), (150, 151), (480, 640), (97, 203), (200, 300)]
+    sizes = [(240, 320), (301, 417), (150, 151), (480, 640), (97, 203), (200, 300), (240, 320)]
     imgs = [syn.synth_gray(h, w, 40 + i) for i, (h, w) in enumerate(sizes)]
-    imgs[4] = np.full((97, 203), 128, np.uint8)          # flat: 0 keypoints, retry path
+    imgs[4] = np.full((97, 203), 128, np.uint8)          # flat: 0 keypoints in both passes
+    # low contrast: under 100 keypoints at ct, the retry at ct/2 finds ~790,
+    # in the same batch as images that keep their first pass
+    imgs[6] = _low_contrast(syn.synth_gray(240, 320, 3))
     return imgs
 
 
@@ -32,9 +39,13 @@ def test_batch_matches_single_and_oracle():
         np.testing.assert_array_equal(res[i][0].cpu().numpy(), kp.cpu().numpy())
         np.testing.assert_array_equal(res[i][1].cpu().numpy(), df.cpu().numpy())
         np.testing.assert_array_equal(res_u8[i][1].cpu().numpy(), du.cpu().numpy())
-    rk, rd, ru = O.detect_describe(imgs[0], O.params())
-    np.testing.assert_array_equal(res[0][0].cpu().numpy(), rk)
+    for i in (0, 6):
+        rk, rd, ru = O.detect_describe(imgs[i], O.params())
+        np.testing.assert_array_equal(res[i][0].cpu().numpy(), rk)
+        np.testing.assert_array_equal(res[i][1].cpu().numpy(), rd)
+        np.testing.assert_array_equal(res_u8[i][1].cpu().numpy(), ru)
     assert res[4][0].shape[0] == 0
+    assert res[6][0].shape[0] >= 100
 
 
 def test_batch_graph_replay_and_tiles_identical():
@@ -71,3 +82,23 @@ def test_batch_non_doubled_and_caps():
         kp, df, _ = s.detect_raw(img)
         np.testing.assert_array_equal(k.cpu().numpy(), kp.cpu().numpy())
         np.testing.assert_array_equal(d.cpu().numpy(), df.cpu().numpy())
+
+
+@pytest.mark.parametrize("cap", [50, 20])
+def test_batch_candidate_cap_overflow(cap):
+    # raster-order cap overflow in the batched extrema write, beside an image
+    # whose capped first pass triggers the retry (cap 20) and one that does not
+    from pysift_amd import PySIFT
+    from pysift_amd.batch import BatchPlan
+    imgs = [syn.synth_gray(240, 320, 3), syn.synth_gray(301, 417, 41), _low_contrast(syn.synth_gray(200, 300, 5))]
+    s = PySIFT(max_cands_per_octave=cap)
+    plan = BatchPlan(s, [i.shape for i in imgs])
+    plan.pack(imgs)
+    plan.run(graph=False)
+    res, res_u8 = plan.results(), plan.results(u8=True)
+    for i, img in enumerate(imgs):
+        rk, rd, ru = O.detect_describe(img, O.params(max_cands_per_octave=cap))
+        print(f"[case] batch cap {cap} image {i} {img.shape}: oracle {len(rk)} kp", flush=True)
+        np.testing.assert_array_equal(res[i][0].cpu().numpy(), rk)
+        np.testing.assert_array_equal(res[i][1].cpu().numpy(), rd)
+        np.testing.assert_array_equal(res_u8[i][1].cpu().numpy(), ru)

@@ -73,12 +73,11 @@ def test_composite_without_overlap():
 
 def test_stitch_multiband_matches_oracle():
     from pysift_amd import GPUPyStitch
-    from pysift_amd.stitch import autocrop
     va, vb, Hgt = syn.synth_pair(seed=1000, view_hw=(384, 512), canvas_hw=(768, 1024))
     st = GPUPyStitch(inlier_thresh=3.0, ransac_iters=2000, verbose=False, save_output=False, blend="multiband")
     pano = st.stitch(va, vb)
     H = st.last_homography
     cw, ch, ox, oy = O.canvas(384, 512, 384, 512, H)
     rp, rv, _, _, rr, _ = O.composite(va, vb, H, cw, ch, ox, oy, 6)
-    np.testing.assert_array_equal(pano, autocrop(rp, rv))
+    np.testing.assert_array_equal(pano, O.autocrop(rp, rv))
     assert st.last_metrics["seam_rmse"] == rr

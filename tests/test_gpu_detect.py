@@ -82,6 +82,29 @@ def test_detect_bitwise_shapes(H, W, kw):
     _compare(img, _sift(**kw), f"{H}x{W} {kw}")
 
 
+def _low_contrast(img, k=0.1):
+    """Contrast scaled by k about 128: below 100 keypoints at the adaptive
+    threshold, so the ct/2 retry (core.py:2064-2070) is what finds them."""
+    return np.clip(np.round(128 + k * (img.astype(np.float64) - 128)), 0, 255).astype(np.uint8)
+
+
+def test_detect_retry_finds_keypoints():
+    img = _low_contrast(syn.synth_gray(240, 320, 3))
+    first, _, _ = O.detect_describe(img, O.params(retry_below=0))
+    assert len(first) < 100                     # the first pass alone gives too few
+    n = _compare(img, _sift(), "low contrast, ct/2 retry")
+    assert n >= 100
+
+
+@pytest.mark.parametrize("cap", [50, 20])
+def test_detect_candidate_cap_overflow(cap):
+    # the per-octave candidate cap keeps the first `cap` in raster order; at
+    # 20 the capped first pass falls below 100 keypoints and the retry runs too
+    img = syn.synth_gray(240, 320, 3)
+    n = _compare(img, _sift(max_cands_per_octave=cap), f"max_cands_per_octave={cap}")
+    assert 0 < n < 818
+
+
 def test_detect_flat_image_is_empty():
     img = np.full((200, 300), 77, np.uint8)
     n = _compare(img, _sift(), "flat")

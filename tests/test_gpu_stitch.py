@@ -54,7 +54,7 @@ def _oracle_stitch(va, vb, thr, n_hyp, max_kp=8000):
 
 def test_stitch_pair_matches_oracle_and_ground_truth():
     from pysift_amd import GPUPyStitch
-    from pysift_amd.stitch import autocrop, compute_canvas_size
+    from pysift_amd.stitch import compute_canvas_size
     va, vb, Hgt = syn.synth_pair(seed=1000, view_hw=(384, 512), canvas_hw=(768, 1024))
     st = GPUPyStitch(inlier_thresh=3.0, ransac_iters=2000, verbose=False, save_output=False)
     pano = st.stitch(va, vb)
@@ -70,7 +70,7 @@ def test_stitch_pair_matches_oracle_and_ground_truth():
     assert err < 6.0, err
     cw, ch, ox, oy = compute_canvas_size(384, 512, 384, 512, rH)
     rp, rv = O.warp_feather(va, vb, rH, cw, ch, ox, oy)
-    np.testing.assert_array_equal(pano, autocrop(rp, rv))
+    np.testing.assert_array_equal(pano, O.autocrop(rp, rv))
 
 
 def test_stitch_errors():
@@ -84,11 +84,11 @@ def test_stitch_errors():
 
 
 def test_autocrop_box_matches_host_autocrop():
-    # the device rectangle against the host restatement of _autocrop
+    # the device rectangle against the oracle's restatement of _autocrop
     # (core.py:3627-3684) on masks with ragged edges, holes, one valid row or
     # column, an all-valid and an empty mask
     import torch
-    from pysift_amd.stitch import autocrop, autocrop_box
+    from pysift_amd.stitch import autocrop_box
     rng = np.random.default_rng(7)
     masks = []
     for _ in range(6):
@@ -109,4 +109,4 @@ def test_autocrop_box_matches_host_autocrop():
         pano = np.arange(m.size * 3, dtype=np.int64).reshape(m.shape + (3,))
         box = autocrop_box(torch.from_numpy(m.astype(np.uint8)).cuda())
         got = pano if box is None else pano[box[0]:box[1], box[2]:box[3]]
-        np.testing.assert_array_equal(got, autocrop(pano, m))
+        np.testing.assert_array_equal(got, O.autocrop(pano, m))
