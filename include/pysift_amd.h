@@ -146,6 +146,21 @@ int msift_match_u8(const uint8_t* d_a, int N, const uint8_t* d_b, int M, double 
                    void* d_ws, size_t ws_bytes, int32_t* d_fwd, int32_t* d_bwd,
                    int32_t* d_pairs, int* d_npairs, void* stream);
 
+/* Allocation-free variant of msift_match_u8 (no counterpart in the
+ * reference: it is for callers that cannot allocate device memory, such as
+ * code under HIP graph capture or a serving loop that forbids hipMalloc).
+ * Same arguments and outputs as msift_match_u8 without the workspace: the
+ * scratch (norms, partial top-2 keys, block counts) lives in a static device
+ * array of 12.8 MB per GPU, held from the first call on that GPU until the
+ * process ends, which limits N and M to 16384 each (larger sizes return
+ * MSIFT_EINVAL).  The stream must belong to the current device.  Every call
+ * on one GPU shares that GPU's scratch, so calls that overlap on one GPU
+ * (concurrent streams or threads) race; serialise them.  Nothing is
+ * allocated and no scratch needs clearing: every call writes all of the
+ * scratch it reads. */
+int msift_match_u8_noalloc(const uint8_t* d_a, int N, const uint8_t* d_b, int M, double ratio, int method,
+                           int32_t* d_fwd, int32_t* d_bwd, int32_t* d_pairs, int* d_npairs, void* stream);
+
 /* Batched matcher (no reference counterpart: the reference matches one pair
  * per call, core.py:2840-2940): n_pairs descriptor-set pairs (a_p: N[p] rows,
  * b_p: M[p] rows of 128 B), packed back to back at the row offsets that

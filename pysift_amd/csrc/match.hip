@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cstring>
 #include <vector>
+#include <mutex>
 
 #pragma clang fp contract(off)
 
@@ -772,16 +773,85 @@ static MatchPlan match_plan(int N, int M) {
 
 size_t match_workspace(int N, int M) { return match_plan(N, M).total; }
 
+// The integer path's share of the workspace: everything before the fp16
+// copies, which only the float path uses.
+static size_t match_int_bytes(const MatchPlan& P) { return P.o_ha; }
+
+static int match_u8_run(const uint8_t* a, int N, const uint8_t* b, int M, double ratio, int method, char* ws,
+                        const MatchPlan& P, int32_t* fwd, int32_t* bwd, int32_t* pairs, int* npairs, hipStream_t s);
+
 int match_u8(const uint8_t* a, int N, const uint8_t* b, int M, double ratio, int method, void* ws_, size_t ws_bytes,
              int32_t* fwd, int32_t* bwd, int32_t* pairs, int* npairs, hipStream_t s) {
     MatchPlan P = match_plan(N, M);
     if (ws_bytes < P.total) { set_error("match_u8: workspace too small"); return MSIFT_EWORKSPACE; }
+    return match_u8_run(a, N, b, M, ratio, method, (char*)ws_, P, fwd, bwd, pairs, npairs, s);
+}
+
+// ------------------------------------------------ allocation-free variant ----
+// Scratch of the integer matcher for N, M <= kNoAllocMaxRows, in a file-scope
+// device array: every GPU the library runs on holds its own copy until the
+// process ends.  Host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device (the array's name is a
+// host-side stand-in, not a device address).
+constexpr int kNoAllocMaxRows = 16384;
+constexpr size_t rup256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kNoAllocScratchBytes =
+    rup256((size_t)kNoAllocMaxRows * 4) * 2 +
+    rup256((size_t)((kNoAllocMaxRows + kColChunk - 1) / kColChunk) * kNoAllocMaxRows * 8) +
+    rup256((size_t)((kNoAllocMaxRows + kRowBlk - 1) / kRowBlk) * kNoAllocMaxRows * 8) +
+    rup256((size_t)((kNoAllocMaxRows + 255) / 256 + 1) * 4) * 2;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];
+
+// Device address of `scratch` on the current device, looked up once per device.
+static int scratch_on_current_device(char** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        set_error("match_u8_noalloc: no current HIP device");
+        return MSIFT_EHIP;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("match_u8_noalloc: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+            return MSIFT_EHIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = (char*)g_scratch_ptr[dev];
+    return MSIFT_OK;
+}
+
+int match_u8_noalloc(const uint8_t* a, int N, const uint8_t* b, int M, double ratio, int method, int32_t* fwd,
+                     int32_t* bwd, int32_t* pairs, int* npairs, hipStream_t s) {
+    if (N > kNoAllocMaxRows || M > kNoAllocMaxRows) {
+        set_error("match_u8_noalloc: N and M must be <= " + std::to_string(kNoAllocMaxRows) + " (got " +
+                  std::to_string(N) + " x " + std::to_string(M) + "); use msift_match_u8 with a workspace");
+        return MSIFT_EINVAL;
+    }
+    const MatchPlan P = match_plan(N, M);
+    if (match_int_bytes(P) > kNoAllocScratchBytes) {   // cannot happen within the row limit
+        set_error("match_u8_noalloc: scratch too small");
+        return MSIFT_EWORKSPACE;
+    }
+    char* ws = nullptr;
+    const int rc = scratch_on_current_device(&ws);
+    if (rc != MSIFT_OK) return rc;
+    return match_u8_run(a, N, b, M, ratio, method, ws, P, fwd, bwd, pairs, npairs, s);
+}
+
+static int match_u8_run(const uint8_t* a, int N, const uint8_t* b, int M, double ratio, int method, char* ws,
+                        const MatchPlan& P, int32_t* fwd, int32_t* bwd, int32_t* pairs, int* npairs, hipStream_t s) {
     if ((N > 0 && (!a || !fwd)) || (M > 0 && (!b || !bwd)) || (!pairs && N > 0 && M > 0)) {
         set_error("match_u8: null buffer");
         return MSIFT_EINVAL;
     }
     if (((uintptr_t)a & 15) || ((uintptr_t)b & 15)) { set_error("match_u8: descriptors must be 16-byte aligned"); return MSIFT_EINVAL; }
-    char* ws = (char*)ws_;
     int* na = (int*)(ws + P.o_na);
     int* nb = (int*)(ws + P.o_nb);
     uint32_t* rowpart = (uint32_t*)(ws + P.o_rowpart);
