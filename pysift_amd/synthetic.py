@@ -5,6 +5,12 @@ blobs (sigma 2-20 px, amplitude 30-120, random sign), random axis-aligned and
 rotated rectangles of random intensity, one 8-px checker patch, N(0,3) noise,
 clipped to 0..255 and rounded.  Shape counts scale with image area so that
 large images keep the 640x480 feature density.
+
+Workloads that name a keypoint density (2k-8k keypoints per image for the
+uneven batch and its pairs, 20k-40k per 12 MP image) add fine texture: small
+Gaussian blobs at a fixed count per megapixel (FINE_BATCH, FINE_12MP), drawn
+from their own random stream after the rest of the content, so images without
+it are unchanged.
 """
 import math
 
@@ -12,9 +18,49 @@ import numpy as np
 
 _BASE_AREA = 640 * 480
 
+# fine texture (blobs per MP, sigma range px, amplitude range) of the workloads
+# that name a keypoint density
+FINE_BATCH = (50000, (0.8, 2.0), (15.0, 60.0))
+FINE_12MP = (2000, (1.0, 3.0), (15.0, 60.0))
 
-def synth_gray(H, W, seed, n_blobs=None, n_rects=None, checker=128, as_float=False):
-    """uint8 (H, W) image (or float64 before quantisation when as_float)."""
+
+def _fine_texture(img, seed, per_mp, sigma, amp, n_sigma=6):
+    """Adds per_mp * area/1e6 small Gaussian blobs (peak amplitude ~U(amp),
+    random sign, sigma ~U(sigma) quantised to n_sigma values) to img in place:
+    each blob is an impulse of mass amplitude * 2 pi sigma^2 split bilinearly
+    over its four nearest pixels, and each sigma group is one Gaussian filter
+    of its impulse image."""
+    from scipy.ndimage import gaussian_filter
+    H, W = img.shape
+    n = int(round(per_mp * H * W / 1e6))
+    if n <= 0:
+        return img
+    rng = np.random.default_rng(np.random.SeedSequence([int(seed), 0xF17E]))
+    cy = rng.uniform(0, H - 1, n)
+    cx = rng.uniform(0, W - 1, n)
+    levels = np.linspace(sigma[0], sigma[1], n_sigma)
+    g = rng.integers(0, n_sigma, n)
+    am = rng.uniform(amp[0], amp[1], n) * rng.choice([-1.0, 1.0], n)
+    y0 = np.minimum(np.floor(cy).astype(np.int64), H - 2)
+    x0 = np.minimum(np.floor(cx).astype(np.int64), W - 2)
+    fy, fx = cy - y0, cx - x0
+    for k, sg in enumerate(levels):
+        sel = g == k
+        if not sel.any():
+            continue
+        m = am[sel] * (2.0 * math.pi * sg * sg)
+        imp = np.zeros((H, W), np.float64)
+        for dy, wy in ((0, 1.0 - fy[sel]), (1, fy[sel])):
+            for dx, wx in ((0, 1.0 - fx[sel]), (1, fx[sel])):
+                np.add.at(imp, (y0[sel] + dy, x0[sel] + dx), m * wy * wx)
+        img += gaussian_filter(imp, sg, mode="constant", truncate=4.0)
+    return img
+
+
+def synth_gray(H, W, seed, n_blobs=None, n_rects=None, checker=128, as_float=False, fine=None):
+    """uint8 (H, W) image (or float64 before quantisation when as_float).
+    fine: optional (blobs per MP, sigma range, amplitude range) of fine
+    texture (FINE_BATCH, FINE_12MP)."""
     rng = np.random.default_rng(seed)
     scale = (H * W) / _BASE_AREA
     nb = int(round(200 * scale)) if n_blobs is None else n_blobs
@@ -57,6 +103,8 @@ def synth_gray(H, W, seed, n_blobs=None, n_rects=None, checker=128, as_float=Fal
         py = int(rng.integers(0, H - cs + 1)); px = int(rng.integers(0, W - cs + 1))
         yy, xx = np.mgrid[0:cs, 0:cs]
         img[py:py + cs, px:px + cs] = np.where(((yy // 8) + (xx // 8)) % 2 == 0, 40.0, 215.0)
+    if fine is not None and H > 1 and W > 1:
+        _fine_texture(img, seed, *fine)
     img += rng.normal(0.0, 3.0, (H, W))
     img = np.clip(img, 0, 255)
     if as_float:
@@ -131,9 +179,10 @@ def synth_batch_sizes(n=64, seed=2000, lo=(240, 320), hi=(1080, 1920)):
     return [(int(a), int(b)) for a, b in zip(Hs, Ws)]
 
 
-def synth_batch(n=64, seed=2000):
-    """Uneven batch: sizes uniform in 320x240 .. 1920x1080."""
-    return [synth_gray(h, w, seed + 1 + i) for i, (h, w) in enumerate(synth_batch_sizes(n, seed))]
+def synth_batch(n=64, seed=2000, fine=FINE_BATCH):
+    """Uneven batch: sizes uniform in 320x240 .. 1920x1080, with the fine
+    texture that gives 2k-8k keypoints per image."""
+    return [synth_gray(h, w, seed + 1 + i, fine=fine) for i, (h, w) in enumerate(synth_batch_sizes(n, seed))]
 
 
 def corner_error(H_est, H_gt, w, h):
@@ -168,13 +217,13 @@ def random_keypoints(B, K, H, W, n_oct=5, S=3, seed=3100, margin=2, max_octave=N
     return out
 
 
-def synth_warped_pair(h, w, seed, content_seed=None):
+def synth_warped_pair(h, w, seed, content_seed=None, fine=None):
     """(A, B, H_gt) uint8 grayscale: A = synth_gray(h, w), B = A warped about
     its centre by a random homography (rotation +-15 deg, scale 0.8-1.25,
     perspective ~U(-2e-4, 2e-4) scaled to the image size, shift ~U(-10 %, 10 %)),
     bilinear, zeros outside; H_gt maps A pixel coordinates to B's."""
     rng = np.random.default_rng(seed)
-    a = synth_gray(h, w, seed + 1000 if content_seed is None else content_seed, as_float=True)
+    a = synth_gray(h, w, seed + 1000 if content_seed is None else content_seed, as_float=True, fine=fine)
     th = math.radians(rng.uniform(-15, 15))
     sc = rng.uniform(0.8, 1.25)
     px, py = rng.uniform(-2e-4, 2e-4, 2) * (1024.0 / max(h, w))
@@ -196,8 +245,15 @@ def synth_warped_pair(h, w, seed, content_seed=None):
     return np.round(a).astype(np.uint8), np.round(b).astype(np.uint8), Hgt / Hgt[2, 2]
 
 
-def synth_pairs_batch(n_pairs=32, seed=2100):
-    """n_pairs warped pairs (synth_warped_pair) at the uneven batch sizes."""
+def synth_pairs_batch(n_pairs=32, seed=2100, fine=FINE_BATCH):
+    """n_pairs warped pairs (synth_warped_pair) at the uneven batch sizes,
+    with the batch's fine texture."""
     sizes = synth_batch_sizes(2 * n_pairs, seed)
-    return [synth_warped_pair(sizes[2 * p][0], sizes[2 * p][1], seed + 7 * p, seed + 1000 + p)
+    return [synth_warped_pair(sizes[2 * p][0], sizes[2 * p][1], seed + 7 * p, seed + 1000 + p, fine=fine)
             for p in range(n_pairs)]
+
+
+def synth_12mp_pair(seed=4000, fine=FINE_12MP):
+    """The 4000x3000 pair: synth_warped_pair with the fine texture that gives
+    20k-40k keypoints per image."""
+    return synth_warped_pair(3000, 4000, seed, fine=fine)
