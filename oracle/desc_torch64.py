@@ -139,9 +139,21 @@ def _normalize(hist, rootsift):
     return v
 
 
-def describe_group(G, ph, ri, ci, angle_deg, rootsift=True):
+def _state(oi0, m2, hist, rootsift):
+    """Discrete decisions of the descriptor: orientation bin of every sample,
+    zero / nonzero gradient magnitude of every sample, the clip at 0.2 and
+    (RootSIFT) the zero / positive pattern of every bin.  Between two inputs
+    with the same state the descriptor is an analytic function of the pixels."""
+    t = hist / torch.sqrt((hist * hist).sum(-1, keepdim=True) + 1e-14)
+    st = [oi0.numpy(), (m2 > 0).numpy(), (t > 0.2).numpy()]
+    if rootsift:
+        st.append((hist > 0).numpy())
+    return st
+
+
+def describe_group(G, ph, ri, ci, angle_deg, rootsift=True, return_state=False):
     """Descriptors (n, 128) float64 of n keypoints with patch half-size ph on
-    one level image G (Ho, Wo) float64."""
+    one level image G (Ho, Wo) float64 (and their discrete state)."""
     H, W = G.shape
     n = len(np.atleast_1d(ri))
     g = _geometry(ph, ri, ci, angle_deg, H, W)
@@ -170,6 +182,8 @@ def describe_group(G, ph, ri, ci, angle_deg, rootsift=True):
             cell = base + yy.clamp(0, 3) * 32 + xx.clamp(0, 3) * 8
             hist = hist.index_add(0, cell + o0, wy * wx * (1.0 - df) * wmag)
             hist = hist.index_add(0, cell + o1, wy * wx * df * wmag)
+    if return_state:
+        return _normalize(hist.view(n, 128), rootsift), _state(oi0, m2, hist.view(n, 128), rootsift)
     return _normalize(hist.view(n, 128), rootsift)
 
 
@@ -178,9 +192,10 @@ def describe(G, ph, ri, ci, angle_deg, rootsift=True):
     return describe_group(G, ph, [ri], [ci], [angle_deg], rootsift)[0]
 
 
-def descriptors_from_pyramid(octs, kps, S=3, sigma0=1.6, rootsift=True):
+def descriptors_from_pyramid(octs, kps, S=3, sigma0=1.6, rootsift=True, state=None):
     """kps (K, 5) rows (octave, si, ri, ci, angle) -> (K, 128); rows with an
-    octave outside the pyramid or si outside [1, S] give zeros."""
+    octave outside the pyramid or si outside [1, S] give zeros.  A list
+    passed as `state` receives the discrete state of every keypoint group."""
     kps = np.asarray(kps, dtype=np.float32).reshape(-1, 5)
     n_oct = len(octs)
     ph = patch_half_sizes(S, sigma0)
@@ -196,7 +211,12 @@ def descriptors_from_pyramid(octs, kps, S=3, sigma0=1.6, rootsift=True):
             sel = np.nonzero(valid & (oi == oo) & (si == ss))[0]
             if len(sel) == 0:
                 continue
-            parts.append(describe_group(octs[oo][ss], ph[ss], kps[sel, 2], kps[sel, 3], kps[sel, 4], rootsift))
+            if state is None:
+                parts.append(describe_group(octs[oo][ss], ph[ss], kps[sel, 2], kps[sel, 3], kps[sel, 4], rootsift))
+            else:
+                d, st = describe_group(octs[oo][ss], ph[ss], kps[sel, 2], kps[sel, 3], kps[sel, 4], rootsift, True)
+                parts.append(d)
+                state.extend(st)
             rows.append(torch.from_numpy(sel))
     if parts:
         out = out.index_copy(0, torch.cat(rows), torch.cat(parts))
@@ -217,21 +237,30 @@ def loss_and_grad(img, kps, Wt, n_oct, S=3, sigma0=1.6, rootsift=True):
     return float(loss.detach()), g
 
 
-def central_difference(img, kps, Wt, pixels, eps=1e-3, n_oct=5, S=3, sigma0=1.6, rootsift=True, radius=None):
+def central_difference(img, kps, Wt, pixels, eps=1e-3, n_oct=5, S=3, sigma0=1.6, rootsift=True, radius=None,
+                       return_checked=False):
     """FP64 central differences (L(img + eps e_p) - L(img - eps e_p)) / (2 eps)
     of the loss at pixels [(y, x), ...].  The pyramid is linear in the image,
     so it is built once for img and once for each impulse e_p and the
     perturbed pyramids are img's +- eps times the impulse's.  With `radius`
     (octave pixels), only keypoints whose centre lies within `radius` of the
     pixel in their own octave are re-evaluated (the others do not change);
-    None re-evaluates all."""
+    None re-evaluates all.
+
+    return_checked: also return, per pixel, whether the descriptor's discrete
+    state (every sample's orientation bin and zero/nonzero magnitude, the clip
+    at 0.2, RootSIFT's zero/positive bins) is the same at img + eps e_p and
+    img - eps e_p for every re-evaluated keypoint.  At such a pixel the loss
+    has no kink on the stencil, so the central difference approximates the
+    derivative to O(eps^2); elsewhere it straddles a point where the loss is
+    not differentiable and bounds no derivative."""
     img = np.asarray(img, dtype=np.float64)
     kps = np.asarray(kps, dtype=np.float32).reshape(-1, 5)
     Wt = torch.as_tensor(np.asarray(Wt, dtype=np.float64).reshape(-1, 128))
     H, W = img.shape
     with torch.no_grad():
         base = pyramid(torch.from_numpy(img), n_oct, S, sigma0)
-        out = []
+        out, checked = [], []
         for (y, x) in pixels:
             sel = np.ones(len(kps), dtype=bool)
             if radius is not None:
@@ -240,13 +269,29 @@ def central_difference(img, kps, Wt, pixels, eps=1e-3, n_oct=5, S=3, sigma0=1.6,
                       (np.abs(kps[:, 3] - x / sc) <= radius)
             if not sel.any():
                 out.append(0.0)
+                checked.append(True)
                 continue
             e = torch.zeros((H, W), dtype=F64)
             e[y, x] = 1.0
             imp = pyramid(e, n_oct, S, sigma0)
-            ls = []
+            ls, states = [], []
             for sgn in (1.0, -1.0):
                 octs = [b + (sgn * eps) * d for b, d in zip(base, imp)]
-                ls.append(float((Wt[sel] * descriptors_from_pyramid(octs, kps[sel], S, sigma0, rootsift)).sum()))
+                st = [] if return_checked else None
+                ls.append(float((Wt[sel] * descriptors_from_pyramid(octs, kps[sel], S, sigma0, rootsift, st)).sum()))
+                states.append(st)
             out.append((ls[0] - ls[1]) / (2 * eps))
+            if return_checked:
+                checked.append(all(np.array_equal(a, b) for a, b in zip(states[0], states[1])))
+    if return_checked:
+        return np.array(out), np.array(checked, dtype=bool)
     return np.array(out)
+
+
+def fd_job(job):
+    """One worker task of central_difference for a process pool:
+    job = (img, kps, Wt, pixels, eps, rootsift, n_oct, radius) -> (fd, checked)."""
+    img, kps, Wt, pixels, eps, rootsift, n_oct, radius = job
+    torch.set_num_threads(1)
+    return central_difference(img, kps, Wt, pixels, eps=eps, n_oct=n_oct, rootsift=rootsift, radius=radius,
+                              return_checked=True)
