@@ -108,8 +108,11 @@ int msift_detect_describe_batched(msift_batch* plan, const uint8_t* d_packed, vo
 /* Differentiable descriptor (no reference counterpart: the reference has no
  * gradient path; the forward is its pyramid, core.py:1234-1352, and
  * descriptor, core.py:693-835 + RootSIFT core.py:1870-1873).
- * B float32 images H x W in [0, 1] (row-major, back to back), single-size
- * base (the handle must have double_image = 0).  d_kps[B*K*5] = (octave,
+ * B float32 images H x W in [0, 1] (row-major, back to back).  With the
+ * handle's double_image set (and H * W below its double_px_cap) the base is
+ * the x2 cubic B-spline upsample (core.py:1270-1274) and octave 0 has
+ * 2H x 2W pixels; otherwise the single-size base.  The handle must describe
+ * the plain descriptor (dsp_n_scales = 0, fp16_pyramid = 0).  d_kps[B*K*5] = (octave,
  * scale index 1..S, row, col in octave pixels, angle in degrees) per slot;
  * an octave < 0 marks an empty slot.  Forward: d_desc[B*K*128].  Backward:
  * given d_gdesc = dL/d(desc) it recomputes the pyramid and writes

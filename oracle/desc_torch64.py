@@ -4,7 +4,8 @@ Test infrastructure only (the oracle of the gradient path): the device
 library never imports it.  It restates, in float64 with torch autograd,
 
 * the Gaussian pyramid of a float image in [0, 1] with a single-size base
-  (pysift/core.py:1234-1352): base blur sqrt(sigma0^2 - 0.5^2), levels
+  or the x2 cubic B-spline base (pysift/core.py:1234-1352, 1270-1274): base
+  blur sqrt(sigma0^2 - 0.5^2) (doubled: sqrt(sigma0^2 - 1)), levels
   1..S by the incremental sigmas, next octave = level S[::2, ::2]; each blur is
   the separable H-then-V correlation with normalised exp(-x^2 / 2 sigma^2) taps,
   radius int(4 sigma + 0.5), half-sample reflect boundary (core.py:475-608);
@@ -66,10 +67,64 @@ def blur(x, sigma):
     return _corr_last(y.transpose(-1, -2), w, r).transpose(-1, -2)
 
 
-def pyramid(img, n_oct, S=3, sigma0=1.6):
-    """img (H, W) float64 -> list over octaves of (S+1, Ho, Wo) level stacks."""
-    base, inc = sigma_schedule(sigma0, S, doubled=False)
-    g0 = blur(img, base)
+# cubic B-spline prefilter taps sqrt(3) z^|k|, z = sqrt(3) - 2, |k| <= 30: the
+# FIR form of the mirror-boundary recursive prefilter of scipy.ndimage.zoom
+# (order 3), as the detector uses for the x2 base (core.py:1274)
+_UPS_K = 30
+_UPS_H0 = float.fromhex("0x1.bb67ae8584caap+0")
+_UPS_Z = float.fromhex("-0x1.126145e9ecd58p-2")
+
+
+def _mirror(i, n):
+    """scipy 'mirror' boundary (period 2n - 2) of index i."""
+    if n == 1:
+        return 0
+    p = 2 * n - 2
+    i = abs(i) % p
+    return p - i if i >= n else i
+
+
+def _bspline_w(t):
+    s = 1.0 - t
+    return [s ** 3 / 6.0, (2.0 / 3.0 - t * t) + 0.5 * t ** 3, (2.0 / 3.0 - s * s) + 0.5 * s ** 3, t ** 3 / 6.0]
+
+
+def spline_matrices(n):
+    """(prefilter (n, n), interpolation (2n, n)) float64 matrices of the x2
+    cubic B-spline upsample along one axis of n samples: output o sits at
+    o (n - 1) / (2n - 1) in input coordinates (zoom's grid_mode=False)."""
+    taps = [_UPS_H0]
+    for _ in range(_UPS_K):
+        taps.append(taps[-1] * _UPS_Z)
+    P = np.zeros((n, n))
+    for y in range(n):
+        for k in range(-_UPS_K, _UPS_K + 1):
+            P[y, _mirror(y + k, n)] += taps[abs(k)]
+    no = 2 * n
+    z = (n - 1) / (no - 1)
+    M = np.zeros((no, n))
+    for o in range(no):
+        t = o * z
+        f = math.floor(t)
+        for b, w in enumerate(_bspline_w(t - f)):
+            M[o, _mirror(f - 1 + b, n)] += w
+    return torch.from_numpy(P), torch.from_numpy(M)
+
+
+def upsample2x(img):
+    """x2 cubic B-spline upsample of (H, W) float64 (differentiable)."""
+    H, W = img.shape
+    Ph, Mh = spline_matrices(H)
+    Pw, Mw = spline_matrices(W)
+    return Mh @ (Ph @ img @ Pw.T) @ Mw.T
+
+
+def pyramid(img, n_oct, S=3, sigma0=1.6, doubled=False):
+    """img (H, W) float64 -> list over octaves of (S+1, Ho, Wo) level stacks;
+    doubled: the base is the x2 spline upsample, blurred to sigma0 from an
+    assumed blur of 1.0 (core.py:1270-1284)."""
+    base, inc = sigma_schedule(sigma0, S, doubled=doubled)
+    g0 = blur(upsample2x(img) if doubled else img, base)
     octs = []
     for _ in range(n_oct):
         levels = [g0]
@@ -223,29 +278,30 @@ def descriptors_from_pyramid(octs, kps, S=3, sigma0=1.6, rootsift=True, state=No
     return out
 
 
-def descriptors(img, kps, n_oct, S=3, sigma0=1.6, rootsift=True):
+def descriptors(img, kps, n_oct, S=3, sigma0=1.6, rootsift=True, doubled=False):
     """img (H, W) float64, kps (K, 5) rows (octave, si, ri, ci, angle) -> (K, 128)."""
-    return descriptors_from_pyramid(pyramid(img, n_oct, S, sigma0), kps, S, sigma0, rootsift)
+    return descriptors_from_pyramid(pyramid(img, n_oct, S, sigma0, doubled), kps, S, sigma0, rootsift)
 
 
-def loss_and_grad(img, kps, Wt, n_oct, S=3, sigma0=1.6, rootsift=True):
+def loss_and_grad(img, kps, Wt, n_oct, S=3, sigma0=1.6, rootsift=True, doubled=False):
     """sum(Wt * descriptors) and its autograd gradient with respect to img."""
     x = torch.as_tensor(img, dtype=F64).clone().requires_grad_(True)
-    d = descriptors(x, kps, n_oct, S, sigma0, rootsift)
+    d = descriptors(x, kps, n_oct, S, sigma0, rootsift, doubled)
     loss = (torch.as_tensor(Wt, dtype=F64).reshape(d.shape) * d).sum()
     (g,) = torch.autograd.grad(loss, x)
     return float(loss.detach()), g
 
 
 def central_difference(img, kps, Wt, pixels, eps=1e-3, n_oct=5, S=3, sigma0=1.6, rootsift=True, radius=None,
-                       return_checked=False):
+                       return_checked=False, doubled=False):
     """FP64 central differences (L(img + eps e_p) - L(img - eps e_p)) / (2 eps)
     of the loss at pixels [(y, x), ...].  The pyramid is linear in the image,
     so it is built once for img and once for each impulse e_p and the
     perturbed pyramids are img's +- eps times the impulse's.  With `radius`
     (octave pixels), only keypoints whose centre lies within `radius` of the
     pixel in their own octave are re-evaluated (the others do not change);
-    None re-evaluates all.
+    None re-evaluates all.  With a doubled base, octave o's pixels are
+    2^(o - 1) image pixels.
 
     return_checked: also return, per pixel, whether the descriptor's discrete
     state (every sample's orientation bin and zero/nonzero magnitude, the clip
@@ -259,12 +315,12 @@ def central_difference(img, kps, Wt, pixels, eps=1e-3, n_oct=5, S=3, sigma0=1.6,
     Wt = torch.as_tensor(np.asarray(Wt, dtype=np.float64).reshape(-1, 128))
     H, W = img.shape
     with torch.no_grad():
-        base = pyramid(torch.from_numpy(img), n_oct, S, sigma0)
+        base = pyramid(torch.from_numpy(img), n_oct, S, sigma0, doubled)
         out, checked = [], []
         for (y, x) in pixels:
             sel = np.ones(len(kps), dtype=bool)
             if radius is not None:
-                sc = 2.0 ** np.maximum(kps[:, 0], 0)
+                sc = 2.0 ** (np.maximum(kps[:, 0], 0) - (1 if doubled else 0))
                 sel = (kps[:, 0] >= 0) & (np.abs(kps[:, 2] - y / sc) <= radius) & \
                       (np.abs(kps[:, 3] - x / sc) <= radius)
             if not sel.any():
@@ -273,7 +329,7 @@ def central_difference(img, kps, Wt, pixels, eps=1e-3, n_oct=5, S=3, sigma0=1.6,
                 continue
             e = torch.zeros((H, W), dtype=F64)
             e[y, x] = 1.0
-            imp = pyramid(e, n_oct, S, sigma0)
+            imp = pyramid(e, n_oct, S, sigma0, doubled)
             ls, states = [], []
             for sgn in (1.0, -1.0):
                 octs = [b + (sgn * eps) * d for b, d in zip(base, imp)]
@@ -290,8 +346,9 @@ def central_difference(img, kps, Wt, pixels, eps=1e-3, n_oct=5, S=3, sigma0=1.6,
 
 def fd_job(job):
     """One worker task of central_difference for a process pool:
-    job = (img, kps, Wt, pixels, eps, rootsift, n_oct, radius) -> (fd, checked)."""
-    img, kps, Wt, pixels, eps, rootsift, n_oct, radius = job
+    job = (img, kps, Wt, pixels, eps, rootsift, n_oct, radius[, doubled]) -> (fd, checked)."""
+    img, kps, Wt, pixels, eps, rootsift, n_oct, radius = job[:8]
+    doubled = job[8] if len(job) > 8 else False
     torch.set_num_threads(1)
     return central_difference(img, kps, Wt, pixels, eps=eps, n_oct=n_oct, rootsift=rootsift, radius=radius,
-                              return_checked=True)
+                              return_checked=True, doubled=doubled)

@@ -3,7 +3,7 @@ forward and hand-written backward (libpysift_amd msift_desc_forward /
 msift_desc_backward).
 
 The forward builds the Gaussian pyramid of float images in [0, 1] (single-
-size base: the handle must have double_image=False) and describes fixed,
+size base, or the x2 spline base with double_image) and describes fixed,
 detached keypoints; the backward recomputes the pyramid (no level is kept
 between the two calls), differentiates the descriptor histogram and its
 normalisation on one wavefront per keypoint, accumulates the per-pixel
@@ -87,12 +87,14 @@ _default = {}
 def sift_descriptors(images, keypoints, sift=None):
     """Descriptors (B, K, 128) of float images (B, H, W) in [0, 1] at fixed
     keypoints (B, K, 5) = (octave, si, row, col, angle deg); differentiable
-    with respect to `images`.  `sift` is a PySIFT with double_image=False
-    (default: PySIFT(double_image=False))."""
+    with respect to `images`.  `sift` is a PySIFT in its plain mode (no DSP,
+    float32 storage); with double_image=True (and H * W below its
+    double_px_cap) octave 0 is the x2 spline upsample and keypoint rows and
+    columns are in its pixels (default: PySIFT(double_image=False))."""
     if sift is None:
         if "s" not in _default:
             _default["s"] = PySIFT(double_image=False)
         sift = _default["s"]
-    if sift.double_image:
-        raise ValueError("the descriptor gradient needs a PySIFT with double_image=False")
+    if sift.dsp or sift.fp16_pyramid:
+        raise ValueError("the descriptor gradient needs the plain descriptor (dsp=False, fp16_pyramid=False)")
     return SiftDescriptorFn.apply(images, keypoints.detach(), sift)

@@ -12,7 +12,12 @@
 // int64 fixed point (2^-32) with integer atomics, which are order-free, so
 // the gradient is bitwise reproducible.  The level gradients then run through
 // the adjoint of each blur (reflect boundary folded back) and of the [::2, ::2]
-// decimation, octave by octave from the top, down to the image.
+// decimation, octave by octave from the top, down to the image.  With a
+// doubled base (double_image, below double_px_cap pixels) the image is first
+// upsampled x2 by the cubic B-spline of the detector (mirror-boundary FIR
+// prefilter down the columns and along the rows, then interpolation, FP64);
+// its adjoint runs the interpolation and the two prefilters transposed, also
+// in FP64, as gathers (every output written by one thread, no atomics).
 #include "sift_dev.hpp"
 #include <cstring>
 
@@ -23,12 +28,16 @@ namespace msift {
 constexpr double kFix = 4294967296.0;   // fixed-point scale of the level gradients
 
 struct GradPlan {
-    int B, H, W, S, n_oct;
+    int B, H, W, S, n_oct, doubled;
     int Ho[kMaxOct], Wo[kMaxOct];
     size_t oG[kMaxOct];    // G levels 0..S of octave o (floats), offsets within an image block
     size_t oGi[kMaxOct];   // int64 gradients of levels 1..S
     size_t oGf[kMaxOct];   // float gradients of levels 0..S
     size_t o_tmp;          // one level of scratch for the adjoint blur
+    // doubled base: upsampled image (2H x 2W float, its gradient in backward),
+    // two H x W double planes (prefilter passes / their adjoints) and the
+    // 2H x W double plane of the separable interpolation adjoint
+    size_t o_up, o_d0, o_d1, o_r;
     size_t per_img;        // bytes per image block (multiple of 256)
     size_t total;
 };
@@ -36,9 +45,10 @@ struct GradPlan {
 static void grad_plan(const msift_handle* h, int B, int H, int W, GradPlan* P) {
     memset(P, 0, sizeof(*P));
     P->B = B; P->H = H; P->W = W; P->S = h->S; P->n_oct = h->p.n_octaves;
+    P->doubled = h->p.double_image && ((long long)H * W < (long long)h->p.double_px_cap);
     size_t off = 0;
     auto take = [&](size_t b) { size_t o = off; off += (b + 255) & ~(size_t)255; return o; };
-    int bh = H, bw = W;
+    int bh = P->doubled ? 2 * H : H, bw = P->doubled ? 2 * W : W;
     for (int o = 0; o < P->n_oct; ++o) {
         P->Ho[o] = bh; P->Wo[o] = bw;
         bh = (bh + 1) / 2; bw = (bw + 1) / 2;
@@ -46,7 +56,13 @@ static void grad_plan(const msift_handle* h, int B, int H, int W, GradPlan* P) {
     for (int o = 0; o < P->n_oct; ++o) P->oG[o] = take((size_t)P->Ho[o] * P->Wo[o] * 4 * (P->S + 1));
     for (int o = 0; o < P->n_oct; ++o) P->oGi[o] = take((size_t)P->Ho[o] * P->Wo[o] * 8 * P->S);
     for (int o = 0; o < P->n_oct; ++o) P->oGf[o] = take((size_t)P->Ho[o] * P->Wo[o] * 4 * (P->S + 1));
-    P->o_tmp = take((size_t)H * W * 4);
+    P->o_tmp = take((size_t)P->Ho[0] * P->Wo[0] * 4);
+    if (P->doubled) {
+        P->o_up = take((size_t)P->Ho[0] * P->Wo[0] * 4);
+        P->o_d0 = take((size_t)H * W * 8);
+        P->o_d1 = take((size_t)H * W * 8);
+        P->o_r = take((size_t)2 * H * W * 8);
+    }
     P->per_img = off;
     P->total = off * (size_t)B;
 }
@@ -298,15 +314,206 @@ __global__ __launch_bounds__(256) void k_adj_h(char* __restrict__ ws, size_t per
     }
 }
 
+// ------------------------------------------------- x2 spline upsample ----
+// Forward on a float image (the detector's spline, core.py:1274, on the
+// values themselves instead of u8 / 255): prefilter down the columns, then
+// along the rows, sum_k h|k| x(mirror(i + k)), |k| <= kUpsK, FP64; then the
+// cubic B-spline interpolation at output o -> o (N - 1) / (2N - 1).
+__global__ __launch_bounds__(256) void k_gups_pf_col(const float* __restrict__ imgs, char* __restrict__ ws,
+                                                     size_t per_img, size_t o_dst, int H, int W, UpsTaps h) {
+    const float* x = imgs + (size_t)blockIdx.y * H * W;
+    double* d = (double*)(ws + (size_t)blockIdx.y * per_img + o_dst);
+    const size_t n = (size_t)H * W;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const int y = (int)(i / W), c = (int)(i - (size_t)y * W);
+        double acc = 0.0;
+        if (y >= kUpsK && y + kUpsK < H) {
+            for (int k = -kUpsK; k <= kUpsK; ++k)
+                acc = fma(h.h[k < 0 ? -k : k], (double)x[(size_t)(y + k) * W + c], acc);
+        } else {
+            for (int k = -kUpsK; k <= kUpsK; ++k)
+                acc = fma(h.h[k < 0 ? -k : k], (double)x[(size_t)mirror_idx(y + k, H) * W + c], acc);
+        }
+        d[i] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gups_pf_row(char* __restrict__ ws, size_t per_img, size_t o_src,
+                                                     size_t o_dst, int H, int W, UpsTaps h) {
+    char* img = ws + (size_t)blockIdx.y * per_img;
+    const double* a = (const double*)(img + o_src);
+    double* d = (double*)(img + o_dst);
+    const size_t n = (size_t)H * W;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const int y = (int)(i / W), c = (int)(i - (size_t)y * W);
+        const double* row = a + (size_t)y * W;
+        double acc = 0.0;
+        if (c >= kUpsK && c + kUpsK < W) {
+            for (int k = -kUpsK; k <= kUpsK; ++k) acc = fma(h.h[k < 0 ? -k : k], row[c + k], acc);
+        } else {
+            for (int k = -kUpsK; k <= kUpsK; ++k) acc = fma(h.h[k < 0 ? -k : k], row[mirror_idx(c + k, W)], acc);
+        }
+        d[i] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gups_interp(char* __restrict__ ws, size_t per_img, size_t o_cf, size_t o_up,
+                                                     int H, int W) {
+    char* img = ws + (size_t)blockIdx.y * per_img;
+    const double* cf = (const double*)(img + o_cf);
+    float* up = (float*)(img + o_up);
+    const int Ho = 2 * H, Wo = 2 * W;
+    const double zy = (double)(H - 1) / (double)(Ho - 1), zx = (double)(W - 1) / (double)(Wo - 1);
+    const size_t n = (size_t)Ho * Wo;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const int oy = (int)(i / Wo), ox = (int)(i - (size_t)oy * Wo);
+        const double yy = (double)oy * zy, fy = floor(yy);
+        const double xx = (double)ox * zx, fx = floor(xx);
+        const int iy = (int)fy, ix = (int)fx;
+        double wy[4], wx[4];
+        bspline_w(yy - fy, wy);
+        bspline_w(xx - fx, wx);
+        double v = 0.0;
+        for (int a = 0; a < 4; ++a) {
+            const double* row = cf + (size_t)mirror_idx(iy - 1 + a, H) * W;
+            double rv = 0.0;
+            for (int b = 0; b < 4; ++b) rv = fma(wx[b], row[mirror_idx(ix - 1 + b, W)], rv);
+            v = fma(wy[a], rv, v);
+        }
+        up[i] = (float)v;
+    }
+}
+
+// Weight of coefficient j in interpolated output o along an axis of N
+// coefficients and 2N outputs: the sum of the B-spline weights of the taps of
+// o that mirror to j (zero if none does).
+__device__ __forceinline__ double interp_weight(int o, int j, int N, double z) {
+    const double t = (double)o * z, f = floor(t);
+    const int i0 = (int)f - 1;
+    double w[4];
+    bspline_w(t - f, w);
+    double acc = 0.0;
+    for (int b = 0; b < 4; ++b)
+        if (mirror_idx(i0 + b, N) == j) acc += w[b];
+    return acc;
+}
+
+// Outputs whose taps can reach coefficient j (N >= 3).  Output o has its taps
+// at positions floor(o z) - 1 .. floor(o z) + 2, within [-1, N + 1]; position
+// p reaches j for p = j, p = -j (j = 1) and p = 2(N - 1) - j (j >= N - 3).
+// Returns a hull [lo, hi] of all of them (the weights are tested exactly).
+__device__ __forceinline__ void interp_range(int j, int N, double z, int* lo, int* hi) {
+    const int No = 2 * N;
+    int a = (int)floor((double)(j - 2) / z) - 1, b = (int)ceil((double)(j + 2) / z) + 1;
+    if (j <= 2) a = 0;
+    if (j >= N - 3) b = No - 1;
+    *lo = a < 0 ? 0 : a;
+    *hi = b > No - 1 ? No - 1 : b;
+}
+
+// interpolation adjoint along x: r[oy][j] = sum_ox w(ox -> j) g_up[oy][ox]
+__global__ __launch_bounds__(256) void k_gups_interp_adj_x(char* __restrict__ ws, size_t per_img, size_t o_up,
+                                                           size_t o_r, int H, int W) {
+    char* img = ws + (size_t)blockIdx.y * per_img;
+    const float* g = (const float*)(img + o_up);
+    double* r = (double*)(img + o_r);
+    const int Wo = 2 * W;
+    const double zx = (double)(W - 1) / (double)(Wo - 1);
+    const size_t n = (size_t)2 * H * W;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const int oy = (int)(i / W), j = (int)(i - (size_t)oy * W);
+        int lo, hi;
+        interp_range(j, W, zx, &lo, &hi);
+        double acc = 0.0;
+        for (int ox = lo; ox <= hi; ++ox) {
+            const double w = interp_weight(ox, j, W, zx);
+            if (w != 0.0) acc = fma(w, (double)g[(size_t)oy * Wo + ox], acc);
+        }
+        r[i] = acc;
+    }
+}
+
+// interpolation adjoint along y: gcf[i][x] = sum_oy w(oy -> i) r[oy][x]
+__global__ __launch_bounds__(256) void k_gups_interp_adj_y(char* __restrict__ ws, size_t per_img, size_t o_r,
+                                                           size_t o_dst, int H, int W) {
+    char* img = ws + (size_t)blockIdx.y * per_img;
+    const double* r = (const double*)(img + o_r);
+    double* d = (double*)(img + o_dst);
+    const double zy = (double)(H - 1) / (double)(2 * H - 1);
+    const size_t n = (size_t)H * W;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const int y = (int)(i / W), x = (int)(i - (size_t)y * W);
+        int lo, hi;
+        interp_range(y, H, zy, &lo, &hi);
+        double acc = 0.0;
+        for (int oy = lo; oy <= hi; ++oy) {
+            const double w = interp_weight(oy, y, H, zy);
+            if (w != 0.0) acc = fma(w, r[(size_t)oy * W + x], acc);
+        }
+        d[i] = acc;
+    }
+}
+
+// Adjoint of the mirror-boundary prefilter d[i] = sum_k h|k| x[mirror(i + k)]
+// along an axis of N samples, at index j: sum over (i, k) with
+// mirror(i + k) = j of h|k| g[i].
+// * N > 2 kUpsK + 2: i + k stays within one mirror fold, so the pairs are
+//   k = j - i, k = -j - i (j >= 1) and k = 2(N - 1) - j - i (j <= N - 2), each
+//   kept when |k| <= kUpsK; away from the edges only the first exists.
+// * smaller N: the window folds several times; every pair is tested.
+__device__ __forceinline__ double pf_adj(const double* __restrict__ g, size_t stride, int N, int j, const UpsTaps& h) {
+    double acc = 0.0;
+    if (N > 2 * kUpsK + 2) {
+        if (j > kUpsK && j < N - 1 - kUpsK) {
+            for (int k = -kUpsK; k <= kUpsK; ++k) acc = fma(h.h[k < 0 ? -k : k], g[(size_t)(j - k) * stride], acc);
+            return acc;
+        }
+        const int lo = j - 2 * kUpsK < 0 ? 0 : j - 2 * kUpsK;
+        const int hi = j + 2 * kUpsK > N - 1 ? N - 1 : j + 2 * kUpsK;
+        for (int i = lo; i <= hi; ++i) {
+            const double gi = g[(size_t)i * stride];
+            const int k1 = j - i, k2 = -j - i, k3 = 2 * (N - 1) - j - i;
+            if (k1 >= -kUpsK && k1 <= kUpsK) acc = fma(h.h[k1 < 0 ? -k1 : k1], gi, acc);
+            if (j >= 1 && k2 >= -kUpsK && k2 <= kUpsK) acc = fma(h.h[k2 < 0 ? -k2 : k2], gi, acc);
+            if (j <= N - 2 && k3 >= -kUpsK && k3 <= kUpsK) acc = fma(h.h[k3 < 0 ? -k3 : k3], gi, acc);
+        }
+        return acc;
+    }
+    for (int i = 0; i < N; ++i)
+        for (int k = -kUpsK; k <= kUpsK; ++k)
+            if (mirror_idx(i + k, N) == j) acc = fma(h.h[k < 0 ? -k : k], g[(size_t)i * stride], acc);
+    return acc;
+}
+
+__global__ __launch_bounds__(256) void k_gups_pf_adj_row(char* __restrict__ ws, size_t per_img, size_t o_src,
+                                                         size_t o_dst, int H, int W, UpsTaps h) {
+    char* img = ws + (size_t)blockIdx.y * per_img;
+    const double* g = (const double*)(img + o_src);
+    double* d = (double*)(img + o_dst);
+    const size_t n = (size_t)H * W;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const int y = (int)(i / W), x = (int)(i - (size_t)y * W);
+        d[i] = pf_adj(g + (size_t)y * W, 1, W, x, h);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gups_pf_adj_col(const char* __restrict__ ws, size_t per_img, size_t o_src,
+                                                         float* __restrict__ gimg, int H, int W, UpsTaps h) {
+    const double* g = (const double*)(ws + (size_t)blockIdx.y * per_img + o_src);
+    float* d = gimg + (size_t)blockIdx.y * H * W;
+    const size_t n = (size_t)H * W;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const int y = (int)(i / W), x = (int)(i - (size_t)y * W);
+        d[i] = (float)pf_adj(g + x, (size_t)W, H, y, h);
+    }
+}
+
 static inline unsigned gcap(size_t n, unsigned cap) { return n < 1 ? 1u : (n < cap ? (unsigned)n : cap); }
+
 
 static int check_args(msift_handle* h, int B, int H, int W, int K, const void* imgs, const void* kps, void* ws) {
     if (!h || B < 1 || H < 3 || W < 3 || K < 0 || !imgs || (K > 0 && !kps) || !ws) {
         set_error("descriptor gradient: bad arguments");
-        return MSIFT_EINVAL;
-    }
-    if (h->p.double_image) {
-        set_error("descriptor gradient: needs double_image = 0 (the spline-upsample adjoint is not built)");
         return MSIFT_EINVAL;
     }
     if (h->p.dsp_n_scales || h->p.fp16_pyramid) {
@@ -321,8 +528,19 @@ static int check_args(msift_handle* h, int B, int H, int W, int K, const void* i
 static int grad_pyramid(msift_handle* h, const GradPlan& P, const float* imgs, char* ws, hipStream_t s) {
     const int S = P.S;
     const size_t zimg = P.per_img / 4;
-    int rc = blur(imgs, (float*)(ws + P.oG[0]), nullptr, nullptr, P.H, P.W, h->sd0[0], h->taps_base[0], s, P.B,
+    int rc;
+    if (P.doubled) {
+        const size_t n = (size_t)P.H * P.W;
+        dim3 grid(gcap((n + 255) / 256, 2048), P.B), grid2(gcap((4 * n + 255) / 256, 2048), P.B);
+        hipLaunchKernelGGL(k_gups_pf_col, grid, dim3(256), 0, s, imgs, ws, P.per_img, P.o_d0, P.H, P.W, h->ups);
+        hipLaunchKernelGGL(k_gups_pf_row, grid, dim3(256), 0, s, ws, P.per_img, P.o_d0, P.o_d1, P.H, P.W, h->ups);
+        hipLaunchKernelGGL(k_gups_interp, grid2, dim3(256), 0, s, ws, P.per_img, P.o_d1, P.o_up, P.H, P.W);
+        rc = blur((const float*)(ws + P.o_up), (float*)(ws + P.oG[0]), nullptr, nullptr, P.Ho[0], P.Wo[0], h->sd0[1],
+                  h->taps_base[1], s, P.B, zimg, zimg);
+    } else {
+        rc = blur(imgs, (float*)(ws + P.oG[0]), nullptr, nullptr, P.H, P.W, h->sd0[0], h->taps_base[0], s, P.B,
                   (size_t)P.H * P.W, zimg);
+    }
     if (rc) return rc;
     for (int o = 0; o < P.n_oct; ++o) {
         const size_t n = (size_t)P.Ho[o] * P.Wo[o];
@@ -428,7 +646,20 @@ int desc_backward(msift_handle* h, const float* imgs, int B, int H, int W, const
             blur_adjoint(P, ws, P.oGf[o] + n * 4 * l, P.oGf[o] + n * 4 * (l - 1), Ho, Wo, h->inc[l - 1],
                          h->taps_inc[l - 1], nullptr, 0, 1, s);
     }
-    blur_adjoint(P, ws, P.oGf[0], 0, H, W, h->sd0[0], h->taps_base[0], gimg, (size_t)H * W, 0, s);
+    if (P.doubled) {
+        // base blur adjoint into the upsampled image's gradient, then the
+        // spline adjoint: interpolation along x and y, row and column prefilters
+        blur_adjoint(P, ws, P.oGf[0], P.o_up, P.Ho[0], P.Wo[0], h->sd0[1], h->taps_base[1], nullptr, 0, 0, s);
+        const size_t n = (size_t)H * W;
+        dim3 grid(gcap((n + 255) / 256, 2048), B), grid2(gcap((2 * n + 255) / 256, 2048), B);
+        hipLaunchKernelGGL(k_gups_interp_adj_x, grid2, dim3(256), 0, s, ws, P.per_img, P.o_up, P.o_r, H, W);
+        hipLaunchKernelGGL(k_gups_interp_adj_y, grid, dim3(256), 0, s, ws, P.per_img, P.o_r, P.o_d1, H, W);
+        hipLaunchKernelGGL(k_gups_pf_adj_row, grid, dim3(256), 0, s, ws, P.per_img, P.o_d1, P.o_d0, H, W, h->ups);
+        hipLaunchKernelGGL(k_gups_pf_adj_col, grid, dim3(256), 0, s, (const char*)ws, P.per_img, P.o_d0, gimg, H, W,
+                           h->ups);
+    } else {
+        blur_adjoint(P, ws, P.oGf[0], 0, H, W, h->sd0[0], h->taps_base[0], gimg, (size_t)H * W, 0, s);
+    }
     MSIFT_LAUNCH_CHECK();
     return MSIFT_OK;
 }

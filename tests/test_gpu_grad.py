@@ -168,9 +168,80 @@ def test_function_and_direct_abi_agree_and_errors():
     torch.cuda.synchronize()
     assert torch.equal(gimg, g)
     with pytest.raises(ValueError):
-        sift_descriptors(ti, tk, PySIFT(double_image=True))
+        sift_descriptors(ti, tk, PySIFT(double_image=False, dsp=True))
     with pytest.raises(ValueError):
         sift_descriptors(ti.double(), tk, sift)
     rc = L.msift_desc_forward(sift._h, _lib.ptr(ti), B, H, W, _lib.ptr(tk), tk.shape[1], _lib.ptr(ws), 16,
                               _lib.ptr(d), _lib.stream_ptr())
     assert rc == _lib.MSIFT_EWORKSPACE
+
+
+# ---- doubled base (double_image=True): the x2 spline upsample and its adjoint ----
+
+def _setup_doubled(B=2, H=96, W=112, K=80, seed=80, rootsift=True):
+    import torch
+    from pysift_amd import PySIFT
+    imgs = syn.synth_float_batch(B, H, W, seed=seed)
+    kps = syn.random_keypoints(B, K, 2 * H, 2 * W, n_oct=5, seed=seed + 1, margin=2)
+    Wt = np.random.default_rng(seed + 2).standard_normal((B, K, 128)).astype(np.float32)
+    sift = PySIFT(double_image=True, rootsift=rootsift)
+    dev = torch.device("cuda")
+    return sift, imgs, kps, Wt, torch.from_numpy(imgs).to(dev), torch.from_numpy(kps).to(dev), \
+        torch.from_numpy(Wt).to(dev)
+
+
+def test_doubled_forward_matches_fp64_restatement():
+    import torch
+    sift, imgs, kps, Wt, ti, tk, tw = _setup_doubled()
+    d, _ = _gpu_grad(sift, ti, tk, tw)
+    d = d.cpu().numpy()
+    for b in range(imgs.shape[0]):
+        ref = T.descriptors(torch.from_numpy(imgs[b].astype(np.float64)), kps[b], 5, doubled=True).numpy()
+        assert np.abs(d[b] - ref).max() < 2e-4, b
+
+
+def test_doubled_spline_and_pyramid_adjoint_match_fp64_vjp():
+    # image gradient against the FP64 vector-Jacobian product of the doubled
+    # pyramid (spline upsample, base blur, chain) with the GPU's level gradients
+    import torch
+    sift, imgs, kps, Wt, ti, tk, tw = _setup_doubled()
+    gimg, ws = _backward_ws(sift, ti, tk, tw)
+    gimg = gimg.cpu().numpy()
+    for b in range(imgs.shape[0]):
+        x = torch.tensor(imgs[b].astype(np.float64), requires_grad=True)
+        octs = T.pyramid(x, 5, doubled=True)
+        acc = torch.zeros((), dtype=torch.float64)
+        for o in range(5):
+            _, dg = _levels(sift, ti, ws, b, o)
+            assert dg.shape[1:] == tuple(octs[o].shape[1:])
+            acc = acc + (octs[o][1:] * torch.from_numpy(dg)).sum()
+        (ref,) = torch.autograd.grad(acc, x)
+        ref = ref.numpy()
+        assert np.linalg.norm(gimg[b] - ref) <= 1e-5 * np.linalg.norm(ref), b
+
+
+@pytest.mark.parametrize("rootsift", [True, False], ids=["rootsift", "l2sift"])
+def test_doubled_gradient_vs_fp64_central_differences(rootsift):
+    # checked pixels (FP64 forward state unchanged over the +-eps stencil, see
+    # test_gpu_grad_config3.py) at eps 1e-3: within 1e-3 norm-wise
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    import os
+    sift, imgs, kps, Wt, ti, tk, tw = _setup_doubled(rootsift=rootsift)
+    _, g = _gpu_grad(sift, ti, tk, tw)
+    g = g.cpu().numpy()
+    B, H, W = imgs.shape
+    rng = np.random.default_rng(81)
+    pix = [(b, int(y), int(x)) for b in range(B) for y, x in zip(rng.integers(0, H, 24), rng.integers(0, W, 24))]
+    jobs = [(imgs[b].astype(np.float64), kps[b], Wt[b], [(y, x)], 1e-3, rootsift, 5, 120, True) for b, y, x in pix]
+    with cf.ProcessPoolExecutor(max_workers=max(1, min(os.cpu_count() or 4, 16) - 1),
+                                mp_context=mp.get_context("spawn")) as ex:
+        res = list(ex.map(T.fd_job, jobs))
+    fd = np.concatenate([r[0] for r in res])
+    ck = np.concatenate([r[1] for r in res])
+    gp = np.array([g[b, y, x] for b, y, x in pix])
+    err = np.linalg.norm(gp[ck] - fd[ck]) / np.linalg.norm(fd[ck])
+    print(f"[case] doubled gradient rootsift={rootsift}: {int(ck.sum())}/{len(ck)} checked, GPU vs FD {err:.3e}",
+          flush=True)
+    assert ck.sum() >= 8
+    assert err <= 1e-3

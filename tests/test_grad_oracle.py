@@ -5,6 +5,8 @@ device backward, so it is checked here against the C oracle's float32
 pyramid and descriptor (same algorithm, other arithmetic) and against float64
 central finite differences of its own loss.
 """
+import math
+
 import numpy as np
 import pytest
 import torch
@@ -18,6 +20,17 @@ def test_pyramid_matches_c_oracle_single_size_base():
     img = syn.synth_gray(72, 88, seed=11)
     p = O.params(n_octaves=3, double_image=False)
     octs = T.pyramid(torch.from_numpy(img.astype(np.float64) / 255.0), 3)
+    for o in range(3):
+        g, _ = O.pyramid_octave(img, p, o)
+        ref = octs[o].numpy()
+        assert ref.shape == g[:4].shape
+        assert np.abs(g[:4] - ref).max() < 3e-6, o
+
+
+def test_pyramid_matches_c_oracle_doubled_base():
+    img = syn.synth_gray(40, 52, seed=12)
+    p = O.params(n_octaves=3, double_image=True)
+    octs = T.pyramid(torch.from_numpy(img.astype(np.float64) / 255.0), 3, doubled=True)
     for o in range(3):
         g, _ = O.pyramid_octave(img, p, o)
         ref = octs[o].numpy()
@@ -81,3 +94,33 @@ def test_central_difference_locality_radius_is_exact():
     full = T.central_difference(img, kps, Wt, pix, n_oct=2)
     local = T.central_difference(img, kps, Wt, pix, n_oct=2, radius=200)
     assert np.array_equal(full, local)
+
+
+@pytest.mark.parametrize("shape", [(3, 3), (2, 5), (17, 29), (64, 80), (101, 67)])
+def test_fp64_spline_upsample_matches_scipy_zoom(shape):
+    # the doubled base of the gradient path: scipy.ndimage.zoom(x, 2, order=3)
+    # (mirror-boundary cubic B-spline, grid_mode=False) in float64
+    import scipy.ndimage as nd
+    x = np.random.default_rng(shape[0] * 7 + shape[1]).random(shape)
+    up = T.upsample2x(torch.from_numpy(x)).numpy()
+    np.testing.assert_allclose(up, nd.zoom(x, 2.0, order=3), rtol=0, atol=1e-13)
+
+
+def test_doubled_pyramid_levels_and_fd_checked_pixels():
+    # doubled base = blur(zoom(x), sqrt(1.6^2 - 1)); the autograd gradient of a
+    # doubled-base descriptor loss agrees with central differences on the
+    # checked pixels
+    img = syn.synth_float_batch(1, 40, 48, seed=91)[0].astype(np.float64)
+    octs = T.pyramid(torch.from_numpy(img), 3, doubled=True)
+    assert octs[0].shape == (4, 80, 96) and octs[1].shape == (4, 40, 48)
+    ref0 = T.blur(T.upsample2x(torch.from_numpy(img)), math.sqrt(1.6 ** 2 - 1.0))
+    np.testing.assert_allclose(octs[0][0].numpy(), ref0.numpy(), rtol=0, atol=1e-15)
+    kps = syn.random_keypoints(1, 12, 80, 96, n_oct=3, seed=92, margin=2)[0]
+    Wt = np.random.default_rng(93).standard_normal((12, 128))
+    _, g = T.loss_and_grad(img, kps, Wt, n_oct=3, rootsift=False, doubled=True)
+    pix = [(5, 7), (20, 30), (33, 11), (12, 40), (25, 25), (38, 45)]
+    fd, ck = T.central_difference(img, kps, Wt, pix, eps=1e-4, n_oct=3, rootsift=False, return_checked=True,
+                                  doubled=True)
+    an = np.array([g[y, x].item() for y, x in pix])
+    assert ck.sum() >= 2
+    np.testing.assert_allclose(an[ck], fd[ck], rtol=1e-5, atol=1e-9)
