@@ -7,10 +7,10 @@ clipped to 0..255 and rounded.  Shape counts scale with image area so that
 large images keep the 640x480 feature density.
 
 Workloads that name a keypoint density (2k-8k keypoints per image for the
-uneven batch and its pairs, 20k-40k per 12 MP image) add fine texture: small
-Gaussian blobs at a fixed count per megapixel (FINE_BATCH, FINE_12MP), drawn
-from their own random stream after the rest of the content, so images without
-it are unchanged.
+32 matched pairs, 20k-40k per 12 MP image) add fine texture: small Gaussian
+blobs (pairs_fine, FINE_12MP), drawn from their own random stream after the
+rest of the content, so images without it are unchanged.  The 64-image
+detection batch keeps the plain recipe.
 """
 import math
 
@@ -20,8 +20,15 @@ _BASE_AREA = 640 * 480
 
 # fine texture (blobs per MP, sigma range px, amplitude range) of the workloads
 # that name a keypoint density
-FINE_BATCH = (50000, (0.8, 2.0), (15.0, 60.0))
 FINE_12MP = (2000, (1.0, 3.0), (15.0, 60.0))
+
+
+def pairs_fine(H, W):
+    """Fine texture of a pair image of H x W: 9100 - 1250 * MP blobs (sigma
+    0.8-2 px, amplitude 15-60), which puts the returned keypoints of the
+    320x240 .. 1920x1080 sizes in 2k-8k."""
+    mp = H * W / 1e6
+    return ((9100.0 - 1250.0 * mp) / mp, (0.8, 2.0), (15.0, 60.0))
 
 
 def _fine_texture(img, seed, per_mp, sigma, amp, n_sigma=6):
@@ -179,10 +186,9 @@ def synth_batch_sizes(n=64, seed=2000, lo=(240, 320), hi=(1080, 1920)):
     return [(int(a), int(b)) for a, b in zip(Hs, Ws)]
 
 
-def synth_batch(n=64, seed=2000, fine=FINE_BATCH):
-    """Uneven batch: sizes uniform in 320x240 .. 1920x1080, with the fine
-    texture that gives 2k-8k keypoints per image."""
-    return [synth_gray(h, w, seed + 1 + i, fine=fine) for i, (h, w) in enumerate(synth_batch_sizes(n, seed))]
+def synth_batch(n=64, seed=2000):
+    """Uneven batch: sizes uniform in 320x240 .. 1920x1080."""
+    return [synth_gray(h, w, seed + 1 + i) for i, (h, w) in enumerate(synth_batch_sizes(n, seed))]
 
 
 def corner_error(H_est, H_gt, w, h):
@@ -245,11 +251,12 @@ def synth_warped_pair(h, w, seed, content_seed=None, fine=None):
     return np.round(a).astype(np.uint8), np.round(b).astype(np.uint8), Hgt / Hgt[2, 2]
 
 
-def synth_pairs_batch(n_pairs=32, seed=2100, fine=FINE_BATCH):
+def synth_pairs_batch(n_pairs=32, seed=2100, dense=True):
     """n_pairs warped pairs (synth_warped_pair) at the uneven batch sizes,
-    with the batch's fine texture."""
+    with the fine texture of pairs_fine (dense=False: the plain recipe)."""
     sizes = synth_batch_sizes(2 * n_pairs, seed)
-    return [synth_warped_pair(sizes[2 * p][0], sizes[2 * p][1], seed + 7 * p, seed + 1000 + p, fine=fine)
+    return [synth_warped_pair(sizes[2 * p][0], sizes[2 * p][1], seed + 7 * p, seed + 1000 + p,
+                              fine=pairs_fine(*sizes[2 * p]) if dense else None)
             for p in range(n_pairs)]
 
 

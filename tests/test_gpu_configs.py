@@ -3,9 +3,9 @@
 * 2 x 1024x768 BGR stitch (CLAHE, detect, exact match, 2000-hypothesis
   RANSAC, feather): matches, inliers, homography and panorama bit-exact; the
   homography within 2 px (mean corner error) of the ground truth.
-* 64 uneven images up to 1920x1080 with 2k-8k keypoints each: the batched
-  graph-replayed pipeline equals the per-image path on every image and the
-  oracle on a fixed sample; the 32 warped pairs matched in one launch and
+* 64 uneven images up to 1920x1080: the batched graph-replayed pipeline
+  equals the per-image path on every image and the oracle on a fixed sample;
+  the 32 warped pairs (2k-8k keypoints per image) matched in one launch and
   verified by batched RANSAC (1024 hypotheses per pair) equal the per-pair
   calls on every pair and the oracle on a fixed sample.
 * one 4000x3000 pair with 20k-40k keypoints each: detection equal to the
@@ -85,16 +85,11 @@ def test_batch64_uneven_equals_per_image_and_oracle():
     torch.cuda.synchronize()
     res = [(k.cpu().numpy(), d.cpu().numpy()) for k, d in plan.results()]
     res_u8 = [d.cpu().numpy() for _, d in plan.results(u8=True)]
-    locs = []
     for i, img in enumerate(imgs):
         kp, df, du = s.detect_raw(img)
         np.testing.assert_array_equal(res[i][0], kp.cpu().numpy(), err_msg=f"image {i} {img.shape}")
         np.testing.assert_array_equal(res[i][1], df.cpu().numpy(), err_msg=f"image {i}")
         np.testing.assert_array_equal(res_u8[i], du.cpu().numpy(), err_msg=f"image {i}")
-        locs.append(len(np.unique(res[i][0][:, :2], axis=0)))
-    print(f"[case] batch64: keypoint locations per image {min(locs)}..{max(locs)}, median {int(np.median(locs))}",
-          flush=True)
-    assert np.median(locs) >= 2000 and max(locs) <= 8000
     for i in _sample([im.shape for im in imgs]):
         rk, rd, ru = O.detect_describe(imgs[i], O.params())
         print(f"[case] batch64 image {i} {imgs[i].shape}: oracle {len(rk)} keypoints", flush=True)
@@ -117,6 +112,10 @@ def test_pairs32_batched_match_ransac_equals_per_pair_and_oracle():
     plan.pack(imgs)
     plan.run(graph=False)
     res = plan.results(u8=True)
+    counts = [k.shape[0] for k, _ in res]
+    print(f"[case] pairs32: keypoints per image {min(counts)}..{max(counts)}, median {int(np.median(counts))}",
+          flush=True)
+    assert 2000 <= min(counts) and max(counts) <= 8000
     A, B = res[0::2], res[1::2]
     mp = MatchBatchPlan([k.shape[0] for k, _ in A], [k.shape[0] for k, _ in B])
     DA = mp.pack([d for _, d in A], mp.Ns)
@@ -182,7 +181,7 @@ def test_12mp_pair_detect_match_ransac_panorama():
         np.testing.assert_array_equal(kp.cpu().numpy(), rk)
         np.testing.assert_array_equal(df.cpu().numpy(), rd)
         np.testing.assert_array_equal(du.cpu().numpy(), ru)
-        assert 20000 <= locs <= 40000
+        assert 20000 <= len(rk) <= 40000 and locs >= 20000
         out.append((kp, du))
     (kpa, dua), (kpb, dub) = out
     rf, rb, rpairs = O.match_u8(dua.cpu().numpy(), dub.cpu().numpy(), 0.75)
