@@ -290,7 +290,7 @@ def bench_config4(torch, dev, iters=3, check=True):
     from pysift_amd.matcher import match_f16, match_u8, match_workspace_bytes
     from pysift_amd.ransac import ransac_homography
     from pysift_amd.stitch import compute_canvas_size
-    a, b, Hgt = syn.synth_warped_pair(3000, 4000, 4000)
+    a, b, Hgt = syn.synth_12mp_pair()
     H, W = a.shape
     s = PySIFT(max_keypoints=40000, max_cands_per_octave=200000, device=dev)
     ta, tb = torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev)
