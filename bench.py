@@ -84,6 +84,79 @@ def cpu_baseline(views, seconds=12.0):
             "cpu_model": model, "host_cpus": os.cpu_count()}
 
 
+def _timed(fn, min_s=0.0):
+    """(seconds per call, calls) of fn() repeated until min_s has passed (once at least)."""
+    n, t0 = 0, time.perf_counter()
+    while True:
+        fn()
+        n += 1
+        if time.perf_counter() - t0 >= min_s:
+            return (time.perf_counter() - t0) / n, n
+
+
+def cpu_legs(threads):
+    """The CPU oracle (the restated classic pipeline) on the other workloads,
+    at `threads` OpenMP threads and at one thread, each on a bounded sample:
+    * exact 40000 x 40000 uint8 matching: a slice of the query rows against
+      all 40000 (the oracle's cost is linear in the rows), scaled to 40000;
+    * 16384-hypothesis FP64 RANSAC on 5000 synthetic correspondences;
+    * the 1024x768 BGR stitch (CLAHE, detect x2, exact match, 2000-hypothesis
+      RANSAC, feather warp);
+    * detectAndCompute over a fixed 8-image sample of the 64-image batch,
+      scaled to images per second."""
+    import numpy as np
+    from oracle import oracle as O
+    from pysift_amd import synthetic as syn
+    out = {}
+    rng = np.random.default_rng(9)
+    n = 40000
+    a = rng.integers(0, 256, (n, 128)).astype(np.uint8)
+    b = a.copy()
+    b[: n // 2] = np.clip(a[: n // 2].astype(int) + rng.integers(-4, 5, (n // 2, 128)), 0, 255).astype(np.uint8)
+    b[n // 2:] = rng.integers(0, 256, (n - n // 2, 128)).astype(np.uint8)
+    pts1 = rng.uniform(0, 1000, (5000, 2))
+    Hs = np.array([[1.02, 0.05, 30.0], [-0.04, 0.98, 12.0], [1e-5, -2e-5, 1.0]])
+    q = np.c_[pts1, np.ones(5000)] @ Hs.T
+    pts2 = q[:, :2] / q[:, 2:] + rng.normal(0, 0.5, (5000, 2))
+    pts2[:1000] = rng.uniform(0, 1000, (1000, 2))
+    va, vb, _ = syn.synth_pair(seed=1000)
+    imgs = syn.synth_batch(64, 2000)
+    sample = imgs[::8]
+    for tag, thr, rows in (("threads", threads, 4000), ("single_thread", 1, 500)):
+        O.set_threads(thr)
+        sec, _ = _timed(lambda: O.match_u8(a[:rows], b, 0.75))
+        ms40k = sec * 1e3 * n / rows
+        leg = {"threads": thr,
+               "match_40000x40000": {"ms": round(ms40k, 1), "tops": round(2.0 * n * n * 128 / (ms40k * 1e-3) / 1e12, 4),
+                                     "sample": f"{rows} x {n} rows, scaled to {n} x {n}"}}
+        sec, _ = _timed(lambda: O.ransac(pts1, pts2, seed=42, n_hyp=16384, thr=3.0))
+        leg["ransac_16384"] = {"ms": round(sec * 1e3, 1), "sample": "5000 correspondences, 20% outliers"}
+
+        def stitch():
+            p4 = O.params(n_octaves=4)
+            kd = []
+            for v in (va, vb):
+                k, _, du = O.detect_describe(O.clahe(O.bgr_to_gray(v), 2.0, (8, 8)), p4)
+                kd.append((k, du))
+            _, _, pr = O.match_u8(kd[0][1], kd[1][1], 0.75)
+            H, _ = O.ransac(kd[0][0][pr[:, 0], :2].astype(np.float64), kd[1][0][pr[:, 1], :2].astype(np.float64),
+                            seed=42, n_hyp=2000, thr=3.0)
+            cw, ch, ox, oy = O.canvas(768, 1024, 768, 1024, H)
+            O.warp_feather(va, vb, H, cw, ch, ox, oy)
+        sec, _ = _timed(stitch)
+        leg["stitch_1024x768"] = {"ms": round(sec * 1e3, 1)}
+        samp = sample if thr > 1 else sample[:2]
+        t0 = time.perf_counter()
+        for im in samp:
+            O.detect_describe(im, O.params())
+        dt = time.perf_counter() - t0
+        leg["batch64_detect"] = {"img_s": round(len(samp) / dt, 2), "mpix_s": round(sum(i.size for i in samp) / 1e6 / dt, 3),
+                                 "sample": f"{len(samp)} of the 64 images ({sum(i.size for i in samp) / 1e6:.2f} MP)"}
+        out[tag] = leg
+    O.set_threads(threads)
+    return out
+
+
 def bench_match(torch, dev, n, method, iters=5):
     """Exact INT8-MFMA matching of two n x 128 uint8 descriptor sets; returns
     (TOPS counted once as 2 n^2 128 / t, ms, mutual pairs)."""
@@ -187,7 +260,16 @@ def bench_batch(torch, dev, n=64, iters=5):
     torch.cuda.synchronize()
     t_single = (time.perf_counter() - t0) * 1e3 / iters
     mp = sum(i.size for i in imgs) / 1e6
-    return {"images": n, "mpix": round(mp, 2), "batched_img_s": round(n / (t_batch * 1e-3), 1),
+    # identical outputs: every image's keypoints and descriptors from the
+    # batched call against the per-image path
+    res = plan.results()
+    identical = True
+    for (kb, db), img in zip(res, dimgs):
+        kp, df, _, cnt = s.launch(img, outs)
+        k = int(cnt.item())
+        identical = identical and kb.shape[0] == k and torch.equal(kb, kp[:k]) and torch.equal(db, df[:k])
+    return {"images": n, "mpix": round(mp, 2), "identical": bool(identical),
+            "batched_img_s": round(n / (t_batch * 1e-3), 1),
             "batched_ms": round(t_batch, 3), "per_image_path_img_s": round(n / (t_single * 1e-3), 1),
             "per_image_path_ms": round(t_single, 3), "speedup": round(t_single / t_batch, 2),
             "blur_tile": tile, "tune_ms": {str(k): round(v, 3) for k, v in tune.items()},
@@ -655,12 +737,15 @@ def main():
         extra["clahe"] = bench_clahe(torch, dev, gray[0])
         extra["match_float_pca64"] = bench_match_float(torch, dev)
         extra["composite_multiband"] = bench_composite(torch, dev, va, vb, Hgt)
+        extra["config4_12mp_pair"] = bench_config4(torch, dev)
         st_ms, met = bench_stitch(va, vb)
         extra["stitch_pair_ms"] = round(st_ms, 3)
         extra["stitch_metrics"] = {k: met[k] for k in ("kp1", "kp2", "matches", "inliers")}
         line["extra"] = extra
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(gray)
+        if not args.no_extra:
+            line["cpu_baseline"]["other_workloads"] = cpu_legs(line["cpu_baseline"]["cores"])
     print(json.dumps(line))
 
 
