@@ -159,14 +159,17 @@ def cpu_legs(threads):
 
 def bench_match(torch, dev, n, method, iters=5):
     """Exact INT8-MFMA matching of two n x 128 uint8 descriptor sets; returns
-    (TOPS counted once as 2 n^2 128 / t, ms, mutual pairs)."""
-    from pysift_amd.matcher import match_u8, match_workspace_bytes
+    (TOPS counted once as 2 n^2 128 / t, ms, mutual pairs).  The two-pass
+    method's tile shape is autotuned first (untimed, cached per size class)."""
+    from pysift_amd.matcher import autotune, match_u8, match_workspace_bytes
     g = torch.Generator(device="cpu").manual_seed(4)
     a = torch.randint(0, 256, (n, 128), dtype=torch.uint8, generator=g)
     b = a.clone()
     b[: n // 2] = torch.clamp(a[: n // 2].int() + torch.randint(-4, 5, (n // 2, 128), generator=g), 0, 255).to(torch.uint8)
     b[n // 2:] = torch.randint(0, 256, (n - n // 2, 128), dtype=torch.uint8, generator=g)
     a, b = a.to(dev), b.to(dev)
+    if method == "twopass":
+        autotune(n, n, dev)
     ws = torch.empty(match_workspace_bytes(n, n), dtype=torch.uint8, device=dev)
     out = match_u8(a, b, 0.75, method, sync=False, ws=ws)
     torch.cuda.synchronize()
@@ -279,7 +282,7 @@ def bench_batch(torch, dev, n=64, iters=5):
 def bench_pairs(torch, dev, n_pairs=32, n_hyp=1024, iters=5):
     """32 homography-warped pairs at the uneven batch sizes: the 64 images are
     detected by one batched graph replay, then every pair is matched in one
-    launch (exact INT8 MFMA, shift method: a - 128 as int8) and verified by batched RANSAC
+    launch (exact INT8 MFMA, two-pass method: a - 128 as int8) and verified by batched RANSAC
     (1024 Philox hypotheses per pair, 3 px), against the per-pair loop
     (match_u8 + RANSAC per pair) on the same descriptors."""
     import numpy as np
@@ -314,7 +317,7 @@ def bench_pairs(torch, dev, n_pairs=32, n_hyp=1024, iters=5):
     rp = RansacBatchPlan(mp.Ns, n_hyp, dev)
 
     def batched():
-        mp.run(DA, DB, 0.75, "shift")
+        mp.run(DA, DB, 0.75, "twopass")
         p1, p2 = mp.gather_points(KA, KB)
         return rp.run(p1, p2, mp.npairs, seed=42, thr=3.0)
 
@@ -331,7 +334,7 @@ def bench_pairs(torch, dev, n_pairs=32, n_hyp=1024, iters=5):
         out = []
         for p in range(n_pairs):
             (ka, da), (kb, db) = A[p], B[p]
-            _, _, pr = match_u8(da, db, 0.75, "shift")
+            _, _, pr = match_u8(da, db, 0.75, "twopass")
             if pr.shape[0] < 4:
                 out.append(None)
                 continue
@@ -369,7 +372,7 @@ def bench_config4(torch, dev, iters=3, check=True):
     import numpy as np
     from pysift_amd import GPUPyStitch, PySIFT
     from pysift_amd import synthetic as syn
-    from pysift_amd.matcher import match_f16, match_u8, match_workspace_bytes
+    from pysift_amd.matcher import autotune, match_f16, match_u8, match_workspace_bytes
     from pysift_amd.ransac import ransac_homography
     from pysift_amd.stitch import compute_canvas_size
     a, b, Hgt = syn.synth_12mp_pair()
@@ -400,7 +403,8 @@ def bench_config4(torch, dev, iters=3, check=True):
     Bp = pyramid_bytes(H, W, 5)
     res["pyramid_roofline"] = {"algorithmic_bytes": Bp, "achieved_gbs": round(Bp / (t_pyr * 1e-3) / 1e9, 1),
                                "frac": round(Bp / (t_pyr * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
-    for method in ("hilo", "shift"):
+    autotune(N, M, dev)
+    for method in ("hilo", "shift", "twopass"):
         match_u8(dua, dub, 0.75, method, sync=False, ws=ws)
         torch.cuda.synchronize()
         e0.record()
@@ -411,7 +415,12 @@ def bench_config4(torch, dev, iters=3, check=True):
         ms = e0.elapsed_time(e1) / iters
         res[f"match_{method}"] = {"ms": round(ms, 3), "tops": round(2.0 * N * M * 128 / (ms * 1e-3) / 1e12, 2),
                                   "pairs": int(npairs.item())}
-    fwd, bwd, pairs = match_u8(dua, dub, 0.75, "hilo", ws=ws)
+    fwd, bwd, pairs = match_u8(dua, dub, 0.75, "twopass", ws=ws)
+    same = True
+    for method in ("hilo", "shift"):
+        f2, b2, p2 = match_u8(dua, dub, 0.75, method, ws=ws)
+        same = same and bool(torch.equal(f2, fwd) and torch.equal(b2, bwd) and torch.equal(p2, pairs))
+    res["methods_identical"] = same
     if check:
         from oracle import oracle as O
         O.set_threads(min(os.cpu_count() or 8, 16))
@@ -646,7 +655,7 @@ def main():
     va, vb, Hgt = syn.synth_pair(seed=1000)
     if args.only:
         if args.only == "match":
-            out = {m: bench_match(torch, dev, args.match_n, m) for m in ("hilo", "shift")}
+            out = {m: bench_match(torch, dev, args.match_n, m) for m in ("hilo", "shift", "twopass")}
             out["f16"] = bench_match_f16(torch, dev, args.match_n)
             print(json.dumps(out))
         elif args.only == "batch":
@@ -726,7 +735,7 @@ def main():
     }
     if not args.no_extra and not args.pyramid_only:
         extra = {}
-        for method in ("hilo", "shift"):
+        for method in ("hilo", "shift", "twopass"):
             tops, ms, npairs = bench_match(torch, dev, args.match_n, method)
             extra[f"match_{method}"] = {"n": args.match_n, "tops": round(tops, 2), "ms": round(ms, 4),
                                         "mfma_int8_peak_tops": PEAK_INT8_TOPS, "pairs": npairs}

@@ -140,6 +140,9 @@ int msift_pyramid_view(msift_handle* h, int H, int W, void* d_ws, int o,
  * _match_features / _match_symmetric (core.py:2840-2940, 2652-2701).
  * d2 = sum (a-b)^2 is computed on INT8 MFMA with INT32 accumulation.
  *   method 0: high/low byte split (a = 128 h + l)  1: signed shift (a-128)
+ *   2: the shift arithmetic as two one-sided passes (top-2 over b of every
+ *      a, then over a of every b): twice the MFMA work, about half the
+ *      integer work per distance of 0 and 1; the fastest on MI355X
  * Outputs: d_fwd[N*3] = best j, d1, d2 over b;  d_bwd[M*3] = best i, d1, d2
  * over a (ties keep the smallest index);  d_pairs[2*min(N,M)] mutual pairs
  * (i, j) passing d1 < ratio^2 d2 in both directions, in increasing i;
@@ -148,6 +151,13 @@ int msift_match_workspace_bytes(int N, int M, size_t* bytes);
 int msift_match_u8(const uint8_t* d_a, int N, const uint8_t* d_b, int M, double ratio, int method,
                    void* d_ws, size_t ws_bytes, int32_t* d_fwd, int32_t* d_bwd,
                    int32_t* d_pairs, int* d_npairs, void* stream);
+
+/* Tile shape of method 2 for the whole process (no reference counterpart;
+ * the reference has no tile choice): shape 0 = 64 a-rows per wave with the
+ * next tile's MFMAs issued ahead, 1 = 64 rows, 2 = 32 rows; chunks_per_block
+ * = 512-column chunks per workgroup, 1..64.  Outputs do not depend on it; the
+ * autotuner (pysift_amd.matcher.autotune) sets it.  Default (0, 4). */
+int msift_match_set_tile(int shape, int chunks_per_block);
 
 /* Allocation-free variant of msift_match_u8 (no counterpart in the
  * reference: it is for callers that cannot allocate device memory, such as

@@ -27,7 +27,7 @@ EXPORTS = [
     "msift_match_batch_gather_points", "msift_ransac_batch_create", "msift_ransac_batch_destroy",
     "msift_ransac_batch_sizes", "msift_ransac_homography_batched", "msift_clahe_workspace_bytes", "msift_clahe",
     "msift_l2_normalize", "msift_pca_workspace_bytes", "msift_pca_fit", "msift_pca_project", "msift_match_sim_f16",
-    "msift_composite_workspace_bytes", "msift_composite_multiband", "msift_match_u8_noalloc",
+    "msift_composite_workspace_bytes", "msift_composite_multiband", "msift_match_u8_noalloc", "msift_match_set_tile",
 ]
 
 
@@ -65,6 +65,8 @@ def _declare(L):
     L.msift_match_workspace_bytes.argtypes = [_I, _I, ctypes.POINTER(_SZ)]
     L.msift_match_u8.restype = _I
     L.msift_match_u8.argtypes = [_P, _I, _P, _I, _D, _I, _P, _SZ, _P, _P, _P, _P, _P]
+    L.msift_match_set_tile.restype = _I
+    L.msift_match_set_tile.argtypes = [_I, _I]
     L.msift_match_u8_noalloc.restype = _I
     L.msift_match_u8_noalloc.argtypes = [_P, _I, _P, _I, _D, _I, _P, _P, _P, _P, _P]
     L.msift_match_f16.restype = _I

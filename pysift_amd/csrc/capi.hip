@@ -164,7 +164,7 @@ int msift_match_workspace_bytes(int N, int M, size_t* bytes) {
 int msift_match_u8(const uint8_t* d_a, int N, const uint8_t* d_b, int M, double ratio, int method, void* d_ws,
                    size_t ws_bytes, int32_t* d_fwd, int32_t* d_bwd, int32_t* d_pairs, int* d_npairs,
                    void* stream) {
-    if (N < 0 || M < 0 || !(ratio > 0.0) || (method != 0 && method != 1) || !d_npairs) {
+    if (N < 0 || M < 0 || !(ratio > 0.0) || (method < 0 || method > 2) || !d_npairs) {
         set_error("msift_match_u8: bad arguments");
         return MSIFT_EINVAL;
     }
@@ -174,12 +174,14 @@ int msift_match_u8(const uint8_t* d_a, int N, const uint8_t* d_b, int M, double 
 
 int msift_match_u8_noalloc(const uint8_t* d_a, int N, const uint8_t* d_b, int M, double ratio, int method,
                            int32_t* d_fwd, int32_t* d_bwd, int32_t* d_pairs, int* d_npairs, void* stream) {
-    if (N < 0 || M < 0 || !(ratio > 0.0) || (method != 0 && method != 1) || !d_npairs) {
+    if (N < 0 || M < 0 || !(ratio > 0.0) || (method < 0 || method > 2) || !d_npairs) {
         set_error("msift_match_u8_noalloc: bad arguments");
         return MSIFT_EINVAL;
     }
     return match_u8_noalloc(d_a, N, d_b, M, ratio, method, d_fwd, d_bwd, d_pairs, d_npairs, (hipStream_t)stream);
 }
+
+int msift_match_set_tile(int shape, int chunks_per_block) { return match_set_tile(shape, chunks_per_block); }
 
 int msift_match_f16(const float* d_a, int N, const float* d_b, int M, double ratio, void* d_ws, size_t ws_bytes,
                     int32_t* d_fwd, int32_t* d_bwd, int32_t* d_pairs, int* d_npairs, void* stream) {

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <vector>
 #include <mutex>
+#include <atomic>
 
 #pragma clang fp contract(off)
 
@@ -75,6 +76,61 @@ __global__ __launch_bounds__(256) void k_norms(const uint8_t* __restrict__ a, in
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m);
     if (lane == 0) out[i] = s;
+}
+
+// one pair of a batched match
+struct MPair {
+    int a_off, N, b_off, M;   // rows of a / b in the packed sets
+    int nrb, ncc, tile0;      // row blocks, column chunks, first tile
+    int blk0, nblk;           // first 256-row block of the mutual pass, its count
+    int tp0, xb1, cg1, xb2, cg2;   // two-pass: first tile; x-blocks and chunk groups of pass 1 and 2
+    size_t o_rowpart, o_colpart;   // workspace offsets of the partial top-2 keys
+};
+
+// two-pass prologue over the rows of a and b (8 threads a row, 16 bytes each):
+// the y' = y - 128 copy, |y'|^2 (the reduction's adj) and the finished key term
+// ((|y'|^2 + 2 sum y' + 2^21) << 9) + (row mod 512) that a row contributes as y
+// (batched: rows of many pairs packed back to back, pairs != nullptr; the row
+// index in the key term is then counted from the row's own pair start)
+__global__ __launch_bounds__(256) void k_tp_prep(const uint8_t* __restrict__ a, int N, const uint8_t* __restrict__ b,
+                                                  int M, uint8_t* __restrict__ ac, uint8_t* __restrict__ bc,
+                                                  int* __restrict__ na, int* __restrict__ nb,
+                                                  uint32_t* __restrict__ ta, uint32_t* __restrict__ tb,
+                                                  const MPair* __restrict__ pairs, int np) {
+    const int r = blockIdx.x * 32 + (threadIdx.x >> 3), q = threadIdx.x & 7;
+    if (r >= N + M) return;   // whole groups of 8 leave together
+    const bool isb = r >= N;
+    const int i = isb ? r - N : r;
+    int off = 0;
+    if (pairs) {   // last pair whose set starts at or before row i (pair 0 starts at 0)
+        int lo = 0, hi = np - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if ((isb ? pairs[mid].b_off : pairs[mid].a_off) <= i) lo = mid; else hi = mid - 1;
+        }
+        off = isb ? pairs[lo].b_off : pairs[lo].a_off;
+    }
+    const uint8_t* src = (isb ? b : a) + (size_t)i * 128 + 16 * q;
+    const v4i v = *reinterpret_cast<const v4i*>(src) ^ (v4i)(int)0x80808080;
+    *reinterpret_cast<v4i*>((isb ? bc : ac) + (size_t)i * 128 + 16 * q) = v;
+    int s2 = 0, s1 = 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int x = (int)(int8_t)(v[e] >> (8 * k));
+            s2 += x * x;
+            s1 += x;
+        }
+#pragma unroll
+    for (int m = 1; m < 8; m <<= 1) {
+        s2 += __shfl_xor(s2, m);
+        s1 += __shfl_xor(s1, m);
+    }
+    if (q == 0) {
+        (isb ? nb : na)[i] = s2;
+        (isb ? tb : ta)[i] = (((uint32_t)(s2 + 2 * s1) + kKeyOff) << 9) + (uint32_t)((i - off) & 511);
+    }
 }
 
 // one 256-row x 512-column tile (row block rb, column chunk cc) of a
@@ -296,6 +352,201 @@ __global__ __launch_bounds__(512) void k_match_tiles(const uint8_t* __restrict__
     match_tile<METHOD>(A, N, B, M, na, nb, rowpart, colpart, blockIdx.x, blockIdx.y, sm);
 }
 
+// ------------------------------------------------------------ two-pass ----
+// method 2 (two passes): the shift method's arithmetic, run as two passes of
+// one kernel, each the top-2 of every descriptor x of X over all of Y (pass
+// 1: X = a, Y = b; pass 2: X = b, Y = a).  The MFMA takes the streamed
+// y' = y - 128 as its A operand and the fixed ~x' = -(x - 128) - 1 as B, so
+// lane l's accumulator holds the distances of ONE x (column l % 32 of the
+// 32x32 output) to 16 y (rows 8 (r / 4) + 4 (l / 32) + r % 4): the top-2 is a
+// register pair per x, one shift-add and one median insert per distance, and
+// the two half-waves merge once per 512-y chunk.  It does twice the fused
+// kernel's MFMA work and none of its per-step column merges, per-chunk row
+// butterflies or per-step barriers: the fused kernel is bound by that VALU
+// and synchronisation work, not by the MFMA.
+//   Keys, with dn = -x'.y' - sum y' (the accumulator) and ty = |y'|^2 + 2 sum y':
+//   ((d - |x'|^2 + 2^21) << 9) | lj = (dn << 10) + (((ty + 2^21) << 9) + lj),
+//   lj = y mod 512: the shift method's row keys, so k_reduce_top2 with adj
+//   |x'|^2 finishes either pass.  k_tp_prep writes the y' copies and the
+//   bracketed term per row, so a stage is a plain copy (global -> LDS direct,
+//   no registers or VALU).
+constexpr int kTpStage = 256;   // y per LDS stage, two stages per 512-y chunk
+constexpr int kTpWaves = 4;     // waves per workgroup
+
+struct TpSmem {
+    alignas(16) uint8_t y[kTpStage * 128];   // y' rows: 16-byte block q of row c at block position q ^ (c % 8)
+    alignas(16) uint32_t t[kTpStage];        // key terms ((ty + 2^21) << 9) + lj
+};
+
+// NT x-tiles of 32 per wave; PIPE: in a stage whose 256 y are all in range,
+// tile t + 1's MFMAs are issued before tile t's inserts (two accumulator sets)
+// x-block xb (kTpWaves * 32 * NT descriptors of X) against 512-y chunks
+// cg * cpw .. +cpw of Y; part[(chunk * NX + x) * 2 + {0, 1}] = that chunk's top-2 keys
+template <int NT, bool PIPE>
+__device__ __forceinline__ void match_side_tile(const uint8_t* __restrict__ X, int NX, const uint8_t* __restrict__ Yc,
+                                                int NY, const uint32_t* __restrict__ ty, uint32_t* __restrict__ part,
+                                                int xb, int cg, int cpw, TpSmem& sm) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int l32 = lane & 31, hi = lane >> 5;
+    const int xw = xb * (kTpWaves * 32 * NT) + w * 32 * NT;   // this wave's first x
+    // B fragments: lane holds x = xw + 32 n + l32, bytes [32 ks + 16 hi, +16), fed as ~x'
+    v4i xf[NT][4];
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+        const int x = min(xw + 32 * n + l32, NX - 1);
+        const v4i* src = reinterpret_cast<const v4i*>(X + (size_t)x * 128);
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) xf[n][ks] = src[2 * ks + hi] ^ (v4i)(int)0x7f7f7f7f;
+    }
+    // LDS offsets of the lane's A fragment blocks (row l32 of a 32-y tile; 32 t % 8 == 0)
+    int yoff[4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) yoff[ks] = l32 * 128 + 16 * ((2 * ks + hi) ^ (l32 & 7));
+    auto fetch = [&](int t, v4i* yf, v4i* tv) {
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) yf[ks] = *reinterpret_cast<const v4i*>(&sm.y[t * 4096 + yoff[ks]]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) tv[q] = *reinterpret_cast<const v4i*>(&sm.t[32 * t + 8 * q + 4 * hi]);
+    };
+    auto mma = [&](const v4i* yf, v16i* acc) {
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+            acc[n] = (v16i)0;
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+                acc[n] = __builtin_amdgcn_mfma_i32_32x32x32_i8(yf[ks], xf[n][ks], acc[n], 0, 0, 0);
+        }
+    };
+    const int nch = (NY + 511) / 512;
+    const int ch0 = cg * cpw, ch1 = min(nch, ch0 + cpw);
+    for (int ch = ch0; ch < ch1; ++ch) {
+        uint32_t k1[NT], k2[NT];
+#pragma unroll
+        for (int n = 0; n < NT; ++n) { k1[n] = 0xffffffffu; k2[n] = 0xffffffffu; }
+        auto insert_full = [&](const v16i* acc, const v4i* tv) {
+#pragma unroll
+            for (int n = 0; n < NT; ++n)
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    top2_insert(k1[n], k2[n], ((uint32_t)acc[n][r] << 10) + (uint32_t)tv[r >> 2][r & 3]);
+        };
+        const int yend = min(NY, ch * 512 + 512);
+        for (int ybase = ch * 512; ybase < yend; ybase += kTpStage) {
+            __syncthreads();   // the previous stage's reads are done
+            // Instruction i of wave w fills stage rows 8 (8 w + i) .. +8 (1 KB): lane L's 16
+            // bytes land at byte 16 L, i.e. row 8 (8 w + i) + L / 8, block position L % 8,
+            // which holds the row's block (L % 8) ^ (row % 8).  Rows past NY repeat row NY - 1
+            // (their keys are never inserted).
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int c = (8 * w + i) * 8 + (lane >> 3), q = (lane & 7) ^ (c & 7);
+                const int y = min(ybase + c, NY - 1);
+                __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void*)(Yc + (size_t)y * 128 + 16 * q),
+                                                 (__attribute__((address_space(3))) void*)&sm.y[(8 * w + i) * 1024], 16,
+                                                 0, 0);
+            }
+            {
+                const int y = min(ybase + 64 * w + lane, NY - 1);
+                __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void*)(ty + y),
+                                                 (__attribute__((address_space(3))) void*)&sm.t[64 * w], 4, 0, 0);
+            }
+            __builtin_amdgcn_s_waitcnt(0);   // the copies have landed in LDS
+            __syncthreads();
+            if (PIPE && ybase + kTpStage <= NY) {
+                v16i accP[NT], accQ[NT];
+                v4i yf[4], tvP[4], tvQ[4];
+                fetch(0, yf, tvP);
+                mma(yf, accP);
+#pragma unroll
+                for (int t = 0; t < kTpStage / 32; t += 2) {
+                    fetch(t + 1, yf, tvQ);
+                    mma(yf, accQ);
+                    insert_full(accP, tvP);
+                    if (t + 2 < kTpStage / 32) {
+                        fetch(t + 2, yf, tvP);
+                        mma(yf, accP);
+                    }
+                    insert_full(accQ, tvQ);
+                }
+                continue;
+            }
+            const int nt = min(kTpStage / 32, (NY - ybase + 31) / 32);
+            for (int t = 0; t < nt; ++t) {
+                v4i yf[4], tv[4];
+                fetch(t, yf, tv);
+                v16i acc[NT];
+                mma(yf, acc);
+                if (ybase + 32 * t + 32 <= NY) {   // uniform over the block
+                    insert_full(acc, tv);
+                } else {
+#pragma unroll
+                    for (int n = 0; n < NT; ++n)
+#pragma unroll
+                        for (int r = 0; r < 16; ++r) {
+                            const bool v = ybase + 32 * t + 8 * (r >> 2) + 4 * hi + (r & 3) < NY;
+                            const uint32_t k = ((uint32_t)acc[n][r] << 10) + (uint32_t)tv[r >> 2][r & 3];
+                            top2_insert(k1[n], k2[n], v ? k : 0xffffffffu);
+                        }
+                }
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < NT; ++n) {
+            const uint32_t o1 = __shfl_xor(k1[n], 32), o2 = __shfl_xor(k2[n], 32);
+            top2_merge(k1[n], k2[n], o1, o2);
+            const int x = xw + 32 * n + l32;
+            if (hi == 0 && x < NX) {
+                part[((size_t)ch * NX + x) * 2] = k1[n];
+                part[((size_t)ch * NX + x) * 2 + 1] = k2[n];
+            }
+        }
+    }
+}
+
+// both passes in one launch: blockIdx.z = pass; blocks past a pass's extent leave
+struct TpPass {
+    const uint8_t* X;
+    const uint8_t* Yc;
+    const uint32_t* ty;
+    uint32_t* part;
+    int NX, NY;
+};
+
+template <int NT, bool PIPE>
+__global__ __launch_bounds__(256) void k_match_side(TpPass p0, TpPass p1, int cpw) {
+    __shared__ TpSmem sm;
+    const TpPass P = blockIdx.z ? p1 : p0;
+    if ((int)blockIdx.x * (kTpWaves * 32 * NT) >= P.NX || (int)blockIdx.y * cpw * 512 >= P.NY) return;
+    match_side_tile<NT, PIPE>(P.X, P.NX, P.Yc, P.NY, P.ty, P.part, blockIdx.x, blockIdx.y, cpw, sm);
+}
+
+// tile shape (x per wave, pipelining) and 512-y chunks per workgroup of the
+// two-pass kernel; results do not depend on them (set by the autotuner)
+static std::atomic<int> g_tp_shape{0}, g_tp_cpw{4};
+
+static void tp_launch(const TpPass& p0, const TpPass& p1, hipStream_t s) {
+    const int shape = g_tp_shape.load(std::memory_order_relaxed), cpw = g_tp_cpw.load(std::memory_order_relaxed);
+    const int xblk = kTpWaves * 32 * (shape == 2 ? 1 : 2);
+    auto groups = [&](int ny) { return ((ny + 511) / 512 + cpw - 1) / cpw; };
+    const dim3 grid((std::max(p0.NX, p1.NX) + xblk - 1) / xblk, std::max(groups(p0.NY), groups(p1.NY)), 2);
+    if (shape == 0)
+        hipLaunchKernelGGL((k_match_side<2, true>), grid, dim3(256), 0, s, p0, p1, cpw);
+    else if (shape == 1)
+        hipLaunchKernelGGL((k_match_side<2, false>), grid, dim3(256), 0, s, p0, p1, cpw);
+    else
+        hipLaunchKernelGGL((k_match_side<1, false>), grid, dim3(256), 0, s, p0, p1, cpw);
+}
+
+int match_set_tile(int shape, int cpw) {
+    if (shape < 0 || shape > 2 || cpw < 1 || cpw > 64) {
+        set_error("match_set_tile: shape must be 0, 1 or 2 and chunks per workgroup in [1, 64]");
+        return MSIFT_EINVAL;
+    }
+    g_tp_shape.store(shape);
+    g_tp_cpw.store(cpw);
+    return MSIFT_OK;
+}
+
 // ---------------------------------------------------------------- float ----
 // Float descriptor path: inputs rounded to fp16, a.b on v_mfma_f32_32x32x16_f16
 // with fp32 accumulation, norms in fp32 from the rounded values.  For unit
@@ -505,18 +756,13 @@ __global__ __launch_bounds__(256) void k_to_f16_pad(const float* __restrict__ in
 // Pairs (a_p, b_p) of descriptor sets packed back to back; one launch covers
 // every tile of every pair, a block finds its pair in the table by a ballot
 // over the pairs' first tile.
-struct MPair {
-    int a_off, N, b_off, M;   // rows of a / b in the packed sets
-    int nrb, ncc, tile0;      // row blocks, column chunks, first tile
-    int blk0, nblk;           // first 256-row block of the mutual pass, its count
-    size_t o_rowpart, o_colpart;   // workspace offsets of the partial top-2 keys
-};
 
+template <bool TWOPASS = false>
 __device__ __forceinline__ int pair_of_tile(const MPair* __restrict__ pairs, int np, int t) {
     int last = 0;
     for (int b0 = 0; b0 < np; b0 += 64) {
         const int k = b0 + (int)(threadIdx.x & 63);
-        const bool le = k < np && pairs[k].tile0 <= t;
+        const bool le = k < np && (TWOPASS ? pairs[k].tp0 : pairs[k].tile0) <= t;
         const unsigned long long m = __ballot(le);
         if (m) last = b0 + 63 - __clzll(m);
         if (m != ~0ull) break;
@@ -540,6 +786,33 @@ __global__ __launch_bounds__(512) void k_match_tiles_b(const MPair* __restrict__
     const int t = (int)blockIdx.x - P.tile0;
     match_tile<METHOD>(A + (size_t)P.a_off * 128, P.N, B + (size_t)P.b_off * 128, P.M, na + P.a_off, nb + P.b_off,
                        (uint32_t*)(ws + P.o_rowpart), (uint32_t*)(ws + P.o_colpart), t % P.nrb, t / P.nrb, sm);
+}
+
+// two-pass tiles of every pair (pass 1 tiles, then pass 2), fixed shape: 64
+// x per wave with the next tile's MFMAs issued ahead, one 512-y chunk a workgroup
+constexpr int kTpBatchXBlk = kTpWaves * 64;
+__global__ __launch_bounds__(256) void k_match_side_b(const MPair* __restrict__ pairs, int np,
+                                                       const uint8_t* __restrict__ A, const uint8_t* __restrict__ B,
+                                                       const uint8_t* __restrict__ Ac, const uint8_t* __restrict__ Bc,
+                                                       const uint32_t* __restrict__ ta, const uint32_t* __restrict__ tb,
+                                                       char* __restrict__ ws) {
+    __shared__ TpSmem sm;
+    __shared__ int s_p;
+    if (threadIdx.x < 64) {
+        const int p = pair_of_tile<true>(pairs, np, (int)blockIdx.x);
+        if (threadIdx.x == 0) s_p = p;
+    }
+    __syncthreads();
+    const MPair P = pairs[s_p];
+    int t = (int)blockIdx.x - P.tp0;
+    if (t < P.xb1 * P.cg1)
+        match_side_tile<2, true>(A + (size_t)P.a_off * 128, P.N, Bc + (size_t)P.b_off * 128, P.M, tb + P.b_off,
+                                 (uint32_t*)(ws + P.o_rowpart), t % P.xb1, t / P.xb1, 1, sm);
+    else {
+        t -= P.xb1 * P.cg1;
+        match_side_tile<2, true>(B + (size_t)P.b_off * 128, P.M, Ac + (size_t)P.a_off * 128, P.N, ta + P.a_off,
+                                 (uint32_t*)(ws + P.o_colpart), t % P.xb2, t / P.xb2, 1, sm);
+    }
 }
 
 // merge per-chunk partials into (best index, d1, d2); key 0xffffffff = none.
@@ -608,6 +881,36 @@ __global__ __launch_bounds__(256) void k_reduce_top2(const uint32_t* __restrict_
     }
 }
 
+// the two-pass method's two reductions in one launch (blockIdx.y = side),
+// both with 9-bit local indices over 512-y chunks
+__global__ __launch_bounds__(256) void k_reduce_top2_2(const uint32_t* __restrict__ part0, int n0, int nparts0,
+                                                        int32_t* __restrict__ out0, const int* __restrict__ adj0,
+                                                        const uint32_t* __restrict__ part1, int n1, int nparts1,
+                                                        int32_t* __restrict__ out1, const int* __restrict__ adj1) {
+    __shared__ uint64_t s_k[3][64][2];
+    const bool side = blockIdx.y != 0;
+    const uint32_t* part = side ? part1 : part0;
+    const int n = side ? n1 : n0, nparts = side ? nparts1 : nparts0;
+    if ((int)blockIdx.x * 64 >= n) return;   // whole block
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    uint64_t k1 = ~0ull, k2 = ~0ull;
+    if (i < n) reduce_top2_scan(part, n, w, 4, nparts, 9, 512, i, k1, k2);
+    if (w > 0) {
+        s_k[w - 1][lane][0] = k1;
+        s_k[w - 1][lane][1] = k2;
+    }
+    __syncthreads();
+    if (w == 0 && i < n) {
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            top2_insert64(k1, k2, s_k[o][lane][0]);
+            top2_insert64(k1, k2, s_k[o][lane][1]);
+        }
+        reduce_top2_write(side ? out1 : out0, i, k1, k2, side ? adj1 : adj0);
+    }
+}
+
 // rows (cols == 0: top-2 over b, into fwd) or columns (top-2 over a, into
 // bwd) of pair blockIdx.y; no partials (an empty side) gives "none"
 __global__ __launch_bounds__(256) void k_reduce_top2_b(const MPair* __restrict__ pairs, const char* __restrict__ ws,
@@ -616,12 +919,14 @@ __global__ __launch_bounds__(256) void k_reduce_top2_b(const MPair* __restrict__
                                                         int method) {
     const MPair P = pairs[blockIdx.y];
     const int n = cols ? P.M : P.N;
-    const int nparts = (P.N > 0 && P.M > 0) ? (cols ? P.nrb : P.ncc) : 0;
+    // method 2: both sides hold 9-bit indices over 512-chunks (rows: of b, columns: of a)
+    const bool c8 = cols && method != 2;
+    const int nparts = (P.N > 0 && P.M > 0) ? (cols ? (method == 2 ? (P.N + 511) / 512 : P.nrb) : P.ncc) : 0;
     const uint32_t* part = (const uint32_t*)(ws + (cols ? P.o_colpart : P.o_rowpart));
     int32_t* out = cols ? bwd + 3 * (size_t)P.b_off : fwd + 3 * (size_t)P.a_off;
-    const int* adj = method == 1 ? (cols ? nb + P.b_off : na + P.a_off) : nullptr;
+    const int* adj = method != 0 ? (cols ? nb + P.b_off : na + P.a_off) : nullptr;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
-        reduce_top2_row(part, n, nparts, cols ? 8 : 9, cols ? kRowBlk : kColChunk, out, i, adj);
+        reduce_top2_row(part, n, nparts, c8 ? 8 : 9, c8 ? kRowBlk : kColChunk, out, i, adj);
 }
 
 __global__ void k_fill_top2_none(int32_t* __restrict__ out, int n) {
@@ -748,7 +1053,7 @@ __global__ __launch_bounds__(256) void k_gather_points_b(const MPair* __restrict
 // ---------------------------------------------------------------- host ----
 struct MatchPlan {
     int nrb_a, ncc_a;   // row blocks of a, column chunks of b
-    size_t o_na, o_nb, o_rowpart, o_colpart, o_cnt, o_off, o_ha, o_hb, total;
+    size_t o_na, o_nb, o_na2, o_nb2, o_ac, o_bc, o_rowpart, o_colpart, o_cnt, o_off, o_ha, o_hb, total;
 };
 
 static MatchPlan match_plan(int N, int M) {
@@ -759,6 +1064,10 @@ static MatchPlan match_plan(int N, int M) {
     P.ncc_a = (M + kColChunk - 1) / kColChunk;
     P.o_na = take((size_t)(N > 0 ? N : 1) * 4);
     P.o_nb = take((size_t)(M > 0 ? M : 1) * 4);
+    P.o_na2 = take((size_t)(N > 0 ? N : 1) * 4);   // method 2's second pass
+    P.o_nb2 = take((size_t)(M > 0 ? M : 1) * 4);
+    P.o_ac = take((size_t)(N > 0 ? N : 1) * 128);   // method 2's y' copies
+    P.o_bc = take((size_t)(M > 0 ? M : 1) * 128);
     P.o_rowpart = take((size_t)P.ncc_a * (N > 0 ? N : 1) * 8);
     P.o_colpart = take((size_t)P.nrb_a * (M > 0 ? M : 1) * 8);
     int nblk = (N + 255) / 256;
@@ -796,7 +1105,7 @@ int match_u8(const uint8_t* a, int N, const uint8_t* b, int M, double ratio, int
 constexpr int kNoAllocMaxRows = 16384;
 constexpr size_t rup256(size_t b) { return (b + 255) & ~(size_t)255; }
 constexpr size_t kNoAllocScratchBytes =
-    rup256((size_t)kNoAllocMaxRows * 4) * 2 +
+    rup256((size_t)kNoAllocMaxRows * 4) * 4 + rup256((size_t)kNoAllocMaxRows * 128) * 2 +
     rup256((size_t)((kNoAllocMaxRows + kColChunk - 1) / kColChunk) * kNoAllocMaxRows * 8) +
     rup256((size_t)((kNoAllocMaxRows + kRowBlk - 1) / kRowBlk) * kNoAllocMaxRows * 8) +
     rup256((size_t)((kNoAllocMaxRows + 255) / 256 + 1) * 4) * 2;
@@ -857,17 +1166,32 @@ static int match_u8_run(const uint8_t* a, int N, const uint8_t* b, int M, double
     uint32_t* rowpart = (uint32_t*)(ws + P.o_rowpart);
     uint32_t* colpart = (uint32_t*)(ws + P.o_colpart);
     if (N > 0 && M > 0) {
-        hipLaunchKernelGGL(k_norms, dim3((N + 3) / 4), dim3(256), 0, s, a, N, method, na);
-        hipLaunchKernelGGL(k_norms, dim3((M + 3) / 4), dim3(256), 0, s, b, M, method == 1 ? 2 : method, nb);
-        dim3 grid(P.nrb_a, P.ncc_a);
-        if (method == 0)
-            hipLaunchKernelGGL(k_match_tiles<0>, grid, dim3(512), 0, s, a, N, b, M, na, nb, rowpart, colpart);
-        else
-            hipLaunchKernelGGL(k_match_tiles<1>, grid, dim3(512), 0, s, a, N, b, M, na, nb, rowpart, colpart);
-        hipLaunchKernelGGL(k_reduce_top2, dim3((N + 63) / 64), dim3(256), 0, s, rowpart, N, P.ncc_a, 9, kColChunk, fwd,
-                           method == 1 ? (const int*)na : nullptr);
-        hipLaunchKernelGGL(k_reduce_top2, dim3((M + 63) / 64), dim3(256), 0, s, colpart, M, P.nrb_a, 8, kRowBlk, bwd,
-                           method == 1 ? (const int*)nb : nullptr);
+        if (method == 2) {
+            // na: |a'|^2, nb2: |b'|^2 (the reductions' adj); nb / na2: b's / a's key terms
+            int* na2 = (int*)(ws + P.o_na2);
+            int* nb2 = (int*)(ws + P.o_nb2);
+            uint8_t* ac = (uint8_t*)(ws + P.o_ac);
+            uint8_t* bc = (uint8_t*)(ws + P.o_bc);
+            const int cha = (M + 511) / 512, chb = (N + 511) / 512;
+            hipLaunchKernelGGL(k_tp_prep, dim3((N + M + 31) / 32), dim3(256), 0, s, a, N, b, M, ac, bc, na, nb2,
+                               (uint32_t*)na2, (uint32_t*)nb, (const MPair*)nullptr, 0);
+            tp_launch(TpPass{a, bc, (const uint32_t*)nb, rowpart, N, M}, TpPass{b, ac, (const uint32_t*)na2, colpart, M, N},
+                      s);
+            hipLaunchKernelGGL(k_reduce_top2_2, dim3((std::max(N, M) + 63) / 64, 2), dim3(256), 0, s, rowpart, N, cha,
+                               fwd, (const int*)na, colpart, M, chb, bwd, (const int*)nb2);
+        } else {
+            hipLaunchKernelGGL(k_norms, dim3((N + 3) / 4), dim3(256), 0, s, a, N, method, na);
+            hipLaunchKernelGGL(k_norms, dim3((M + 3) / 4), dim3(256), 0, s, b, M, method == 1 ? 2 : method, nb);
+            dim3 grid(P.nrb_a, P.ncc_a);
+            if (method == 0)
+                hipLaunchKernelGGL(k_match_tiles<0>, grid, dim3(512), 0, s, a, N, b, M, na, nb, rowpart, colpart);
+            else
+                hipLaunchKernelGGL(k_match_tiles<1>, grid, dim3(512), 0, s, a, N, b, M, na, nb, rowpart, colpart);
+            hipLaunchKernelGGL(k_reduce_top2, dim3((N + 63) / 64), dim3(256), 0, s, rowpart, N, P.ncc_a, 9, kColChunk,
+                               fwd, method == 1 ? (const int*)na : nullptr);
+            hipLaunchKernelGGL(k_reduce_top2, dim3((M + 63) / 64), dim3(256), 0, s, colpart, M, P.nrb_a, 8, kRowBlk,
+                               bwd, method == 1 ? (const int*)nb : nullptr);
+        }
     } else {
         if (N > 0) hipLaunchKernelGGL(k_fill_top2_none, dim3((N + 255) / 256), dim3(256), 0, s, fwd, N);
         if (M > 0) hipLaunchKernelGGL(k_fill_top2_none, dim3((M + 255) / 256), dim3(256), 0, s, bwd, M);
@@ -889,8 +1213,8 @@ static int match_u8_run(const uint8_t* a, int N, const uint8_t* b, int M, double
 struct msift_match_batch {
     int np;
     std::vector<msift::MPair> pairs;
-    int sumN, sumM, tiles, blks, max_rows, max_cols, max_blk;
-    size_t o_na, o_nb, o_cnt, o_off, o_stmp, total;
+    int sumN, sumM, tiles, tp_tiles, blks, max_rows, max_cols, max_blk;
+    size_t o_na, o_nb, o_ac, o_bc, o_ta, o_tb, o_cnt, o_off, o_stmp, total;
     msift::MPair* d_pairs;
 };
 
@@ -905,7 +1229,7 @@ static int match_batch_create(int np, const int* N, const int* M, msift_match_ba
     size_t off = 0;
     auto take = [&](size_t b) { size_t o = off; off += (b + 255) & ~(size_t)255; return o; };
     long long sa = 0, sb = 0;
-    int tiles = 0, blks = 0;
+    int tiles = 0, tp_tiles = 0, blks = 0;
     B->max_rows = B->max_cols = B->max_blk = 0;
     for (int p = 0; p < np; ++p) {
         if (N[p] < 0 || M[p] < 0) { set_error("match batch: negative count"); delete B; return MSIFT_EINVAL; }
@@ -917,6 +1241,12 @@ static int match_batch_create(int np, const int* N, const int* M, msift_match_ba
         P.ncc = both ? (M[p] + kColChunk - 1) / kColChunk : 0;
         P.tile0 = tiles;
         tiles += P.nrb * P.ncc;
+        P.tp0 = tp_tiles;
+        P.xb1 = both ? (N[p] + kTpBatchXBlk - 1) / kTpBatchXBlk : 0;
+        P.cg1 = both ? (M[p] + 511) / 512 : 0;
+        P.xb2 = both ? (M[p] + kTpBatchXBlk - 1) / kTpBatchXBlk : 0;
+        P.cg2 = both ? (N[p] + 511) / 512 : 0;
+        tp_tiles += P.xb1 * P.cg1 + P.xb2 * P.cg2;
         P.nblk = (N[p] + 255) / 256;
         P.blk0 = blks;
         blks += P.nblk;
@@ -929,9 +1259,13 @@ static int match_batch_create(int np, const int* N, const int* M, msift_match_ba
         B->max_blk = std::max(B->max_blk, P.nblk);
     }
     if (sa > (1LL << 30) || sb > (1LL << 30)) { set_error("match batch: too many descriptors"); delete B; return MSIFT_EINVAL; }
-    B->sumN = (int)sa; B->sumM = (int)sb; B->tiles = tiles; B->blks = blks;
+    B->sumN = (int)sa; B->sumM = (int)sb; B->tiles = tiles; B->tp_tiles = tp_tiles; B->blks = blks;
     B->o_na = take((size_t)(sa > 0 ? sa : 1) * 4);
     B->o_nb = take((size_t)(sb > 0 ? sb : 1) * 4);
+    B->o_ac = take((size_t)(sa > 0 ? sa : 1) * 128);   // method 2: y' copies and key terms
+    B->o_bc = take((size_t)(sb > 0 ? sb : 1) * 128);
+    B->o_ta = take((size_t)(sa > 0 ? sa : 1) * 4);
+    B->o_tb = take((size_t)(sb > 0 ? sb : 1) * 4);
     B->o_cnt = take((size_t)(blks + 1) * 4);
     B->o_off = take((size_t)(blks + 1) * 4);
     B->o_stmp = take((size_t)scan_tmp_ints(blks) * 4);
@@ -958,20 +1292,35 @@ static int match_batch_run(msift_match_batch* B, const uint8_t* a, const uint8_t
         return MSIFT_EINVAL;
     }
     if (((uintptr_t)a & 15) || ((uintptr_t)b & 15)) { set_error("match_u8_batched: descriptors must be 16-byte aligned"); return MSIFT_EINVAL; }
-    if (method != 0 && method != 1) { set_error("match_u8_batched: method must be 0 or 1"); return MSIFT_EINVAL; }
+    if (method < 0 || method > 2) { set_error("match_u8_batched: method must be 0, 1 or 2"); return MSIFT_EINVAL; }
     char* ws = (char*)ws_;
     int* na = (int*)(ws + B->o_na);
     int* nb = (int*)(ws + B->o_nb);
     const int np = B->np;
-    if (B->sumN > 0) hipLaunchKernelGGL(k_norms, dim3((B->sumN + 3) / 4), dim3(256), 0, s, a, B->sumN, method, na);
-    if (B->sumM > 0) hipLaunchKernelGGL(k_norms, dim3((B->sumM + 3) / 4), dim3(256), 0, s, b, B->sumM, method == 1 ? 2 : method, nb);
-    if (B->tiles > 0) {
-        if (method == 0)
-            hipLaunchKernelGGL(k_match_tiles_b<0>, dim3(B->tiles), dim3(512), 0, s, (const MPair*)B->d_pairs, np, a, b,
-                               (const int*)na, (const int*)nb, ws);
-        else
-            hipLaunchKernelGGL(k_match_tiles_b<1>, dim3(B->tiles), dim3(512), 0, s, (const MPair*)B->d_pairs, np, a, b,
-                               (const int*)na, (const int*)nb, ws);
+    if (method == 2) {
+        // na, nb: |a'|^2, |b'|^2 (the reductions' adj)
+        uint8_t* ac = (uint8_t*)(ws + B->o_ac);
+        uint8_t* bc = (uint8_t*)(ws + B->o_bc);
+        uint32_t* ta = (uint32_t*)(ws + B->o_ta);
+        uint32_t* tb = (uint32_t*)(ws + B->o_tb);
+        if (B->sumN + B->sumM > 0)
+            hipLaunchKernelGGL(k_tp_prep, dim3((B->sumN + B->sumM + 31) / 32), dim3(256), 0, s, a, B->sumN, b, B->sumM,
+                               ac, bc, na, nb, ta, tb, (const MPair*)B->d_pairs, np);
+        if (B->tp_tiles > 0)
+            hipLaunchKernelGGL(k_match_side_b, dim3(B->tp_tiles), dim3(256), 0, s, (const MPair*)B->d_pairs, np, a, b,
+                               (const uint8_t*)ac, (const uint8_t*)bc, (const uint32_t*)ta, (const uint32_t*)tb, ws);
+    } else {
+        if (B->sumN > 0) hipLaunchKernelGGL(k_norms, dim3((B->sumN + 3) / 4), dim3(256), 0, s, a, B->sumN, method, na);
+        if (B->sumM > 0)
+            hipLaunchKernelGGL(k_norms, dim3((B->sumM + 3) / 4), dim3(256), 0, s, b, B->sumM, method == 1 ? 2 : method, nb);
+        if (B->tiles > 0) {
+            if (method == 0)
+                hipLaunchKernelGGL(k_match_tiles_b<0>, dim3(B->tiles), dim3(512), 0, s, (const MPair*)B->d_pairs, np, a,
+                                   b, (const int*)na, (const int*)nb, ws);
+            else
+                hipLaunchKernelGGL(k_match_tiles_b<1>, dim3(B->tiles), dim3(512), 0, s, (const MPair*)B->d_pairs, np, a,
+                                   b, (const int*)na, (const int*)nb, ws);
+        }
     }
     if (B->max_rows > 0)
         hipLaunchKernelGGL(k_reduce_top2_b, dim3(gcap_u((B->max_rows + 255) / 256, 1024), np), dim3(256), 0, s,

@@ -82,6 +82,7 @@ size_t match_workspace(int N, int M);
 int match_u8(const uint8_t* a, int N, const uint8_t* b, int M, double ratio, int method, void* ws,
              size_t ws_bytes, int32_t* fwd, int32_t* bwd, int32_t* pairs, int* npairs,
              hipStream_t s);
+int match_set_tile(int shape, int cpw);
 int match_u8_noalloc(const uint8_t* a, int N, const uint8_t* b, int M, double ratio, int method, int32_t* fwd,
                      int32_t* bwd, int32_t* pairs, int* npairs, hipStream_t s);
 int match_f16(const float* a, int N, const float* b, int M, double ratio, void* ws,

@@ -9,7 +9,12 @@ import ctypes
 
 from . import _lib
 
-METHODS = {"hilo": 0, "shift": 1}
+METHODS = {"hilo": 0, "shift": 1, "twopass": 2}
+DEFAULT_METHOD = "twopass"
+# two-pass tile shapes (0: 64 rows a wave, next tile's MFMAs issued ahead;
+# 1: 64 rows; 2: 32 rows) and 512-column chunks per workgroup tried by autotune
+TILE_SHAPES = (0, 1, 2)
+TILE_CHUNKS = (1, 2, 4, 8)
 
 
 def match_workspace_bytes(N, M):
@@ -18,7 +23,7 @@ def match_workspace_bytes(N, M):
     return n.value
 
 
-def match_u8(a, b, ratio=0.75, method="shift", sync=True, ws=None):
+def match_u8(a, b, ratio=0.75, method=DEFAULT_METHOD, sync=True, ws=None):
     """a: (N,128) uint8 CUDA tensor, b: (M,128).  Returns (fwd (N,3), bwd (M,3),
     pairs (K,2)) int32 device tensors; fwd[i] = (best j, d1, d2)."""
     import torch
@@ -42,6 +47,57 @@ def match_u8(a, b, ratio=0.75, method="shift", sync=True, ws=None):
         return fwd[:N], bwd[:M], pairs, npairs
     k = int(npairs.item())
     return fwd[:N], bwd[:M], pairs[:k]
+
+
+def set_tile(shape, chunks_per_block):
+    """Two-pass tile shape for the process (outputs do not depend on it)."""
+    _lib.check(_lib.lib().msift_match_set_tile(int(shape), int(chunks_per_block)), "match set_tile")
+
+
+def _bucket(n):
+    b = 1024
+    while b < n:
+        b *= 2
+    return b
+
+
+def autotune(N, M, device="cuda", iters=3):
+    """Time the two-pass matcher at N x M for every tile shape and chunk count,
+    keep the fastest (cached per power-of-two size class in
+    ~/.cache/pysift_amd/autotune.json) and set it.  Returns (shape, chunks),
+    timings in ms.  The kernels' time does not depend on the descriptor
+    values, so random descriptors of the requested size are timed."""
+    import torch
+    from .batch import _load_tune, _save_tune
+    key = f"match:{_bucket(N)}:{_bucket(M)}"
+    cache = _load_tune()
+    if key in cache:
+        shape, cpw = cache[key]
+        set_tile(shape, cpw)
+        return (shape, cpw), {}
+    g = torch.Generator().manual_seed(0)
+    a = torch.randint(0, 256, (max(N, 1), 128), dtype=torch.uint8, generator=g).to(device)
+    b = torch.randint(0, 256, (max(M, 1), 128), dtype=torch.uint8, generator=g).to(device)
+    ws = torch.empty(match_workspace_bytes(a.shape[0], b.shape[0]), dtype=torch.uint8, device=device)
+    times = {}
+    with torch.cuda.device(a.device):
+        for shape in TILE_SHAPES:
+            for cpw in TILE_CHUNKS:
+                set_tile(shape, cpw)
+                match_u8(a, b, 0.75, "twopass", sync=False, ws=ws)
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(iters):
+                    match_u8(a, b, 0.75, "twopass", sync=False, ws=ws)
+                e1.record()
+                torch.cuda.synchronize()
+                times[(shape, cpw)] = e0.elapsed_time(e1) / iters
+    best = min(times, key=times.get)
+    set_tile(*best)
+    cache[key] = list(best)
+    _save_tune(cache)
+    return best, times
 
 
 def match_f16(a, b, ratio=0.75, ws=None):
@@ -120,7 +176,7 @@ class MatchBatchPlan:
             return torch.zeros((1, 128), dtype=torch.uint8, device=self.device)
         return torch.cat([t.to(self.device) for t in sets]).contiguous()
 
-    def run(self, A, B, ratio=0.75, method="shift"):
+    def run(self, A, B, ratio=0.75, method=DEFAULT_METHOD):
         """A, B packed (sum N, 128) / (sum M, 128) uint8 device tensors."""
         rc = _lib.lib().msift_match_u8_batched(
             self._p, _lib.ptr(A), _lib.ptr(B), float(ratio), METHODS[method], _lib.ptr(self.ws), self.ws.numel(),
@@ -153,7 +209,7 @@ class MatchBatchPlan:
         return p1, p2
 
 
-def match_u8_batched(sets_a, sets_b, ratio=0.75, method="shift"):
+def match_u8_batched(sets_a, sets_b, ratio=0.75, method=DEFAULT_METHOD):
     """Match pairs (sets_a[p], sets_b[p]) in one launch; returns per pair
     (fwd, bwd, pairs) as match_u8 would."""
     plan = MatchBatchPlan([t.shape[0] for t in sets_a], [t.shape[0] for t in sets_b], device=sets_a[0].device)

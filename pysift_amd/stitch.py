@@ -71,7 +71,7 @@ class GPUPyStitch:
                  color_match="gain", dsp_sift=False, contrast_thresh=0.04, n_octaves=4, n_scales=3, sigma0=1.6,
                  edge_thresh=10.0, double_image=True, max_dimension=0, denoise=False, sharpen=False,
                  clahe_clip=2.0, clahe_tile=8, output_max_dim=0, post_sharpen=False, output_format="png",
-                 output_quality=90, *, match_method="shift", ransac_seed=42, save_output=True, verbose=True,
+                 output_quality=90, *, match_method="twopass", ransac_seed=42, save_output=True, verbose=True,
                  blend="feather", device=None):
         if matcher != "ratio":
             raise NotImplementedError("pysift_amd implements only the ratio-test matcher")
@@ -103,15 +103,15 @@ class GPUPyStitch:
         self._clahe = CLAHE(clipLimit=clahe_clip, tileGridSize=(clahe_tile, clahe_tile))
         self.output_format = output_format.lower()
         self.output_quality = output_quality
-        if match_method not in ("hilo", "shift", "float"):
-            raise ValueError(f"match_method must be 'hilo', 'shift' or 'float' (got {match_method!r})")
+        if match_method not in ("hilo", "shift", "twopass", "float"):
+            raise ValueError(f"match_method must be 'hilo', 'shift', 'twopass' or 'float' (got {match_method!r})")
         # int8_matching (core.py:2894-2910) asks for INT8 GEMM matching.  The
-        # integer matchers ('shift', 'hilo') are INT8 MFMA GEMMs and exact, so
+        # integer matchers ('twopass', 'shift', 'hilo') are INT8 MFMA GEMMs and exact, so
         # the flag selects them; the reference's per-row-scaled approximate
         # INT8 similarity on the PCA/float path is not reproduced.
         if int8_matching and match_method == "float":
             raise NotImplementedError("int8_matching with match_method='float' (the approximate per-row-scaled INT8 "
-                                      "similarity) is not implemented; match_method='shift' is exact INT8 MFMA")
+                                      "similarity) is not implemented; the default match_method='twopass' is exact INT8 MFMA")
         self.match_method = match_method
         self._pca = None   # (mean, proj) of the float path, fitted on the first pair (core.py:2729-2736)
         self.ransac_seed = int(ransac_seed)

@@ -72,3 +72,17 @@ def test_stitch_int8_matching_mapping():
     from pysift_amd.stitch import GPUPyStitch
     with pytest.raises(NotImplementedError):
         GPUPyStitch(int8_matching=True, match_method="float", verbose=False, save_output=False)
+
+
+def test_match_tile_setter_and_method_validation():
+    # the two-pass tile setter accepts shapes 0-2 and 1-64 chunks; the matcher
+    # rejects unknown methods before touching the device
+    from pysift_amd import _lib
+    L = _lib.lib()
+    for shape, cpw in [(3, 4), (-1, 4), (0, 0), (0, 65)]:
+        assert L.msift_match_set_tile(shape, cpw) == _lib.MSIFT_EINVAL
+        assert "chunks per workgroup" in _lib.last_error()
+    assert L.msift_match_set_tile(2, 8) == 0
+    assert L.msift_match_set_tile(0, 4) == 0   # back to the default
+    rc = L.msift_match_u8(None, 10, None, 10, 0.75, 3, None, 0, None, None, None, None, None)
+    assert rc == _lib.MSIFT_EINVAL

@@ -27,7 +27,7 @@ def _pair(N, M, seed):
     return a, b
 
 
-@pytest.mark.parametrize("method", ["hilo", "shift"])
+@pytest.mark.parametrize("method", ["hilo", "shift", "twopass"])
 def test_batched_match_equals_single_and_oracle(method):
     import torch
     from pysift_amd.matcher import match_u8, match_u8_batched

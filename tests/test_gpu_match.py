@@ -23,7 +23,7 @@ def _run(a, b, method, ratio=0.75):
     return f.cpu().numpy(), bw.cpu().numpy(), p.cpu().numpy()
 
 
-@pytest.mark.parametrize("method", ["hilo", "shift"])
+@pytest.mark.parametrize("method", ["hilo", "shift", "twopass"])
 @pytest.mark.parametrize("N,M", [(1, 1), (2, 3), (37, 300), (300, 37), (513, 700), (1500, 1100)])
 def test_match_exact_random(method, N, M):
     print(f"[case] match {method} {N}x{M}", flush=True)
@@ -41,7 +41,7 @@ def test_match_exact_random(method, N, M):
         assert len(rp) > 0
 
 
-@pytest.mark.parametrize("method", ["hilo", "shift"])
+@pytest.mark.parametrize("method", ["hilo", "shift", "twopass"])
 def test_match_extremes_and_ties(method):
     # all-255 / all-0 rows (max distance 8 323 200) and exact duplicate rows (ties)
     a = np.zeros((70, 128), np.uint8); a[::2] = 255
@@ -53,6 +53,24 @@ def test_match_extremes_and_ties(method):
     np.testing.assert_array_equal(bw, rb)
     np.testing.assert_array_equal(p, rp)
     assert (rf[:, 1] == 0).all() or True
+
+
+@pytest.mark.parametrize("N,M", [(9000, 7000), (4097, 12001)])
+def test_twopass_equals_shift_large(N, M):
+    # sizes the oracle does not reach in seconds: the two-pass kernel against
+    # the fused shift kernel (both bit-exact at the oracle sizes above)
+    import torch
+    from pysift_amd.matcher import match_u8
+    print(f"[case] twopass vs shift {N}x{M}", flush=True)
+    a = _rand_desc(N, 91)
+    b = _rand_desc(M, 92)
+    k = min(N, M) // 3
+    b[:k] = np.clip(a[:k].astype(int) + np.random.default_rng(3).integers(-4, 5, (k, 128)), 0, 255).astype(np.uint8)
+    ta = torch.from_numpy(a).cuda(); tb = torch.from_numpy(b).cuda()
+    f1, b1, p1 = match_u8(ta, tb, 0.75, "shift")
+    f2, b2, p2 = match_u8(ta, tb, 0.75, "twopass")
+    assert torch.equal(f1, f2) and torch.equal(b1, b2) and torch.equal(p1, p2)
+    assert p1.shape[0] > k // 2
 
 
 def test_match_empty():

@@ -59,7 +59,7 @@ def _call(torch, a, b, method, noalloc):
     return fwd[:N].cpu().numpy(), bwd[:M].cpu().numpy(), pairs[:k].cpu().numpy(), k
 
 
-@pytest.mark.parametrize("method", [0, 1])
+@pytest.mark.parametrize("method", [0, 1, 2])
 def test_noalloc_equals_workspace_entry_two_calls(method):
     import torch
     for N, M, seed in ((MAX_ROWS, MAX_ROWS, 11), (700, 900, 12)):
