@@ -373,8 +373,13 @@ __global__ __launch_bounds__(512) void k_match_tiles(const uint8_t* __restrict__
 constexpr int kTpStage = 256;   // y per LDS stage, two stages per 512-y chunk
 constexpr int kTpWaves = 4;     // waves per workgroup
 
+// block swizzle of the staged rows: distinct over any 8 aligned consecutive rows
+// and over the rows of one parity among 16, so the 16-byte A-fragment reads
+// of 8 (32 banks) or 16 (64 banks) consecutive lanes hit disjoint banks
+__device__ __forceinline__ int tp_swz(int c) { return (c & 7) ^ ((c >> 3) & 1); }
+
 struct TpSmem {
-    alignas(16) uint8_t y[kTpStage * 128];   // y' rows: 16-byte block q of row c at block position q ^ (c % 8)
+    alignas(16) uint8_t y[kTpStage * 128];   // y' rows: 16-byte block q of row c at block position q ^ tp_swz(c)
     alignas(16) uint32_t t[kTpStage];        // key terms ((ty + 2^21) << 9) + lj
 };
 
@@ -398,10 +403,10 @@ __device__ __forceinline__ void match_side_tile(const uint8_t* __restrict__ X, i
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks) xf[n][ks] = src[2 * ks + hi] ^ (v4i)(int)0x7f7f7f7f;
     }
-    // LDS offsets of the lane's A fragment blocks (row l32 of a 32-y tile; 32 t % 8 == 0)
+    // LDS offsets of the lane's A fragment blocks (row l32 of a 32-y tile; tp_swz(32 t + l32) = tp_swz(l32))
     int yoff[4];
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) yoff[ks] = l32 * 128 + 16 * ((2 * ks + hi) ^ (l32 & 7));
+    for (int ks = 0; ks < 4; ++ks) yoff[ks] = l32 * 128 + 16 * ((2 * ks + hi) ^ tp_swz(l32));
     auto fetch = [&](int t, v4i* yf, v4i* tv) {
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks) yf[ks] = *reinterpret_cast<const v4i*>(&sm.y[t * 4096 + yoff[ks]]);
@@ -435,11 +440,11 @@ __device__ __forceinline__ void match_side_tile(const uint8_t* __restrict__ X, i
             __syncthreads();   // the previous stage's reads are done
             // Instruction i of wave w fills stage rows 8 (8 w + i) .. +8 (1 KB): lane L's 16
             // bytes land at byte 16 L, i.e. row 8 (8 w + i) + L / 8, block position L % 8,
-            // which holds the row's block (L % 8) ^ (row % 8).  Rows past NY repeat row NY - 1
+            // which holds the row's block (L % 8) ^ tp_swz(row).  Rows past NY repeat row NY - 1
             // (their keys are never inserted).
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
-                const int c = (8 * w + i) * 8 + (lane >> 3), q = (lane & 7) ^ (c & 7);
+                const int c = (8 * w + i) * 8 + (lane >> 3), q = (lane & 7) ^ tp_swz(c);
                 const int y = min(ybase + c, NY - 1);
                 __builtin_amdgcn_global_load_lds((__attribute__((address_space(1))) void*)(Yc + (size_t)y * 128 + 16 * q),
                                                  (__attribute__((address_space(3))) void*)&sm.y[(8 * w + i) * 1024], 16,
