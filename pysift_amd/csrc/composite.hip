@@ -354,125 +354,186 @@ __global__ __launch_bounds__(kCT) void k_cmp_sobel2(const float* __restrict__ dx
     cost[st->vertical ? o : (size_t)u * ch + v] = c;
 }
 
-// one block: the seam's dynamic programme and backtrack.  Lines are rows
-// (vertical seam) or columns; a cell belongs to the overlap exactly when its
-// cost is below 1e9.  The two dp lines live in LDS up to kDpLds cells (in
-// global memory beyond).  The backtrack runs in windows of kWin lines: the
-// seam moves at most one cell per line, so the block first loads those
-// lines' back-pointers within kWin cells of the current position into LDS,
-// then one thread walks them.
-constexpr int kDpLds = 9088;
-constexpr int kDpJ = (kDpLds + 1023) / 1024;
-constexpr int kMaxLines = 65536;  // LDS bitmask of the lines holding overlap
+// The seam's dynamic programme over the lines holding overlap (rows for a
+// vertical seam, columns otherwise; a cell belongs to the overlap exactly when
+// its cost is below 1e9), in bands of kDpK lines.  Line i's value at n needs
+// line i - 1 at n - 1, n, n + 1 only, so the workgroup that owns cells
+// [s0, s1) of a band can compute them from the band's input line over
+// [s0 - kDpK, s1 + kDpK), shrinking the valid range by one cell a side per
+// line (not at the line's ends, where the neighbours are +inf).  Every cell is
+// thus computed by the same operations as a line-by-line pass, partly twice;
+// each workgroup writes the back-pointers and the band's output values of the
+// cells it owns.  One launch per band; the backtrack runs in a last single
+// workgroup.
+constexpr int kDpS = 256;               // cells a workgroup owns
+constexpr int kDpK = 128;               // lines per band
+constexpr int kDpR = kDpS + 2 * kDpK;   // cells a workgroup computes
+constexpr int kDpT = 256;               // threads
+constexpr int kDpC = kDpR / kDpT;       // cells per thread
+constexpr int kDpPf = 8;                // lines of costs in flight
 constexpr int kWin = 64;
 
-__global__ __launch_bounds__(1024) void k_cmp_dp(const float* __restrict__ cost, const uint32_t* __restrict__ rowflags,
-                                                 const uint32_t* __restrict__ colflags, int cw, int ch,
-                                                 const CmpStats* __restrict__ st, double* __restrict__ gdpa,
-                                                 double* __restrict__ gdpb, uint8_t* __restrict__ bt,
-                                                 int32_t* __restrict__ lines, int32_t* __restrict__ seam) {
-    __shared__ double ldp[2][kDpLds];
+struct DpGeom {
+    int L, N;
+    const uint32_t* lf;
+};
+
+__device__ __forceinline__ DpGeom dp_geom(const CmpStats* st, const uint32_t* rowflags, const uint32_t* colflags,
+                                          int cw, int ch) {
+    const bool vert = st->vertical;
+    return DpGeom{vert ? ch : cw, vert ? cw : ch, vert ? rowflags : colflags};
+}
+
+// overlap lines in order into lines[0 .. nl), nl into *nlp, and the first
+// line's values (its costs, +inf off the overlap) into dp0.  One block.
+__global__ __launch_bounds__(1024) void k_cmp_dp_lines(const float* __restrict__ cost,
+                                                       const uint32_t* __restrict__ rowflags,
+                                                       const uint32_t* __restrict__ colflags, int cw, int ch,
+                                                       const CmpStats* __restrict__ st, int32_t* __restrict__ lines,
+                                                       int* __restrict__ nlp, double* __restrict__ dp0) {
+    __shared__ int wsum[16];
+    __shared__ int s_nl, s_first;
+    if (!st->has_overlap) return;
+    const DpGeom g = dp_geom(st, rowflags, colflags, cw, ch);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    // thread tid takes lines [tid * per, (tid + 1) * per)
+    const int per = (g.L + 1023) / 1024;
+    const int l0 = min(g.L, tid * per), l1 = min(g.L, l0 + per);
+    int cnt = 0;
+    for (int l = l0; l < l1; ++l) cnt += (g.lf[l] & 1u) ? 1 : 0;
+    int incl = cnt;   // inclusive scan over the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    if (tid == 0) {
+        int acc = 0;
+        for (int k = 0; k < 16; ++k) { const int v = wsum[k]; wsum[k] = acc; acc += v; }
+        s_nl = acc;
+        *nlp = acc;
+    }
+    __syncthreads();
+    int pos = wsum[w] + incl - cnt;
+    for (int l = l0; l < l1; ++l)
+        if (g.lf[l] & 1u) {
+            if (pos == 0) s_first = l;
+            lines[pos++] = l;
+        }
+    __syncthreads();
+    if (s_nl == 0) return;
+    const int first = s_first;
+    for (int n = tid; n < g.N; n += 1024) {
+        const float c = cost[(size_t)first * g.N + n];
+        dp0[n] = c < 1e9f ? (double)c : 1e18;
+    }
+}
+
+// band b: lines i0 = 1 + b kDpK .. min(nl, i0 + kDpK) - 1 from dp_in (line
+// i0 - 1) to dp_out; blockIdx.x owns cells [kDpS x, +kDpS)
+__global__ __launch_bounds__(kDpT) void k_cmp_dp_band(const float* __restrict__ cost,
+                                                      const uint32_t* __restrict__ rowflags,
+                                                      const uint32_t* __restrict__ colflags, int cw, int ch,
+                                                      const CmpStats* __restrict__ st,
+                                                      const int32_t* __restrict__ lines, const int* __restrict__ nlp,
+                                                      int band, const double* __restrict__ dp_in,
+                                                      double* __restrict__ dp_out, uint8_t* __restrict__ bt) {
+    __shared__ double d[2][kDpR];
+    if (!st->has_overlap) return;
+    const DpGeom g = dp_geom(st, rowflags, colflags, cw, ch);
+    const int N = g.N, nl = *nlp;
+    const int i0 = 1 + band * kDpK;
+    if (i0 >= nl) return;
+    const int i1 = min(nl, i0 + kDpK);
+    const int s0 = blockIdx.x * kDpS, s1 = min(N, s0 + kDpS);
+    if (s0 >= N) return;
+    const int a = max(0, s0 - kDpK), e = min(N, s1 + kDpK);   // computed cells [a, e)
+    const int tid = threadIdx.x;
+    const double INF = 1e18;
+    for (int x = tid; x < e - a; x += kDpT) d[0][x] = dp_in[a + x];
+    // costs of the next kDpPf lines: slot k holds line i for i = k (mod kDpPf);
+    // thread tid's cells are a + tid + kDpT c
+    float cb[kDpPf][kDpC];
+    auto load_line = [&](int i, float* dst) {
+        if (i >= i1) return;   // uniform
+        const float* row = cost + (size_t)lines[i] * N;
+#pragma unroll
+        for (int c = 0; c < kDpC; ++c) {
+            const int n = a + tid + kDpT * c;
+            dst[c] = n < e ? row[n] : 1e9f;
+        }
+    };
+#pragma unroll
+    for (int k = 0; k < kDpPf; ++k) load_line(i0 + k, cb[k]);
+    __syncthreads();
+    int cur = 0;
+    for (int ib = i0; ib < i1; ib += kDpPf) {
+#pragma unroll
+        for (int k = 0; k < kDpPf; ++k) {
+            const int i = ib + k;
+            if (i >= i1) break;   // uniform
+            const int t = i - i0 + 1;
+            const int lo = a == 0 ? 0 : a + t, hi = e == N ? N : e - t;
+            const int l = lines[i];
+#pragma unroll
+            for (int c = 0; c < kDpC; ++c) {
+                const int x = tid + kDpT * c, n = a + x;
+                if (n < lo || n >= hi) continue;
+                const float cst = cb[k][c];
+                const bool ov = cst < 1e9f;
+                const double pl = n > 0 ? d[cur][x - 1] : INF, pm = d[cur][x];
+                const double pr = n < N - 1 ? d[cur][x + 1] : INF;
+                int arg = 0;
+                double best = pl;
+                if (pm < best) { best = pm; arg = 1; }
+                if (pr < best) { best = pr; arg = 2; }
+                d[cur ^ 1][x] = ov ? (double)cst + best : INF;
+                if (n >= s0 && n < s1) bt[(size_t)l * N + n] = (uint8_t)arg;
+            }
+            load_line(i + kDpPf, cb[k]);
+            cur ^= 1;
+            __syncthreads();
+        }
+    }
+    for (int n = s0 + tid; n < s1; n += kDpT) dp_out[n] = d[cur][n - a];
+}
+
+// the first minimum over the last line's overlap cells, then the backtrack.
+// The backtrack runs in windows of kWin lines: the seam moves at most one cell
+// per line, so the block first loads those lines' back-pointers within kWin
+// cells of the current position into LDS, then one thread walks them.
+__global__ __launch_bounds__(1024) void k_cmp_dp_back(const float* __restrict__ cost,
+                                                      const uint32_t* __restrict__ rowflags,
+                                                      const uint32_t* __restrict__ colflags, int cw, int ch,
+                                                      const CmpStats* __restrict__ st,
+                                                      const int32_t* __restrict__ lines, const int* __restrict__ nlp,
+                                                      const double* __restrict__ dpa, const double* __restrict__ dpb,
+                                                      const uint8_t* __restrict__ bt, int32_t* __restrict__ seam) {
     __shared__ uint8_t win[kWin][2 * kWin + 1];
-    __shared__ uint32_t ovl[kMaxLines / 32];
     __shared__ double rv[16];
     __shared__ int ri[16];
     __shared__ int s_pos;
     if (!st->has_overlap) return;
-    const bool vert = st->vertical;
-    const int L = vert ? ch : cw, N = vert ? cw : ch;
-    const uint32_t* lf = vert ? rowflags : colflags;
-    const double INF = 1e18;
+    const DpGeom g = dp_geom(st, rowflags, colflags, cw, ch);
+    const int L = g.L, N = g.N, nl = *nlp;
     const int tid = threadIdx.x;
-    const bool in_lds = N <= kDpLds;
-    for (int w = tid; w < (L + 31) / 32; w += 1024) {
-        uint32_t bits = 0;
-        for (int b = 0; b < 32; ++b) {
-            const int l = 32 * w + b;
-            if (l < L && (lf[l] & 1u)) bits |= 1u << b;
-        }
-        ovl[w] = bits;
+    if (nl == 0) {
+        for (int l = tid; l < L; l += 1024) seam[l] = 0;
+        return;
     }
-    __syncthreads();
-    auto has_ov = [&](int l) { return (ovl[l >> 5] >> (l & 31)) & 1u; };
-    // forward pass over the lines holding overlap.  LDS path: the dp lines
-    // are indexed in the __shared__ array directly (LDS instructions, which
-    // do not wait on the global loads in flight) and the next line's costs
-    // are loaded while the current one is computed.
-    int nl = 0;
-    int l = 0;
-    while (l < L && !has_ov(l)) ++l;
-    int pb = 0;                 // LDS path: ldp[pb] holds the previous line
-    double* dp = gdpa;          // global path
-    double* nd = gdpb;
-    float cur[kDpJ], nxt[kDpJ];
-#pragma unroll
-    for (int j = 0; j < kDpJ; ++j) {
-        const int n = tid + 1024 * j;
-        cur[j] = (in_lds && n < N) ? cost[(size_t)l * N + n] : 1e9f;
-    }
-    while (l < L) {
-        int ln = l + 1;
-        while (ln < L && !has_ov(ln)) ++ln;
-        if (tid == 0) lines[nl] = l;
-        if (in_lds) {
-#pragma unroll
-            for (int j = 0; j < kDpJ; ++j) {
-                const int n = tid + 1024 * j;
-                nxt[j] = (ln < L && n < N) ? cost[(size_t)ln * N + n] : 1e9f;
-            }
-#pragma unroll
-            for (int j = 0; j < kDpJ; ++j) {
-                const int n = tid + 1024 * j;
-                if (n >= N) continue;
-                const float c = cur[j];
-                const bool ov = c < 1e9f;
-                if (nl == 0) {
-                    ldp[pb ^ 1][n] = ov ? (double)c : INF;
-                } else {
-                    const double pl = n > 0 ? ldp[pb][n - 1] : INF, pm = ldp[pb][n];
-                    const double pr = n < N - 1 ? ldp[pb][n + 1] : INF;
-                    int a = 0;
-                    double best = pl;
-                    if (pm < best) { best = pm; a = 1; }
-                    if (pr < best) { best = pr; a = 2; }
-                    ldp[pb ^ 1][n] = ov ? (double)c + best : INF;
-                    bt[(size_t)l * N + n] = (uint8_t)a;
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < kDpJ; ++j) cur[j] = nxt[j];
-            pb ^= 1;
-        } else {
-            for (int n = tid; n < N; n += 1024) {
-                const float c = cost[(size_t)l * N + n];
-                const bool ov = c < 1e9f;
-                if (nl == 0) {
-                    nd[n] = ov ? (double)c : INF;
-                } else {
-                    const double pl = n > 0 ? dp[n - 1] : INF, pm = dp[n], pr = n < N - 1 ? dp[n + 1] : INF;
-                    int a = 0;
-                    double best = pl;
-                    if (pm < best) { best = pm; a = 1; }
-                    if (pr < best) { best = pr; a = 2; }
-                    nd[n] = ov ? (double)c + best : INF;
-                    bt[(size_t)l * N + n] = (uint8_t)a;
-                }
-            }
-            double* t = dp; dp = nd; nd = t;
-        }
-        __syncthreads();
-        nl++;
-        l = ln;
-    }
-    if (in_lds) dp = ldp[pb];
+    const double INF = 1e18;
+    // bands 0 .. nb - 1 alternate dpa -> dpb -> dpa ...; the last line is in dp
+    const int nb = (nl - 1 + kDpK - 1) / kDpK;
+    const double* dp = (nb & 1) ? dpb : dpa;
     const int last = lines[nl - 1];
-    // first minimum over the last line's overlap cells
     double bv = INF;
     int bi = 0x7fffffff;
     for (int n = tid; n < N; n += 1024) {
         const float c = cost[(size_t)last * N + n];
-        const double d = c < 1e9f ? dp[n] : INF;
-        if (d < bv || (d == bv && n < bi)) { bv = d; bi = n; }
+        const double v = c < 1e9f ? dp[n] : INF;
+        if (v < bv || (v == bv && n < bi)) { bv = v; bi = n; }
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
@@ -495,11 +556,11 @@ __global__ __launch_bounds__(1024) void k_cmp_dp(const float* __restrict__ cost,
     for (int i = nl - 1; i > 0; i -= kWin) {
         const int cnt = min(kWin, i);           // entries i, i - 1, ..., i - cnt + 1
         const int pos0 = s_pos;
-        for (int e = tid; e < cnt * (2 * kWin + 1); e += 1024) {
-            const int j = e / (2 * kWin + 1), d = e - j * (2 * kWin + 1);
-            int p = pos0 + d - kWin;
+        for (int q = tid; q < cnt * (2 * kWin + 1); q += 1024) {
+            const int j = q / (2 * kWin + 1), dd = q - j * (2 * kWin + 1);
+            int p = pos0 + dd - kWin;
             p = p < 0 ? 0 : (p > N - 1 ? N - 1 : p);
-            win[j][d] = bt[(size_t)lines[i - j] * N + p];
+            win[j][dd] = bt[(size_t)lines[i - j] * N + p];
         }
         __syncthreads();
         if (tid == 0) {
@@ -655,7 +716,8 @@ __global__ void k_cmp_zero(unsigned long long* __restrict__ p, int n) {
 struct CmpPlan {
     int L, hs[kMaxLevels], ws[kMaxLevels];
     size_t P, o_wu, o_c2, o_m1, o_m2, o_rowf, o_colf, o_st, o_costc, o_avg, o_dx, o_dy, o_cost, o_seam0, o_tmp,
-        o_lvl[kMaxLevels], o_res[kMaxLevels], o_dpa, o_dpb, o_bt, o_seam, o_lines, o_rowacc, o_hist, o_lut, zero_bytes, total;
+        o_lvl[kMaxLevels], o_res[kMaxLevels], o_dpa, o_dpb, o_bt, o_seam, o_lines, o_nl, o_rowacc, o_hist, o_lut,
+        zero_bytes, total;
 };
 
 CmpPlan cmp_plan(int cw, int ch, int n_levels) {
@@ -685,6 +747,7 @@ CmpPlan cmp_plan(int cw, int ch, int n_levels) {
     Pl.o_bt = take(P);
     Pl.o_seam = take((size_t)N * 4);
     Pl.o_lines = take((size_t)N * 4);
+    Pl.o_nl = take(4);
     Pl.total = off;
     return Pl;
 }
@@ -785,9 +848,27 @@ int composite(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h2, 
     hipLaunchKernelGGL(k_cmp_sobel2, g2d, dim3(kCT), 0, s, (const float*)dx, (const float*)dy, (const float*)costc,
                        (const uint8_t*)m1, (const uint8_t*)m2, cw, ch, (const CmpStats*)st, cost);
     int32_t* seam = (int32_t*)(ws + Pl.o_seam);
-    hipLaunchKernelGGL(k_cmp_dp, dim3(1), dim3(1024), 0, s, (const float*)cost, (const uint32_t*)rowf,
-                       (const uint32_t*)colf, cw, ch, (const CmpStats*)st, (double*)(ws + Pl.o_dpa),
-                       (double*)(ws + Pl.o_dpb), (uint8_t*)(ws + Pl.o_bt), (int32_t*)(ws + Pl.o_lines), seam);
+    {
+        // the seam: overlap lines, one launch per band of kDpK lines (launches past
+        // the last band return at once), backtrack
+        double* dpa = (double*)(ws + Pl.o_dpa);
+        double* dpb = (double*)(ws + Pl.o_dpb);
+        int32_t* lines = (int32_t*)(ws + Pl.o_lines);
+        int* nlp = (int*)(ws + Pl.o_nl);
+        uint8_t* bt = (uint8_t*)(ws + Pl.o_bt);
+        const int Lmax = std::max(cw, ch), Nmax = std::max(cw, ch);
+        hipLaunchKernelGGL(k_cmp_dp_lines, dim3(1), dim3(1024), 0, s, (const float*)cost, (const uint32_t*)rowf,
+                           (const uint32_t*)colf, cw, ch, (const CmpStats*)st, lines, nlp, dpa);
+        const int nbands = (Lmax - 1 + kDpK - 1) / kDpK;
+        for (int b = 0; b < nbands; ++b)
+            hipLaunchKernelGGL(k_cmp_dp_band, dim3((Nmax + kDpS - 1) / kDpS), dim3(kDpT), 0, s, (const float*)cost,
+                               (const uint32_t*)rowf, (const uint32_t*)colf, cw, ch, (const CmpStats*)st,
+                               (const int32_t*)lines, (const int*)nlp, b, (const double*)((b & 1) ? dpb : dpa),
+                               (b & 1) ? dpa : dpb, bt);
+        hipLaunchKernelGGL(k_cmp_dp_back, dim3(1), dim3(1024), 0, s, (const float*)cost, (const uint32_t*)rowf,
+                           (const uint32_t*)colf, cw, ch, (const CmpStats*)st, (const int32_t*)lines,
+                           (const int*)nlp, (const double*)dpa, (const double*)dpb, (const uint8_t*)bt, seam);
+    }
     hipLaunchKernelGGL(k_cmp_seam_mask, g2d, dim3(kCT), 0, s, (const uint8_t*)m1, (const uint8_t*)m2,
                        (const int32_t*)seam, cw, (const CmpStats*)st, seam0);
     hipLaunchKernelGGL(k_cmp_blur_rows, g2d, dim3(kCT), 0, s, (const float*)seam0, cw, w25, (const CmpStats*)st, tmp);
