@@ -10,7 +10,8 @@
 //     covariance = Gram / n;
 //   - eigenvectors by the cyclic parallel Jacobi method in one workgroup (the
 //     matrix in LDS, row stride 129 doubles so column walks hit distinct
-//     banks; the eigenvector matrix, transposed, in global memory): each
+//     banks; the rotations recorded and applied to the transposed eigenvector
+//     matrix afterwards, one wave per column): each
 //     sweep is 127 rounds of the circle-method pairing of the 128 indices
 //     (round r: (r, 127) and ((r + k) % 127, (r - k) % 127), k = 1..63); the
 //     64 rotations of a round are computed from the same matrix, then applied
@@ -143,109 +144,180 @@ __device__ __forceinline__ void jacobi_cs(double app, double aqq, double apq, do
     s = t * c;
 }
 
-// one workgroup: Jacobi sweeps on cov, then eigenvalue order, proj and evals
-__global__ __launch_bounds__(1024) void k_pca_jacobi(const double* __restrict__ cov, double* __restrict__ Vt, int dims,
-                                                     float* __restrict__ proj, double* __restrict__ evals,
-                                                     int* __restrict__ sweeps_out) {
+// one workgroup: Jacobi sweeps on cov.  The eigenvector updates are not done
+// here: every round that rotates records its 64 pairs (p | q << 8 | rot << 16)
+// and (c, s) in rpq / rcs, k_pca_vt applies them, and the diagonal (the
+// eigenvalues) goes to diag.
+__global__ __launch_bounds__(1024) void k_pca_jacobi(const double* __restrict__ cov, int* __restrict__ rpq,
+                                                     double* __restrict__ rcs, int* __restrict__ nrounds,
+                                                     double* __restrict__ diag, int* __restrict__ sweeps_out) {
     __shared__ double A[128 * kAStride];
     __shared__ double Cs[64], Ss[64];
     __shared__ int Ps[64], Qs[64], Rs[64];
-    __shared__ int order[128];
     const int tid = threadIdx.x;
-    for (int e = tid; e < 16384; e += 1024) {
-        A[(e >> 7) * kAStride + (e & 127)] = cov[e];
-        Vt[e] = (e >> 7) == (e & 127) ? 1.0 : 0.0;
-    }
+    for (int e = tid; e < 16384; e += 1024) A[(e >> 7) * kAStride + (e & 127)] = cov[e];
     __syncthreads();
-    int sweeps = 0;
+    int sweeps = 0, g = 0;
     for (; sweeps < kPcaMaxSweeps; ++sweeps) {
         int rotated = 0;
         for (int r = 0; r < 127; ++r) {
-            int rot = 0;
+            int rot = 0, p = 0, q = 0;
+            double c = 1.0, sn = 0.0;
             if (tid < 64) {
-                int p, q;
                 if (tid == 0) { p = r; q = 127; }
                 else { p = (r + tid) % 127; q = (r - tid + 127) % 127; }
                 if (p > q) { const int t = p; p = q; q = t; }
                 const double app = A[p * kAStride + p], aqq = A[q * kAStride + q], apq = A[p * kAStride + q];
                 rot = fabs(apq) > 1e-15 * sqrt(fabs(app * aqq));
-                double c = 1.0, s = 0.0;
-                if (rot) jacobi_cs(app, aqq, apq, c, s);
-                Ps[tid] = p; Qs[tid] = q; Rs[tid] = rot; Cs[tid] = c; Ss[tid] = s;
+                if (rot) jacobi_cs(app, aqq, apq, c, sn);
+                Ps[tid] = p; Qs[tid] = q; Rs[tid] = rot; Cs[tid] = c; Ss[tid] = sn;
             }
             if (!__syncthreads_or(rot)) continue;
             rotated = 1;
-            // One pass per round.  The row phase (rows p_k, q_k rotated by pair k)
-            // and then the column phase (columns p_m, q_m by pair m) touch the
-            // 2x2 block {p_k, q_k} x {p_m, q_m} only through its own four values,
-            // so one thread applies both phases to its blocks in that order (the
-            // same operations as two passes over the matrix) and zeroes a_pq of
-            // a rotated pair; 64 x 64 blocks and 64 x 128 Vt items over 1024
-            // threads, every value owned by one thread.  The Vt loads (global
-            // memory) are issued first so that their latency overlaps the blocks.
-            double v0[8], v1[8];
+            if (tid < 64) {
+                rpq[(size_t)g * 64 + tid] = p | (q << 8) | (rot << 16);
+                rcs[((size_t)g * 64 + tid) * 2] = c;
+                rcs[((size_t)g * 64 + tid) * 2 + 1] = sn;
+            }
+            ++g;
+            // The row phase (rows p_k, q_k rotated by pair k) and then the column
+            // phase (columns p_m, q_m by pair m) touch the 2x2 block {p_k, q_k} x
+            // {p_m, q_m} only through its own four values, so one thread applies
+            // both phases to its blocks in that order (the same operations as two
+            // passes over the matrix) and zeroes a_pq of a rotated pair; 64 x 64
+            // blocks over 1024 threads, every value owned by one thread.
+            // pair m = tid % 64 is the same in all four blocks, pair k = tid / 64 + 16 u
+            // is uniform over the wave; both are read before any A store
+            const int m = tid & 63;
+            const int pm = Ps[m], qm = Qs[m], rm = Rs[m];
+            const double cm = Cs[m], sm = Ss[m];
+            int pkk[4], qkk[4], rkk[4];
+            double ckk[4], skk[4];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int it = tid + 1024 * u, k = it >> 7, i = it & 127;
-                v0[u] = Vt[Ps[k] * 128 + i];
-                v1[u] = Vt[Qs[k] * 128 + i];
+            for (int u = 0; u < 4; ++u) {
+                const int k = (tid >> 6) + 16 * u;
+                pkk[u] = Ps[k]; qkk[u] = Qs[k]; rkk[u] = Rs[k]; ckk[u] = Cs[k]; skk[u] = Ss[k];
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const int bi = tid + 1024 * u, k = bi >> 6, m = bi & 63;
-                const int pk = Ps[k], qk = Qs[k], pm = Ps[m], qm = Qs[m];
+                const int k = (tid >> 6) + 16 * u;
+                const int pk = pkk[u], qk = qkk[u];
                 double a00 = A[pk * kAStride + pm], a01 = A[pk * kAStride + qm];
                 double a10 = A[qk * kAStride + pm], a11 = A[qk * kAStride + qm];
-                if (Rs[k]) {
-                    const double c = Cs[k], s = Ss[k];
-                    const double n00 = c * a00 - s * a10, n10 = s * a00 + c * a10;
-                    const double n01 = c * a01 - s * a11, n11 = s * a01 + c * a11;
+                if (rkk[u]) {
+                    const double cc = ckk[u], ss = skk[u];
+                    const double n00 = cc * a00 - ss * a10, n10 = ss * a00 + cc * a10;
+                    const double n01 = cc * a01 - ss * a11, n11 = ss * a01 + cc * a11;
                     a00 = n00; a10 = n10; a01 = n01; a11 = n11;
                 }
-                if (Rs[m]) {
-                    const double c = Cs[m], s = Ss[m];
-                    const double n00 = c * a00 - s * a01, n01 = s * a00 + c * a01;
-                    const double n10 = c * a10 - s * a11, n11 = s * a10 + c * a11;
+                if (rm) {
+                    const double cc = cm, ss = sm;
+                    const double n00 = cc * a00 - ss * a01, n01 = ss * a00 + cc * a01;
+                    const double n10 = cc * a10 - ss * a11, n11 = ss * a10 + cc * a11;
                     a00 = n00; a01 = n01; a10 = n10; a11 = n11;
                 }
-                if (k == m && Rs[k]) { a01 = 0.0; a10 = 0.0; }
+                if (k == m && rkk[u]) { a01 = 0.0; a10 = 0.0; }
                 A[pk * kAStride + pm] = a00; A[pk * kAStride + qm] = a01;
                 A[qk * kAStride + pm] = a10; A[qk * kAStride + qm] = a11;
-            }
-            {
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int it = tid + 1024 * u, k = it >> 7, i = it & 127;
-                    if (!Rs[k]) continue;
-                    const double c = Cs[k], s = Ss[k];
-                    Vt[Ps[k] * 128 + i] = c * v0[u] - s * v1[u];
-                    Vt[Qs[k] * 128 + i] = s * v0[u] + c * v1[u];
-                }
             }
             __syncthreads();
         }
         if (!rotated) break;
     }
+    if (tid < 128) diag[tid] = A[tid * (kAStride + 1)];
     if (tid == 0) {
-        for (int j = 0; j < 128; ++j) order[j] = j;
-        for (int a = 0; a < 128; ++a) {
-            int best = a;
-            for (int b = a + 1; b < 128; ++b) {
-                const double vb = A[order[b] * (kAStride + 1)], vbest = A[order[best] * (kAStride + 1)];
-                if (vb > vbest || (vb == vbest && order[b] < order[best])) best = b;
-            }
-            const int t = order[a]; order[a] = order[best]; order[best] = t;
-        }
         *sweeps_out = sweeps;
+        *nrounds = g;
+    }
+}
+
+// the recorded rotations applied to the identity: wave w of block b carries
+// column i = 4 b + w of Vt (Vt[p][i], Vt[q][i] per pair, the operations of a
+// row update of Vt), lane k pair k of every round, the column in LDS.  The
+// next kPcaPf rounds' records are in flight while a round is applied.
+constexpr int kPcaPf = 4;
+__global__ __launch_bounds__(256) void k_pca_vt(const int* __restrict__ rpq, const double* __restrict__ rcs,
+                                                const int* __restrict__ nrounds, double* __restrict__ Vt) {
+    __shared__ double col[4][128];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blockIdx.x * 4 + w;
+    double* v = col[w];
+    v[lane] = lane == i ? 1.0 : 0.0;
+    v[lane + 64] = lane + 64 == i ? 1.0 : 0.0;
+    const int G = *nrounds;
+    int pq[kPcaPf];
+    double c[kPcaPf], sn[kPcaPf];
+#pragma unroll
+    for (int d = 0; d < kPcaPf; ++d) {
+        const size_t e = (size_t)d * 64 + lane;
+        pq[d] = d < G ? rpq[e] : 0;
+        c[d] = d < G ? rcs[2 * e] : 1.0;
+        sn[d] = d < G ? rcs[2 * e + 1] : 0.0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int g0 = 0; g0 < G; g0 += kPcaPf) {
+#pragma unroll
+        for (int d = 0; d < kPcaPf; ++d) {
+            const int g = g0 + d;
+            if (g >= G) break;   // uniform
+            const int p = pq[d] & 255, q = (pq[d] >> 8) & 255;
+            if (pq[d] >> 16) {
+                const double v0 = v[p], v1 = v[q];
+                v[p] = c[d] * v0 - sn[d] * v1;
+                v[q] = sn[d] * v0 + c[d] * v1;
+            }
+            // the round's 64 pairs are disjoint; the next round reads the
+            // updated column (one wave: LDS operations complete in order)
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            const int gn = g + kPcaPf;
+            if (gn < G) {
+                const size_t e = (size_t)gn * 64 + lane;
+                pq[d] = rpq[e];
+                c[d] = rcs[2 * e];
+                sn[d] = rcs[2 * e + 1];
+            }
+        }
+    }
+    Vt[(size_t)lane * 128 + i] = v[lane];
+    Vt[(size_t)(lane + 64) * 128 + i] = v[lane + 64];
+}
+
+// eigenvalue order (descending, ties: lower index), evals, proj.  Thread t
+// ranks eigenvalue t by counting those ordered before it (a strict order, so
+// the ranks are a permutation: the order a selection sort by the same
+// comparison produces).
+__global__ __launch_bounds__(128) void k_pca_finish(const double* __restrict__ Vt, const double* __restrict__ diag,
+                                                    int dims, float* __restrict__ proj, double* __restrict__ evals) {
+    __shared__ int order[128];
+    __shared__ double dv[128];
+    const int tid = threadIdx.x;
+    dv[tid] = diag[tid];
+    __syncthreads();
+    {
+        const double mine = dv[tid];
+        int rank = 0;
+        for (int b = 0; b < 128; ++b) {
+            const double vb = dv[b];
+            rank += (vb > mine || (vb == mine && b < tid)) ? 1 : 0;
+        }
+        order[rank] = tid;
     }
     __syncthreads();
-    if (tid < 128) evals[tid] = A[order[tid] * (kAStride + 1)];
+    evals[tid] = dv[order[tid]];
     if (tid < dims) {
         const double* v = Vt + (size_t)order[dims - 1 - tid] * 128;
+        // first largest magnitude: the loads do not depend on the running argmax
         int arg = 0;
-        for (int j = 1; j < 128; ++j)
-            if (fabs(v[j]) > fabs(v[arg])) arg = j;
+        double best = fabs(v[0]);
+#pragma unroll 8
+        for (int j = 1; j < 128; ++j) {
+            const double a = fabs(v[j]);
+            if (a > best) { best = a; arg = j; }
+        }
         const double sg = v[arg] < 0.0 ? -1.0 : 1.0;
+#pragma unroll 8
         for (int j = 0; j < 128; ++j) proj[j * dims + tid] = (float)(sg * v[j]);
     }
 }
@@ -281,7 +353,7 @@ __global__ __launch_bounds__(256) void k_pca_project(const float* __restrict__ x
 
 struct PcaPlan {
     int n, nch;
-    size_t o_x, o_part, o_mean64, o_cov, o_vt, total;
+    size_t o_x, o_part, o_mean64, o_cov, o_vt, o_rpq, o_rcs, o_diag, o_nr, total;
 };
 
 PcaPlan pca_plan(int N, int M) {
@@ -295,6 +367,11 @@ PcaPlan pca_plan(int N, int M) {
     P.o_mean64 = take(128 * sizeof(double));
     P.o_cov = take(16384 * sizeof(double));
     P.o_vt = take(16384 * sizeof(double));
+    // the Jacobi rotations, at most kPcaMaxSweeps x 127 rounds of 64 pairs
+    P.o_rpq = take((size_t)kPcaMaxSweeps * 127 * 64 * sizeof(int));
+    P.o_rcs = take((size_t)kPcaMaxSweeps * 127 * 64 * 2 * sizeof(double));
+    P.o_diag = take(128 * sizeof(double));
+    P.o_nr = take(sizeof(int));
     P.total = off;
     return P;
 }
@@ -333,7 +410,14 @@ int pca_fit(const float* a, int N, const float* b, int M, int dims, void* ws_, s
     hipLaunchKernelGGL(k_pca_mean, dim3(1), dim3(128), 0, s, (const double*)part, P.nch, P.n, mean64, mean);
     hipLaunchKernelGGL(k_pca_gram_part, dim3(P.nch), dim3(256), 0, s, (const float*)X, P.n, (const double*)mean64, part);
     hipLaunchKernelGGL(k_pca_cov, dim3(64), dim3(256), 0, s, (const double*)part, P.nch, P.n, cov);
-    hipLaunchKernelGGL(k_pca_jacobi, dim3(1), dim3(1024), 0, s, (const double*)cov, vt, dims, proj, evals, sweeps);
+    int* rpq = (int*)(ws + P.o_rpq);
+    double* rcs = (double*)(ws + P.o_rcs);
+    double* diag = (double*)(ws + P.o_diag);
+    int* nr = (int*)(ws + P.o_nr);
+    hipLaunchKernelGGL(k_pca_jacobi, dim3(1), dim3(1024), 0, s, (const double*)cov, rpq, rcs, nr, diag, sweeps);
+    hipLaunchKernelGGL(k_pca_vt, dim3(32), dim3(256), 0, s, (const int*)rpq, (const double*)rcs, (const int*)nr, vt);
+    hipLaunchKernelGGL(k_pca_finish, dim3(1), dim3(128), 0, s, (const double*)vt, (const double*)diag, dims, proj,
+                       evals);
     MSIFT_LAUNCH_CHECK();
     return MSIFT_OK;
 }
