@@ -163,8 +163,9 @@ int msift_match_set_tile(int shape, int chunks_per_block);
  * reference: it is for callers that cannot allocate device memory, such as
  * code under HIP graph capture or a serving loop that forbids hipMalloc).
  * Same arguments and outputs as msift_match_u8 without the workspace: the
- * scratch (norms, partial top-2 keys, block counts) lives in a static device
- * array of 12.8 MB per GPU, held from the first call on that GPU until the
+ * scratch (norms, the two-pass method's y' copies and key terms, partial
+ * top-2 keys, block counts) lives in a static device
+ * array of 17.0 MB per GPU, held from the first call on that GPU until the
  * process ends, which limits N and M to 16384 each (larger sizes return
  * MSIFT_EINVAL).  The stream must belong to the current device.  Every call
  * on one GPU shares that GPU's scratch, so calls that overlap on one GPU
