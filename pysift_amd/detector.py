@@ -131,7 +131,10 @@ class PySIFT:
         if isinstance(gray, np.ndarray):
             if gray.dtype != np.uint8 or gray.ndim != 2:
                 raise ValueError("gray must be a 2-D uint8 array")
-            return torch.from_numpy(np.ascontiguousarray(gray)).to(dev, non_blocking=False)
+            arr = np.ascontiguousarray(gray)
+            if not arr.flags.writeable:   # torch.from_numpy needs a writable buffer
+                arr = arr.copy()
+            return torch.from_numpy(arr).to(dev, non_blocking=False)
         if not isinstance(gray, torch.Tensor):
             gray = torch.from_dlpack(gray)
         if gray.dtype != torch.uint8 or gray.dim() != 2:

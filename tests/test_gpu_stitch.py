@@ -83,7 +83,7 @@ def test_stitch_errors():
         st.stitch(flat)
 
 
-def test_autocrop_box_matches_host_autocrop():
+def test_autocrop_box_matches_oracle_autocrop():
     # the device rectangle against the oracle's restatement of _autocrop
     # (core.py:3627-3684) on masks with ragged edges, holes, one valid row or
     # column, an all-valid and an empty mask
