@@ -1102,3 +1102,13 @@ def test_autocrop_row_column_convex_masks():
         else:
             assert (y0, y1, x0, x1) == (rows[0], rows[-1] + 1, cols[0], cols[-1] + 1), t
     assert n_checked >= 10
+
+
+def test_u8_unit_by_double_product_is_exact():
+    # the device column prefilter forms (float)v / 255 as (float)(v * (1 / 255.0))
+    # in double: equal to the correctly rounded float quotient for every byte
+    v = np.arange(256)
+    q = np.float32(v) / np.float32(255)
+    np.testing.assert_array_equal((v.astype(np.float64) * (1.0 / 255.0)).astype(np.float32), q)
+    # a plausible slip, a float reciprocal, differs for some bytes
+    assert not np.array_equal(np.float32(v) * np.float32(1.0 / 255.0), q)
