@@ -85,7 +85,7 @@ __device__ __forceinline__ float ups_interp(const double* __restrict__ cf, int H
 
 // tiled forms of the three spline kernels (whole block, 256 threads); every
 // output is the same fma chain as ups_pf_col / ups_pf_row / ups_interp
-constexpr int kUpsColRB = 8;              // column prefilter: rows per thread
+constexpr int kUpsColRB = 16;             // column prefilter: rows per thread
 constexpr int kUpsColTW = 64, kUpsColTH = 4 * kUpsColRB;
 constexpr int kUpsRowPT = 8;              // row prefilter: outputs per thread
 constexpr int kUpsRowTH = 32, kUpsRowTW = 8 * kUpsRowPT;   // its 32 x 64 tile
