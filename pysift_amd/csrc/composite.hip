@@ -53,6 +53,7 @@ struct W9 { float w[9]; };
 struct W25 { float w[25]; };
 
 __device__ __forceinline__ int refl_half(int p, int n) {
+    if ((unsigned)p < (unsigned)n) return p;
     if (n == 1) return 0;
     const int period = 2 * n;
     p %= period;
@@ -61,6 +62,7 @@ __device__ __forceinline__ int refl_half(int p, int n) {
 }
 
 __device__ __forceinline__ int refl_101(int p, int n) {
+    if ((unsigned)p < (unsigned)n) return p;
     if (n == 1) return 0;
     const int period = 2 * n - 2;
     p %= period;
@@ -358,20 +360,29 @@ __global__ __launch_bounds__(kCT) void k_cmp_sobel2(const float* __restrict__ dx
 // vertical seam, columns otherwise; a cell belongs to the overlap exactly when
 // its cost is below 1e9), in bands of kDpK lines.  Line i's value at n needs
 // line i - 1 at n - 1, n, n + 1 only, so the workgroup that owns cells
-// [s0, s1) of a band can compute them from the band's input line over
-// [s0 - kDpK, s1 + kDpK), shrinking the valid range by one cell a side per
-// line (not at the line's ends, where the neighbours are +inf).  Every cell is
-// thus computed by the same operations as a line-by-line pass, partly twice;
-// each workgroup writes the back-pointers and the band's output values of the
-// cells it owns.  One launch per band; the backtrack runs in a last single
-// workgroup.
-constexpr int kDpS = 256;               // cells a workgroup owns
-constexpr int kDpK = 128;               // lines per band
+// [s0, s0 + kDpS) of a band can compute them from the band's input line over
+// [s0 - kDpK, s0 + kDpS + kDpK), the valid range shrinking by one cell a side
+// per line (not at the line's ends, where the neighbours are +inf).  Every
+// cell is thus computed by the same operations as a line-by-line pass, partly
+// twice; each workgroup writes the back-pointers and the band's output values
+// of the cells it owns.  One launch per band; the backtrack runs in a last
+// single workgroup.
+//
+// A workgroup is one wave: lane j holds cells a + j + 64 c (c < kDpC) in
+// registers and takes its neighbours' values from the adjacent lanes by DPP
+// wave rotations, so a line needs no LDS round trip and no barrier.
+constexpr int kDpS = 64;                // cells a workgroup owns
+constexpr int kDpK = 64;                // lines per band
 constexpr int kDpR = kDpS + 2 * kDpK;   // cells a workgroup computes
-constexpr int kDpT = 256;               // threads
-constexpr int kDpC = kDpR / kDpT;       // cells per thread
+constexpr int kDpC = kDpR / 64;         // cells per lane
 constexpr int kDpPf = 8;                // lines of costs in flight
-constexpr int kWin = 64;
+constexpr int kWin = 64;                // backtrack window (lines)
+constexpr int kWw = 2 * kWin + 1;       // cells a window can reach
+constexpr int kFill = (kWin * kWw + 1023) / 1024;   // back-pointers per thread and window
+static_assert(kDpS == 64 && kDpK == 64, "k_cmp_dp_band: one owned cell and one band line per lane");
+
+// back-pointer row stride: N rounded up to a whole number of workgroups' cells
+__host__ __device__ __forceinline__ int dp_bt_stride(int N) { return (N + kDpS - 1) / kDpS * kDpS; }
 
 struct DpGeom {
     int L, N;
@@ -431,79 +442,133 @@ __global__ __launch_bounds__(1024) void k_cmp_dp_lines(const float* __restrict__
     }
 }
 
+// 64-bit value of lane j - 1 (lane 0: lane 63) and of lane j + 1 (lane 63: lane 0)
+__device__ __forceinline__ double dpp_ror1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x13C, 0xF, 0xF, false);          // wave_ror:1
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x13C, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double dpp_rol1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x134, 0xF, 0xF, false);          // wave_rol:1
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x134, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // band b: lines i0 = 1 + b kDpK .. min(nl, i0 + kDpK) - 1 from dp_in (line
 // i0 - 1) to dp_out; blockIdx.x owns cells [kDpS x, +kDpS)
-__global__ __launch_bounds__(kDpT) void k_cmp_dp_band(const float* __restrict__ cost,
-                                                      const uint32_t* __restrict__ rowflags,
-                                                      const uint32_t* __restrict__ colflags, int cw, int ch,
-                                                      const CmpStats* __restrict__ st,
-                                                      const int32_t* __restrict__ lines, const int* __restrict__ nlp,
-                                                      int band, const double* __restrict__ dp_in,
-                                                      double* __restrict__ dp_out, uint8_t* __restrict__ bt) {
-    __shared__ double d[2][kDpR];
+__global__ __launch_bounds__(64) void k_cmp_dp_band(const float* __restrict__ cost,
+                                                    const uint32_t* __restrict__ rowflags,
+                                                    const uint32_t* __restrict__ colflags, int cw, int ch,
+                                                    const CmpStats* __restrict__ st,
+                                                    const int32_t* __restrict__ lines, const int* __restrict__ nlp,
+                                                    int band, const double* __restrict__ dp_in,
+                                                    double* __restrict__ dp_out, uint8_t* __restrict__ bt) {
     if (!st->has_overlap) return;
     const DpGeom g = dp_geom(st, rowflags, colflags, cw, ch);
     const int N = g.N, nl = *nlp;
     const int i0 = 1 + band * kDpK;
     if (i0 >= nl) return;
     const int i1 = min(nl, i0 + kDpK);
-    const int s0 = blockIdx.x * kDpS, s1 = min(N, s0 + kDpS);
+    const int s0 = blockIdx.x * kDpS;
     if (s0 >= N) return;
-    const int a = max(0, s0 - kDpK), e = min(N, s1 + kDpK);   // computed cells [a, e)
-    const int tid = threadIdx.x;
+    const int a = max(0, s0 - kDpK), e = min(N, s0 + kDpS + kDpK);   // computed cells [a, e)
+    const int lane = threadIdx.x;
     const double INF = 1e18;
-    for (int x = tid; x < e - a; x += kDpT) d[0][x] = dp_in[a + x];
-    // costs of the next kDpPf lines: slot k holds line i for i = k (mod kDpPf);
-    // thread tid's cells are a + tid + kDpT c
+    // The owned cells are register cown = (s0 - a) / 64 of every lane (s0 - a
+    // is 0 or kDpK); the owned cell s0 + lane may lie in the row's padding past
+    // N, which bt rows of dp_bt_stride(N) bytes hold.  Cells outside the valid
+    // range are computed too, from whatever their neighbours hold: no valid
+    // cell ever reads them.
+    const int np = dp_bt_stride(N);
+    const int cown = (s0 - a) >> 6;
+    // the band's line numbers, lane q holding line i0 + q (clamped to the band)
+    const int lnv = lines[min(i0 + lane, i1 - 1)];
+    double dv[kDpC];
+    bool okl[kDpC], okr[kDpC];
+#pragma unroll
+    for (int c = 0; c < kDpC; ++c) {
+        const int n = a + lane + 64 * c;
+        dv[c] = n < e ? dp_in[n] : INF;
+        okl[c] = n > 0;
+        okr[c] = n < N - 1;
+    }
+    // Costs of the next kDpPf lines: slot k holds line i for i = k (mod kDpPf).
+    // Every line issues the same two loads per lane (clamped addresses, cells
+    // past e are never valid) and one back-pointer store, with no early exit
+    // from the unrolled steps, so the compiler waits only for the load a step
+    // needs.
     float cb[kDpPf][kDpC];
     auto load_line = [&](int i, float* dst) {
-        if (i >= i1) return;   // uniform
-        const float* row = cost + (size_t)lines[i] * N;
+        const int q = (i < i1 ? i : i1 - 1) - i0;   // past the band: a line of the band, never used
+        const float* row = cost + (size_t)__builtin_amdgcn_readlane(lnv, q) * N;
 #pragma unroll
         for (int c = 0; c < kDpC; ++c) {
-            const int n = a + tid + kDpT * c;
-            dst[c] = n < e ? row[n] : 1e9f;
+            const int n = a + lane + 64 * c;
+            dst[c] = row[n < e ? n : e - 1];
         }
     };
 #pragma unroll
     for (int k = 0; k < kDpPf; ++k) load_line(i0 + k, cb[k]);
-    __syncthreads();
-    int cur = 0;
-    for (int ib = i0; ib < i1; ib += kDpPf) {
+    int own = 0;
+    uint8_t* btl = bt + s0 + lane;
+    // one line; a step past the band's last line (live false, only in the last
+    // group) changes nothing and stores the last line's back-pointer again
+    // (same lane, same value)
+    auto step = [&](int k, int i, bool live) __attribute__((always_inline)) {
+        const int l = __builtin_amdgcn_readlane(lnv, (live ? i : i1 - 1) - i0);
+        double rr[kDpC], rl[kDpC], nv[kDpC];
 #pragma unroll
-        for (int k = 0; k < kDpPf; ++k) {
-            const int i = ib + k;
-            if (i >= i1) break;   // uniform
-            const int t = i - i0 + 1;
-            const int lo = a == 0 ? 0 : a + t, hi = e == N ? N : e - t;
-            const int l = lines[i];
-#pragma unroll
-            for (int c = 0; c < kDpC; ++c) {
-                const int x = tid + kDpT * c, n = a + x;
-                if (n < lo || n >= hi) continue;
-                const float cst = cb[k][c];
-                const bool ov = cst < 1e9f;
-                const double pl = n > 0 ? d[cur][x - 1] : INF, pm = d[cur][x];
-                const double pr = n < N - 1 ? d[cur][x + 1] : INF;
-                int arg = 0;
-                double best = pl;
-                if (pm < best) { best = pm; arg = 1; }
-                if (pr < best) { best = pr; arg = 2; }
-                d[cur ^ 1][x] = ov ? (double)cst + best : INF;
-                if (n >= s0 && n < s1) bt[(size_t)l * N + n] = (uint8_t)arg;
-            }
-            load_line(i + kDpPf, cb[k]);
-            cur ^= 1;
-            __syncthreads();
+        for (int c = 0; c < kDpC; ++c) {
+            rr[c] = dpp_ror1(dv[c]);
+            rl[c] = dpp_rol1(dv[c]);
         }
+        int aown = 0;
+#pragma unroll
+        for (int c = 0; c < kDpC; ++c) {
+            double pl = lane == 0 ? (c > 0 ? rr[c - 1] : INF) : rr[c];
+            double pr = lane == 63 ? (c < kDpC - 1 ? rl[c + 1] : INF) : rl[c];
+            pl = okl[c] ? pl : INF;
+            pr = okr[c] ? pr : INF;
+            const double pm = dv[c];
+            int arg = 0;
+            double best = pl;
+            if (pm < best) { best = pm; arg = 1; }
+            if (pr < best) { best = pr; arg = 2; }
+            const float cst = cb[k][c];
+            nv[c] = cst < 1e9f ? (double)cst + best : INF;
+            aown = c == cown ? arg : aown;
+        }
+        if (live) {
+#pragma unroll
+            for (int c = 0; c < kDpC; ++c) dv[c] = nv[c];
+            own = aown;
+        }
+        btl[(size_t)l * np] = (uint8_t)own;
+        load_line(i + kDpPf, cb[k]);
+    };
+    // Whole groups of kDpPf lines, then the last partial group: the main loop
+    // has no branch inside, so the compiler counts the loads in flight exactly.
+    int ib = i0;
+    for (; ib + kDpPf <= i1; ib += kDpPf) {
+#pragma unroll
+        for (int k = 0; k < kDpPf; ++k) step(k, ib + k, true);
     }
-    for (int n = s0 + tid; n < s1; n += kDpT) dp_out[n] = d[cur][n - a];
+    if (ib < i1) {
+#pragma unroll
+        for (int k = 0; k < kDpPf; ++k) step(k, ib + k, ib + k < i1);
+    }
+    double vo = dv[0];
+#pragma unroll
+    for (int c = 1; c < kDpC; ++c) vo = c == cown ? dv[c] : vo;
+    if (s0 + lane < N) dp_out[s0 + lane] = vo;
 }
 
-// the first minimum over the last line's overlap cells, then the backtrack.
-// The backtrack runs in windows of kWin lines: the seam moves at most one cell
-// per line, so the block first loads those lines' back-pointers within kWin
-// cells of the current position into LDS, then one thread walks them.
+// the first minimum over the last line's overlap cells, then the backtrack in
+// windows of kWin lines: the seam moves at most one cell per line, so the block
+// loads those lines' back-pointers within kWin cells of the current position
+// into LDS and resolves the window's cells by a scan of the back-pointer maps.
 __global__ __launch_bounds__(1024) void k_cmp_dp_back(const float* __restrict__ cost,
                                                       const uint32_t* __restrict__ rowflags,
                                                       const uint32_t* __restrict__ colflags, int cw, int ch,
@@ -511,13 +576,15 @@ __global__ __launch_bounds__(1024) void k_cmp_dp_back(const float* __restrict__ 
                                                       const int32_t* __restrict__ lines, const int* __restrict__ nlp,
                                                       const double* __restrict__ dpa, const double* __restrict__ dpb,
                                                       const uint8_t* __restrict__ bt, int32_t* __restrict__ seam) {
-    __shared__ uint8_t win[kWin][2 * kWin + 1];
+    __shared__ uint8_t F[2][kWin][kWw];
+    __shared__ int s_ln[2][kWin], s_row[2][kWin];
     __shared__ double rv[16];
     __shared__ int ri[16];
     __shared__ int s_pos;
     if (!st->has_overlap) return;
     const DpGeom g = dp_geom(st, rowflags, colflags, cw, ch);
     const int L = g.L, N = g.N, nl = *nlp;
+    const int np = dp_bt_stride(N);
     const int tid = threadIdx.x;
     if (nl == 0) {
         for (int l = tid; l < L; l += 1024) seam[l] = 0;
@@ -551,29 +618,71 @@ __global__ __launch_bounds__(1024) void k_cmp_dp_back(const float* __restrict__ 
     __syncthreads();
     const int sc = s_pos;
     for (int l = tid; l < L; l += 1024) seam[l] = sc;
+    if (tid < min(kWin, nl - 1)) {   // the first window's line numbers
+        s_row[0][tid] = lines[nl - 1 - tid];
+        s_ln[0][tid] = lines[nl - 2 - tid];
+    }
     __syncthreads();
-    // walk back: entry i (from nl - 1 down to 1) sets seam[lines[i - 1]]
-    for (int i = nl - 1; i > 0; i -= kWin) {
+    // Walk back: entry i (from nl - 1 down to 1) sets seam[lines[i - 1]] to
+    // clamp(pos + bt[lines[i]][pos] - 1, 0, N - 1), pos being the seam's cell
+    // on line lines[i].  A window of kWin entries moves the seam at most kWin
+    // cells, so in window coordinates x = pos - pos0 + kWin (pos0: the cell at
+    // the window's first entry) each entry j is a map T_j of [0, 2 kWin] into
+    // itself.  An inclusive scan of the maps under composition (log2 kWin
+    // doubling rounds, F_j <- F_j o F_(j - d)) gives every entry's cell at
+    // once; entries at x off the seam's path may hold clamped garbage, which no
+    // evaluation on the path reads.
+    int pos0 = sc, wb = 0;
+    for (int i = nl - 1; i > 0; i -= kWin, wb ^= 1) {
         const int cnt = min(kWin, i);           // entries i, i - 1, ..., i - cnt + 1
-        const int pos0 = s_pos;
-        for (int q = tid; q < cnt * (2 * kWin + 1); q += 1024) {
-            const int j = q / (2 * kWin + 1), dd = q - j * (2 * kWin + 1);
-            int p = pos0 + dd - kWin;
+        // All of a thread's back-pointer loads are in flight at once: the
+        // indexes are clamped and nothing branches; a thread past the window's
+        // last entry computes and writes that entry's own values again.
+        const int tot = cnt * kWw;
+        uint8_t v[kFill];
+        int pk[kFill], jk[kFill], xk[kFill];
+#pragma unroll
+        for (int k = 0; k < kFill; ++k) {
+            const int q = min(tid + 1024 * k, tot - 1);
+            const int j = q / kWw, x = q - j * kWw;
+            int p = pos0 + x - kWin;
             p = p < 0 ? 0 : (p > N - 1 ? N - 1 : p);
-            win[j][dd] = bt[(size_t)lines[i - j] * N + p];
+            pk[k] = p;
+            jk[k] = j;
+            xk[k] = x;
+            v[k] = bt[(size_t)s_row[wb][j] * np + p];
         }
-        __syncthreads();
-        if (tid == 0) {
-            int pos = pos0;
-            for (int j = 0; j < cnt; ++j) {
-                int b = pos + (int)win[j][pos - pos0 + kWin] - 1;
-                b = b < 0 ? 0 : (b > N - 1 ? N - 1 : b);
-                seam[lines[i - j - 1]] = b;
-                pos = b;
+#pragma unroll
+        for (int k = 0; k < kFill; ++k) {
+            int b = pk[k] + (int)v[k] - 1;
+            b = b < 0 ? 0 : (b > N - 1 ? N - 1 : b);
+            const int t = b - pos0 + kWin;
+            F[0][jk[k]][xk[k]] = (uint8_t)(t < 0 ? 0 : (t > kWw - 1 ? kWw - 1 : t));
+        }
+        lds_barrier();
+        // the next window's line numbers load during this window's scan
+        const int inx = i - kWin;
+        if (inx > 0 && tid < min(kWin, inx)) {
+            s_row[wb ^ 1][tid] = lines[inx - tid];
+            s_ln[wb ^ 1][tid] = lines[inx - tid - 1];
+        }
+        int cur = 0;
+        for (int d = 1; d < cnt; d <<= 1) {
+            uint8_t nv[kFill];
+#pragma unroll
+            for (int k = 0; k < kFill; ++k) {
+                const int j = jk[k], x = xk[k];
+                const int y = j >= d ? (int)F[cur][j - d][x] : x;
+                nv[k] = F[cur][j][y];
             }
-            s_pos = pos;
+#pragma unroll
+            for (int k = 0; k < kFill; ++k) F[cur ^ 1][jk[k]][xk[k]] = nv[k];
+            lds_barrier();
+            cur ^= 1;
         }
-        __syncthreads();
+        if (tid < cnt) seam[s_ln[wb][tid]] = pos0 + (int)F[cur][tid][kWin] - kWin;
+        pos0 += (int)F[cur][cnt - 1][kWin] - kWin;
+        lds_barrier();   // F and this window's line numbers are rewritten next
     }
 }
 
@@ -591,14 +700,25 @@ __global__ __launch_bounds__(kCT) void k_cmp_seam_mask(const uint8_t* __restrict
 }
 
 // 25-tap Gaussian of the seam mask: rows into tmp, then columns into the
-// mask plane (inside the overlap; the unblurred mask elsewhere)
+// mask plane (inside the overlap; the unblurred mask elsewhere).  The row pass
+// stages the block's 256 + 24 source values in LDS once.
 __global__ __launch_bounds__(kCT) void k_cmp_blur_rows(const float* __restrict__ in, int cw, W25 w,
                                                        const CmpStats* __restrict__ st, float* __restrict__ out) {
-    const int u = blockIdx.x * kCT + threadIdx.x, v = blockIdx.y;
-    if (u >= cw || !st->has_overlap) return;
+    __shared__ float t[kCT + 24];
+    if (!st->has_overlap) return;
+    const int u0 = blockIdx.x * kCT, v = blockIdx.y;
+    const float* row = in + (size_t)v * cw;
+    const int hi = min(u0 + kCT, cw) + 12;   // sources [u0 - 12, hi) feed the block's outputs
+    for (int k = threadIdx.x; k < kCT + 24; k += kCT) {
+        const int p = u0 - 12 + k;
+        if (p < hi) t[k] = row[refl_101(p, cw)];
+    }
+    __syncthreads();
+    const int u = u0 + threadIdx.x;
+    if (u >= cw) return;
     float acc = 0.0f;
 #pragma unroll
-    for (int k = 0; k < 25; ++k) acc = acc + w.w[k] * in[(size_t)v * cw + refl_101(u + k - 12, cw)];
+    for (int k = 0; k < 25; ++k) acc = acc + w.w[k] * t[threadIdx.x + k];
     out[(size_t)v * cw + u] = acc;
 }
 
@@ -619,26 +739,48 @@ __global__ __launch_bounds__(kCT) void k_cmp_blur_cols(const float* __restrict__
     mask[o] = r;
 }
 
-// pyramid down, rows at the kept even rows: grid (ceil(W / 256), Ho, planes)
+// pyramid down, rows at the kept even rows: grid (ceil(W / 256),
+// ceil(Ho / kPdR), planes).  A thread makes kPdR output rows of its column from
+// 2 kPdR + 7 input rows held in registers (rows past the plane's end are
+// reflected like any other and never used).
+constexpr int kPdR = 8;
 __global__ __launch_bounds__(kCT) void k_pyr_down_rows(const float* __restrict__ in, int H, int W, int Ho, W9 w,
                                                        float* __restrict__ tmp) {
-    const int x = blockIdx.x * kCT + threadIdx.x, yo = blockIdx.y, q = blockIdx.z;
+    const int x = blockIdx.x * kCT + threadIdx.x, yo0 = blockIdx.y * kPdR, q = blockIdx.z;
     if (x >= W) return;
-    const float* src = in + (size_t)q * H * W;
-    float acc = 0.0f;
+    const float* src = in + (size_t)q * H * W + x;
+    float r[2 * kPdR + 7];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) acc = fmaf(src[(size_t)refl_half(2 * yo + k - 4, H) * W + x], w.w[k], acc);
-    tmp[(size_t)q * Ho * W + (size_t)yo * W + x] = acc;
+    for (int m = 0; m < 2 * kPdR + 7; ++m) r[m] = src[(size_t)refl_half(2 * yo0 + m - 4, H) * W];
+    float* dst = tmp + (size_t)q * Ho * W + (size_t)yo0 * W + x;
+#pragma unroll
+    for (int i = 0; i < kPdR; ++i) {
+        if (yo0 + i >= Ho) break;
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) acc = fmaf(r[2 * i + k], w.w[k], acc);
+        dst[(size_t)i * W] = acc;
+    }
 }
 
+// then columns: grid (ceil(Wo / 256), Ho, planes); the block's 2 x 256 + 8
+// source values are staged in LDS once
 __global__ __launch_bounds__(kCT) void k_pyr_down_cols(const float* __restrict__ tmp, int W, int Ho, int Wo, W9 w,
                                                        float* __restrict__ out) {
-    const int xo = blockIdx.x * kCT + threadIdx.x, yo = blockIdx.y, q = blockIdx.z;
-    if (xo >= Wo) return;
+    __shared__ float t[2 * kCT + 8];
+    const int xo0 = blockIdx.x * kCT, yo = blockIdx.y, q = blockIdx.z;
     const float* src = tmp + (size_t)q * Ho * W + (size_t)yo * W;
+    const int hi = 2 * min(xo0 + kCT, Wo) + 3;   // sources [2 xo0 - 4, hi) feed the block's outputs
+    for (int k = threadIdx.x; k < 2 * kCT + 8; k += kCT) {
+        const int p = 2 * xo0 - 4 + k;
+        if (p < hi) t[k] = src[refl_half(p, W)];
+    }
+    __syncthreads();
+    const int xo = xo0 + threadIdx.x;
+    if (xo >= Wo) return;
     float acc = 0.0f;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) acc = fmaf(src[refl_half(2 * xo + k - 4, W)], w.w[k], acc);
+    for (int k = 0; k < 9; ++k) acc = fmaf(t[2 * threadIdx.x + k], w.w[k], acc);
     out[(size_t)q * Ho * Wo + (size_t)yo * Wo + xo] = acc;
 }
 
@@ -670,24 +812,74 @@ __global__ __launch_bounds__(kCT) void k_pyr_top(const float* __restrict__ lvl, 
     res[c * n + o] = m * lvl[c * n + o] + (1.0f - m) * lvl[(c + 3) * n + o];
 }
 
-// level k from level k + 1: res_k = up(res_k+1) + m (g1 - up g1') + (1 - m) (g2 - up g2')
+// level k from level k + 1: res_k = up(res_k+1) + m (g1 - up g1') + (1 - m) (g2 - up g2'),
+// the three channels per thread: grid (ceil(W / 256), H).  The block's two
+// source rows of the nine upsampled planes (g1' and g2' of level k + 1, its
+// result) are staged in LDS when the block's source columns fit kUpSpan.
+constexpr int kUpSpan = 160;
+static_assert(kUpSpan <= kCT, "k_pyr_up_blend stages one source column per thread");
+__device__ __forceinline__ float zoom_row2(const float* __restrict__ r0, const float* __restrict__ r1, int x0, int x1,
+                                           double wy0, double wy1, double wx0, double wx1) {
+    double acc = (double)r0[x0] * (wy0 * wx0);
+    acc = acc + (double)r0[x1] * (wy0 * wx1);
+    acc = acc + (double)r1[x0] * (wy1 * wx0);
+    acc = acc + (double)r1[x1] * (wy1 * wx1);
+    return (float)acc;
+}
+
 __global__ __launch_bounds__(kCT) void k_pyr_up_blend(const float* __restrict__ lvk, int H, int W,
                                                       const float* __restrict__ lvn, int h, int w,
                                                       const float* __restrict__ resn, float* __restrict__ res) {
-    const int x = blockIdx.x * kCT + threadIdx.x, y = blockIdx.y, c = blockIdx.z;
-    if (x >= W) return;
-    int y0, y1, x0, x1;
-    double wy0, wy1, wx0, wx1;
+    __shared__ float sr[9][2][kUpSpan];
+    const int xb = blockIdx.x * kCT, y = blockIdx.y, x = xb + threadIdx.x;
+    int y0, y1;
+    double wy0, wy1;
     zoom_coord(y, h, H, y0, y1, wy0, wy1);
+    int xa, span;
+    {
+        int i0, i1;
+        double a0, a1;
+        zoom_coord(xb, w, W, i0, i1, a0, a1);
+        xa = i0;
+        zoom_coord(min(xb + kCT, W) - 1, w, W, i0, i1, a0, a1);
+        span = i1 - xa + 1;
+    }
+    const size_t nn = (size_t)h * w;
+    const bool staged = span <= kUpSpan;   // uniform
+    if (staged) {
+        const int cc = threadIdx.x;   // kUpSpan <= kCT: one column per thread
+        if (cc < span) {
+#pragma unroll
+            for (int pl = 0; pl < 9; ++pl) {
+                const float* base = (pl < 6 ? lvn + (size_t)pl * nn : resn + (size_t)(pl - 6) * nn) + xa + cc;
+                sr[pl][0][cc] = base[(size_t)y0 * w];
+                sr[pl][1][cc] = base[(size_t)y1 * w];
+            }
+        }
+        __syncthreads();
+    }
+    if (x >= W) return;
+    int x0, x1;
+    double wx0, wx1;
     zoom_coord(x, w, W, x0, x1, wx0, wx1);
-    const size_t n = (size_t)H * W, nn = (size_t)h * w, o = (size_t)y * W + x;
-    const float u1 = zoom_at(lvn + c * nn, w, y0, y1, x0, x1, wy0, wy1, wx0, wx1);
-    const float u2 = zoom_at(lvn + (c + 3) * nn, w, y0, y1, x0, x1, wy0, wy1, wx0, wx1);
-    const float ur = zoom_at(resn + c * nn, w, y0, y1, x0, x1, wy0, wy1, wx0, wx1);
+    const size_t n = (size_t)H * W, o = (size_t)y * W + x;
     const float m = lvk[6 * n + o];
-    const float l1 = lvk[c * n + o] - u1, l2 = lvk[(c + 3) * n + o] - u2;
-    const float lb = m * l1 + (1.0f - m) * l2;
-    res[c * n + o] = ur + lb;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float u1, u2, ur;
+        if (staged) {
+            u1 = zoom_row2(sr[c][0], sr[c][1], x0 - xa, x1 - xa, wy0, wy1, wx0, wx1);
+            u2 = zoom_row2(sr[c + 3][0], sr[c + 3][1], x0 - xa, x1 - xa, wy0, wy1, wx0, wx1);
+            ur = zoom_row2(sr[6 + c][0], sr[6 + c][1], x0 - xa, x1 - xa, wy0, wy1, wx0, wx1);
+        } else {
+            u1 = zoom_at(lvn + c * nn, w, y0, y1, x0, x1, wy0, wy1, wx0, wx1);
+            u2 = zoom_at(lvn + (c + 3) * nn, w, y0, y1, x0, x1, wy0, wy1, wx0, wx1);
+            ur = zoom_at(resn + c * nn, w, y0, y1, x0, x1, wy0, wy1, wx0, wx1);
+        }
+        const float l1 = lvk[c * n + o] - u1, l2 = lvk[(c + 3) * n + o] - u2;
+        const float lb = m * l1 + (1.0f - m) * l2;
+        res[c * n + o] = ur + lb;
+    }
 }
 
 __global__ __launch_bounds__(kCT) void k_cmp_final(const float* __restrict__ res, const uint8_t* __restrict__ m1,
@@ -744,7 +936,8 @@ CmpPlan cmp_plan(int cw, int ch, int n_levels) {
     for (int k = 0; k < n_levels; ++k) Pl.o_res[k] = take((size_t)3 * Pl.hs[k] * Pl.ws[k] * 4);
     const int N = std::max(cw, ch);
     Pl.o_dpa = take((size_t)N * 8); Pl.o_dpb = take((size_t)N * 8);
-    Pl.o_bt = take(P);
+    // back-pointers: L lines of dp_bt_stride(N) bytes, for either seam orientation
+    Pl.o_bt = take(std::max((size_t)ch * dp_bt_stride(cw), (size_t)cw * dp_bt_stride(ch)));
     Pl.o_seam = take((size_t)N * 4);
     Pl.o_lines = take((size_t)N * 4);
     Pl.o_nl = take(4);
@@ -861,7 +1054,7 @@ int composite(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h2, 
                            (const uint32_t*)colf, cw, ch, (const CmpStats*)st, lines, nlp, dpa);
         const int nbands = (Lmax - 1 + kDpK - 1) / kDpK;
         for (int b = 0; b < nbands; ++b)
-            hipLaunchKernelGGL(k_cmp_dp_band, dim3((Nmax + kDpS - 1) / kDpS), dim3(kDpT), 0, s, (const float*)cost,
+            hipLaunchKernelGGL(k_cmp_dp_band, dim3((Nmax + kDpS - 1) / kDpS), dim3(64), 0, s, (const float*)cost,
                                (const uint32_t*)rowf, (const uint32_t*)colf, cw, ch, (const CmpStats*)st,
                                (const int32_t*)lines, (const int*)nlp, b, (const double*)((b & 1) ? dpb : dpa),
                                (b & 1) ? dpa : dpb, bt);
@@ -876,7 +1069,7 @@ int composite(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h2, 
                        (const uint8_t*)m1, (const uint8_t*)m2, cw, ch, w25, (const CmpStats*)st, lvl0 + 6 * P);
     for (int k = 1; k < n_levels; ++k) {
         const int Hk = Pl.hs[k - 1], Wk = Pl.ws[k - 1], Ho = Pl.hs[k], Wo = Pl.ws[k];
-        hipLaunchKernelGGL(k_pyr_down_rows, dim3((Wk + kCT - 1) / kCT, Ho, 7), dim3(kCT), 0, s,
+        hipLaunchKernelGGL(k_pyr_down_rows, dim3((Wk + kCT - 1) / kCT, (Ho + kPdR - 1) / kPdR, 7), dim3(kCT), 0, s,
                            (const float*)(ws + Pl.o_lvl[k - 1]), Hk, Wk, Ho, w9, tmp);
         hipLaunchKernelGGL(k_pyr_down_cols, dim3((Wo + kCT - 1) / kCT, Ho, 7), dim3(kCT), 0, s, (const float*)tmp, Wk,
                            Ho, Wo, w9, (float*)(ws + Pl.o_lvl[k]));
@@ -885,7 +1078,7 @@ int composite(const uint8_t* img1, int h1, int w1, const uint8_t* img2, int h2, 
     hipLaunchKernelGGL(k_pyr_top, dim3((Pl.ws[kL] + kCT - 1) / kCT, Pl.hs[kL], 3), dim3(kCT), 0, s,
                        (const float*)(ws + Pl.o_lvl[kL]), Pl.hs[kL], Pl.ws[kL], (float*)(ws + Pl.o_res[kL]));
     for (int k = n_levels - 2; k >= 0; --k)
-        hipLaunchKernelGGL(k_pyr_up_blend, dim3((Pl.ws[k] + kCT - 1) / kCT, Pl.hs[k], 3), dim3(kCT), 0, s,
+        hipLaunchKernelGGL(k_pyr_up_blend, dim3((Pl.ws[k] + kCT - 1) / kCT, Pl.hs[k]), dim3(kCT), 0, s,
                            (const float*)(ws + Pl.o_lvl[k]), Pl.hs[k], Pl.ws[k], (const float*)(ws + Pl.o_lvl[k + 1]),
                            Pl.hs[k + 1], Pl.ws[k + 1], (const float*)(ws + Pl.o_res[k + 1]),
                            (float*)(ws + Pl.o_res[k]));

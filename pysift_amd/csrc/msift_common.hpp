@@ -295,4 +295,16 @@ __device__ __forceinline__ unsigned long long lanemask_lt() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() is a
+// release/acquire fence on every address space, so it also waits for all of the
+// wave's outstanding global loads and stores (vmcnt(0)) before the barrier; a
+// loop that keeps global loads in flight across barriers (prefetch) or stores
+// results it never re-reads uses this instead.  Global memory written before it
+// is NOT visible to the other waves after it.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 }  // namespace msift

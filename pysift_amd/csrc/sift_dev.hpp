@@ -1132,8 +1132,10 @@ __device__ __forceinline__ void describe_keypoint_block(const float* __restrict_
                                                         const KpTables& tab, bool q16,
                                                         unsigned long long* hist, float* v0o, float* v1o) {
     const int tid = threadIdx.x;
+    // the block's barriers order LDS only: the previous keypoint's descriptor
+    // stores need not drain before the next keypoint's histogram starts
     for (int i = tid; i < kDescCopies * 128; i += kDescT) hist[i] = 0ull;
-    __syncthreads();
+    lds_barrier();
     unsigned long long* hw = hist + (tid >> 6) * 128;
     const int nd = tab.n_dsp > 0 ? tab.n_dsp : 1;
     for (int d = 0; d < nd; ++d) {
@@ -1145,7 +1147,7 @@ __device__ __forceinline__ void describe_keypoint_block(const float* __restrict_
         if (q16) desc_hist_scale<true>(G, H, W, ri, ci, angle_deg, phd, hw, tid);
         else desc_hist_scale<false>(G, H, W, ri, ci, angle_deg, phd, hw, tid);
     }
-    __syncthreads();
+    lds_barrier();
     float v0 = 0.0f, v1 = 0.0f;
     if (tid < 64) {
         unsigned long long s0 = 0, s1 = 0;
@@ -1164,7 +1166,7 @@ __device__ __forceinline__ void describe_keypoint_block(const float* __restrict_
     }
     *v0o = v0;
     *v1o = v1;
-    __syncthreads();                       // hist is reused by the caller's next keypoint
+    lds_barrier();                         // hist is reused by the caller's next keypoint
 }
 
 __device__ __forceinline__ void store_descriptor(float v0, float v1, int lane, float* out_f, uint8_t* out_u8) {
