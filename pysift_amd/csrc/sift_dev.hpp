@@ -392,8 +392,12 @@ __device__ __forceinline__ void blur_tile(const float* __restrict__ in, float* _
                 if (lx >= IW) { lx -= IW; off += W - IW; }
             }
         } else {
-            // one reflection suffices unless the image is narrower than the halo
-            const bool small = (H <= R) || (W <= R);
+            // one reflection suffices unless the image is narrower than the halo or
+            // the tile reaches past 2H (2W), as a tile taller than what is left of
+            // the level can (the last 8-row tile of a 13-row level at R = 12 would
+            // otherwise read row -2); the periodic form gives the same index wherever
+            // one reflection suffices, so stored outputs are unchanged
+            const bool small = (H <= R) || (W <= R) || (y0 + TH + R > 2 * H) || (x0 + TW + R > 2 * W);
             int ly = ly0, lx = lx0;
 #pragma unroll
             for (int j = 0; j < NL; ++j) {

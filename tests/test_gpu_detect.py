@@ -36,14 +36,17 @@ def _pyramid_from_ws(s, H, W, o):
     return gs.cpu().numpy(), ds.cpu().numpy()
 
 
-@pytest.mark.parametrize("H,W,dbl", [(97, 131, True), (240, 320, True), (61, 77, False)])
-def test_pyramid_bitwise(H, W, dbl):
-    print(f"[case] pyramid {H}x{W} doubled={dbl}", flush=True)
+# (104, 150) doubled, 5 octaves: the last octave is 13 x 19, narrower and
+# shorter than an R = 12 tile's halo reaches past 2W / 2H (periodic reflection)
+@pytest.mark.parametrize("H,W,dbl,n_oct", [(97, 131, True, 4), (240, 320, True, 4), (61, 77, False, 4),
+                                           (104, 150, True, 5)])
+def test_pyramid_bitwise(H, W, dbl, n_oct):
+    print(f"[case] pyramid {H}x{W} doubled={dbl} octaves={n_oct}", flush=True)
     img = syn.synth_gray(H, W, 11)
-    s = _sift(double_image=dbl, n_octaves=4)
+    s = _sift(double_image=dbl, n_octaves=n_oct)
     s.detect_raw(img)
     p = _oracle_params(s)
-    for o in range(4):
+    for o in range(n_oct):
         g_ref, d_ref = O.pyramid_octave(img, p, o)
         g, d = _pyramid_from_ws(s, H, W, o)
         assert g.shape == g_ref.shape
