@@ -154,7 +154,8 @@ int msift_match_u8(const uint8_t* d_a, int N, const uint8_t* d_b, int M, double 
 
 /* Tile shape of method 2 for the whole process (no reference counterpart;
  * the reference has no tile choice): shape 0 = 64 a-rows per wave with the
- * next tile's MFMAs issued ahead, 1 = 64 rows, 2 = 32 rows; chunks_per_block
+ * next tile's MFMAs issued ahead, 1 = 64 rows, 2 = 32 rows, 3 = 96 rows,
+ * 4 = 128 rows (each staged y fragment feeds more MFMAs); chunks_per_block
  * = 512-column chunks per workgroup, 1..64.  Outputs do not depend on it; the
  * autotuner (pysift_amd.matcher.autotune) sets it.  Default (0, 4). */
 int msift_match_set_tile(int shape, int chunks_per_block);

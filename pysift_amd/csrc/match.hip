@@ -531,20 +531,22 @@ static std::atomic<int> g_tp_shape{0}, g_tp_cpw{4};
 
 static void tp_launch(const TpPass& p0, const TpPass& p1, hipStream_t s) {
     const int shape = g_tp_shape.load(std::memory_order_relaxed), cpw = g_tp_cpw.load(std::memory_order_relaxed);
-    const int xblk = kTpWaves * 32 * (shape == 2 ? 1 : 2);
+    static const int nt_of[5] = {2, 2, 1, 3, 4};
+    const int xblk = kTpWaves * 32 * nt_of[shape];
     auto groups = [&](int ny) { return ((ny + 511) / 512 + cpw - 1) / cpw; };
     const dim3 grid((std::max(p0.NX, p1.NX) + xblk - 1) / xblk, std::max(groups(p0.NY), groups(p1.NY)), 2);
-    if (shape == 0)
-        hipLaunchKernelGGL((k_match_side<2, true>), grid, dim3(256), 0, s, p0, p1, cpw);
-    else if (shape == 1)
-        hipLaunchKernelGGL((k_match_side<2, false>), grid, dim3(256), 0, s, p0, p1, cpw);
-    else
-        hipLaunchKernelGGL((k_match_side<1, false>), grid, dim3(256), 0, s, p0, p1, cpw);
+    switch (shape) {
+    case 0: hipLaunchKernelGGL((k_match_side<2, true>), grid, dim3(256), 0, s, p0, p1, cpw); break;
+    case 1: hipLaunchKernelGGL((k_match_side<2, false>), grid, dim3(256), 0, s, p0, p1, cpw); break;
+    case 2: hipLaunchKernelGGL((k_match_side<1, false>), grid, dim3(256), 0, s, p0, p1, cpw); break;
+    case 3: hipLaunchKernelGGL((k_match_side<3, false>), grid, dim3(256), 0, s, p0, p1, cpw); break;
+    default: hipLaunchKernelGGL((k_match_side<4, false>), grid, dim3(256), 0, s, p0, p1, cpw); break;
+    }
 }
 
 int match_set_tile(int shape, int cpw) {
-    if (shape < 0 || shape > 2 || cpw < 1 || cpw > 64) {
-        set_error("match_set_tile: shape must be 0, 1 or 2 and chunks per workgroup in [1, 64]");
+    if (shape < 0 || shape > 4 || cpw < 1 || cpw > 64) {
+        set_error("match_set_tile: shape must be in [0, 4] and chunks per workgroup in [1, 64]");
         return MSIFT_EINVAL;
     }
     g_tp_shape.store(shape);

@@ -12,8 +12,9 @@ from . import _lib
 METHODS = {"hilo": 0, "shift": 1, "twopass": 2}
 DEFAULT_METHOD = "twopass"
 # two-pass tile shapes (0: 64 rows a wave, next tile's MFMAs issued ahead;
-# 1: 64 rows; 2: 32 rows) and 512-column chunks per workgroup tried by autotune
-TILE_SHAPES = (0, 1, 2)
+# 1: 64 rows; 2: 32 rows; 3: 96 rows; 4: 128 rows) and 512-column chunks per
+# workgroup tried by autotune
+TILE_SHAPES = (0, 1, 2, 3, 4)
 TILE_CHUNKS = (1, 2, 4, 8)
 
 
@@ -69,7 +70,7 @@ def autotune(N, M, device="cuda", iters=3):
     values, so random descriptors of the requested size are timed."""
     import torch
     from .batch import _load_tune, _save_tune
-    key = f"match:{_bucket(N)}:{_bucket(M)}"
+    key = f"match5:{_bucket(N)}:{_bucket(M)}"
     cache = _load_tune()
     if key in cache:
         shape, cpw = cache[key]

@@ -75,11 +75,11 @@ def test_stitch_int8_matching_mapping():
 
 
 def test_match_tile_setter_and_method_validation():
-    # the two-pass tile setter accepts shapes 0-2 and 1-64 chunks; the matcher
+    # the two-pass tile setter accepts shapes 0-4 and 1-64 chunks; the matcher
     # rejects unknown methods before touching the device
     from pysift_amd import _lib
     L = _lib.lib()
-    for shape, cpw in [(3, 4), (-1, 4), (0, 0), (0, 65)]:
+    for shape, cpw in [(5, 4), (-1, 4), (0, 0), (0, 65)]:
         assert L.msift_match_set_tile(shape, cpw) == _lib.MSIFT_EINVAL
         assert "chunks per workgroup" in _lib.last_error()
     assert L.msift_match_set_tile(2, 8) == 0

@@ -73,6 +73,29 @@ def test_twopass_equals_shift_large(N, M):
     assert p1.shape[0] > k // 2
 
 
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
+def test_twopass_every_tile_shape(shape):
+    # every tile shape the autotuner may pick (32-128 x per wave) and chunk
+    # counts that leave partial chunk groups give the oracle's result
+    from pysift_amd.matcher import set_tile
+    N, M = 1500, 1300
+    a = _rand_desc(N, 77)
+    b = _rand_desc(M, 78)
+    k = min(N, M) // 2
+    b[:k] = np.clip(a[:k].astype(int) + np.random.default_rng(2).integers(-3, 4, (k, 128)), 0, 255).astype(np.uint8)
+    rf, rb, rp = O.match_u8(a, b, 0.75)
+    try:
+        for cpw in (1, 2, 3):
+            print(f"[case] twopass shape {shape} chunks {cpw} {N}x{M}", flush=True)
+            set_tile(shape, cpw)
+            f, bw, p = _run(a, b, "twopass")
+            np.testing.assert_array_equal(f, rf)
+            np.testing.assert_array_equal(bw, rb)
+            np.testing.assert_array_equal(p, rp)
+    finally:
+        set_tile(0, 4)
+
+
 def test_match_empty():
     a = _rand_desc(0, 1); b = _rand_desc(10, 2)
     f, bw, p = _run(a, b, "hilo")
