@@ -586,7 +586,7 @@ void release_side_stream(msift_handle* h) {
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(h->side_dev);
     (void)hipStreamDestroy(h->side);
-    for (int i = 0; i <= kMaxOct; ++i) (void)hipEventDestroy(h->fork[i]);
+    for (int i = 0; i <= kMaxOct + 1; ++i) (void)hipEventDestroy(h->fork[i]);
     (void)hipSetDevice(cur);
     h->side_init = false;
 }
@@ -601,7 +601,7 @@ int side_stream(msift_handle* h, hipStream_t s, hipStream_t* out) {
     MSIFT_HIP(hipSetDevice(dev));
     hipError_t e = hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking);
     int ne = 0;
-    for (; e == hipSuccess && ne <= kMaxOct; ++ne) e = hipEventCreateWithFlags(&h->fork[ne], hipEventDisableTiming);
+    for (; e == hipSuccess && ne <= kMaxOct + 1; ++ne) e = hipEventCreateWithFlags(&h->fork[ne], hipEventDisableTiming);
     (void)hipSetDevice(cur);
     if (e != hipSuccess) {
         set_error(std::string("side stream: ") + hipGetErrorString(e));
@@ -638,17 +638,27 @@ int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch
     }
     if (prof) MSIFT_HIP(hipEventRecord(h->ev[0], s));
 
-    // A1: adaptive threshold
-    hipLaunchKernelGGL(k_zero_i32, dim3(2), dim3(256), 0, s, (int*)(ws + P.o_hist), 256 + 16);
+    int rc;
+    // the second stream of the handle: the threshold (needed first by the
+    // extrema) and the pyramid levels off the octave-to-octave chain run there
+    hipStream_t side = nullptr;
+    if (P.n_oct > 1 && (rc = side_stream(h, s, &side))) return rc;
+    // A1: adaptive threshold, off the pyramid's dependent chain
+    hipStream_t ts = s;
+    if (side) {
+        MSIFT_HIP(hipEventRecord(h->fork[kMaxOct + 1], s));
+        MSIFT_HIP(hipStreamWaitEvent(side, h->fork[kMaxOct + 1], 0));
+        ts = side;
+    }
+    hipLaunchKernelGGL(k_zero_i32, dim3(2), dim3(256), 0, ts, (int*)(ws + P.o_hist), 256 + 16);
     {
-        hipLaunchKernelGGL(k_hist256, dim3(grid_cap(H, 128)), dim3(256), 0, s, img, H, W, pitch,
+        hipLaunchKernelGGL(k_hist256, dim3(grid_cap(H, 128)), dim3(256), 0, ts, img, H, W, pitch,
                            (int*)(ws + P.o_hist));
-        hipLaunchKernelGGL(k_thresh, dim3(1), dim3(64), 0, s, (const int*)(ws + P.o_hist), (double)H * (double)W,
+        hipLaunchKernelGGL(k_thresh, dim3(1), dim3(64), 0, ts, (const int*)(ws + P.o_hist), (double)H * (double)W,
                            p.contrast_thresh, S, p.adaptive_thresh, thr);
     }
     // A3: base image
     float* G0 = oi.G[0];
-    int rc;
     if (P.doubled) {
         double* pf0 = (double*)(ws + P.o_pf0);
         double* pf1 = (double*)(ws + P.o_pf1);
@@ -678,8 +688,6 @@ int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch
     // the side stream (after a fork event recorded once G[S] exists) while the
     // launch stream goes on to octave o+1; one join event brings them back
     // before the extrema.
-    hipStream_t side = nullptr;
-    if (P.n_oct > 1 && (rc = side_stream(h, s, &side))) return rc;
     for (int o = 0; o < P.n_oct; ++o) {
         int Ho = P.Ho[o], Wo = P.Wo[o];
         size_t n = (size_t)Ho * Wo;

@@ -29,7 +29,7 @@ struct msift_handle {
     // second stream for the pyramid levels off the octave-to-octave chain,
     // with its fork/join events; created on first use on a device
     hipStream_t side;
-    hipEvent_t fork[msift::kMaxOct + 1];
+    hipEvent_t fork[msift::kMaxOct + 2];   // per octave, the join, the threshold's fork
     int side_dev;
     bool side_init;
 };
