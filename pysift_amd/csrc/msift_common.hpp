@@ -212,6 +212,54 @@ MSIFT_HD float atan2_f(float y, float x) {
     return ys ? -a : a;
 }
 
+// The same operations as exp_f and atan2_f without their special-case
+// branches, for the per-pixel loops of detection, whose inputs come from the
+// level values of a u8 image (finite): exp_f_le88 for x <= 88 that is not
+// NaN (the orientation and descriptor weights take x <= 0), where x < -87
+// becomes a select of 0; atan2_f_finite for finite y, x, where the zero
+// cases become selects.  Identical bits to the general forms on those
+// inputs; the branch-free form keeps a wave's pixels in one basic block.
+__device__ __forceinline__ float exp_f_le88(float x) {
+    const float n = floorf(x * 0x1.715476p+0f + 0.5f);
+    float r = fmaf(n, -0x1.62e4p-1f, x);
+    r = fmaf(n, -0x1.7f7d1cp-20f, r);
+    float p = 0x1.a01a02p-13f;
+    p = fmaf(p, r, 0x1.6c16c2p-10f);
+    p = fmaf(p, r, 0x1.111112p-7f);
+    p = fmaf(p, r, 0x1.555556p-5f);
+    p = fmaf(p, r, 0x1.555556p-3f);
+    p = fmaf(p, r, 0.5f);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
+    const float v = p * __builtin_bit_cast(float, (uint32_t)((int)n + 127) << 23);
+    return x < -87.0f ? 0.0f : v;
+}
+
+__device__ __forceinline__ float atan2_f_finite(float y, float x) {
+    const float PI = 0x1.921fb6p+1f, PIO2 = 0x1.921fb6p+0f;
+    const bool ys = signbit(y), xs = signbit(x);
+    const float ax = fabsf(x), ay = fabsf(y);
+    const bool swap = ay > ax;
+    const float t = (swap ? ax : ay) / (swap ? ay : ax);   // 0 / 0 only where a select below applies
+    const float s2 = t * t;
+    float p = -0x1.cbd234p-10f;
+    p = fmaf(p, s2, 0x1.5f85d2p-7f);
+    p = fmaf(p, s2, -0x1.f8b94ep-6f);
+    p = fmaf(p, s2, 0x1.d77a5cp-5f);
+    p = fmaf(p, s2, -0x1.572366p-4f);
+    p = fmaf(p, s2, 0x1.c02fd6p-4f);
+    p = fmaf(p, s2, -0x1.2415a4p-3f);
+    p = fmaf(p, s2, 0x1.999084p-3f);
+    p = fmaf(p, s2, -0x1.555534p-2f);
+    p = fmaf(p, s2, 1.0f);
+    float a = t * p;
+    a = swap ? PIO2 - a : a;
+    a = xs ? PI - a : a;
+    a = ax == 0.0f ? PIO2 : a;                    // y != 0, x == 0
+    a = ay == 0.0f ? (xs ? PI : 0.0f) : a;        // y == 0 (takes precedence, as in atan2_f)
+    return ys ? -a : a;
+}
+
 // C fmodf(x, 360) for |x| < 1440 (every subtraction is exact): the same
 // subtractions as a loop while |x| >= 360, unrolled to three selects
 // x rounded to binary16 (nearest, ties to even) and back: the value a

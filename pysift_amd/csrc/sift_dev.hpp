@@ -794,9 +794,9 @@ __device__ __forceinline__ int orient_keypoint(const float* __restrict__ G, int 
             float dx = gr[k] - gl[k];
             float dy = gd[k] - gu[k];
             float mag = sqrtf(dx * dx + dy * dy);
-            float ori = fmod360(atan2_f(dy, dx) * 57.29577951f + 360.0f);
+            float ori = fmod360(atan2_f_finite(dy, dx) * 57.29577951f + 360.0f);
             float dr = (float)r - ri, dc = (float)c - ci;
-            float wt = exp_f(-(dr * dr + dc * dc) * inv_2sw2);
+            float wt = exp_f_le88(-(dr * dr + dc * dc) * inv_2sw2);
             float bin_f = ori * 0.1f;
             int bin_i = ((int)bin_f) % 36;
             int bin_r = (bin_i + 1) % 36;
@@ -926,8 +926,8 @@ __device__ __forceinline__ void desc_hist(const float* __restrict__ G, int H, in
             float dx = gr[k] - gl[k];
             float dy = gd[k] - gu[k];
             float mag = sqrtf(dx * dx + dy * dy);
-            float ori_deg = fmod360((atan2_f(dy, dx) - ang_rad) * 57.29577951f + 360.0f);
-            float gw = exp_f(-(xr * xr + yr * yr) * inv_2s2);
+            float ori_deg = fmod360((atan2_f_finite(dy, dx) - ang_rad) * 57.29577951f + 360.0f);
+            float gw = exp_f_le88(-(xr * xr + yr * yr) * inv_2s2);
             float wmag = okk[k] ? mag * gw : 0.0f;
             float ob = ori_deg * 0.02222222f;
             int oi0 = (int)floorf(ob);
@@ -1081,8 +1081,8 @@ __device__ __forceinline__ void desc_hist_fx(const float* __restrict__ G, int H,
             const float dx = gr[k] - gl[k];
             const float dy = gd[k] - gu[k];
             const float mag = sqrtf(dx * dx + dy * dy);
-            const float ori_deg = fmod360((atan2_f(dy, dx) - ang_rad) * 57.29577951f + 360.0f);
-            const float gw = exp_f(-(xr * xr + yr * yr) * inv_2s2);
+            const float ori_deg = fmod360((atan2_f_finite(dy, dx) - ang_rad) * 57.29577951f + 360.0f);
+            const float gw = exp_f_le88(-(xr * xr + yr * yr) * inv_2s2);
             // a pixel outside the patch / window adds 0 to every target
             const float wms = fminf((okk[k] ? mag * gw : 0.0f) * kDescFx, kDescFxMax);
             const float ob = ori_deg * 0.02222222f;
