@@ -26,7 +26,7 @@ REF_MS_PER_MP = 72.8 / (768 * 1024 / 1e6)   # reference RTX 3050, 768x1024
 
 # HBM bytes the pyramid kernels move per 1024x768 view (FETCH_SIZE + WRITE_SIZE
 # counter runs, scaled by a 256 MiB copy kernel): profiles/pyramid_1024x768_traffic.md
-PYRAMID_TRAFFIC_BYTES = 329_371_984
+PYRAMID_TRAFFIC_BYTES = 329_431_580
 
 
 def octave_pixels(H, W, n_oct, doubled):
