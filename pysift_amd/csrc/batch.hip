@@ -474,7 +474,7 @@ __global__ __launch_bounds__(kDescT) void k_bdescriptor(const BImg* __restrict__
                                                      float* __restrict__ out_kp, float* __restrict__ out_f,
                                                      uint8_t* __restrict__ out_u8, int* __restrict__ counts,
                                                      int* __restrict__ offsets) {
-    __shared__ unsigned long long hist[kDescCopies * 128];
+    __shared__ unsigned long long hist[kDescHist];
     const int lane = threadIdx.x;
     const int m = aoff[(size_t)n * maxkp];
     if (blockIdx.x == 0)

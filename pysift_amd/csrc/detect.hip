@@ -394,7 +394,7 @@ __global__ __launch_bounds__(kDescT) void k_descriptor(OctInfo oi, const Kp* __r
                                                     double cs, int rootsift, float* __restrict__ out_kp,
                                                     float* __restrict__ out_f, uint8_t* __restrict__ out_u8,
                                                     int* __restrict__ count) {
-    __shared__ unsigned long long hist[kDescCopies * 128];
+    __shared__ unsigned long long hist[kDescHist];
     const int m = *mp;
     const int lane = threadIdx.x;
     if (blockIdx.x == 0 && lane == 0) *count = m;

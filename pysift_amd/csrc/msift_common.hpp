@@ -219,6 +219,25 @@ MSIFT_HD float atan2_f(float y, float x) {
 // becomes a select of 0; atan2_f_finite for finite y, x, where the zero
 // cases become selects.  Identical bits to the general forms on those
 // inputs; the branch-free form keeps a wave's pixels in one basic block.
+// exp_f for -87 <= x <= 88, where exp_f takes neither special case: the
+// descriptor's Gaussian weight, whose argument -(xr^2 + yr^2) / (2 sd^2)
+// stays above -3.2 over the patch (|xr|, |yr| <= ph sqrt 2, sd = 0.5657 ph)
+__device__ __forceinline__ float exp_f_mid(float x) {
+    const float n = floorf(x * 0x1.715476p+0f + 0.5f);
+    float r = fmaf(n, -0x1.62e4p-1f, x);
+    r = fmaf(n, -0x1.7f7d1cp-20f, r);
+    float p = 0x1.a01a02p-13f;
+    p = fmaf(p, r, 0x1.6c16c2p-10f);
+    p = fmaf(p, r, 0x1.111112p-7f);
+    p = fmaf(p, r, 0x1.555556p-5f);
+    p = fmaf(p, r, 0x1.555556p-3f);
+    p = fmaf(p, r, 0.5f);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
+    // p 2^n by the exponent add, the same value as the multiply by 2^n (normal here)
+    return __builtin_ldexpf(p, (int)n);
+}
+
 __device__ __forceinline__ float exp_f_le88(float x) {
     const float n = floorf(x * 0x1.715476p+0f + 0.5f);
     float r = fmaf(n, -0x1.62e4p-1f, x);
@@ -233,6 +252,13 @@ __device__ __forceinline__ float exp_f_le88(float x) {
     p = fmaf(p, r, 1.0f);
     const float v = p * __builtin_bit_cast(float, (uint32_t)((int)n + 127) << 23);
     return x < -87.0f ? 0.0f : v;
+}
+
+// fmod360 for |x| < 720: at most one exact subtraction
+__device__ __forceinline__ float fmod360_lt720(float x) {
+    float a = fabsf(x);
+    a = a >= 360.0f ? a - 360.0f : a;
+    return copysignf(a, x);
 }
 
 __device__ __forceinline__ float atan2_f_finite(float y, float x) {
@@ -255,7 +281,8 @@ __device__ __forceinline__ float atan2_f_finite(float y, float x) {
     float a = t * p;
     a = swap ? PIO2 - a : a;
     a = xs ? PI - a : a;
-    a = ax == 0.0f ? PIO2 : a;                    // y != 0, x == 0
+    // y != 0, x == 0 needs no select: t = 0 / |y| = 0, a = PIO2 - 0, and
+    // PI - PIO2 == PIO2 in float for x == -0
     a = ay == 0.0f ? (xs ? PI : 0.0f) : a;        // y == 0 (takes precedence, as in atan2_f)
     return ys ? -a : a;
 }

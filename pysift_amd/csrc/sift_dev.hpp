@@ -794,7 +794,8 @@ __device__ __forceinline__ int orient_keypoint(const float* __restrict__ G, int 
             float dx = gr[k] - gl[k];
             float dy = gd[k] - gu[k];
             float mag = sqrtf(dx * dx + dy * dy);
-            float ori = fmod360(atan2_f_finite(dy, dx) * 57.29577951f + 360.0f);
+            // atan2 in [-pi, pi]: the argument lies in [179.99, 540.01]
+            float ori = fmod360_lt720(atan2_f_finite(dy, dx) * 57.29577951f + 360.0f);
             float dr = (float)r - ri, dc = (float)c - ci;
             float wt = exp_f_le88(-(dr * dr + dc * dc) * inv_2sw2);
             float bin_f = ori * 0.1f;
@@ -1021,6 +1022,17 @@ constexpr int kDescT = 256;            // threads per keypoint
 constexpr int kDescCopies = kDescT / 64;   // one histogram copy per wave
 constexpr float kDescFx = 0x1p30f;
 constexpr float kDescFxMax = 0x1.fffffep31f;
+// A wave's copy holds padded cells (y, x) for y, x in -1..4 at ((y + 1) 6 +
+// (x + 1)) 9, each with orientation slots 0..8, slot 8 standing for bin 0
+// reached as bin 7 + 1.  A pixel's 8 trilinear targets then sit at fixed
+// offsets 0, 1, 9, 10, 54, 55, 63, 64 from one address.  The reference clamps
+// the spatial targets to 0..3 and wraps the orientation; here the clamped and
+// wrapped slots are added into their bins after the pixel loop, which gives
+// the same integer sums.
+constexpr int kDescCell = 9;
+constexpr int kDescRow = 6 * kDescCell;
+constexpr int kDescSlots = 6 * kDescRow;   // 324 per copy
+constexpr int kDescHist = kDescCopies * kDescSlots;
 
 // Pixel p of the side x side patch is visited as p = (f K) mod side^2 for
 // f = tid + kDescT i, K = 97 (101 when 97 divides side): a bijection that
@@ -1046,6 +1058,10 @@ __device__ __forceinline__ void desc_hist_fx(const float* __restrict__ G, int H,
     const int srow = step / side, scol = step - srow * side;
     const int p0 = (tid * K) % total_px;
     int rq = p0 / side, cq = p0 - rq * side;
+    // the level as a buffer of H W floats (reads past it return 0)
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(G), (short)0, H * W * 4, 0x00020000);
+    const int w4 = W * 4, w8 = W * 8;
     for (int f0 = tid; f0 < total_px; f0 += kDescT * kPf) {
         float gl[kPf], gr[kPf], gu[kPf], gd[kPf], xrs[kPf], yrs[kPf];
         bool okk[kPf];
@@ -1066,12 +1082,25 @@ __device__ __forceinline__ void desc_hist_fx(const float* __restrict__ G, int H,
             if (!kInterior) ok = ok && !(rr < 1 || rr >= H - 1 || cc < 1 || cc >= W - 1);
             okk[k] = ok;
             xrs[k] = xr; yrs[k] = yr;
-            gl[k] = gr[k] = gu[k] = gd[k] = 0.0f;
-            if (ok) {
-                const float* g = G + (size_t)rr * W + cc;
-                gl[k] = g[-1]; gr[k] = g[1]; gu[k] = g[-W]; gd[k] = g[W];
-                if (Q16) { gl[k] = f16_round(gl[k]); gr[k] = f16_round(gr[k]); gu[k] = f16_round(gu[k]); gd[k] = f16_round(gd[k]); }
+            if (kInterior) {
+                // every (rr, cc) of the patch has its 4 neighbours in the level:
+                // the taps are read for all pixels, the weight drops the others.
+                // One byte offset per pixel, (rr - 1, cc - 1), and the four taps at
+                // fixed offsets from it (W 4 and W 8 in scalar registers).  Row
+                // and width stay below 2^23, so the 24-bit multiply is exact.
+                const int at = (__mul24(rr - 1, W) + cc - 1) * 4;
+                gu[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, at + 4, 0, 0));
+                gl[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, at, w4, 0));
+                gr[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, at + 8, w4, 0));
+                gd[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, at + 4, w8, 0));
+            } else {
+                gl[k] = gr[k] = gu[k] = gd[k] = 0.0f;
+                if (ok) {
+                    const float* g = G + (size_t)rr * W + cc;
+                    gl[k] = g[-1]; gr[k] = g[1]; gu[k] = g[-W]; gd[k] = g[W];
+                }
             }
+            if (Q16) { gl[k] = f16_round(gl[k]); gr[k] = f16_round(gr[k]); gu[k] = f16_round(gu[k]); gd[k] = f16_round(gd[k]); }
         }
 #pragma unroll
         for (int k = 0; k < kPf; ++k) {
@@ -1081,51 +1110,74 @@ __device__ __forceinline__ void desc_hist_fx(const float* __restrict__ G, int H,
             const float dx = gr[k] - gl[k];
             const float dy = gd[k] - gu[k];
             const float mag = sqrtf(dx * dx + dy * dy);
-            const float ori_deg = fmod360((atan2_f_finite(dy, dx) - ang_rad) * 57.29577951f + 360.0f);
-            const float gw = exp_f_le88(-(xr * xr + yr * yr) * inv_2s2);
-            // a pixel outside the patch / window adds 0 to every target
-            const float wms = fminf((okk[k] ? mag * gw : 0.0f) * kDescFx, kDescFxMax);
+            // atan2 in [-pi, pi], ang_rad in [0, 2 pi): the argument lies in (-180.01, 540.01]
+            const float ori_deg = fmod360_lt720((atan2_f_finite(dy, dx) - ang_rad) * 57.29577951f + 360.0f);
+            const float gw = exp_f_mid(-(xr * xr + yr * yr) * inv_2s2);
+            const float wms = fminf(mag * gw * kDescFx, kDescFxMax);
             const float ob = ori_deg * 0.02222222f;
-            const int oi0 = (int)floorf(ob);
-            const float df = ob - (float)oi0;
-            const int xi0 = (int)floorf(xb), yi0 = (int)floorf(yb);
-            const float dxb = xb - (float)xi0, dyb = yb - (float)yi0;
-            const int xi0c = xi0 < 0 ? 0 : (xi0 > 3 ? 3 : xi0);
-            const int yi0c = yi0 < 0 ? 0 : (yi0 > 3 ? 3 : yi0);
-            const int xi1c = xi0 + 1 < 0 ? 0 : (xi0 + 1 > 3 ? 3 : xi0 + 1);
-            const int yi1c = yi0 + 1 < 0 ? 0 : (yi0 + 1 > 3 ? 3 : yi0 + 1);
-            // & 7 is the residue mod 8 in [0, 8), for negative oi0 too
-            const int oi0w = oi0 & 7;
-            const int oi1w = (oi0 + 1) & 7;
+            // (float)(int)floorf(v) == floorf(v) for the |v| < 2^24 here
+            const float fo = floorf(ob), fx = floorf(xb), fy = floorf(yb);
+            const int oi0 = (int)fo;
+            const float df = ob - fo;
+            const int xi0 = (int)fx, yi0 = (int)fy;
+            const float dxb = xb - fx, dyb = yb - fy;
+            // inside the window xb, yb lie in [-0.5, 3.5), so xi0, yi0 lie in
+            // [-1, 3].  A pixel outside it adds nothing (its lanes skip the
+            // atomics: sent to one slot they would serialise on it).  & 7 is
+            // the residue mod 8 in [0, 8), for negative oi0 too.
+            if (!okk[k]) continue;
+            const int slot = __mul24(yi0, kDescRow) + __mul24(xi0, kDescCell) + (oi0 & 7) + kDescRow + kDescCell;
             const float wy0 = 1.0f - dyb, wy1 = dyb, wx0 = 1.0f - dxb, wx1 = dxb, wo0 = 1.0f - df, wo1 = df;
-            unsigned long long* h00 = hw + yi0c * 32 + xi0c * 8;
-            unsigned long long* h01 = hw + yi0c * 32 + xi1c * 8;
-            unsigned long long* h10 = hw + yi1c * 32 + xi0c * 8;
-            unsigned long long* h11 = hw + yi1c * 32 + xi1c * 8;
-            atomicAdd(h00 + oi0w, (unsigned long long)(unsigned)(wy0 * wx0 * wo0 * wms));
-            atomicAdd(h00 + oi1w, (unsigned long long)(unsigned)(wy0 * wx0 * wo1 * wms));
-            atomicAdd(h01 + oi0w, (unsigned long long)(unsigned)(wy0 * wx1 * wo0 * wms));
-            atomicAdd(h01 + oi1w, (unsigned long long)(unsigned)(wy0 * wx1 * wo1 * wms));
-            atomicAdd(h10 + oi0w, (unsigned long long)(unsigned)(wy1 * wx0 * wo0 * wms));
-            atomicAdd(h10 + oi1w, (unsigned long long)(unsigned)(wy1 * wx0 * wo1 * wms));
-            atomicAdd(h11 + oi0w, (unsigned long long)(unsigned)(wy1 * wx1 * wo0 * wms));
-            atomicAdd(h11 + oi1w, (unsigned long long)(unsigned)(wy1 * wx1 * wo1 * wms));
+            unsigned long long* h = hw + slot;
+            atomicAdd(h, (unsigned long long)(unsigned)(wy0 * wx0 * wo0 * wms));
+            atomicAdd(h + 1, (unsigned long long)(unsigned)(wy0 * wx0 * wo1 * wms));
+            atomicAdd(h + kDescCell, (unsigned long long)(unsigned)(wy0 * wx1 * wo0 * wms));
+            atomicAdd(h + kDescCell + 1, (unsigned long long)(unsigned)(wy0 * wx1 * wo1 * wms));
+            atomicAdd(h + kDescRow, (unsigned long long)(unsigned)(wy1 * wx0 * wo0 * wms));
+            atomicAdd(h + kDescRow + 1, (unsigned long long)(unsigned)(wy1 * wx0 * wo1 * wms));
+            atomicAdd(h + kDescRow + kDescCell, (unsigned long long)(unsigned)(wy1 * wx1 * wo0 * wms));
+            atomicAdd(h + kDescRow + kDescCell + 1, (unsigned long long)(unsigned)(wy1 * wx1 * wo1 * wms));
         }
     }
+}
+
+// bin b = 32 y + 8 x + o of the reference's histogram from the padded copy:
+// the padded cells that its clamp sends to (y, x) (row -1 to 0, row 4 to 3,
+// likewise for columns), and slot 8 into orientation 0
+__device__ __forceinline__ unsigned long long desc_bin_sum(const unsigned long long* h, int b) {
+    const int y = b >> 5, x = (b >> 3) & 3, o = b & 7;
+    const int ye = y == 0 ? 0 : (y == 3 ? 5 : -1);
+    const int xe = x == 0 ? 0 : (x == 3 ? 5 : -1);
+    unsigned long long s = 0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int ys = i ? ye : y + 1;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int xs = j ? xe : x + 1;
+            if (ys >= 0 && xs >= 0) {
+                const unsigned long long* c = h + ys * kDescRow + xs * kDescCell;
+                s += c[o];
+                if (o == 0) s += c[8];
+            }
+        }
+    }
+    return s;
 }
 
 // sift_descriptor_coop (core.py:693-835), or with tab.n_dsp scales
 // sift_descriptor_dsp_coop (core.py:845-997: one histogram over the patch
 // half-sizes max(4, (int)(ph m + 0.5f)), divided by the number of scales),
 // + RootSIFT, for one keypoint on a kDescT-thread block (every thread calls
-// it): integer histogram in hist (kDescCopies x 128, zeroed here), then wave
+// it): integer histogram in hist (kDescCopies padded copies, zeroed here), then wave
 // 0 converts and normalises; thread l < 64 returns bins 2l, 2l+1.  q16: the
 // level is read as binary16 (fp16 mode, octaves >= 2).
 template <bool Q16>
 __device__ __forceinline__ void desc_hist_scale(const float* __restrict__ G, int H, int W, float ri, float ci,
                                                 float angle_deg, int ph, unsigned long long* hw, int tid) {
     const int iri = (int)ri, ici = (int)ci;
-    if (iri - ph >= 2 && iri + ph < H - 2 && ici - ph >= 2 && ici + ph < W - 2)
+    // the interior form addresses the level by 32-bit byte offsets
+    if (iri - ph >= 2 && iri + ph < H - 2 && ici - ph >= 2 && ici + ph < W - 2 && (long long)H * W < (1ll << 29))
         desc_hist_fx<true, Q16>(G, H, W, ri, ci, angle_deg, ph, hw, tid);
     else
         desc_hist_fx<false, Q16>(G, H, W, ri, ci, angle_deg, ph, hw, tid);
@@ -1138,9 +1190,9 @@ __device__ __forceinline__ void describe_keypoint_block(const float* __restrict_
     const int tid = threadIdx.x;
     // the block's barriers order LDS only: the previous keypoint's descriptor
     // stores need not drain before the next keypoint's histogram starts
-    for (int i = tid; i < kDescCopies * 128; i += kDescT) hist[i] = 0ull;
+    for (int i = tid; i < kDescHist; i += kDescT) hist[i] = 0ull;
     lds_barrier();
-    unsigned long long* hw = hist + (tid >> 6) * 128;
+    unsigned long long* hw = hist + (tid >> 6) * kDescSlots;
     const int nd = tab.n_dsp > 0 ? tab.n_dsp : 1;
     for (int d = 0; d < nd; ++d) {
         int phd = ph;
@@ -1152,14 +1204,17 @@ __device__ __forceinline__ void describe_keypoint_block(const float* __restrict_
         else desc_hist_scale<false>(G, H, W, ri, ci, angle_deg, phd, hw, tid);
     }
     lds_barrier();
+    // the copies summed into copy 0, slot by slot
+    for (int i = tid; i < kDescSlots; i += kDescT) {
+        unsigned long long s = hist[i];
+#pragma unroll
+        for (int w = 1; w < kDescCopies; ++w) s += hist[w * kDescSlots + i];
+        hist[i] = s;
+    }
+    lds_barrier();
     float v0 = 0.0f, v1 = 0.0f;
     if (tid < 64) {
-        unsigned long long s0 = 0, s1 = 0;
-#pragma unroll
-        for (int w = 0; w < kDescCopies; ++w) {
-            s0 += hist[w * 128 + 2 * tid];
-            s1 += hist[w * 128 + 2 * tid + 1];
-        }
+        const unsigned long long s0 = desc_bin_sum(hist, 2 * tid), s1 = desc_bin_sum(hist, 2 * tid + 1);
         float h0 = (float)((double)s0 * 0x1p-30), h1 = (float)((double)s1 * 0x1p-30);
         if (tab.n_dsp > 0) {
             const float inv_n = 1.0f / (float)tab.n_dsp;
