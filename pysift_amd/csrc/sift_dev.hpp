@@ -741,6 +741,69 @@ __device__ __forceinline__ bool refine_candidate(const float* __restrict__ D, in
 }
 
 // ----------------------------------------------------------- orientation --
+// The window's pixels into lane-private 36-bin columns (orientation_hist,
+// core.py:228-333): pixel p of the n_rows x n_cols window at (r_min, c_min)
+// goes to lane p % 64's column P[bin*64 + ((lane + bin) & 63)].
+template <bool Q16>
+__device__ __forceinline__ void orient_accum(const float* __restrict__ G, int H, int W, float ri, float ci,
+                                             float inv_2sw2, int r_min, int c_min, int n_rows, int n_cols,
+                                             float* P, int lane) {
+    const int total = n_rows * n_cols;
+    // chunks of kPf pixels per lane: all gradient taps of a chunk are loaded
+    // before any is used (memory-level parallelism); accumulation order per
+    // lane stays p = lane, lane + 64, ...
+    constexpr int kPf = 8;
+    const int q64 = 64 / n_cols, r64 = 64 - q64 * n_cols;
+    int rq = lane / n_cols, cq = lane - rq * n_cols;
+    for (int p0 = lane; p0 < total; p0 += 64 * kPf) {
+        float gl[kPf], gr[kPf], gu[kPf], gd[kPf];
+        int rk[kPf], ck[kPf];
+#pragma unroll
+        for (int k = 0; k < kPf; ++k) {
+            const int p = p0 + 64 * k;
+            rk[k] = r_min + rq; ck[k] = c_min + cq;
+            cq += r64; rq += q64;
+            if (cq >= n_cols) { cq -= n_cols; rq += 1; }
+            gl[k] = gr[k] = gu[k] = gd[k] = 0.0f;
+            if (p < total) {
+                const float* g = G + (size_t)rk[k] * W + ck[k];
+                gl[k] = g[-1]; gr[k] = g[1]; gu[k] = g[-W]; gd[k] = g[W];
+            }
+            if (Q16) { gl[k] = f16_round(gl[k]); gr[k] = f16_round(gr[k]); gu[k] = f16_round(gu[k]); gd[k] = f16_round(gd[k]); }
+        }
+        // branch-free: pixels past the end add +0.0f (bins are sums of
+        // non-negative terms, x + 0 == x bitwise)
+#pragma unroll
+        for (int k = 0; k < kPf; ++k) {
+            const int p = p0 + 64 * k;
+            int r = rk[k];
+            int c = ck[k];
+            float dx = gr[k] - gl[k];
+            float dy = gd[k] - gu[k];
+            float mag = sqrtf(dx * dx + dy * dy);
+            // atan2 in [-pi, pi]: the argument lies in [179.99, 540.01]
+            float ori = fmod360_lt720(atan2_f_finite(dy, dx) * 57.29577951f + 360.0f);
+            float dr = (float)r - ri, dc = (float)c - ci;
+            float wt = exp_f_le88(-(dr * dr + dc * dc) * inv_2sw2);
+            // ori in [0, 360) gives bin_f in [0, 36): 359.99997f * 0.1f rounds
+            // to 35.9999962f, so (int)bin_f % 36 == (int)bin_f
+            float bin_f = ori * 0.1f;
+            int bin_i = (int)bin_f;
+            int bin_r = bin_i == 35 ? 0 : bin_i + 1;
+            float frac = bin_f - floorf(bin_f);
+            // two distinct bins: both reads issued before the stores.  The
+            // partial of (bin, lane) sits at bin * 64 + ((lane + bin) & 63).
+            float w0 = mag * wt * (1.0f - frac), w1 = mag * wt * frac;
+            if (p >= total) { w0 = 0.0f; w1 = 0.0f; }
+            float* s0 = P + bin_i * 64 + ((lane + bin_i) & 63);
+            float* s1 = P + bin_r * 64 + ((lane + bin_r) & 63);
+            const float q0 = *s0, q1 = *s1;
+            *s0 = q0 + w0;
+            *s1 = q1 + w1;
+        }
+    }
+}
+
 // orientation_hist (core.py:228-333) for one keypoint on one wave: pixel p of
 // the window goes to lane p % 64's private 36-bin column P[bin*64 + lane].
 // Every lane returns; lane 0's out[]/n are the result.  Q16: samples are read
@@ -761,58 +824,7 @@ __device__ __forceinline__ int orient_keypoint(const float* __restrict__ G, int 
     int c_max = (int)ci + radius; if (c_max > W - 2) c_max = W - 2;
     int n_cols = c_max - c_min + 1; if (n_cols < 1) n_cols = 1;
     int n_rows = r_max - r_min + 1; if (n_rows < 0) n_rows = 0;
-    int total = n_rows * n_cols;
-    // chunks of kPf pixels per lane: all gradient taps of a chunk are loaded
-    // before any is used (memory-level parallelism); accumulation order per
-    // lane stays p = lane, lane + 64, ...
-    constexpr int kPf = 8;
-    const int q64 = 64 / n_cols, r64 = 64 - q64 * n_cols;
-    int rq = lane / n_cols, cq = lane - rq * n_cols;
-    for (int p0 = lane; p0 < total; p0 += 64 * kPf) {
-        float gl[kPf], gr[kPf], gu[kPf], gd[kPf];
-        int rk[kPf], ck[kPf];
-#pragma unroll
-        for (int k = 0; k < kPf; ++k) {
-            const int p = p0 + 64 * k;
-            rk[k] = r_min + rq; ck[k] = c_min + cq;
-            cq += r64; rq += q64;
-            if (cq >= n_cols) { cq -= n_cols; rq += 1; }
-            gl[k] = gr[k] = gu[k] = gd[k] = 0.0f;
-            if (p < total) {
-                const float* g = G + (size_t)rk[k] * W + ck[k];
-                gl[k] = g[-1]; gr[k] = g[1]; gu[k] = g[-W]; gd[k] = g[W];
-                if (Q16) { gl[k] = f16_round(gl[k]); gr[k] = f16_round(gr[k]); gu[k] = f16_round(gu[k]); gd[k] = f16_round(gd[k]); }
-            }
-        }
-        // branch-free: pixels past the end add +0.0f (bins are sums of
-        // non-negative terms, x + 0 == x bitwise)
-#pragma unroll
-        for (int k = 0; k < kPf; ++k) {
-            const int p = p0 + 64 * k;
-            int r = rk[k];
-            int c = ck[k];
-            float dx = gr[k] - gl[k];
-            float dy = gd[k] - gu[k];
-            float mag = sqrtf(dx * dx + dy * dy);
-            // atan2 in [-pi, pi]: the argument lies in [179.99, 540.01]
-            float ori = fmod360_lt720(atan2_f_finite(dy, dx) * 57.29577951f + 360.0f);
-            float dr = (float)r - ri, dc = (float)c - ci;
-            float wt = exp_f_le88(-(dr * dr + dc * dc) * inv_2sw2);
-            float bin_f = ori * 0.1f;
-            int bin_i = ((int)bin_f) % 36;
-            int bin_r = (bin_i + 1) % 36;
-            float frac = bin_f - floorf(bin_f);
-            // two distinct bins: both reads issued before the stores.  The
-            // partial of (bin, lane) sits at bin * 64 + ((lane + bin) & 63).
-            float w0 = mag * wt * (1.0f - frac), w1 = mag * wt * frac;
-            if (p >= total) { w0 = 0.0f; w1 = 0.0f; }
-            float* s0 = P + bin_i * 64 + ((lane + bin_i) & 63);
-            float* s1 = P + bin_r * 64 + ((lane + bin_r) & 63);
-            const float q0 = *s0, q1 = *s1;
-            *s0 = q0 + w0;
-            *s1 = q1 + w1;
-        }
-    }
+    orient_accum<Q16>(G, H, W, ri, ci, inv_2sw2, r_min, c_min, n_rows, n_cols, P, lane);
     // lane b < 36 owns bin b: it combines the 64 partials in the pairing of
     // the xor-butterfly tree (t[l] += t[l + off], off = 32 .. 1; wave_tree_f
     // gives every lane that value); the lane rotation spreads these reads
@@ -1058,10 +1070,6 @@ __device__ __forceinline__ void desc_hist_fx(const float* __restrict__ G, int H,
     const int srow = step / side, scol = step - srow * side;
     const int p0 = (tid * K) % total_px;
     int rq = p0 / side, cq = p0 - rq * side;
-    // the level as a buffer of H W floats (reads past it return 0)
-    const __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(G), (short)0, H * W * 4, 0x00020000);
-    const int w4 = W * 4, w8 = W * 8;
     for (int f0 = tid; f0 < total_px; f0 += kDescT * kPf) {
         float gl[kPf], gr[kPf], gu[kPf], gd[kPf], xrs[kPf], yrs[kPf];
         bool okk[kPf];
@@ -1084,15 +1092,9 @@ __device__ __forceinline__ void desc_hist_fx(const float* __restrict__ G, int H,
             xrs[k] = xr; yrs[k] = yr;
             if (kInterior) {
                 // every (rr, cc) of the patch has its 4 neighbours in the level:
-                // the taps are read for all pixels, the weight drops the others.
-                // One byte offset per pixel, (rr - 1, cc - 1), and the four taps at
-                // fixed offsets from it (W 4 and W 8 in scalar registers).  Row
-                // and width stay below 2^23, so the 24-bit multiply is exact.
-                const int at = (__mul24(rr - 1, W) + cc - 1) * 4;
-                gu[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, at + 4, 0, 0));
-                gl[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, at, w4, 0));
-                gr[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, at + 8, w4, 0));
-                gd[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, at + 4, w8, 0));
+                // the taps are read for all pixels, the atomics skip the others
+                const float* g = G + (size_t)rr * W + cc;
+                gl[k] = g[-1]; gr[k] = g[1]; gu[k] = g[-W]; gd[k] = g[W];
             } else {
                 gl[k] = gr[k] = gu[k] = gd[k] = 0.0f;
                 if (ok) {
@@ -1176,8 +1178,7 @@ template <bool Q16>
 __device__ __forceinline__ void desc_hist_scale(const float* __restrict__ G, int H, int W, float ri, float ci,
                                                 float angle_deg, int ph, unsigned long long* hw, int tid) {
     const int iri = (int)ri, ici = (int)ci;
-    // the interior form addresses the level by 32-bit byte offsets
-    if (iri - ph >= 2 && iri + ph < H - 2 && ici - ph >= 2 && ici + ph < W - 2 && (long long)H * W < (1ll << 29))
+    if (iri - ph >= 2 && iri + ph < H - 2 && ici - ph >= 2 && ici + ph < W - 2)
         desc_hist_fx<true, Q16>(G, H, W, ri, ci, angle_deg, ph, hw, tid);
     else
         desc_hist_fx<false, Q16>(G, H, W, ri, ci, angle_deg, ph, hw, tid);
