@@ -131,8 +131,9 @@ int msift_desc_backward(msift_handle* h, const float* d_imgs, int B, int H, int 
 int msift_grad_level_view(msift_handle* h, int B, int H, int W, void* d_ws, int b, int o, int* Ho, int* Wo,
                           float** d_gauss, long long** d_grad_fixed);
 
-/* Octave-level access for tests: Gaussian (S+3) and DoG (S+2) stacks of
- * octave `o` inside a workspace filled by msift_detect_describe. */
+/* Octave-level access for tests: the Gaussian stack (S+3 levels) of octave
+ * `o` inside a workspace filled by msift_detect_describe.  The DoG is not
+ * stored (its readers form D[s] = G[s+1] - G[s]): *d_dog is set to NULL. */
 int msift_pyramid_view(msift_handle* h, int H, int W, void* d_ws, int o,
                        int* Ho, int* Wo, float** d_gauss, float** d_dog);
 

@@ -24,7 +24,8 @@ struct BImg {
 
 struct BOct {
     int img, o, H, W;
-    size_t G_off, D_off;
+    int q16;            // levels read as binary16 (fp16 mode, octaves >= 2)
+    size_t G_off;
 };
 
 struct BJob {           // one image's share of one pyramid step
@@ -282,7 +283,7 @@ __global__ __launch_bounds__(256) void k_brefine(const BOct* __restrict__ oct, i
         Kp k;
         int si;
         uint8_t ok = 0;
-        if (refine_candidate((const float*)(ws + O.D_off), O.H, O.W, S, border, cd.x, cd.y, cd.z,
+        if (refine_candidate((const float*)(ws + O.G_off), O.q16, O.H, O.W, S, border, cd.x, cd.y, cd.z,
                              (double)thr[2 * O.img + thr_idx], edge32, &k.ri, &k.ci, &k.resp, &si)) {
             k.o_si = (O.img << 16) | (O.o << 8) | si;
             kslot[t] = k;
@@ -596,8 +597,8 @@ static void plan_steps(msift_batch* B) {
                 size_t HW4 = (size_t)O.H * O.W * 4;
                 BJob j;
                 j.in_off = O.G_off + HW4 * lv;
-                j.g_off = (lv + 1 == S + 2) ? kNone : O.G_off + HW4 * (lv + 1);
-                j.d_off = O.D_off + HW4 * lv;
+                j.g_off = O.G_off + HW4 * (lv + 1);
+                j.d_off = kNone;   // the DoG is formed by its readers
                 j.dec_off = (lv + 1 == S && o + 1 < B->n_oct) ? B->oct[B->img[i].oct0 + o + 1].G_off : kNone;
                 j.H = O.H; j.W = O.W; j.decW = (O.W + 1) / 2;
                 j.q16 = o >= h->tab.q16_oct;
@@ -660,18 +661,19 @@ int batch_create(msift_handle* h, int n, const int* Hs, const int* Ws, msift_bat
             BOct O;
             memset(&O, 0, sizeof(O));
             O.img = i; O.o = o; O.H = bh; O.W = bw;
+            O.q16 = o >= h->tab.q16_oct;
             size_t hw = (size_t)bh * bw;
             O.G_off = take(hw * 4 * (B->S + 3));
-            O.D_off = take(hw * 4 * (B->S + 2));
-            B->ext.push_back(ext_geometry(O.D_off, bh, bw, p.border, B->S, i, &nj, &nr, &nu));
+            B->ext.push_back(ext_geometry(O.G_off, O.q16, bh, bw, p.border, B->S, i, &nj, &nr, &nu));
             B->oct.push_back(O);
             bh = (bh + 1) / 2; bw = (bw + 1) / 2;
         }
-        // spline scratch and base image alias octave 0's DoG stack when it fits
+        // spline scratch and base image alias octave 0's levels 1..S+2 (written
+        // only after the base level exists) when they fit
         size_t n_in = (size_t)I.H * I.W;
         size_t need = I.doubled ? n_in * (16 + 16) : n_in * 4;
-        size_t d0 = (size_t)B->oct[I.oct0].H * B->oct[I.oct0].W * 4 * (B->S + 2);
-        size_t sb = need <= d0 ? B->oct[I.oct0].D_off : take(need);
+        const size_t lv0 = (size_t)B->oct[I.oct0].H * B->oct[I.oct0].W * 4;
+        size_t sb = need <= lv0 * (B->S + 2) ? B->oct[I.oct0].G_off + lv0 : take(need);
         if (I.doubled) { I.pf0_off = sb; I.pf1_off = sb + n_in * 8; I.base_off = sb + n_in * 16; }
         else { I.base_off = sb; I.pf0_off = I.pf1_off = 0; }
         B->img.push_back(I);

@@ -225,7 +225,7 @@ __global__ __launch_bounds__(256) void k_ext_write(ExtTabArg tab, int n_oct, int
 // ======================================================================
 // A6: sub-pixel refinement (core.py:343-463) in FP64
 // ======================================================================
-__global__ __launch_bounds__(256) void k_refine(OctInfo oi, const int4* __restrict__ cands,
+__global__ __launch_bounds__(256) void k_refine(OctInfo oi, int q16_oct, const int4* __restrict__ cands,
                                                  const int* __restrict__ offs, int capc,
                                                  const float* __restrict__ thr, int thr_idx, float edge32,
                                                  uint8_t* __restrict__ valid, Kp* __restrict__ kslot,
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(256) void k_refine(OctInfo oi, const int4* __restri
         int4 cd = cands[t];
         Kp k;
         int si;
-        if (refine_candidate(oi.D[o], oi.H[o], oi.W[o], oi.S, oi.border, cd.x, cd.y, cd.z, (double)thr[thr_idx],
+        if (refine_candidate(oi.G[o], o >= q16_oct, oi.H[o], oi.W[o], oi.S, oi.border, cd.x, cd.y, cd.z, (double)thr[thr_idx],
                              edge32, &k.ri, &k.ci, &k.resp, &si)) {
             k.o_si = (o << 8) | si;
             kslot[t] = k;
@@ -439,7 +439,7 @@ struct Carve {
 struct Plan {
     int H, W, doubled, n_oct, S, capc, maxkp, cap_e, n_slots;
     int Ho[kMaxOct], Wo[kMaxOct];
-    size_t oG[kMaxOct], oD[kMaxOct];
+    size_t oG[kMaxOct];
     size_t o_unit, o_pf0, o_pf1, o_hist, o_thr, o_ints, o_masks, o_ecnt, o_eoff, o_cands, o_ccount,
         o_valid, o_kslot, o_ccnt, o_coff, o_k1, o_k2, o_ang, o_nang, o_aoff, o_E, o_rank, o_stmp;
     int n_ext_jobs, n_ext_rows, n_cmp_blk;
@@ -462,16 +462,15 @@ static void make_plan(const msift_handle* h, int H, int W, Plan* P) {
         P->Ho[o] = bh; P->Wo[o] = bw;
         size_t n = (size_t)bh * bw;
         P->oG[o] = c.take(n * sizeof(float) * (P->S + 3));
-        P->oD[o] = c.take(n * sizeof(float) * (P->S + 2));
         bh = (bh + 1) / 2; bw = (bw + 1) / 2;
     }
-    // scratch for the base image: aliases octave 0's DoG stack, which is
-    // written only after the base level exists
+    // scratch for the base image: aliases octave 0's levels 1..S+2, which
+    // are written only after the base level exists
     size_t n_in = (size_t)H * W;
     size_t scratch = P->doubled ? n_in * (sizeof(double) * 2 + sizeof(float) * 4) : n_in * sizeof(float);
-    size_t d0 = (size_t)P->Ho[0] * P->Wo[0] * sizeof(float) * (P->S + 2);
+    const size_t lv0 = (size_t)P->Ho[0] * P->Wo[0] * sizeof(float);
     size_t sbase;
-    if (scratch <= d0) sbase = P->oD[0];
+    if (scratch <= lv0 * (P->S + 2)) sbase = P->oG[0] + lv0;
     else sbase = c.take(scratch);
     if (P->doubled) {
         P->o_pf0 = sbase;
@@ -488,7 +487,8 @@ static void make_plan(const msift_handle* h, int H, int W, Plan* P) {
     // extrema: wave jobs, (scale, row) slots, lane-mask units
     int nj = 0, nr = 0;
     long long nu = 0;
-    for (int o = 0; o < P->n_oct; ++o) P->ext[o] = ext_geometry(P->oD[o], P->Ho[o], P->Wo[o], p.border, P->S, 0, &nj, &nr, &nu);
+    for (int o = 0; o < P->n_oct; ++o)
+        P->ext[o] = ext_geometry(P->oG[o], o >= h->tab.q16_oct, P->Ho[o], P->Wo[o], p.border, P->S, 0, &nj, &nr, &nu);
     P->n_ext_jobs = nj;
     P->n_ext_rows = nr;
     P->o_masks = c.take((size_t)(nu > 0 ? nu : 1) * 8);
@@ -520,7 +520,6 @@ static OctInfo oct_info(const Plan& P, char* ws, int border) {
     for (int o = 0; o < P.n_oct; ++o) {
         oi.H[o] = P.Ho[o]; oi.W[o] = P.Wo[o];
         oi.G[o] = (float*)(ws + P.oG[o]);
-        oi.D[o] = (float*)(ws + P.oD[o]);
         oi.blk0[o] = P.ext[o].row0;
     }
     oi.blk0[P.n_oct] = P.n_ext_rows;
@@ -540,7 +539,7 @@ int pyramid_view(msift_handle* h, int H, int W, void* ws, int o, int* Ho, int* W
     if (o < 0 || o >= P.n_oct) { set_error("octave out of range"); return MSIFT_EINVAL; }
     *Ho = P.Ho[o]; *Wo = P.Wo[o];
     *g = (float*)((char*)ws + P.oG[o]);
-    *d = (float*)((char*)ws + P.oD[o]);
+    *d = nullptr;   // the DoG is formed by its readers, not stored
     return MSIFT_OK;
 }
 
@@ -568,7 +567,7 @@ static int keypoint_pass(const Plan& P, char* ws, const OctInfo& oi, msift_handl
                            (const int*)rowcnt, (const int*)rowoff, P.capc, (int4*)(ws + P.o_cands), gate, gb);
     }
     hipLaunchKernelGGL(k_refine, dim3(grid_cap((P.n_slots + 255) / 256, 1024)), dim3(256), 0, s, oi,
-                       (const int4*)(ws + P.o_cands), (const int*)(ws + P.o_eoff), P.capc, thr, thr_idx,
+                       h->tab.q16_oct, (const int4*)(ws + P.o_cands), (const int*)(ws + P.o_eoff), P.capc, thr, thr_idx,
                        h->edge32, (uint8_t*)(ws + P.o_valid), (Kp*)(ws + P.o_kslot), gate, gb);
     hipLaunchKernelGGL(k_flag_count, dim3(grid_cap(P.n_cmp_blk, 1024)), dim3(256), 0, s,
                        (const uint8_t*)(ws + P.o_valid), P.n_slots, P.n_cmp_blk, (int*)(ws + P.o_ccnt), gate, gb);
@@ -694,8 +693,7 @@ int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch
         const bool fork = o + 1 < P.n_oct;
         for (int lv = 0; lv < S + 2; ++lv) {
             float* dec = (lv + 1 == S && fork) ? oi.G[o + 1] : nullptr;
-            // the top level is only needed through its DoG
-            float* gnext = (lv + 1 == S + 2) ? nullptr : oi.G[o] + n * (lv + 1);
+            float* gnext = oi.G[o] + n * (lv + 1);
             hipStream_t st = s;
             if (fork && lv >= S) {
                 if (lv == S) {
@@ -704,7 +702,7 @@ int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch
                 }
                 st = side;
             }
-            if ((rc = blur(oi.G[o] + n * lv, gnext, oi.D[o] + n * lv, dec, Ho, Wo, h->inc[lv],
+            if ((rc = blur(oi.G[o] + n * lv, gnext, nullptr, dec, Ho, Wo, h->inc[lv],
                            h->taps_inc[lv], st, 1, 0, 0, h->blur_th, o >= h->tab.q16_oct)))
                 return rc;
         }

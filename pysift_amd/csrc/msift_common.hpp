@@ -338,7 +338,6 @@ struct OctInfo {
     int n_oct, S, border;
     int H[kMaxOct], W[kMaxOct];
     float* G[kMaxOct];   // (S+3) levels
-    float* D[kMaxOct];   // (S+2) levels
     int blk0[kMaxOct + 1];  // first extrema block of each octave
     int nseg[kMaxOct];      // 256-column segments per row
 };

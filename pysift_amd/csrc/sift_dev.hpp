@@ -493,10 +493,13 @@ struct BlurSmem {
 // --------------------------------------------------------------- extrema --
 // 3x3x3 extrema (core.py:260-340): D[s][r][c] with |v| > thr that is >= (or
 // <=) all 26 neighbours at scales 1..S, rows/cols in [border, dim-border).
+// The DoG is not stored: D[s] = G[s+1] - G[s] is formed where it is read,
+// the subtraction of the pyramid (core.py:146), between the binary16 values
+// in the fp16 mode's octaves (core.py:1306-1308, 1362-1364).
 // One lane per column: a wave covers 62 interior columns (lanes 1..62; lanes
 // 0 and 63 load the halo columns) and walks kExtRB rows of up to 3 scales,
 // keeping each level's row maxima / minima over (c-1, c, c+1) for the rows
-// r-1, r, r+1 in registers, so every DoG value is loaded once per strip.
+// r-1, r, r+1 in registers, so every Gaussian value is loaded once per strip.
 // v >= every neighbour  <=>  v >= the max over the 3x3x3 cube including v.
 // Output per (scale, row, 62-column segment): a 64-bit lane mask, and the
 // per-(scale, row) candidate count (integer atomics, order-free).  The write
@@ -506,7 +509,8 @@ constexpr int kExtSeg = 62;
 constexpr int kExtRB = 16;
 
 struct ExtOct {          // extrema geometry of one (image, octave)
-    size_t D_off;        // DoG stack (S+2 levels, H x W) in the workspace
+    size_t G_off;        // Gaussian stack (S+3 levels, H x W) in the workspace
+    int q16;             // levels read as binary16 (fp16 mode)
     int H, W, Hi, Wi;    // size, interior size (minus 2*border)
     int nseg, nstrip;    // 62-column segments, kExtRB-row strips (0 = no interior)
     int img;             // image index (threshold and retry gate)
@@ -528,7 +532,13 @@ __device__ __forceinline__ float min3f(float a, float b, float c) { return fminf
 
 // one wave job: rows [strip*kExtRB, +kExtRB) x segment `seg` of an octave,
 // all scales; must be called by the whole wave (uniform control flow)
-__device__ __forceinline__ void ext_strip(const float* __restrict__ D, const ExtOct& O, int S, int border,
+template <bool Q16>
+__device__ __forceinline__ float dog_value(float g1, float g0) {
+    return Q16 ? f16_round(g1) - f16_round(g0) : g1 - g0;
+}
+
+template <bool Q16>
+__device__ __forceinline__ void ext_strip(const float* __restrict__ G, const ExtOct& O, int S, int border,
                                           int strip, int seg, float thr, unsigned long long* __restrict__ masks,
                                           int* __restrict__ rowcnt) {
     const int lane = threadIdx.x & 63;
@@ -540,21 +550,33 @@ __device__ __forceinline__ void ext_strip(const float* __restrict__ D, const Ext
     int col = border + cx;
     if (col > O.W - 1) col = O.W - 1;                   // halo lane past the edge: unused value
     for (int s0 = 1; s0 <= S; s0 += 3) {
-        const int gs = S - s0 + 1 < 3 ? S - s0 + 1 : 3;  // scales s0 .. s0+gs-1, levels s0-1 .. s0+gs
-        const float* lv[5];
+        // scales s0 .. s0+gs-1 read DoG levels s0-1 .. s0+gs, i.e. Gaussian
+        // levels s0-1 .. s0+gs+1; slots past them repeat the top DoG level
+        const int gs = S - s0 + 1 < 3 ? S - s0 + 1 : 3;
+        // level bases are wave-uniform (scalar registers); a lane adds its
+        // element offset r W + col (below H W, 32 bits)
+        const float* lg[6];
 #pragma unroll
-        for (int l = 0; l < 5; ++l) {
-            int L = s0 - 1 + (l < gs + 1 ? l : gs + 1);
-            lv[l] = D + (size_t)L * HW + col;
+        for (int m = 0; m < 6; ++m) {
+            int L = s0 - 1 + (m < gs + 2 ? m : gs + 2);
+            lg[m] = G + (size_t)L * HW;
         }
-        float a0[5], a1[5], a2[5], b0[5], b1[5], b2[5], c1[5], c2[5], x[5];
+        float a0[5], a1[5], a2[5], b0[5], b1[5], b2[5], c1[5], c2[5], x[6];
 #pragma unroll
         for (int l = 0; l < 5; ++l) { a1[l] = a2[l] = b1[l] = b2[l] = c2[l] = 0.0f; }
-        auto load = [&](int r, float* v) {
+        auto load = [&](int r, float* g) {
+            const unsigned off = (unsigned)r * (unsigned)O.W + (unsigned)col;
 #pragma unroll
-            for (int l = 0; l < 5; ++l) v[l] = lv[l][(size_t)r * O.W];
+            for (int m = 0; m < 6; ++m) g[m] = lg[m][off];
         };
-        auto push = [&](const float* v) {
+        auto push = [&](const float* g) {
+            float v[5];
+#pragma unroll
+            for (int l = 0; l < 5; ++l) v[l] = dog_value<Q16>(g[l + 1], g[l]);
+            if (gs < 3) {
+                v[4] = gs == 2 ? v[3] : v[2];
+                if (gs == 1) v[3] = v[2];
+            }
 #pragma unroll
             for (int l = 0; l < 5; ++l) {
                 const float xl = dpp_from_left(v[l]), xr = dpp_from_right(v[l]);
@@ -624,10 +646,10 @@ __device__ __forceinline__ void ext_write_row(const ExtOct& O, int io, int rs, i
 }
 
 // host-side geometry of one octave's extrema jobs (counters advanced)
-inline ExtOct ext_geometry(size_t D_off, int H, int W, int border, int S, int img, int* jobs, int* rows,
+inline ExtOct ext_geometry(size_t G_off, int q16, int H, int W, int border, int S, int img, int* jobs, int* rows,
                            long long* units) {
     ExtOct e;
-    e.D_off = D_off; e.H = H; e.W = W; e.img = img;
+    e.G_off = G_off; e.q16 = q16; e.H = H; e.W = W; e.img = img;
     e.Hi = H - 2 * border; e.Wi = W - 2 * border;
     if (H < 3 || W < 3 || e.Hi <= 0 || e.Wi <= 0) { e.Hi = 0; e.Wi = 0; e.nseg = 0; e.nstrip = 0; }
     else { e.nseg = (e.Wi + kExtSeg - 1) / kExtSeg; e.nstrip = (e.Hi + kExtRB - 1) / kExtRB; }
@@ -647,13 +669,20 @@ __device__ __forceinline__ void ext_count_waves(const Tab& tab, int n_io, int nj
                                                 unsigned long long* __restrict__ masks, int* __restrict__ rowcnt) {
     const int nw = gridDim.x * (blockDim.x >> 6);
     int io = 0;
-    for (int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); j < njobs; j += nw) {
+    // the job index is wave-uniform: in a scalar register, with all it selects
+    for (int j = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); j < njobs;
+         j += nw) {
         io = tab.find_job(j, io, n_io);
         const ExtOct O = tab.get(io);
         if (gate && gate[O.img] >= below) continue;
         const int jj = j - O.job0;
         const int strip = jj / O.nseg, seg = jj - strip * O.nseg;
-        ext_strip((const float*)(ws + O.D_off), O, S, border, strip, seg, thr[2 * O.img + thr_idx], masks, rowcnt);
+        if (O.q16)
+            ext_strip<true>((const float*)(ws + O.G_off), O, S, border, strip, seg, thr[2 * O.img + thr_idx], masks,
+                            rowcnt);
+        else
+            ext_strip<false>((const float*)(ws + O.G_off), O, S, border, strip, seg, thr[2 * O.img + thr_idx], masks,
+                             rowcnt);
     }
 }
 
@@ -678,13 +707,17 @@ __device__ __forceinline__ void ext_write_waves(const Tab& tab, int n_io, int nr
 }
 
 // ---------------------------------------------------------------- refine --
-// refine_keypoints (core.py:343-463) in FP64 on one octave's DoG stack;
-// returns true and fills ri, ci, resp, si for an accepted keypoint
-__device__ __forceinline__ bool refine_candidate(const float* __restrict__ D, int H, int W, int S, int bd, int si,
-                                                 int row, int col, double thr, float edge32, float* ri, float* ci,
-                                                 float* resp_out, int* si_out) {
+// refine_keypoints (core.py:343-463) in FP64 on one octave's DoG, formed
+// from the Gaussian stack G as the extrema form it; returns true and fills
+// ri, ci, resp, si for an accepted keypoint
+template <bool Q16>
+__device__ __forceinline__ bool refine_candidate_t(const float* __restrict__ G, int H, int W, int S, int bd,
+                                                   int si, int row, int col, double thr, float edge32, float* ri,
+                                                   float* ci, float* resp_out, int* si_out) {
     const size_t HW = (size_t)H * W;
-#define DV(s_, r_, c_) ((double)D[(size_t)(s_) * HW + (size_t)(r_) * W + (c_)])
+#define DV(s_, r_, c_)                                                                                   \
+    ((double)dog_value<Q16>(G[(size_t)((s_) + 1) * HW + (size_t)(r_) * W + (c_)],                         \
+                            G[(size_t)(s_) * HW + (size_t)(r_) * W + (c_)]))
     double ox = 0.0, oy = 0.0, os = 0.0;
     bool conv = false;
     for (int it = 0; it < 5; ++it) {
@@ -738,6 +771,13 @@ __device__ __forceinline__ bool refine_candidate(const float* __restrict__ D, in
     }
 #undef DV
     return ok;
+}
+
+__device__ __forceinline__ bool refine_candidate(const float* __restrict__ G, int q16, int H, int W, int S, int bd,
+                                                 int si, int row, int col, double thr, float edge32, float* ri,
+                                                 float* ci, float* resp_out, int* si_out) {
+    return q16 ? refine_candidate_t<true>(G, H, W, S, bd, si, row, col, thr, edge32, ri, ci, resp_out, si_out)
+               : refine_candidate_t<false>(G, H, W, S, bd, si, row, col, thr, edge32, ri, ci, resp_out, si_out);
 }
 
 // ----------------------------------------------------------- orientation --
