@@ -29,9 +29,8 @@ struct BOct {
 };
 
 struct BJob {           // one image's share of one pyramid step
-    size_t in_off, g_off, d_off, dec_off;   // SIZE_MAX = not written
+    size_t in_off, g_off, dec_off;   // SIZE_MAX = not written
     int H, W, decW, tiles_x, tile0;
-    int q16;            // DoG between binary16 values (fp16 mode, octaves >= 2)
 };
 
 struct PixJob { int img; int px0; };          // per-image pixel ranges for the spline kernels
@@ -189,9 +188,7 @@ __global__ __launch_bounds__(256) void k_bblur(const BJob* __restrict__ jobs, in
     const int t = tile - J.tile0;
     const int ty = t / J.tiles_x, tx = t - ty * J.tiles_x;
     blur_tile<R, TH>((const float*)(ws + J.in_off), J.g_off == kNone ? nullptr : (float*)(ws + J.g_off),
-                     J.d_off == kNone ? nullptr : (float*)(ws + J.d_off),
-                     J.dec_off == kNone ? nullptr : (float*)(ws + J.dec_off), J.H, J.W, J.decW, taps, tx, ty, sIn,
-                     J.q16);
+                     J.dec_off == kNone ? nullptr : (float*)(ws + J.dec_off), J.H, J.W, J.decW, taps, tx, ty, sIn);
 }
 
 __global__ __launch_bounds__(256) void k_bcopy_level(const BJob* __restrict__ jobs, int njobs, char* ws) {
@@ -202,7 +199,6 @@ __global__ __launch_bounds__(256) void k_bcopy_level(const BJob* __restrict__ jo
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
         float v = in[i];
         if (J.g_off != kNone) ((float*)(ws + J.g_off))[i] = v;
-        if (J.d_off != kNone) ((float*)(ws + J.d_off))[i] = v - v;
         int y = (int)(i / J.W), x = (int)(i - (size_t)y * J.W);
         if (J.dec_off != kNone && !(y & 1) && !(x & 1)) ((float*)(ws + J.dec_off))[(size_t)(y >> 1) * J.decW + (x >> 1)] = v;
     }
@@ -583,8 +579,8 @@ static void plan_steps(msift_batch* B) {
             if (I.doubled != d) continue;
             const BOct& O = B->oct[I.oct0];
             BJob j;
-            j.in_off = I.base_off; j.g_off = O.G_off; j.d_off = kNone; j.dec_off = kNone;
-            j.H = O.H; j.W = O.W; j.decW = (O.W + 1) / 2; j.q16 = 0;
+            j.in_off = I.base_off; j.g_off = O.G_off; j.dec_off = kNone;
+            j.H = O.H; j.W = O.W; j.decW = (O.W + 1) / 2;
             js.push_back(j);
         }
         if (!js.empty()) add_step(js, &h->taps_base[d], h->sd0[d]);
@@ -598,10 +594,8 @@ static void plan_steps(msift_batch* B) {
                 BJob j;
                 j.in_off = O.G_off + HW4 * lv;
                 j.g_off = O.G_off + HW4 * (lv + 1);
-                j.d_off = kNone;   // the DoG is formed by its readers
                 j.dec_off = (lv + 1 == S && o + 1 < B->n_oct) ? B->oct[B->img[i].oct0 + o + 1].G_off : kNone;
                 j.H = O.H; j.W = O.W; j.decW = (O.W + 1) / 2;
-                j.q16 = o >= h->tab.q16_oct;
                 js.push_back(j);
             }
             add_step(js, &h->taps_inc[lv], h->inc[lv]);

@@ -92,19 +92,17 @@ __global__ __launch_bounds__(256) void k_u8_unit(const uint8_t* __restrict__ img
 
 // a level whose sigma is below 0.01 equals its input (core.py:559-560)
 __global__ __launch_bounds__(256) void k_copy_level(const float* __restrict__ in, float* __restrict__ gout,
-                                                     float* __restrict__ dout, float* __restrict__ dec, int H, int W,
+                                                     float* __restrict__ dec, int H, int W,
                                                      int decW, size_t zin, size_t zout) {
     const size_t z = blockIdx.z;
     in += z * zin;
     if (gout) gout += z * zout;
-    if (dout) dout += z * zout;
     if (dec) dec += z * zout;
     size_t n = (size_t)H * W;
     size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     float v = in[i];
     if (gout) gout[i] = v;
-    if (dout) dout[i] = v - v;
     int y = (int)(i / W), x = (int)(i - (size_t)y * W);
     if (dec && !(y & 1) && !(x & 1)) dec[(size_t)(y >> 1) * decW + (x >> 1)] = v;
 }
@@ -112,22 +110,22 @@ __global__ __launch_bounds__(256) void k_copy_level(const float* __restrict__ in
 // ======================================================================
 // A2/A4: one pyramid level.  G[s+1] = blur(G[s]) (H pass then V pass, each
 // output a fused-multiply-add chain over taps in ascending order, the
-// reference's gauss_h/gauss_v, core.py:475-547), D[s] = G[s+1] - G[s]
-// (core.py:146), and optionally the next octave's base G[S][::2, ::2].
-// A 64 x TH output tile is staged through LDS with its reflected halo.
+// reference's gauss_h/gauss_v, core.py:475-547), and optionally the next
+// octave's base G[S][::2, ::2].  A 64 x TH output tile is staged through LDS
+// with its reflected halo.  D[s] = G[s+1] - G[s] (core.py:146) is formed by
+// the extrema and refinement kernels from the stored levels.
 // ======================================================================
 // gridDim.z images of one size: pointers advance by zin (input) and zout
 // (outputs) floats per image
 template <int R, int TH>
 __global__ __launch_bounds__(256) void k_blur(const float* __restrict__ in, float* __restrict__ gout,
-                                               float* __restrict__ dout, float* __restrict__ dec,
-                                               int H, int W, int decW, Taps taps, size_t zin, size_t zout, int q16) {
+                                               float* __restrict__ dec,
+                                               int H, int W, int decW, Taps taps, size_t zin, size_t zout) {
     __shared__ __attribute__((aligned(16))) float sIn[BlurSmem<R, TH>::nIn];
     const size_t z = blockIdx.z;
     const int t = xcd_remap((int)(blockIdx.x + blockIdx.y * gridDim.x), (int)(gridDim.x * gridDim.y));
-    blur_tile<R, TH>(in + z * zin, gout ? gout + z * zout : nullptr, dout ? dout + z * zout : nullptr,
-                     dec ? dec + z * zout : nullptr, H, W, decW, taps, t % (int)gridDim.x, t / (int)gridDim.x, sIn,
-                     q16);
+    blur_tile<R, TH>(in + z * zin, gout ? gout + z * zout : nullptr, dec ? dec + z * zout : nullptr, H, W, decW,
+                     taps, t % (int)gridDim.x, t / (int)gridDim.x, sIn);
 }
 
 // Tile height for a level: the tallest tile (least halo re-read) whose tile
@@ -145,38 +143,38 @@ static int blur_tile_h(int R, int H, int W, int nb, int th) {
 }
 
 template <int R, int TH>
-static void blur_launch_th(const float* in, float* gout, float* dout, float* dec, int H, int W, int decW,
-                           const Taps& t, hipStream_t s, int nb, size_t zin, size_t zout, int q16) {
+static void blur_launch_th(const float* in, float* gout, float* dec, int H, int W, int decW,
+                           const Taps& t, hipStream_t s, int nb, size_t zin, size_t zout) {
     dim3 grid((W + 63) / 64, (H + TH - 1) / TH, nb);
-    hipLaunchKernelGGL((k_blur<R, TH>), grid, dim3(256), 0, s, in, gout, dout, dec, H, W, decW, t, zin, zout, q16);
+    hipLaunchKernelGGL((k_blur<R, TH>), grid, dim3(256), 0, s, in, gout, dec, H, W, decW, t, zin, zout);
 }
 
 template <int R>
-static void blur_launch(const float* in, float* gout, float* dout, float* dec, int H, int W, int decW,
-                        const Taps& t, hipStream_t s, int nb, size_t zin, size_t zout, int th, int q16) {
+static void blur_launch(const float* in, float* gout, float* dec, int H, int W, int decW,
+                        const Taps& t, hipStream_t s, int nb, size_t zin, size_t zout, int th) {
     const int TH = blur_tile_h(R, H, W, nb, th);
     if constexpr (R <= 12) {
-        if (TH == 64) { blur_launch_th<R, 64>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout, q16); return; }
+        if (TH == 64) { blur_launch_th<R, 64>(in, gout, dec, H, W, decW, t, s, nb, zin, zout); return; }
     }
     if constexpr (R <= 24) {
-        if (TH == 32) { blur_launch_th<R, 32>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout, q16); return; }
+        if (TH == 32) { blur_launch_th<R, 32>(in, gout, dec, H, W, decW, t, s, nb, zin, zout); return; }
     }
-    if (TH == 16) { blur_launch_th<R, 16>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout, q16); return; }
-    blur_launch_th<R, 8>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout, q16);
+    if (TH == 16) { blur_launch_th<R, 16>(in, gout, dec, H, W, decW, t, s, nb, zin, zout); return; }
+    blur_launch_th<R, 8>(in, gout, dec, H, W, decW, t, s, nb, zin, zout);
 }
 
-int blur(const float* in, float* gout, float* dout, float* dec, int H, int W, double sigma, const Taps& t,
-         hipStream_t s, int nb, size_t zin, size_t zout, int th, int q16) {
+int blur(const float* in, float* gout, float* dec, int H, int W, double sigma, const Taps& t,
+         hipStream_t s, int nb, size_t zin, size_t zout, int th) {
     int decW = (W + 1) / 2;
     if (sigma < 0.01) {
         // _fast_gaussian returns its input unchanged (core.py:559-560)
         size_t n = (size_t)H * W;
-        hipLaunchKernelGGL(k_copy_level, dim3((unsigned)((n + 255) / 256), 1, nb), dim3(256), 0, s, in, gout, dout,
+        hipLaunchKernelGGL(k_copy_level, dim3((unsigned)((n + 255) / 256), 1, nb), dim3(256), 0, s, in, gout,
                            dec, H, W, decW, zin, zout);
         return MSIFT_OK;
     }
     switch (t.r) {
-#define MSIFT_BLUR_CASE(RR) case RR: blur_launch<RR>(in, gout, dout, dec, H, W, decW, t, s, nb, zin, zout, th, q16); break;
+#define MSIFT_BLUR_CASE(RR) case RR: blur_launch<RR>(in, gout, dec, H, W, decW, t, s, nb, zin, zout, th); break;
         MSIFT_BLUR_CASE(0) MSIFT_BLUR_CASE(1) MSIFT_BLUR_CASE(2) MSIFT_BLUR_CASE(3) MSIFT_BLUR_CASE(4)
         MSIFT_BLUR_CASE(5) MSIFT_BLUR_CASE(6) MSIFT_BLUR_CASE(7) MSIFT_BLUR_CASE(8) MSIFT_BLUR_CASE(9)
         MSIFT_BLUR_CASE(10) MSIFT_BLUR_CASE(11) MSIFT_BLUR_CASE(12) MSIFT_BLUR_CASE(13) MSIFT_BLUR_CASE(14)
@@ -672,13 +670,13 @@ int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch
         const int nt_i = ((2 * W + kUpsInTW - 1) / kUpsInTW) * ((2 * H + kUpsInTH - 1) / kUpsInTH);
         hipLaunchKernelGGL(k_ups_interp, dim3(grid_cap(nt_i, 8192)), dim3(256), 0, s, pf1, H, W, zy, zx, up,
                            p.fp16_pyramid);
-        if ((rc = blur(up, G0, nullptr, nullptr, 2 * H, 2 * W, h->sd0[1], h->taps_base[1], s, 1, 0, 0, h->blur_th)))
+        if ((rc = blur(up, G0, nullptr, 2 * H, 2 * W, h->sd0[1], h->taps_base[1], s, 1, 0, 0, h->blur_th)))
             return rc;
     } else {
         float* unit = (float*)(ws + P.o_unit);
         hipLaunchKernelGGL(k_u8_unit, dim3((W + 255) / 256, H), dim3(256), 0, s, img, pitch, H, W, unit,
                            p.fp16_pyramid);
-        if ((rc = blur(unit, G0, nullptr, nullptr, H, W, h->sd0[0], h->taps_base[0], s, 1, 0, 0, h->blur_th)))
+        if ((rc = blur(unit, G0, nullptr, H, W, h->sd0[0], h->taps_base[0], s, 1, 0, 0, h->blur_th)))
             return rc;
     }
     MSIFT_LAUNCH_CHECK();
@@ -702,8 +700,8 @@ int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch
                 }
                 st = side;
             }
-            if ((rc = blur(oi.G[o] + n * lv, gnext, nullptr, dec, Ho, Wo, h->inc[lv],
-                           h->taps_inc[lv], st, 1, 0, 0, h->blur_th, o >= h->tab.q16_oct)))
+            if ((rc = blur(oi.G[o] + n * lv, gnext, dec, Ho, Wo, h->inc[lv],
+                           h->taps_inc[lv], st, 1, 0, 0, h->blur_th)))
                 return rc;
         }
     }

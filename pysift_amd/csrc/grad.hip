@@ -535,10 +535,10 @@ static int grad_pyramid(msift_handle* h, const GradPlan& P, const float* imgs, c
         hipLaunchKernelGGL(k_gups_pf_col, grid, dim3(256), 0, s, imgs, ws, P.per_img, P.o_d0, P.H, P.W, h->ups);
         hipLaunchKernelGGL(k_gups_pf_row, grid, dim3(256), 0, s, ws, P.per_img, P.o_d0, P.o_d1, P.H, P.W, h->ups);
         hipLaunchKernelGGL(k_gups_interp, grid2, dim3(256), 0, s, ws, P.per_img, P.o_d1, P.o_up, P.H, P.W);
-        rc = blur((const float*)(ws + P.o_up), (float*)(ws + P.oG[0]), nullptr, nullptr, P.Ho[0], P.Wo[0], h->sd0[1],
+        rc = blur((const float*)(ws + P.o_up), (float*)(ws + P.oG[0]), nullptr, P.Ho[0], P.Wo[0], h->sd0[1],
                   h->taps_base[1], s, P.B, zimg, zimg);
     } else {
-        rc = blur(imgs, (float*)(ws + P.oG[0]), nullptr, nullptr, P.H, P.W, h->sd0[0], h->taps_base[0], s, P.B,
+        rc = blur(imgs, (float*)(ws + P.oG[0]), nullptr, P.H, P.W, h->sd0[0], h->taps_base[0], s, P.B,
                   (size_t)P.H * P.W, zimg);
     }
     if (rc) return rc;
@@ -547,7 +547,7 @@ static int grad_pyramid(msift_handle* h, const GradPlan& P, const float* imgs, c
         float* G = (float*)(ws + P.oG[o]);
         for (int lv = 0; lv < S; ++lv) {
             float* dec = (lv + 1 == S && o + 1 < P.n_oct) ? (float*)(ws + P.oG[o + 1]) : nullptr;
-            if ((rc = blur(G + n * lv, G + n * (lv + 1), nullptr, dec, P.Ho[o], P.Wo[o], h->inc[lv], h->taps_inc[lv], s,
+            if ((rc = blur(G + n * lv, G + n * (lv + 1), dec, P.Ho[o], P.Wo[o], h->inc[lv], h->taps_inc[lv], s,
                            P.B, zimg, zimg)))
                 return rc;
         }

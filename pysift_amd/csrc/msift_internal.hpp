@@ -67,13 +67,12 @@ int detect_workspace(msift_handle* h, int H, int W, size_t* bytes);
 int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch, void* ws,
                     size_t ws_bytes, int cap, float* kp, float* df, uint8_t* du, int* count,
                     float* phase_ms, hipStream_t s);
-// one pyramid level: gout = blur(in) (and dout = gout - in, dec = gout[::2, ::2]
-// when non-NULL); sigma < 0.01 copies.  nb images of one size, pointers
+// one pyramid level: gout = blur(in) (and dec = gout[::2, ::2] when
+// non-NULL); sigma < 0.01 copies.  nb images of one size, pointers
 // advancing by zin / zout floats per image; th = tile height (0: chosen from
-// the level size; results do not depend on it); q16: DoG between binary16
-// values (fp16 mode, octaves >= 2)
-int blur(const float* in, float* gout, float* dout, float* dec, int H, int W, double sigma, const Taps& t,
-         hipStream_t s, int nb = 1, size_t zin = 0, size_t zout = 0, int th = 0, int q16 = 0);
+// the level size; results do not depend on it)
+int blur(const float* in, float* gout, float* dec, int H, int W, double sigma, const Taps& t,
+         hipStream_t s, int nb = 1, size_t zin = 0, size_t zout = 0, int th = 0);
 int pyramid_view(msift_handle* h, int H, int W, void* ws, int o, int* Ho, int* Wo, float** g,
                  float** d);
 

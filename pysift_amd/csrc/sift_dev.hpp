@@ -354,15 +354,14 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
 
 // ------------------------------------------------------------------ blur --
 // One 64 x TH output tile of G[s+1] = blur(G[s]) (H pass then V pass, fused
-// multiply-add chains over the taps in ascending order), D[s] = G[s+1] - G[s]
-// and the next octave's base G[S][::2, ::2].  The tile and its reflected
-// halo are staged through LDS.  q16 (fp16 mode, octaves >= 2): the DoG is
-// taken between the binary16 values the reference stores (core.py:1306-1308,
-// 1362-1364); the chain itself stays float32.
+// multiply-add chains over the taps in ascending order) and the next
+// octave's base G[S][::2, ::2].  The tile and its reflected halo are staged
+// through LDS.  The DoG is not written here: its readers form it from the
+// stored levels.
 template <int R, int TH>
 __device__ __forceinline__ void blur_tile(const float* __restrict__ in, float* __restrict__ gout,
-                                          float* __restrict__ dout, float* __restrict__ dec, int H, int W, int decW,
-                                          const Taps& taps, int bx, int by, float* sIn, int q16) {
+                                          float* __restrict__ dec, int H, int W, int decW,
+                                          const Taps& taps, int bx, int by, float* sIn) {
     constexpr int TW = 64;
     constexpr int IW = TW + 2 * R;
     constexpr int IWP = (IW + 4 + 3) & ~3;
@@ -429,16 +428,9 @@ __device__ __forceinline__ void blur_tile(const float* __restrict__ in, float* _
     // H pass, in place: row ly's 64 outputs overwrite the first 64 floats of
     // row ly of the input tile.  A row's 16 float4 items fall in one wave and
     // one iteration, whose LDS reads all precede its writes, so a row is read
-    // whole before it is overwritten, and no other wave touches it.  The
-    // input values the DoG subtracts are taken into registers first.
+    // whole before it is overwritten, and no other wave touches it.
     constexpr int RG = TH / 4;
     const int lx = tid & 63, g = tid >> 6;
-    float g0v[RG];
-    if (dout) {   // uniform over the block
-#pragma unroll
-        for (int j = 0; j < RG; ++j) g0v[j] = sIn[(g * RG + j + R) * IWP + lx + R];
-        __syncthreads();
-    }
     for (int i = tid; i < IH * (TW / 4); i += 256) {
         int ly = i / (TW / 4), q = i - ly * (TW / 4);
         float* row = sIn + ly * IWP;
@@ -473,10 +465,6 @@ __device__ __forceinline__ void blur_tile(const float* __restrict__ in, float* _
         if (gy < H && gx < W) {
             size_t o = (size_t)gy * W + gx;
             if (gout) gout[o] = s;
-            if (dout) {
-                const float g0 = g0v[j];
-                dout[o] = q16 ? f16_round(s) - f16_round(g0) : s - g0;
-            }
             if (dec && !(gy & 1) && !(gx & 1)) dec[(size_t)(gy >> 1) * decW + (gx >> 1)] = s;
         }
     }
