@@ -26,7 +26,7 @@ REF_MS_PER_MP = 72.8 / (768 * 1024 / 1e6)   # reference RTX 3050, 768x1024
 
 # HBM bytes the pyramid kernels move per 1024x768 view (FETCH_SIZE + WRITE_SIZE
 # counter runs, scaled by a 256 MiB copy kernel): profiles/pyramid_1024x768_traffic.md
-PYRAMID_TRAFFIC_BYTES = 329_431_580
+PYRAMID_TRAFFIC_BYTES = 262_225_042
 
 
 def octave_pixels(H, W, n_oct, doubled):
@@ -39,9 +39,11 @@ def octave_pixels(H, W, n_oct, doubled):
 
 
 def pyramid_bytes(H, W, n_oct, S=3, doubled=True):
-    """Algorithmic pyramid bytes: 1 B per input pixel + 4 B x (S Gaussian
-    levels consumed downstream + S+2 DoG levels) per octave pixel."""
-    return H * W + 4 * (S + S + 2) * octave_pixels(H, W, n_oct, doubled)
+    """Algorithmic pyramid bytes: 1 B per input pixel + 4 B x the S+3
+    Gaussian levels per octave pixel that the pyramid hands to its readers
+    (the extrema form the DoG from all of them; orientation and descriptors
+    read levels 1..S)."""
+    return H * W + 4 * (S + 3) * octave_pixels(H, W, n_oct, doubled)
 
 
 def cpu_baseline(views, seconds=12.0):
