@@ -39,7 +39,8 @@ def test_create_validates_parameters():
     assert L.msift_create(ctypes.byref(good), ctypes.byref(h)) == 0
     n = ctypes.c_size_t()
     assert L.msift_workspace_bytes(h, 480, 640, ctypes.byref(n)) == 0
-    assert n.value > 480 * 640 * 4 * 44
+    # the doubled octave 0 alone holds S+3 = 6 float levels (24 B per pixel)
+    assert n.value > 480 * 640 * 4 * 24
     assert L.msift_destroy(h) == 0
     bad = _lib.MsiftParams(0, 3, 1.6, 0.04, 10.0, 1, 1, 16_000_000, 25000, 8000, 5, 100, 1)
     assert L.msift_create(ctypes.byref(bad), ctypes.byref(h)) == _lib.MSIFT_EINVAL
