@@ -732,7 +732,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": PYRAMID_TRAFFIC_BYTES,
                      "kernel": "pyramid (k_ups_* + 26 x k_blur) per 1024x768 view",
-                     "algorithmic_bytes": B, "avg_ms": round(t_pyr * 1e3, 4)},
+                     "algorithmic_bytes": B,
+                     "bytes_model": "1 B per input pixel + 4 B x (S+3) Gaussian levels per octave pixel",
+                     "avg_ms": round(t_pyr * 1e3, 4)},
         "phase_ms": {k: round(v, 4) for k, v in s._phase_timings.items()},
     }
     if not args.no_extra and not args.pyramid_only:
