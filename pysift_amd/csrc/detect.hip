@@ -1,4 +1,5 @@
-// Detect + describe on one image: Gaussian/DoG pyramid, 3x3x3 extrema with
+// Detect + describe on one image: Gaussian pyramid (the DoG formed by its
+// readers), 3x3x3 extrema with
 // deterministic raster-order compaction, FP64 sub-pixel refinement,
 // orientation and 4x4x8 descriptor histograms with wave64 lane-private
 // partials.  Restates PySIFT.detectAndCompute (pysift/core.py:2021-2172).
@@ -751,7 +752,7 @@ int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch
         float t[7];
         for (int i = 0; i < 6; ++i) MSIFT_HIP(hipEventElapsedTime(&t[i], h->ev[i], h->ev[i + 1]));
         MSIFT_HIP(hipEventElapsedTime(&t[6], h->ev[0], h->ev[6]));
-        // gauss_pyr (incl. DoG), dog (fused: 0), extrema (+refine), stack/cap,
+        // gauss_pyr, dog (formed by the extrema: 0), extrema (+refine), stack/cap,
         // orient, descriptor (+expand/sort), cv_kpt (0 on device), total
         phase_ms[0] = t[0]; phase_ms[1] = t[1]; phase_ms[2] = t[2]; phase_ms[3] = t[3];
         phase_ms[4] = t[4]; phase_ms[5] = t[5]; phase_ms[6] = 0.0f; phase_ms[7] = t[6];
