@@ -1,8 +1,7 @@
 // Detect + describe on one image: Gaussian pyramid (the DoG formed by its
-// readers), 3x3x3 extrema with
-// deterministic raster-order compaction, FP64 sub-pixel refinement,
-// orientation and 4x4x8 descriptor histograms with wave64 lane-private
-// partials.  Restates PySIFT.detectAndCompute (pysift/core.py:2021-2172).
+// readers), 3x3x3 extrema with deterministic raster-order compaction, FP64
+// sub-pixel refinement, orientation histograms in wave64 lane-private
+// partials and 4x4x8 descriptor histograms as exact integer sums.  Restates PySIFT.detectAndCompute (pysift/core.py:2021-2172).
 #include "sift_dev.hpp"
 #include <algorithm>
 #include <cmath>
