@@ -1,7 +1,8 @@
 // Detect + describe on one image: Gaussian pyramid (the DoG formed by its
 // readers), 3x3x3 extrema with deterministic raster-order compaction, FP64
 // sub-pixel refinement, orientation histograms in wave64 lane-private
-// partials and 4x4x8 descriptor histograms as exact integer sums.  Restates PySIFT.detectAndCompute (pysift/core.py:2021-2172).
+// partials and 4x4x8 descriptor histograms as exact integer sums.  Restates
+// PySIFT.detectAndCompute (pysift/core.py:2021-2172).
 #include "sift_dev.hpp"
 #include <algorithm>
 #include <cmath>
