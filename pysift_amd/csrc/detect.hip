@@ -439,7 +439,8 @@ struct Plan {
     int H, W, doubled, n_oct, S, capc, maxkp, cap_e, n_slots;
     int Ho[kMaxOct], Wo[kMaxOct];
     size_t oG[kMaxOct];
-    size_t o_unit, o_pf0, o_pf1, o_hist, o_thr, o_ints, o_masks, o_ecnt, o_eoff, o_cands, o_ccount,
+    size_t zero_ints;   // ints from o_hist on that the call's first kernel zeroes
+    size_t o_unit, o_pf0, o_pf1, o_hist, o_thr, o_ints, o_masks, o_ecnt, o_ecnt2, o_eoff, o_cands, o_ccount,
         o_valid, o_kslot, o_ccnt, o_coff, o_k1, o_k2, o_ang, o_nang, o_aoff, o_E, o_rank, o_stmp;
     int n_ext_jobs, n_ext_rows, n_cmp_blk;
     ExtOct ext[kMaxOct];
@@ -479,10 +480,6 @@ static void make_plan(const msift_handle* h, int H, int W, Plan* P) {
         P->o_unit = sbase;
         P->o_pf0 = P->o_pf1 = 0;
     }
-    // histogram (256) + counters (16) zeroed together by one kernel
-    P->o_hist = c.take((256 + 16) * sizeof(int));
-    P->o_ints = P->o_hist + 256 * sizeof(int);
-    P->o_thr = c.take(4 * sizeof(float));
     // extrema: wave jobs, (scale, row) slots, lane-mask units
     int nj = 0, nr = 0;
     long long nu = 0;
@@ -490,8 +487,16 @@ static void make_plan(const msift_handle* h, int H, int W, Plan* P) {
         P->ext[o] = ext_geometry(P->oG[o], o >= h->tab.q16_oct, P->Ho[o], P->Wo[o], p.border, P->S, 0, &nj, &nr, &nu);
     P->n_ext_jobs = nj;
     P->n_ext_rows = nr;
-    P->o_masks = c.take((size_t)(nu > 0 ? nu : 1) * 8);
+    // zeroed together by the call's first kernel: histogram (256) + counters
+    // (16), the row counts of both keypoint passes, the rank counts
+    P->o_hist = c.take((256 + 16) * sizeof(int));
+    P->o_ints = P->o_hist + 256 * sizeof(int);
     P->o_ecnt = c.take((size_t)(nr + 1) * sizeof(int));
+    P->o_ecnt2 = c.take((size_t)(nr + 1) * sizeof(int));
+    P->o_rank = c.take((size_t)P->cap_e * sizeof(int));
+    P->zero_ints = (c.off - P->o_hist) / sizeof(int);
+    P->o_thr = c.take(4 * sizeof(float));
+    P->o_masks = c.take((size_t)(nu > 0 ? nu : 1) * 8);
     P->o_eoff = c.take((size_t)(nr + 1) * sizeof(int));
     P->n_slots = P->n_oct * P->capc;
     P->o_cands = c.take((size_t)P->n_slots * sizeof(int4));
@@ -507,7 +512,6 @@ static void make_plan(const msift_handle* h, int H, int W, Plan* P) {
     P->o_nang = c.take((size_t)(P->maxkp + 1) * sizeof(int));
     P->o_aoff = c.take((size_t)(P->maxkp + 1) * sizeof(int));
     P->o_E = c.take((size_t)P->cap_e * sizeof(Ent));
-    P->o_rank = c.take((size_t)P->cap_e * sizeof(int));
     P->o_stmp = c.take((size_t)scan_tmp_ints(std::max(P->n_ext_rows, P->maxkp + 1)) * sizeof(int));
     P->total = c.off;
 }
@@ -551,10 +555,10 @@ static int keypoint_pass(const Plan& P, char* ws, const OctInfo& oi, msift_handl
     float* thr = (float*)(ws + P.o_thr);
     ExtTabArg tab;
     memcpy(tab.o, P.ext, sizeof(P.ext));
-    int* rowcnt = (int*)(ws + P.o_ecnt);
+    // each pass counts into its own row counts, zeroed at the call's start
+    int* rowcnt = (int*)(ws + (gate ? P.o_ecnt2 : P.o_ecnt));
     int* rowoff = (int*)(ws + P.o_eoff);
     if (P.n_ext_rows > 0) {
-        hipLaunchKernelGGL(k_zero_i32, dim3((P.n_ext_rows + 255) / 256), dim3(256), 0, s, rowcnt, P.n_ext_rows);
         hipLaunchKernelGGL(k_ext_count, dim3(grid_cap((P.n_ext_jobs + 3) / 4, 4096)), dim3(256), 0, s, tab, P.n_oct,
                            P.n_ext_jobs, (const char*)ws, P.S, h->p.border, thr, thr_idx, gate, gb,
                            (unsigned long long*)(ws + P.o_masks), rowcnt);
@@ -648,7 +652,10 @@ int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch
         MSIFT_HIP(hipStreamWaitEvent(side, h->fork[kMaxOct + 1], 0));
         ts = side;
     }
-    hipLaunchKernelGGL(k_zero_i32, dim3(2), dim3(256), 0, ts, (int*)(ws + P.o_hist), 256 + 16);
+    // histogram, counters, both passes' row counts and the rank counts; their
+    // readers on the launch stream follow the join of the second stream
+    hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((P.zero_ints + 255) / 256)), dim3(256), 0, ts, (int*)(ws + P.o_hist),
+                       (int)P.zero_ints);
     {
         hipLaunchKernelGGL(k_hist256, dim3(grid_cap(H, 128)), dim3(256), 0, ts, img, H, W, pitch,
                            (int*)(ws + P.o_hist));
@@ -734,7 +741,6 @@ int detect_describe(msift_handle* h, const uint8_t* img, int H, int W, int pitch
         hipLaunchKernelGGL(k_expand, dim3((P.maxkp + 255) / 256), dim3(256), 0, s, (const Kp*)(ws + P.o_k2),
                            (const int*)&ints[2], (const float4*)(ws + P.o_ang), (const int*)(ws + P.o_nang),
                            (const int*)(ws + P.o_aoff), cs, (Ent*)(ws + P.o_E));
-        hipLaunchKernelGGL(k_zero_i32, dim3((P.cap_e + 255) / 256), dim3(256), 0, s, (int*)(ws + P.o_rank), P.cap_e);
         {
             hipLaunchKernelGGL(k_rank_xy, dim3(grid_cap((P.cap_e + 255) / 256, 8), grid_cap((P.cap_e + kRankJ - 1) / kRankJ, 128)),
                                dim3(256), 0, s, (const Ent*)(ws + P.o_E), (const int*)&ints[3],
